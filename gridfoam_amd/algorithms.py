@@ -1,0 +1,298 @@
+"""SIMPLE / PISO pressure-velocity drivers on the HIP path.
+
+The drivers stay host Python, as in the reference (src/gridfoam/algorithms/
+simple.py:287-408, piso.py:227-326, utils/pressure_correction.py:68-363),
+and call only native kernels for the per-step work: fused momentum
+assembly, batched 3-component BiCGSTAB, rAU/HbyA, interpolation and flux
+assembly, adjustPhi, pressure assembly with the hanging-face correction,
+PCG (FP64 or FP32-in-FP64-refinement), flux and velocity correction.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import logging
+from dataclasses import dataclass
+from typing import Dict, Optional, Tuple
+
+import torch
+
+from . import _native as nat
+from .config import PISOAlgorithm, SIMPLEAlgorithm
+from .fields import CellField, DirichletBC, FaceField
+from .fv import ddt_coefficients, fvc, fvm
+from .fvmatrix import Equation, FvMatrix
+from .grid import OctreeGrid, create_grid
+from .solvers import LinearSolver, SolveStats, create_solver, resolve_solver
+
+logger = logging.getLogger(__name__)
+
+
+@dataclass(frozen=True)
+class PressureSolveResult:  # pressure_correction.py:43-65
+    matrix: FvMatrix
+    stats: Tuple[SolveStats, ...]
+    initial_residual: Optional[torch.Tensor]
+    last_initial_residual: Optional[torch.Tensor]
+
+
+def needs_reference_value(p: CellField) -> bool:
+    """reference_value.py:14-30."""
+    return not any(isinstance(bc, DirichletBC) for bc in p.bcs.values())
+
+
+def set_reference_value(mat: FvMatrix, ref_cell: int, ref_value: float) -> None:
+    """reference_value.py:33-51."""
+    nat.call("gf_set_reference", nat.ptr(mat._diag), nat.ptr(mat._source), ctypes.c_int32(ref_cell),
+             ctypes.c_double(ref_value), nat.stream())
+
+
+def field_initial_residual(mat: FvMatrix, field: CellField) -> torch.Tensor:
+    """residual.py:12-36 -> device scalar ``|A psi - b|_2 / (|b|_2 + eps)``."""
+    g = mat.grid
+    nb = (g.num_cells + 255) // 256
+    scratch = torch.zeros(2 * nb + 8, dtype=torch.float64, device=g.device)
+    out = torch.zeros(1, dtype=torch.float64, device=g.device)
+    nat.call("gf_initial_residual", g.mesh_ref, ctypes.c_int32(field.nc), nat.ptr(mat._diag), nat.ptr(mat._upper),
+             nat.ptr(mat._lower), nat.ptr(mat._source), nat.ptr(field.data), nat.ptr(scratch), nat.ptr(out),
+             nat.stream())
+    return out[0]
+
+
+def correct_flux(phi: FaceField, U: CellField, update_internal: bool = False) -> None:
+    """fv/flux.py:132-162."""
+    g = U.grid
+    st = U.boundary_state()
+    if update_internal:
+        Uf = fvc.interpolate(U)
+        nat.call("gf_face_flux", g.mesh_ref, nat.ptr(Uf.single_buffer), nat.ptr(Uf.bnd_buffer), nat.ptr(st.covered),
+                 nat.ptr(phi.single_buffer), nat.ptr(phi.bnd_buffer), nat.stream())
+    nat.call("gf_constrain_bnd_flux", g.mesh_ref, nat.ptr(st.psi_b), nat.ptr(st.covered), nat.ptr(phi.bnd_buffer),
+             nat.stream())
+
+
+def compute_phi_hbya(phi_hbya: FaceField, HbyA: CellField, U: CellField, *, rAU: Optional[CellField] = None,
+                     phi: Optional[FaceField] = None, abc=None) -> FaceField:
+    """fv/flux.py:74-129 (+ rAU_f * ddtCorr when rAU/phi are given)."""
+    g = U.grid
+    Hf = fvc.interpolate(HbyA)
+    st = U.boundary_state()
+    nat.call("gf_face_flux", g.mesh_ref, nat.ptr(Hf.single_buffer), nat.ptr(Hf.bnd_buffer), nat.ptr(st.covered),
+             nat.ptr(phi_hbya.single_buffer), nat.ptr(phi_hbya.bnd_buffer), nat.stream())
+    if rAU is not None:
+        _, b, c = abc
+        older = U.older_data if c != 0.0 else None
+        phi_older = phi.older_single_data if c != 0.0 else None
+        nat.call("gf_ddt_corr_add", g.mesh_ref, nat.ptr(U.old_data), nat.ptr(older), nat.ptr(phi.old_single_data),
+                 nat.ptr(phi_older), nat.ptr(rAU.data), ctypes.c_double(b), ctypes.c_double(c), ctypes.c_double(g.dt),
+                 nat.ptr(phi_hbya.single_buffer), nat.stream())
+    nat.call("gf_constrain_bnd_flux", g.mesh_ref, nat.ptr(st.psi_b), nat.ptr(st.nonassignable),
+             nat.ptr(phi_hbya.bnd_buffer), nat.stream())
+    return phi_hbya
+
+
+def adjust_phi(phi: FaceField, U: CellField, p: Optional[CellField] = None) -> None:
+    """fv/adjust_phi.py:44-127 (decided and applied on the device, no sync)."""
+    if p is not None and any(isinstance(bc, DirichletBC) for bc in p.bcs.values()):
+        return
+    st = U.boundary_state()
+    nat.call("gf_adjust_phi", U.grid.mesh_ref, nat.ptr(st.adjust_kind), nat.ptr(phi.bnd_buffer), None, nat.stream())
+
+
+def solve_pressure_poisson(p: CellField, rAtU: CellField, div_phi_hbya: torch.Tensor, solver: LinearSolver, *,
+                           n_non_orthogonal_correctors: int, p_needs_ref: bool, p_ref_cell: Optional[int] = None,
+                           p_ref_value: Optional[float] = None, final_solver: Optional[LinearSolver] = None,
+                           compute_residual: bool = True) -> PressureSolveResult:
+    """pressure_correction.py:68-160."""
+    g = p.grid
+    mat = None
+    stats = None
+    first = last = None
+    for corr in range(n_non_orthogonal_correctors + 1):
+        mat = fvm.assemble(p, lap=-1.0, gamma=rAtU.data)
+        nat.call("gf_predictor_source", ctypes.c_int32(g.num_cells), ctypes.c_int32(1), nat.ptr(mat._source),
+                 nat.ptr(g.cell_volumes), nat.ptr(div_phi_hbya), nat.ptr(mat._source), nat.stream())
+        if p_needs_ref:
+            if p_ref_cell is None or p_ref_value is None:
+                raise ValueError("p_ref_cell and p_ref_value are required when p_needs_ref is True")
+            set_reference_value(mat, p_ref_cell, p_ref_value)
+        eq = Equation(p, mat)
+        if compute_residual:
+            last = field_initial_residual(mat, p)
+            if first is None:
+                first = last
+        s = final_solver if (final_solver is not None and corr == n_non_orthogonal_correctors) else solver
+        res = s.solve(eq)
+        p.data = res.solution
+        stats = res.stats
+    return PressureSolveResult(mat, stats, first, last)
+
+
+def correct_phi(phi: FaceField, phi_hbya: FaceField, p_eqn: FvMatrix, p: CellField, rAtU: CellField) -> None:
+    """pressure_correction.py:270-305 (no snapped immersed cells to close)."""
+    g = p.grid
+    st = p.boundary_state()
+    nat.call("gf_correct_phi", g.mesh_ref, nat.ptr(phi_hbya.single_buffer), nat.ptr(phi_hbya.bnd_buffer),
+             nat.ptr(p_eqn._upper), nat.ptr(p_eqn._lower), nat.ptr(p_eqn.face_flux_correction), nat.ptr(p.data),
+             nat.ptr(rAtU.data), nat.ptr(st.frac), nat.ptr(st.ref_v), nat.ptr(st.ref_g), nat.ptr(st.covered),
+             nat.ptr(phi.single_buffer), nat.ptr(phi.bnd_buffer), nat.stream())
+
+
+def correct_velocity(U: CellField, HbyA: CellField, rAtU: CellField, p: CellField) -> None:
+    """pressure_correction.py:338-363."""
+    gp = fvc.grad(p)
+    nat.call("gf_correct_velocity", ctypes.c_int32(U.grid.num_cells), nat.ptr(HbyA.data), nat.ptr(rAtU.data),
+             nat.ptr(gp), nat.ptr(U.data), nat.stream())
+
+
+class AlgorithmBase:
+    """algorithms/base.py:46-100 (field set-up shared by SIMPLE and PISO)."""
+
+    def __init__(self, grid: OctreeGrid, transient: bool):
+        self._grid = grid
+        cfg = grid.sim_config
+        cond = cfg.conditions
+        self.U = CellField(grid, "U", (3,), transient=transient, condition=cond.get("U"))
+        self.p = CellField(grid, "p", (), condition=cond.get("p"))
+        self.phi = FaceField(grid, "phi")
+        self.phi_hbya = FaceField(grid, "phiHbyA")
+        self.rAU = CellField(grid, "rAU")
+        self.rAtU = self.rAU
+        self.HbyA = CellField(grid, "HbyA", (3,))
+        self._solvers: Dict[str, LinearSolver] = {k: create_solver(v) for k, v in cfg.fvSolution.solvers.items()}
+        self.nu = float(cfg.properties.transport.nu)
+        self.adjust_phi_enabled = cfg.fvSolution.adjustPhi
+        alg = cfg.fvSolution.algorithm
+        self.n_non_orthogonal_correctors = alg.nNonOrthogonalCorrectors
+        self.p_needs_ref = needs_reference_value(self.p)
+        if self.p_needs_ref:
+            if alg.pRefCell is None:
+                raise ValueError("pRefCell is required when p_needs_ref is True")
+            if alg.pRefValue is None:
+                raise ValueError("pRefValue is required when p_needs_ref is True")
+            self.p_ref_cell, self.p_ref_value = alg.pRefCell, alg.pRefValue
+        else:
+            self.p_ref_cell = self.p_ref_value = None
+        self.solve_stats: Dict[str, Tuple[SolveStats, ...]] = {}
+        self.last_p_residual: Optional[torch.Tensor] = None
+
+    @property
+    def grid(self) -> OctreeGrid:
+        return self._grid
+
+    @property
+    def solvers(self) -> Dict[str, LinearSolver]:
+        return self._solvers
+
+    def _grad_p_into(self) -> torch.Tensor:
+        return fvc.grad(self.p)
+
+    def _pressure_corrector(self, UEqn: FvMatrix, *, transient: bool, abc, final: bool) -> PressureSolveResult:
+        g = self.grid
+        nat.call("gf_rau_hbya", g.mesh_ref, nat.ptr(UEqn._diag), nat.ptr(UEqn._upper), nat.ptr(UEqn._lower),
+                 nat.ptr(UEqn._source), nat.ptr(self.U.data), nat.ptr(self.rAU.data), nat.ptr(self.HbyA.data),
+                 nat.stream())
+        if transient:
+            compute_phi_hbya(self.phi_hbya, self.HbyA, self.U, rAU=self.rAU, phi=self.phi, abc=abc)
+        else:
+            compute_phi_hbya(self.phi_hbya, self.HbyA, self.U)
+        if self.adjust_phi_enabled:
+            adjust_phi(self.phi_hbya, self.U, self.p)
+        div_phi_hbya = fvc.div(self.phi_hbya)
+        p_solver = resolve_solver(self.solvers, self.p.name, is_final=final) if transient else self.solvers["p"]
+        res = solve_pressure_poisson(self.p, self.rAtU, div_phi_hbya, self.solvers["p"],
+                                     n_non_orthogonal_correctors=self.n_non_orthogonal_correctors,
+                                     p_needs_ref=self.p_needs_ref, p_ref_cell=self.p_ref_cell,
+                                     p_ref_value=self.p_ref_value, final_solver=p_solver if transient else None)
+        self.solve_stats[self.p.name] = res.stats
+        self.last_p_residual = res.initial_residual
+        return res
+
+
+class SIMPLE(AlgorithmBase):
+    """algorithms/simple.py:141-408 (steady; consistent=False)."""
+
+    def __init__(self, grid: OctreeGrid):
+        super().__init__(grid, transient=False)
+        alg = grid.sim_config.fvSolution.algorithm
+        assert isinstance(alg, SIMPLEAlgorithm)
+        if alg.consistent:
+            raise ValueError("SIMPLEC (consistent: true) is not part of this build")
+        self.alpha_U = alg.relaxationFactors.equations["U"]
+        self.alpha_p = alg.relaxationFactors.equations["p"]
+        correct_flux(self.phi, self.U, update_internal=True)
+
+    def step(self) -> None:
+        g, U, p = self.grid, self.U, self.p
+        UEqn = fvm.assemble(U, div=1.0, lap=-1.0, phi=self.phi, gamma=self.nu)
+        nat.call("gf_relax_momentum", ctypes.c_int32(g.num_cells), ctypes.c_double(self.alpha_U),
+                 nat.ptr(UEqn._diag), nat.ptr(UEqn._source), nat.ptr(U.data), nat.stream())
+        gp = fvc.grad(p)
+        pred = FvMatrix.__new__(FvMatrix)
+        pred.field, pred.grid = U, g
+        pred._diag, pred._upper, pred._lower = UEqn._diag, UEqn._upper, UEqn._lower
+        pred.face_flux_correction = None
+        pred._source = torch.empty_like(UEqn._source)
+        nat.call("gf_predictor_source", ctypes.c_int32(g.num_cells), ctypes.c_int32(3), nat.ptr(UEqn._source),
+                 nat.ptr(g.cell_volumes), nat.ptr(gp), nat.ptr(pred._source), nat.stream())
+        res = self.solvers["U"].solve(Equation(U, pred))
+        U.data = res.solution
+        self.solve_stats[U.name] = res.stats
+        p_old = p.data.clone()
+        pres = self._pressure_corrector(UEqn, transient=False, abc=None, final=False)
+        correct_phi(self.phi, self.phi_hbya, pres.matrix, p, self.rAtU)
+        nat.call("gf_relax_field", ctypes.c_int64(g.num_cells), ctypes.c_double(self.alpha_p), nat.ptr(p_old),
+                 nat.ptr(p.data), nat.stream())
+        correct_velocity(U, self.HbyA, self.rAtU, p)
+
+
+class PISO(AlgorithmBase):
+    """algorithms/piso.py:86-326."""
+
+    def __init__(self, grid: OctreeGrid):
+        super().__init__(grid, transient=True)
+        alg = grid.sim_config.fvSolution.algorithm
+        assert isinstance(alg, PISOAlgorithm)
+        self.n_correctors = alg.nCorrectors
+        correct_flux(self.phi, self.U, update_internal=True)
+        self.U.update_history(reset=True)
+        self.phi.update_history(reset=True)
+
+    def step(self) -> None:
+        g, U, p = self.grid, self.U, self.p
+        abc = ddt_coefficients(U)
+        UEqn = fvm.assemble(U, ddt=1.0, div=1.0, lap=-1.0, phi=self.phi, gamma=self.nu, abc=abc)
+        gp = fvc.grad(p)
+        pred = FvMatrix.__new__(FvMatrix)
+        pred.field, pred.grid = U, g
+        pred._diag, pred._upper, pred._lower = UEqn._diag, UEqn._upper, UEqn._lower
+        pred.face_flux_correction = None
+        pred._source = torch.empty_like(UEqn._source)
+        nat.call("gf_predictor_source", ctypes.c_int32(g.num_cells), ctypes.c_int32(3), nat.ptr(UEqn._source),
+                 nat.ptr(g.cell_volumes), nat.ptr(gp), nat.ptr(pred._source), nat.stream())
+        res = self.solvers["U"].solve(Equation(U, pred))
+        U.data = res.solution
+        self.solve_stats[U.name] = res.stats
+        for i in range(self.n_correctors):
+            pres = self._pressure_corrector(UEqn, transient=True, abc=abc, final=i == self.n_correctors - 1)
+            correct_phi(self.phi, self.phi_hbya, pres.matrix, p, self.rAtU)
+            correct_velocity(U, self.HbyA, self.rAtU, p)
+        U.update_history()
+        self.phi.update_history()
+
+
+def create_algorithm(grid: OctreeGrid):
+    """algorithms/factory.py:9-38."""
+    alg = grid.sim_config.fvSolution.algorithm
+    if isinstance(alg, SIMPLEAlgorithm):
+        return SIMPLE(grid)
+    if isinstance(alg, PISOAlgorithm):
+        return PISO(grid)
+    raise ValueError(f"unsupported algorithm {alg}")
+
+
+def create_case(config, device: str = "cuda"):
+    """Grid + algorithm from a validated GridfoamConfig."""
+    grid = create_grid(config, device=device)
+    grid.sim_config = config.simulator
+    return grid, create_algorithm(grid)

@@ -1,0 +1,288 @@
+"""YAML case configuration (subset of src/gridfoam/meta/config.py, Python 3.10).
+
+Field names, defaults and validation follow the reference schema for the
+parts the pressure-velocity path reads: fluxel (domain, root resolution,
+refinement regions, leaf refinement), control, fvSchemes, fvSolution
+(algorithm, solvers, adjustPhi), conditions and properties.  Two solver keys
+are additions of this build: ``mixed_precision`` (FP32 inner Krylov solve in
+an FP64 refinement loop) and ``inner_rel_tolerance``.
+"""
+
+from __future__ import annotations
+
+import pathlib
+from enum import Enum
+from typing import Any, Dict, List, Optional, Union
+
+import yaml
+from pydantic import BaseModel, Field, field_validator, model_validator
+
+
+class _StrEnum(str, Enum):
+    def __str__(self) -> str:  # behave like enum.StrEnum (3.11+)
+        return str(self.value)
+
+
+class SolverType(_StrEnum):  # meta/enums.py:186-196
+    CG = "cg"
+    BiCGSTAB = "bicgstab"
+    PyAMG = "pyamg"
+
+
+class PreconditionerType(_StrEnum):  # meta/enums.py:222-228
+    NONE = "none"
+    JACOBI = "jacobi"
+
+
+class NormType(_StrEnum):  # meta/enums.py:199-219
+    L_inf = "l_inf"
+    L_2 = "l_2"
+
+    def to_norm_order(self) -> Union[int, float]:
+        return float("inf") if self is NormType.L_inf else 2
+
+
+class DivScheme(_StrEnum):  # meta/enums.py:126-146 (upwind/linear implemented)
+    UPWIND = "upwind"
+    LINEAR = "linear"
+    VANLEER = "vanleer"
+    MINMOD = "minmod"
+    SUPERBEE = "superbee"
+    MONOTONIZED_CENTRAL = "monotonized_central"
+    LIMITEDLINEAR = "limitedlinear"
+    KOREN = "koren"
+
+
+class DdtScheme(_StrEnum):
+    EULER = "euler"
+    BACKWARD = "backward"
+
+
+class GradScheme(_StrEnum):
+    LINEAR = "linear"
+    LEASTSQUARE = "leastsquare"
+
+
+class LaplacianScheme(_StrEnum):  # meta/enums.py:162-183
+    LINEAR = "linear"
+    CORRECTED = "corrected"
+    UNCORRECTED = "uncorrected"
+    GAUSS_LINEAR_CORRECTED = "Gauss linear corrected"
+    GAUSS_LINEAR_UNCORRECTED = "Gauss linear uncorrected"
+    GAUSS_HARMONIC_CORRECTED = "Gauss harmonic corrected"
+    GAUSS_HARMONIC_UNCORRECTED = "Gauss harmonic uncorrected"
+
+
+class SnGradScheme(_StrEnum):
+    UNCORRECTED = "uncorrected"
+    CORRECTED = "corrected"
+
+
+class BoundaryConditionType(_StrEnum):  # meta/enums.py:265-275
+    DIRICHLET = "dirichlet"
+    NEUMANN = "neumann"
+    EMPTY = "empty"
+    SLIP = "slip"
+    INLET_OUTLET = "inlet_outlet"
+    FIXED_FLUX_PRESSURE = "fixed_flux_pressure"
+
+
+class DomainBoundaryPatch(_StrEnum):  # meta/enums.py:53-79
+    X_MINUS = "x_minus"
+    X_PLUS = "x_plus"
+    Y_MINUS = "y_minus"
+    Y_PLUS = "y_plus"
+    Z_MINUS = "z_minus"
+    Z_PLUS = "z_plus"
+
+    def to_direction(self) -> int:
+        """dir id with axis = dir // 2 and sign = 2 (dir % 2) - 1."""
+        return list(DomainBoundaryPatch).index(self)
+
+
+class AlgorithmType(_StrEnum):
+    SIMPLE = "simple"
+    PISO = "piso"
+    PIMPLE = "pimple"
+    MANUAL = "manual"
+
+
+class PrecisionType(_StrEnum):
+    FLOAT32 = "float32"
+    FLOAT64 = "float64"
+
+
+class DomainConfig(BaseModel, frozen=True):
+    lower: List[float]
+    upper: List[float]
+
+
+class RefinementRegionConfig(BaseModel, frozen=True):  # meta/config.py:47-87
+    name: Optional[str] = None
+    min: List[float]
+    max: List[float]
+    level: int
+
+    @field_validator("min", "max")
+    @classmethod
+    def _corner(cls, v: List[float]) -> List[float]:
+        if len(v) != 3:
+            raise ValueError("refinement region corners must have 3 values")
+        return v
+
+    @field_validator("level")
+    @classmethod
+    def _level(cls, v: int) -> int:
+        if v < 0:
+            raise ValueError("refinement region level must be non-negative")
+        return v
+
+    @model_validator(mode="after")
+    def _bounds(self) -> "RefinementRegionConfig":
+        if any(lo >= hi for lo, hi in zip(self.min, self.max)):
+            raise ValueError("refinement region min must be less than max")
+        return self
+
+
+class FluxelConfig(BaseModel, frozen=True):  # meta/config.py:90-145
+    domain: DomainConfig
+    root_resolution: List[int]
+    target_level: int = 0
+    n_leaf_refinement: int = 0
+    refinement_regions: List[RefinementRegionConfig] = Field(default_factory=list)
+    mesh_path: Optional[pathlib.Path] = None
+    ibm_type: str = "axis_projected"
+    motion: str = "static"
+
+    @model_validator(mode="after")
+    def _no_surface(self) -> "FluxelConfig":
+        if self.mesh_path is not None:
+            raise ValueError("surface (STL/OBJ) immersed boundaries are not supported by this build")
+        return self
+
+
+class ControlConfig(BaseModel, frozen=True):
+    deltaT: float
+    endTime: float
+    writeInterval: int = 1
+    precision: PrecisionType = PrecisionType.FLOAT64
+    output: Optional[Dict[str, Any]] = None
+
+
+class fvSchemesConfig(BaseModel, frozen=True):
+    ddtSchemes: Optional[Dict[str, DdtScheme]] = None
+    gradSchemes: Optional[Dict[str, GradScheme]] = None
+    divSchemes: Optional[Dict[str, DivScheme]] = None
+    laplacianSchemes: Optional[Dict[str, LaplacianScheme]] = None
+    snGradSchemes: Optional[Dict[str, SnGradScheme]] = None
+
+
+class SolverConfig(BaseModel, frozen=True):  # meta/config.py:269-316
+    method: SolverType
+    preconditioner: PreconditionerType = PreconditionerType.NONE
+    tolerance: float = 1e-6
+    rel_tolerance: float = 1e-6
+    max_iter: int = 1000
+    norm_type: NormType = NormType.L_2
+    max_restart: int = 5
+    log_interval: int = 25
+    mixed_precision: bool = False
+    inner_rel_tolerance: float = 1e-4
+
+
+class RelaxationFactorsConfig(BaseModel, frozen=True):
+    equations: Dict[str, float] = Field(default_factory=dict)
+
+
+class SIMPLEAlgorithm(BaseModel, frozen=True):
+    type: AlgorithmType = AlgorithmType.SIMPLE
+    nNonOrthogonalCorrectors: int = Field(default=0, ge=0)
+    residualControl: Dict[str, Any] = Field(default_factory=dict)
+    consistent: bool = False
+    relaxationFactors: RelaxationFactorsConfig = Field(default_factory=RelaxationFactorsConfig)
+    pRefCell: Optional[int] = Field(default=None, ge=0)
+    pRefValue: Optional[float] = None
+
+
+class PISOAlgorithm(BaseModel, frozen=True):
+    type: AlgorithmType = AlgorithmType.PISO
+    nNonOrthogonalCorrectors: int = Field(default=0, ge=0)
+    nCorrectors: int = Field(default=2, ge=1)
+    pRefCell: Optional[int] = Field(default=None, ge=0)
+    pRefValue: Optional[float] = None
+
+
+class fvSolutionConfig(BaseModel, frozen=True):
+    algorithm: Union[SIMPLEAlgorithm, PISOAlgorithm]
+    solvers: Dict[str, SolverConfig] = Field(default_factory=dict)
+    adjustPhi: bool = True
+
+    @field_validator("algorithm", mode="before")
+    @classmethod
+    def _pick(cls, v: Any) -> Any:
+        if isinstance(v, dict):
+            kind = str(v.get("type", "")).lower()
+            if kind == "simple":
+                return SIMPLEAlgorithm(**v)
+            if kind == "piso":
+                return PISOAlgorithm(**v)
+            raise ValueError(f"unsupported algorithm type {kind!r}")
+        return v
+
+
+class BoundaryConditionConfig(BaseModel, frozen=True):  # meta/config.py:552-585
+    type: BoundaryConditionType
+    patches: List[DomainBoundaryPatch]
+    value: Optional[Union[float, List[float]]] = None
+
+
+class ConditionConfig(BaseModel, frozen=True):  # meta/config.py:741-766
+    dimension: Optional[Dict[str, Union[float, int]]] = None
+    internal: Union[float, List[float]] = 0.0
+    boundary: Dict[str, BoundaryConditionConfig] = Field(default_factory=dict)
+    export: bool = True
+
+
+class TransportConfig(BaseModel, frozen=True):
+    type: str = "newtonian"
+    nu: float
+
+
+class TurbulenceConfig(BaseModel, frozen=True):
+    type: str = "laminar"
+
+
+class PropertiesConfig(BaseModel, frozen=True):
+    transport: TransportConfig
+    turbulence: TurbulenceConfig = Field(default_factory=TurbulenceConfig)
+
+
+class SimulatorConfig(BaseModel, frozen=True):
+    control: ControlConfig
+    fvSchemes: fvSchemesConfig = Field(default_factory=fvSchemesConfig)
+    fvSolution: fvSolutionConfig
+    conditions: Dict[str, ConditionConfig] = Field(default_factory=dict)
+    properties: PropertiesConfig
+    device: str = "cuda"
+
+    @model_validator(mode="after")
+    def _simple_time(self) -> "SimulatorConfig":  # meta/config.py:852-865
+        if isinstance(self.fvSolution.algorithm, SIMPLEAlgorithm):
+            if self.control.deltaT != 1:
+                raise ValueError("SIMPLE requires deltaT = 1")
+        return self
+
+
+class GridfoamConfig(BaseModel, frozen=True):
+    version: Optional[str] = None
+    fluxel: FluxelConfig
+    simulator: SimulatorConfig
+
+
+def load_config(path: Union[str, pathlib.Path, dict]) -> GridfoamConfig:
+    """Parse and validate a case YAML file (or an already-parsed dict)."""
+    data = path if isinstance(path, dict) else yaml.safe_load(pathlib.Path(path).read_text())
+    return GridfoamConfig.model_validate(data)
+
+
+CASES = pathlib.Path(__file__).resolve().parent / "cases"

@@ -1,0 +1,884 @@
+// Finite-volume operators of the pressure-velocity path (FP64).
+//
+// Every face-to-cell sum is a gather over the per-cell CSR of incident
+// faces (own_row / nbr_row+nbr_face / bnd_row), so there are no atomic
+// scatters and the results are run-to-run deterministic.  The reference
+// builds the same sums with torch index_add_ (FP64 atomics on GPU).
+// Partial sums are grouped as the reference groups them (owner-side sum,
+// then neighbour-side sum, then boundary terms).
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "gf_common.h"
+
+namespace gf {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+
+namespace {
+
+// ---------------------------------------------------------------- boundary state
+// boundary_ops.py:311-326 (covered faces) and :545-593 (zero-gradient fill)
+__global__ void k_bc_state(const gf_mesh m, int nc, const double *psi, const double *frac, const double *rv,
+                           const double *rg, const uint8_t *covered, double *psi_b) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m.n_bnd) return;
+    const int c = m.bnd_owner[k];
+    if (!covered[k]) {
+        for (int q = 0; q < nc; ++q) psi_b[k * nc + q] = psi[c * nc + q];
+        return;
+    }
+    const double f = frac[k], d = m.bnd_mag_d[k];
+    for (int q = 0; q < nc; ++q) psi_b[k * nc + q] = f * rv[k * nc + q] + (1.0 - f) * (psi[c * nc + q] + d * rg[k * nc + q]);
+}
+
+// slip.py:79-101: remove the wall-normal component of the cell velocity
+__global__ void k_bc_slip(const gf_mesh m, const double *psi, const uint8_t *mask, double *rv) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m.n_bnd || !mask[k]) return;
+    const int c = m.bnd_owner[k], a = m.bnd_dir[k] / 2;
+    const double sgn = m.bnd_sign[k];
+    double n[3] = {0.0, 0.0, 0.0};
+    n[a] = sgn;
+    const double *u = psi + 3 * c;
+    const double un = (u[0] * n[0] + u[1] * n[1]) + u[2] * n[2];
+    for (int d = 0; d < 3; ++d) rv[3 * k + d] = u[d] - un * n[d];
+}
+
+// ---------------------------------------------------------------- least squares
+constexpr int kMaxNc = 3;
+
+__device__ __forceinline__ void lsq_wd(double d0, double d1, double d2, double (&wd)[3]) {
+    const double d2s = (d0 * d0 + d1 * d1) + d2 * d2;
+    const double w2 = 1.0 / (d2s > 0.0 ? d2s : 1.0);
+    wd[0] = w2 * d0;
+    wd[1] = w2 * d1;
+    wd[2] = w2 * d2;
+}
+
+template <int NC>
+__global__ void k_lsq_grad(const gf_mesh m, const double *psi, const double *psi_b, const uint8_t *covered,
+                           const int32_t *cell_list, int n_list, double *grad) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_list) return;
+    const int c = cell_list ? cell_list[i] : i;
+    const double *cc = m.centers + 3 * c;
+    double ao[NC][3] = {}, an[NC][3] = {}, ab[NC][3] = {};
+    double pc[NC];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) pc[q] = psi[c * NC + q];
+    for (int f = m.own_row[c]; f < m.own_row[c + 1]; ++f) {
+        const int n = m.neighbour[f];
+        const double *cn = m.centers + 3 * n;
+        double wd[3];
+        lsq_wd(cn[0] - cc[0], cn[1] - cc[1], cn[2] - cc[2], wd);
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+            const double dp = psi[n * NC + q] - pc[q];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) ao[q][j] += dp * wd[j];
+        }
+    }
+    for (int k = m.nbr_row[c]; k < m.nbr_row[c + 1]; ++k) {
+        const int o = m.owner[m.nbr_face[k]];
+        const double *co = m.centers + 3 * o;
+        double wd[3];
+        lsq_wd(cc[0] - co[0], cc[1] - co[1], cc[2] - co[2], wd);
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+            const double dp = pc[q] - psi[o * NC + q];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) an[q][j] += dp * wd[j];
+        }
+    }
+    for (int k = m.bnd_row[c]; k < m.bnd_row[c + 1]; ++k) {
+        if (!covered[k]) continue;
+        double wd[3];
+        lsq_wd(m.bnd_d[3 * k + 0], m.bnd_d[3 * k + 1], m.bnd_d[3 * k + 2], wd);
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+            const double dp = psi_b[k * NC + q] - pc[q];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) ab[q][j] += dp * wd[j];
+        }
+    }
+    const double *inv = m.ata_inv + 9 * c;
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+        double atb[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) atb[j] = (ao[q][j] + an[q][j]) + ab[q][j];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+            grad[(c * NC + q) * 3 + r] = (inv[3 * r + 0] * atb[0] + inv[3 * r + 1] * atb[1]) + inv[3 * r + 2] * atb[2];
+    }
+}
+
+// ---------------------------------------------------------------- interpolation
+template <int NC>
+__global__ void k_interp_linear(const gf_mesh m, const double *psi, double *psi_f) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.n_faces) return;
+    const int o = m.owner[f], n = m.neighbour[f];
+    const double w = m.w[f];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) psi_f[f * NC + q] = w * psi[o * NC + q] + (1.0 - w) * psi[n * NC + q];
+}
+
+template <int NC>
+__device__ __forceinline__ double hang_interp_corr(const gf_mesh &m, const double *grad, int h, int q) {
+    const int f = m.hang_face[h];
+    const int o = m.owner[f], n = m.neighbour[f];
+    const double w = m.w[f];
+    const double *go = grad + (o * NC + q) * 3, *gn = grad + (n * NC + q) * 3, *off = m.hang_offset + 3 * h;
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) acc += (w * go[j] + (1.0 - w) * gn[j]) * off[j];
+    return acc;
+}
+
+template <int NC>
+__device__ __forceinline__ double hang_sn_corr(const gf_mesh &m, const double *grad, int h, int q) {
+    const int f = m.hang_face[h];
+    const int o = m.owner[f], n = m.neighbour[f];
+    const double *go = grad + (o * NC + q) * 3, *gn = grad + (n * NC + q) * 3;
+    const double *mo = m.hang_move_o + 3 * h, *mn = m.hang_move_n + 3 * h;
+    const double cn = (gn[0] * mn[0] + gn[1] * mn[1]) + gn[2] * mn[2];
+    const double co = (go[0] * mo[0] + go[1] * mo[1]) + go[2] * mo[2];
+    return m.delta[f] * (cn - co);
+}
+
+template <int NC>
+__global__ void k_interp_hang(const gf_mesh m, const double *grad, double *psi_f) {
+    int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= m.n_hang) return;
+    const int f = m.hang_face[h];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) psi_f[f * NC + q] += hang_interp_corr<NC>(m, grad, h, q);
+}
+
+template <int NC>
+__global__ void k_sn_grad_hang(const gf_mesh m, const double *grad, double *corr) {
+    int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= m.n_hang) return;
+#pragma unroll
+    for (int q = 0; q < NC; ++q) corr[h * NC + q] = hang_sn_corr<NC>(m, grad, h, q);
+}
+
+// ---------------------------------------------------------------- divergence
+template <int NC>
+__global__ void k_fvc_div(const gf_mesh m, const double *phi_s, const double *phi_b, double *out) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.n_cells) return;
+    double so[NC] = {}, sn[NC] = {}, sb[NC] = {};
+    for (int f = m.own_row[c]; f < m.own_row[c + 1]; ++f)
+#pragma unroll
+        for (int q = 0; q < NC; ++q) so[q] += phi_s[f * NC + q];
+    for (int k = m.nbr_row[c]; k < m.nbr_row[c + 1]; ++k) {
+        const int f = m.nbr_face[k];
+#pragma unroll
+        for (int q = 0; q < NC; ++q) sn[q] += -phi_s[f * NC + q];
+    }
+    for (int k = m.bnd_row[c]; k < m.bnd_row[c + 1]; ++k)
+#pragma unroll
+        for (int q = 0; q < NC; ++q) sb[q] += phi_b[k * NC + q];
+    const double V = m.volumes[c];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) out[c * NC + q] = ((so[q] + sn[q]) + sb[q]) / V;
+}
+
+// ---------------------------------------------------------------- assembly
+__device__ __forceinline__ double gamma_face(const gf_assembly &a, const gf_mesh &m, int f) {
+    if (!a.gamma_cell) return a.gamma;
+    const double w = m.w[f];
+    return w * a.gamma_cell[m.owner[f]] + (1.0 - w) * a.gamma_cell[m.neighbour[f]];
+}
+
+template <int NC>
+__global__ void k_asm_faces(const gf_mesh m, const gf_assembly a, double *upper, double *lower, double *ffc) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.n_faces) return;
+    double up = 0.0, lo = 0.0;
+    if (a.s_div != 0.0) {
+        const double ph = a.phi_s[f], w = m.w[f];
+        double ud, ld;
+        if (a.div_scheme == 0) {
+            ud = fmin(ph, 0.0);
+            ld = -fmax(ph, 0.0);
+        } else {
+            ud = ph * (1.0 - w);
+            ld = -ph * w;
+        }
+        up = a.s_div * ud;
+        lo = a.s_div * ld;
+    }
+    if (a.s_lap != 0.0) {
+        const double coeff = gamma_face(a, m, f) * m.mag_sf[f] * m.delta[f];
+        up = up + a.s_lap * coeff;
+        lo = lo + a.s_lap * coeff;
+    }
+    upper[f] = up;
+    lower[f] = lo;
+    if (ffc)
+#pragma unroll
+        for (int q = 0; q < NC; ++q) ffc[f * NC + q] = 0.0;
+}
+
+template <int NC>
+__global__ void k_asm_cells(const gf_mesh m, const gf_assembly a, double *diag, double *source) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.n_cells) return;
+    const double V = m.volumes[c];
+    double dg = 0.0;
+    double src[NC] = {};
+    if (a.s_ddt != 0.0) {
+        const double vdt = V / a.dt;
+        dg = a.s_ddt * (a.ddt_a * vdt);
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+            double hist = a.ddt_b * a.psi_old[c * NC + q];
+            if (a.ddt_c != 0.0) hist = hist - a.ddt_c * a.psi_older[c * NC + q];
+            src[q] = a.s_ddt * (vdt * hist);
+        }
+    }
+    if (a.s_div != 0.0) {
+        double so = 0.0, sn = 0.0, sb = 0.0;
+        for (int f = m.own_row[c]; f < m.own_row[c + 1]; ++f) {
+            const double ph = a.phi_s[f];
+            so += a.div_scheme == 0 ? fmax(ph, 0.0) : ph * m.w[f];
+        }
+        for (int k = m.nbr_row[c]; k < m.nbr_row[c + 1]; ++k) {
+            const int f = m.nbr_face[k];
+            const double ph = a.phi_s[f];
+            sn += a.div_scheme == 0 ? -fmin(ph, 0.0) : -ph * (1.0 - m.w[f]);
+        }
+        double bs[NC] = {};
+        for (int k = m.bnd_row[c]; k < m.bnd_row[c + 1]; ++k) {
+            if (!a.covered[k]) continue;
+            const double fr = a.frac[k], flux = a.phi_b[k], dd = m.bnd_mag_d[k];
+            sb += flux * (1.0 - fr);
+#pragma unroll
+            for (int q = 0; q < NC; ++q) {
+                const double bsrc = fr * a.ref_v[k * NC + q] + (1.0 - fr) * dd * a.ref_g[k * NC + q];
+                bs[q] += -(flux * bsrc);
+            }
+        }
+        dg = dg + a.s_div * ((so + sn) + sb);
+#pragma unroll
+        for (int q = 0; q < NC; ++q) src[q] = src[q] + a.s_div * bs[q];
+    }
+    if (a.s_lap != 0.0) {
+        double so = 0.0, sn = 0.0, sb = 0.0;
+        for (int f = m.own_row[c]; f < m.own_row[c + 1]; ++f) so += -(gamma_face(a, m, f) * m.mag_sf[f] * m.delta[f]);
+        for (int k = m.nbr_row[c]; k < m.nbr_row[c + 1]; ++k) {
+            const int f = m.nbr_face[k];
+            sn += -(gamma_face(a, m, f) * m.mag_sf[f] * m.delta[f]);
+        }
+        double bs[NC] = {};
+        const double gb = a.gamma_cell ? a.gamma_cell[c] : a.gamma;
+        for (int k = m.bnd_row[c]; k < m.bnd_row[c + 1]; ++k) {
+            if (!a.covered[k]) continue;
+            const double fr = a.frac[k];
+            const double cond = gb * m.bnd_mag_sf[k];
+            const double coef = fr != 0.0 ? fr / m.bnd_mag_d[k] : 0.0;
+            const double cv = cond * coef, cg = cond * (1.0 - fr);
+            sb += -cv;
+#pragma unroll
+            for (int q = 0; q < NC; ++q) bs[q] += -(cv * a.ref_v[k * NC + q] + cg * a.ref_g[k * NC + q]);
+        }
+        dg = dg + a.s_lap * ((so + sn) + sb);
+#pragma unroll
+        for (int q = 0; q < NC; ++q) src[q] = src[q] + a.s_lap * bs[q];
+    }
+    diag[c] = dg;
+#pragma unroll
+    for (int q = 0; q < NC; ++q) source[c * NC + q] = src[q];
+}
+
+// Hanging-face explicit sources, applied after k_asm_cells by the one thread
+// that owns each hang cell (fvm/laplacian.py:163-177, schemes/div.py:126-137).
+template <int NC>
+__global__ void k_asm_hang(const gf_mesh m, const gf_assembly a, double *source, double *ffc) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.n_hang_cells) return;
+    const int c = m.hang_cell[i];
+    const bool do_div = a.s_div != 0.0 && a.div_scheme == 1;
+    const bool do_lap = a.s_lap != 0.0 && a.lap_corrected;
+    if (!do_div && !do_lap) return;
+    double add[NC] = {};
+    for (int k = m.hc_row[i]; k < m.hc_row[i + 1]; ++k) {
+        const int code = m.hc_code[k];
+        const bool own = code >= 0;
+        const int h = own ? code : -code - 1;
+        const int f = m.hang_face[h];
+        const int o = m.owner[f], n = m.neighbour[f];
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+            if (do_div) {
+                const double w = m.w[f];
+                const double base = w * a.psi[o * NC + q] + (1.0 - w) * a.psi[n * NC + q];
+                const double corrected = base + hang_interp_corr<NC>(m, a.grad, h, q);
+                const double sface = -a.phi_s[f] * (corrected - base);
+                add[q] += a.s_div * (own ? sface : -sface);
+            }
+            if (do_lap) {
+                const double lsrc = (gamma_face(a, m, f) * m.mag_sf[f]) * hang_sn_corr<NC>(m, a.grad, h, q);
+                add[q] += a.s_lap * (own ? -lsrc : lsrc);
+                if (own && ffc) ffc[f * NC + q] = a.s_lap * lsrc;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NC; ++q) source[c * NC + q] += add[q];
+}
+
+// ---------------------------------------------------------------- LDU algebra
+template <int NC>
+__global__ void k_ldu_multiply(const gf_mesh m, const double *diag, const double *upper, const double *lower,
+                               const double *x, double *y) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.n_cells) return;
+    double so[NC] = {}, sn[NC] = {};
+    for (int f = m.own_row[c]; f < m.own_row[c + 1]; ++f) {
+        const int n = m.neighbour[f];
+        const double u = upper[f];
+#pragma unroll
+        for (int q = 0; q < NC; ++q) so[q] += u * x[n * NC + q];
+    }
+    for (int k = m.nbr_row[c]; k < m.nbr_row[c + 1]; ++k) {
+        const int f = m.nbr_face[k];
+        const int o = m.owner[f];
+        const double l = lower[f];
+#pragma unroll
+        for (int q = 0; q < NC; ++q) sn[q] += l * x[o * NC + q];
+    }
+    const double d = diag[c];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) y[c * NC + q] = (d * x[c * NC + q] + so[q]) + sn[q];
+}
+
+__global__ void k_rau_hbya(const gf_mesh m, const double *diag, const double *upper, const double *lower,
+                           const double *source, const double *U, double *rAU, double *HbyA) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.n_cells) return;
+    double so[3] = {}, sn[3] = {};
+    for (int f = m.own_row[c]; f < m.own_row[c + 1]; ++f) {
+        const int n = m.neighbour[f];
+        const double u = upper[f];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) so[q] += u * U[n * 3 + q];
+    }
+    for (int k = m.nbr_row[c]; k < m.nbr_row[c + 1]; ++k) {
+        const int f = m.nbr_face[k];
+        const int o = m.owner[f];
+        const double l = lower[f];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) sn[q] += l * U[o * 3 + q];
+    }
+    const double d = diag[c], V = m.volumes[c];
+    const double r = 1.0 / (d / V);
+    rAU[c] = r;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        const double x = U[c * 3 + q];
+        const double ax = (d * x + so[q]) + sn[q];
+        const double H = (source[c * 3 + q] - ax + d * x) / V;
+        HbyA[c * 3 + q] = r * H;
+    }
+}
+
+template <int NC>
+__global__ void k_ldu_flux(const gf_mesh m, double sign, const double *upper, const double *lower,
+                           const double *ffc, const double *psi, double *flux) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.n_faces) return;
+    const int o = m.owner[f], n = m.neighbour[f];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+        double v = upper[f] * psi[n * NC + q] - lower[f] * psi[o * NC + q];
+        if (ffc) v = v + ffc[f * NC + q];
+        flux[f * NC + q] = sign * v;
+    }
+}
+
+// ---------------------------------------------------------------- fluxes
+__global__ void k_face_flux_int(const gf_mesh m, const double *Uf, double *phi_s) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.n_faces) return;
+    phi_s[f] = Uf[3 * f + m.axis[f]] * m.mag_sf[f];
+}
+__global__ void k_face_flux_bnd(const gf_mesh m, const double *Ub, const uint8_t *covered, const uint8_t *mask,
+                                double *phi_b) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m.n_bnd) return;
+    if (mask && !mask[k]) return;
+    const int a = m.bnd_dir[k] / 2;
+    const double sf = m.bnd_sign[k] * m.bnd_mag_sf[k];
+    phi_b[k] = (covered && !covered[k]) ? 0.0 : Ub[3 * k + a] * sf;
+}
+
+__global__ void k_ddt_corr_add(const gf_mesh m, const double *U0, const double *U00, const double *phi0,
+                               const double *phi00, const double *rAU, double b, double cc, double dt,
+                               double *phi_s) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.n_faces) return;
+    const int o = m.owner[f], n = m.neighbour[f], a = m.axis[f];
+    const double w = m.w[f], sf = m.mag_sf[f];
+    const double u0f = w * U0[3 * o + a] + (1.0 - w) * U0[3 * n + a];
+    const double corr = phi0[f] - u0f * sf;
+    double hist = b * corr;
+    if (cc != 0.0) {
+        const double u00f = w * U00[3 * o + a] + (1.0 - w) * U00[3 * n + a];
+        hist = hist - cc * (phi00[f] - u00f * sf);
+    }
+    const double tiny = 2.2250738585072014e-308;
+    const double coupling = 1.0 - fmin(fabs(corr) / (fabs(phi0[f]) + tiny), 1.0);
+    const double rf = w * rAU[o] + (1.0 - w) * rAU[n];
+    phi_s[f] = phi_s[f] + rf * (coupling * hist / dt);
+}
+
+// adjust_phi.py:44-127 -- one workgroup, fixed summation order
+__global__ void k_adjust_phi(const gf_mesh m, const int8_t *kind, double *phi_b) {
+    double v[3] = {0.0, 0.0, 0.0};  // mass_in, fixed_out, adjustable_out
+    for (int k = threadIdx.x; k < m.n_bnd; k += blockDim.x) {
+        const int8_t t = kind[k];
+        if (!t) continue;
+        const double ph = phi_b[k];
+        if (ph < 0.0) v[0] -= ph;
+        if (ph > 0.0) {
+            if (t == 1) v[1] += ph;
+            else v[2] += ph;
+        }
+    }
+    const bool mx[3] = {false, false, false};
+    block_reduce<3, 1024>(v, mx);
+    __shared__ double corr;
+    __shared__ int apply;
+    if (threadIdx.x == 0) {
+        apply = v[2] > 0.0;
+        corr = apply ? (v[0] - v[1]) / v[2] : 1.0;
+    }
+    __syncthreads();
+    if (!apply) return;
+    for (int k = threadIdx.x; k < m.n_bnd; k += blockDim.x)
+        if (kind[k] == 2 && phi_b[k] > 0.0) phi_b[k] = phi_b[k] * corr;
+}
+
+__global__ void k_correct_phi_int(const gf_mesh m, const double *phi_h, const double *upper, const double *lower,
+                                  const double *ffc, const double *p, double *phi_s) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.n_faces) return;
+    const int o = m.owner[f], n = m.neighbour[f];
+    double v = upper[f] * p[n] - lower[f] * p[o];
+    if (ffc) v = v + ffc[f];
+    phi_s[f] = phi_h[f] - (-v);
+}
+__global__ void k_correct_phi_bnd(const gf_mesh m, const double *phi_hb, const double *p, const double *rAtU,
+                                  const double *frac, const double *rv, const double *rg, const uint8_t *covered,
+                                  double *phi_b) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m.n_bnd) return;
+    double out = phi_hb[k];
+    if (covered[k]) {
+        const int c = m.bnd_owner[k];
+        const double fr = frac[k];
+        const double coef = fr != 0.0 ? fr / m.bnd_mag_d[k] : 0.0;
+        const double sng = coef * (rv[k] - p[c]) + (1.0 - fr) * rg[k];
+        out = out - rAtU[c] * m.bnd_mag_sf[k] * sng;
+    }
+    phi_b[k] = out;
+}
+
+__global__ void k_correct_velocity(int C, const double *HbyA, const double *rAtU, const double *gp, double *U) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    const double r = rAtU[c];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) U[3 * c + q] = HbyA[3 * c + q] - r * gp[3 * c + q];
+}
+
+__global__ void k_set_reference(double *diag, double *source, int cell, double value) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double rd = diag[cell];
+    diag[cell] = diag[cell] + rd;
+    source[cell] = source[cell] + rd * value;
+}
+
+__global__ void k_axpby(int64_t n, double a, const double *x, double b, const double *y, double *out) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = a * x[i] + b * y[i];
+}
+
+// residual.py:12-36 -- ||A psi - b|| / (||b|| + eps), deterministic two-level sum
+template <int NC>
+__global__ __launch_bounds__(kBlock) void k_init_res(const gf_mesh m, const double *diag, const double *upper,
+                                                     const double *lower, const double *src, const double *psi,
+                                                     double *partials, unsigned int *counter, double *out) {
+    int c = blockIdx.x * kBlock + threadIdx.x;
+    double v[2] = {0.0, 0.0};
+    if (c < m.n_cells) {
+        double so[NC] = {}, sn[NC] = {};
+        for (int f = m.own_row[c]; f < m.own_row[c + 1]; ++f)
+#pragma unroll
+            for (int q = 0; q < NC; ++q) so[q] += upper[f] * psi[m.neighbour[f] * NC + q];
+        for (int k = m.nbr_row[c]; k < m.nbr_row[c + 1]; ++k) {
+            const int f = m.nbr_face[k];
+#pragma unroll
+            for (int q = 0; q < NC; ++q) sn[q] += lower[f] * psi[m.owner[f] * NC + q];
+        }
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+            const double ax = (diag[c] * psi[c * NC + q] + so[q]) + sn[q];
+            const double r = ax - src[c * NC + q];
+            v[0] += r * r;
+            v[1] += src[c * NC + q] * src[c * NC + q];
+        }
+    }
+    const bool mx[2] = {false, false};
+    block_reduce<2, kBlock>(v, mx);
+    const int nb = gridDim.x;
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = v[0];
+        partials[nb + blockIdx.x] = v[1];
+    }
+    if (!last_block_done(counter)) return;
+    double a = reduce_partials<kBlock>(partials, nb, false);
+    double b = reduce_partials<kBlock>(partials + nb, nb, false);
+    if (threadIdx.x == 0) out[0] = sqrt(a) / (sqrt(b) + 2.220446049250313e-16);
+}
+
+__global__ void k_zero_u32(unsigned int *p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// ldu_grads.py:11-53
+template <int NC>
+__global__ void k_ldu_grads(const gf_mesh m, const double *lam, const double *x, double *gd, double *gu,
+                            double *gl) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m.n_cells) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < NC; ++q) s += lam[i * NC + q] * x[i * NC + q];
+        gd[i] = -s;
+    }
+    if (i < m.n_faces) {
+        const int o = m.owner[i], n = m.neighbour[i];
+        double su = 0.0, sl = 0.0;
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+            su += lam[o * NC + q] * x[n * NC + q];
+            sl += lam[n * NC + q] * x[o * NC + q];
+        }
+        gu[i] = -su;
+        gl[i] = -sl;
+    }
+}
+
+// simple.py:302-311: A_new = A/alpha ; source += ((1-alpha)/alpha) A_old U
+__global__ void k_relax_momentum(int C, double alpha, double *diag, double *source, const double *U) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    const double a_old = diag[c];
+    diag[c] = a_old / alpha;
+    const double rc = ((1.0 - alpha) / alpha) * a_old;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) source[3 * c + q] = source[3 * c + q] + rc * U[3 * c + q];
+}
+
+// piso.py:247-250 and pressure_correction.py:121-124: out = source - V x
+__global__ void k_predictor_source(int C, int nc, const double *source, const double *V, const double *x, double *out) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    for (int q = 0; q < nc; ++q) out[nc * c + q] = source[nc * c + q] - x[nc * c + q] * V[c];
+}
+
+// simple.py:386: p = p_old + alpha (p - p_old)
+__global__ void k_relax_field(int64_t n, double alpha, const double *old, double *x) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    x[i] = old[i] + alpha * (x[i] - old[i]);
+}
+
+}  // namespace
+}  // namespace gf
+
+using namespace gf;
+
+#define GF_NC_DISPATCH(nc, KERNEL, GRID, ...)                                                          \
+    do {                                                                                               \
+        if ((nc) == 1)                                                                                 \
+            hipLaunchKernelGGL(KERNEL<1>, dim3(GRID), dim3(kBlock), 0, s, __VA_ARGS__);               \
+        else if ((nc) == 3)                                                                            \
+            hipLaunchKernelGGL(KERNEL<3>, dim3(GRID), dim3(kBlock), 0, s, __VA_ARGS__);               \
+        else {                                                                                         \
+            set_error("component count must be 1 or 3");                                               \
+            return GF_EINVAL;                                                                          \
+        }                                                                                              \
+    } while (0)
+
+extern "C" const char *gf_last_error(void) { return g_err.c_str(); }
+
+extern "C" int gf_device_count(int *count) {
+    GF_REQUIRE(count, "null argument");
+    GF_CHECK_HIP(hipGetDeviceCount(count));
+    return GF_OK;
+}
+
+extern "C" int gf_bc_state(const gf_mesh *m, int32_t nc, const double *psi, const double *frac, const double *ref_v,
+                           const double *ref_g, const uint8_t *covered, double *psi_b, gf_stream_t stream) {
+    GF_REQUIRE(m && psi && frac && ref_v && ref_g && covered && psi_b, "null argument");
+    if (m->n_bnd == 0) return GF_OK;
+    hipLaunchKernelGGL(k_bc_state, dim3(grid_for(m->n_bnd)), dim3(kBlock), 0, S(stream), *m, nc, psi, frac, ref_v,
+                       ref_g, covered, psi_b);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
+extern "C" int gf_bc_slip(const gf_mesh *m, const double *psi, const uint8_t *slip_mask, double *ref_v,
+                          gf_stream_t stream) {
+    GF_REQUIRE(m && psi && slip_mask && ref_v, "null argument");
+    if (m->n_bnd == 0) return GF_OK;
+    hipLaunchKernelGGL(k_bc_slip, dim3(grid_for(m->n_bnd)), dim3(kBlock), 0, S(stream), *m, psi, slip_mask, ref_v);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
+extern "C" int gf_lsq_grad(const gf_mesh *m, int32_t nc, const double *psi, const double *psi_b,
+                           const uint8_t *covered, int32_t hang_only, double *grad, gf_stream_t stream) {
+    GF_REQUIRE(m && psi && psi_b && covered && grad, "null argument");
+    hipStream_t s = S(stream);
+    const int32_t *list = hang_only ? m->hang_cell : nullptr;
+    const int n = hang_only ? m->n_hang_cells : m->n_cells;
+    if (n == 0) return GF_OK;
+    GF_NC_DISPATCH(nc, k_lsq_grad, grid_for(n), *m, psi, psi_b, covered, list, n, grad);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
+extern "C" int gf_interpolate(const gf_mesh *m, int32_t nc, const double *psi, const double *grad, double *psi_f,
+                              gf_stream_t stream) {
+    GF_REQUIRE(m && psi && psi_f, "null argument");
+    hipStream_t s = S(stream);
+    if (m->n_faces == 0) return GF_OK;
+    GF_NC_DISPATCH(nc, k_interp_linear, grid_for(m->n_faces), *m, psi, psi_f);
+    if (m->n_hang > 0) {
+        GF_REQUIRE(grad, "hanging faces need a gradient");
+        GF_NC_DISPATCH(nc, k_interp_hang, grid_for(m->n_hang), *m, grad, psi_f);
+    }
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
+extern "C" int gf_fvc_div(const gf_mesh *m, int32_t nc, const double *phi_s, const double *phi_b, double *out,
+                          gf_stream_t stream) {
+    GF_REQUIRE(m && phi_s && phi_b && out, "null argument");
+    hipStream_t s = S(stream);
+    GF_NC_DISPATCH(nc, k_fvc_div, grid_for(m->n_cells), *m, phi_s, phi_b, out);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
+extern "C" int gf_sn_grad_hang(const gf_mesh *m, int32_t nc, const double *grad, double *corr, gf_stream_t stream) {
+    GF_REQUIRE(m && grad && corr, "null argument");
+    hipStream_t s = S(stream);
+    if (m->n_hang == 0) return GF_OK;
+    GF_NC_DISPATCH(nc, k_sn_grad_hang, grid_for(m->n_hang), *m, grad, corr);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
+extern "C" int gf_assemble(const gf_mesh *m, const gf_assembly *a, double *diag, double *upper, double *lower,
+                           double *source, double *ffc, gf_stream_t stream) {
+    GF_REQUIRE(m && a && diag && upper && lower && source, "null argument");
+    GF_REQUIRE(a->s_ddt == 0.0 || (a->psi_old && a->dt > 0.0), "ddt needs old values and dt > 0");
+    GF_REQUIRE(a->s_ddt == 0.0 || a->ddt_c == 0.0 || a->psi_older, "backward ddt needs older values");
+    GF_REQUIRE(a->s_div == 0.0 || (a->phi_s && a->phi_b), "div needs face flux");
+    GF_REQUIRE(a->covered && a->frac && a->ref_v && a->ref_g, "boundary state missing");
+    const bool need_grad = m->n_hang > 0 && ((a->s_div != 0.0 && a->div_scheme == 1) || (a->s_lap != 0.0 && a->lap_corrected));
+    GF_REQUIRE(!need_grad || (a->grad && a->psi), "hanging faces need psi and its gradient");
+    hipStream_t s = S(stream);
+    GF_NC_DISPATCH(a->nc, k_asm_faces, grid_for(m->n_faces), *m, *a, upper, lower, ffc);
+    GF_NC_DISPATCH(a->nc, k_asm_cells, grid_for(m->n_cells), *m, *a, diag, source);
+    if (need_grad && m->n_hang_cells > 0)
+        GF_NC_DISPATCH(a->nc, k_asm_hang, grid_for(m->n_hang_cells), *m, *a, source, ffc);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
+extern "C" int gf_ldu_multiply(const gf_mesh *m, int32_t nc, const double *diag, const double *upper,
+                               const double *lower, const double *x, double *y, gf_stream_t stream) {
+    GF_REQUIRE(m && diag && upper && lower && x && y, "null argument");
+    hipStream_t s = S(stream);
+    GF_NC_DISPATCH(nc, k_ldu_multiply, grid_for(m->n_cells), *m, diag, upper, lower, x, y);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
+extern "C" int gf_rau_hbya(const gf_mesh *m, const double *diag, const double *upper, const double *lower,
+                           const double *source, const double *U, double *rAU, double *HbyA, gf_stream_t stream) {
+    GF_REQUIRE(m && diag && upper && lower && source && U && rAU && HbyA, "null argument");
+    hipLaunchKernelGGL(k_rau_hbya, dim3(grid_for(m->n_cells)), dim3(kBlock), 0, S(stream), *m, diag, upper, lower,
+                       source, U, rAU, HbyA);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
+extern "C" int gf_ldu_flux(const gf_mesh *m, int32_t nc, double sign, const double *upper, const double *lower,
+                           const double *ffc, const double *psi, double *flux, gf_stream_t stream) {
+    GF_REQUIRE(m && upper && lower && psi && flux, "null argument");
+    hipStream_t s = S(stream);
+    if (m->n_faces == 0) return GF_OK;
+    GF_NC_DISPATCH(nc, k_ldu_flux, grid_for(m->n_faces), *m, sign, upper, lower, ffc, psi, flux);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
+extern "C" int gf_face_flux(const gf_mesh *m, const double *Uf_s, const double *Uf_b, const uint8_t *covered,
+                            double *phi_s, double *phi_b, gf_stream_t stream) {
+    GF_REQUIRE(m && Uf_s && Uf_b && covered && phi_s && phi_b, "null argument");
+    hipStream_t s = S(stream);
+    if (m->n_faces)
+        hipLaunchKernelGGL(k_face_flux_int, dim3(grid_for(m->n_faces)), dim3(kBlock), 0, s, *m, Uf_s, phi_s);
+    if (m->n_bnd)
+        hipLaunchKernelGGL(k_face_flux_bnd, dim3(grid_for(m->n_bnd)), dim3(kBlock), 0, s, *m, Uf_b, covered,
+                           (const uint8_t *)nullptr, phi_b);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
+extern "C" int gf_ddt_corr_add(const gf_mesh *m, const double *U_old, const double *U_older, const double *phi_old,
+                               const double *phi_older, const double *rAU, double b, double c, double dt,
+                               double *phi_s, gf_stream_t stream) {
+    GF_REQUIRE(m && U_old && phi_old && rAU && phi_s && dt > 0.0, "null argument");
+    GF_REQUIRE(c == 0.0 || (U_older && phi_older), "backward ddtCorr needs older history");
+    if (m->n_faces == 0) return GF_OK;
+    hipLaunchKernelGGL(k_ddt_corr_add, dim3(grid_for(m->n_faces)), dim3(kBlock), 0, S(stream), *m, U_old, U_older,
+                       phi_old, phi_older, rAU, b, c, dt, phi_s);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
+extern "C" int gf_constrain_bnd_flux(const gf_mesh *m, const double *psi_b, const uint8_t *mask, double *phi_b,
+                                     gf_stream_t stream) {
+    GF_REQUIRE(m && psi_b && mask && phi_b, "null argument");
+    if (m->n_bnd == 0) return GF_OK;
+    hipLaunchKernelGGL(k_face_flux_bnd, dim3(grid_for(m->n_bnd)), dim3(kBlock), 0, S(stream), *m, psi_b,
+                       (const uint8_t *)nullptr, mask, phi_b);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
+extern "C" int gf_adjust_phi(const gf_mesh *m, const int8_t *kind, double *phi_b, double *scratch,
+                             gf_stream_t stream) {
+    (void)scratch;
+    GF_REQUIRE(m && kind && phi_b, "null argument");
+    if (m->n_bnd == 0) return GF_OK;
+    hipLaunchKernelGGL(k_adjust_phi, dim3(1), dim3(1024), 0, S(stream), *m, kind, phi_b);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
+extern "C" int gf_correct_phi(const gf_mesh *m, const double *phi_hbya_s, const double *phi_hbya_b,
+                              const double *upper, const double *lower, const double *ffc, const double *p,
+                              const double *rAtU, const double *frac, const double *ref_v, const double *ref_g,
+                              const uint8_t *covered, double *phi_s, double *phi_b, gf_stream_t stream) {
+    GF_REQUIRE(m && phi_hbya_s && phi_hbya_b && upper && lower && p && rAtU && frac && ref_v && ref_g && covered &&
+                   phi_s && phi_b,
+               "null argument");
+    hipStream_t s = S(stream);
+    if (m->n_faces)
+        hipLaunchKernelGGL(k_correct_phi_int, dim3(grid_for(m->n_faces)), dim3(kBlock), 0, s, *m, phi_hbya_s, upper,
+                           lower, ffc, p, phi_s);
+    if (m->n_bnd)
+        hipLaunchKernelGGL(k_correct_phi_bnd, dim3(grid_for(m->n_bnd)), dim3(kBlock), 0, s, *m, phi_hbya_b, p, rAtU,
+                           frac, ref_v, ref_g, covered, phi_b);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
+extern "C" int gf_correct_velocity(int32_t n_cells, const double *HbyA, const double *rAtU, const double *grad_p,
+                                   double *U, gf_stream_t stream) {
+    GF_REQUIRE(HbyA && rAtU && grad_p && U, "null argument");
+    hipLaunchKernelGGL(k_correct_velocity, dim3(grid_for(n_cells)), dim3(kBlock), 0, S(stream), n_cells, HbyA, rAtU,
+                       grad_p, U);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
+extern "C" int gf_set_reference(double *diag, double *source, int32_t cell, double value, gf_stream_t stream) {
+    GF_REQUIRE(diag && source && cell >= 0, "bad reference arguments");
+    hipLaunchKernelGGL(k_set_reference, dim3(1), dim3(64), 0, S(stream), diag, source, cell, value);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
+extern "C" int gf_axpby(int64_t n, double a, const double *x, double b, const double *y, double *out,
+                        gf_stream_t stream) {
+    GF_REQUIRE(x && y && out, "null argument");
+    if (n == 0) return GF_OK;
+    hipLaunchKernelGGL(k_axpby, dim3(grid_for(n)), dim3(kBlock), 0, S(stream), n, a, x, b, y, out);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
+extern "C" int gf_initial_residual(const gf_mesh *m, int32_t nc, const double *diag, const double *upper,
+                                   const double *lower, const double *source, const double *psi, double *scratch,
+                                   double *out, gf_stream_t stream) {
+    GF_REQUIRE(m && diag && upper && lower && source && psi && scratch && out, "null argument");
+    hipStream_t s = S(stream);
+    const int nb = grid_for(m->n_cells);
+    unsigned int *counter = reinterpret_cast<unsigned int *>(scratch + 2 * nb);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, counter, 2);
+    GF_NC_DISPATCH(nc, k_init_res, nb, *m, diag, upper, lower, source, psi, scratch, counter, out);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
+extern "C" int gf_ldu_grads(const gf_mesh *m, int32_t nc, const double *lambda, const double *x, double *g_diag,
+                            double *g_upper, double *g_lower, gf_stream_t stream) {
+    GF_REQUIRE(m && lambda && x && g_diag && g_upper && g_lower, "null argument");
+    hipStream_t s = S(stream);
+    const int n = m->n_cells > m->n_faces ? m->n_cells : m->n_faces;
+    GF_NC_DISPATCH(nc, k_ldu_grads, grid_for(n), *m, lambda, x, g_diag, g_upper, g_lower);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
+extern "C" int gf_relax_momentum(int32_t n_cells, double alpha, double *diag, double *source, const double *U,
+                                 gf_stream_t stream) {
+    GF_REQUIRE(diag && source && U && alpha > 0.0, "bad relaxation arguments");
+    hipLaunchKernelGGL(k_relax_momentum, dim3(grid_for(n_cells)), dim3(kBlock), 0, S(stream), n_cells, alpha, diag,
+                       source, U);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
+extern "C" int gf_predictor_source(int32_t n_cells, int32_t nc, const double *source, const double *volumes,
+                                   const double *x, double *out, gf_stream_t stream) {
+    GF_REQUIRE(source && volumes && x && out && (nc == 1 || nc == 3), "bad arguments");
+    hipLaunchKernelGGL(k_predictor_source, dim3(grid_for(n_cells)), dim3(kBlock), 0, S(stream), n_cells, nc, source,
+                       volumes, x, out);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
+extern "C" int gf_relax_field(int64_t n, double alpha, const double *old, double *x, gf_stream_t stream) {
+    GF_REQUIRE(old && x, "null argument");
+    if (n == 0) return GF_OK;
+    hipLaunchKernelGGL(k_relax_field, dim3(grid_for(n)), dim3(kBlock), 0, S(stream), n, alpha, old, x);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
+extern "C" int gf_abi_sizes(int64_t out[3]) {
+    GF_REQUIRE(out, "null argument");
+    out[0] = (int64_t)sizeof(gf_mesh);
+    out[1] = (int64_t)sizeof(gf_assembly);
+    out[2] = (int64_t)sizeof(gf_krylov_cfg);
+    return GF_OK;
+}

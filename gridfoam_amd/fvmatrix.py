@@ -1,0 +1,180 @@
+"""LDU matrix container on the device (src/gridfoam/core/fvmatrix.py).
+
+Storage is the reference's: ``diag [C]``, ``upper/lower [F]`` (owner row:
+upper, neighbour row: lower), ``source [C, *cs]`` and an optional explicit
+face-flux correction ``[F_single, *cs]``.  Products, H, flux and the algebra
+run on the HIP library.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from typing import Optional
+
+import torch
+
+from . import _native as nat
+from .fields import CellField
+
+
+def _require(t: torch.Tensor, shape, what: str) -> None:  # core/shapes.py:29-36
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{what}: expected shape {tuple(shape)}, got {tuple(t.shape)}")
+
+
+class FvMatrix:
+    def __init__(self, field: CellField):
+        g = field.grid
+        self.field = field
+        self.grid = g
+        C, F = g.num_cells, g.num_internal_faces
+        dev = g.device
+        self._diag = torch.zeros(C, dtype=torch.float64, device=dev)
+        self._upper = torch.zeros(max(F, 1), dtype=torch.float64, device=dev)
+        self._lower = torch.zeros(max(F, 1), dtype=torch.float64, device=dev)
+        self._source = torch.zeros((C, *field.component_shape), dtype=torch.float64, device=dev)
+        self.face_flux_correction: Optional[torch.Tensor] = None
+
+    # ---- storage ---------------------------------------------------------
+    @property
+    def num_components(self) -> int:
+        return self.field.nc
+
+    @property
+    def diag(self) -> torch.Tensor:
+        return self._diag
+
+    @diag.setter
+    def diag(self, v: torch.Tensor) -> None:
+        _require(v, (self.grid.num_cells,), "matrix diag")
+        self._diag = v.contiguous()
+
+    @property
+    def upper(self) -> torch.Tensor:
+        return self._upper[: self.grid.num_internal_faces]
+
+    @upper.setter
+    def upper(self, v: torch.Tensor) -> None:
+        _require(v, (self.grid.num_internal_faces,), "matrix upper")
+        self._upper[: self.grid.num_internal_faces] = v
+
+    @property
+    def lower(self) -> torch.Tensor:
+        return self._lower[: self.grid.num_internal_faces]
+
+    @lower.setter
+    def lower(self, v: torch.Tensor) -> None:
+        _require(v, (self.grid.num_internal_faces,), "matrix lower")
+        self._lower[: self.grid.num_internal_faces] = v
+
+    @property
+    def source(self) -> torch.Tensor:
+        return self._source
+
+    @source.setter
+    def source(self, v: torch.Tensor) -> None:
+        _require(v, (self.grid.num_cells, *self.field.component_shape), "matrix source")
+        self._source = v.contiguous()
+
+    # ---- algebra (fvmatrix.py:181-240) -------------------------------------
+    def _combine(self, other: "FvMatrix", sign: float) -> "FvMatrix":
+        res = FvMatrix(self.field)
+        st = nat.stream()
+        for a, b, out in ((self._diag, other._diag, res._diag), (self._upper, other._upper, res._upper),
+                          (self._lower, other._lower, res._lower), (self._source, other._source, res._source)):
+            nat.call("gf_axpby", ctypes.c_int64(a.numel()), ctypes.c_double(1.0), nat.ptr(a),
+                     ctypes.c_double(sign), nat.ptr(b), nat.ptr(out), st)
+        fa, fb = self.face_flux_correction, other.face_flux_correction
+        if fa is None and fb is None:
+            res.face_flux_correction = None
+        else:
+            za = fa if fa is not None else torch.zeros_like(fb)
+            zb = fb if fb is not None else torch.zeros_like(fa)
+            out = torch.empty_like(za)
+            nat.call("gf_axpby", ctypes.c_int64(za.numel()), ctypes.c_double(1.0), nat.ptr(za),
+                     ctypes.c_double(sign), nat.ptr(zb), nat.ptr(out), st)
+            res.face_flux_correction = out
+        return res
+
+    def __add__(self, other: "FvMatrix") -> "FvMatrix":
+        return self._combine(other, 1.0)
+
+    def __sub__(self, other: "FvMatrix") -> "FvMatrix":
+        return self._combine(other, -1.0)
+
+    def __neg__(self) -> "FvMatrix":
+        zero = FvMatrix(self.field)
+        return zero._combine(self, -1.0)
+
+    def with_source(self, source: torch.Tensor) -> "FvMatrix":
+        res = FvMatrix(self.field)
+        res.source = source.clone()
+        res._diag = self._diag.clone()
+        res._upper = self._upper.clone()
+        res._lower = self._lower.clone()
+        if self.face_flux_correction is not None:
+            res.face_flux_correction = self.face_flux_correction.clone()
+        return res
+
+    def as_transpose(self) -> "FvMatrix":
+        """fvmatrix.py:396-412 -- shares storage, swaps upper and lower."""
+        view = FvMatrix.__new__(FvMatrix)
+        view.field, view.grid = self.field, self.grid
+        view._diag, view._upper, view._lower = self._diag, self._lower, self._upper
+        view._source = self._source
+        view.face_flux_correction = self.face_flux_correction
+        return view
+
+    # ---- operators ---------------------------------------------------------
+    def multiply(self, x: torch.Tensor) -> torch.Tensor:
+        """fvmatrix.py:292-323 -- deterministic gather SpMV."""
+        nc = 1 if x.dim() == 1 else x.shape[1]
+        y = torch.empty_like(x)
+        nat.call("gf_ldu_multiply", self.grid.mesh_ref, ctypes.c_int32(nc), nat.ptr(self._diag),
+                 nat.ptr(self._upper), nat.ptr(self._lower), nat.ptr(x.contiguous()), nat.ptr(y), nat.stream())
+        return y
+
+    def A(self) -> torch.Tensor:
+        return self._diag / self.grid.cell_volumes
+
+    def H(self, x: torch.Tensor) -> torch.Tensor:
+        d = self._diag if x.dim() == 1 else self._diag[:, None]
+        V = self.grid.cell_volumes if x.dim() == 1 else self.grid.cell_volumes[:, None]
+        return (self._source - self.multiply(x) + d * x) / V
+
+    def H1(self) -> torch.Tensor:
+        g = self.grid
+        h1 = torch.zeros_like(self._diag)
+        h1.index_add_(0, g.owner, -self.upper)
+        h1.index_add_(0, g.neighbour, -self.lower)
+        return h1 / g.cell_volumes
+
+    def flux(self, psi: torch.Tensor) -> torch.Tensor:
+        """fvmatrix.py:424-468."""
+        nc = 1 if psi.dim() == 1 else psi.shape[1]
+        F = self.grid.num_internal_faces
+        out = torch.empty((max(F, 1),) + tuple(psi.shape[1:]), dtype=torch.float64, device=psi.device)
+        ffc = self.face_flux_correction
+        nat.call("gf_ldu_flux", self.grid.mesh_ref, ctypes.c_int32(nc), ctypes.c_double(1.0), nat.ptr(self._upper),
+                 nat.ptr(self._lower), nat.ptr(ffc), nat.ptr(psi.contiguous()), nat.ptr(out), nat.stream())
+        return out[:F]
+
+    def with_fixed_values(self, cells: torch.Tensor, values: torch.Tensor) -> "FvMatrix":
+        """fvmatrix.py:325-394; cell constraints only arise from immersed
+        Dirichlet snapping, which this build does not produce."""
+        if cells.numel() == 0:
+            return self
+        raise NotImplementedError("immersed Dirichlet cell constraints are not part of this build")
+
+
+class Equation:
+    """core/equation.py:6-68 (no immersed constraints: solve matrix = input)."""
+
+    def __init__(self, target: CellField, fv_matrix: FvMatrix):
+        self.name = target.name
+        self.target = target
+        self.fv_matrix = fv_matrix
+
+
+def equation(target: CellField, fv_matrix: FvMatrix) -> Equation:
+    return Equation(target, fv_matrix)

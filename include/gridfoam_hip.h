@@ -1,0 +1,248 @@
+/*
+ * gridfoam_hip.h -- C ABI of the MI355X (gfx950) pressure-velocity path.
+ *
+ * The reference (ricosjp/gridfoam) has no FFI: its hot path sits behind
+ * Python plugin interfaces.  Every entry point below names the reference
+ * interface it replaces (file:line, relative to the reference checkout).
+ *
+ * Conventions
+ *   - every function is extern "C", takes device pointers, element counts
+ *     and a hipStream_t, and returns 0 on success or a negative code;
+ *     gf_last_error() returns a thread-local message for the last failure;
+ *   - time-step calls allocate no device memory; only the set-up helpers
+ *     (gf_octree_build, gf_exclusive_scan_i32, gf_fill_nbr_lists,
+ *     gf_hang_cell_lists) allocate temporary buffers, once per grid;
+ *   - fields are row-major: a cell field with nc components is [C][nc];
+ *   - all FV assembly is FP64; Krylov solves run in FP64 or in FP32 inside
+ *     an FP64 iterative-refinement loop.
+ */
+#ifndef GRIDFOAM_HIP_H
+#define GRIDFOAM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t *gf_stream_t; /* == hipStream_t */
+
+#define GF_OK 0
+#define GF_EINVAL (-1)
+#define GF_EHIP (-2)
+#define GF_ENOMEM (-3)
+
+/* Device-resident face-neighbour operator.  Filled by the host from
+ * buffers it owns (gridfoam_amd/core/grid.py); passed by pointer to every
+ * call.  Replaces the index/geometry tensors of GridBase
+ * (src/gridfoam/core/grid/base.py:19-275) and the cached FaceGeometry
+ * (src/gridfoam/fv/kernels/face_geometry.py:20-72). */
+typedef struct gf_mesh {
+    int32_t n_cells, n_faces, n_bnd, n_hang, n_hang_cells, n_slices;
+    int64_t nnz_pad; /* padded entries of the sliced-ELL operator */
+    /* topology */
+    const int32_t *owner, *neighbour, *axis; /* [F]  neighbour on +axis side */
+    const int32_t *bnd_owner, *bnd_dir;      /* [B]  dir = 2*axis + (side>0) */
+    const int32_t *own_row;                  /* [C+1] faces owned by a cell */
+    const int32_t *nbr_row, *nbr_face;       /* [C+1],[F] faces seen from the neighbour */
+    const int32_t *bnd_row;                  /* [C+1] boundary faces of a cell */
+    const int32_t *sell_ptr;                 /* [n_slices+1] slice start (entries) */
+    const int32_t *sell_col;                 /* [nnz_pad] column (cell) */
+    const int32_t *sell_code;                /* [nnz_pad] f: upper[f]; -f-1: lower[f]; INT32_MIN: pad */
+    /* geometry (FP64) */
+    const double *centers, *sizes, *volumes; /* [C,3],[C,3],[C] */
+    const double *w, *mag_sf, *delta;        /* [F] */
+    const double *bnd_d, *bnd_mag_d, *bnd_mag_sf, *bnd_sign; /* [B,3],[B],[B],[B] */
+    const double *ata_inv;                   /* [C,9] */
+    /* hanging (2:1) faces */
+    const int32_t *hang_face;                /* [H] face index */
+    const double *hang_offset, *hang_move_o, *hang_move_n; /* [H,3] */
+    const int32_t *hang_cell;                /* [HC] cells adjacent to a hanging face */
+    const int32_t *hc_row, *hc_code;         /* [HC+1], h (owner side) or -h-1 (neighbour side) */
+} gf_mesh;
+
+const char *gf_last_error(void);
+int gf_device_count(int *count);
+/* sizeof(gf_mesh), sizeof(gf_assembly), sizeof(gf_krylov_cfg): lets a
+ * binding check its struct layouts against the library (no GPU needed) */
+int gf_abi_sizes(int64_t out[3]);
+
+/* ---- octree face-neighbour operator (replaces fluxel-cfd's
+ * FluxelManager.build_axis_projected_mesh, src/gridfoam/core/grid/factory.py:32-65,
+ * consumed at src/gridfoam/core/grid/axis_projected.py:97-161) ---------- */
+typedef struct gf_octree gf_octree;
+/* counts_out = {n_cells, n_faces, n_bnd_faces, lmax} */
+int gf_octree_build(const double lower[3], const double upper[3], const int32_t root[3],
+                    int32_t n_regions, const double *rmin, const double *rmax,
+                    const int32_t *rlevel, int32_t n_leaf, gf_octree **out,
+                    int64_t counts_out[4], gf_stream_t stream);
+/* device-to-device copy into caller buffers */
+int gf_octree_fetch(const gf_octree *t, int32_t *level, int32_t *ijk, int32_t *f_owner,
+                    int32_t *f_nbr, int32_t *f_axis, int32_t *b_owner, int32_t *b_dir,
+                    gf_stream_t stream);
+void gf_octree_free(gf_octree *t);
+
+/* ---- grid geometry (axis_projected.py:130-161, :715-811) ---------------- */
+int gf_grid_geometry(int32_t n_cells, int32_t n_faces, int32_t n_bnd, const double lower[3],
+                     const double upper[3], const int32_t root[3], const int32_t *level,
+                     const int32_t *ijk, const int32_t *owner, const int32_t *nbr,
+                     const int32_t *axis, const int32_t *b_owner, const int32_t *b_dir,
+                     double *centers, double *sizes, double *volumes, double *face_centers,
+                     double *sf, double *bnd_centers, double *bnd_sf, gf_stream_t stream);
+
+/* ---- face geometry + operator layout (face_geometry.py:107-262) ---------
+ * Phase 1 computes per-face weights and hanging flags and the per-cell row
+ * counts; the host scans counts (gf_exclusive_scan) and calls phase 2. */
+int gf_face_geometry(const gf_mesh *m, const double *face_centers, double *w, double *mag_sf,
+                     double *delta, uint8_t *is_hang, double h_min, gf_stream_t stream);
+int gf_row_counts(const gf_mesh *m, int32_t *own_cnt, int32_t *nbr_cnt, int32_t *bnd_cnt,
+                  gf_stream_t stream);
+int gf_exclusive_scan_i32(const int32_t *in, int32_t *out, int32_t n, gf_stream_t stream);
+int gf_fill_nbr_lists(const gf_mesh *m, int32_t *nbr_face, gf_stream_t stream);
+int gf_sell_widths(const gf_mesh *m, int32_t *slice_len, gf_stream_t stream);
+int gf_sell_fill(const gf_mesh *m, int32_t *sell_col, int32_t *sell_code, gf_stream_t stream);
+int gf_lsq_ata_inv(const gf_mesh *m, double *ata_inv, gf_stream_t stream);
+int gf_hang_geometry(const gf_mesh *m, const double *face_centers, double *offset,
+                     double *move_o, double *move_n, gf_stream_t stream);
+int gf_hang_cell_lists(const gf_mesh *m, const int32_t *cell_to_hc, int32_t *hc_code,
+                       gf_stream_t stream);
+
+/* ---- boundary state (boundary_ops.py:311-326, :545-593) -----------------
+ * bc arrays are per boundary face: frac [B], ref_v [B,nc], ref_g [B,nc],
+ * covered [B] (1 when a non-empty boundary condition owns the face). */
+int gf_bc_state(const gf_mesh *m, int32_t nc, const double *psi, const double *frac,
+                const double *ref_v, const double *ref_g, const uint8_t *covered,
+                double *psi_b, gf_stream_t stream);
+/* slip: ref_v = psi_O - (psi_O . n) n on selected faces (boundaries/basic/slip.py:46-101) */
+int gf_bc_slip(const gf_mesh *m, const double *psi, const uint8_t *slip_mask, double *ref_v,
+               gf_stream_t stream);
+
+/* ---- explicit operators (fv/kernels/*.py, fv/fvc/*.py) ------------------ */
+/* least_squares.py:29-101; hang_only evaluates only cells next to hanging faces */
+int gf_lsq_grad(const gf_mesh *m, int32_t nc, const double *psi, const double *psi_b,
+                const uint8_t *covered, int32_t hang_only, double *grad, gf_stream_t stream);
+/* interpolate.py:20-63 + face_interpolation.py:114-197; grad valid on hang cells */
+int gf_interpolate(const gf_mesh *m, int32_t nc, const double *psi, const double *grad,
+                   double *psi_f, gf_stream_t stream);
+/* fvc/div.py:15-64 */
+int gf_fvc_div(const gf_mesh *m, int32_t nc, const double *phi_s, const double *phi_b,
+               double *out, gf_stream_t stream);
+/* fvc/sn_grad + face_interpolation.py:200-246: hanging snGrad correction [H,nc] */
+int gf_sn_grad_hang(const gf_mesh *m, int32_t nc, const double *grad, double *corr,
+                    gf_stream_t stream);
+
+/* ---- implicit assembly (fvm/laplacian.py:101-201, fvm/div.py:15-73,
+ * fvm/ddt.py:7-40, fvmatrix.py:181-211) -----------------------------------
+ * One fused kernel family: result = s_ddt*ddt + s_div*div(phi,psi)
+ *                                   + s_lap*laplacian(gamma,psi).
+ * Terms with a zero sign are skipped.  gamma is a scalar (gamma_cell NULL)
+ * or a cell field. Hanging-face explicit parts come from grad (hang cells). */
+typedef struct gf_assembly {
+    int32_t nc;
+    double s_ddt, s_div, s_lap;
+    /* ddt: a, b, c weights, dt; psi_old/psi_older [C,nc] (older may be NULL) */
+    double ddt_a, ddt_b, ddt_c, dt;
+    const double *psi_old, *psi_older;
+    /* div: scheme 0 upwind, 1 linear; phi_s [F], phi_b [B] */
+    int32_t div_scheme;
+    const double *phi_s, *phi_b;
+    /* laplacian */
+    double gamma;
+    const double *gamma_cell;
+    int32_t lap_corrected;
+    /* field + boundary state */
+    const double *psi, *grad;          /* [C,nc], [C,nc,3] (hang cells) */
+    const double *frac, *ref_v, *ref_g, *psi_b; /* boundary state */
+    const uint8_t *covered;
+} gf_assembly;
+int gf_assemble(const gf_mesh *m, const gf_assembly *a, double *diag, double *upper,
+                double *lower, double *source, double *ffc, gf_stream_t stream);
+
+/* ---- LDU algebra (fvmatrix.py:242-323, :424-468) ------------------------ */
+int gf_ldu_multiply(const gf_mesh *m, int32_t nc, const double *diag, const double *upper,
+                    const double *lower, const double *x, double *y, gf_stream_t stream);
+/* rAU = 1/(diag/V); HbyA = rAU * (b - A x + diag x)/V   (piso.py:264-267) */
+int gf_rau_hbya(const gf_mesh *m, const double *diag, const double *upper, const double *lower,
+                const double *source, const double *U, double *rAU, double *HbyA,
+                gf_stream_t stream);
+/* flux = upper psi_N - lower psi_O + ffc on internal faces */
+int gf_ldu_flux(const gf_mesh *m, int32_t nc, double sign, const double *upper,
+                const double *lower, const double *ffc, const double *psi, double *flux,
+                gf_stream_t stream);
+
+/* ---- flux helpers (flux.py:32-162, ddt_corr.py:13-78, adjust_phi.py:44-127,
+ * pressure_correction.py:197-363) ---------------------------------------- */
+int gf_face_flux(const gf_mesh *m, const double *Uf_s, const double *Uf_b,
+                 const uint8_t *covered, double *phi_s, double *phi_b, gf_stream_t stream);
+int gf_ddt_corr_add(const gf_mesh *m, const double *U_old, const double *U_older,
+                    const double *phi_old, const double *phi_older, const double *rAU, double b,
+                    double c, double dt, double *phi_s, gf_stream_t stream);
+int gf_constrain_bnd_flux(const gf_mesh *m, const double *psi_b, const uint8_t *mask,
+                          double *phi_b, gf_stream_t stream);
+/* kind per boundary face: 0 none, 1 fixed (dirichlet), 2 adjustable */
+int gf_adjust_phi(const gf_mesh *m, const int8_t *kind, double *phi_b, double *scratch,
+                  gf_stream_t stream);
+/* phi_s = phi_hbya_s + sign_p * flux; boundary: phi_b -= rAtU_P |Sf| snGrad_b */
+int gf_correct_phi(const gf_mesh *m, const double *phi_hbya_s, const double *phi_hbya_b,
+                   const double *upper, const double *lower, const double *ffc, const double *p,
+                   const double *rAtU, const double *frac, const double *ref_v,
+                   const double *ref_g, const uint8_t *covered, double *phi_s, double *phi_b,
+                   gf_stream_t stream);
+/* U = HbyA - rAtU grad p */
+int gf_correct_velocity(int32_t n_cells, const double *HbyA, const double *rAtU,
+                        const double *grad_p, double *U, gf_stream_t stream);
+/* diag[ref] += diag[ref]; source[ref] += diag_old*value (reference_value.py:33-51) */
+int gf_set_reference(double *diag, double *source, int32_t cell, double value,
+                     gf_stream_t stream);
+/* SIMPLE momentum under-relaxation: diag /= alpha; source += ((1-alpha)/alpha) diag_old U
+ * (algorithms/simple.py:302-311) */
+int gf_relax_momentum(int32_t n_cells, double alpha, double *diag, double *source, const double *U,
+                      gf_stream_t stream);
+/* out = source - x V: momentum predictor source - V grad(p) (piso.py:247-250,
+ * simple.py:315-318) and pressure source - div(phiHbyA) V (pressure_correction.py:121-124) */
+int gf_predictor_source(int32_t n_cells, int32_t nc, const double *source, const double *volumes,
+                        const double *x, double *out, gf_stream_t stream);
+/* x = old + alpha (x - old)  (simple.py:386) */
+int gf_relax_field(int64_t n, double alpha, const double *old, double *x, gf_stream_t stream);
+/* out = a*x + b*y elementwise (FvMatrix +/-) */
+int gf_axpby(int64_t n, double a, const double *x, double b, const double *y, double *out,
+             gf_stream_t stream);
+/* ||A psi - b||_2 / (||b||_2 + eps) written to out[0] (residual.py:12-36) */
+int gf_initial_residual(const gf_mesh *m, int32_t nc, const double *diag, const double *upper,
+                        const double *lower, const double *source, const double *psi,
+                        double *scratch, double *out, gf_stream_t stream);
+
+/* ---- Krylov solves (solvers/cg.py:75-174, bicgstab.py:76-272,
+ * preconditioners.py:76-96, krylov.py:34-82, base.py:140-162) -------------
+ * Solves nrhs independent components that share one LDU matrix in a single
+ * batched launch sequence; results equal per-component solves.
+ * stats_out[nrhs][4] = {initial_residual, final_residual, iterations, converged}. */
+typedef struct gf_krylov_cfg {
+    int32_t method;        /* 0 cg, 1 bicgstab */
+    int32_t precond;       /* 0 none, 1 jacobi */
+    int32_t norm;          /* 0 L2, 1 Linf */
+    int32_t max_iter, max_restart;
+    double atol, rtol;
+    int32_t mixed;         /* cg only: FP32 inner solve inside FP64 refinement */
+    double inner_rtol;     /* mixed: inner relative reduction per sweep */
+    int32_t check_every;   /* host polls the device state every k iterations */
+} gf_krylov_cfg;
+/* workspace bytes needed for (C, nrhs, cfg); caller allocates once */
+int64_t gf_krylov_workspace_bytes(const gf_mesh *m, int32_t nrhs, const gf_krylov_cfg *cfg);
+int gf_krylov_solve(const gf_mesh *m, const gf_krylov_cfg *cfg, int32_t nrhs, int32_t transpose,
+                    const double *diag, const double *upper, const double *lower,
+                    const double *b /* [C,nrhs] */, double *x /* [C,nrhs], in/out */,
+                    void *workspace, double *stats_out, gf_stream_t stream);
+/* ldu_grads.py:11-53: gd = -sum lambda x, gu = -sum lambda_O x_N, gl = -sum lambda_N x_O */
+int gf_ldu_grads(const gf_mesh *m, int32_t nc, const double *lambda, const double *x,
+                 double *g_diag, double *g_upper, double *g_lower, gf_stream_t stream);
+/* y = A x on the sliced-ELL operator (benchmark entry; FP64 or FP32 values) */
+int gf_sell_pack(const gf_mesh *m, int32_t transpose, const double *upper, const double *lower,
+                 void *val, int32_t fp32, gf_stream_t stream);
+int gf_sell_spmv(const gf_mesh *m, const void *diag, const void *val, const void *x, void *y,
+                 int32_t fp32, gf_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GRIDFOAM_HIP_H */

@@ -1,0 +1,985 @@
+"""ORACLE -- test infrastructure only.
+
+FP64 CPU (numpy) restatement of gridfoam's pressure-velocity path, used by
+``tests/`` as the checker, by ``__graft_entry__.smoke()`` and by the
+``cpu_baseline`` leg of ``bench.py``.  Nothing in ``gridfoam_amd`` imports
+it; the product runs only on the HIP library.
+
+Scatters use ``np.bincount`` (sequential, deterministic summation order),
+never atomics.  Each function cites the reference file:line it restates.
+Every function is pinned in ``tests/test_oracle_*.py`` by the reference's own
+known-answer tests, closed forms and invariants.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+import pathlib
+from dataclasses import dataclass, field
+from typing import Any
+
+import numpy as np
+
+_HERE = pathlib.Path(__file__).resolve().parent
+_LIB: ctypes.CDLL | None = None
+
+PATCHES = ("x_minus", "x_plus", "y_minus", "y_plus", "z_minus", "z_plus")
+
+
+# ---------------------------------------------------------------------------
+# Octree build (C restatement, oracle/octree_oracle.c)
+# ---------------------------------------------------------------------------
+def _lib() -> ctypes.CDLL:
+    global _LIB
+    if _LIB is None:
+        path = _HERE / "liboctree_oracle.so"
+        if not path.exists():
+            raise RuntimeError(f"oracle library missing: {path} (run make -C oracle)")
+        lib = ctypes.CDLL(str(path))
+        lib.oct_oracle_build.restype = ctypes.c_void_p
+        lib.oct_oracle_fetch.restype = None
+        lib.oct_oracle_free.restype = None
+        lib.oct_oracle_free.argtypes = [ctypes.c_void_p]
+        _LIB = lib
+    return _LIB
+
+
+def _ptr(a: np.ndarray) -> ctypes.c_void_p:
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+@dataclass
+class OctreeTopology:
+    lmax: int
+    level: np.ndarray  # [C] int32
+    ijk: np.ndarray  # [C,3] int32 (coordinates at the cell's own level)
+    owner: np.ndarray  # [F] int32
+    neighbour: np.ndarray  # [F] int32
+    axis: np.ndarray  # [F] int32
+    bnd_owner: np.ndarray  # [F_b] int32
+    bnd_dir: np.ndarray  # [F_b] int32
+
+
+def build_octree(lower, upper, root, regions=(), n_leaf: int = 0) -> OctreeTopology:
+    """Octree leaves and face lists (contract: core/grid/factory.py:32-65)."""
+    lib = _lib()
+    lo = np.ascontiguousarray(lower, dtype=np.float64)
+    hi = np.ascontiguousarray(upper, dtype=np.float64)
+    rt = np.ascontiguousarray(root, dtype=np.int32)
+    nr = len(regions)
+    rmin = np.ascontiguousarray([r[0] for r in regions] or [[0, 0, 0]], dtype=np.float64)
+    rmax = np.ascontiguousarray([r[1] for r in regions] or [[0, 0, 0]], dtype=np.float64)
+    rlev = np.ascontiguousarray([r[2] for r in regions] or [0], dtype=np.int32)
+    counts = np.zeros(4, dtype=np.int64)
+    h = lib.oct_oracle_build(
+        _ptr(lo), _ptr(hi), _ptr(rt), ctypes.c_int32(nr), _ptr(rmin), _ptr(rmax),
+        _ptr(rlev), ctypes.c_int32(n_leaf), _ptr(counts),
+    )
+    try:
+        C, F, B, lmax = (int(v) for v in counts)
+        level = np.zeros(C, np.int32)
+        ijk = np.zeros((C, 3), np.int32)
+        fo = np.zeros(F, np.int32)
+        fn = np.zeros(F, np.int32)
+        fa = np.zeros(F, np.int32)
+        bo = np.zeros(B, np.int32)
+        bd = np.zeros(B, np.int32)
+        lib.oct_oracle_fetch(ctypes.c_void_p(h), _ptr(level), _ptr(ijk), _ptr(fo), _ptr(fn),
+                             _ptr(fa), _ptr(bo), _ptr(bd))
+    finally:
+        lib.oct_oracle_free(ctypes.c_void_p(h))
+    return OctreeTopology(lmax, level, ijk, fo, fn, fa, bo, bd)
+
+
+# ---------------------------------------------------------------------------
+# Grid geometry  (core/grid/axis_projected.py:130-161, :715-811)
+# ---------------------------------------------------------------------------
+def _scatter(idx: np.ndarray, vals: np.ndarray, n: int) -> np.ndarray:
+    """Deterministic index_add over axis 0 (any trailing shape)."""
+    if vals.ndim == 1:
+        return np.bincount(idx, weights=vals, minlength=n)
+    flat = vals.reshape(vals.shape[0], -1)
+    out = np.stack([np.bincount(idx, weights=flat[:, k], minlength=n)
+                    for k in range(flat.shape[1])], axis=1)
+    return out.reshape((n,) + vals.shape[1:])
+
+
+class Grid:
+    """Restates AxisProjectedGrid's topology + geometry for grids without IB."""
+
+    def __init__(self, lower, upper, root, regions=(), n_leaf=0, dt=1.0):
+        topo = build_octree(lower, upper, root, regions, n_leaf)
+        self.topo = topo
+        self.lower = np.asarray(lower, np.float64)
+        self.upper = np.asarray(upper, np.float64)
+        self.root = np.asarray(root, np.int64)
+        self.dt = float(dt)
+        lev = topo.level.astype(np.int64)
+        ext = self.upper - self.lower
+        self.cell_sizes = ext[None, :] / (self.root[None, :] * (2.0 ** lev)[:, None])
+        self.cell_centers = self.lower[None, :] + (topo.ijk + 0.5) * self.cell_sizes
+        s = self.cell_sizes
+        self.cell_volumes = s[:, 0] * s[:, 1] * s[:, 2]
+        self.owner = topo.owner.astype(np.int64)
+        self.neighbour = topo.neighbour.astype(np.int64)
+        self.axis = topo.axis.astype(np.int64)
+        self.bnd_owner = topo.bnd_owner.astype(np.int64)
+        self.bnd_dir = topo.bnd_dir.astype(np.int64)
+        self.num_cells = len(lev)
+        self.num_faces = len(self.owner)
+        self.num_bnd = len(self.bnd_owner)
+        # axis_projected.py:715-737 -- overlap-box centre
+        cO, cN = self.cell_centers[self.owner], self.cell_centers[self.neighbour]
+        sO, sN = s[self.owner], s[self.neighbour]
+        fmin = np.maximum(cO - sO / 2.0, cN - sN / 2.0)
+        fmax = np.minimum(cO + sO / 2.0, cN + sN / 2.0)
+        self.face_centers = (fmin + fmax) / 2.0
+        # axis_projected.py:740-767 -- finer-cell cross section on the axis
+        mins = np.minimum(sO, sN)
+        fi = np.arange(self.num_faces)
+        area = (mins[:, 0] * mins[:, 1] * mins[:, 2]) / mins[fi, self.axis]
+        self.Sf = np.zeros((self.num_faces, 3))
+        self.Sf[fi, self.axis] = area
+        # axis_projected.py:770-811
+        bi = np.arange(self.num_bnd)
+        baxis = self.bnd_dir // 2
+        sign = 2.0 * (self.bnd_dir % 2) - 1.0
+        normals = np.zeros((self.num_bnd, 3))
+        normals[bi, baxis] = sign
+        cb, sb = self.cell_centers[self.bnd_owner], s[self.bnd_owner]
+        self.bnd_face_centers = cb + 0.5 * sb * normals
+        barea = sign * (sb[:, 0] * sb[:, 1] * sb[:, 2]) / sb[bi, baxis]
+        self.bnd_Sf = np.zeros((self.num_bnd, 3))
+        self.bnd_Sf[bi, baxis] = barea
+        self.geo = FaceGeometry(self)
+
+    def patch_mask(self, patch: str) -> np.ndarray:
+        return self.bnd_dir == PATCHES.index(patch)
+
+
+class FaceGeometry:
+    """fv/kernels/face_geometry.py:107-188 (all faces single-sided: no IB)."""
+
+    def __init__(self, g: Grid):
+        C = g.cell_centers
+        o, n, ax = g.owner, g.neighbour, g.axis
+        fi = np.arange(g.num_faces)
+        d_all = C[n] - C[o]
+        d_fN = C[n] - g.face_centers
+        d_n = np.abs(d_all[fi, ax])
+        self.w = np.abs(d_fN[fi, ax]) / d_n
+        self.mag_Sf = np.sqrt(np.sum(g.Sf * g.Sf, axis=1))
+        self.delta = 1.0 / d_n
+        centroid = self.w[:, None] * C[o] + (1.0 - self.w)[:, None] * C[n]
+        offset = g.face_centers - centroid
+        tol = 1.0e-8 * float(g.cell_sizes.min())
+        hang = np.sqrt(np.sum(offset * offset, axis=1)) > tol
+        self.hang_idx = np.nonzero(hang)[0]
+        self.hang_offset = offset[self.hang_idx]
+        h = self.hang_idx
+        mO = g.face_centers[h] - C[o[h]]
+        mN = g.face_centers[h] - C[n[h]]
+        mO[np.arange(len(h)), ax[h]] = 0.0
+        mN[np.arange(len(h)), ax[h]] = 0.0
+        self.hang_move_O, self.hang_move_N = mO, mN
+        self.hang_cell_mask = np.zeros(g.num_cells, bool)
+        self.hang_cell_mask[o[h]] = True
+        self.hang_cell_mask[n[h]] = True
+        # face_geometry.py:191-262 -- geometric LSQ normal matrix and pinv
+        ata = np.zeros((g.num_cells, 3, 3))
+        outer = _lsq_outer(d_all)
+        ata += _scatter(o, outer, g.num_cells)
+        ata += _scatter(n, outer, g.num_cells)
+        d_b = g.bnd_face_centers - C[g.bnd_owner]
+        ata += _scatter(g.bnd_owner, _lsq_outer(d_b), g.num_cells)
+        self.lsq_ata_inv = np.linalg.pinv(ata, hermitian=True)
+
+
+def _lsq_outer(d: np.ndarray) -> np.ndarray:
+    d2 = np.sum(d * d, axis=1)
+    w2 = 1.0 / np.where(d2 > 0, d2, 1.0)
+    return w2[:, None, None] * d[:, :, None] * d[:, None, :]
+
+
+# ---------------------------------------------------------------------------
+# Fields and boundary conditions  (boundaries/*.py, fv/boundary_ops.py)
+# ---------------------------------------------------------------------------
+@dataclass
+class BC:
+    kind: str  # dirichlet | neumann | empty | slip
+    value: np.ndarray | float = 0.0
+
+    @property
+    def assignable(self) -> bool:  # boundaries/base.py:38-48; dirichlet/slip False
+        return self.kind not in ("dirichlet", "slip")
+
+
+@dataclass
+class Field:
+    name: str
+    data: np.ndarray
+    bcs: dict[str, BC] = field(default_factory=dict)
+    old: np.ndarray | None = None
+    older: np.ndarray | None = None
+
+    @property
+    def cshape(self) -> tuple[int, ...]:
+        return tuple(self.data.shape[1:])
+
+
+@dataclass
+class Batch:
+    patch: str
+    mask: np.ndarray
+    cells: np.ndarray
+    d_vec: np.ndarray
+    mag_d: np.ndarray
+    Sf: np.ndarray
+    mag_Sf: np.ndarray
+
+
+def batches(g: Grid, fld: Field) -> list[Batch]:
+    """boundary_ops.py:121-148 (domain patches, empty skipped)."""
+    out = []
+    for patch, bc in fld.bcs.items():
+        if bc.kind == "empty":
+            continue
+        m = g.patch_mask(patch)
+        if not m.any():
+            continue
+        cells = g.bnd_owner[m]
+        d = g.bnd_face_centers[m] - g.cell_centers[cells]
+        Sf = g.bnd_Sf[m]
+        out.append(Batch(patch, m, cells, d, np.sqrt(np.sum(d * d, axis=1)), Sf,
+                         np.sqrt(np.sum(Sf * Sf, axis=1))))
+    return out
+
+
+def bc_evaluate(g: Grid, fld: Field, b: Batch):
+    """(fraction, ref_v, ref_g): dirichlet.py:46-73, neumann.py:42-73, slip.py:46-101."""
+    bc = fld.bcs[b.patch]
+    n = len(b.cells)
+    cs = fld.cshape
+    if bc.kind == "dirichlet":
+        v = np.broadcast_to(np.asarray(bc.value, np.float64), cs)
+        return np.ones(n), np.ones((n,) + cs) * v, np.zeros((n,) + cs)
+    if bc.kind == "neumann":
+        v = np.broadcast_to(np.asarray(bc.value, np.float64), cs)
+        return np.zeros(n), np.zeros((n,) + cs), np.ones((n,) + cs) * v
+    if bc.kind == "slip":
+        if cs == ():
+            return np.zeros(n), np.zeros(n), np.zeros(n)
+        nvec = np.sign(b.Sf)
+        psi = fld.data[b.cells]
+        un = np.sum(psi * nvec, axis=-1)
+        return np.ones(n), psi - un[:, None] * nvec, np.zeros((n,) + cs)
+    raise ValueError(f"unsupported bc {bc.kind}")
+
+
+def bc_state(g: Grid, fld: Field, b: Batch):
+    """boundary_ops.py:311-326 -> (f, ref_v, ref_g, psi_b)."""
+    f, rv, rg = bc_evaluate(g, fld, b)
+    psi = fld.data[b.cells]
+    fb = f.reshape((-1,) + (1,) * len(fld.cshape))
+    db = b.mag_d.reshape(fb.shape)
+    psi_b = fb * rv + (1.0 - fb) * (psi + db * rg)
+    return f, rv, rg, psi_b
+
+
+def value_grad_coeff(b: Batch, f: np.ndarray) -> np.ndarray:
+    """boundary_ops.py:405-422 (no IB, so no snapped faces)."""
+    active = f != 0
+    if np.any(active & (b.mag_d == 0)):
+        raise ValueError("A mixed boundary at zero distance needs a cell constraint")
+    dist = np.where(active, b.mag_d, 1.0)
+    return np.where(active, f / dist, 0.0)
+
+
+def _bcast(v: np.ndarray, like: np.ndarray) -> np.ndarray:
+    return v.reshape(v.shape + (1,) * (like.ndim - v.ndim))
+
+
+# ---------------------------------------------------------------------------
+# LDU matrix  (core/fvmatrix.py)
+# ---------------------------------------------------------------------------
+class Matrix:
+    def __init__(self, g: Grid, cshape=()):
+        self.g = g
+        self.cshape = tuple(cshape)
+        self.diag = np.zeros(g.num_cells)
+        self.upper = np.zeros(g.num_faces)
+        self.lower = np.zeros(g.num_faces)
+        self.source = np.zeros((g.num_cells,) + self.cshape)
+        self.ffc: np.ndarray | None = None
+
+    def copy(self) -> "Matrix":
+        m = Matrix(self.g, self.cshape)
+        m.diag, m.upper, m.lower = self.diag.copy(), self.upper.copy(), self.lower.copy()
+        m.source = self.source.copy()
+        m.ffc = None if self.ffc is None else self.ffc.copy()
+        return m
+
+    def _combine(self, other: "Matrix", sign: float) -> "Matrix":  # fvmatrix.py:166-201
+        m = Matrix(self.g, self.cshape)
+        m.diag = self.diag + sign * other.diag
+        m.upper = self.upper + sign * other.upper
+        m.lower = self.lower + sign * other.lower
+        m.source = self.source + sign * other.source
+        a, b = self.ffc, other.ffc
+        if a is None:
+            m.ffc = None if b is None else sign * b
+        else:
+            m.ffc = a.copy() if b is None else a + sign * b
+        return m
+
+    def __add__(self, o):
+        return self._combine(o, 1.0)
+
+    def __sub__(self, o):
+        return self._combine(o, -1.0)
+
+    def __neg__(self):  # fvmatrix.py:203-211
+        m = Matrix(self.g, self.cshape)
+        m.diag, m.upper, m.lower, m.source = -self.diag, -self.upper, -self.lower, -self.source
+        m.ffc = None if self.ffc is None else -self.ffc
+        return m
+
+    def multiply(self, x: np.ndarray) -> np.ndarray:  # fvmatrix.py:292-323
+        g = self.g
+        res = _bcast(self.diag, x) * x
+        res = res + _scatter(g.owner, _bcast(self.upper, x[g.neighbour]) * x[g.neighbour], g.num_cells)
+        res = res + _scatter(g.neighbour, _bcast(self.lower, x[g.owner]) * x[g.owner], g.num_cells)
+        return res
+
+    def A(self) -> np.ndarray:  # fvmatrix.py:242-248
+        return self.diag / self.g.cell_volumes
+
+    def H(self, x: np.ndarray) -> np.ndarray:  # fvmatrix.py:266-290
+        V = _bcast(self.g.cell_volumes, x)
+        return (self.source - self.multiply(x) + _bcast(self.diag, x) * x) / V
+
+    def H1(self) -> np.ndarray:  # fvmatrix.py:250-264
+        g = self.g
+        h1 = _scatter(g.owner, -self.upper, g.num_cells) + _scatter(g.neighbour, -self.lower, g.num_cells)
+        return h1 / g.cell_volumes
+
+    def flux(self, psi: np.ndarray) -> np.ndarray:  # fvmatrix.py:424-468
+        g = self.g
+        pN, pO = psi[g.neighbour], psi[g.owner]
+        out = _bcast(self.upper, pN) * pN - _bcast(self.lower, pO) * pO
+        if self.ffc is not None:
+            out = out + self.ffc
+        return out
+
+    def transpose(self) -> "Matrix":  # fvmatrix.py:396-412
+        m = Matrix(self.g, self.cshape)
+        m.diag, m.upper, m.lower, m.source, m.ffc = self.diag, self.lower, self.upper, self.source, self.ffc
+        return m
+
+
+# ---------------------------------------------------------------------------
+# Explicit kernels  (fv/kernels/*.py, fv/fvc/*.py)
+# ---------------------------------------------------------------------------
+def lsq_grad(g: Grid, fld: Field) -> np.ndarray:
+    """fv/kernels/least_squares.py:29-101 (full-stencil form)."""
+    C = g.cell_centers
+    psi = fld.data
+    o, n = g.owner, g.neighbour
+    d = C[n] - C[o]
+    d2 = np.sum(d * d, axis=1)
+    wd = (1.0 / np.where(d2 > 0, d2, 1.0))[:, None] * d
+    dpsi = psi[n] - psi[o]
+    atb_face = dpsi[..., None] * wd.reshape((len(d),) + (1,) * len(fld.cshape) + (3,))
+    atb = _scatter(o, atb_face, g.num_cells) + _scatter(n, atb_face, g.num_cells)
+    for b in batches(g, fld):
+        _, _, _, psi_b = bc_state(g, fld, b)
+        db = b.d_vec
+        dd = np.sum(db * db, axis=1)
+        wdb = (1.0 / np.where(dd > 0, dd, 1.0))[:, None] * db
+        dp = psi_b - psi[b.cells]
+        atb = atb + _scatter(b.cells, dp[..., None] * wdb.reshape((len(db),) + (1,) * len(fld.cshape) + (3,)),
+                             g.num_cells)
+    return np.einsum("nij,n...j->n...i", g.geo.lsq_ata_inv, atb)
+
+
+def linear_face_values(g: Grid, psi: np.ndarray) -> np.ndarray:
+    """face_interpolation.py:114-138."""
+    w = _bcast(g.geo.w, psi[g.owner])
+    return w * psi[g.owner] + (1.0 - w) * psi[g.neighbour]
+
+
+def correct_face_values(g: Grid, base: np.ndarray, grad: np.ndarray) -> np.ndarray:
+    """face_interpolation.py:141-197 -- hanging face-centre offset."""
+    geo = g.geo
+    h = geo.hang_idx
+    if len(h) == 0:
+        return base
+    gO, gN = grad[g.owner[h]], grad[g.neighbour[h]]
+    w = _bcast(geo.w[h], gO)
+    gf = w * gO + (1.0 - w) * gN
+    corr = np.einsum("n...j,nj->n...", gf, geo.hang_offset)
+    out = base.copy()
+    out[h] += corr
+    return out
+
+
+def sn_grad_hang_correction(g: Grid, grad: np.ndarray) -> np.ndarray:
+    """face_interpolation.py:200-246."""
+    geo = g.geo
+    h = geo.hang_idx
+    cN = np.einsum("n...j,nj->n...", grad[g.neighbour[h]], geo.hang_move_N)
+    cO = np.einsum("n...j,nj->n...", grad[g.owner[h]], geo.hang_move_O)
+    diff = cN - cO
+    return _bcast(geo.delta[h], diff) * diff
+
+
+def sn_grad(g: Grid, fld: Field, corrected: bool = True) -> np.ndarray:
+    """fv/schemes/sn_grad.py:109-181."""
+    psi = fld.data
+    diff = psi[g.neighbour] - psi[g.owner]
+    base = _bcast(g.geo.delta, diff) * diff
+    if not corrected or len(g.geo.hang_idx) == 0:
+        return base
+    out = base.copy()
+    out[g.geo.hang_idx] += sn_grad_hang_correction(g, lsq_grad(g, fld))
+    return out
+
+
+@dataclass
+class FaceValues:
+    single: np.ndarray
+    bnd: np.ndarray
+
+
+def interpolate(g: Grid, fld: Field) -> FaceValues:
+    """fv/schemes/interpolate.py:20-63 + boundary_ops.py:545-593."""
+    base = linear_face_values(g, fld.data)
+    bnd = fld.data[g.bnd_owner].copy()  # zero-gradient extrapolation by default
+    for b in batches(g, fld):
+        bnd[b.mask] = bc_state(g, fld, b)[3]
+    if len(g.geo.hang_idx):
+        single = correct_face_values(g, base, lsq_grad(g, fld))
+    else:
+        single = base
+    return FaceValues(single, bnd)
+
+
+def fvc_div(g: Grid, phi: FaceValues) -> np.ndarray:
+    """fv/fvc/div.py:15-64."""
+    d = _scatter(g.owner, phi.single, g.num_cells)
+    d = d + _scatter(g.neighbour, -phi.single, g.num_cells)
+    d = d + _scatter(g.bnd_owner, phi.bnd, g.num_cells)
+    return d / _bcast(g.cell_volumes, d)
+
+
+def covered_domain(g: Grid, fld: Field) -> np.ndarray:
+    cov = np.zeros(g.num_bnd, bool)
+    for b in batches(g, fld):
+        cov |= b.mask
+    return cov
+
+
+def flux_from_face_velocity(g: Grid, Uf: FaceValues, U: Field) -> FaceValues:
+    """fv/flux.py:32-71."""
+    single = np.sum(Uf.single * g.Sf, axis=1)
+    dom = np.sum(Uf.bnd * g.bnd_Sf, axis=1)
+    dom = np.where(~covered_domain(g, U), 0.0, dom)
+    return FaceValues(single, dom)
+
+
+def correct_flux(g: Grid, U: Field, phi: FaceValues | None = None) -> FaceValues:
+    """fv/flux.py:132-162 (update_internal=True when phi is None)."""
+    if phi is None:
+        phi = flux_from_face_velocity(g, interpolate(g, U), U)
+    for b in batches(g, U):
+        phi.bnd[b.mask] = np.sum(bc_state(g, U, b)[3] * b.Sf, axis=1)
+    return phi
+
+
+def ddt_coefficients(scheme: str, have_older: bool, dt: float, dt0: float | None):
+    """fv/schemes/ddt.py:13-58."""
+    if scheme == "euler" or not have_older:
+        return 1.0, 1.0, 0.0
+    a = 1.0 + dt / (dt + dt0)
+    c = (dt / dt0) * dt / (dt + dt0)
+    return a, a + c, c
+
+
+def ddt_corr(g: Grid, U: Field, phi_old: np.ndarray, abc, phi_older=None) -> np.ndarray:
+    """fv/fvc/ddt_corr.py:13-78."""
+    w = g.geo.w[:, None]
+    U0f = w * U.old[g.owner] + (1.0 - w) * U.old[g.neighbour]
+    corr = phi_old - np.sum(U0f * g.Sf, axis=1)
+    _, b, c = abc
+    hist = b * corr
+    if c != 0.0:
+        U00f = w * U.older[g.owner] + (1.0 - w) * U.older[g.neighbour]
+        hist = hist - c * (phi_older - np.sum(U00f * g.Sf, axis=1))
+    eps = np.finfo(np.float64).tiny
+    coupling = 1.0 - np.minimum(np.abs(corr) / (np.abs(phi_old) + eps), 1.0)
+    return coupling * hist / g.dt
+
+
+# ---------------------------------------------------------------------------
+# Implicit operators  (fv/fvm/*.py, fv/schemes/div.py)
+# ---------------------------------------------------------------------------
+def fvm_laplacian(g: Grid, gamma, fld: Field, corrected: bool = True) -> Matrix:
+    """fv/fvm/laplacian.py:101-201 (linear gamma interpolation)."""
+    geo = g.geo
+    m = Matrix(g, fld.cshape)
+    if np.ndim(gamma) == 0:
+        gamma_f = float(gamma)
+    else:
+        gamma_f = geo.w * gamma[g.owner] + (1.0 - geo.w) * gamma[g.neighbour]
+    coeff = gamma_f * geo.mag_Sf * geo.delta
+    coeff = np.broadcast_to(coeff, (g.num_faces,)).copy()
+    m.upper = coeff.copy()
+    m.lower = coeff.copy()
+    m.diag = m.diag + _scatter(g.owner, -coeff, g.num_cells) + _scatter(g.neighbour, -coeff, g.num_cells)
+    h = geo.hang_idx
+    if corrected and len(h):
+        corr = sn_grad_hang_correction(g, lsq_grad(g, fld))
+        gh = gamma_f if np.ndim(gamma_f) == 0 else gamma_f[h]
+        src = _bcast(gh * geo.mag_Sf[h], corr) * corr
+        m.source = m.source + _scatter(g.owner[h], -src, g.num_cells) + _scatter(g.neighbour[h], src, g.num_cells)
+        ffc = np.zeros((g.num_faces,) + fld.cshape)
+        ffc[h] = src
+        m.ffc = ffc
+    for b in batches(g, fld):
+        f, rv, rg, _ = bc_state(g, fld, b)
+        gb = gamma if np.ndim(gamma) == 0 else gamma[b.cells]
+        cond = gb * b.mag_Sf
+        c_value = cond * value_grad_coeff(b, f)
+        c_grad = cond * (1.0 - f)
+        src = _bcast(c_value, rv) * rv + _bcast(c_grad, rg) * rg
+        m.diag = m.diag + _scatter(b.cells, -c_value, g.num_cells)
+        m.source = m.source + _scatter(b.cells, -src, g.num_cells)
+    return m
+
+
+def fvm_div(g: Grid, phi: FaceValues, fld: Field, scheme: str) -> Matrix:
+    """fv/fvm/div.py:15-73 with schemes/div.py:36-139 (upwind, linear)."""
+    geo = g.geo
+    m = Matrix(g, fld.cshape)
+    ph = phi.single
+    if scheme == "upwind":
+        pos, neg = np.maximum(ph, 0.0), np.minimum(ph, 0.0)
+        upper, lower, dO, dN = neg, -pos, pos, -neg
+        sface = np.zeros((g.num_faces,) + fld.cshape)
+    elif scheme == "linear":
+        w = geo.w
+        dO, upper = ph * w, ph * (1.0 - w)
+        lower, dN = -ph * w, -ph * (1.0 - w)
+        sface = np.zeros((g.num_faces,) + fld.cshape)
+        if len(geo.hang_idx):
+            base = linear_face_values(g, fld.data)
+            corrected = correct_face_values(g, base, lsq_grad(g, fld))
+            sface = -_bcast(ph, base) * (corrected - base)
+    else:
+        raise ValueError(f"unsupported div scheme {scheme}")
+    m.upper = upper.copy()
+    m.lower = lower.copy()
+    m.diag = m.diag + _scatter(g.owner, dO, g.num_cells) + _scatter(g.neighbour, dN, g.num_cells)
+    m.source = m.source + _scatter(g.owner, sface, g.num_cells) + _scatter(g.neighbour, -sface, g.num_cells)
+    for b in batches(g, fld):
+        f, rv, rg, _ = bc_state(g, fld, b)
+        fv = _bcast(f, rv)
+        bsrc = fv * rv + (1.0 - fv) * _bcast(b.mag_d, rg) * rg
+        flux = phi.bnd[b.mask]
+        m.diag = m.diag + _scatter(b.cells, flux * (1.0 - f), g.num_cells)
+        m.source = m.source + _scatter(b.cells, -_bcast(flux, bsrc) * bsrc, g.num_cells)
+    return m
+
+
+def fvm_ddt(g: Grid, fld: Field, abc) -> Matrix:
+    """fv/fvm/ddt.py:7-40."""
+    a, b, c = abc
+    m = Matrix(g, fld.cshape)
+    vdt = g.cell_volumes / g.dt
+    m.diag = m.diag + a * vdt
+    hist = b * fld.old
+    if c != 0.0:
+        hist = hist - c * fld.older
+    m.source = m.source + _bcast(vdt, hist) * hist
+    return m
+
+
+# ---------------------------------------------------------------------------
+# Krylov solvers  (solvers/*.py)
+# ---------------------------------------------------------------------------
+@dataclass
+class Stats:
+    solver: str
+    initial_residual: float
+    final_residual: float
+    iterations: int
+    converged: bool
+
+
+@dataclass
+class SolverCfg:
+    method: str = "cg"
+    preconditioner: str = "none"
+    tolerance: float = 1e-6
+    rel_tolerance: float = 1e-6
+    max_iter: int = 1000
+    norm_type: str = "l_2"
+    max_restart: int = 5
+
+
+def _norm(v: np.ndarray, kind: str) -> float:
+    return float(np.max(np.abs(v))) if kind == "l_inf" else float(np.sqrt(np.dot(v, v)))
+
+
+def _inv_diag(m: Matrix, kind: str) -> np.ndarray:
+    """solvers/preconditioners.py:76-96."""
+    if kind == "none":
+        return np.ones_like(m.diag)
+    eps = np.finfo(np.float64).eps
+    safe = np.where(m.diag == 0, 1.0, m.diag)
+    return np.where(np.abs(m.diag) < eps, 1.0, 1.0 / safe)
+
+
+def cg_single(m: Matrix, b: np.ndarray, x: np.ndarray, cfg: SolverCfg, inv_d: np.ndarray):
+    """solvers/cg.py:75-174."""
+    r = b - m.multiply(x)
+    nr0 = _norm(r, cfg.norm_type)
+    thresh = max(cfg.tolerance, cfg.rel_tolerance * nr0)
+    if nr0 < thresh:
+        return x, Stats("cg", nr0, nr0, 0, True)
+    z = inv_d * r
+    p = z.copy()
+    rz_old = np.dot(r, z)
+    final, conv, it = nr0, False, 0
+    for it in range(1, cfg.max_iter + 1):
+        Ap = m.multiply(p)
+        alpha = rz_old / np.dot(p, Ap)
+        x = x + alpha * p
+        r = r - alpha * Ap
+        final = _norm(r, cfg.norm_type)
+        if final < thresh:
+            conv = True
+            break
+        z = inv_d * r
+        rz_new = np.dot(r, z)
+        p = z + p * (rz_new / rz_old)
+        rz_old = rz_new
+    return x, Stats("cg", nr0, final, it, conv)
+
+
+def bicgstab_single(m: Matrix, b: np.ndarray, x: np.ndarray, cfg: SolverCfg, inv_d: np.ndarray):
+    """solvers/bicgstab.py:76-272."""
+    r = b - m.multiply(x)
+    nr0 = _norm(r, cfg.norm_type)
+    thresh = max(cfg.tolerance, cfg.rel_tolerance * nr0)
+    if nr0 < thresh:
+        return x, Stats("bicgstab", nr0, nr0, 0, True)
+
+    def l2(v):
+        return float(np.sqrt(np.dot(v, v)))
+
+    def restart(r):
+        return r.copy(), r.copy(), np.dot(r, r)
+
+    r_hat, p, rho_old = restart(r)
+    restarts, final, conv, it = 0, nr0, False, 0
+    for it in range(1, cfg.max_iter + 1):
+        y = inv_d * p
+        v = m.multiply(y)
+        den = np.dot(r_hat, v)
+        if abs(den) <= 1e-6 * l2(r_hat) * l2(v):
+            if restarts >= cfg.max_restart:
+                break
+            restarts += 1
+            r_hat, p, rho_old = restart(r)
+            continue
+        alpha = rho_old / den
+        s = r - alpha * v
+        final = _norm(s, cfg.norm_type)
+        if final < thresh:
+            x = x + alpha * y
+            conv = True
+            break
+        z = inv_d * s
+        t = m.multiply(z)
+        dom = np.dot(t, t)
+        if abs(dom) <= 1e-6 * l2(t) * l2(t):
+            if restarts >= cfg.max_restart:
+                break
+            restarts += 1
+            r_hat, p, rho_old = restart(r)
+            continue
+        omega = np.dot(t, s) / dom
+        if abs(omega) <= 1e-6:
+            if restarts >= cfg.max_restart:
+                break
+            restarts += 1
+            r_hat, p, rho_old = restart(r)
+            continue
+        x = x + alpha * y + omega * z
+        r = s - omega * t
+        final = _norm(r, cfg.norm_type)
+        if final < thresh:
+            conv = True
+            break
+        rho_new = np.dot(r_hat, r)
+        if abs(rho_new) <= 1e-6 * l2(r_hat) * l2(r):
+            if restarts >= cfg.max_restart:
+                break
+            restarts += 1
+            r_hat, p, rho_old = restart(r)
+            continue
+        beta = (rho_new / rho_old) * (alpha / omega)
+        p = r + beta * (p - omega * v)
+        rho_old = rho_new
+    return x, Stats("bicgstab", nr0, final, it, conv)
+
+
+def solve(m: Matrix, x0: np.ndarray, cfg: SolverCfg):
+    """solvers/krylov.py:34-82 -- one shared preconditioner, per component."""
+    inv_d = _inv_diag(m, cfg.preconditioner)
+    single = cg_single if cfg.method == "cg" else bicgstab_single
+    if x0.ndim == 1:
+        x, st = single(m, m.source, x0.copy(), cfg, inv_d)
+        return x, (st,)
+    xs, sts = [], []
+    for k in range(x0.shape[1]):
+        x, st = single(m, m.source[:, k], x0[:, k].copy(), cfg, inv_d)
+        xs.append(x)
+        sts.append(st)
+    return np.stack(xs, axis=1), tuple(sts)
+
+
+def initial_residual(m: Matrix, psi: np.ndarray) -> float:
+    """algorithms/utils/residual.py:12-36."""
+    res = m.multiply(psi) - m.source
+    return float(np.linalg.norm(res.ravel()) / (np.linalg.norm(m.source.ravel()) + np.finfo(np.float64).eps))
+
+
+# ---------------------------------------------------------------------------
+# Pressure-velocity coupling  (algorithms/utils/pressure_correction.py)
+# ---------------------------------------------------------------------------
+def adjust_phi(g: Grid, phi: FaceValues, U: Field, p: Field) -> bool:
+    """fv/adjust_phi.py:44-127."""
+    if any(bc.kind == "dirichlet" for bc in p.bcs.values()):
+        return False
+    mass_in = fixed_out = adj_out = 0.0
+    for patch, bc in U.bcs.items():
+        m = g.patch_mask(patch)
+        if not m.any():
+            continue
+        ph = phi.bnd[m]
+        mass_in -= ph[ph < 0].sum()
+        if bc.kind == "dirichlet":
+            fixed_out += ph[ph > 0].sum()
+        else:
+            adj_out += ph[ph > 0].sum()
+    if adj_out <= 0.0:
+        return False
+    corr = (mass_in - fixed_out) / adj_out
+    for patch, bc in U.bcs.items():
+        if bc.kind == "dirichlet":
+            continue
+        m = g.patch_mask(patch)
+        ph = phi.bnd[m]
+        phi.bnd[m] = np.where(ph > 0, ph * corr, ph)
+    return True
+
+
+def needs_reference(p: Field) -> bool:  # algorithms/utils/reference_value.py:14-30
+    return not any(bc.kind == "dirichlet" for bc in p.bcs.values())
+
+
+def solve_pressure(g, p, rAtU, div_hbya, cfg_p, cfg_final, n_nonorth, ref, corrected=True):
+    """pressure_correction.py:68-160 + reference_value.py:33-51."""
+    div_src = div_hbya * g.cell_volumes
+    first = None
+    for corr in range(n_nonorth + 1):
+        mat = -fvm_laplacian(g, rAtU, p, corrected)
+        mat.source = mat.source - div_src
+        if ref is not None:
+            cell, val = ref
+            rd = mat.diag[cell]
+            mat.diag[cell] += rd
+            mat.source[cell] += rd * val
+        res0 = initial_residual(mat, p.data)
+        if first is None:
+            first = res0
+        cfg = cfg_final if (cfg_final is not None and corr == n_nonorth) else cfg_p
+        p.data, st = solve(mat, p.data, cfg)
+    return mat, st, first
+
+
+def correct_phi(g, phi_hbya: FaceValues, pmat: Matrix, p: Field, rAtU: np.ndarray) -> FaceValues:
+    """pressure_correction.py:197-217, :270-305."""
+    flux_p = (-pmat).flux(p.data)
+    out = FaceValues(phi_hbya.single - flux_p, phi_hbya.bnd.copy())
+    for b in batches(g, p):
+        f, rv, rg, _ = bc_state(g, p, b)
+        coeff = value_grad_coeff(b, f)
+        sng = coeff * (rv - p.data[b.cells]) + (1.0 - f) * rg
+        out.bnd[b.mask] = out.bnd[b.mask] - rAtU[b.cells] * b.mag_Sf * sng
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Case description and drivers  (algorithms/simple.py, piso.py)
+# ---------------------------------------------------------------------------
+def _bc_from_cfg(entry: dict, cshape) -> BC:
+    kind = entry["type"]
+    if kind in ("dirichlet", "neumann"):
+        v = entry.get("value", 0.0)
+        return BC(kind, np.asarray(v, np.float64))
+    if kind in ("empty", "slip"):
+        return BC(kind)
+    raise ValueError(f"unsupported boundary type {kind}")
+
+
+def field_from_cfg(g: Grid, name: str, cond: dict, cshape) -> Field:
+    data = np.zeros((g.num_cells,) + tuple(cshape))
+    data[:] = np.asarray(cond.get("internal", 0.0), np.float64)
+    bcs: dict[str, BC] = {}
+    for entry in cond.get("boundary", {}).values():
+        bc = _bc_from_cfg(entry, cshape)
+        for patch in entry["patches"]:
+            bcs[patch] = bc
+    return Field(name, data, bcs)
+
+
+def grid_from_cfg(cfg: dict) -> Grid:
+    fx = cfg["fluxel"]
+    regions = [(r["min"], r["max"], r["level"]) for r in fx.get("refinement_regions", []) or []]
+    return Grid(fx["domain"]["lower"], fx["domain"]["upper"], fx["root_resolution"], regions,
+                fx.get("n_leaf_refinement", 0), cfg["simulator"]["control"]["deltaT"])
+
+
+def _solver_cfg(d: dict) -> SolverCfg:
+    keys = ("method", "preconditioner", "tolerance", "rel_tolerance", "max_iter", "norm_type", "max_restart")
+    return SolverCfg(**{k: d[k] for k in keys if k in d})
+
+
+class Case:
+    """SIMPLE / PISO drivers over the restated operators."""
+
+    def __init__(self, cfg: dict, grid: Grid | None = None):
+        sim = cfg["simulator"]
+        self.g = g = grid if grid is not None else grid_from_cfg(cfg)
+        cond = sim["conditions"]
+        self.U = field_from_cfg(g, "U", cond["U"], (3,))
+        self.p = field_from_cfg(g, "p", cond["p"], ())
+        self.nu = float(sim["properties"]["transport"]["nu"])
+        fvs = sim["fvSolution"]
+        self.alg = fvs["algorithm"]
+        self.kind = self.alg["type"]
+        self.adjust = fvs.get("adjustPhi", True)
+        self.solvers = {k: _solver_cfg(v) for k, v in fvs["solvers"].items()}
+        sch = sim.get("fvSchemes", {}) or {}
+
+        def look(group, key, default):
+            d = sch.get(group) or {}
+            return d.get(key, d.get("default", default))
+
+        self.div_scheme = look("divSchemes", "div(phi, U)", "upwind")
+        lap_u = look("laplacianSchemes", "laplacian(U)", "corrected")
+        lap_p = look("laplacianSchemes", "laplacian(p)", "corrected")
+        self.lap_u_corr = lap_u not in ("uncorrected", "Gauss linear uncorrected")
+        self.lap_p_corr = lap_p not in ("uncorrected", "Gauss linear uncorrected")
+        self.ddt_scheme = look("ddtSchemes", "ddt(U)", "euler")
+        self.ref = None
+        if needs_reference(self.p):
+            self.ref = (int(self.alg["pRefCell"]), float(self.alg["pRefValue"]))
+        self.phi = correct_flux(g, self.U)
+        self.U.old = self.U.data.copy()
+        self.U.older = None
+        self.phi_old = self.phi.single.copy()
+        self.phi_older = None
+        self.prev_dt = None
+        self.stats: dict[str, Any] = {}
+
+    def _momentum(self, transient: bool):
+        g, U = self.g, self.U
+        m = fvm_div(g, self.phi, U, self.div_scheme) - fvm_laplacian(g, self.nu, U, self.lap_u_corr)
+        if transient:
+            abc = ddt_coefficients(self.ddt_scheme, U.older is not None, g.dt, self.prev_dt)
+            m = fvm_ddt(g, U, abc) + m
+        return m
+
+    def _pressure_pass(self, UEqn: Matrix, transient: bool, final: bool, abc=None):
+        g, U, p = self.g, self.U, self.p
+        rAU = 1.0 / UEqn.A()
+        HbyA = Field("HbyA", rAU[:, None] * UEqn.H(U.data))
+        Hf = interpolate(g, HbyA)
+        phi_h = flux_from_face_velocity(g, Hf, U)
+        if transient:
+            rAUf = linear_face_values(g, rAU)
+            phi_h.single = phi_h.single + rAUf * ddt_corr(g, U, self.phi_old, abc, self.phi_older)
+        for b in batches(g, U):  # flux.py:74-87 constrainHbyA
+            if not U.bcs[b.patch].assignable:
+                phi_h.bnd[b.mask] = np.sum(bc_state(g, U, b)[3] * b.Sf, axis=1)
+        if self.adjust:
+            adjust_phi(g, phi_h, U, p)
+        div_h = fvc_div(g, phi_h)
+        cfg_final = None
+        if final:
+            cfg_final = self.solvers.get("pFinal", self.solvers.get("p_final"))
+        pmat, st, res0 = solve_pressure(g, p, rAU, div_h, self.solvers["p"], cfg_final,
+                                        int(self.alg.get("nNonOrthogonalCorrectors", 0)), self.ref,
+                                        self.lap_p_corr)
+        self.stats["p"] = st
+        self.stats["p_initial_residual"] = res0
+        return HbyA, rAU, phi_h, pmat
+
+    def step_simple(self):
+        """algorithms/simple.py:287-408."""
+        g, U, p = self.g, self.U, self.p
+        eq = self.alg.get("relaxationFactors", {}).get("equations", {})
+        aU, ap = float(eq["U"]), float(eq["p"])
+        UEqn = self._momentum(False)
+        A_old = UEqn.diag.copy()
+        UEqn.diag = A_old / aU
+        UEqn.source = UEqn.source + (((1.0 - aU) / aU) * A_old)[:, None] * U.data
+        gp = lsq_grad(g, p)
+        pred = UEqn.copy()
+        pred.source = UEqn.source - g.cell_volumes[:, None] * gp
+        U.data, self.stats["U"] = solve(pred, U.data, self.solvers["U"])
+        p_old = p.data.copy()
+        HbyA, rAU, phi_h, pmat = self._pressure_pass(UEqn, False, False)
+        self.phi = correct_phi(g, phi_h, pmat, p, rAU)
+        p.data = p_old + ap * (p.data - p_old)
+        U.data = HbyA.data - rAU[:, None] * lsq_grad(g, p)
+
+    def step_piso(self):
+        """algorithms/piso.py:227-326."""
+        g, U, p = self.g, self.U, self.p
+        abc = ddt_coefficients(self.ddt_scheme, U.older is not None, g.dt, self.prev_dt)
+        UEqn = self._momentum(True)
+        gp = lsq_grad(g, p)
+        pred = UEqn.copy()
+        pred.source = UEqn.source - g.cell_volumes[:, None] * gp
+        U.data, self.stats["U"] = solve(pred, U.data, self.solvers["U"])
+        n_corr = int(self.alg.get("nCorrectors", 2))
+        for i in range(n_corr):
+            HbyA, rAU, phi_h, pmat = self._pressure_pass(UEqn, True, i == n_corr - 1, abc)
+            self.phi = correct_phi(g, phi_h, pmat, p, rAU)
+            U.data = HbyA.data - rAU[:, None] * lsq_grad(g, p)
+        # field.py:241-255 -- history advance once per step
+        U.older, U.old = U.old, U.data.copy()
+        self.phi_older, self.phi_old = self.phi_old, self.phi.single.copy()
+        self.prev_dt = g.dt
+
+    def step(self):
+        if self.kind == "simple":
+            self.step_simple()
+        elif self.kind == "piso":
+            self.step_piso()
+        else:
+            raise ValueError(f"unsupported algorithm {self.kind}")
+
+
+def continuity_max(g: Grid, phi: FaceValues) -> float:
+    return float(np.max(np.abs(fvc_div(g, phi))))
+
+
+def _unused(*_a):  # keeps linters quiet about math import on some paths
+    return math.inf

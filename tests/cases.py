@@ -1,0 +1,53 @@
+"""Small case configurations shared by the CPU and GPU parity tests."""
+
+from __future__ import annotations
+
+import copy
+
+import yaml
+
+from gridfoam_amd.config import CASES
+
+FLUXEL = {
+    # tests/conftest.py:48-109 of the reference: 4x4x1, unit x 0.1
+    "uniform2d": dict(lower=[0, 0, 0], upper=[1, 1, 0.1], root=[4, 4, 1], regions=[], n_leaf=0),
+    # tests/helpers/configs.py:51-108: one centred level-1 region
+    "refined2d": dict(lower=[0, 0, 0], upper=[1, 1, 0.1], root=[4, 4, 1],
+                      regions=[([0.25, 0.25, 0.0], [0.75, 0.75, 0.1], 1)], n_leaf=0),
+    # tests/helpers/configs.py:111-: 4^3 root, centred level-1 cube
+    "refined3d": dict(lower=[0, 0, 0], upper=[1, 1, 1], root=[4, 4, 4],
+                      regions=[([0.25, 0.25, 0.25], [0.75, 0.75, 0.75], 1)], n_leaf=0),
+    # the 3-D octree cavity pattern on an 8^3 root grid
+    "walls8": dict(lower=[0, 0, 0], upper=[1, 1, 1], root=[8, 8, 8],
+                   regions=[([0, 0, 0], [0.1, 1, 1], 2), ([0.9, 0, 0], [1, 1, 1], 2),
+                            ([0, 0, 0], [1, 0.1, 1], 2), ([0, 0.9, 0], [1, 1, 1], 2),
+                            ([0, 0, 0], [1, 1, 0.1], 2), ([0, 0, 0.9], [1, 1, 1], 2)], n_leaf=0),
+    # odd root block counts, nested regions and a leaf refinement
+    "odd": dict(lower=[-1, 0, 2], upper=[2, 1, 3.5], root=[3, 5, 2],
+                regions=[([0.1, 0.1, 2.1], [0.7, 0.4, 2.6], 2), ([-0.5, 0.6, 3.0], [0.3, 0.9, 3.4], 1)], n_leaf=1),
+}
+
+
+def case_dict(name: str) -> dict:
+    return yaml.safe_load((CASES / name).read_text())
+
+
+def small_cavity3d(root=8, steps=None) -> dict:
+    """The octree PISO cavity case on a smaller root grid."""
+    cfg = copy.deepcopy(case_dict("cavity3d_octree_piso.yaml"))
+    cfg["fluxel"]["root_resolution"] = [root, root, root]
+    h_min = 1.0 / (root * 4)
+    cfg["simulator"]["control"]["deltaT"] = 0.5 * h_min
+    cfg["simulator"]["control"]["endTime"] = 0.5 * h_min * (steps or 10)
+    return cfg
+
+
+def fluxel_block(name: str) -> dict:
+    f = FLUXEL[name]
+    return {
+        "domain": {"lower": f["lower"], "upper": f["upper"]},
+        "root_resolution": f["root"],
+        "target_level": 0,
+        "n_leaf_refinement": f["n_leaf"],
+        "refinement_regions": [{"min": list(r[0]), "max": list(r[1]), "level": r[2]} for r in f["regions"]],
+    }
