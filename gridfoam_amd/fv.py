@@ -176,7 +176,8 @@ class fvm:
             if isinstance(gamma, torch.Tensor):
                 if tuple(gamma.shape) != (g.num_cells,):
                     raise ValueError("diffusivity must have shape [C]")
-                a.gamma_cell = nat.ptr(gamma.contiguous())
+                gamma_c = gamma.contiguous()  # keep alive until the launch below
+                a.gamma_cell = nat.ptr(gamma_c)
             else:
                 a.gamma = float(gamma)
             need_grad |= corr

@@ -129,9 +129,10 @@ class FvMatrix:
     def multiply(self, x: torch.Tensor) -> torch.Tensor:
         """fvmatrix.py:292-323 -- deterministic gather SpMV."""
         nc = 1 if x.dim() == 1 else x.shape[1]
-        y = torch.empty_like(x)
+        xc = x.contiguous()
+        y = torch.empty_like(xc)
         nat.call("gf_ldu_multiply", self.grid.mesh_ref, ctypes.c_int32(nc), nat.ptr(self._diag),
-                 nat.ptr(self._upper), nat.ptr(self._lower), nat.ptr(x.contiguous()), nat.ptr(y), nat.stream())
+                 nat.ptr(self._upper), nat.ptr(self._lower), nat.ptr(xc), nat.ptr(y), nat.stream())
         return y
 
     def A(self) -> torch.Tensor:
@@ -155,8 +156,9 @@ class FvMatrix:
         F = self.grid.num_internal_faces
         out = torch.empty((max(F, 1),) + tuple(psi.shape[1:]), dtype=torch.float64, device=psi.device)
         ffc = self.face_flux_correction
+        pc = psi.contiguous()
         nat.call("gf_ldu_flux", self.grid.mesh_ref, ctypes.c_int32(nc), ctypes.c_double(1.0), nat.ptr(self._upper),
-                 nat.ptr(self._lower), nat.ptr(ffc), nat.ptr(psi.contiguous()), nat.ptr(out), nat.stream())
+                 nat.ptr(self._lower), nat.ptr(ffc), nat.ptr(pc), nat.ptr(out), nat.stream())
         return out[:F]
 
     def with_fixed_values(self, cells: torch.Tensor, values: torch.Tensor) -> "FvMatrix":

@@ -105,9 +105,10 @@ class LinearSolver:
         cfg = self._cfg(mixed)
         ws = self._ws.get(g, nrhs, cfg)
         x = x0.detach().clone().contiguous()
+        bc = b.detach().contiguous()
         stats = (ctypes.c_double * (4 * nrhs))()
         nat.call("gf_krylov_solve", g.mesh_ref, ctypes.byref(cfg), ctypes.c_int32(nrhs), ctypes.c_int32(int(transpose)),
-                 nat.ptr(A._diag), nat.ptr(A._upper), nat.ptr(A._lower), nat.ptr(b.detach().contiguous()),
+                 nat.ptr(A._diag), nat.ptr(A._upper), nat.ptr(A._lower), nat.ptr(bc),
                  nat.ptr(x), nat.ptr(ws), stats, nat.stream())
         out = []
         for r in range(nrhs):
@@ -192,7 +193,8 @@ class _ImplicitAdjoint(torch.autograd.Function):
         gd = torch.empty_like(A._diag)
         gu = torch.empty_like(A._upper)
         gl = torch.empty_like(A._lower)
-        nat.call("gf_ldu_grads", g.mesh_ref, ctypes.c_int32(nc), nat.ptr(lam.contiguous()), nat.ptr(x.contiguous()),
+        lam_c, x_c = lam.contiguous(), x.contiguous()
+        nat.call("gf_ldu_grads", g.mesh_ref, ctypes.c_int32(nc), nat.ptr(lam_c), nat.ptr(x_c),
                  nat.ptr(gd), nat.ptr(gu), nat.ptr(gl), nat.stream())
         F = g.num_internal_faces
         return gd, gu[:F], gl[:F], lam, None, None, None

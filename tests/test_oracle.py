@@ -1,0 +1,309 @@
+"""Pins the FP64 oracle (oracle/) against the reference's own known-answer
+tests, closed forms and invariants.  CPU only."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import fv_oracle as fo
+from tests.cases import FLUXEL
+
+
+def _grid(name, dt=0.01):
+    f = FLUXEL[name]
+    return fo.Grid(f["lower"], f["upper"], f["root"], f["regions"], f["n_leaf"], dt=dt)
+
+
+# ----------------------------------------------------------------- octree build
+@pytest.mark.parametrize("root", [(4, 4, 1), (3, 5, 2), (7, 1, 1), (2, 2, 2)])
+def test_uniform_closed_forms(root):
+    """F = (nx-1) ny nz + nx (ny-1) nz + nx ny (nz-1); F_b = 2 (ny nz + nx nz + nx ny)."""
+    nx, ny, nz = root
+    t = fo.build_octree([0, 0, 0], [1, 1, 1], root)
+    assert len(t.level) == nx * ny * nz
+    assert len(t.owner) == (nx - 1) * ny * nz + nx * (ny - 1) * nz + nx * ny * (nz - 1)
+    assert len(t.bnd_owner) == 2 * (ny * nz + nx * nz + nx * ny)
+
+
+def _morton(i, j, k):
+    key = 0
+    for b in range(21):
+        key |= ((i >> b) & 1) << (3 * b) | ((j >> b) & 1) << (3 * b + 1) | ((k >> b) & 1) << (3 * b + 2)
+    return key
+
+
+@pytest.mark.parametrize("name", list(FLUXEL))
+def test_octree_invariants(name):
+    f = FLUXEL[name]
+    g = _grid(name)
+    t = g.topo
+    lev = t.level.astype(np.int64)
+    # Morton order of min corners at the finest level, unique keys
+    s = t.lmax - lev
+    keys = [_morton(int(a) << int(b), int(c) << int(b), int(d) << int(b)) for (a, c, d), b in zip(t.ijk, s)]
+    assert all(k1 < k2 for k1, k2 in zip(keys, keys[1:]))
+    # neighbour on the +axis side, touching, 2:1 balanced (axis_projected.py:763-767)
+    C = g.cell_centers
+    o, n, a = g.owner, g.neighbour, g.axis
+    fi = np.arange(len(o))
+    assert np.all(C[n, a] > C[o, a])
+    gap = (C[n, a] - g.cell_sizes[n, a] / 2) - (C[o, a] + g.cell_sizes[o, a] / 2)
+    assert np.all(np.abs(gap) < 1e-12)
+    assert np.all(np.abs(lev[o] - lev[n]) <= 1)
+    # canonical order: owner, axis, neighbour
+    order = np.lexsort((n, a, o))
+    assert np.array_equal(order, fi)
+    # every cell face is covered once: internal + domain areas equal the six cell faces
+    area = np.zeros((g.num_cells, 6))
+    Sfa = g.Sf[fi, a]
+    np.add.at(area, (o, 2 * a + 1), Sfa)
+    np.add.at(area, (n, 2 * a), Sfa)
+    bd = g.bnd_dir
+    np.add.at(area, (g.bnd_owner, bd), np.abs(g.bnd_Sf[np.arange(g.num_bnd), bd // 2]))
+    sz = g.cell_sizes
+    want = np.stack([sz[:, 1] * sz[:, 2]] * 2 + [sz[:, 0] * sz[:, 2]] * 2 + [sz[:, 0] * sz[:, 1]] * 2, axis=1)
+    np.testing.assert_allclose(area, want, rtol=1e-12)
+    # refinement regions reached their level on every intersecting cell
+    lo = g.cell_centers - sz / 2
+    hi = g.cell_centers + sz / 2
+    for rmin, rmax, rl in f["regions"]:
+        hit = np.all((lo < np.asarray(rmax)) & (hi > np.asarray(rmin)), axis=1)
+        assert np.all(lev[hit] >= rl + f["n_leaf"])
+    # total volume
+    vol = np.prod(np.asarray(f["upper"], float) - np.asarray(f["lower"], float))
+    assert abs(g.cell_volumes.sum() - vol) < 1e-12 * vol
+
+
+def test_cavity_octree_count():
+    """32^3 root with level-2 wall slabs of thickness 0.1: 128^3-100^3 + 50^3-48^3 + 24^3 cells."""
+    t = fo.build_octree([0, 0, 0], [1, 1, 1], [32, 32, 32],
+                        [([0, 0, 0], [0.1, 1, 1], 2), ([0.9, 0, 0], [1, 1, 1], 2), ([0, 0, 0], [1, 0.1, 1], 2),
+                         ([0, 0.9, 0], [1, 1, 1], 2), ([0, 0, 0], [1, 1, 0.1], 2), ([0, 0, 0.9], [1, 1, 1], 2)])
+    assert len(t.level) == 128 ** 3 - 100 ** 3 + 50 ** 3 - 48 ** 3 + 24 ** 3 == 1125384
+
+
+def test_balance_forces_split():
+    """A level-2 box next to root cells forces level-1 cells in between."""
+    t = fo.build_octree([0, 0, 0], [4, 1, 1], [4, 1, 1], [([0.0, 0, 0], [0.2, 0.2, 0.2], 2)])
+    lev = t.level
+    assert lev.max() == 2 and 1 in lev
+    g_owner, g_nbr = t.owner, t.neighbour
+    assert np.all(np.abs(lev[g_owner] - lev[g_nbr]) <= 1)
+
+
+# ----------------------------------------------------------------- solvers (KATs)
+def _diag_matrix(g, d):
+    m = fo.Matrix(g)
+    m.diag = np.asarray(d, float)
+    return m
+
+
+@pytest.mark.parametrize("atol,rtol,expected,iters,conv", [
+    (2.0, 0.0, (0.0, 0.0), 0, True),
+    (0.6, 0.0, (2 / 3, 2 / 3), 1, True),
+    (0.2, 0.0, (13 / 15, 7 / 15), 1, True),
+    (1e-12, 0.5, (2 / 3, 2 / 3), 1, True),
+    (1e-12, 0.2, (13 / 15, 7 / 15), 1, True),
+    (1e-12, 0.0, (13 / 15, 7 / 15), 1, False),
+])
+def test_oracle_bicgstab_stages(atol, rtol, expected, iters, conv):
+    """tests/unit/test_solver_stats.py:112-161 of the reference."""
+    g = _grid("uniform2d")
+    d = np.ones(g.num_cells)
+    d[1] = 2.0
+    m = _diag_matrix(g, d)
+    m.source = np.zeros(g.num_cells)
+    m.source[:2] = 1.0
+    x, st = fo.solve(m, np.zeros(g.num_cells), fo.SolverCfg("bicgstab", "none", atol, rtol, 1))
+    exp = np.zeros(g.num_cells)
+    exp[:2] = expected
+    np.testing.assert_allclose(x, exp, rtol=1.3e-6, atol=1e-5)
+    assert st[0].iterations == iters and st[0].converged is conv
+    assert st[0].initial_residual == pytest.approx(2 ** 0.5)
+    assert st[0].final_residual == pytest.approx(np.linalg.norm(m.source - m.multiply(x)))
+
+
+def test_oracle_jacobi_quarter():
+    """tests/unit/solvers/test_preconditioners.py:21-32."""
+    g = _grid("uniform2d")
+    m = _diag_matrix(g, np.full(g.num_cells, 4.0))
+    np.testing.assert_allclose(fo._inv_diag(m, "jacobi"), 0.25)
+    np.testing.assert_allclose(fo._inv_diag(m, "none"), 1.0)
+
+
+def test_oracle_cg_threshold_semantics():
+    """base.py:140-162: max(atol, rtol |r0|) with strict <; zero-iteration exit."""
+    g = _grid("uniform2d")
+    m = _diag_matrix(g, np.full(g.num_cells, 1.0))
+    m.source = np.ones(g.num_cells)
+    x, st = fo.solve(m, np.zeros(g.num_cells), fo.SolverCfg("cg", "none", 4.0 + 1e-12, 0.0, 10))
+    assert st[0].iterations == 0 and st[0].converged  # |r0| = 4 < 4 + 1e-12
+    x, st = fo.solve(m, np.zeros(g.num_cells), fo.SolverCfg("cg", "none", 4.0, 0.0, 10))
+    assert st[0].iterations == 1  # |r0| = 4 is not < 4
+
+
+def _laplace_system(name, seed=0):
+    g = _grid(name)
+    rng = np.random.default_rng(seed)
+    p = fo.Field("p", np.zeros(g.num_cells), {k: fo.BC("neumann", 0.0) for k in fo.PATCHES})
+    gamma = 0.5 + rng.random(g.num_cells)
+    m = -fo.fvm_laplacian(g, gamma, p, False)
+    m.diag[0] *= 2
+    b = rng.standard_normal(g.num_cells)
+    m.source = b
+    return g, m, b
+
+
+def _dense(m):
+    g = m.g
+    A = np.diag(m.diag.copy())
+    np.add.at(A, (g.owner, g.neighbour), m.upper)
+    np.add.at(A, (g.neighbour, g.owner), m.lower)
+    return A
+
+
+@pytest.mark.parametrize("name", ["refined2d", "refined3d", "walls8"])
+def test_oracle_pcg_matches_dense_solve(name):
+    """Dense reference solve (test_component_shapes.py:111-166 pattern)."""
+    g, m, b = _laplace_system(name)
+    A = _dense(m)
+    x_ref = np.linalg.solve(A, b)
+    x, st = fo.solve(m, np.zeros(g.num_cells), fo.SolverCfg("cg", "jacobi", 0.0, 1e-13, 5000))
+    assert st[0].converged
+    assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) < 1e-9
+    # SpMV against the dense product and the transpose identity y.Ax = A^T y.x
+    rng = np.random.default_rng(1)
+    v, y = rng.standard_normal(g.num_cells), rng.standard_normal(g.num_cells)
+    m.lower = m.lower * 1.3  # make it non-symmetric
+    A = _dense(m)
+    np.testing.assert_allclose(m.multiply(v), A @ v, rtol=1e-13, atol=1e-13)
+    assert abs(y @ m.multiply(v) - m.transpose().multiply(y) @ v) < 1e-12 * abs(y @ m.multiply(v))
+
+
+def test_oracle_bicgstab_nonsymmetric_dense():
+    g, m, b = _laplace_system("refined3d")
+    m.upper = m.upper + 0.05
+    m.lower = m.lower - 0.05
+    A = _dense(m)
+    x_ref = np.linalg.solve(A, b)
+    x, st = fo.solve(m, np.zeros(g.num_cells), fo.SolverCfg("bicgstab", "jacobi", 1e-13, 0.0, 2000))
+    assert st[0].converged
+    assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) < 1e-9
+
+
+# ----------------------------------------------------------------- FV closed forms
+def _linear_field(g, coef=(0.3, -1.7, 2.2), c0=0.4, bcs="dirichlet"):
+    """tests/helpers/fields.py:15-49: linear psi with consistent BCs."""
+    coef = np.asarray(coef)
+    psi = g.cell_centers @ coef + c0
+    fld = fo.Field("psi", psi, {})
+    for patch in fo.PATCHES:
+        fld.bcs[patch] = fo.BC("neumann", 0.0)  # replaced per face below
+    return fld, coef
+
+
+@pytest.mark.parametrize("name", ["refined2d", "refined3d", "walls8", "odd"])
+def test_lsq_gradient_exact_for_linear_fields(name):
+    """leastsquare is exact for linear fields on every octree cell (grad.py:473-496)."""
+    g = _grid(name)
+    coef = np.array([0.3, -1.7, 2.2])
+    psi = g.cell_centers @ coef + 0.4
+    # Dirichlet BCs cannot vary along a patch in the value-fraction form used
+    # here, so use outward-gradient Neumann values that are exact per patch.
+    bcs = {}
+    for d, patch in enumerate(fo.PATCHES):
+        sign = 2 * (d % 2) - 1
+        bcs[patch] = fo.BC("neumann", sign * coef[d // 2])
+    fld = fo.Field("psi", psi, bcs)
+    grad = fo.lsq_grad(g, fld)
+    np.testing.assert_allclose(grad, np.broadcast_to(coef, grad.shape), rtol=0, atol=1e-11)
+
+
+@pytest.mark.parametrize("name", ["refined2d", "refined3d", "walls8"])
+def test_corrected_laplacian_flux_exact_for_linear_fields(name):
+    """tests/integration/fv/test_laplacian_schemes.py:29-42: matrix flux is
+    exact (1e-12) for linear fields on refined meshes; uncorrected is not."""
+    g = _grid(name)
+    coef = np.array([0.3, -1.7, 2.2])
+    psi = g.cell_centers @ coef + 0.4
+    bcs = {patch: fo.BC("neumann", (2 * (d % 2) - 1) * coef[d // 2]) for d, patch in enumerate(fo.PATCHES)}
+    fld = fo.Field("psi", psi, bcs)
+    m = fo.fvm_laplacian(g, 1.0, fld, True)
+    exact = g.geo.mag_Sf * coef[g.axis]
+    np.testing.assert_allclose(m.flux(psi), exact, rtol=1e-12, atol=1e-12)
+    if len(g.geo.hang_idx):
+        mu = fo.fvm_laplacian(g, 1.0, fld, False)
+        assert np.abs(mu.flux(psi) - exact).max() > 1e-3
+
+
+def test_div_of_linear_velocity_is_one():
+    """tests/integration/fv/test_div.py:34-61: div(U=(x,0,0)) = 1, div(uniform) = 0."""
+    g = _grid("refined3d")
+    U = fo.Field("U", np.stack([g.cell_centers[:, 0], 0 * g.cell_centers[:, 0], 0 * g.cell_centers[:, 0]], 1), {})
+    for d, patch in enumerate(fo.PATCHES):
+        U.bcs[patch] = fo.BC("neumann", np.array([(2 * (d % 2) - 1) * (d // 2 == 0), 0.0, 0.0], float))
+    phi = fo.flux_from_face_velocity(g, fo.interpolate(g, U), U)
+    np.testing.assert_allclose(fo.fvc_div(g, phi), 1.0, atol=1e-12)
+    U.data = np.ones_like(U.data)
+    for patch in fo.PATCHES:
+        U.bcs[patch] = fo.BC("dirichlet", np.ones(3))
+    phi = fo.flux_from_face_velocity(g, fo.interpolate(g, U), U)
+    np.testing.assert_allclose(fo.fvc_div(g, phi), 0.0, atol=1e-12)
+
+
+def test_ddt_weights():
+    """tests/integration/fv/test_ddt_schemes.py:76-99 and test_backward_algorithms.py:20-72."""
+    assert fo.ddt_coefficients("euler", True, 0.1, 0.1) == (1.0, 1.0, 0.0)
+    assert fo.ddt_coefficients("backward", False, 0.1, None) == (1.0, 1.0, 0.0)
+    np.testing.assert_allclose(fo.ddt_coefficients("backward", True, 0.1, 0.1), (1.5, 2.0, 0.5))
+    np.testing.assert_allclose(fo.ddt_coefficients("backward", True, 0.3, 0.2), (1.6, 2.5, 0.9), rtol=1e-12)
+    # BDF2 differentiates the unequal-step quadratic q = t^2 exactly:
+    # q(0) = 0, q(0.2) = 0.04, q(0.5) = 0.25  ->  dq/dt(0.5) = 1
+    g = _grid("uniform2d", dt=0.3)
+    q = fo.Field("q", np.full(g.num_cells, 0.25), {})
+    q.old, q.older = np.full(g.num_cells, 0.04), np.zeros(g.num_cells)
+    m = fo.fvm_ddt(g, q, fo.ddt_coefficients("backward", True, 0.3, 0.2))
+    np.testing.assert_allclose((m.multiply(q.data) - m.source) / g.cell_volumes, 1.0, rtol=1e-12)
+
+
+def test_poisson_mms_order():
+    """tests/integration/fv/test_spatial_convergence.py:20-140: manufactured
+    Poisson solution converges with order > 1.8 on uniform grids."""
+    errs = []
+    for n in (8, 16, 32):
+        g = fo.Grid([0, 0, 0], [1, 1, 1.0 / n], [n, n, 1])
+        x, y = g.cell_centers[:, 0], g.cell_centers[:, 1]
+        exact = np.cos(np.pi * x) * np.cos(np.pi * y)
+        f = -2 * np.pi ** 2 * exact
+        p = fo.Field("p", np.zeros(g.num_cells), {k: fo.BC("neumann", 0.0) for k in fo.PATCHES[:4]})
+        p.bcs["z_minus"] = fo.BC("empty")
+        p.bcs["z_plus"] = fo.BC("empty")
+        m = fo.fvm_laplacian(g, 1.0, p, True)
+        m = -m
+        m.source = -f * g.cell_volumes
+        m.diag[0] *= 2
+        m.source[0] += m.diag[0] / 2 * exact[0]
+        xs, st = fo.solve(m, np.zeros(g.num_cells), fo.SolverCfg("cg", "jacobi", 0.0, 1e-12, 5000))
+        err = xs - exact
+        err -= err.mean()
+        errs.append(np.sqrt(np.mean(err ** 2)))
+    orders = np.log2(np.asarray(errs[:-1]) / np.asarray(errs[1:]))
+    assert np.all(orders > 1.8), orders
+
+
+def test_simple_cavity_conserves_mass_and_converges():
+    """test_pressure_coupling.py:66-197: |div phi| small after SIMPLE steps."""
+    import yaml
+
+    from gridfoam_amd.config import CASES
+
+    cfg = yaml.safe_load((CASES / "cavity2d_simple.yaml").read_text())
+    cfg["fluxel"]["root_resolution"] = [16, 16, 1]
+    cfg["fluxel"]["domain"]["upper"] = [1.0, 1.0, 1.0 / 16]
+    case = fo.Case(cfg)
+    for _ in range(10):
+        case.step()
+    div = fo.fvc_div(case.g, case.phi) * case.g.cell_volumes
+    assert np.abs(div).max() < 1e-8
+    assert case.U.data[:, 0].max() > 0.1
