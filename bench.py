@@ -1,0 +1,225 @@
+#!/usr/bin/env python
+"""Throughput of the PISO pressure-velocity step on one MI355X.
+
+Workload: 3-D lid-driven cavity on an octree (unit cube, 32^3 root blocks,
+two refinement levels in cells within 0.1 of any wall, 2:1 balanced,
+1,125,384 cells), Re = 1000, PISO with 2 correctors at CFL 0.5.  One "step"
+is one PISO time step.  The run is made twice from the same initial state:
+plain FP64 and with the FP32-SpMV PCG inside the FP64 refinement loop; the
+headline value is the mixed-precision rate when its final U and p are
+within 1e-6 relative L2 of the plain FP64 run.
+
+Also measured: the dominant kernel (the PCG SpMV) timed with HIP events on
+the solver's stream, against the HBM roofline, and the FP64 CPU oracle on
+the host cores as a bounded sample of the same step.
+"""
+
+from __future__ import annotations
+
+import argparse
+import copy
+import json
+import os
+import pathlib
+import sys
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+CASE = "cavity3d_octree_piso.yaml"
+
+
+def _log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _cfg(mixed: bool) -> dict:
+    import yaml
+
+    from gridfoam_amd.config import CASES
+
+    cfg = yaml.safe_load((CASES / CASE).read_text())
+    for k in ("p", "pFinal"):
+        cfg["simulator"]["fvSolution"]["solvers"][k]["mixed_precision"] = bool(mixed)
+    return cfg
+
+
+def run_gpu(mixed: bool, steps: int, warmup: int):
+    import torch
+
+    from gridfoam_amd.algorithms import create_case
+    from gridfoam_amd.config import load_config
+
+    grid, alg = create_case(load_config(_cfg(mixed)), device="cuda:0")
+    for _ in range(warmup):
+        alg.step()
+    torch.cuda.synchronize()
+    p_iters = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        alg.step()
+        p_iters += sum(s.iterations for s in alg.p_solve_history[-alg.n_correctors:])
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return grid, alg, dt, p_iters
+
+
+def spmv_roofline(grid, pmat, fp32: bool, launches: int = 200) -> dict:
+    """Average duration of the PCG SpMV kernel (the solver's own kernel,
+    standalone launch) with HIP events on the current stream."""
+    import ctypes
+
+    import torch
+
+    from gridfoam_amd import _native as nat
+
+    dt = torch.float32 if fp32 else torch.float64
+    val = torch.empty(grid.mesh.nnz_pad, dtype=dt, device=grid.device)
+    nat.call("gf_sell_pack", grid.mesh_ref, ctypes.c_int32(0), nat.ptr(pmat._upper), nat.ptr(pmat._lower),
+             nat.ptr(val), ctypes.c_int32(int(fp32)), nat.stream())
+    diag = pmat._diag.contiguous()  # FP64 in both operators
+    x = torch.randn(grid.num_cells, dtype=dt, device=grid.device)
+    y = torch.empty_like(x)
+    scratch = torch.empty(int(nat.lib().gf_sell_spmv_scratch_bytes()), dtype=torch.uint8, device=grid.device)
+    args = (grid.mesh_ref, nat.ptr(diag), nat.ptr(val), nat.ptr(x), nat.ptr(y), ctypes.c_int32(int(fp32)),
+            nat.ptr(scratch), nat.stream())
+    for _ in range(10):
+        nat.call("gf_sell_spmv", *args)
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(launches):
+        nat.call("gf_sell_spmv", *args)
+    e1.record(s)
+    torch.cuda.synchronize()
+    avg_s = e0.elapsed_time(e1) / 1e3 / launches  # includes the tiny state-reset kernel
+    C, F = grid.num_cells, grid.num_internal_faces
+    # algorithmic bytes per SpMV (symmetric LDU, int32 indices):
+    # B = s_d C + s_v F + 8 F + 2 s_x C ; FP64: 24C + 16F ; mixed operator
+    # (FP32 values/vectors, FP64 diagonal): 16C + 12F
+    sd, sv, sx = (8, 4, 4) if fp32 else (8, 8, 8)
+    B = sd * C + sv * F + 8 * F + 2 * sx * C
+    achieved = B / avg_s / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _traffic(fp32), "kernel": "k_mx_spmv<1>" if fp32 else "k_cg_spmv<double, 1>",
+            "bytes_per_launch": int(B), "avg_us": round(avg_s * 1e6, 2)}
+
+
+def _traffic(fp32: bool):
+    """HBM bytes per SpMV launch from the committed rocprofv3 counter runs."""
+    p = ROOT / "profiles" / "spmv_traffic.json"
+    if not p.exists():
+        return None
+    try:
+        d = json.loads(p.read_text())
+        return d.get("fp32" if fp32 else "fp64", {}).get("bytes_per_launch")
+    except (ValueError, OSError):
+        return None
+
+
+def cpu_baseline(gpu_p_iters_per_step: float) -> dict:
+    """FP64 oracle on the host: one PISO step of the same case with the
+    pressure CG capped at 0 iterations, plus 20 timed CG iterations on that
+    step's pressure matrix, scaled to the GPU run's CG iterations per step."""
+    import numpy as np
+
+    from oracle import fv_oracle as fo
+
+    cfg = _cfg(False)
+    for k in ("p", "pFinal"):
+        cfg["simulator"]["fvSolution"]["solvers"][k]["max_iter"] = 0
+    case = fo.Case(cfg)
+    t0 = time.perf_counter()
+    case.step()
+    t_step = time.perf_counter() - t0
+    g = case.g
+    rAU = 1.0 / case._momentum(True).A()
+    pm = -fo.fvm_laplacian(g, rAU, case.p, True)
+    pm.source = np.random.default_rng(0).standard_normal(g.num_cells)
+    scfg = fo.SolverCfg("cg", "jacobi", 0.0, 1e-30, 20)
+    t0 = time.perf_counter()
+    fo.solve(pm, np.zeros(g.num_cells), scfg)
+    t_it = (time.perf_counter() - t0) / 20
+    t_est = t_step + gpu_p_iters_per_step * t_it
+    return {"value": round(g.num_cells / t_est / 1e6, 4), "unit": "Mcell-steps/s", "cores": 1, "kind": "port",
+            "sample": (f"one PISO step of the same {g.num_cells}-cell case (pressure CG capped at 0 iterations, "
+                       f"{t_step:.1f} s) + 20 timed CG iterations ({t_it * 1e3:.0f} ms each), scaled to "
+                       f"{gpu_p_iters_per_step:.0f} CG iterations per step as on the GPU; numpy FP64, 1 thread")}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--mode", choices=("both", "plain", "mixed"), default="both")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only; multi-GPU runs are not part of this build", file=sys.stderr)
+        return 2
+    import numpy as np
+    import torch
+
+    torch.cuda.set_device(0)
+    out = {}
+    for mode in (("plain", "mixed") if a.mode == "both" else (a.mode,)):
+        _log(f"{mode}: {a.warmup} warmup + {a.steps} timed steps")
+        grid, alg, dt, iters = run_gpu(mode == "mixed", a.steps, a.warmup)
+        out[mode] = dict(grid=grid, alg=alg, dt=dt, iters=iters,
+                         U=alg.U.data.detach().cpu().numpy(), p=alg.p.data.detach().cpu().numpy())
+        _log(f"{mode}: {dt:.2f} s, {dt / a.steps * 1e3:.1f} ms/step, {iters / a.steps:.0f} p-CG iterations/step")
+    head = "mixed" if "mixed" in out else "plain"
+    rel = None
+    if "mixed" in out and "plain" in out:
+        P, M = out["plain"], out["mixed"]
+        rel = {"U": float(np.linalg.norm(M["U"] - P["U"]) / np.linalg.norm(P["U"])),
+               "p": float(np.linalg.norm(M["p"] - P["p"]) / np.linalg.norm(P["p"]))}
+        if max(rel.values()) > 1e-6:
+            _log(f"mixed precision error {rel} exceeds 1e-6: headline falls back to plain FP64")
+            head = "plain"
+    H = out[head]
+    grid = H["grid"]
+    C = grid.num_cells
+    value = C * a.steps / H["dt"] / 1e6
+    roof = spmv_roofline(grid, H["alg"].last_p_matrix, fp32=(head == "mixed"))
+    line = {
+        "metric": "Mcell-steps/s at <=1e-6 rel err vs plain FP64",
+        "value": round(value, 3),
+        "unit": "Mcell-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(H["dt"] / a.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-in-f64" if head == "mixed" else "f64",
+        "data": "synthetic (lid-driven cavity initial state: U=0, p=0)",
+        "config": {"workload": "3-D lid-driven cavity, octree 32^3 root + 2 wall levels, PISO 2 correctors, Re=1000",
+                   "cells": C, "internal_faces": grid.num_internal_faces, "parallelism": "single GPU"},
+        "roofline": roof,
+    }
+    if "plain" in out:
+        line["plain_fp64"] = {"value": round(C * a.steps / out["plain"]["dt"] / 1e6, 3),
+                              "ms_per_step": round(out["plain"]["dt"] / a.steps * 1e3, 3),
+                              "p_cg_iters_per_step": round(out["plain"]["iters"] / a.steps, 1)}
+    if "mixed" in out:
+        line["mixed"] = {"value": round(C * a.steps / out["mixed"]["dt"] / 1e6, 3),
+                         "ms_per_step": round(out["mixed"]["dt"] / a.steps * 1e3, 3),
+                         "p_cg_iters_per_step": round(out["mixed"]["iters"] / a.steps, 1)}
+    if rel is not None:
+        line["mixed_vs_fp64_rel_l2"] = rel
+        line["speedup_mixed_vs_fp64"] = round(out["plain"]["dt"] / out["mixed"]["dt"], 3)
+    if not a.no_cpu:
+        ref_iters = out["plain"]["iters"] / a.steps if "plain" in out else H["iters"] / a.steps
+        _log("cpu baseline (FP64 oracle, bounded sample)")
+        line["cpu_baseline"] = cpu_baseline(ref_iters)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -175,6 +175,8 @@ class AlgorithmBase:
             self.p_ref_cell = self.p_ref_value = None
         self.solve_stats: Dict[str, Tuple[SolveStats, ...]] = {}
         self.last_p_residual: Optional[torch.Tensor] = None
+        self.last_p_matrix: Optional[FvMatrix] = None
+        self.p_solve_history: list = []
 
     @property
     def grid(self) -> OctreeGrid:
@@ -206,6 +208,8 @@ class AlgorithmBase:
                                      p_ref_value=self.p_ref_value, final_solver=p_solver if transient else None)
         self.solve_stats[self.p.name] = res.stats
         self.last_p_residual = res.initial_residual
+        self.last_p_matrix = res.matrix
+        self.p_solve_history.extend(res.stats)
         return res
 
 

@@ -516,7 +516,7 @@ __global__ void k_axpby(int64_t n, double a, const double *x, double b, const do
 template <int NC>
 __global__ __launch_bounds__(kBlock) void k_init_res(const gf_mesh m, const double *diag, const double *upper,
                                                      const double *lower, const double *src, const double *psi,
-                                                     double *partials, unsigned int *counter, double *out) {
+                                                     double *partials) {
     int c = blockIdx.x * kBlock + threadIdx.x;
     double v[2] = {0.0, 0.0};
     if (c < m.n_cells) {
@@ -544,15 +544,13 @@ __global__ __launch_bounds__(kBlock) void k_init_res(const gf_mesh m, const doub
         partials[blockIdx.x] = v[0];
         partials[nb + blockIdx.x] = v[1];
     }
-    if (!last_block_done(counter)) return;
-    double a = reduce_partials<kBlock>(partials, nb, false);
-    double b = reduce_partials<kBlock>(partials + nb, nb, false);
-    if (threadIdx.x == 0) out[0] = sqrt(a) / (sqrt(b) + 2.220446049250313e-16);
 }
 
-__global__ void k_zero_u32(unsigned int *p, int n) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0u;
+// second level in one workgroup (no cross-workgroup fences on multi-XCD parts)
+__global__ __launch_bounds__(1024) void k_init_res_fin(const double *partials, int nb, double *out) {
+    double a = reduce_partials<1024>(partials, nb, false);
+    double b = reduce_partials<1024>(partials + nb, nb, false);
+    if (threadIdx.x == 0) out[0] = sqrt(a) / (sqrt(b) + 2.220446049250313e-16);
 }
 
 // ldu_grads.py:11-53
@@ -832,9 +830,8 @@ extern "C" int gf_initial_residual(const gf_mesh *m, int32_t nc, const double *d
     GF_REQUIRE(m && diag && upper && lower && source && psi && scratch && out, "null argument");
     hipStream_t s = S(stream);
     const int nb = grid_for(m->n_cells);
-    unsigned int *counter = reinterpret_cast<unsigned int *>(scratch + 2 * nb);
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, counter, 2);
-    GF_NC_DISPATCH(nc, k_init_res, nb, *m, diag, upper, lower, source, psi, scratch, counter, out);
+    GF_NC_DISPATCH(nc, k_init_res, nb, *m, diag, upper, lower, source, psi, scratch);
+    hipLaunchKernelGGL(k_init_res_fin, dim3(1), dim3(1024), 0, s, scratch, nb, out);
     GF_CHECK_LAUNCH();
     return GF_OK;
 }

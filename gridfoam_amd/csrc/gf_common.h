@@ -85,23 +85,9 @@ __device__ __forceinline__ void block_reduce(double (&v)[NQ], const bool (&is_ma
     __syncthreads();
 }
 
-// Last-block-done: returns true in exactly one block, after every block's
-// partials are visible.  The counter is reset by that block for reuse.
-__device__ __forceinline__ bool last_block_done(unsigned int *counter) {
-    __shared__ bool is_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned int prev = atomicAdd(counter, 1u);
-        is_last = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (is_last) {
-        __threadfence();
-        if (threadIdx.x == 0) *counter = 0u;
-    }
-    return is_last;
-}
+// No "last workgroup done" reductions here: on the 8-XCD MI355X every
+// device-scope fence writes back the issuing XCD's L2, which costs hundreds
+// of microseconds per kernel.  Partials are reduced by the next launch.
 
 // Deterministic reduction of partials[n] by one block (fixed strided order
 // then fixed tree).  Result valid in thread 0.

@@ -3,110 +3,199 @@
 // Restates solvers/cg.py:75-174 (PCG), solvers/bicgstab.py:76-272
 // (right-preconditioned BiCGSTAB with restarts), solvers/preconditioners.py:
 // 76-96 (Jacobi) and solvers/base.py:140-162 (threshold max(atol, rtol*|r0|),
-// strict <).  The host never synchronises per iteration: scalars live in a
-// device state block, every kernel checks the per-component phase and turns
-// into a no-op once its component has finished, and the host polls the state
-// every `check_every` iterations while the next chunk is already queued.
+// strict <).
 //
-// Data layout: vectors are component-major [nrhs][C] so one SpMV streams the
-// matrix once for all components; matrix values are repacked once per solve
-// into slice-of-64 ELL order (one slice = one wavefront, coalesced loads).
+// Synchronisation design for the 8-XCD MI355X (no intra-kernel fences):
+//  * a kernel that needs a global sum writes one partial per workgroup; the
+//    NEXT kernel reduces those partials in every workgroup (same fixed
+//    order everywhere, so every workgroup derives the same scalars);
+//  * solver state (scalars, phase, counters) is double-buffered by launch
+//    sequence number s: launch s reads state[(s-1)&1] and its workgroup 0
+//    writes state[s&1], so no workgroup reads what another one writes;
+//  * a component that has finished turns every later kernel into a no-op,
+//    and the host polls the state every `check_every` iterations while the
+//    next chunk is already queued.
+// Vectors are component-major [nrhs][C] so one SpMV streams the matrix once
+// for all components; values are repacked once per solve into slice-of-64
+// ELL order (one slice = one wavefront, coalesced column/value loads).
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
-#include <vector>
 
 #include "gf_common.h"
 
 namespace gf {
 namespace {
 
-enum Phase : int { PH_ACTIVE = 0, PH_RESTART = 1, PH_FIN_S = 2, PH_DONE = 3 };
+enum Phase : int { PH_ACTIVE = 0, PH_RESTART = 1, PH_DONE = 3 };
 
 struct KState {
     double thresh, norm0, final_res;
     double rz_old, alpha, beta;
     double rho_old, omega, rhat2;
-    double scale;       // mixed: norm of the FP64 residual the inner solve sees
-    double inner_thresh;
+    double scale, inner_thresh;
+    double s1, coarse;  // mixed: 1^T A 1 and the pending constant-mode step
     int iter, phase, converged, restarts;
-    int total_iter, outer_phase, outer_converged, sweeps;
-};
-
-struct Ops {
-    // matrix
-    const int32_t *sell_ptr, *sell_col;
-    int C;
-    // control
-    int nrhs, norm_inf, max_iter, max_restart, precond;
-    double atol, rtol;
+    int total_iter, outer_phase, sweep_start, total_before;
 };
 
 constexpr int kMaxRhs = 3;
-constexpr int kNQ = 4;  // partial quantities per rhs per kernel
+constexpr int kNQ = 3;        // partial quantities per rhs
+constexpr int RB = 1024;      // threads per workgroup in solver kernels
+constexpr int NBMAX = 1024;   // capacity of the per-workgroup partials
+
+// Workgroups used by the solver kernels: 2 per CU on the 256-CU MI355X by
+// default (GRIDFOAM_KRYLOV_WG overrides it for tuning).
+int solver_workgroups(int C) {
+    static int cap = [] {
+        const char *e = getenv("GRIDFOAM_KRYLOV_WG");
+        int v = e ? atoi(e) : 512;
+        return v < 1 ? 1 : (v > NBMAX ? NBMAX : v);
+    }();
+    const int need = (C + RB - 1) / RB;
+    return need < cap ? need : cap;
+}
+
+struct Ops {
+    const int32_t *sell_ptr, *sell_col;
+    int C;
+    int nb;  // workgroups used by vector kernels (<= NBMAX)
+    int norm_inf, max_iter, max_restart;
+    double atol, rtol, inner_rtol;
+    KState *st;        // [2][kMaxRhs]
+    double *partials;  // [2][kMaxRhs][kNQ][NBMAX]
+};
 
 __device__ __forceinline__ double nrm_part(double v, bool inf) { return inf ? fabs(v) : v * v; }
 __device__ __forceinline__ double nrm_fin(double v, bool inf) { return inf ? v : sqrt(v); }
 
-template <typename VT, int NR>
-__device__ __forceinline__ void sell_row(const int32_t *__restrict__ sell_ptr,
-                                         const int32_t *__restrict__ sell_col,
-                                         const VT *__restrict__ val, VT dg, int i,
-                                         const VT *__restrict__ x, int C, VT (&acc)[NR]) {
+// Row i of the slice-of-64 ELL operator.  The slice width W is uniform in a
+// wavefront, so entries are processed in batches of 8 with wave-uniform
+// guards: all column and value loads of a batch are issued before any
+// gather, which turns W dependent load chains into ceil(W/8).
+constexpr int kBatch = 8;
+
+template <typename VT, typename AT, int NR>
+__device__ __forceinline__ void sell_row_batched(const int32_t *__restrict__ sell_ptr,
+                                                 const int32_t *__restrict__ sell_col, const VT *__restrict__ val,
+                                                 AT dg, int i, const VT *__restrict__ x, int C, AT (&acc)[NR]) {
     const int s = i >> 6, lane = i & 63;
     const int base = sell_ptr[s];
     const int W = (sell_ptr[s + 1] - base) >> 6;
 #pragma unroll
-    for (int r = 0; r < NR; ++r) acc[r] = dg * x[(size_t)r * C + i];
-    for (int k = 0; k < W; ++k) {
-        const int e = base + k * 64 + lane;
-        const int col = __builtin_nontemporal_load(sell_col + e);
-        const VT v = __builtin_nontemporal_load(val + e);
+    for (int r = 0; r < NR; ++r) acc[r] = dg * (AT)x[(size_t)r * C + i];
+    for (int kb = 0; kb < W; kb += kBatch) {
+        int col[kBatch];
+        VT v[kBatch];
 #pragma unroll
-        for (int r = 0; r < NR; ++r) acc[r] += v * x[(size_t)r * C + col];
+        for (int u = 0; u < kBatch; ++u) {
+            col[u] = i;
+            v[u] = (VT)0;
+            if (kb + u < W) {
+                const int e = base + (kb + u) * 64 + lane;
+                col[u] = __builtin_nontemporal_load(sell_col + e);
+                v[u] = __builtin_nontemporal_load(val + e);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            if (kb + u < W) {
+#pragma unroll
+                for (int r = 0; r < NR; ++r) acc[r] += (AT)v[u] * (AT)x[(size_t)r * C + col[u]];
+            }
+        }
     }
 }
 
-// Reduce kNQ partial quantities per rhs and hand them to thread 0 of the
-// last block.  partials layout: [NR][kNQ][gridDim.x].
+template <typename VT, int NR>
+__device__ __forceinline__ void sell_row(const int32_t *__restrict__ sell_ptr, const int32_t *__restrict__ sell_col,
+                                         const VT *__restrict__ val, VT dg, int i, const VT *__restrict__ x, int C,
+                                         VT (&acc)[NR]) {
+    sell_row_batched<VT, VT, NR>(sell_ptr, sell_col, val, dg, i, x, C, acc);
+}
+
+// Contiguous row range of workgroup b (chunks of RB rows).
+__device__ __forceinline__ void row_range(int C, int nb, int b, int &lo, int &hi) {
+    const int per = ((C + nb - 1) / nb + 63) & ~63;  // keep slices inside one workgroup
+    lo = b * per;
+    hi = min(C, lo + per);
+}
+
+// ---- state plumbing -----------------------------------------------------
 template <int NR>
-__device__ __forceinline__ bool finish_partials(double (&v)[NR][kNQ], const bool (&is_max)[kNQ],
-                                                double *partials, unsigned int *counter,
-                                                double (&tot)[NR][kNQ]) {
-    double flat[NR * kNQ];
-    bool fm[NR * kNQ];
+struct Ctx {
+    KState *sh;  // shared copy
+};
+
+template <int NR>
+__device__ __forceinline__ void load_state(const Ops &o, int s, KState *sh) {
+    if (threadIdx.x < NR) sh[threadIdx.x] = o.st[((s - 1) & 1) * kMaxRhs + threadIdx.x];
+    __syncthreads();
+}
+template <int NR>
+__device__ __forceinline__ void store_state(const Ops &o, int s, const KState *sh) {
+    __syncthreads();
+    if (blockIdx.x == 0 && threadIdx.x < NR) o.st[(s & 1) * kMaxRhs + threadIdx.x] = sh[threadIdx.x];
+}
+
+// Reduce the previous launch's partials: tot[r][q] in every thread.
+template <int NR, int NQ>
+__device__ __forceinline__ void reduce_prev(const Ops &o, int s, int nb_prev, const bool (&is_max)[NQ],
+                                            double (&tot)[NR][NQ]) {
+    __shared__ double lds[NR * NQ][RB / kWave];
+    __shared__ double res[NR * NQ];
+    const double *P = o.partials + (size_t)((s - 1) & 1) * kMaxRhs * kNQ * NBMAX;
+    const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
 #pragma unroll
     for (int r = 0; r < NR; ++r)
 #pragma unroll
-        for (int q = 0; q < kNQ; ++q) {
-            flat[r * kNQ + q] = v[r][q];
-            fm[r * kNQ + q] = is_max[q];
+        for (int q = 0; q < NQ; ++q) {
+            double v = threadIdx.x < nb_prev ? P[((size_t)r * kNQ + q) * NBMAX + threadIdx.x] : 0.0;
+            v = is_max[q] ? wave_max(v) : wave_sum(v);
+            if (lane == 0) lds[r * NQ + q][wid] = v;
         }
-    block_reduce<NR * kNQ, kBlock>(flat, fm);
-    const int nb = gridDim.x;
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int r = 0; r < NR; ++r)
-#pragma unroll
-            for (int q = 0; q < kNQ; ++q) partials[((size_t)r * kNQ + q) * nb + blockIdx.x] = flat[r * kNQ + q];
+    __syncthreads();
+    if (threadIdx.x < NR * NQ) {
+        const int q = threadIdx.x % NQ;
+        double a = lds[threadIdx.x][0];
+        for (int k = 1; k < RB / kWave; ++k) a = is_max[q] ? fmax(a, lds[threadIdx.x][k]) : a + lds[threadIdx.x][k];
+        res[threadIdx.x] = a;
     }
-    if (!last_block_done(counter)) return false;
+    __syncthreads();
 #pragma unroll
     for (int r = 0; r < NR; ++r)
 #pragma unroll
-        for (int q = 0; q < kNQ; ++q) {
-            double a = is_max[q] ? 0.0 : 0.0;
-            for (int b = threadIdx.x; b < nb; b += kBlock) {
-                double pv = partials[((size_t)r * kNQ + q) * nb + b];
-                a = is_max[q] ? (pv > a ? pv : a) : a + pv;
-            }
-            flat[r * kNQ + q] = a;
+        for (int q = 0; q < NQ; ++q) tot[r][q] = res[r * NQ + q];
+}
+
+// Write this workgroup's partials for the next launch.
+template <int NR, int NQ>
+__device__ __forceinline__ void write_partials(const Ops &o, int s, double (&v)[NR][NQ], const bool (&is_max)[NQ]) {
+    __shared__ double lds[NR * NQ][RB / kWave];
+    double *P = o.partials + (size_t)(s & 1) * kMaxRhs * kNQ * NBMAX;
+    const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            double x = is_max[q] ? wave_max(v[r][q]) : wave_sum(v[r][q]);
+            if (lane == 0) lds[r * NQ + q][wid] = x;
         }
-    block_reduce<NR * kNQ, kBlock>(flat, fm);
+    __syncthreads();
+    if (threadIdx.x < NR * NQ) {
+        const int r = threadIdx.x / NQ, q = threadIdx.x % NQ;
+        double a = lds[threadIdx.x][0];
+        for (int k = 1; k < RB / kWave; ++k) a = is_max[q] ? fmax(a, lds[threadIdx.x][k]) : a + lds[threadIdx.x][k];
+        P[((size_t)r * kNQ + q) * NBMAX + blockIdx.x] = a;
+    }
+}
+
+template <int NR>
+__device__ __forceinline__ bool any_phase(const KState *sh, int ph) {
+    bool a = false;
 #pragma unroll
-    for (int r = 0; r < NR; ++r)
-#pragma unroll
-        for (int q = 0; q < kNQ; ++q) tot[r][q] = flat[r * kNQ + q];
-    return true;
+    for (int r = 0; r < NR; ++r) a |= sh[r].phase == ph;
+    return a;
 }
 
 // ---------------------------------------------------------------- packing
@@ -139,7 +228,6 @@ __global__ void k_diag_prep(int C, const double *__restrict__ diag, int precond,
     inv_d[i] = (VT)inv;
 }
 
-// [C][nrhs] (row-major cell field) <-> [nrhs][C]
 __global__ void k_to_soa(int C, int nrhs, const double *__restrict__ in, double *__restrict__ out) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= C) return;
@@ -151,519 +239,715 @@ __global__ void k_from_soa(int C, int nrhs, const double *__restrict__ in, doubl
     for (int r = 0; r < nrhs; ++r) out[(size_t)i * nrhs + r] = in[(size_t)r * C + i];
 }
 
-// ---------------------------------------------------------------- CG (FP64 plain path)
-// r = b - A x ; z = M^-1 r ; p = z ; rz = r.z ; |r0|
-template <int NR>
-__global__ __launch_bounds__(kBlock) void k_cg_init(Ops o, const double *val, const double *dg,
-                                                    const double *inv_d, const double *b,
-                                                    const double *x, double *r_, double *z_,
-                                                    double *p_, KState *st, double *partials,
-                                                    unsigned int *counter) {
-    const int C = o.C;
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    double v[NR][kNQ] = {};
-    if (i < C) {
-        double ax[NR];
-        sell_row<double, NR>(o.sell_ptr, o.sell_col, val, dg[i], i, x, C, ax);
-        const double id = inv_d[i];
+__global__ void k_zero_state(KState *st, int n) {
+    int i = threadIdx.x;
+    if (i < n) st[i] = KState{};
+}
+
+// ---------------------------------------------------------------- PCG
+// r = b - A x ; z = M^-1 r ; p = z ; partials |r|, r.z   (cg.py:88-118)
+template <typename VT, int NR>
+__global__ __launch_bounds__(RB) void k_cg_init(Ops o, int s, const VT *val, const VT *dg, const VT *inv_d,
+                                                const VT *b, const VT *x, VT *r_, VT *z_, VT *p_) {
+    __shared__ KState sh[NR];
+    load_state<NR>(o, s, sh);
+    int lo, hi;
+    row_range(o.C, o.nb, blockIdx.x, lo, hi);
+    double v[NR][2] = {};
+    for (int i = lo + threadIdx.x; i < hi; i += RB) {
+        VT ax[NR];
+        sell_row<VT, NR>(o.sell_ptr, o.sell_col, val, dg[i], i, x, o.C, ax);
+        const VT id = inv_d[i];
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
-            const size_t k = (size_t)r * C + i;
-            double rr = b[k] - ax[r];
-            double zz = id * rr;
+            const size_t k = (size_t)r * o.C + i;
+            VT rr = b[k] - ax[r];
+            VT zz = id * rr;
             r_[k] = rr;
             z_[k] = zz;
             p_[k] = zz;
-            v[r][0] = nrm_part(rr, o.norm_inf);
-            v[r][1] = rr * zz;
+            v[r][0] = o.norm_inf ? fmax(v[r][0], fabs((double)rr)) : v[r][0] + (double)rr * (double)rr;
+            v[r][1] += (double)rr * (double)zz;
         }
     }
-    const bool mx[kNQ] = {(bool)o.norm_inf, false, false, false};
-    double tot[NR][kNQ];
-    if (!finish_partials<NR>(v, mx, partials, counter, tot)) return;
+    const bool mx[2] = {(bool)o.norm_inf, false};
+    write_partials<NR, 2>(o, s, v, mx);
+    store_state<NR>(o, s, sh);
+}
+
+// Decide on |r0| (one workgroup): thresh, zero-iteration exit, rz_old.
+template <int NR>
+__global__ __launch_bounds__(RB) void k_cg_init_fin(Ops o, int s, int nb_prev) {
+    __shared__ KState sh[NR];
+    load_state<NR>(o, s, sh);
+    const bool mx[2] = {(bool)o.norm_inf, false};
+    double tot[NR][2];
+    reduce_prev<NR, 2>(o, s, nb_prev, mx, tot);
     if (threadIdx.x == 0) {
         for (int r = 0; r < NR; ++r) {
-            KState &s = st[r];
-            double n0 = nrm_fin(tot[r][0], o.norm_inf);
-            s.norm0 = n0;
-            s.final_res = n0;
-            s.thresh = fmax(o.atol, o.rtol * n0);
-            s.iter = 0;
-            s.restarts = 0;
-            s.rz_old = tot[r][1];
-            s.converged = n0 < s.thresh;
-            s.phase = s.converged ? PH_DONE : PH_ACTIVE;
+            KState &S = sh[r];
+            const double n0 = nrm_fin(tot[r][0], o.norm_inf);
+            S.norm0 = n0;
+            S.final_res = n0;
+            S.thresh = fmax(o.atol, o.rtol * n0);
+            S.iter = 0;
+            S.restarts = 0;
+            S.rz_old = tot[r][1];
+            S.converged = n0 < S.thresh;
+            S.phase = S.converged ? PH_DONE : PH_ACTIVE;
+            S.outer_phase = PH_DONE;
         }
     }
+    store_state<NR>(o, s, sh);
 }
 
-// Ap = A p ; alpha = rz_old / (p . Ap)
+// Ap = A p ; partial p.Ap
 template <typename VT, int NR>
-__global__ __launch_bounds__(kBlock) void k_cg_spmv(Ops o, const VT *val, const VT *dg, const VT *p_,
-                                                    VT *ap_, KState *st, double *partials,
-                                                    unsigned int *counter) {
-    bool any = false;
+__global__ __launch_bounds__(RB) void k_cg_spmv(Ops o, int s, const VT *val, const VT *dg, const VT *p_, VT *ap_) {
+    __shared__ KState sh[NR];
+    load_state<NR>(o, s, sh);
+    if (any_phase<NR>(sh, PH_ACTIVE)) {
+        int lo, hi;
+        row_range(o.C, o.nb, blockIdx.x, lo, hi);
+        double v[NR][1] = {};
+        for (int i = lo + threadIdx.x; i < hi; i += RB) {
+            VT ap[NR];
+            sell_row<VT, NR>(o.sell_ptr, o.sell_col, val, dg[i], i, p_, o.C, ap);
 #pragma unroll
-    for (int r = 0; r < NR; ++r) any |= st[r].phase == PH_ACTIVE;
-    if (!any) return;
-    const int C = o.C;
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    double v[NR][kNQ] = {};
-    if (i < C) {
-        VT ap[NR];
-        sell_row<VT, NR>(o.sell_ptr, o.sell_col, val, dg[i], i, p_, C, ap);
-#pragma unroll
-        for (int r = 0; r < NR; ++r) {
-            const size_t k = (size_t)r * C + i;
-            ap_[k] = ap[r];
-            v[r][0] = (double)p_[k] * (double)ap[r];
+            for (int r = 0; r < NR; ++r) {
+                const size_t k = (size_t)r * o.C + i;
+                ap_[k] = ap[r];
+                v[r][0] += (double)p_[k] * (double)ap[r];
+            }
         }
+        const bool mx[1] = {false};
+        write_partials<NR, 1>(o, s, v, mx);
     }
-    const bool mx[kNQ] = {false, false, false, false};
-    double tot[NR][kNQ];
-    if (!finish_partials<NR>(v, mx, partials, counter, tot)) return;
+    store_state<NR>(o, s, sh);
+}
+
+// alpha = rz_old / p.Ap ; x += alpha p ; r -= alpha Ap ; z = M^-1 r ; partials |r|, r.z
+template <typename VT, int NR>
+__global__ __launch_bounds__(RB) void k_cg_update(Ops o, int s, const VT *inv_d, VT *x_, VT *r_, VT *z_,
+                                                  const VT *p_, const VT *ap_) {
+    __shared__ KState sh[NR];
+    load_state<NR>(o, s, sh);
+    if (!any_phase<NR>(sh, PH_ACTIVE)) {
+        store_state<NR>(o, s, sh);
+        return;
+    }
+    const bool mx1[1] = {false};
+    double pap[NR][1];
+    reduce_prev<NR, 1>(o, s, o.nb, mx1, pap);
     if (threadIdx.x == 0)
         for (int r = 0; r < NR; ++r)
-            if (st[r].phase == PH_ACTIVE) st[r].alpha = st[r].rz_old / tot[r][0];
-}
-
-// x += alpha p ; r -= alpha Ap ; |r| check ; z = M^-1 r ; beta
-template <typename VT, int NR>
-__global__ __launch_bounds__(kBlock) void k_cg_update(Ops o, const VT *inv_d, VT *x_, VT *r_, VT *z_,
-                                                      const VT *p_, const VT *ap_, KState *st,
-                                                      double *partials, unsigned int *counter,
-                                                      int inner) {
+            if (sh[r].phase == PH_ACTIVE) sh[r].alpha = sh[r].rz_old / pap[r][0];
+    __syncthreads();
+    int lo, hi;
+    row_range(o.C, o.nb, blockIdx.x, lo, hi);
+    double v[NR][2] = {};
     bool act[NR];
-    bool any = false;
+    VT a[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
-        act[r] = st[r].phase == PH_ACTIVE;
-        any |= act[r];
+        act[r] = sh[r].phase == PH_ACTIVE;
+        a[r] = (VT)sh[r].alpha;
     }
-    if (!any) return;
-    const int C = o.C;
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    double v[NR][kNQ] = {};
-    if (i < C) {
+    for (int i = lo + threadIdx.x; i < hi; i += RB) {
         const VT id = inv_d[i];
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
             if (!act[r]) continue;
-            const size_t k = (size_t)r * C + i;
-            const VT a = (VT)st[r].alpha;
-            VT xx = x_[k] + a * p_[k];
-            VT rr = r_[k] - a * ap_[k];
+            const size_t k = (size_t)r * o.C + i;
+            VT xx = x_[k] + a[r] * p_[k];
+            VT rr = r_[k] - a[r] * ap_[k];
             VT zz = id * rr;
             x_[k] = xx;
             r_[k] = rr;
             z_[k] = zz;
-            v[r][0] = nrm_part((double)rr, o.norm_inf);
-            v[r][1] = (double)rr * (double)zz;
+            v[r][0] = o.norm_inf ? fmax(v[r][0], fabs((double)rr)) : v[r][0] + (double)rr * (double)rr;
+            v[r][1] += (double)rr * (double)zz;
         }
     }
-    const bool mx[kNQ] = {(bool)o.norm_inf, false, false, false};
-    double tot[NR][kNQ];
-    if (!finish_partials<NR>(v, mx, partials, counter, tot)) return;
-    if (threadIdx.x == 0) {
-        for (int r = 0; r < NR; ++r) {
-            KState &s = st[r];
-            if (s.phase != PH_ACTIVE) continue;
-            s.iter += 1;
-            if (inner) s.total_iter += 1;
-            double nr = nrm_fin(tot[r][0], o.norm_inf);
-            const double th = inner ? s.inner_thresh : s.thresh;
-            if (!inner) s.final_res = nr;
-            if (nr < th) {
-                if (!inner) s.converged = 1;
-                s.phase = PH_DONE;
-            } else if (s.iter >= o.max_iter) {
-                s.phase = PH_DONE;
-            } else {
-                s.beta = tot[r][1] / s.rz_old;
-                s.rz_old = tot[r][1];
-            }
-        }
-    }
+    const bool mx[2] = {(bool)o.norm_inf, false};
+    write_partials<NR, 2>(o, s, v, mx);
+    store_state<NR>(o, s, sh);
 }
 
-// p = z + beta p
+// |r| < thresh ? done : beta = r.z_new / r.z_old ; p = z + beta p
+// `inner` selects the mixed-precision sweep rules (inner threshold, budget).
 template <typename VT, int NR>
-__global__ __launch_bounds__(kBlock) void k_cg_pupdate(int C, const VT *z_, VT *p_, const KState *st) {
+__global__ __launch_bounds__(RB) void k_cg_pupdate(Ops o, int s, int k_iter, int inner, const VT *z_, VT *p_) {
+    __shared__ KState sh[NR];
+    load_state<NR>(o, s, sh);
+    if (!any_phase<NR>(sh, PH_ACTIVE)) {
+        store_state<NR>(o, s, sh);
+        return;
+    }
+    const bool mx[2] = {(bool)o.norm_inf, false};
+    double tot[NR][2];
+    reduce_prev<NR, 2>(o, s, o.nb, mx, tot);
+    if (threadIdx.x == 0) {
+        for (int r = 0; r < NR; ++r) {
+            KState &S = sh[r];
+            if (S.phase != PH_ACTIVE) continue;
+            const double nr = nrm_fin(tot[r][0], o.norm_inf);
+            if (!inner) {
+                S.iter += 1;
+                S.final_res = nr;
+                if (nr < S.thresh) {
+                    S.converged = 1;
+                    S.phase = PH_DONE;
+                    continue;
+                }
+                if (S.iter >= o.max_iter) {
+                    S.phase = PH_DONE;
+                    continue;
+                }
+            } else {
+                const int done_iters = k_iter - S.sweep_start + 1;
+                S.iter = done_iters;
+                S.total_iter = S.total_before + done_iters;
+                if (nr < S.inner_thresh || S.total_iter >= o.max_iter) {
+                    S.phase = PH_DONE;
+                    continue;
+                }
+            }
+            S.beta = tot[r][1] / S.rz_old;
+            S.rz_old = tot[r][1];
+        }
+    }
+    __syncthreads();
     bool act[NR];
-    bool any = false;
+    VT be[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
-        act[r] = st[r].phase == PH_ACTIVE;
-        any |= act[r];
+        act[r] = sh[r].phase == PH_ACTIVE;
+        be[r] = (VT)sh[r].beta;
     }
-    if (!any) return;
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= C) return;
+    int lo, hi;
+    row_range(o.C, o.nb, blockIdx.x, lo, hi);
+    for (int i = lo + threadIdx.x; i < hi; i += RB)
 #pragma unroll
-    for (int r = 0; r < NR; ++r) {
-        if (!act[r]) continue;
-        const size_t k = (size_t)r * C + i;
-        p_[k] = z_[k] + p_[k] * (VT)st[r].beta;
-    }
+        for (int r = 0; r < NR; ++r) {
+            if (!act[r]) continue;
+            const size_t k = (size_t)r * o.C + i;
+            p_[k] = z_[k] + p_[k] * be[r];
+        }
+    store_state<NR>(o, s, sh);
 }
 
 // ---------------------------------------------------------------- mixed precision
-// Outer FP64 residual: r64 = b - A64 x64 ; |r64| ; decide outer convergence.
-// On the first sweep it also fixes thresh from |r0| (base.py:140-162).
-template <int NR>
-__global__ __launch_bounds__(kBlock) void k_mx_residual(Ops o, const double *val, const double *dg,
-                                                        const double *b, const double *x,
-                                                        double *r64, KState *st, double *partials,
-                                                        unsigned int *counter, int first) {
-    bool act[NR];
-    bool any = false;
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-        act[r] = first || (st[r].outer_phase == PH_ACTIVE && st[r].phase == PH_DONE);
-        any |= act[r];
-    }
-    if (!any) return;
-    const int C = o.C;
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    double v[NR][kNQ] = {};
-    if (i < C) {
-        double ax[NR];
-        sell_row<double, NR>(o.sell_ptr, o.sell_col, val, dg[i], i, x, C, ax);
-#pragma unroll
-        for (int r = 0; r < NR; ++r) {
-            if (!act[r]) continue;
-            const size_t k = (size_t)r * C + i;
-            double rr = b[k] - ax[r];
-            r64[k] = rr;
-            v[r][0] = nrm_part(rr, o.norm_inf);
-            v[r][1] = rr * rr;
-        }
-    }
-    const bool mx[kNQ] = {(bool)o.norm_inf, false, false, false};
-    double tot[NR][kNQ];
-    if (!finish_partials<NR>(v, mx, partials, counter, tot)) return;
-    if (threadIdx.x == 0) {
-        for (int r = 0; r < NR; ++r) {
-            if (!act[r]) continue;
-            KState &s = st[r];
-            double nr = nrm_fin(tot[r][0], o.norm_inf);
-            if (first) {
-                s.norm0 = nr;
-                s.thresh = fmax(o.atol, o.rtol * nr);
-                s.total_iter = 0;
-                s.sweeps = 0;
-                s.converged = 0;
-            }
-            s.final_res = nr;
-            if (nr < s.thresh) {
-                s.converged = 1;
-                s.outer_phase = PH_DONE;
-                s.phase = PH_DONE;
-            } else if (s.total_iter >= o.max_iter) {
-                s.outer_phase = PH_DONE;
-                s.phase = PH_DONE;
-            } else {
-                s.outer_phase = PH_ACTIVE;
-                s.scale = sqrt(tot[r][1]);  // L2 scaling keeps FP32 in range
-                s.phase = PH_RESTART;       // inner init pending
-            }
-        }
-    }
-}
+// FP32-storage PCG with FP64 residual replacement ("reliable updates").
+// Off-diagonal values and every CG vector are stored in FP32; the diagonal
+// is stored in FP64 and rebuilt so that each row of the FP32 operator has
+// the row sum of the FP64 operator.  FP32 x FP32 products are exact in FP64
+// and are accumulated in FP64, so the near-null (constant) mode of a
+// reference-pinned Neumann pressure matrix does not suffer cancellation.
+// When the recursive residual has dropped by `reliable` since the last
+// refinement (or below the threshold, or the budget ran out) the FP32
+// correction is added to the FP64 solution and the FP64 residual b - A x
+// replaces the recursive one; convergence is decided on that true residual.
+constexpr double kReliable = 0.1;
 
-// Inner FP32 PCG init from the scaled FP64 residual (x32 = 0).
-template <int NR>
-__global__ __launch_bounds__(kBlock) void k_mx_inner_init(Ops o, const float *inv_d, const double *r64,
-                                                          float *x_, float *r_, float *z_, float *p_,
-                                                          KState *st, double *partials,
-                                                          unsigned int *counter, double inner_rtol) {
-    bool act[NR];
-    bool any = false;
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-        act[r] = st[r].phase == PH_RESTART;
-        any |= act[r];
-    }
-    if (!any) return;
-    const int C = o.C;
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    double v[NR][kNQ] = {};
-    if (i < C) {
-        const float id = inv_d[i];
-#pragma unroll
-        for (int r = 0; r < NR; ++r) {
-            if (!act[r]) continue;
-            const size_t k = (size_t)r * C + i;
-            const double inv_s = 1.0 / st[r].scale;
-            float rr = (float)(r64[k] * inv_s);
-            float zz = id * rr;
-            x_[k] = 0.0f;
-            r_[k] = rr;
-            z_[k] = zz;
-            p_[k] = zz;
-            v[r][0] = (double)rr * (double)rr;
-            v[r][1] = (double)rr * (double)zz;
-        }
-    }
-    const bool mx[kNQ] = {false, false, false, false};
-    double tot[NR][kNQ];
-    if (!finish_partials<NR>(v, mx, partials, counter, tot)) return;
-    if (threadIdx.x == 0) {
-        for (int r = 0; r < NR; ++r) {
-            if (!act[r]) continue;
-            KState &s = st[r];
-            s.rz_old = tot[r][1];
-            s.inner_thresh = inner_rtol * sqrt(tot[r][0]);
-            s.iter = 0;
-            s.sweeps += 1;
-            s.phase = PH_ACTIVE;
-        }
-    }
-}
-
-// x64 += scale * x32 after an inner sweep finished.
-template <int NR>
-__global__ __launch_bounds__(kBlock) void k_mx_correct(int C, const float *x32, double *x64,
-                                                       const KState *st) {
-    bool act[NR];
-    bool any = false;
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-        act[r] = st[r].outer_phase == PH_ACTIVE && st[r].phase == PH_DONE;
-        any |= act[r];
-    }
-    if (!any) return;
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+// val32 = (float) val64 ; dgc = diag + sum_row(val64 - val32)
+__global__ void k_mx_prep(int C, const int32_t *sell_ptr, const double *val64, const double *diag, float *val32,
+                          double *dgc) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= C) return;
+    const int sl = i >> 6, lane = i & 63;
+    const int base = sell_ptr[sl];
+    const int W = (sell_ptr[sl + 1] - base) >> 6;
+    double corr = 0.0;
+    for (int k = 0; k < W; ++k) {
+        const int e = base + k * 64 + lane;
+        const double v = val64[e];
+        const float v32 = (float)v;
+        val32[e] = v32;
+        corr += v - (double)v32;
+    }
+    dgc[i] = diag[i] + corr;
+}
+
+// FP32 storage, exact products, FP64 accumulation.
+template <int NR>
+__device__ __forceinline__ void sell_row_mx(const int32_t *__restrict__ sell_ptr, const int32_t *__restrict__ sell_col,
+                                            const float *__restrict__ val, double dg, int i,
+                                            const float *__restrict__ x, int C, double (&acc)[NR]) {
+    sell_row_batched<float, double, NR>(sell_ptr, sell_col, val, dg, i, x, C, acc);
+}
+
+// Ap = A32 p ; partial p.Ap
+template <int NR>
+__global__ __launch_bounds__(RB) void k_mx_spmv(Ops o, int s, const float *val, const double *dgc, const float *p_,
+                                                float *ap_) {
+    __shared__ KState sh[NR];
+    load_state<NR>(o, s, sh);
+    if (any_phase<NR>(sh, PH_ACTIVE)) {
+        int lo, hi;
+        row_range(o.C, o.nb, blockIdx.x, lo, hi);
+        double v[NR][1] = {};
+        for (int i = lo + threadIdx.x; i < hi; i += RB) {
+            double ap[NR];
+            sell_row_mx<NR>(o.sell_ptr, o.sell_col, val, dgc[i], i, p_, o.C, ap);
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                const size_t k = (size_t)r * o.C + i;
+                const float a = (float)ap[r];
+                ap_[k] = a;
+                v[r][0] += (double)p_[k] * (double)a;
+            }
+        }
+        const bool mx[1] = {false};
+        write_partials<NR, 1>(o, s, v, mx);
+    }
+    store_state<NR>(o, s, sh);
+}
+
+// |r| check: refinement pending, or beta and p = z + beta p
+template <int NR>
+__global__ __launch_bounds__(RB) void k_mx_pupdate(Ops o, int s, const float *z_, float *p_) {
+    __shared__ KState sh[NR];
+    load_state<NR>(o, s, sh);
+    if (!any_phase<NR>(sh, PH_ACTIVE)) {
+        store_state<NR>(o, s, sh);
+        return;
+    }
+    const bool mx[2] = {(bool)o.norm_inf, false};
+    double tot[NR][2];
+    reduce_prev<NR, 2>(o, s, o.nb, mx, tot);
+    if (threadIdx.x == 0) {
+        for (int r = 0; r < NR; ++r) {
+            KState &S = sh[r];
+            if (S.phase != PH_ACTIVE) continue;
+            S.iter += 1;
+            const double nr = nrm_fin(tot[r][0], o.norm_inf);
+            if (nr < kReliable * S.scale || nr < S.thresh || S.iter >= o.max_iter) {
+                S.phase = PH_RESTART;  // refinement pending
+                continue;
+            }
+            S.beta = tot[r][1] / S.rz_old;
+            S.rz_old = tot[r][1];
+        }
+    }
+    __syncthreads();
+    bool act[NR];
+    float be[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
-        if (!act[r]) continue;
-        const size_t k = (size_t)r * C + i;
-        x64[k] += st[r].scale * (double)x32[k];
+        act[r] = sh[r].phase == PH_ACTIVE;
+        be[r] = (float)sh[r].beta;
     }
+    int lo, hi;
+    row_range(o.C, o.nb, blockIdx.x, lo, hi);
+    for (int i = lo + threadIdx.x; i < hi; i += RB)
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            if (!act[r]) continue;
+            const size_t k = (size_t)r * o.C + i;
+            p_[k] = z_[k] + p_[k] * be[r];
+        }
+    store_state<NR>(o, s, sh);
+}
+
+// Refinement, part 1: x64 += x32 ; x32 = 0
+template <int NR>
+__global__ __launch_bounds__(RB) void k_mx_accumulate(Ops o, int s, float *x32, double *x64) {
+    __shared__ KState sh[NR];
+    load_state<NR>(o, s, sh);
+    if (any_phase<NR>(sh, PH_RESTART)) {
+        int lo, hi;
+        row_range(o.C, o.nb, blockIdx.x, lo, hi);
+        for (int i = lo + threadIdx.x; i < hi; i += RB)
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                if (sh[r].phase != PH_RESTART) continue;
+                const size_t k = (size_t)r * o.C + i;
+                x64[k] += (double)x32[k];
+                x32[k] = 0.0f;
+            }
+    }
+    store_state<NR>(o, s, sh);
+}
+
+// Refinement, part 2: r64 = b - A64 x64 ; r32 = r64 ; z32 = M^-1 r32 ;
+// partials |r64| (norm type), r32.z32.  `first` also starts the solve.
+template <int NR>
+__global__ __launch_bounds__(RB) void k_mx_residual(Ops o, int s, int first, const double *val, const double *dg,
+                                                    const float *inv_d, const double *b, const double *x64,
+                                                    float *r32, float *z32) {
+    __shared__ KState sh[NR];
+    load_state<NR>(o, s, sh);
+    bool act[NR];
+    bool any = false;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        act[r] = first || sh[r].phase == PH_RESTART;
+        any |= act[r];
+    }
+    if (any) {
+        int lo, hi;
+        row_range(o.C, o.nb, blockIdx.x, lo, hi);
+        double v[NR][2] = {};
+        for (int i = lo + threadIdx.x; i < hi; i += RB) {
+            double ax[NR];
+            sell_row<double, NR>(o.sell_ptr, o.sell_col, val, dg[i], i, x64, o.C, ax);
+            const float id = inv_d[i];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                if (!act[r]) continue;
+                const size_t k = (size_t)r * o.C + i;
+                const double rr = b[k] - ax[r];
+                const float r32v = (float)rr;
+                const float zz = id * r32v;
+                r32[k] = r32v;
+                z32[k] = zz;
+                v[r][0] = o.norm_inf ? fmax(v[r][0], fabs(rr)) : v[r][0] + rr * rr;
+                v[r][1] += (double)r32v * (double)zz;
+            }
+        }
+        const bool mx[2] = {(bool)o.norm_inf, false};
+        write_partials<NR, 2>(o, s, v, mx);
+    }
+    store_state<NR>(o, s, sh);
+}
+
+// Refinement, part 3: decide on the true residual; restart the direction
+// recurrence with p = z + beta p (beta = 0 on the first call).
+template <int NR>
+__global__ __launch_bounds__(RB) void k_mx_refine(Ops o, int s, int first, const float *z32, float *p32, float *x32) {
+    __shared__ KState sh[NR];
+    load_state<NR>(o, s, sh);
+    bool act[NR];
+    bool any = false;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        act[r] = first || sh[r].phase == PH_RESTART;
+        any |= act[r];
+    }
+    if (any) {
+        const bool mx[2] = {(bool)o.norm_inf, false};
+        double tot[NR][2];
+        reduce_prev<NR, 2>(o, s, o.nb, mx, tot);
+        if (threadIdx.x == 0) {
+            for (int r = 0; r < NR; ++r) {
+                if (!act[r]) continue;
+                KState &S = sh[r];
+                const double nr = nrm_fin(tot[r][0], o.norm_inf);
+                if (first) {
+                    S.norm0 = nr;
+                    S.thresh = fmax(o.atol, o.rtol * nr);
+                    S.iter = 0;
+                    S.converged = 0;
+                }
+                S.final_res = nr;
+                if (nr < S.thresh) {
+                    S.converged = 1;
+                    S.phase = PH_DONE;
+                } else if (S.iter >= o.max_iter) {
+                    S.phase = PH_DONE;
+                } else {
+                    S.beta = first ? 0.0 : tot[r][1] / S.rz_old;
+                    S.rz_old = tot[r][1];
+                    S.scale = nr;  // true residual at this refinement
+                    S.phase = PH_ACTIVE;
+                }
+            }
+        }
+        __syncthreads();
+        int lo, hi;
+        row_range(o.C, o.nb, blockIdx.x, lo, hi);
+        for (int i = lo + threadIdx.x; i < hi; i += RB)
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                if (!act[r] || sh[r].phase != PH_ACTIVE) continue;
+                const size_t k = (size_t)r * o.C + i;
+                p32[k] = first ? z32[k] : z32[k] + p32[k] * (float)sh[r].beta;
+                if (first) x32[k] = 0.0f;
+            }
+    }
+    store_state<NR>(o, s, sh);
 }
 
 // ---------------------------------------------------------------- BiCGSTAB (FP64)
-// restart: r_hat = r ; p = r ; rho_old = r.r   (bicgstab.py:248-260)
-template <int NR>
-__global__ __launch_bounds__(kBlock) void k_bi_restart(Ops o, const double *r_, double *rh_, double *p_,
-                                                       KState *st, double *partials,
-                                                       unsigned int *counter, int force) {
-    bool act[NR];
-    bool any = false;
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-        act[r] = force ? st[r].phase == PH_ACTIVE : st[r].phase == PH_RESTART;
-        any |= act[r];
+__device__ __forceinline__ void bi_breakdown(KState &S, const Ops &o) {
+    // bicgstab.py:238-246: restart while the budget lasts, else stop
+    if (S.restarts >= o.max_restart || S.iter >= o.max_iter) {
+        S.phase = PH_DONE;
+    } else {
+        S.restarts += 1;
+        S.phase = PH_RESTART;
     }
-    if (!any) return;
-    const int C = o.C;
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    double v[NR][kNQ] = {};
-    if (i < C) {
+}
+
+// r = b - A x ; partial |r| ; r_hat = r ; p = r ; partial r.r  (bicgstab.py:88-118)
+template <int NR>
+__global__ __launch_bounds__(RB) void k_bi_init(Ops o, int s, const double *val, const double *dg, const double *b,
+                                                const double *x, double *r_, double *rh_, double *p_) {
+    __shared__ KState sh[NR];
+    load_state<NR>(o, s, sh);
+    int lo, hi;
+    row_range(o.C, o.nb, blockIdx.x, lo, hi);
+    double v[NR][2] = {};
+    for (int i = lo + threadIdx.x; i < hi; i += RB) {
+        double ax[NR];
+        sell_row<double, NR>(o.sell_ptr, o.sell_col, val, dg[i], i, x, o.C, ax);
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
-            if (!act[r]) continue;
-            const size_t k = (size_t)r * C + i;
-            double rr = r_[k];
+            const size_t k = (size_t)r * o.C + i;
+            const double rr = b[k] - ax[r];
+            r_[k] = rr;
             rh_[k] = rr;
             p_[k] = rr;
-            v[r][0] = rr * rr;
+            v[r][0] = o.norm_inf ? fmax(v[r][0], fabs(rr)) : v[r][0] + rr * rr;
+            v[r][1] += rr * rr;
         }
     }
-    const bool mx[kNQ] = {false, false, false, false};
-    double tot[NR][kNQ];
-    if (!finish_partials<NR>(v, mx, partials, counter, tot)) return;
-    if (threadIdx.x == 0)
-        for (int r = 0; r < NR; ++r)
-            if (act[r]) {
-                st[r].rho_old = tot[r][0];
-                st[r].phase = PH_ACTIVE;
-            }
+    const bool mx[2] = {(bool)o.norm_inf, false};
+    write_partials<NR, 2>(o, s, v, mx);
+    store_state<NR>(o, s, sh);
 }
 
-// y = M^-1 p ; v = A y ; den = r_hat.v ; breakdown test ; alpha
 template <int NR>
-__global__ __launch_bounds__(kBlock) void k_bi_v(Ops o, const double *val, const double *dg,
-                                                 const double *inv_d, const double *y_,
-                                                 const double *rh_, double *v_, KState *st,
-                                                 double *partials, unsigned int *counter) {
-    bool act[NR];
-    bool any = false;
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-        act[r] = st[r].phase == PH_ACTIVE;
-        any |= act[r];
-    }
-    if (!any) return;
-    const int C = o.C;
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    double v[NR][kNQ] = {};
-    if (i < C) {
-        double av[NR];
-        sell_row<double, NR>(o.sell_ptr, o.sell_col, val, dg[i], i, y_, C, av);
-#pragma unroll
+__global__ __launch_bounds__(RB) void k_bi_init_fin(Ops o, int s, int nb_prev) {
+    __shared__ KState sh[NR];
+    load_state<NR>(o, s, sh);
+    const bool mx[2] = {(bool)o.norm_inf, false};
+    double tot[NR][2];
+    reduce_prev<NR, 2>(o, s, nb_prev, mx, tot);
+    if (threadIdx.x == 0)
         for (int r = 0; r < NR; ++r) {
-            const size_t k = (size_t)r * C + i;
-            v_[k] = av[r];
-            const double h = rh_[k];
-            v[r][0] = h * av[r];
-            v[r][1] = h * h;
-            v[r][2] = av[r] * av[r];
+            KState &S = sh[r];
+            const double n0 = nrm_fin(tot[r][0], o.norm_inf);
+            S.norm0 = n0;
+            S.final_res = n0;
+            S.thresh = fmax(o.atol, o.rtol * n0);
+            S.iter = 0;
+            S.restarts = 0;
+            S.rho_old = tot[r][1];
+            S.converged = n0 < S.thresh;
+            S.phase = S.converged ? PH_DONE : PH_ACTIVE;
         }
+    store_state<NR>(o, s, sh);
+}
+
+// restart (if pending): r_hat = r ; p = r ; partial r.r
+template <int NR>
+__global__ __launch_bounds__(RB) void k_bi_restart(Ops o, int s, const double *r_, double *rh_, double *p_) {
+    __shared__ KState sh[NR];
+    load_state<NR>(o, s, sh);
+    if (any_phase<NR>(sh, PH_RESTART)) {
+        int lo, hi;
+        row_range(o.C, o.nb, blockIdx.x, lo, hi);
+        double v[NR][1] = {};
+        for (int i = lo + threadIdx.x; i < hi; i += RB)
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                if (sh[r].phase != PH_RESTART) continue;
+                const size_t k = (size_t)r * o.C + i;
+                const double rr = r_[k];
+                rh_[k] = rr;
+                p_[k] = rr;
+                v[r][0] += rr * rr;
+            }
+        const bool mx[1] = {false};
+        write_partials<NR, 1>(o, s, v, mx);
     }
-    const bool mx[kNQ] = {false, false, false, false};
-    double tot[NR][kNQ];
-    if (!finish_partials<NR>(v, mx, partials, counter, tot)) return;
-    if (threadIdx.x == 0) {
-        for (int r = 0; r < NR; ++r) {
-            KState &s = st[r];
-            if (s.phase != PH_ACTIVE) continue;
-            s.iter += 1;
-            s.rhat2 = tot[r][1];
-            const double den = tot[r][0];
-            if (fabs(den) <= 1e-6 * sqrt(tot[r][1]) * sqrt(tot[r][2])) {
-                if (s.restarts >= o.max_restart || s.iter >= o.max_iter) {
-                    s.phase = PH_DONE;
-                } else {
-                    s.restarts += 1;
-                    s.phase = PH_RESTART;
+    store_state<NR>(o, s, sh);
+}
+
+// (finish restart: rho_old) ; y = M^-1 p
+template <int NR>
+__global__ __launch_bounds__(RB) void k_bi_y(Ops o, int s, const double *inv_d, const double *p_, double *y_) {
+    __shared__ KState sh[NR];
+    load_state<NR>(o, s, sh);
+    if (any_phase<NR>(sh, PH_RESTART)) {
+        const bool mx[1] = {false};
+        double tot[NR][1];
+        reduce_prev<NR, 1>(o, s, o.nb, mx, tot);
+        if (threadIdx.x == 0)
+            for (int r = 0; r < NR; ++r)
+                if (sh[r].phase == PH_RESTART) {
+                    sh[r].rho_old = tot[r][0];
+                    sh[r].phase = PH_ACTIVE;
                 }
-            } else {
-                s.alpha = s.rho_old / den;
+        __syncthreads();
+    }
+    if (any_phase<NR>(sh, PH_ACTIVE)) {
+        int lo, hi;
+        row_range(o.C, o.nb, blockIdx.x, lo, hi);
+        for (int i = lo + threadIdx.x; i < hi; i += RB) {
+            const double id = inv_d[i];
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                if (sh[r].phase != PH_ACTIVE) continue;
+                const size_t k = (size_t)r * o.C + i;
+                y_[k] = id * p_[k];
             }
         }
     }
+    store_state<NR>(o, s, sh);
 }
 
-// y = M^-1 p  (elementwise; also z = M^-1 s with the same kernel)
+// v = A y ; partials r_hat.v, |r_hat|^2, |v|^2
 template <int NR>
-__global__ __launch_bounds__(kBlock) void k_bi_precond(int C, const double *inv_d, const double *in,
-                                                       double *out, const KState *st, int want_phase) {
-    bool act[NR];
-    bool any = false;
+__global__ __launch_bounds__(RB) void k_bi_v(Ops o, int s, const double *val, const double *dg, const double *y_,
+                                             const double *rh_, double *v_) {
+    __shared__ KState sh[NR];
+    load_state<NR>(o, s, sh);
+    if (any_phase<NR>(sh, PH_ACTIVE)) {
+        int lo, hi;
+        row_range(o.C, o.nb, blockIdx.x, lo, hi);
+        double v[NR][3] = {};
+        for (int i = lo + threadIdx.x; i < hi; i += RB) {
+            double av[NR];
+            sell_row<double, NR>(o.sell_ptr, o.sell_col, val, dg[i], i, y_, o.C, av);
 #pragma unroll
-    for (int r = 0; r < NR; ++r) {
-        act[r] = st[r].phase == want_phase;
-        any |= act[r];
-    }
-    if (!any) return;
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= C) return;
-    const double id = inv_d[i];
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-        if (!act[r]) continue;
-        const size_t k = (size_t)r * C + i;
-        out[k] = id * in[k];
-    }
-}
-
-// s = r - alpha v ; |s| ; converged via s -> PH_FIN_S
-template <int NR>
-__global__ __launch_bounds__(kBlock) void k_bi_s(Ops o, const double *r_, const double *v_, double *s_,
-                                                 KState *st, double *partials,
-                                                 unsigned int *counter) {
-    bool act[NR];
-    bool any = false;
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-        act[r] = st[r].phase == PH_ACTIVE;
-        any |= act[r];
-    }
-    if (!any) return;
-    const int C = o.C;
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    double v[NR][kNQ] = {};
-    if (i < C) {
-#pragma unroll
-        for (int r = 0; r < NR; ++r) {
-            if (!act[r]) continue;
-            const size_t k = (size_t)r * C + i;
-            double ss = r_[k] - st[r].alpha * v_[k];
-            s_[k] = ss;
-            v[r][0] = nrm_part(ss, o.norm_inf);
-        }
-    }
-    const bool mx[kNQ] = {(bool)o.norm_inf, false, false, false};
-    double tot[NR][kNQ];
-    if (!finish_partials<NR>(v, mx, partials, counter, tot)) return;
-    if (threadIdx.x == 0) {
-        for (int r = 0; r < NR; ++r) {
-            if (!act[r]) continue;
-            KState &s = st[r];
-            double ns = nrm_fin(tot[r][0], o.norm_inf);
-            s.final_res = ns;
-            if (ns < s.thresh) {
-                s.converged = 1;
-                s.phase = PH_FIN_S;
+            for (int r = 0; r < NR; ++r) {
+                const size_t k = (size_t)r * o.C + i;
+                v_[k] = av[r];
+                const double h = rh_[k];
+                v[r][0] += h * av[r];
+                v[r][1] += h * h;
+                v[r][2] += av[r] * av[r];
             }
         }
+        const bool mx[3] = {false, false, false};
+        write_partials<NR, 3>(o, s, v, mx);
     }
+    store_state<NR>(o, s, sh);
 }
 
-// converged via s: x += alpha y, then finish
+// iteration count ; breakdown test on r_hat.v ; alpha ; s = r - alpha v ; partial |s|
 template <int NR>
-__global__ __launch_bounds__(kBlock) void k_bi_fin_s(int C, const double *y_, double *x_, KState *st,
-                                                     unsigned int *counter) {
-    bool act[NR];
-    bool any = false;
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-        act[r] = st[r].phase == PH_FIN_S;
-        any |= act[r];
+__global__ __launch_bounds__(RB) void k_bi_s(Ops o, int s, const double *r_, const double *v_, double *s_) {
+    __shared__ KState sh[NR];
+    load_state<NR>(o, s, sh);
+    if (!any_phase<NR>(sh, PH_ACTIVE)) {
+        store_state<NR>(o, s, sh);
+        return;
     }
-    if (!any) return;
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < C) {
-#pragma unroll
-        for (int r = 0; r < NR; ++r) {
-            if (!act[r]) continue;
-            const size_t k = (size_t)r * C + i;
-            x_[k] = x_[k] + st[r].alpha * y_[k];
-        }
-    }
-    if (!last_block_done(counter)) return;
+    const bool mx3[3] = {false, false, false};
+    double tot[NR][3];
+    reduce_prev<NR, 3>(o, s, o.nb, mx3, tot);
     if (threadIdx.x == 0)
-        for (int r = 0; r < NR; ++r)
-            if (act[r]) st[r].phase = PH_DONE;
+        for (int r = 0; r < NR; ++r) {
+            KState &S = sh[r];
+            if (S.phase != PH_ACTIVE) continue;
+            S.iter += 1;
+            S.rhat2 = tot[r][1];
+            const double den = tot[r][0];
+            if (fabs(den) <= 1e-6 * sqrt(tot[r][1]) * sqrt(tot[r][2]))
+                bi_breakdown(S, o);
+            else
+                S.alpha = S.rho_old / den;
+        }
+    __syncthreads();
+    int lo, hi;
+    row_range(o.C, o.nb, blockIdx.x, lo, hi);
+    double v[NR][1] = {};
+    for (int i = lo + threadIdx.x; i < hi; i += RB)
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            if (sh[r].phase != PH_ACTIVE) continue;
+            const size_t k = (size_t)r * o.C + i;
+            const double ss = r_[k] - sh[r].alpha * v_[k];
+            s_[k] = ss;
+            v[r][0] = o.norm_inf ? fmax(v[r][0], fabs(ss)) : v[r][0] + ss * ss;
+        }
+    const bool mx[1] = {(bool)o.norm_inf};
+    write_partials<NR, 1>(o, s, v, mx);
+    store_state<NR>(o, s, sh);
 }
 
-// t = A z ; omega = t.s / t.t with both breakdown tests
+// |s| < thresh ? x += alpha y, done : z = M^-1 s
 template <int NR>
-__global__ __launch_bounds__(kBlock) void k_bi_t(Ops o, const double *val, const double *dg,
-                                                 const double *z_, const double *s_, double *t_,
-                                                 KState *st, double *partials,
-                                                 unsigned int *counter) {
-    bool act[NR];
-    bool any = false;
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-        act[r] = st[r].phase == PH_ACTIVE;
-        any |= act[r];
+__global__ __launch_bounds__(RB) void k_bi_z(Ops o, int s, const double *inv_d, const double *y_, const double *s_,
+                                             double *x_, double *z_) {
+    __shared__ KState sh[NR];
+    __shared__ int conv_s[NR];
+    load_state<NR>(o, s, sh);
+    if (!any_phase<NR>(sh, PH_ACTIVE)) {
+        store_state<NR>(o, s, sh);
+        return;
     }
-    if (!any) return;
-    const int C = o.C;
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    double v[NR][kNQ] = {};
-    if (i < C) {
-        double at[NR];
-        sell_row<double, NR>(o.sell_ptr, o.sell_col, val, dg[i], i, z_, C, at);
+    const bool mx1[1] = {(bool)o.norm_inf};
+    double tot[NR][1];
+    reduce_prev<NR, 1>(o, s, o.nb, mx1, tot);
+    if (threadIdx.x == 0)
+        for (int r = 0; r < NR; ++r) {
+            conv_s[r] = 0;
+            KState &S = sh[r];
+            if (S.phase != PH_ACTIVE) continue;
+            const double ns = nrm_fin(tot[r][0], o.norm_inf);
+            S.final_res = ns;
+            if (ns < S.thresh) {
+                S.converged = 1;
+                S.phase = PH_DONE;
+                conv_s[r] = 1;
+            }
+        }
+    __syncthreads();
+    int lo, hi;
+    row_range(o.C, o.nb, blockIdx.x, lo, hi);
+    for (int i = lo + threadIdx.x; i < hi; i += RB) {
+        const double id = inv_d[i];
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
-            const size_t k = (size_t)r * C + i;
-            t_[k] = at[r];
-            v[r][0] = at[r] * at[r];
-            v[r][1] = at[r] * s_[k];
+            const size_t k = (size_t)r * o.C + i;
+            if (conv_s[r])
+                x_[k] = x_[k] + sh[r].alpha * y_[k];
+            else if (sh[r].phase == PH_ACTIVE)
+                z_[k] = id * s_[k];
         }
     }
-    const bool mx[kNQ] = {false, false, false, false};
-    double tot[NR][kNQ];
-    if (!finish_partials<NR>(v, mx, partials, counter, tot)) return;
-    if (threadIdx.x == 0) {
+    store_state<NR>(o, s, sh);
+}
+
+// t = A z ; partials t.t, t.s
+template <int NR>
+__global__ __launch_bounds__(RB) void k_bi_t(Ops o, int s, const double *val, const double *dg, const double *z_,
+                                             const double *s_, double *t_) {
+    __shared__ KState sh[NR];
+    load_state<NR>(o, s, sh);
+    if (any_phase<NR>(sh, PH_ACTIVE)) {
+        int lo, hi;
+        row_range(o.C, o.nb, blockIdx.x, lo, hi);
+        double v[NR][2] = {};
+        for (int i = lo + threadIdx.x; i < hi; i += RB) {
+            double at[NR];
+            sell_row<double, NR>(o.sell_ptr, o.sell_col, val, dg[i], i, z_, o.C, at);
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                const size_t k = (size_t)r * o.C + i;
+                t_[k] = at[r];
+                v[r][0] += at[r] * at[r];
+                v[r][1] += at[r] * s_[k];
+            }
+        }
+        const bool mx[2] = {false, false};
+        write_partials<NR, 2>(o, s, v, mx);
+    }
+    store_state<NR>(o, s, sh);
+}
+
+// omega (+ breakdowns) ; x += alpha y + omega z ; r = s - omega t ; partials |r|, r_hat.r, |r|^2
+template <int NR>
+__global__ __launch_bounds__(RB) void k_bi_update(Ops o, int s, const double *y_, const double *z_, const double *s_,
+                                                  const double *t_, const double *rh_, double *x_, double *r_) {
+    __shared__ KState sh[NR];
+    load_state<NR>(o, s, sh);
+    if (!any_phase<NR>(sh, PH_ACTIVE)) {
+        store_state<NR>(o, s, sh);
+        return;
+    }
+    const bool mx2[2] = {false, false};
+    double tot[NR][2];
+    reduce_prev<NR, 2>(o, s, o.nb, mx2, tot);
+    if (threadIdx.x == 0)
         for (int r = 0; r < NR; ++r) {
-            KState &s = st[r];
-            if (s.phase != PH_ACTIVE) continue;
+            KState &S = sh[r];
+            if (S.phase != PH_ACTIVE) continue;
             const double tt = tot[r][0];
             bool brk = fabs(tt) <= 1e-6 * sqrt(tt) * sqrt(tt);
             double om = 0.0;
@@ -671,149 +955,83 @@ __global__ __launch_bounds__(kBlock) void k_bi_t(Ops o, const double *val, const
                 om = tot[r][1] / tt;
                 brk = fabs(om) <= 1e-6;
             }
-            if (brk) {
-                if (s.restarts >= o.max_restart || s.iter >= o.max_iter) {
-                    s.phase = PH_DONE;
-                } else {
-                    s.restarts += 1;
-                    s.phase = PH_RESTART;
-                }
-            } else {
-                s.omega = om;
-            }
+            if (brk)
+                bi_breakdown(S, o);
+            else
+                S.omega = om;
         }
-    }
-}
-
-// x += alpha y + omega z ; r = s - omega t ; |r| ; rho_new ; beta
-template <int NR>
-__global__ __launch_bounds__(kBlock) void k_bi_update(Ops o, const double *y_, const double *z_,
-                                                      const double *s_, const double *t_,
-                                                      const double *rh_, double *x_, double *r_,
-                                                      KState *st, double *partials,
-                                                      unsigned int *counter) {
-    bool act[NR];
-    bool any = false;
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-        act[r] = st[r].phase == PH_ACTIVE;
-        any |= act[r];
-    }
-    if (!any) return;
-    const int C = o.C;
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    double v[NR][kNQ] = {};
-    if (i < C) {
+    __syncthreads();
+    int lo, hi;
+    row_range(o.C, o.nb, blockIdx.x, lo, hi);
+    double v[NR][3] = {};
+    for (int i = lo + threadIdx.x; i < hi; i += RB)
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
-            if (!act[r]) continue;
-            const size_t k = (size_t)r * C + i;
-            const double a = st[r].alpha, w = st[r].omega;
-            double xx = x_[k] + a * y_[k] + w * z_[k];
-            double rr = s_[k] - w * t_[k];
+            if (sh[r].phase != PH_ACTIVE) continue;
+            const size_t k = (size_t)r * o.C + i;
+            const double a = sh[r].alpha, w = sh[r].omega;
+            const double xx = x_[k] + a * y_[k] + w * z_[k];
+            const double rr = s_[k] - w * t_[k];
             x_[k] = xx;
             r_[k] = rr;
-            v[r][0] = nrm_part(rr, o.norm_inf);
-            v[r][1] = rh_[k] * rr;
-            v[r][2] = rr * rr;
+            v[r][0] = o.norm_inf ? fmax(v[r][0], fabs(rr)) : v[r][0] + rr * rr;
+            v[r][1] += rh_[k] * rr;
+            v[r][2] += rr * rr;
         }
+    const bool mx[3] = {(bool)o.norm_inf, false, false};
+    write_partials<NR, 3>(o, s, v, mx);
+    store_state<NR>(o, s, sh);
+}
+
+// |r| check ; rho breakdown ; beta ; p = r + beta (p - omega v)
+template <int NR>
+__global__ __launch_bounds__(RB) void k_bi_p(Ops o, int s, const double *r_, const double *v_, double *p_) {
+    __shared__ KState sh[NR];
+    load_state<NR>(o, s, sh);
+    if (!any_phase<NR>(sh, PH_ACTIVE)) {
+        store_state<NR>(o, s, sh);
+        return;
     }
-    const bool mx[kNQ] = {(bool)o.norm_inf, false, false, false};
-    double tot[NR][kNQ];
-    if (!finish_partials<NR>(v, mx, partials, counter, tot)) return;
-    if (threadIdx.x == 0) {
+    const bool mx3[3] = {(bool)o.norm_inf, false, false};
+    double tot[NR][3];
+    reduce_prev<NR, 3>(o, s, o.nb, mx3, tot);
+    if (threadIdx.x == 0)
         for (int r = 0; r < NR; ++r) {
-            if (!act[r]) continue;
-            KState &s = st[r];
-            double nr = nrm_fin(tot[r][0], o.norm_inf);
-            s.final_res = nr;
-            if (nr < s.thresh) {
-                s.converged = 1;
-                s.phase = PH_DONE;
+            KState &S = sh[r];
+            if (S.phase != PH_ACTIVE) continue;
+            const double nr = nrm_fin(tot[r][0], o.norm_inf);
+            S.final_res = nr;
+            if (nr < S.thresh) {
+                S.converged = 1;
+                S.phase = PH_DONE;
                 continue;
             }
             const double rho_new = tot[r][1];
-            if (fabs(rho_new) <= 1e-6 * sqrt(s.rhat2) * sqrt(tot[r][2])) {
-                if (s.restarts >= o.max_restart || s.iter >= o.max_iter) {
-                    s.phase = PH_DONE;
-                } else {
-                    s.restarts += 1;
-                    s.phase = PH_RESTART;
-                }
+            if (fabs(rho_new) <= 1e-6 * sqrt(S.rhat2) * sqrt(tot[r][2])) {
+                bi_breakdown(S, o);
                 continue;
             }
-            s.beta = (rho_new / s.rho_old) * (s.alpha / s.omega);
-            s.rho_old = rho_new;
-            if (s.iter >= o.max_iter) s.phase = PH_DONE;
+            S.beta = (rho_new / S.rho_old) * (S.alpha / S.omega);
+            S.rho_old = rho_new;
+            if (S.iter >= o.max_iter) S.phase = PH_DONE;
         }
-    }
-}
-
-// p = r + beta (p - omega v)
-template <int NR>
-__global__ __launch_bounds__(kBlock) void k_bi_p(int C, const double *r_, const double *v_, double *p_,
-                                                 const KState *st) {
-    bool act[NR];
-    bool any = false;
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-        act[r] = st[r].phase == PH_ACTIVE;
-        any |= act[r];
-    }
-    if (!any) return;
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= C) return;
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-        if (!act[r]) continue;
-        const size_t k = (size_t)r * C + i;
-        p_[k] = r_[k] + st[r].beta * (p_[k] - st[r].omega * v_[k]);
-    }
-}
-
-// bicgstab init: r = b - A x ; |r0| ; thresh
-template <int NR>
-__global__ __launch_bounds__(kBlock) void k_bi_init(Ops o, const double *val, const double *dg,
-                                                    const double *b, const double *x, double *r_,
-                                                    KState *st, double *partials,
-                                                    unsigned int *counter) {
-    const int C = o.C;
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    double v[NR][kNQ] = {};
-    if (i < C) {
-        double ax[NR];
-        sell_row<double, NR>(o.sell_ptr, o.sell_col, val, dg[i], i, x, C, ax);
+    __syncthreads();
+    int lo, hi;
+    row_range(o.C, o.nb, blockIdx.x, lo, hi);
+    for (int i = lo + threadIdx.x; i < hi; i += RB)
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
-            const size_t k = (size_t)r * C + i;
-            double rr = b[k] - ax[r];
-            r_[k] = rr;
-            v[r][0] = nrm_part(rr, o.norm_inf);
+            if (sh[r].phase != PH_ACTIVE) continue;
+            const size_t k = (size_t)r * o.C + i;
+            p_[k] = r_[k] + sh[r].beta * (p_[k] - sh[r].omega * v_[k]);
         }
-    }
-    const bool mx[kNQ] = {(bool)o.norm_inf, false, false, false};
-    double tot[NR][kNQ];
-    if (!finish_partials<NR>(v, mx, partials, counter, tot)) return;
-    if (threadIdx.x == 0) {
-        for (int r = 0; r < NR; ++r) {
-            KState &s = st[r];
-            double n0 = nrm_fin(tot[r][0], o.norm_inf);
-            s.norm0 = n0;
-            s.final_res = n0;
-            s.thresh = fmax(o.atol, o.rtol * n0);
-            s.iter = 0;
-            s.restarts = 0;
-            s.converged = n0 < s.thresh;
-            s.phase = s.converged ? PH_DONE : PH_ACTIVE;
-        }
-    }
+    store_state<NR>(o, s, sh);
 }
 
 // ---------------------------------------------------------------- plain SpMV entry
 template <typename VT>
-__global__ __launch_bounds__(kBlock) void k_sell_spmv(const int32_t *sell_ptr, const int32_t *sell_col,
-                                                      int C, const VT *val, const VT *dg, const VT *x,
-                                                      VT *y) {
+__global__ __launch_bounds__(kBlock) void k_sell_spmv(const int32_t *sell_ptr, const int32_t *sell_col, int C,
+                                                      const VT *val, const VT *dg, const VT *x, VT *y) {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= C) return;
     VT acc[1];
@@ -837,9 +1055,8 @@ struct WS {
 struct Layout {
     double *val64, *dg64, *inv64, *b, *x, *r, *z, *p, *ap, *y, *s, *t, *rh, *v;
     float *val32, *dg32, *inv32, *x32, *r32, *z32, *p32, *ap32;
-    double *partials;
+    double *partials, *dgc;
     KState *st;
-    unsigned int *counters;
     size_t bytes;
 };
 
@@ -847,7 +1064,6 @@ Layout layout(char *base, int64_t C, int64_t nnz, int nrhs, const gf_krylov_cfg 
     WS w{base, 0};
     Layout L{};
     const size_t n = (size_t)C * nrhs;
-    const int nb = grid_for(C);
     L.val64 = w.take<double>(nnz);
     L.dg64 = w.take<double>(C);
     L.inv64 = w.take<double>(C);
@@ -858,6 +1074,7 @@ Layout layout(char *base, int64_t C, int64_t nnz, int nrhs, const gf_krylov_cfg 
         L.val32 = w.take<float>(nnz);
         L.dg32 = w.take<float>(C);
         L.inv32 = w.take<float>(C);
+        L.dgc = w.take<double>(C);
         L.x32 = w.take<float>(n);
         L.r32 = w.take<float>(n);
         L.z32 = w.take<float>(n);
@@ -875,14 +1092,13 @@ Layout layout(char *base, int64_t C, int64_t nnz, int nrhs, const gf_krylov_cfg 
             L.v = L.ap;
         }
     }
-    L.partials = w.take<double>((size_t)kMaxRhs * kNQ * nb);
-    L.st = w.take<KState>(kMaxRhs);
-    L.counters = w.take<unsigned int>(32);
+    L.partials = w.take<double>((size_t)2 * kMaxRhs * kNQ * NBMAX);
+    L.st = w.take<KState>(2 * kMaxRhs);
     L.bytes = w.off + 256;
     return L;
 }
 
-// Pinned staging for the polled device state (one small buffer per thread).
+// Pinned staging for the polled device state.
 KState *host_stage() {
     static thread_local KState *buf = nullptr;
     if (!buf) {
@@ -892,160 +1108,145 @@ KState *host_stage() {
     return buf;
 }
 
-template <int NR>
-int run_cg(const Ops &o, const Layout &L, const gf_krylov_cfg &cfg, hipStream_t s, double *stats) {
-    const int C = o.C, nb = grid_for(C);
-    unsigned int *cnt = L.counters;
-    KState *hst = host_stage();
-    GF_REQUIRE(hst != nullptr, "pinned staging allocation failed");
-    hipEvent_t ev[2];
-    GF_CHECK_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-    GF_CHECK_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-    const int every = cfg.check_every > 0 ? cfg.check_every : 8;
-    int rc = GF_OK;
-    if (!cfg.mixed) {
-        hipLaunchKernelGGL(k_cg_init<NR>, dim3(nb), dim3(kBlock), 0, s, o, L.val64, L.dg64, L.inv64, L.b,
-                           L.x, L.r, L.z, L.p, L.st, L.partials, cnt + 0);
-    } else {
-        hipLaunchKernelGGL(k_mx_residual<NR>, dim3(nb), dim3(kBlock), 0, s, o, L.val64, L.dg64, L.b, L.x,
-                           L.r, L.st, L.partials, cnt + 0, 1);
-        hipLaunchKernelGGL(k_mx_inner_init<NR>, dim3(nb), dim3(kBlock), 0, s, o, L.inv32, L.r, L.x32,
-                           L.r32, L.z32, L.p32, L.st, L.partials, cnt + 1, cfg.inner_rtol);
+struct Poller {
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    KState *h = nullptr;
+    int chunk = 0;
+    int init() {
+        h = host_stage();
+        GF_REQUIRE(h != nullptr, "pinned staging allocation failed");
+        GF_CHECK_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+        GF_CHECK_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+        return GF_OK;
     }
-    GF_CHECK_LAUNCH();
-    int launched = 0, chunk = 0;
-    const int cap = cfg.max_iter + 8;  // each chunk is `every` iterations
-    for (;;) {
-        for (int k = 0; k < every; ++k) {
-            if (!cfg.mixed) {
-                hipLaunchKernelGGL((k_cg_spmv<double, NR>), dim3(nb), dim3(kBlock), 0, s, o, L.val64,
-                                   L.dg64, L.p, L.ap, L.st, L.partials, cnt + 2);
-                hipLaunchKernelGGL((k_cg_update<double, NR>), dim3(nb), dim3(kBlock), 0, s, o, L.inv64,
-                                   L.x, L.r, L.z, L.p, L.ap, L.st, L.partials, cnt + 3, 0);
-                hipLaunchKernelGGL((k_cg_pupdate<double, NR>), dim3(nb), dim3(kBlock), 0, s, C, L.z, L.p,
-                                   L.st);
-            } else {
-                hipLaunchKernelGGL((k_cg_spmv<float, NR>), dim3(nb), dim3(kBlock), 0, s, o, L.val32,
-                                   L.dg32, L.p32, L.ap32, L.st, L.partials, cnt + 2);
-                hipLaunchKernelGGL((k_cg_update<float, NR>), dim3(nb), dim3(kBlock), 0, s, o, L.inv32,
-                                   L.x32, L.r32, L.z32, L.p32, L.ap32, L.st, L.partials, cnt + 3, 1);
-                hipLaunchKernelGGL((k_cg_pupdate<float, NR>), dim3(nb), dim3(kBlock), 0, s, C, L.z32,
-                                   L.p32, L.st);
-            }
-        }
-        if (cfg.mixed) {
-            hipLaunchKernelGGL(k_mx_correct<NR>, dim3(nb), dim3(kBlock), 0, s, C, L.x32, L.x, L.st);
-            hipLaunchKernelGGL(k_mx_residual<NR>, dim3(nb), dim3(kBlock), 0, s, o, L.val64, L.dg64, L.b,
-                               L.x, L.r, L.st, L.partials, cnt + 0, 0);
-            hipLaunchKernelGGL(k_mx_inner_init<NR>, dim3(nb), dim3(kBlock), 0, s, o, L.inv32, L.r,
-                               L.x32, L.r32, L.z32, L.p32, L.st, L.partials, cnt + 1, cfg.inner_rtol);
-        }
-        GF_CHECK_LAUNCH();
-        launched += every;
-        KState *slot = hst + (chunk & 1) * kMaxRhs;
-        GF_CHECK_HIP(hipMemcpyAsync(slot, L.st, sizeof(KState) * NR, hipMemcpyDeviceToHost, s));
-        GF_CHECK_HIP(hipEventRecord(ev[chunk & 1], s));
+    ~Poller() {
+        if (ev[0]) (void)hipEventDestroy(ev[0]);
+        if (ev[1]) (void)hipEventDestroy(ev[1]);
+    }
+    // queue a copy of state[s&1]; returns 1 when the previous chunk's copy says all done
+    template <typename F>
+    int poll(const Layout &L, int s, int nrhs, hipStream_t st, F all_done, bool *done) {
+        KState *slot = h + (chunk & 1) * kMaxRhs;
+        GF_CHECK_HIP(hipMemcpyAsync(slot, L.st + (s & 1) * kMaxRhs, sizeof(KState) * nrhs, hipMemcpyDeviceToHost, st));
+        GF_CHECK_HIP(hipEventRecord(ev[chunk & 1], st));
+        *done = false;
         if (chunk > 0) {
             GF_CHECK_HIP(hipEventSynchronize(ev[(chunk - 1) & 1]));
-            KState *prev = hst + ((chunk - 1) & 1) * kMaxRhs;
-            bool all = true;
-            for (int r = 0; r < NR; ++r)
-                all &= cfg.mixed ? prev[r].outer_phase == PH_DONE : prev[r].phase == PH_DONE;
-            if (all) break;
+            *done = all_done(h + ((chunk - 1) & 1) * kMaxRhs);
         }
         ++chunk;
-        if (!cfg.mixed && launched > cap + 2 * every) {
-            set_error("cg: state machine did not finish within max_iter");
-            rc = GF_EINVAL;
-            break;
-        }
-        if (cfg.mixed && launched > 64 * (cap + every)) {
-            set_error("mixed cg: refinement did not finish");
-            rc = GF_EINVAL;
-            break;
-        }
-        if (chunk > 2 && every < 64) {
-            // grow the chunk once the solve is clearly long
-        }
+        return GF_OK;
     }
-    GF_CHECK_HIP(hipStreamSynchronize(s));
-    GF_CHECK_HIP(hipMemcpy(hst, L.st, sizeof(KState) * NR, hipMemcpyDeviceToHost));
+};
+
+template <int NR>
+int finish(const Layout &L, int s, const gf_krylov_cfg &cfg, hipStream_t st, double *stats) {
+    KState *h = host_stage();
+    GF_CHECK_HIP(hipStreamSynchronize(st));
+    GF_CHECK_HIP(hipMemcpy(h, L.st + (s & 1) * kMaxRhs, sizeof(KState) * NR, hipMemcpyDeviceToHost));
     for (int r = 0; r < NR; ++r) {
-        stats[r * 4 + 0] = hst[r].norm0;
-        stats[r * 4 + 1] = hst[r].final_res;
-        stats[r * 4 + 2] = cfg.mixed ? hst[r].total_iter : hst[r].iter;
-        stats[r * 4 + 3] = hst[r].converged;
+        stats[r * 4 + 0] = h[r].norm0;
+        stats[r * 4 + 1] = h[r].final_res;
+        stats[r * 4 + 2] = h[r].iter;
+        stats[r * 4 + 3] = h[r].converged;
     }
-    hipEventDestroy(ev[0]);
-    hipEventDestroy(ev[1]);
-    return rc;
+    return GF_OK;
 }
 
 template <int NR>
-int run_bicgstab(const Ops &o, const Layout &L, const gf_krylov_cfg &cfg, hipStream_t s, double *stats) {
-    const int C = o.C, nb = grid_for(C);
-    unsigned int *cnt = L.counters;
-    KState *hst = host_stage();
-    GF_REQUIRE(hst != nullptr, "pinned staging allocation failed");
-    hipEvent_t ev[2];
-    GF_CHECK_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
-    GF_CHECK_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
-    const int every = cfg.check_every > 0 ? cfg.check_every : 4;
-    hipLaunchKernelGGL(k_bi_init<NR>, dim3(nb), dim3(kBlock), 0, s, o, L.val64, L.dg64, L.b, L.x, L.r,
-                       L.st, L.partials, cnt + 0);
-    hipLaunchKernelGGL(k_bi_restart<NR>, dim3(nb), dim3(kBlock), 0, s, o, L.r, L.rh, L.p, L.st,
-                       L.partials, cnt + 1, 1);
+int run_cg(Ops o, const Layout &L, const gf_krylov_cfg &cfg, hipStream_t st, double *stats) {
+    const int nb = o.nb;
+    Poller P;
+    if (int rc = P.init()) return rc;
+    const int every = cfg.check_every > 0 ? cfg.check_every : 8;
+    int s = 1;  // launch sequence; state[0] holds the zeroed initial state
+    const dim3 G(nb), B(RB), G1(1);
+    if (!cfg.mixed) {
+        hipLaunchKernelGGL((k_cg_init<double, NR>), G, B, 0, st, o, s++, L.val64, L.dg64, L.inv64, L.b, L.x, L.r, L.z,
+                           L.p);
+        hipLaunchKernelGGL(k_cg_init_fin<NR>, G1, B, 0, st, o, s++, nb);
+    } else {
+        hipLaunchKernelGGL(k_mx_residual<NR>, G, B, 0, st, o, s++, 1, L.val64, L.dg64, L.inv32, L.b, L.x, L.r32,
+                           L.z32);
+        hipLaunchKernelGGL(k_mx_refine<NR>, G, B, 0, st, o, s++, 1, L.z32, L.p32, L.x32);
+    }
     GF_CHECK_LAUNCH();
-    int launched = 0, chunk = 0, rc = GF_OK;
+    int k = 1;
+    const long cap = (long)cfg.max_iter + 4L * every;
     for (;;) {
-        for (int k = 0; k < every; ++k) {
-            hipLaunchKernelGGL(k_bi_restart<NR>, dim3(nb), dim3(kBlock), 0, s, o, L.r, L.rh, L.p, L.st,
-                               L.partials, cnt + 1, 0);
-            hipLaunchKernelGGL(k_bi_precond<NR>, dim3(nb), dim3(kBlock), 0, s, C, L.inv64, L.p, L.y, L.st,
-                               (int)PH_ACTIVE);
-            hipLaunchKernelGGL(k_bi_v<NR>, dim3(nb), dim3(kBlock), 0, s, o, L.val64, L.dg64, L.inv64, L.y,
-                               L.rh, L.v, L.st, L.partials, cnt + 2);
-            hipLaunchKernelGGL(k_bi_s<NR>, dim3(nb), dim3(kBlock), 0, s, o, L.r, L.v, L.s, L.st,
-                               L.partials, cnt + 3);
-            hipLaunchKernelGGL(k_bi_fin_s<NR>, dim3(nb), dim3(kBlock), 0, s, C, L.y, L.x, L.st, cnt + 4);
-            hipLaunchKernelGGL(k_bi_precond<NR>, dim3(nb), dim3(kBlock), 0, s, C, L.inv64, L.s, L.z, L.st,
-                               (int)PH_ACTIVE);
-            hipLaunchKernelGGL(k_bi_t<NR>, dim3(nb), dim3(kBlock), 0, s, o, L.val64, L.dg64, L.z, L.s, L.t,
-                               L.st, L.partials, cnt + 5);
-            hipLaunchKernelGGL(k_bi_update<NR>, dim3(nb), dim3(kBlock), 0, s, o, L.y, L.z, L.s, L.t, L.rh,
-                               L.x, L.r, L.st, L.partials, cnt + 6);
-            hipLaunchKernelGGL(k_bi_p<NR>, dim3(nb), dim3(kBlock), 0, s, C, L.r, L.v, L.p, L.st);
+        for (int j = 0; j < every; ++j, ++k) {
+            if (!cfg.mixed) {
+                hipLaunchKernelGGL((k_cg_spmv<double, NR>), G, B, 0, st, o, s++, L.val64, L.dg64, L.p, L.ap);
+                hipLaunchKernelGGL((k_cg_update<double, NR>), G, B, 0, st, o, s++, L.inv64, L.x, L.r, L.z, L.p, L.ap);
+                hipLaunchKernelGGL((k_cg_pupdate<double, NR>), G, B, 0, st, o, s++, k, 0, L.z, L.p);
+            } else {
+                hipLaunchKernelGGL(k_mx_spmv<NR>, G, B, 0, st, o, s++, L.val32, L.dgc, L.p32, L.ap32);
+                hipLaunchKernelGGL((k_cg_update<float, NR>), G, B, 0, st, o, s++, L.inv32, L.x32, L.r32, L.z32, L.p32,
+                                   L.ap32);
+                hipLaunchKernelGGL(k_mx_pupdate<NR>, G, B, 0, st, o, s++, L.z32, L.p32);
+            }
+        }
+        if (cfg.mixed) {  // FP64 refinement (no-ops unless a component asked for it)
+            hipLaunchKernelGGL(k_mx_accumulate<NR>, G, B, 0, st, o, s++, L.x32, L.x);
+            hipLaunchKernelGGL(k_mx_residual<NR>, G, B, 0, st, o, s++, 0, L.val64, L.dg64, L.inv32, L.b, L.x, L.r32,
+                               L.z32);
+            hipLaunchKernelGGL(k_mx_refine<NR>, G, B, 0, st, o, s++, 0, L.z32, L.p32, L.x32);
         }
         GF_CHECK_LAUNCH();
-        launched += every;
-        KState *slot = hst + (chunk & 1) * kMaxRhs;
-        GF_CHECK_HIP(hipMemcpyAsync(slot, L.st, sizeof(KState) * NR, hipMemcpyDeviceToHost, s));
-        GF_CHECK_HIP(hipEventRecord(ev[chunk & 1], s));
-        if (chunk > 0) {
-            GF_CHECK_HIP(hipEventSynchronize(ev[(chunk - 1) & 1]));
-            KState *prev = hst + ((chunk - 1) & 1) * kMaxRhs;
-            bool all = true;
-            for (int r = 0; r < NR; ++r) all &= prev[r].phase == PH_DONE;
-            if (all) break;
-        }
-        ++chunk;
-        if (launched > cfg.max_iter + 4 * every + 8) {
-            set_error("bicgstab: state machine did not finish within max_iter");
-            rc = GF_EINVAL;
-            break;
+        bool done = false;
+        auto all_done = [&](const KState *h) {
+            bool a = true;
+            for (int r = 0; r < NR; ++r) a &= h[r].phase == PH_DONE;
+            return a;
+        };
+        if (int rc = P.poll(L, s - 1, NR, st, all_done, &done)) return rc;
+        if (done) break;
+        if (k > cap) {
+            set_error("cg: solver state machine did not finish");
+            return GF_EINVAL;
         }
     }
-    GF_CHECK_HIP(hipStreamSynchronize(s));
-    GF_CHECK_HIP(hipMemcpy(hst, L.st, sizeof(KState) * NR, hipMemcpyDeviceToHost));
-    for (int r = 0; r < NR; ++r) {
-        stats[r * 4 + 0] = hst[r].norm0;
-        stats[r * 4 + 1] = hst[r].final_res;
-        stats[r * 4 + 2] = hst[r].iter;
-        stats[r * 4 + 3] = hst[r].converged;
+    return finish<NR>(L, s - 1, cfg, st, stats);
+}
+
+template <int NR>
+int run_bicgstab(Ops o, const Layout &L, const gf_krylov_cfg &cfg, hipStream_t st, double *stats) {
+    const int nb = o.nb;
+    Poller P;
+    if (int rc = P.init()) return rc;
+    const int every = cfg.check_every > 0 ? cfg.check_every : 4;
+    int s = 1;
+    const dim3 G(nb), B(RB), G1(1);
+    hipLaunchKernelGGL(k_bi_init<NR>, G, B, 0, st, o, s++, L.val64, L.dg64, L.b, L.x, L.r, L.rh, L.p);
+    hipLaunchKernelGGL(k_bi_init_fin<NR>, G1, B, 0, st, o, s++, nb);
+    GF_CHECK_LAUNCH();
+    int k = 0;
+    for (;;) {
+        for (int j = 0; j < every; ++j, ++k) {
+            hipLaunchKernelGGL(k_bi_restart<NR>, G, B, 0, st, o, s++, L.r, L.rh, L.p);
+            hipLaunchKernelGGL(k_bi_y<NR>, G, B, 0, st, o, s++, L.inv64, L.p, L.y);
+            hipLaunchKernelGGL(k_bi_v<NR>, G, B, 0, st, o, s++, L.val64, L.dg64, L.y, L.rh, L.v);
+            hipLaunchKernelGGL(k_bi_s<NR>, G, B, 0, st, o, s++, L.r, L.v, L.s);
+            hipLaunchKernelGGL(k_bi_z<NR>, G, B, 0, st, o, s++, L.inv64, L.y, L.s, L.x, L.z);
+            hipLaunchKernelGGL(k_bi_t<NR>, G, B, 0, st, o, s++, L.val64, L.dg64, L.z, L.s, L.t);
+            hipLaunchKernelGGL(k_bi_update<NR>, G, B, 0, st, o, s++, L.y, L.z, L.s, L.t, L.rh, L.x, L.r);
+            hipLaunchKernelGGL(k_bi_p<NR>, G, B, 0, st, o, s++, L.r, L.v, L.p);
+        }
+        GF_CHECK_LAUNCH();
+        bool done = false;
+        auto all_done = [&](const KState *h) {
+            bool a = true;
+            for (int r = 0; r < NR; ++r) a &= h[r].phase == PH_DONE;
+            return a;
+        };
+        if (int rc = P.poll(L, s - 1, NR, st, all_done, &done)) return rc;
+        if (done) break;
+        if (k > cfg.max_iter + 4 * every + 8) {
+            set_error("bicgstab: solver state machine did not finish");
+            return GF_EINVAL;
+        }
     }
-    hipEventDestroy(ev[0]);
-    hipEventDestroy(ev[1]);
-    return rc;
+    return finish<NR>(L, s - 1, cfg, st, stats);
 }
 
 }  // namespace
@@ -1060,30 +1261,29 @@ extern "C" int64_t gf_krylov_workspace_bytes(const gf_mesh *m, int32_t nrhs, con
 }
 
 extern "C" int gf_krylov_solve(const gf_mesh *m, const gf_krylov_cfg *cfg, int32_t nrhs, int32_t transpose,
-                               const double *diag, const double *upper, const double *lower,
-                               const double *b, double *x, void *workspace, double *stats_out,
-                               gf_stream_t stream) {
+                               const double *diag, const double *upper, const double *lower, const double *b,
+                               double *x, void *workspace, double *stats_out, gf_stream_t stream) {
     GF_REQUIRE(m && cfg && diag && upper && lower && b && x && workspace && stats_out, "null argument");
     GF_REQUIRE(nrhs == 1 || nrhs == 3, "nrhs must be 1 or 3");
     GF_REQUIRE(cfg->method == 0 || cfg->method == 1, "unknown Krylov method");
     GF_REQUIRE(!(cfg->method == 1 && cfg->mixed), "mixed precision is implemented for cg only");
     GF_REQUIRE(cfg->max_iter >= 1, "max_iter must be >= 1");
+    GF_REQUIRE(m->n_cells >= 1, "empty grid");
     hipStream_t s = S(stream);
     const int C = m->n_cells;
     Layout L = layout(reinterpret_cast<char *>(workspace), C, m->nnz_pad, nrhs, *cfg);
     const int64_t nnz = m->nnz_pad;
-    hipLaunchKernelGGL(k_sell_pack<double>, dim3(grid_for(nnz)), dim3(kBlock), 0, s, nnz, m->sell_code, upper,
-                       lower, transpose, L.val64);
-    hipLaunchKernelGGL(k_diag_prep<double>, dim3(grid_for(C)), dim3(kBlock), 0, s, C, diag, cfg->precond,
-                       L.dg64, L.inv64);
+    hipLaunchKernelGGL(k_sell_pack<double>, dim3(grid_for(nnz)), dim3(kBlock), 0, s, nnz, m->sell_code, upper, lower,
+                       transpose, L.val64);
+    hipLaunchKernelGGL(k_diag_prep<double>, dim3(grid_for(C)), dim3(kBlock), 0, s, C, diag, cfg->precond, L.dg64,
+                       L.inv64);
     if (cfg->method == 0 && cfg->mixed) {
-        hipLaunchKernelGGL(k_sell_pack<float>, dim3(grid_for(nnz)), dim3(kBlock), 0, s, nnz, m->sell_code,
-                           upper, lower, transpose, L.val32);
-        hipLaunchKernelGGL(k_diag_prep<float>, dim3(grid_for(C)), dim3(kBlock), 0, s, C, diag, cfg->precond,
-                           L.dg32, L.inv32);
+        hipLaunchKernelGGL(k_mx_prep, dim3(grid_for(C)), dim3(kBlock), 0, s, C, m->sell_ptr, L.val64, diag, L.val32,
+                           L.dgc);
+        hipLaunchKernelGGL(k_diag_prep<float>, dim3(grid_for(C)), dim3(kBlock), 0, s, C, diag, cfg->precond, L.dg32,
+                           L.inv32);
     }
-    GF_CHECK_HIP(hipMemsetAsync(L.counters, 0, 32 * sizeof(unsigned int), s));
-    GF_CHECK_HIP(hipMemsetAsync(L.st, 0, sizeof(KState) * kMaxRhs, s));
+    hipLaunchKernelGGL(k_zero_state, dim3(1), dim3(64), 0, s, L.st, 2 * kMaxRhs);
     if (nrhs == 1) {
         GF_CHECK_HIP(hipMemcpyAsync(L.b, b, sizeof(double) * C, hipMemcpyDeviceToDevice, s));
         GF_CHECK_HIP(hipMemcpyAsync(L.x, x, sizeof(double) * C, hipMemcpyDeviceToDevice, s));
@@ -1092,8 +1292,19 @@ extern "C" int gf_krylov_solve(const gf_mesh *m, const gf_krylov_cfg *cfg, int32
         hipLaunchKernelGGL(k_to_soa, dim3(grid_for(C)), dim3(kBlock), 0, s, C, nrhs, x, L.x);
     }
     GF_CHECK_LAUNCH();
-    Ops o{m->sell_ptr, m->sell_col, C, nrhs, cfg->norm, cfg->max_iter, cfg->max_restart, cfg->precond,
-          cfg->atol, cfg->rtol};
+    Ops o{};
+    o.sell_ptr = m->sell_ptr;
+    o.sell_col = m->sell_col;
+    o.C = C;
+    o.nb = solver_workgroups(C);
+    o.norm_inf = cfg->norm;
+    o.max_iter = cfg->max_iter;
+    o.max_restart = cfg->max_restart;
+    o.atol = cfg->atol;
+    o.rtol = cfg->rtol;
+    o.inner_rtol = cfg->inner_rtol;
+    o.st = L.st;
+    o.partials = L.partials;
     int rc;
     if (cfg->method == 0)
         rc = nrhs == 1 ? run_cg<1>(o, L, *cfg, s, stats_out) : run_cg<3>(o, L, *cfg, s, stats_out);
@@ -1109,31 +1320,45 @@ extern "C" int gf_krylov_solve(const gf_mesh *m, const gf_krylov_cfg *cfg, int32
     return GF_OK;
 }
 
-extern "C" int gf_sell_pack(const gf_mesh *m, int32_t transpose, const double *upper, const double *lower,
-                            void *val, int32_t fp32, gf_stream_t stream) {
+extern "C" int gf_sell_pack(const gf_mesh *m, int32_t transpose, const double *upper, const double *lower, void *val,
+                            int32_t fp32, gf_stream_t stream) {
     GF_REQUIRE(m && upper && lower && val, "null argument");
     const int64_t nnz = m->nnz_pad;
     if (fp32)
         hipLaunchKernelGGL(k_sell_pack<float>, dim3(grid_for(nnz)), dim3(kBlock), 0, S(stream), nnz, m->sell_code,
                            upper, lower, transpose, (float *)val);
     else
-        hipLaunchKernelGGL(k_sell_pack<double>, dim3(grid_for(nnz)), dim3(kBlock), 0, S(stream), nnz,
-                           m->sell_code, upper, lower, transpose, (double *)val);
+        hipLaunchKernelGGL(k_sell_pack<double>, dim3(grid_for(nnz)), dim3(kBlock), 0, S(stream), nnz, m->sell_code,
+                           upper, lower, transpose, (double *)val);
     GF_CHECK_LAUNCH();
     return GF_OK;
 }
 
-extern "C" int gf_sell_spmv(const gf_mesh *m, const void *diag, const void *val, const void *x, void *y,
-                            int32_t fp32, gf_stream_t stream) {
-    GF_REQUIRE(m && diag && val && x && y, "null argument");
-    const int C = m->n_cells;
-    if (fp32)
-        hipLaunchKernelGGL(k_sell_spmv<float>, dim3(grid_for(C)), dim3(kBlock), 0, S(stream), m->sell_ptr,
-                           m->sell_col, C, (const float *)val, (const float *)diag, (const float *)x, (float *)y);
+extern "C" int64_t gf_sell_spmv_scratch_bytes(void) {
+    return (int64_t)(sizeof(double) * 2 * kMaxRhs * kNQ * NBMAX + sizeof(KState) * 2 * kMaxRhs + 1024);
+}
+
+// The solver's own SpMV kernel (same code, launch geometry and p.Ap partial
+// as inside PCG), launched standalone so its duration can be timed.
+extern "C" int gf_sell_spmv(const gf_mesh *m, const void *diag, const void *val, const void *x, void *y, int32_t fp32,
+                            void *scratch, gf_stream_t stream) {
+    GF_REQUIRE(m && diag && val && x && y && scratch, "null argument");
+    hipStream_t s = S(stream);
+    WS w{reinterpret_cast<char *>(scratch), 0};
+    Ops o{};
+    o.sell_ptr = m->sell_ptr;
+    o.sell_col = m->sell_col;
+    o.C = m->n_cells;
+    o.nb = solver_workgroups(o.C);
+    o.partials = w.take<double>((size_t)2 * kMaxRhs * kNQ * NBMAX);
+    o.st = w.take<KState>(2 * kMaxRhs);
+    hipLaunchKernelGGL(k_zero_state, dim3(1), dim3(64), 0, s, o.st, 2 * kMaxRhs);  // phase 0: active
+    if (fp32)  // mixed-precision operator: FP32 values and vectors, FP64 diagonal and accumulation
+        hipLaunchKernelGGL(k_mx_spmv<1>, dim3(o.nb), dim3(RB), 0, s, o, 1, (const float *)val, (const double *)diag,
+                           (const float *)x, (float *)y);
     else
-        hipLaunchKernelGGL(k_sell_spmv<double>, dim3(grid_for(C)), dim3(kBlock), 0, S(stream), m->sell_ptr,
-                           m->sell_col, C, (const double *)val, (const double *)diag, (const double *)x,
-                           (double *)y);
+        hipLaunchKernelGGL((k_cg_spmv<double, 1>), dim3(o.nb), dim3(RB), 0, s, o, 1, (const double *)val,
+                           (const double *)diag, (const double *)x, (double *)y);
     GF_CHECK_LAUNCH();
     return GF_OK;
 }

@@ -236,11 +236,13 @@ int gf_krylov_solve(const gf_mesh *m, const gf_krylov_cfg *cfg, int32_t nrhs, in
 /* ldu_grads.py:11-53: gd = -sum lambda x, gu = -sum lambda_O x_N, gl = -sum lambda_N x_O */
 int gf_ldu_grads(const gf_mesh *m, int32_t nc, const double *lambda, const double *x,
                  double *g_diag, double *g_upper, double *g_lower, gf_stream_t stream);
-/* y = A x on the sliced-ELL operator (benchmark entry; FP64 or FP32 values) */
+/* y = A x with the solver's own SpMV kernel on the sliced-ELL operator
+ * (benchmark entry; FP64 or FP32 values; scratch of gf_sell_spmv_scratch_bytes()) */
 int gf_sell_pack(const gf_mesh *m, int32_t transpose, const double *upper, const double *lower,
                  void *val, int32_t fp32, gf_stream_t stream);
+int64_t gf_sell_spmv_scratch_bytes(void);
 int gf_sell_spmv(const gf_mesh *m, const void *diag, const void *val, const void *x, void *y,
-                 int32_t fp32, gf_stream_t stream);
+                 int32_t fp32, void *scratch, gf_stream_t stream);
 
 #ifdef __cplusplus
 }
