@@ -67,8 +67,9 @@ def run_gpu(mixed: bool, steps: int, warmup: int):
 
 
 def spmv_roofline(grid, pmat, fp32: bool, launches: int = 200) -> dict:
-    """Average duration of the PCG SpMV kernel (the solver's own kernel,
-    standalone launch) with HIP events on the current stream."""
+    """Average duration of the PCG direction/SpMV kernel (the solver's own
+    kernel, launched on its own by gf_sell_spmv) with HIP events on the
+    stream it is launched on."""
     import ctypes
 
     import torch
@@ -82,28 +83,33 @@ def spmv_roofline(grid, pmat, fp32: bool, launches: int = 200) -> dict:
     diag = pmat._diag.contiguous()  # FP64 in both operators
     x = torch.randn(grid.num_cells, dtype=dt, device=grid.device)
     y = torch.empty_like(x)
-    scratch = torch.empty(int(nat.lib().gf_sell_spmv_scratch_bytes()), dtype=torch.uint8, device=grid.device)
-    args = (grid.mesh_ref, nat.ptr(diag), nat.ptr(val), nat.ptr(x), nat.ptr(y), ctypes.c_int32(int(fp32)),
-            nat.ptr(scratch), nat.stream())
-    for _ in range(10):
-        nat.call("gf_sell_spmv", *args)
-    s = torch.cuda.current_stream()
+    scratch = torch.empty(int(nat.lib().gf_sell_spmv_scratch_bytes(grid.mesh_ref)), dtype=torch.uint8,
+                          device=grid.device)
+
+    def run(reps: int) -> None:
+        nat.call("gf_sell_spmv", grid.mesh_ref, nat.ptr(diag), nat.ptr(val), nat.ptr(x), nat.ptr(y),
+                 ctypes.c_int32(int(fp32)), ctypes.c_int32(reps), nat.ptr(scratch), nat.stream())
+
+    run(10)
+    s = torch.cuda.current_stream()  # the stream gf_sell_spmv launches on
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(s)
-    for _ in range(launches):
-        nat.call("gf_sell_spmv", *args)
+    run(launches)
     e1.record(s)
     torch.cuda.synchronize()
-    avg_s = e0.elapsed_time(e1) / 1e3 / launches  # includes the tiny state-reset kernel
+    avg_s = e0.elapsed_time(e1) / 1e3 / launches  # + one tiny state set-up kernel per `launches`
     C, F = grid.num_cells, grid.num_internal_faces
-    # algorithmic bytes per SpMV (symmetric LDU, int32 indices):
-    # B = s_d C + s_v F + 8 F + 2 s_x C ; FP64: 24C + 16F ; mixed operator
-    # (FP32 values/vectors, FP64 diagonal): 16C + 12F
+    # Algorithmic bytes of one launch of the PCG direction/SpMV kernel
+    # (k_cg_dir): FP64 diagonal s_d C, off-diagonal values s_v 2F and int32
+    # columns 4 2F, and four vectors of C (z and p_old read, p_new and Ap
+    # written; gathered neighbours counted once).  FP64: 40C + 24F ; mixed
+    # operator (FP32 values/vectors, FP64 diagonal): 24C + 16F.
     sd, sv, sx = (8, 4, 4) if fp32 else (8, 8, 8)
-    B = sd * C + sv * F + 8 * F + 2 * sx * C
+    B = sd * C + (sv + 4) * 2 * F + 4 * sx * C
     achieved = B / avg_s / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _traffic(fp32), "kernel": "k_mx_spmv<1>" if fp32 else "k_cg_spmv<double, 1>",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _traffic(fp32),
+            "kernel": "k_cg_dir<float, 1, true>" if fp32 else "k_cg_dir<double, 1, false>",
             "bytes_per_launch": int(B), "avg_us": round(avg_s * 1e6, 2)}
 
 

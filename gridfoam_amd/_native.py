@@ -94,8 +94,8 @@ _SIGS: dict[str, tuple[Any, list[Any]]] = {
     "gf_krylov_solve": (_i32, [_P(GfMesh), _P(GfKrylovCfg), _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gf_ldu_grads": (_i32, [_P(GfMesh), _i32, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gf_sell_pack": (_i32, [_P(GfMesh), _i32, _vp, _vp, _vp, _i32, _vp]),
-    "gf_sell_spmv_scratch_bytes": (_i64, []),
-    "gf_sell_spmv": (_i32, [_P(GfMesh), _vp, _vp, _vp, _vp, _i32, _vp, _vp]),
+    "gf_sell_spmv_scratch_bytes": (_i64, [_P(GfMesh)]),
+    "gf_sell_spmv": (_i32, [_P(GfMesh), _vp, _vp, _vp, _vp, _i32, _i32, _vp, _vp]),
     "gf_relax_momentum": (_i32, [_i32, _f64, _vp, _vp, _vp, _vp]),
     "gf_predictor_source": (_i32, [_i32, _i32, _vp, _vp, _vp, _vp, _vp]),
     "gf_relax_field": (_i32, [_i64, _f64, _vp, _vp, _vp]),

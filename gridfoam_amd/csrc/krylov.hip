@@ -33,16 +33,17 @@ struct KState {
     double thresh, norm0, final_res;
     double rz_old, alpha, beta;
     double rho_old, omega, rhat2;
-    double scale, inner_thresh;
-    double s1, coarse;  // mixed: 1^T A 1 and the pending constant-mode step
+    double scale;  // mixed precision: true residual at the last refinement
     int iter, phase, converged, restarts;
-    int total_iter, outer_phase, sweep_start, total_before;
+    int fresh;  // PCG: 1 = first direction (p = z), 2 = residual just replaced, 0 = steady
+    int pcur;   // PCG: which of the two p buffers holds the current direction
 };
 
 constexpr int kMaxRhs = 3;
+constexpr double kReliable = 0.1;  // mixed PCG: refine when |r| fell 10x since the last true residual
 constexpr int kNQ = 3;        // partial quantities per rhs
 constexpr int RB = 1024;      // threads per workgroup in solver kernels
-constexpr int NBMAX = 1024;   // capacity of the per-workgroup partials
+constexpr int NBMAX = 1024;   // cap on the vector-kernel workgroup count
 
 // Workgroups used by the solver kernels: 2 per CU on the 256-CU MI355X by
 // default (GRIDFOAM_KRYLOV_WG overrides it for tuning).
@@ -56,15 +57,54 @@ int solver_workgroups(int C) {
     return need < cap ? need : cap;
 }
 
+constexpr int SB = 256;  // threads per workgroup in SpMV kernels (one row per thread)
+
 struct Ops {
     const int32_t *sell_ptr, *sell_col;
     int C;
-    int nb;  // workgroups used by vector kernels (<= NBMAX)
+    int nb;       // BiCGSTAB vector kernels: workgroups (<= NBMAX)
+    int nbs;      // BiCGSTAB SpMV kernels: workgroups (ceil(C / SB))
+    int gs;       // PCG SpMV-type kernels: workgroups (ceil(C / SB) rounded up to a multiple of 8)
+    int gu;       // PCG vector kernels: workgroups (a multiple of 8)
+    int pstride;  // partials per (rhs, quantity): the largest workgroup count
     int norm_inf, max_iter, max_restart;
-    double atol, rtol, inner_rtol;
+    double atol, rtol;
     KState *st;        // [2][kMaxRhs]
-    double *partials;  // [2][kMaxRhs][kNQ][NBMAX]
+    double *partials;  // [2][kMaxRhs][kNQ][pstride]
 };
+
+inline int round8(int n) { return (n + 7) / 8 * 8; }
+inline int spmv_workgroups(int C) { return (C + SB - 1) / SB; }
+inline int pcg_spmv_workgroups(int C) { return round8(spmv_workgroups(C)); }
+inline int pcg_vector_workgroups(int C) { return round8(solver_workgroups(C)); }
+inline int partial_stride(int C) {
+    int m = solver_workgroups(C);
+    for (int v : {spmv_workgroups(C), pcg_spmv_workgroups(C), pcg_vector_workgroups(C)}) m = v > m ? v : m;
+    return m;
+}
+
+// Tile of workgroup b in a grid of G (a multiple of 8) workgroups.  The
+// hardware hands workgroups to the 8 XCDs round-robin, so XCD k = b % 8 gets
+// the contiguous tiles [k G/8, (k+1) G/8): each XCD's L2 then holds one
+// Morton-contiguous part of the mesh, where nearly all gathered neighbours lie.
+__device__ __forceinline__ int xcd_tile(int b, int G) { return (b & 7) * (G >> 3) + (b >> 3); }
+
+// Contiguous rows of tile t when C rows are split into G tiles (slices of 64 kept whole).
+__device__ __forceinline__ void chunk_range(int C, int G, int t, int &lo, int &hi) {
+    const int per = ((C + G - 1) / G + 63) & ~63;
+    lo = min(C, t * per);
+    hi = min(C, lo + per);
+}
+
+// Workgroup barrier for LDS traffic only.  __syncthreads() also waits for
+// every global load the wave has in flight; here loads issued before the
+// barrier stay in flight while the workgroup exchanges values through LDS.
+__device__ __forceinline__ void lds_sync() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0); vmcnt and expcnt left alone
+    __builtin_amdgcn_s_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
 
 __device__ __forceinline__ double nrm_part(double v, bool inf) { return inf ? fabs(v) : v * v; }
 __device__ __forceinline__ double nrm_fin(double v, bool inf) { return inf ? v : sqrt(v); }
@@ -121,47 +161,50 @@ __device__ __forceinline__ void row_range(int C, int nb, int b, int &lo, int &hi
     hi = min(C, lo + per);
 }
 
-// ---- state plumbing -----------------------------------------------------
-template <int NR>
-struct Ctx {
-    KState *sh;  // shared copy
-};
+// Rows of SpMV workgroup b: one row per thread.
+__device__ __forceinline__ void spmv_range(int C, int b, int &lo, int &hi) {
+    lo = b * SB;
+    hi = min(C, lo + SB);
+}
 
+// ---- state plumbing -----------------------------------------------------
 template <int NR>
 __device__ __forceinline__ void load_state(const Ops &o, int s, KState *sh) {
     if (threadIdx.x < NR) sh[threadIdx.x] = o.st[((s - 1) & 1) * kMaxRhs + threadIdx.x];
-    __syncthreads();
+    lds_sync();
 }
 template <int NR>
 __device__ __forceinline__ void store_state(const Ops &o, int s, const KState *sh) {
-    __syncthreads();
+    lds_sync();
     if (blockIdx.x == 0 && threadIdx.x < NR) o.st[(s & 1) * kMaxRhs + threadIdx.x] = sh[threadIdx.x];
 }
 
 // Reduce the previous launch's partials: tot[r][q] in every thread.
-template <int NR, int NQ>
+template <int NR, int NQ, int BS = RB>
 __device__ __forceinline__ void reduce_prev(const Ops &o, int s, int nb_prev, const bool (&is_max)[NQ],
                                             double (&tot)[NR][NQ]) {
-    __shared__ double lds[NR * NQ][RB / kWave];
+    __shared__ double lds[NR * NQ][BS / kWave];
     __shared__ double res[NR * NQ];
-    const double *P = o.partials + (size_t)((s - 1) & 1) * kMaxRhs * kNQ * NBMAX;
+    const double *P = o.partials + (size_t)((s - 1) & 1) * kMaxRhs * kNQ * o.pstride;
     const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
 #pragma unroll
     for (int r = 0; r < NR; ++r)
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-            double v = threadIdx.x < nb_prev ? P[((size_t)r * kNQ + q) * NBMAX + threadIdx.x] : 0.0;
+            const double *Pq = P + ((size_t)r * kNQ + q) * o.pstride;
+            double v = 0.0;
+            for (int j = threadIdx.x; j < nb_prev; j += BS) v = is_max[q] ? fmax(v, Pq[j]) : v + Pq[j];
             v = is_max[q] ? wave_max(v) : wave_sum(v);
             if (lane == 0) lds[r * NQ + q][wid] = v;
         }
-    __syncthreads();
+    lds_sync();
     if (threadIdx.x < NR * NQ) {
         const int q = threadIdx.x % NQ;
         double a = lds[threadIdx.x][0];
-        for (int k = 1; k < RB / kWave; ++k) a = is_max[q] ? fmax(a, lds[threadIdx.x][k]) : a + lds[threadIdx.x][k];
+        for (int k = 1; k < BS / kWave; ++k) a = is_max[q] ? fmax(a, lds[threadIdx.x][k]) : a + lds[threadIdx.x][k];
         res[threadIdx.x] = a;
     }
-    __syncthreads();
+    lds_sync();
 #pragma unroll
     for (int r = 0; r < NR; ++r)
 #pragma unroll
@@ -169,10 +212,10 @@ __device__ __forceinline__ void reduce_prev(const Ops &o, int s, int nb_prev, co
 }
 
 // Write this workgroup's partials for the next launch.
-template <int NR, int NQ>
+template <int NR, int NQ, int BS = RB>
 __device__ __forceinline__ void write_partials(const Ops &o, int s, double (&v)[NR][NQ], const bool (&is_max)[NQ]) {
-    __shared__ double lds[NR * NQ][RB / kWave];
-    double *P = o.partials + (size_t)(s & 1) * kMaxRhs * kNQ * NBMAX;
+    __shared__ double lds[NR * NQ][BS / kWave];
+    double *P = o.partials + (size_t)(s & 1) * kMaxRhs * kNQ * o.pstride;
     const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
 #pragma unroll
     for (int r = 0; r < NR; ++r)
@@ -181,12 +224,12 @@ __device__ __forceinline__ void write_partials(const Ops &o, int s, double (&v)[
             double x = is_max[q] ? wave_max(v[r][q]) : wave_sum(v[r][q]);
             if (lane == 0) lds[r * NQ + q][wid] = x;
         }
-    __syncthreads();
+    lds_sync();
     if (threadIdx.x < NR * NQ) {
         const int r = threadIdx.x / NQ, q = threadIdx.x % NQ;
         double a = lds[threadIdx.x][0];
-        for (int k = 1; k < RB / kWave; ++k) a = is_max[q] ? fmax(a, lds[threadIdx.x][k]) : a + lds[threadIdx.x][k];
-        P[((size_t)r * kNQ + q) * NBMAX + blockIdx.x] = a;
+        for (int k = 1; k < BS / kWave; ++k) a = is_max[q] ? fmax(a, lds[threadIdx.x][k]) : a + lds[threadIdx.x][k];
+        P[((size_t)r * kNQ + q) * o.pstride + blockIdx.x] = a;
     }
 }
 
@@ -239,39 +282,319 @@ __global__ void k_from_soa(int C, int nrhs, const double *__restrict__ in, doubl
     for (int r = 0; r < nrhs; ++r) out[(size_t)i * nrhs + r] = in[(size_t)r * C + i];
 }
 
+// Steady PCG state with beta = 0 for the SpMV benchmark entry.
+__global__ void k_spmv_bench_state(KState *st, int n) {
+    int i = threadIdx.x;
+    if (i < n) {
+        KState S{};
+        S.phase = PH_ACTIVE;
+        S.fresh = 2;
+        S.rz_old = 1.0;
+        S.thresh = -1.0;
+        S.scale = 0.0;
+        st[i] = S;
+    }
+}
+
 __global__ void k_zero_state(KState *st, int n) {
     int i = threadIdx.x;
     if (i < n) st[i] = KState{};
 }
 
 // ---------------------------------------------------------------- PCG
-// r = b - A x ; z = M^-1 r ; p = z ; partials |r|, r.z   (cg.py:88-118)
-template <typename VT, int NR>
-__global__ __launch_bounds__(RB) void k_cg_init(Ops o, int s, const VT *val, const VT *dg, const VT *inv_d,
-                                                const VT *b, const VT *x, VT *r_, VT *z_, VT *p_) {
+// Two launches per iteration, equal in exact arithmetic to cg.py:120-170:
+//   k_cg_dir : test |r| left by the previous update (cg.py:150-158);
+//              beta = r.z / r.z_old ; p = z + beta p_old written to the
+//              other p buffer, and Ap.  Every gathered p_j is rebuilt from
+//              z_j and p_old_j with the same fma its owner row uses, so Ap
+//              is A times the stored p ; partial p.Ap.
+//   k_cg_upd : alpha = r.z_old / p.Ap ; x += alpha p ; r -= alpha Ap ;
+//              z = M^-1 r ; partials |r|, r.z.
+// The current p buffer is tracked in the solver state (pcur), so launches
+// that turn into no-ops never desynchronise it.
+
+template <typename VT>
+__device__ __forceinline__ VT dir_fma(VT beta, VT p_old, VT z);
+template <>
+__device__ __forceinline__ double dir_fma<double>(double beta, double p_old, double z) {
+    return __builtin_fma(beta, p_old, z);
+}
+template <>
+__device__ __forceinline__ float dir_fma<float>(float beta, float p_old, float z) {
+    return __builtin_fmaf(beta, p_old, z);
+}
+
+// Convergence test on the previous update's residual, beta and p = z + beta p_old ; Ap ; partial p.Ap.
+// Mixed precision (MX): the test only decides whether the FP64 refinement is due.
+// The row's matrix entries, its own operands and the first batch of gathered
+// neighbours are loaded before the workgroup reduces the previous partials,
+// so those loads are in flight during the reduction.
+template <typename VT, int NR, bool MX>
+__global__ __launch_bounds__(SB) void k_cg_dir(Ops o, int s, int nb_prev, const VT *__restrict__ val,
+                                               const double *__restrict__ dg, const VT *__restrict__ z,
+                                               VT *__restrict__ p0, VT *__restrict__ p1, VT *__restrict__ ap) {
     __shared__ KState sh[NR];
+    __shared__ int start_s[NR];
     load_state<NR>(o, s, sh);
-    int lo, hi;
-    row_range(o.C, o.nb, blockIdx.x, lo, hi);
-    double v[NR][2] = {};
-    for (int i = lo + threadIdx.x; i < hi; i += RB) {
-        VT ax[NR];
-        sell_row<VT, NR>(o.sell_ptr, o.sell_col, val, dg[i], i, x, o.C, ax);
-        const VT id = inv_d[i];
+    if (!any_phase<NR>(sh, PH_ACTIVE)) {
+        store_state<NR>(o, s, sh);
+        return;
+    }
+    const int i = xcd_tile(blockIdx.x, gridDim.x) * SB + threadIdx.x;
+    const int ic = i < o.C ? i : o.C - 1;
+    const VT *pold[NR];
+    VT *pnew[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {  // current direction in buffer pcur; the new one goes to the other
+        pold[r] = (sh[r].pcur ? p1 : p0) + (size_t)r * o.C;
+        pnew[r] = (sh[r].pcur ? p0 : p1) + (size_t)r * o.C;
+    }
+    // ---- loads issued ahead of the reduction
+    const int sl = ic >> 6, lane = ic & 63;
+    const int base = o.sell_ptr[sl];
+    const int W = (o.sell_ptr[sl + 1] - base) >> 6;
+    const double d = dg[ic];
+    VT zi[NR], poi[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        zi[r] = z[(size_t)r * o.C + ic];
+        poi[r] = pold[r][ic];
+    }
+    int col[kBatch];
+    VT v[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+        col[u] = ic;
+        v[u] = (VT)0;
+        if (u < W) {
+            const int e = base + u * 64 + lane;
+            col[u] = __builtin_nontemporal_load(o.sell_col + e);
+            v[u] = __builtin_nontemporal_load(val + e);
+        }
+    }
+    VT zj[kBatch][NR], pj[kBatch][NR];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u)
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
+            zj[u][r] = z[(size_t)r * o.C + col[u]];
+            pj[u][r] = pold[r][col[u]];
+        }
+    // ---- previous partials: |r|, r.z
+    const bool mx[2] = {(bool)o.norm_inf, false};
+    double tot[NR][2];
+    reduce_prev<NR, 2, SB>(o, s, nb_prev, mx, tot);
+    if (threadIdx.x == 0) {
+        for (int r = 0; r < NR; ++r) {
+            KState &S = sh[r];
+            start_s[r] = S.fresh == 1;
+            if (S.phase != PH_ACTIVE) continue;
+            if (S.fresh == 0) {
+                const double nr = nrm_fin(tot[r][0], o.norm_inf);
+                S.iter += 1;
+                if (!MX) {
+                    S.final_res = nr;
+                    if (nr < S.thresh) {
+                        S.converged = 1;
+                        S.phase = PH_DONE;
+                        continue;
+                    }
+                    if (S.iter >= o.max_iter) {
+                        S.phase = PH_DONE;
+                        continue;
+                    }
+                } else if (nr < kReliable * S.scale || nr < S.thresh || S.iter >= o.max_iter) {
+                    S.phase = PH_RESTART;  // FP64 refinement due
+                    continue;
+                }
+            }
+            if (S.fresh != 1) {
+                S.beta = tot[r][1] / S.rz_old;
+                S.rz_old = tot[r][1];
+            } else {
+                S.beta = 0.0;
+            }
+            S.fresh = 0;
+            S.pcur ^= 1;
+        }
+    }
+    lds_sync();
+    bool act[NR], start[NR];
+    VT be[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        act[r] = sh[r].phase == PH_ACTIVE;
+        start[r] = start_s[r] != 0;
+        be[r] = (VT)sh[r].beta;
+    }
+    double acc[NR];
+    VT pi[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        pi[r] = start[r] ? zi[r] : dir_fma<VT>(be[r], poi[r], zi[r]);
+        acc[r] = d * (double)pi[r];
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u)
+        if (u < W)
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                const VT q = start[r] ? zj[u][r] : dir_fma<VT>(be[r], pj[u][r], zj[u][r]);
+                acc[r] += (double)v[u] * (double)q;
+            }
+    for (int kb = kBatch; kb < W; kb += kBatch) {  // rows wider than one batch
+        int c2[kBatch];
+        VT v2[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            c2[u] = ic;
+            v2[u] = (VT)0;
+            if (kb + u < W) {
+                const int e = base + (kb + u) * 64 + lane;
+                c2[u] = __builtin_nontemporal_load(o.sell_col + e);
+                v2[u] = __builtin_nontemporal_load(val + e);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u)
+            if (kb + u < W)
+#pragma unroll
+                for (int r = 0; r < NR; ++r) {
+                    const VT zz = z[(size_t)r * o.C + c2[u]];
+                    const VT q = start[r] ? zz : dir_fma<VT>(be[r], pold[r][c2[u]], zz);
+                    acc[r] += (double)v2[u] * (double)q;
+                }
+    }
+    double pv[NR][1] = {};
+    if (i < o.C) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            if (!act[r]) continue;
+            const VT a = (VT)acc[r];
+            pnew[r][i] = pi[r];
+            ap[(size_t)r * o.C + i] = a;
+            pv[r][0] = (double)pi[r] * (double)a;
+        }
+    }
+    const bool mx1[1] = {false};
+    write_partials<NR, 1, SB>(o, s, pv, mx1);
+    store_state<NR>(o, s, sh);
+}
+
+// alpha = rz_old / p.Ap ; x += alpha p ; r -= alpha Ap ; z = M^-1 r ; partials |r|, r.z.
+// Each thread's first rows (kHoist of them) are loaded before the partial
+// reduction that yields alpha; kHoist keeps the FP64 kernel within 64 VGPRs
+// (two 1024-thread workgroups per CU).
+template <typename VT, int NR>
+__global__ __launch_bounds__(RB) void k_cg_upd(Ops o, int s, const VT *__restrict__ inv_d, VT *__restrict__ x_,
+                                               VT *__restrict__ r_, VT *__restrict__ z_, const VT *__restrict__ p0,
+                                               const VT *__restrict__ p1, const VT *__restrict__ ap_) {
+    constexpr int kHoist = (sizeof(VT) == 4 ? 4 : 2) / NR > 0 ? (sizeof(VT) == 4 ? 4 : 2) / NR : 1;
+    __shared__ KState sh[NR];
+    load_state<NR>(o, s, sh);
+    if (!any_phase<NR>(sh, PH_ACTIVE)) {
+        store_state<NR>(o, s, sh);
+        return;
+    }
+    int lo, hi;
+    chunk_range(o.C, gridDim.x, xcd_tile(blockIdx.x, gridDim.x), lo, hi);
+    const VT *p_[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) p_[r] = (sh[r].pcur ? p1 : p0) + (size_t)r * o.C;
+    VT hx[kHoist][NR], hr[kHoist][NR], hp[kHoist][NR], ha[kHoist][NR], hid[kHoist];
+#pragma unroll
+    for (int h = 0; h < kHoist; ++h) {
+        const int i = lo + threadIdx.x + h * RB;
+        const int ic = i < hi ? i : o.C - 1;
+        hid[h] = inv_d[ic];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const size_t k = (size_t)r * o.C + ic;
+            hx[h][r] = x_[k];
+            hr[h][r] = r_[k];
+            hp[h][r] = p_[r][ic];
+            ha[h][r] = ap_[k];
+        }
+    }
+    const bool mx1[1] = {false};
+    double pap[NR][1];
+    reduce_prev<NR, 1, RB>(o, s, o.gs, mx1, pap);
+    if (threadIdx.x == 0)
+        for (int r = 0; r < NR; ++r)
+            if (sh[r].phase == PH_ACTIVE) sh[r].alpha = sh[r].rz_old / pap[r][0];
+    lds_sync();
+    double v[NR][2] = {};
+    bool act[NR];
+    VT a[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        act[r] = sh[r].phase == PH_ACTIVE;
+        a[r] = (VT)sh[r].alpha;
+    }
+    auto row = [&](int i, VT id, const VT (&xv)[NR], const VT (&rv)[NR], const VT (&pv)[NR], const VT (&av)[NR]) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            if (!act[r]) continue;
             const size_t k = (size_t)r * o.C + i;
-            VT rr = b[k] - ax[r];
-            VT zz = id * rr;
+            const VT xx = xv[r] + a[r] * pv[r];
+            const VT rr = rv[r] - a[r] * av[r];
+            const VT zz = id * rr;
+            x_[k] = xx;
             r_[k] = rr;
             z_[k] = zz;
-            p_[k] = zz;
             v[r][0] = o.norm_inf ? fmax(v[r][0], fabs((double)rr)) : v[r][0] + (double)rr * (double)rr;
             v[r][1] += (double)rr * (double)zz;
         }
+    };
+#pragma unroll
+    for (int h = 0; h < kHoist; ++h) {
+        const int i = lo + threadIdx.x + h * RB;
+        if (i < hi) row(i, hid[h], hx[h], hr[h], hp[h], ha[h]);
+    }
+    for (int i = lo + threadIdx.x + kHoist * RB; i < hi; i += RB) {  // chunks beyond kHoist rows per thread
+        VT xv[NR], rv[NR], pv[NR], av[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const size_t k = (size_t)r * o.C + i;
+            xv[r] = x_[k];
+            rv[r] = r_[k];
+            pv[r] = p_[r][i];
+            av[r] = ap_[k];
+        }
+        row(i, inv_d[i], xv, rv, pv, av);
     }
     const bool mx[2] = {(bool)o.norm_inf, false};
-    write_partials<NR, 2>(o, s, v, mx);
+    write_partials<NR, 2, RB>(o, s, v, mx);
+    store_state<NR>(o, s, sh);
+}
+
+// r = b - A x ; z = M^-1 r ; partials |r|, r.z  (cg.py:88-118)
+template <int NR>
+__global__ __launch_bounds__(SB) void k_cg_init(Ops o, int s, const double *__restrict__ val,
+                                                const double *__restrict__ dg, const double *__restrict__ inv_d,
+                                                const double *__restrict__ b, const double *__restrict__ x,
+                                                double *__restrict__ r_, double *__restrict__ z_) {
+    __shared__ KState sh[NR];
+    load_state<NR>(o, s, sh);
+    const int i = xcd_tile(blockIdx.x, gridDim.x) * SB + threadIdx.x;
+    double v[NR][2] = {};
+    if (i < o.C) {
+        double ax[NR];
+        sell_row<double, NR>(o.sell_ptr, o.sell_col, val, dg[i], i, x, o.C, ax);
+        const double id = inv_d[i];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            const size_t k = (size_t)r * o.C + i;
+            const double rr = b[k] - ax[r];
+            const double zz = id * rr;
+            r_[k] = rr;
+            z_[k] = zz;
+            v[r][0] = o.norm_inf ? fabs(rr) : rr * rr;
+            v[r][1] = rr * zz;
+        }
+    }
+    const bool mx[2] = {(bool)o.norm_inf, false};
+    write_partials<NR, 2, SB>(o, s, v, mx);
     store_state<NR>(o, s, sh);
 }
 
@@ -282,7 +605,7 @@ __global__ __launch_bounds__(RB) void k_cg_init_fin(Ops o, int s, int nb_prev) {
     load_state<NR>(o, s, sh);
     const bool mx[2] = {(bool)o.norm_inf, false};
     double tot[NR][2];
-    reduce_prev<NR, 2>(o, s, nb_prev, mx, tot);
+    reduce_prev<NR, 2, RB>(o, s, nb_prev, mx, tot);
     if (threadIdx.x == 0) {
         for (int r = 0; r < NR; ++r) {
             KState &S = sh[r];
@@ -295,145 +618,10 @@ __global__ __launch_bounds__(RB) void k_cg_init_fin(Ops o, int s, int nb_prev) {
             S.rz_old = tot[r][1];
             S.converged = n0 < S.thresh;
             S.phase = S.converged ? PH_DONE : PH_ACTIVE;
-            S.outer_phase = PH_DONE;
+            S.fresh = 1;
+            S.pcur = 0;
         }
     }
-    store_state<NR>(o, s, sh);
-}
-
-// Ap = A p ; partial p.Ap
-template <typename VT, int NR>
-__global__ __launch_bounds__(RB) void k_cg_spmv(Ops o, int s, const VT *val, const VT *dg, const VT *p_, VT *ap_) {
-    __shared__ KState sh[NR];
-    load_state<NR>(o, s, sh);
-    if (any_phase<NR>(sh, PH_ACTIVE)) {
-        int lo, hi;
-        row_range(o.C, o.nb, blockIdx.x, lo, hi);
-        double v[NR][1] = {};
-        for (int i = lo + threadIdx.x; i < hi; i += RB) {
-            VT ap[NR];
-            sell_row<VT, NR>(o.sell_ptr, o.sell_col, val, dg[i], i, p_, o.C, ap);
-#pragma unroll
-            for (int r = 0; r < NR; ++r) {
-                const size_t k = (size_t)r * o.C + i;
-                ap_[k] = ap[r];
-                v[r][0] += (double)p_[k] * (double)ap[r];
-            }
-        }
-        const bool mx[1] = {false};
-        write_partials<NR, 1>(o, s, v, mx);
-    }
-    store_state<NR>(o, s, sh);
-}
-
-// alpha = rz_old / p.Ap ; x += alpha p ; r -= alpha Ap ; z = M^-1 r ; partials |r|, r.z
-template <typename VT, int NR>
-__global__ __launch_bounds__(RB) void k_cg_update(Ops o, int s, const VT *inv_d, VT *x_, VT *r_, VT *z_,
-                                                  const VT *p_, const VT *ap_) {
-    __shared__ KState sh[NR];
-    load_state<NR>(o, s, sh);
-    if (!any_phase<NR>(sh, PH_ACTIVE)) {
-        store_state<NR>(o, s, sh);
-        return;
-    }
-    const bool mx1[1] = {false};
-    double pap[NR][1];
-    reduce_prev<NR, 1>(o, s, o.nb, mx1, pap);
-    if (threadIdx.x == 0)
-        for (int r = 0; r < NR; ++r)
-            if (sh[r].phase == PH_ACTIVE) sh[r].alpha = sh[r].rz_old / pap[r][0];
-    __syncthreads();
-    int lo, hi;
-    row_range(o.C, o.nb, blockIdx.x, lo, hi);
-    double v[NR][2] = {};
-    bool act[NR];
-    VT a[NR];
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-        act[r] = sh[r].phase == PH_ACTIVE;
-        a[r] = (VT)sh[r].alpha;
-    }
-    for (int i = lo + threadIdx.x; i < hi; i += RB) {
-        const VT id = inv_d[i];
-#pragma unroll
-        for (int r = 0; r < NR; ++r) {
-            if (!act[r]) continue;
-            const size_t k = (size_t)r * o.C + i;
-            VT xx = x_[k] + a[r] * p_[k];
-            VT rr = r_[k] - a[r] * ap_[k];
-            VT zz = id * rr;
-            x_[k] = xx;
-            r_[k] = rr;
-            z_[k] = zz;
-            v[r][0] = o.norm_inf ? fmax(v[r][0], fabs((double)rr)) : v[r][0] + (double)rr * (double)rr;
-            v[r][1] += (double)rr * (double)zz;
-        }
-    }
-    const bool mx[2] = {(bool)o.norm_inf, false};
-    write_partials<NR, 2>(o, s, v, mx);
-    store_state<NR>(o, s, sh);
-}
-
-// |r| < thresh ? done : beta = r.z_new / r.z_old ; p = z + beta p
-// `inner` selects the mixed-precision sweep rules (inner threshold, budget).
-template <typename VT, int NR>
-__global__ __launch_bounds__(RB) void k_cg_pupdate(Ops o, int s, int k_iter, int inner, const VT *z_, VT *p_) {
-    __shared__ KState sh[NR];
-    load_state<NR>(o, s, sh);
-    if (!any_phase<NR>(sh, PH_ACTIVE)) {
-        store_state<NR>(o, s, sh);
-        return;
-    }
-    const bool mx[2] = {(bool)o.norm_inf, false};
-    double tot[NR][2];
-    reduce_prev<NR, 2>(o, s, o.nb, mx, tot);
-    if (threadIdx.x == 0) {
-        for (int r = 0; r < NR; ++r) {
-            KState &S = sh[r];
-            if (S.phase != PH_ACTIVE) continue;
-            const double nr = nrm_fin(tot[r][0], o.norm_inf);
-            if (!inner) {
-                S.iter += 1;
-                S.final_res = nr;
-                if (nr < S.thresh) {
-                    S.converged = 1;
-                    S.phase = PH_DONE;
-                    continue;
-                }
-                if (S.iter >= o.max_iter) {
-                    S.phase = PH_DONE;
-                    continue;
-                }
-            } else {
-                const int done_iters = k_iter - S.sweep_start + 1;
-                S.iter = done_iters;
-                S.total_iter = S.total_before + done_iters;
-                if (nr < S.inner_thresh || S.total_iter >= o.max_iter) {
-                    S.phase = PH_DONE;
-                    continue;
-                }
-            }
-            S.beta = tot[r][1] / S.rz_old;
-            S.rz_old = tot[r][1];
-        }
-    }
-    __syncthreads();
-    bool act[NR];
-    VT be[NR];
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-        act[r] = sh[r].phase == PH_ACTIVE;
-        be[r] = (VT)sh[r].beta;
-    }
-    int lo, hi;
-    row_range(o.C, o.nb, blockIdx.x, lo, hi);
-    for (int i = lo + threadIdx.x; i < hi; i += RB)
-#pragma unroll
-        for (int r = 0; r < NR; ++r) {
-            if (!act[r]) continue;
-            const size_t k = (size_t)r * o.C + i;
-            p_[k] = z_[k] + p_[k] * be[r];
-        }
     store_state<NR>(o, s, sh);
 }
 
@@ -444,11 +632,11 @@ __global__ __launch_bounds__(RB) void k_cg_pupdate(Ops o, int s, int k_iter, int
 // the row sum of the FP64 operator.  FP32 x FP32 products are exact in FP64
 // and are accumulated in FP64, so the near-null (constant) mode of a
 // reference-pinned Neumann pressure matrix does not suffer cancellation.
-// When the recursive residual has dropped by `reliable` since the last
+// When the recursive residual has dropped by `kReliable` since the last
 // refinement (or below the threshold, or the budget ran out) the FP32
 // correction is added to the FP64 solution and the FP64 residual b - A x
-// replaces the recursive one; convergence is decided on that true residual.
-constexpr double kReliable = 0.1;
+// replaces the recursive one; the direction recurrence continues, and
+// convergence is decided on that true residual only.
 
 // val32 = (float) val64 ; dgc = diag + sum_row(val64 - val32)
 __global__ void k_mx_prep(int C, const int32_t *sell_ptr, const double *val64, const double *diag, float *val32,
@@ -469,95 +657,15 @@ __global__ void k_mx_prep(int C, const int32_t *sell_ptr, const double *val64, c
     dgc[i] = diag[i] + corr;
 }
 
-// FP32 storage, exact products, FP64 accumulation.
-template <int NR>
-__device__ __forceinline__ void sell_row_mx(const int32_t *__restrict__ sell_ptr, const int32_t *__restrict__ sell_col,
-                                            const float *__restrict__ val, double dg, int i,
-                                            const float *__restrict__ x, int C, double (&acc)[NR]) {
-    sell_row_batched<float, double, NR>(sell_ptr, sell_col, val, dg, i, x, C, acc);
-}
-
-// Ap = A32 p ; partial p.Ap
-template <int NR>
-__global__ __launch_bounds__(RB) void k_mx_spmv(Ops o, int s, const float *val, const double *dgc, const float *p_,
-                                                float *ap_) {
-    __shared__ KState sh[NR];
-    load_state<NR>(o, s, sh);
-    if (any_phase<NR>(sh, PH_ACTIVE)) {
-        int lo, hi;
-        row_range(o.C, o.nb, blockIdx.x, lo, hi);
-        double v[NR][1] = {};
-        for (int i = lo + threadIdx.x; i < hi; i += RB) {
-            double ap[NR];
-            sell_row_mx<NR>(o.sell_ptr, o.sell_col, val, dgc[i], i, p_, o.C, ap);
-#pragma unroll
-            for (int r = 0; r < NR; ++r) {
-                const size_t k = (size_t)r * o.C + i;
-                const float a = (float)ap[r];
-                ap_[k] = a;
-                v[r][0] += (double)p_[k] * (double)a;
-            }
-        }
-        const bool mx[1] = {false};
-        write_partials<NR, 1>(o, s, v, mx);
-    }
-    store_state<NR>(o, s, sh);
-}
-
-// |r| check: refinement pending, or beta and p = z + beta p
-template <int NR>
-__global__ __launch_bounds__(RB) void k_mx_pupdate(Ops o, int s, const float *z_, float *p_) {
-    __shared__ KState sh[NR];
-    load_state<NR>(o, s, sh);
-    if (!any_phase<NR>(sh, PH_ACTIVE)) {
-        store_state<NR>(o, s, sh);
-        return;
-    }
-    const bool mx[2] = {(bool)o.norm_inf, false};
-    double tot[NR][2];
-    reduce_prev<NR, 2>(o, s, o.nb, mx, tot);
-    if (threadIdx.x == 0) {
-        for (int r = 0; r < NR; ++r) {
-            KState &S = sh[r];
-            if (S.phase != PH_ACTIVE) continue;
-            S.iter += 1;
-            const double nr = nrm_fin(tot[r][0], o.norm_inf);
-            if (nr < kReliable * S.scale || nr < S.thresh || S.iter >= o.max_iter) {
-                S.phase = PH_RESTART;  // refinement pending
-                continue;
-            }
-            S.beta = tot[r][1] / S.rz_old;
-            S.rz_old = tot[r][1];
-        }
-    }
-    __syncthreads();
-    bool act[NR];
-    float be[NR];
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-        act[r] = sh[r].phase == PH_ACTIVE;
-        be[r] = (float)sh[r].beta;
-    }
-    int lo, hi;
-    row_range(o.C, o.nb, blockIdx.x, lo, hi);
-    for (int i = lo + threadIdx.x; i < hi; i += RB)
-#pragma unroll
-        for (int r = 0; r < NR; ++r) {
-            if (!act[r]) continue;
-            const size_t k = (size_t)r * o.C + i;
-            p_[k] = z_[k] + p_[k] * be[r];
-        }
-    store_state<NR>(o, s, sh);
-}
-
 // Refinement, part 1: x64 += x32 ; x32 = 0
 template <int NR>
-__global__ __launch_bounds__(RB) void k_mx_accumulate(Ops o, int s, float *x32, double *x64) {
+__global__ __launch_bounds__(RB) void k_mx_accumulate(Ops o, int s, float *__restrict__ x32,
+                                                      double *__restrict__ x64) {
     __shared__ KState sh[NR];
     load_state<NR>(o, s, sh);
     if (any_phase<NR>(sh, PH_RESTART)) {
         int lo, hi;
-        row_range(o.C, o.nb, blockIdx.x, lo, hi);
+        chunk_range(o.C, gridDim.x, xcd_tile(blockIdx.x, gridDim.x), lo, hi);
         for (int i = lo + threadIdx.x; i < hi; i += RB)
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
@@ -571,11 +679,13 @@ __global__ __launch_bounds__(RB) void k_mx_accumulate(Ops o, int s, float *x32, 
 }
 
 // Refinement, part 2: r64 = b - A64 x64 ; r32 = r64 ; z32 = M^-1 r32 ;
-// partials |r64| (norm type), r32.z32.  `first` also starts the solve.
+// partials |r64| (norm type), r32.z32.  `first` also starts the solve (x32 = 0).
 template <int NR>
-__global__ __launch_bounds__(RB) void k_mx_residual(Ops o, int s, int first, const double *val, const double *dg,
-                                                    const float *inv_d, const double *b, const double *x64,
-                                                    float *r32, float *z32) {
+__global__ __launch_bounds__(SB) void k_mx_residual(Ops o, int s, int first, const double *__restrict__ val,
+                                                    const double *__restrict__ dg, const float *__restrict__ inv_d,
+                                                    const double *__restrict__ b, const double *__restrict__ x64,
+                                                    float *__restrict__ r32, float *__restrict__ z32,
+                                                    float *__restrict__ x32) {
     __shared__ KState sh[NR];
     load_state<NR>(o, s, sh);
     bool act[NR];
@@ -586,10 +696,9 @@ __global__ __launch_bounds__(RB) void k_mx_residual(Ops o, int s, int first, con
         any |= act[r];
     }
     if (any) {
-        int lo, hi;
-        row_range(o.C, o.nb, blockIdx.x, lo, hi);
+        const int i = xcd_tile(blockIdx.x, gridDim.x) * SB + threadIdx.x;
         double v[NR][2] = {};
-        for (int i = lo + threadIdx.x; i < hi; i += RB) {
+        if (i < o.C) {
             double ax[NR];
             sell_row<double, NR>(o.sell_ptr, o.sell_col, val, dg[i], i, x64, o.C, ax);
             const float id = inv_d[i];
@@ -602,43 +711,52 @@ __global__ __launch_bounds__(RB) void k_mx_residual(Ops o, int s, int first, con
                 const float zz = id * r32v;
                 r32[k] = r32v;
                 z32[k] = zz;
-                v[r][0] = o.norm_inf ? fmax(v[r][0], fabs(rr)) : v[r][0] + rr * rr;
-                v[r][1] += (double)r32v * (double)zz;
+                if (first) x32[k] = 0.0f;
+                v[r][0] = o.norm_inf ? fabs(rr) : rr * rr;
+                v[r][1] = (double)r32v * (double)zz;
             }
         }
         const bool mx[2] = {(bool)o.norm_inf, false};
-        write_partials<NR, 2>(o, s, v, mx);
+        write_partials<NR, 2, SB>(o, s, v, mx);
     }
     store_state<NR>(o, s, sh);
 }
 
-// Refinement, part 3: decide on the true residual; restart the direction
-// recurrence with p = z + beta p (beta = 0 on the first call).
+// Refinement, part 3 (one workgroup): decide on the true residual, or pass
+// the last update's |r|, r.z through.  Either way one reduced partial set is
+// written for the k_cg_dir launch that follows.
 template <int NR>
-__global__ __launch_bounds__(RB) void k_mx_refine(Ops o, int s, int first, const float *z32, float *p32, float *x32) {
+__global__ __launch_bounds__(RB) void k_mx_refine(Ops o, int s, int first, int nb_res, int nb_upd) {
     __shared__ KState sh[NR];
+    __shared__ double red[NR][2];
     load_state<NR>(o, s, sh);
-    bool act[NR];
-    bool any = false;
+    bool rep[NR];
+    bool any_rep = false, any_act = false;
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
-        act[r] = first || sh[r].phase == PH_RESTART;
-        any |= act[r];
+        rep[r] = first || sh[r].phase == PH_RESTART;
+        any_rep |= rep[r];
+        any_act |= sh[r].phase == PH_ACTIVE;
     }
-    if (any) {
+    if (any_rep || any_act) {
         const bool mx[2] = {(bool)o.norm_inf, false};
         double tot[NR][2];
-        reduce_prev<NR, 2>(o, s, o.nb, mx, tot);
+        reduce_prev<NR, 2, RB>(o, s, any_rep ? nb_res : nb_upd, mx, tot);
         if (threadIdx.x == 0) {
             for (int r = 0; r < NR; ++r) {
-                if (!act[r]) continue;
+                red[r][0] = tot[r][0];
+                red[r][1] = tot[r][1];
+                if (!rep[r]) continue;
                 KState &S = sh[r];
                 const double nr = nrm_fin(tot[r][0], o.norm_inf);
                 if (first) {
                     S.norm0 = nr;
                     S.thresh = fmax(o.atol, o.rtol * nr);
                     S.iter = 0;
+                    S.restarts = 0;
                     S.converged = 0;
+                    S.rz_old = tot[r][1];
+                    S.pcur = 0;
                 }
                 S.final_res = nr;
                 if (nr < S.thresh) {
@@ -647,24 +765,18 @@ __global__ __launch_bounds__(RB) void k_mx_refine(Ops o, int s, int first, const
                 } else if (S.iter >= o.max_iter) {
                     S.phase = PH_DONE;
                 } else {
-                    S.beta = first ? 0.0 : tot[r][1] / S.rz_old;
-                    S.rz_old = tot[r][1];
                     S.scale = nr;  // true residual at this refinement
+                    S.fresh = first ? 1 : 2;
                     S.phase = PH_ACTIVE;
                 }
             }
         }
-        __syncthreads();
-        int lo, hi;
-        row_range(o.C, o.nb, blockIdx.x, lo, hi);
-        for (int i = lo + threadIdx.x; i < hi; i += RB)
-#pragma unroll
-            for (int r = 0; r < NR; ++r) {
-                if (!act[r] || sh[r].phase != PH_ACTIVE) continue;
-                const size_t k = (size_t)r * o.C + i;
-                p32[k] = first ? z32[k] : z32[k] + p32[k] * (float)sh[r].beta;
-                if (first) x32[k] = 0.0f;
-            }
+        lds_sync();
+        if (blockIdx.x == 0 && threadIdx.x < NR * 2) {
+            double *P = o.partials + (size_t)(s & 1) * kMaxRhs * kNQ * o.pstride;
+            const int r = threadIdx.x / 2, q = threadIdx.x % 2;
+            P[((size_t)r * kNQ + q) * o.pstride] = red[r][q];
+        }
     }
     store_state<NR>(o, s, sh);
 }
@@ -682,14 +794,14 @@ __device__ __forceinline__ void bi_breakdown(KState &S, const Ops &o) {
 
 // r = b - A x ; partial |r| ; r_hat = r ; p = r ; partial r.r  (bicgstab.py:88-118)
 template <int NR>
-__global__ __launch_bounds__(RB) void k_bi_init(Ops o, int s, const double *val, const double *dg, const double *b,
+__global__ __launch_bounds__(SB) void k_bi_init(Ops o, int s, const double *val, const double *dg, const double *b,
                                                 const double *x, double *r_, double *rh_, double *p_) {
     __shared__ KState sh[NR];
     load_state<NR>(o, s, sh);
     int lo, hi;
-    row_range(o.C, o.nb, blockIdx.x, lo, hi);
+    spmv_range(o.C, blockIdx.x, lo, hi);
     double v[NR][2] = {};
-    for (int i = lo + threadIdx.x; i < hi; i += RB) {
+    for (int i = lo + threadIdx.x; i < hi; i += SB) {
         double ax[NR];
         sell_row<double, NR>(o.sell_ptr, o.sell_col, val, dg[i], i, x, o.C, ax);
 #pragma unroll
@@ -704,7 +816,7 @@ __global__ __launch_bounds__(RB) void k_bi_init(Ops o, int s, const double *val,
         }
     }
     const bool mx[2] = {(bool)o.norm_inf, false};
-    write_partials<NR, 2>(o, s, v, mx);
+    write_partials<NR, 2, SB>(o, s, v, mx);
     store_state<NR>(o, s, sh);
 }
 
@@ -791,15 +903,15 @@ __global__ __launch_bounds__(RB) void k_bi_y(Ops o, int s, const double *inv_d, 
 
 // v = A y ; partials r_hat.v, |r_hat|^2, |v|^2
 template <int NR>
-__global__ __launch_bounds__(RB) void k_bi_v(Ops o, int s, const double *val, const double *dg, const double *y_,
+__global__ __launch_bounds__(SB) void k_bi_v(Ops o, int s, const double *val, const double *dg, const double *y_,
                                              const double *rh_, double *v_) {
     __shared__ KState sh[NR];
     load_state<NR>(o, s, sh);
     if (any_phase<NR>(sh, PH_ACTIVE)) {
         int lo, hi;
-        row_range(o.C, o.nb, blockIdx.x, lo, hi);
+        spmv_range(o.C, blockIdx.x, lo, hi);
         double v[NR][3] = {};
-        for (int i = lo + threadIdx.x; i < hi; i += RB) {
+        for (int i = lo + threadIdx.x; i < hi; i += SB) {
             double av[NR];
             sell_row<double, NR>(o.sell_ptr, o.sell_col, val, dg[i], i, y_, o.C, av);
 #pragma unroll
@@ -813,7 +925,7 @@ __global__ __launch_bounds__(RB) void k_bi_v(Ops o, int s, const double *val, co
             }
         }
         const bool mx[3] = {false, false, false};
-        write_partials<NR, 3>(o, s, v, mx);
+        write_partials<NR, 3, SB>(o, s, v, mx);
     }
     store_state<NR>(o, s, sh);
 }
@@ -829,7 +941,7 @@ __global__ __launch_bounds__(RB) void k_bi_s(Ops o, int s, const double *r_, con
     }
     const bool mx3[3] = {false, false, false};
     double tot[NR][3];
-    reduce_prev<NR, 3>(o, s, o.nb, mx3, tot);
+    reduce_prev<NR, 3>(o, s, o.nbs, mx3, tot);
     if (threadIdx.x == 0)
         for (int r = 0; r < NR; ++r) {
             KState &S = sh[r];
@@ -906,15 +1018,15 @@ __global__ __launch_bounds__(RB) void k_bi_z(Ops o, int s, const double *inv_d, 
 
 // t = A z ; partials t.t, t.s
 template <int NR>
-__global__ __launch_bounds__(RB) void k_bi_t(Ops o, int s, const double *val, const double *dg, const double *z_,
+__global__ __launch_bounds__(SB) void k_bi_t(Ops o, int s, const double *val, const double *dg, const double *z_,
                                              const double *s_, double *t_) {
     __shared__ KState sh[NR];
     load_state<NR>(o, s, sh);
     if (any_phase<NR>(sh, PH_ACTIVE)) {
         int lo, hi;
-        row_range(o.C, o.nb, blockIdx.x, lo, hi);
+        spmv_range(o.C, blockIdx.x, lo, hi);
         double v[NR][2] = {};
-        for (int i = lo + threadIdx.x; i < hi; i += RB) {
+        for (int i = lo + threadIdx.x; i < hi; i += SB) {
             double at[NR];
             sell_row<double, NR>(o.sell_ptr, o.sell_col, val, dg[i], i, z_, o.C, at);
 #pragma unroll
@@ -926,7 +1038,7 @@ __global__ __launch_bounds__(RB) void k_bi_t(Ops o, int s, const double *val, co
             }
         }
         const bool mx[2] = {false, false};
-        write_partials<NR, 2>(o, s, v, mx);
+        write_partials<NR, 2, SB>(o, s, v, mx);
     }
     store_state<NR>(o, s, sh);
 }
@@ -943,7 +1055,7 @@ __global__ __launch_bounds__(RB) void k_bi_update(Ops o, int s, const double *y_
     }
     const bool mx2[2] = {false, false};
     double tot[NR][2];
-    reduce_prev<NR, 2>(o, s, o.nb, mx2, tot);
+    reduce_prev<NR, 2>(o, s, o.nbs, mx2, tot);
     if (threadIdx.x == 0)
         for (int r = 0; r < NR; ++r) {
             KState &S = sh[r];
@@ -1053,8 +1165,8 @@ struct WS {
 };
 
 struct Layout {
-    double *val64, *dg64, *inv64, *b, *x, *r, *z, *p, *ap, *y, *s, *t, *rh, *v;
-    float *val32, *dg32, *inv32, *x32, *r32, *z32, *p32, *ap32;
+    double *val64, *dg64, *inv64, *b, *x, *r, *z, *p, *p1, *ap, *y, *s, *t, *rh, *v;
+    float *val32, *dg32, *inv32, *x32, *r32, *z32, *p32, *p32b, *ap32;
     double *partials, *dgc;
     KState *st;
     size_t bytes;
@@ -1079,10 +1191,12 @@ Layout layout(char *base, int64_t C, int64_t nnz, int nrhs, const gf_krylov_cfg 
         L.r32 = w.take<float>(n);
         L.z32 = w.take<float>(n);
         L.p32 = w.take<float>(n);
+        L.p32b = w.take<float>(n);
         L.ap32 = w.take<float>(n);
     } else {
         L.z = w.take<double>(n);
         L.p = w.take<double>(n);
+        if (cfg.method == 0) L.p1 = w.take<double>(n);
         L.ap = w.take<double>(n);
         if (cfg.method == 1) {
             L.y = w.take<double>(n);
@@ -1092,7 +1206,7 @@ Layout layout(char *base, int64_t C, int64_t nnz, int nrhs, const gf_krylov_cfg 
             L.v = L.ap;
         }
     }
-    L.partials = w.take<double>((size_t)2 * kMaxRhs * kNQ * NBMAX);
+    L.partials = w.take<double>((size_t)2 * kMaxRhs * kNQ * partial_stride((int)C));
     L.st = w.take<KState>(2 * kMaxRhs);
     L.bytes = w.off + 256;
     return L;
@@ -1155,42 +1269,48 @@ int finish(const Layout &L, int s, const gf_krylov_cfg &cfg, hipStream_t st, dou
 
 template <int NR>
 int run_cg(Ops o, const Layout &L, const gf_krylov_cfg &cfg, hipStream_t st, double *stats) {
-    const int nb = o.nb;
     Poller P;
     if (int rc = P.init()) return rc;
     const int every = cfg.check_every > 0 ? cfg.check_every : 8;
     int s = 1;  // launch sequence; state[0] holds the zeroed initial state
-    const dim3 G(nb), B(RB), G1(1);
+    const dim3 GS(o.gs), BS(SB), GU(o.gu), BU(RB), G1(1);
+    int first_prev;  // partial sets left for the first k_cg_dir of a chunk
     if (!cfg.mixed) {
-        hipLaunchKernelGGL((k_cg_init<double, NR>), G, B, 0, st, o, s++, L.val64, L.dg64, L.inv64, L.b, L.x, L.r, L.z,
-                           L.p);
-        hipLaunchKernelGGL(k_cg_init_fin<NR>, G1, B, 0, st, o, s++, nb);
+        hipLaunchKernelGGL(k_cg_init<NR>, GS, BS, 0, st, o, s++, L.val64, L.dg64, L.inv64, L.b, L.x, L.r, L.z);
+        hipLaunchKernelGGL(k_cg_init_fin<NR>, G1, BU, 0, st, o, s++, o.gs);
+        first_prev = 0;  // the first direction needs no partials
     } else {
-        hipLaunchKernelGGL(k_mx_residual<NR>, G, B, 0, st, o, s++, 1, L.val64, L.dg64, L.inv32, L.b, L.x, L.r32,
-                           L.z32);
-        hipLaunchKernelGGL(k_mx_refine<NR>, G, B, 0, st, o, s++, 1, L.z32, L.p32, L.x32);
+        hipLaunchKernelGGL(k_mx_residual<NR>, GS, BS, 0, st, o, s++, 1, L.val64, L.dg64, L.inv32, L.b, L.x, L.r32,
+                           L.z32, L.x32);
+        hipLaunchKernelGGL(k_mx_refine<NR>, G1, BU, 0, st, o, s++, 1, o.gs, o.gu);
+        first_prev = 1;
     }
     GF_CHECK_LAUNCH();
     int k = 1;
     const long cap = (long)cfg.max_iter + 4L * every;
     for (;;) {
         for (int j = 0; j < every; ++j, ++k) {
+            const int nb_prev = j == 0 ? first_prev : o.gu;
             if (!cfg.mixed) {
-                hipLaunchKernelGGL((k_cg_spmv<double, NR>), G, B, 0, st, o, s++, L.val64, L.dg64, L.p, L.ap);
-                hipLaunchKernelGGL((k_cg_update<double, NR>), G, B, 0, st, o, s++, L.inv64, L.x, L.r, L.z, L.p, L.ap);
-                hipLaunchKernelGGL((k_cg_pupdate<double, NR>), G, B, 0, st, o, s++, k, 0, L.z, L.p);
+                hipLaunchKernelGGL((k_cg_dir<double, NR, false>), GS, BS, 0, st, o, s++, nb_prev, L.val64, L.dg64,
+                                   L.z, L.p, L.p1, L.ap);
+                hipLaunchKernelGGL((k_cg_upd<double, NR>), GU, BU, 0, st, o, s++, L.inv64, L.x, L.r, L.z, L.p, L.p1,
+                                   L.ap);
             } else {
-                hipLaunchKernelGGL(k_mx_spmv<NR>, G, B, 0, st, o, s++, L.val32, L.dgc, L.p32, L.ap32);
-                hipLaunchKernelGGL((k_cg_update<float, NR>), G, B, 0, st, o, s++, L.inv32, L.x32, L.r32, L.z32, L.p32,
-                                   L.ap32);
-                hipLaunchKernelGGL(k_mx_pupdate<NR>, G, B, 0, st, o, s++, L.z32, L.p32);
+                hipLaunchKernelGGL((k_cg_dir<float, NR, true>), GS, BS, 0, st, o, s++, nb_prev, L.val32, L.dgc,
+                                   L.z32, L.p32, L.p32b, L.ap32);
+                hipLaunchKernelGGL((k_cg_upd<float, NR>), GU, BU, 0, st, o, s++, L.inv32, L.x32, L.r32, L.z32, L.p32,
+                                   L.p32b, L.ap32);
             }
         }
         if (cfg.mixed) {  // FP64 refinement (no-ops unless a component asked for it)
-            hipLaunchKernelGGL(k_mx_accumulate<NR>, G, B, 0, st, o, s++, L.x32, L.x);
-            hipLaunchKernelGGL(k_mx_residual<NR>, G, B, 0, st, o, s++, 0, L.val64, L.dg64, L.inv32, L.b, L.x, L.r32,
-                               L.z32);
-            hipLaunchKernelGGL(k_mx_refine<NR>, G, B, 0, st, o, s++, 0, L.z32, L.p32, L.x32);
+            hipLaunchKernelGGL(k_mx_accumulate<NR>, GU, BU, 0, st, o, s++, L.x32, L.x);
+            hipLaunchKernelGGL(k_mx_residual<NR>, GS, BS, 0, st, o, s++, 0, L.val64, L.dg64, L.inv32, L.b, L.x, L.r32,
+                               L.z32, L.x32);
+            hipLaunchKernelGGL(k_mx_refine<NR>, G1, BU, 0, st, o, s++, 0, o.gs, o.gu);
+            first_prev = 1;
+        } else {
+            first_prev = o.gu;
         }
         GF_CHECK_LAUNCH();
         bool done = false;
@@ -1216,19 +1336,19 @@ int run_bicgstab(Ops o, const Layout &L, const gf_krylov_cfg &cfg, hipStream_t s
     if (int rc = P.init()) return rc;
     const int every = cfg.check_every > 0 ? cfg.check_every : 4;
     int s = 1;
-    const dim3 G(nb), B(RB), G1(1);
-    hipLaunchKernelGGL(k_bi_init<NR>, G, B, 0, st, o, s++, L.val64, L.dg64, L.b, L.x, L.r, L.rh, L.p);
-    hipLaunchKernelGGL(k_bi_init_fin<NR>, G1, B, 0, st, o, s++, nb);
+    const dim3 G(nb), B(RB), G1(1), GS(o.nbs), BS(SB);
+    hipLaunchKernelGGL(k_bi_init<NR>, GS, BS, 0, st, o, s++, L.val64, L.dg64, L.b, L.x, L.r, L.rh, L.p);
+    hipLaunchKernelGGL(k_bi_init_fin<NR>, G1, B, 0, st, o, s++, o.nbs);
     GF_CHECK_LAUNCH();
     int k = 0;
     for (;;) {
         for (int j = 0; j < every; ++j, ++k) {
             hipLaunchKernelGGL(k_bi_restart<NR>, G, B, 0, st, o, s++, L.r, L.rh, L.p);
             hipLaunchKernelGGL(k_bi_y<NR>, G, B, 0, st, o, s++, L.inv64, L.p, L.y);
-            hipLaunchKernelGGL(k_bi_v<NR>, G, B, 0, st, o, s++, L.val64, L.dg64, L.y, L.rh, L.v);
+            hipLaunchKernelGGL(k_bi_v<NR>, GS, BS, 0, st, o, s++, L.val64, L.dg64, L.y, L.rh, L.v);
             hipLaunchKernelGGL(k_bi_s<NR>, G, B, 0, st, o, s++, L.r, L.v, L.s);
             hipLaunchKernelGGL(k_bi_z<NR>, G, B, 0, st, o, s++, L.inv64, L.y, L.s, L.x, L.z);
-            hipLaunchKernelGGL(k_bi_t<NR>, G, B, 0, st, o, s++, L.val64, L.dg64, L.z, L.s, L.t);
+            hipLaunchKernelGGL(k_bi_t<NR>, GS, BS, 0, st, o, s++, L.val64, L.dg64, L.z, L.s, L.t);
             hipLaunchKernelGGL(k_bi_update<NR>, G, B, 0, st, o, s++, L.y, L.z, L.s, L.t, L.rh, L.x, L.r);
             hipLaunchKernelGGL(k_bi_p<NR>, G, B, 0, st, o, s++, L.r, L.v, L.p);
         }
@@ -1267,6 +1387,7 @@ extern "C" int gf_krylov_solve(const gf_mesh *m, const gf_krylov_cfg *cfg, int32
     GF_REQUIRE(nrhs == 1 || nrhs == 3, "nrhs must be 1 or 3");
     GF_REQUIRE(cfg->method == 0 || cfg->method == 1, "unknown Krylov method");
     GF_REQUIRE(!(cfg->method == 1 && cfg->mixed), "mixed precision is implemented for cg only");
+    GF_REQUIRE(!(cfg->mixed && nrhs != 1), "mixed precision is implemented for one right-hand side");
     GF_REQUIRE(cfg->max_iter >= 1, "max_iter must be >= 1");
     GF_REQUIRE(m->n_cells >= 1, "empty grid");
     hipStream_t s = S(stream);
@@ -1297,12 +1418,15 @@ extern "C" int gf_krylov_solve(const gf_mesh *m, const gf_krylov_cfg *cfg, int32
     o.sell_col = m->sell_col;
     o.C = C;
     o.nb = solver_workgroups(C);
+    o.nbs = spmv_workgroups(C);
+    o.gs = pcg_spmv_workgroups(C);
+    o.gu = pcg_vector_workgroups(C);
+    o.pstride = partial_stride(C);
     o.norm_inf = cfg->norm;
     o.max_iter = cfg->max_iter;
     o.max_restart = cfg->max_restart;
     o.atol = cfg->atol;
     o.rtol = cfg->rtol;
-    o.inner_rtol = cfg->inner_rtol;
     o.st = L.st;
     o.partials = L.partials;
     int rc;
@@ -1334,15 +1458,21 @@ extern "C" int gf_sell_pack(const gf_mesh *m, int32_t transpose, const double *u
     return GF_OK;
 }
 
-extern "C" int64_t gf_sell_spmv_scratch_bytes(void) {
-    return (int64_t)(sizeof(double) * 2 * kMaxRhs * kNQ * NBMAX + sizeof(KState) * 2 * kMaxRhs + 1024);
+extern "C" int64_t gf_sell_spmv_scratch_bytes(const gf_mesh *m) {
+    if (!m || m->n_cells < 1) return 0;
+    return (int64_t)(sizeof(double) * 2 * kMaxRhs * kNQ * partial_stride(m->n_cells) + sizeof(KState) * 2 * kMaxRhs +
+                     2 * sizeof(double) * (size_t)m->n_cells + 4 * 256);
 }
 
-// The solver's own SpMV kernel (same code, launch geometry and p.Ap partial
-// as inside PCG), launched standalone so its duration can be timed.
+// Benchmark entry: the PCG direction/SpMV kernel in its steady state with
+// beta = 0, so p = z = x and y = A x; it also writes p to scratch, reads the
+// previous direction (zeros) and reduces the (empty) previous partials, as
+// inside the solver.  Launched `reps` times after one state set-up.
 extern "C" int gf_sell_spmv(const gf_mesh *m, const void *diag, const void *val, const void *x, void *y, int32_t fp32,
-                            void *scratch, gf_stream_t stream) {
+                            int32_t reps, void *scratch, gf_stream_t stream) {
     GF_REQUIRE(m && diag && val && x && y && scratch, "null argument");
+    GF_REQUIRE(m->n_cells >= 1, "empty grid");
+    GF_REQUIRE(reps >= 1, "reps must be >= 1");
     hipStream_t s = S(stream);
     WS w{reinterpret_cast<char *>(scratch), 0};
     Ops o{};
@@ -1350,15 +1480,28 @@ extern "C" int gf_sell_spmv(const gf_mesh *m, const void *diag, const void *val,
     o.sell_col = m->sell_col;
     o.C = m->n_cells;
     o.nb = solver_workgroups(o.C);
-    o.partials = w.take<double>((size_t)2 * kMaxRhs * kNQ * NBMAX);
+    o.nbs = spmv_workgroups(o.C);
+    o.gs = pcg_spmv_workgroups(o.C);
+    o.gu = pcg_vector_workgroups(o.C);
+    o.pstride = partial_stride(o.C);
+    o.partials = w.take<double>((size_t)2 * kMaxRhs * kNQ * o.pstride);
     o.st = w.take<KState>(2 * kMaxRhs);
-    hipLaunchKernelGGL(k_zero_state, dim3(1), dim3(64), 0, s, o.st, 2 * kMaxRhs);  // phase 0: active
-    if (fp32)  // mixed-precision operator: FP32 values and vectors, FP64 diagonal and accumulation
-        hipLaunchKernelGGL(k_mx_spmv<1>, dim3(o.nb), dim3(RB), 0, s, o, 1, (const float *)val, (const double *)diag,
-                           (const float *)x, (float *)y);
-    else
-        hipLaunchKernelGGL((k_cg_spmv<double, 1>), dim3(o.nb), dim3(RB), 0, s, o, 1, (const double *)val,
-                           (const double *)diag, (const double *)x, (double *)y);
+    const size_t vb = fp32 ? sizeof(float) : sizeof(double);
+    char *p0 = reinterpret_cast<char *>(w.take<double>((size_t)o.C));
+    char *p1 = p0 + vb * o.C;
+    GF_CHECK_HIP(hipMemsetAsync(p0, 0, vb * o.C, s));
+    hipLaunchKernelGGL(k_spmv_bench_state, dim3(1), dim3(64), 0, s, o.st, 2 * kMaxRhs);
+    // Every launch uses sequence number 1: it reads state[0] (never written
+    // here) and writes state[1] and partials[1].
+    for (int k = 0; k < reps; ++k) {
+        if (fp32)  // mixed-precision operator: FP32 values and vectors, FP64 diagonal and accumulation
+            hipLaunchKernelGGL((k_cg_dir<float, 1, true>), dim3(o.gs), dim3(SB), 0, s, o, 1, 0, (const float *)val,
+                               (const double *)diag, (const float *)x, (float *)p0, (float *)p1, (float *)y);
+        else
+            hipLaunchKernelGGL((k_cg_dir<double, 1, false>), dim3(o.gs), dim3(SB), 0, s, o, 1, 0,
+                               (const double *)val, (const double *)diag, (const double *)x, (double *)p0,
+                               (double *)p1, (double *)y);
+    }
     GF_CHECK_LAUNCH();
     return GF_OK;
 }

@@ -102,6 +102,8 @@ class LinearSolver:
                       mixed: Optional[bool] = None) -> Tuple[torch.Tensor, Tuple[SolveStats, ...]]:
         g = A.grid
         nrhs = 1 if b.dim() == 1 else b.shape[1]
+        if nrhs > 1:
+            mixed = False  # the FP32 inner solve takes one right-hand side; components solve in FP64
         cfg = self._cfg(mixed)
         ws = self._ws.get(g, nrhs, cfg)
         x = x0.detach().clone().contiguous()

@@ -236,13 +236,15 @@ int gf_krylov_solve(const gf_mesh *m, const gf_krylov_cfg *cfg, int32_t nrhs, in
 /* ldu_grads.py:11-53: gd = -sum lambda x, gu = -sum lambda_O x_N, gl = -sum lambda_N x_O */
 int gf_ldu_grads(const gf_mesh *m, int32_t nc, const double *lambda, const double *x,
                  double *g_diag, double *g_upper, double *g_lower, gf_stream_t stream);
-/* y = A x with the solver's own SpMV kernel on the sliced-ELL operator
- * (benchmark entry; FP64 or FP32 values; scratch of gf_sell_spmv_scratch_bytes()) */
+/* Pack LDU off-diagonals into sliced-ELL order (FP64 or FP32 values). */
 int gf_sell_pack(const gf_mesh *m, int32_t transpose, const double *upper, const double *lower,
                  void *val, int32_t fp32, gf_stream_t stream);
-int64_t gf_sell_spmv_scratch_bytes(void);
+/* y = A x with the solver's own SpMV kernel, `reps` times in a row (benchmark
+ * entry: FP64 operator, or FP32 values/vectors with an FP64 diagonal and FP64
+ * accumulation); scratch of gf_sell_spmv_scratch_bytes(m) bytes */
+int64_t gf_sell_spmv_scratch_bytes(const gf_mesh *m);
 int gf_sell_spmv(const gf_mesh *m, const void *diag, const void *val, const void *x, void *y,
-                 int32_t fp32, void *scratch, gf_stream_t stream);
+                 int32_t fp32, int32_t reps, void *scratch, gf_stream_t stream);
 
 #ifdef __cplusplus
 }
