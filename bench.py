@@ -66,10 +66,40 @@ def run_gpu(mixed: bool, steps: int, warmup: int):
     return grid, alg, dt, p_iters
 
 
+def laplacian_operator(grid, seed: int = 0):
+    """Pressure-type operator for the SpMV measurement: -laplacian(gamma, p)
+    with all-Neumann walls, gamma ~ U(0.5, 1.5) (torch generator seed 0),
+    the reference cell's diagonal doubled as set_reference_value does."""
+    import torch
+
+    from gridfoam_amd.fields import CellField, NeumannBC
+    from gridfoam_amd.fv import fvm
+
+    gen = torch.Generator(device="cpu").manual_seed(seed)
+    gamma = (0.5 + torch.rand(grid.num_cells, generator=gen, dtype=torch.float64)).to(grid.device)
+    P = CellField(grid, "p_spmv", ())
+    P.add_boundary_conditions({k: NeumannBC(0.0) for k in
+                               ("x_minus", "x_plus", "y_minus", "y_plus", "z_minus", "z_plus")})
+    pm = fvm.assemble(P, lap=-1.0, gamma=gamma, corrected=False)
+    pm.diag[0] = 2 * pm.diag[0]
+    return pm
+
+
+def uniform_grid(n: int):
+    """Uniform n^3 octree over the unit cube (no refinement)."""
+    from gridfoam_amd.config import load_config
+    from gridfoam_amd.grid import create_grid
+
+    cfg = _cfg(False)
+    cfg["fluxel"]["root_resolution"] = [n, n, n]
+    cfg["fluxel"]["refinement_regions"] = []
+    return create_grid(load_config(cfg), device="cuda:0")
+
+
 def spmv_roofline(grid, pmat, fp32: bool, launches: int = 200) -> dict:
-    """Average duration of the PCG direction/SpMV kernel (the solver's own
-    kernel, launched on its own by gf_sell_spmv) with HIP events on the
-    stream it is launched on."""
+    """Average duration of the PCG direction/SpMV kernel k_cg_dir (the
+    solver's own kernel, launched on its own by gf_sell_spmv), timed with
+    HIP events on the stream it is launched on."""
     import ctypes
 
     import torch
@@ -81,7 +111,8 @@ def spmv_roofline(grid, pmat, fp32: bool, launches: int = 200) -> dict:
     nat.call("gf_sell_pack", grid.mesh_ref, ctypes.c_int32(0), nat.ptr(pmat._upper), nat.ptr(pmat._lower),
              nat.ptr(val), ctypes.c_int32(int(fp32)), nat.stream())
     diag = pmat._diag.contiguous()  # FP64 in both operators
-    x = torch.randn(grid.num_cells, dtype=dt, device=grid.device)
+    gen = torch.Generator(device="cpu").manual_seed(1)
+    x = torch.randn(grid.num_cells, generator=gen, dtype=torch.float64).to(device=grid.device, dtype=dt)
     y = torch.empty_like(x)
     scratch = torch.empty(int(nat.lib().gf_sell_spmv_scratch_bytes(grid.mesh_ref)), dtype=torch.uint8,
                           device=grid.device)
@@ -98,19 +129,48 @@ def spmv_roofline(grid, pmat, fp32: bool, launches: int = 200) -> dict:
     e1.record(s)
     torch.cuda.synchronize()
     avg_s = e0.elapsed_time(e1) / 1e3 / launches  # + one tiny state set-up kernel per `launches`
+    # check: y = A x against the matrix's own FP64 multiply
+    ref = pmat.multiply(x.double())
+    err = float(torch.linalg.norm(y.double() - ref) / torch.linalg.norm(ref))
     C, F = grid.num_cells, grid.num_internal_faces
-    # Algorithmic bytes of one launch of the PCG direction/SpMV kernel
-    # (k_cg_dir): FP64 diagonal s_d C, off-diagonal values s_v 2F and int32
-    # columns 4 2F, and four vectors of C (z and p_old read, p_new and Ap
-    # written; gathered neighbours counted once).  FP64: 40C + 24F ; mixed
-    # operator (FP32 values/vectors, FP64 diagonal): 24C + 16F.
+    # Algorithmic bytes of one k_cg_dir launch: the symmetric LDU SpMV
+    # (diagonal s_d C, one off-diagonal value per face s_v F, two int32
+    # indices per face 8F, x read and Ap written 2 s_x C) plus the direction
+    # update it carries (p_old read, p_new written: 2 s_x C).
+    # FP64: 40C + 16F ; mixed operator (FP32 values/vectors, FP64 diagonal): 24C + 12F.
     sd, sv, sx = (8, 4, 4) if fp32 else (8, 8, 8)
-    B = sd * C + (sv + 4) * 2 * F + 4 * sx * C
+    B = sd * C + sv * F + 8 * F + 4 * sx * C
     achieved = B / avg_s / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _traffic(fp32),
             "kernel": "k_cg_dir<float, 1, true>" if fp32 else "k_cg_dir<double, 1, false>",
-            "bytes_per_launch": int(B), "avg_us": round(avg_s * 1e6, 2)}
+            "cells": C, "bytes_per_launch": int(B), "avg_us": round(avg_s * 1e6, 2), "rel_err_vs_fp64": err}
+
+
+def stream_copy(a, b) -> None:
+    import ctypes
+
+    from gridfoam_amd import _native as nat
+
+    nat.call("gf_stream_copy", nat.ptr(a), nat.ptr(b), ctypes.c_int64(a.numel() * a.element_size()), nat.stream())
+
+
+def copy_peak(nbytes: int = 1 << 30, launches: int = 50) -> float:
+    """Rate of the library's plain copy kernel (read + write bytes / s) on
+    1 GiB, a measured HBM reference beside the vendor peak."""
+    import torch
+
+    a = torch.ones(nbytes // 8, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    stream_copy(a, b)
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(launches):
+        stream_copy(a, b)
+    e1.record(s)
+    torch.cuda.synchronize()
+    return round(nbytes * launches / (e0.elapsed_time(e1) / 1e3) / 1e9, 1)
 
 
 def _traffic(fp32: bool):
@@ -162,6 +222,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--mode", choices=("both", "plain", "mixed"), default="both")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-large", action="store_true", help="skip the 256^3 SpMV measurement")
+    ap.add_argument("--spmv-only", choices=("fp32", "fp64"), default=None,
+                    help="only launch the SpMV kernel on the cavity mesh (for counter runs) and exit")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only; multi-GPU runs are not part of this build", file=sys.stderr)
@@ -170,6 +233,20 @@ def main() -> int:
     import torch
 
     torch.cuda.set_device(0)
+    if a.spmv_only:
+        from gridfoam_amd.algorithms import create_case
+        from gridfoam_amd.config import load_config
+
+        grid, _ = create_case(load_config(_cfg(False)), device="cuda:0")
+        r = spmv_roofline(grid, laplacian_operator(grid), fp32=a.spmv_only == "fp32", launches=20)
+        # calibration for the HBM counters: k_stream_copy reads 256 MiB and writes 256 MiB
+        x = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            stream_copy(x, y)
+        torch.cuda.synchronize()
+        print(json.dumps({"spmv_only": r}), flush=True)
+        return 0
     out = {}
     for mode in (("plain", "mixed") if a.mode == "both" else (a.mode,)):
         _log(f"{mode}: {a.warmup} warmup + {a.steps} timed steps")
@@ -190,7 +267,8 @@ def main() -> int:
     grid = H["grid"]
     C = grid.num_cells
     value = C * a.steps / H["dt"] / 1e6
-    roof = spmv_roofline(grid, H["alg"].last_p_matrix, fp32=(head == "mixed"))
+    _log("SpMV roofline on the cavity mesh")
+    roof = spmv_roofline(grid, laplacian_operator(grid), fp32=(head == "mixed"))
     line = {
         "metric": "Mcell-steps/s at <=1e-6 rel err vs plain FP64",
         "value": round(value, 3),
@@ -216,6 +294,14 @@ def main() -> int:
         line["mixed"] = {"value": round(C * a.steps / out["mixed"]["dt"] / 1e6, 3),
                          "ms_per_step": round(out["mixed"]["dt"] / a.steps * 1e3, 3),
                          "p_cg_iters_per_step": round(out["mixed"]["iters"] / a.steps, 1)}
+    roof["copy_peak_GBs"] = copy_peak()
+    if not a.no_large:
+        _log("SpMV roofline on a uniform 256^3 mesh (working set above the 256 MB MALL)")
+        big = uniform_grid(256)
+        pm = laplacian_operator(big)
+        line["roofline_256cubed"] = {"fp32": spmv_roofline(big, pm, fp32=True, launches=50),
+                                     "fp64": spmv_roofline(big, pm, fp32=False, launches=50)}
+        del big, pm
     if rel is not None:
         line["mixed_vs_fp64_rel_l2"] = rel
         line["speedup_mixed_vs_fp64"] = round(out["plain"]["dt"] / out["mixed"]["dt"], 3)

@@ -99,6 +99,7 @@ _SIGS: dict[str, tuple[Any, list[Any]]] = {
     "gf_relax_momentum": (_i32, [_i32, _f64, _vp, _vp, _vp, _vp]),
     "gf_predictor_source": (_i32, [_i32, _i32, _vp, _vp, _vp, _vp, _vp]),
     "gf_relax_field": (_i32, [_i64, _f64, _vp, _vp, _vp]),
+    "gf_stream_copy": (_i32, [_vp, _vp, _i64, _vp]),
 }
 
 _LIB: ctypes.CDLL | None = None

@@ -872,6 +872,28 @@ extern "C" int gf_relax_field(int64_t n, double alpha, const double *old, double
     return GF_OK;
 }
 
+// Plain 16-byte-per-lane copy: the HBM reference stream that the benchmark
+// times beside the solver kernels and uses to scale the HBM counters.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(kBlock) void k_stream_copy(const f32x4 *__restrict__ src, f32x4 *__restrict__ dst,
+                                                        int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        dst[i] = __builtin_nontemporal_load(src + i);
+}
+
+extern "C" int gf_stream_copy(const void *src, void *dst, int64_t bytes, gf_stream_t stream) {
+    GF_REQUIRE(src && dst, "null argument");
+    GF_REQUIRE(bytes % 16 == 0 && ((uintptr_t)src | (uintptr_t)dst) % 16 == 0,
+               "gf_stream_copy needs 16-byte aligned buffers and sizes");
+    const int64_t n = bytes / 16;
+    if (n == 0) return GF_OK;
+    const int64_t g = (n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_stream_copy, dim3((int)(g < 65536 ? g : 65536)), dim3(kBlock), 0, S(stream),
+                       (const f32x4 *)src, (f32x4 *)dst, n);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
 extern "C" int gf_abi_sizes(int64_t out[3]) {
     GF_REQUIRE(out, "null argument");
     out[0] = (int64_t)sizeof(gf_mesh);

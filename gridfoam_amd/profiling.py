@@ -1,0 +1,58 @@
+"""Summaries of rocprofv3 output for the profiles/ notes.
+
+``python -m gridfoam_amd.profiling kernels DIR [N]`` prints per-kernel call
+counts and durations from the ``*_kernel_trace.csv`` files under DIR;
+``python -m gridfoam_amd.profiling counters DIR`` prints, per kernel, the
+mean value per dispatch of every counter in the ``*_counter_collection.csv``
+files under DIR (run rocprofv3 with ``--output-format csv``).
+"""
+
+from __future__ import annotations
+
+import collections
+import csv
+import pathlib
+import re
+import sys
+
+
+def _short(name: str) -> str:
+    name = re.sub(r"\(anonymous namespace\)::", "", name)
+    name = re.sub(r"^void ", "", name)
+    return re.sub(r"\(.*$", "", name)[:64]
+
+
+def kernels(root: str, top: int = 25) -> str:
+    agg = collections.defaultdict(list)
+    for f in pathlib.Path(root).rglob("*_kernel_trace.csv"):
+        with open(f, newline="") as fh:
+            for row in csv.DictReader(fh):
+                agg[_short(row["Kernel_Name"])].append(int(row["End_Timestamp"]) - int(row["Start_Timestamp"]))
+    tot = sum(sum(v) for v in agg.values()) or 1
+    lines = [f"{'kernel':64s} {'calls':>7s} {'total_ms':>9s} {'avg_us':>8s} {'min_us':>8s} {'max_us':>8s} {'%':>5s}"]
+    for k, v in sorted(agg.items(), key=lambda kv: -sum(kv[1]))[:top]:
+        lines.append(f"{k:64s} {len(v):7d} {sum(v) / 1e6:9.2f} {sum(v) / len(v) / 1e3:8.2f} {min(v) / 1e3:8.2f} "
+                     f"{max(v) / 1e3:8.2f} {100 * sum(v) / tot:5.1f}")
+    return "\n".join(lines)
+
+
+def counters(root: str) -> str:
+    agg = collections.defaultdict(list)
+    for f in pathlib.Path(root).rglob("*_counter_collection.csv"):
+        with open(f, newline="") as fh:
+            for row in csv.DictReader(fh):
+                agg[(_short(row["Kernel_Name"]), row["Counter_Name"])].append(float(row["Counter_Value"]))
+    lines = [f"{'kernel':64s} {'counter':>12s} {'dispatches':>10s} {'mean/dispatch':>14s}"]
+    for (k, c), v in sorted(agg.items()):
+        lines.append(f"{k:64s} {c:>12s} {len(v):10d} {sum(v) / len(v):14.1f}")
+    return "\n".join(lines)
+
+
+if __name__ == "__main__":
+    mode, root = sys.argv[1], sys.argv[2]
+    if mode == "kernels":
+        print(kernels(root, int(sys.argv[3]) if len(sys.argv) > 3 else 25))
+    elif mode == "counters":
+        print(counters(root))
+    else:
+        raise SystemExit(f"unknown mode {mode!r}")

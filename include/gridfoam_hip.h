@@ -204,6 +204,9 @@ int gf_predictor_source(int32_t n_cells, int32_t nc, const double *source, const
                         const double *x, double *out, gf_stream_t stream);
 /* x = old + alpha (x - old)  (simple.py:386) */
 int gf_relax_field(int64_t n, double alpha, const double *old, double *x, gf_stream_t stream);
+/* dst = src, 16 bytes per lane (no counterpart in the reference: the HBM
+ * reference stream the benchmark times and scales its HBM counters with) */
+int gf_stream_copy(const void *src, void *dst, int64_t bytes, gf_stream_t stream);
 /* out = a*x + b*y elementwise (FvMatrix +/-) */
 int gf_axpby(int64_t n, double a, const double *x, double b, const double *y, double *out,
              gf_stream_t stream);
