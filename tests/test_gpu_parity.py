@@ -357,3 +357,158 @@ def test_piso_octree_cavity_short_run_matches_oracle():
     gpu_imb = np.abs(_np(fvc.div(alg.phi)) * vol).max()
     ora_imb = np.abs(fo.fvc_div(case.g, case.phi) * case.g.cell_volumes).max()
     assert gpu_imb < 10.0 * ora_imb + 1e-14
+
+
+# ----------------------------------------------------------------- golden fixtures
+def _golden(name):
+    import pathlib
+
+    return np.load(pathlib.Path(__file__).parent / "golden" / name)
+
+
+def test_octree_matches_golden():
+    g, _ = _grid("refined3d")
+    d = _golden("octree_refined3d.npz")
+    t = g.topology_host()
+    for k in ("level", "ijk", "owner", "neighbour", "axis", "bnd_owner", "bnd_dir"):
+        np.testing.assert_array_equal(t[k], d[k])
+
+
+def test_simple_cavity2d_500_iterations_match_golden():
+    """64x64 cavity, SIMPLE, the case's 500 outer iterations in plain FP64:
+    U and p within 1e-10 relative L2 of the frozen oracle state, and every
+    pressure solve within one iteration of the oracle's."""
+    from gridfoam_amd.algorithms import create_case
+    from gridfoam_amd.config import load_config
+
+    d = _golden("cavity2d_simple_500.npz")
+    n = int(d["iterations"])
+    _say(f"simple cavity2d {n} its vs golden")
+    grid, alg = create_case(load_config(case_dict("cavity2d_simple.yaml")))
+    its = []
+    for _ in range(n):
+        alg.step()
+        its.append(sum(s.iterations for s in alg.solve_stats["p"]))
+    assert _rel(_np(alg.U.data), d["U"]) < 1e-10
+    assert _rel(_np(alg.p.data), d["p"]) < 1e-10
+    assert np.max(np.abs(np.asarray(its) - d["p_iterations"])) <= 1
+
+
+# ----------------------------------------------------------------- SpMV benchmark entry
+@pytest.mark.parametrize("fp32", [False, True])
+def test_sell_spmv_entry_matches_ldu_multiply(fp32):
+    """gf_sell_spmv runs the solver's direction/SpMV kernel with beta = 0: y = A x."""
+    import ctypes
+
+    from gridfoam_amd import _native as nat
+
+    g, o = _grid("walls8")
+    P, oP, gamma, b = _poisson(g, o)
+    from gridfoam_amd.fv import fvm
+
+    pm = fvm.assemble(P, lap=-1.0, gamma=torch.tensor(gamma, device=g.device), corrected=False)
+    dt = torch.float32 if fp32 else torch.float64
+    _say(f"sell_spmv fp32={fp32}")
+    val = torch.empty(g.mesh.nnz_pad, dtype=dt, device=g.device)
+    nat.call("gf_sell_pack", g.mesh_ref, ctypes.c_int32(0), nat.ptr(pm._upper), nat.ptr(pm._lower), nat.ptr(val),
+             ctypes.c_int32(int(fp32)), nat.stream())
+    x = torch.tensor(np.random.default_rng(1).standard_normal(g.num_cells), device=g.device).to(dt)
+    y = torch.empty_like(x)
+    scratch = torch.empty(int(nat.lib().gf_sell_spmv_scratch_bytes(g.mesh_ref)), dtype=torch.uint8, device=g.device)
+    diag = pm._diag.contiguous()
+    nat.call("gf_sell_spmv", g.mesh_ref, nat.ptr(diag), nat.ptr(val), nat.ptr(x), nat.ptr(y), ctypes.c_int32(int(fp32)),
+             ctypes.c_int32(3), nat.ptr(scratch), nat.stream())
+    ref = _np(pm.multiply(x.double()))
+    assert _rel(_np(y), ref) < (1e-6 if fp32 else 1e-14)
+
+
+# ----------------------------------------------------------------- implicit adjoint
+def _nonsym_scalar_matrix(g, seed=7):
+    from gridfoam_amd.fields import CellField, FaceField, NeumannBC
+    from gridfoam_amd.fv import fvm
+
+    rng = np.random.default_rng(seed)
+    T = CellField(g, "T_adj", ())
+    T.add_boundary_conditions({k: NeumannBC(0.0) for k in
+                               ("x_minus", "x_plus", "y_minus", "y_plus", "z_minus", "z_plus")})
+    T.old_data = T.data.clone()
+    phi = FaceField(g, "phi_adj")
+    phi.single_data = torch.tensor(0.05 * rng.standard_normal(g.num_internal_faces), device=g.device)
+    m = fvm.assemble(T, ddt=1.0, div=1.0, lap=-1.0, phi=phi, gamma=0.01, abc=(1.0, 1.0, 0.0), corrected=False)
+    m.source = torch.tensor(rng.standard_normal(g.num_cells), device=g.device)
+    return T, m
+
+
+def test_transpose_solve_inverts_the_transpose():
+    """solve_transpose (adjoint/attach.py:72-74): A^T lambda = rhs, checked with
+    the LDU transpose multiply (fvmatrix.py:396-412)."""
+    from gridfoam_amd.config import SolverConfig
+    from gridfoam_amd.solvers import create_solver
+
+    g, _ = _grid("walls8")
+    _say("transpose solve")
+    T, m = _nonsym_scalar_matrix(g)
+    rhs = torch.tensor(np.random.default_rng(2).standard_normal(g.num_cells), device=g.device)
+    s = create_solver(SolverConfig(method="bicgstab", preconditioner="jacobi", tolerance=0.0,
+                                   rel_tolerance=1e-13, max_iter=2000))
+    lam = s.solve_transpose(m, rhs)
+    back = m.as_transpose().multiply(lam)
+    assert _rel(_np(back), _np(rhs)) < 1e-11
+    # and it is not the primal solve
+    assert _rel(_np(m.multiply(lam)), _np(rhs)) > 1e-6
+
+
+@pytest.mark.parametrize("method", ["cg", "bicgstab"])
+def test_adjoint_gradient_matches_central_differences(method):
+    """d J / d(diag, upper, lower, source) of J = w . x(A, b) through the
+    implicit adjoint, against central finite differences along random
+    directions (test_linear_solver_adjoint.py:141-191 checks the same
+    gradients with gradcheck)."""
+    from gridfoam_amd.config import SolverConfig
+    from gridfoam_amd.fvmatrix import Equation
+    from gridfoam_amd.solvers import create_solver
+
+    g, o = _grid("walls8")
+    _say(f"adjoint FD {method}")
+    if method == "cg":
+        T, oP, gamma, b = _poisson(g, o)
+        from gridfoam_amd.fv import fvm
+
+        m = fvm.assemble(T, lap=-1.0, gamma=torch.tensor(gamma, device=g.device), corrected=False)
+        m.diag = m.diag + m.diag.abs().mean()  # well-conditioned SPD, so central differences resolve 1e-5
+        m.source = torch.tensor(b, device=g.device)
+    else:
+        T, m = _nonsym_scalar_matrix(g)
+    cfg = SolverConfig(method=method, preconditioner="jacobi", tolerance=0.0, rel_tolerance=1e-13, max_iter=5000)
+    solver = create_solver(cfg)
+    rng = np.random.default_rng(11)
+    w = torch.tensor(rng.standard_normal(g.num_cells), device=g.device)
+    base = {k: getattr(m, k).detach().clone() for k in ("diag", "upper", "lower", "source")}
+
+    def J(params, grad=False):
+        for k, v in params.items():
+            setattr(m, k, v)
+        T.data = torch.zeros_like(T.data)
+        x = solver.solve(Equation(T, m)).solution
+        return (w * x).sum()
+
+    params = {k: v.clone().requires_grad_(True) for k, v in base.items()}
+    Jv = J(params)
+    grads = torch.autograd.grad(Jv, [params[k] for k in base])
+    for k, gk in zip(base, grads):
+        d = torch.tensor(rng.standard_normal(base[k].shape), device=g.device)
+        if method == "cg" and k in ("upper", "lower"):  # keep the operator symmetric for CG
+            d = torch.tensor(rng.standard_normal(base["upper"].shape), device=g.device) if k == "upper" else d
+        h = 1e-6 * float(base[k].abs().max())
+        with torch.no_grad():
+            if method == "cg" and k in ("upper", "lower"):
+                dd = d
+                p_plus = {**base, "upper": base["upper"] + h * dd, "lower": base["lower"] + h * dd}
+                p_minus = {**base, "upper": base["upper"] - h * dd, "lower": base["lower"] - h * dd}
+                analytic = float((grads[1] * dd).sum() + (grads[2] * dd).sum())
+            else:
+                p_plus = {**base, k: base[k] + h * d}
+                p_minus = {**base, k: base[k] - h * d}
+                analytic = float((gk * d).sum())
+            fd = float(J(p_plus) - J(p_minus)) / (2 * h)
+        assert abs(analytic - fd) <= 1e-5 * max(abs(fd), 1e-30), (k, analytic, fd)

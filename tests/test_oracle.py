@@ -307,3 +307,66 @@ def test_simple_cavity_conserves_mass_and_converges():
     div = fo.fvc_div(case.g, case.phi) * case.g.cell_volumes
     assert np.abs(div).max() < 1e-8
     assert case.U.data[:, 0].max() > 0.1
+
+
+# ----------------------------------------------------------------- golden fixtures
+def test_oracle_matches_golden_octree():
+    """tests/golden/octree_refined3d.npz (integer build, bit-exact)."""
+    import pathlib
+
+    d = np.load(pathlib.Path(__file__).parent / "golden" / "octree_refined3d.npz")
+    f = FLUXEL["refined3d"]
+    t = fo.build_octree(f["lower"], f["upper"], f["root"], f["regions"], f["n_leaf"])
+    for k in ("level", "ijk", "owner", "neighbour", "axis", "bnd_owner", "bnd_dir"):
+        np.testing.assert_array_equal(getattr(t, k), d[k])
+
+
+def test_oracle_matches_golden_cavity2d_first_iterations():
+    """First 5 SIMPLE iterations of the 64x64 cavity reproduce the frozen
+    pressure-solve iteration counts (the full 500-iteration state is checked
+    on the GPU path)."""
+    import pathlib
+
+    from tests.cases import case_dict
+
+    d = np.load(pathlib.Path(__file__).parent / "golden" / "cavity2d_simple_500.npz")
+    case = fo.Case(case_dict("cavity2d_simple.yaml"))
+    for k in range(5):
+        case.step()
+        assert sum(st.iterations for st in case.stats["p"]) == d["p_iterations"][k]
+
+
+# ----------------------------------------------------------------- LDU algebra
+def _dense_by_loops(g, m):
+    """Dense matrix from the LDU arrays by explicit loops (independent of Matrix.multiply)."""
+    C = g.num_cells
+    A = np.zeros((C, C))
+    for i in range(C):
+        A[i, i] = m.diag[i]
+    for f, (o, n) in enumerate(zip(g.owner, g.neighbour)):
+        A[o, n] += m.upper[f]
+        A[n, o] += m.lower[f]
+    return A
+
+
+def test_ldu_multiply_transpose_and_h_operators():
+    """fvmatrix.py:242-323, :396-412 against a dense matrix built by loops;
+    transpose identity <A x, y> = <x, A^T y> (test_linear_solver_adjoint.py:121-138)."""
+    g = _grid("refined2d")
+    rng = np.random.default_rng(5)
+    m = fo.Matrix(g)
+    m.diag = 3.0 + rng.random(g.num_cells)
+    m.upper = rng.standard_normal(g.num_faces)
+    m.lower = rng.standard_normal(g.num_faces)
+    m.source = rng.standard_normal(g.num_cells)
+    A = _dense_by_loops(g, m)
+    x, y = rng.standard_normal(g.num_cells), rng.standard_normal(g.num_cells)
+    np.testing.assert_allclose(m.multiply(x), A @ x, rtol=0, atol=1e-12)
+    At = m.transpose()
+    np.testing.assert_allclose(At.multiply(y), A.T @ y, rtol=0, atol=1e-12)
+    assert abs((A @ x) @ y - x @ At.multiply(y)) < 1e-12 * np.linalg.norm(A @ x) * np.linalg.norm(y)
+    V = g.cell_volumes
+    np.testing.assert_allclose(m.A(), np.diag(A) / V, rtol=1e-15)
+    off = A - np.diag(np.diag(A))
+    np.testing.assert_allclose(m.H(x), (m.source - off @ x) / V, rtol=1e-13, atol=1e-12)
+    np.testing.assert_allclose(m.H1(), -off.sum(axis=1) / V, rtol=1e-13, atol=1e-12)
