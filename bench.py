@@ -35,6 +35,9 @@ def _log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+INNER_RTOL = None  # --inner-rtol: refinement trigger of the mixed solve (None: the config default)
+
+
 def _cfg(mixed: bool) -> dict:
     import yaml
 
@@ -43,6 +46,8 @@ def _cfg(mixed: bool) -> dict:
     cfg = yaml.safe_load((CASES / CASE).read_text())
     for k in ("p", "pFinal"):
         cfg["simulator"]["fvSolution"]["solvers"][k]["mixed_precision"] = bool(mixed)
+        if mixed and INNER_RTOL is not None:
+            cfg["simulator"]["fvSolution"]["solvers"][k]["inner_rel_tolerance"] = INNER_RTOL
     return cfg
 
 
@@ -225,7 +230,11 @@ def main() -> int:
     ap.add_argument("--no-large", action="store_true", help="skip the 256^3 SpMV measurement")
     ap.add_argument("--spmv-only", choices=("fp32", "fp64"), default=None,
                     help="only launch the SpMV kernel on the cavity mesh (for counter runs) and exit")
+    ap.add_argument("--inner-rtol", type=float, default=None,
+                    help="refinement trigger of the mixed solve (default: the solver config's inner_rel_tolerance)")
     a = ap.parse_args()
+    global INNER_RTOL
+    INNER_RTOL = a.inner_rtol
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only; multi-GPU runs are not part of this build", file=sys.stderr)
         return 2

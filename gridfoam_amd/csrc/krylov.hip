@@ -19,6 +19,7 @@
 // for all components; values are repacked once per solve into slice-of-64
 // ELL order (one slice = one wavefront, coalesced column/value loads).
 #include <algorithm>
+#include <type_traits>
 #include <cstdlib>
 #include <cmath>
 
@@ -40,17 +41,16 @@ struct KState {
 };
 
 constexpr int kMaxRhs = 3;
-constexpr double kReliable = 0.1;  // mixed PCG: refine when |r| fell 10x since the last true residual
 constexpr int kNQ = 3;        // partial quantities per rhs
 constexpr int RB = 1024;      // threads per workgroup in solver kernels
 constexpr int NBMAX = 1024;   // cap on the vector-kernel workgroup count
 
-// Workgroups used by the solver kernels: 2 per CU on the 256-CU MI355X by
+// Workgroups used by the vector kernels: one per CU on the 256-CU MI355X by
 // default (GRIDFOAM_KRYLOV_WG overrides it for tuning).
 int solver_workgroups(int C) {
     static int cap = [] {
         const char *e = getenv("GRIDFOAM_KRYLOV_WG");
-        int v = e ? atoi(e) : 512;
+        int v = e ? atoi(e) : 256;
         return v < 1 ? 1 : (v > NBMAX ? NBMAX : v);
     }();
     const int need = (C + RB - 1) / RB;
@@ -64,23 +64,34 @@ struct Ops {
     int C;
     int nb;       // BiCGSTAB vector kernels: workgroups (<= NBMAX)
     int nbs;      // BiCGSTAB SpMV kernels: workgroups (ceil(C / SB))
-    int gs;       // PCG SpMV-type kernels: workgroups (ceil(C / SB) rounded up to a multiple of 8)
+    int tb;       // PCG SpMV-type kernels: rows (= threads) per workgroup, 256, 512 or 1024
+    int gs;       // PCG SpMV-type kernels: workgroups (ceil(C / tb) rounded up to a multiple of 8)
     int gu;       // PCG vector kernels: workgroups (a multiple of 8)
     int pstride;  // partials per (rhs, quantity): the largest workgroup count
     int norm_inf, max_iter, max_restart;
     double atol, rtol;
+    double reliable;   // mixed PCG: refine once |r| < reliable * (last true residual)
     KState *st;        // [2][kMaxRhs]
     double *partials;  // [2][kMaxRhs][kNQ][pstride]
 };
 
 inline int round8(int n) { return (n + 7) / 8 * 8; }
 inline int spmv_workgroups(int C) { return (C + SB - 1) / SB; }
-inline int pcg_spmv_workgroups(int C) { return round8(spmv_workgroups(C)); }
+// Rows per workgroup of the PCG SpMV-type kernels (GRIDFOAM_PCG_TILE overrides it for tuning).
+int pcg_tile() {
+    static int tb = [] {
+        const char *e = getenv("GRIDFOAM_PCG_TILE");
+        const int v = e ? atoi(e) : 256;
+        return (v == 512 || v == 1024) ? v : 256;
+    }();
+    return tb;
+}
+inline int pcg_spmv_workgroups(int C) { return round8((C + pcg_tile() - 1) / pcg_tile()); }
 inline int pcg_vector_workgroups(int C) { return round8(solver_workgroups(C)); }
 inline int partial_stride(int C) {
     int m = solver_workgroups(C);
-    for (int v : {spmv_workgroups(C), pcg_spmv_workgroups(C), pcg_vector_workgroups(C)}) m = v > m ? v : m;
-    return m;
+    for (int v : {spmv_workgroups(C), round8(spmv_workgroups(C)), pcg_vector_workgroups(C)}) m = v > m ? v : m;
+    return m;  // pcg_spmv_workgroups(C) <= round8(spmv_workgroups(C)) for every tile size
 }
 
 // Tile of workgroup b in a grid of G (a multiple of 8) workgroups.  The
@@ -324,13 +335,19 @@ __device__ __forceinline__ float dir_fma<float>(float beta, float p_old, float z
     return __builtin_fmaf(beta, p_old, z);
 }
 
+// Entries per row loaded ahead of the partial reduction: six covers every
+// row of a uniform 3-D region and keeps the FP32 kernel within 64 VGPRs.
+constexpr int kHead = 6;
+
 // Convergence test on the previous update's residual, beta and p = z + beta p_old ; Ap ; partial p.Ap.
-// Mixed precision (MX): the test only decides whether the FP64 refinement is due.
+// Mixed precision (MX): the test only decides whether the FP64 refinement is due;
+// the reliable-update trigger is only taken where the host queued a
+// refinement right after this iteration (may_refine), so no launches idle.
 // The row's matrix entries, its own operands and the first batch of gathered
 // neighbours are loaded before the workgroup reduces the previous partials,
 // so those loads are in flight during the reduction.
-template <typename VT, int NR, bool MX>
-__global__ __launch_bounds__(SB) void k_cg_dir(Ops o, int s, int nb_prev, const VT *__restrict__ val,
+template <typename VT, int NR, bool MX, int TB>
+__global__ __launch_bounds__(TB) void k_cg_dir(Ops o, int s, int nb_prev, int may_refine, const VT *__restrict__ val,
                                                const double *__restrict__ dg, const VT *__restrict__ z,
                                                VT *__restrict__ p0, VT *__restrict__ p1, VT *__restrict__ ap) {
     __shared__ KState sh[NR];
@@ -340,7 +357,7 @@ __global__ __launch_bounds__(SB) void k_cg_dir(Ops o, int s, int nb_prev, const 
         store_state<NR>(o, s, sh);
         return;
     }
-    const int i = xcd_tile(blockIdx.x, gridDim.x) * SB + threadIdx.x;
+    const int i = xcd_tile(blockIdx.x, gridDim.x) * TB + threadIdx.x;
     const int ic = i < o.C ? i : o.C - 1;
     const VT *pold[NR];
     VT *pnew[NR];
@@ -360,10 +377,10 @@ __global__ __launch_bounds__(SB) void k_cg_dir(Ops o, int s, int nb_prev, const 
         zi[r] = z[(size_t)r * o.C + ic];
         poi[r] = pold[r][ic];
     }
-    int col[kBatch];
-    VT v[kBatch];
+    int col[kHead];
+    VT v[kHead];
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u) {
+    for (int u = 0; u < kHead; ++u) {
         col[u] = ic;
         v[u] = (VT)0;
         if (u < W) {
@@ -372,9 +389,9 @@ __global__ __launch_bounds__(SB) void k_cg_dir(Ops o, int s, int nb_prev, const 
             v[u] = __builtin_nontemporal_load(val + e);
         }
     }
-    VT zj[kBatch][NR], pj[kBatch][NR];
+    VT zj[kHead][NR], pj[kHead][NR];
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u)
+    for (int u = 0; u < kHead; ++u)
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
             zj[u][r] = z[(size_t)r * o.C + col[u]];
@@ -383,7 +400,7 @@ __global__ __launch_bounds__(SB) void k_cg_dir(Ops o, int s, int nb_prev, const 
     // ---- previous partials: |r|, r.z
     const bool mx[2] = {(bool)o.norm_inf, false};
     double tot[NR][2];
-    reduce_prev<NR, 2, SB>(o, s, nb_prev, mx, tot);
+    reduce_prev<NR, 2, TB>(o, s, nb_prev, mx, tot);
     if (threadIdx.x == 0) {
         for (int r = 0; r < NR; ++r) {
             KState &S = sh[r];
@@ -403,7 +420,7 @@ __global__ __launch_bounds__(SB) void k_cg_dir(Ops o, int s, int nb_prev, const 
                         S.phase = PH_DONE;
                         continue;
                     }
-                } else if (nr < kReliable * S.scale || nr < S.thresh || S.iter >= o.max_iter) {
+                } else if (nr < S.thresh || S.iter >= o.max_iter || (may_refine && nr < o.reliable * S.scale)) {
                     S.phase = PH_RESTART;  // FP64 refinement due
                     continue;
                 }
@@ -435,14 +452,14 @@ __global__ __launch_bounds__(SB) void k_cg_dir(Ops o, int s, int nb_prev, const 
         acc[r] = d * (double)pi[r];
     }
 #pragma unroll
-    for (int u = 0; u < kBatch; ++u)
+    for (int u = 0; u < kHead; ++u)
         if (u < W)
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
                 const VT q = start[r] ? zj[u][r] : dir_fma<VT>(be[r], pj[u][r], zj[u][r]);
                 acc[r] += (double)v[u] * (double)q;
             }
-    for (int kb = kBatch; kb < W; kb += kBatch) {  // rows wider than one batch
+    for (int kb = kHead; kb < W; kb += kBatch) {  // rows wider than the head batch
         int c2[kBatch];
         VT v2[kBatch];
 #pragma unroll
@@ -477,7 +494,7 @@ __global__ __launch_bounds__(SB) void k_cg_dir(Ops o, int s, int nb_prev, const 
         }
     }
     const bool mx1[1] = {false};
-    write_partials<NR, 1, SB>(o, s, pv, mx1);
+    write_partials<NR, 1, TB>(o, s, pv, mx1);
     store_state<NR>(o, s, sh);
 }
 
@@ -569,14 +586,14 @@ __global__ __launch_bounds__(RB) void k_cg_upd(Ops o, int s, const VT *__restric
 }
 
 // r = b - A x ; z = M^-1 r ; partials |r|, r.z  (cg.py:88-118)
-template <int NR>
-__global__ __launch_bounds__(SB) void k_cg_init(Ops o, int s, const double *__restrict__ val,
+template <int NR, int TB>
+__global__ __launch_bounds__(TB) void k_cg_init(Ops o, int s, const double *__restrict__ val,
                                                 const double *__restrict__ dg, const double *__restrict__ inv_d,
                                                 const double *__restrict__ b, const double *__restrict__ x,
                                                 double *__restrict__ r_, double *__restrict__ z_) {
     __shared__ KState sh[NR];
     load_state<NR>(o, s, sh);
-    const int i = xcd_tile(blockIdx.x, gridDim.x) * SB + threadIdx.x;
+    const int i = xcd_tile(blockIdx.x, gridDim.x) * TB + threadIdx.x;
     double v[NR][2] = {};
     if (i < o.C) {
         double ax[NR];
@@ -594,7 +611,7 @@ __global__ __launch_bounds__(SB) void k_cg_init(Ops o, int s, const double *__re
         }
     }
     const bool mx[2] = {(bool)o.norm_inf, false};
-    write_partials<NR, 2, SB>(o, s, v, mx);
+    write_partials<NR, 2, TB>(o, s, v, mx);
     store_state<NR>(o, s, sh);
 }
 
@@ -632,7 +649,7 @@ __global__ __launch_bounds__(RB) void k_cg_init_fin(Ops o, int s, int nb_prev) {
 // the row sum of the FP64 operator.  FP32 x FP32 products are exact in FP64
 // and are accumulated in FP64, so the near-null (constant) mode of a
 // reference-pinned Neumann pressure matrix does not suffer cancellation.
-// When the recursive residual has dropped by `kReliable` since the last
+// When the recursive residual has dropped by `reliable` since the last
 // refinement (or below the threshold, or the budget ran out) the FP32
 // correction is added to the FP64 solution and the FP64 residual b - A x
 // replaces the recursive one; the direction recurrence continues, and
@@ -680,8 +697,8 @@ __global__ __launch_bounds__(RB) void k_mx_accumulate(Ops o, int s, float *__res
 
 // Refinement, part 2: r64 = b - A64 x64 ; r32 = r64 ; z32 = M^-1 r32 ;
 // partials |r64| (norm type), r32.z32.  `first` also starts the solve (x32 = 0).
-template <int NR>
-__global__ __launch_bounds__(SB) void k_mx_residual(Ops o, int s, int first, const double *__restrict__ val,
+template <int NR, int TB>
+__global__ __launch_bounds__(TB) void k_mx_residual(Ops o, int s, int first, const double *__restrict__ val,
                                                     const double *__restrict__ dg, const float *__restrict__ inv_d,
                                                     const double *__restrict__ b, const double *__restrict__ x64,
                                                     float *__restrict__ r32, float *__restrict__ z32,
@@ -696,7 +713,7 @@ __global__ __launch_bounds__(SB) void k_mx_residual(Ops o, int s, int first, con
         any |= act[r];
     }
     if (any) {
-        const int i = xcd_tile(blockIdx.x, gridDim.x) * SB + threadIdx.x;
+        const int i = xcd_tile(blockIdx.x, gridDim.x) * TB + threadIdx.x;
         double v[NR][2] = {};
         if (i < o.C) {
             double ax[NR];
@@ -717,7 +734,7 @@ __global__ __launch_bounds__(SB) void k_mx_residual(Ops o, int s, int first, con
             }
         }
         const bool mx[2] = {(bool)o.norm_inf, false};
-        write_partials<NR, 2, SB>(o, s, v, mx);
+        write_partials<NR, 2, TB>(o, s, v, mx);
     }
     store_state<NR>(o, s, sh);
 }
@@ -1267,20 +1284,20 @@ int finish(const Layout &L, int s, const gf_krylov_cfg &cfg, hipStream_t st, dou
     return GF_OK;
 }
 
-template <int NR>
+template <int NR, int TB>
 int run_cg(Ops o, const Layout &L, const gf_krylov_cfg &cfg, hipStream_t st, double *stats) {
     Poller P;
     if (int rc = P.init()) return rc;
     const int every = cfg.check_every > 0 ? cfg.check_every : 8;
     int s = 1;  // launch sequence; state[0] holds the zeroed initial state
-    const dim3 GS(o.gs), BS(SB), GU(o.gu), BU(RB), G1(1);
+    const dim3 GS(o.gs), BS(TB), GU(o.gu), BU(RB), G1(1);
     int first_prev;  // partial sets left for the first k_cg_dir of a chunk
     if (!cfg.mixed) {
-        hipLaunchKernelGGL(k_cg_init<NR>, GS, BS, 0, st, o, s++, L.val64, L.dg64, L.inv64, L.b, L.x, L.r, L.z);
+        hipLaunchKernelGGL((k_cg_init<NR, TB>), GS, BS, 0, st, o, s++, L.val64, L.dg64, L.inv64, L.b, L.x, L.r, L.z);
         hipLaunchKernelGGL(k_cg_init_fin<NR>, G1, BU, 0, st, o, s++, o.gs);
         first_prev = 0;  // the first direction needs no partials
     } else {
-        hipLaunchKernelGGL(k_mx_residual<NR>, GS, BS, 0, st, o, s++, 1, L.val64, L.dg64, L.inv32, L.b, L.x, L.r32,
+        hipLaunchKernelGGL((k_mx_residual<NR, TB>), GS, BS, 0, st, o, s++, 1, L.val64, L.dg64, L.inv32, L.b, L.x, L.r32,
                            L.z32, L.x32);
         hipLaunchKernelGGL(k_mx_refine<NR>, G1, BU, 0, st, o, s++, 1, o.gs, o.gu);
         first_prev = 1;
@@ -1292,12 +1309,13 @@ int run_cg(Ops o, const Layout &L, const gf_krylov_cfg &cfg, hipStream_t st, dou
         for (int j = 0; j < every; ++j, ++k) {
             const int nb_prev = j == 0 ? first_prev : o.gu;
             if (!cfg.mixed) {
-                hipLaunchKernelGGL((k_cg_dir<double, NR, false>), GS, BS, 0, st, o, s++, nb_prev, L.val64, L.dg64,
+                hipLaunchKernelGGL((k_cg_dir<double, NR, false, TB>), GS, BS, 0, st, o, s++, nb_prev, 0, L.val64, L.dg64,
                                    L.z, L.p, L.p1, L.ap);
                 hipLaunchKernelGGL((k_cg_upd<double, NR>), GU, BU, 0, st, o, s++, L.inv64, L.x, L.r, L.z, L.p, L.p1,
                                    L.ap);
             } else {
-                hipLaunchKernelGGL((k_cg_dir<float, NR, true>), GS, BS, 0, st, o, s++, nb_prev, L.val32, L.dgc,
+                hipLaunchKernelGGL((k_cg_dir<float, NR, true, TB>), GS, BS, 0, st, o, s++, nb_prev, j == every - 1,
+                                   L.val32, L.dgc,
                                    L.z32, L.p32, L.p32b, L.ap32);
                 hipLaunchKernelGGL((k_cg_upd<float, NR>), GU, BU, 0, st, o, s++, L.inv32, L.x32, L.r32, L.z32, L.p32,
                                    L.p32b, L.ap32);
@@ -1305,7 +1323,7 @@ int run_cg(Ops o, const Layout &L, const gf_krylov_cfg &cfg, hipStream_t st, dou
         }
         if (cfg.mixed) {  // FP64 refinement (no-ops unless a component asked for it)
             hipLaunchKernelGGL(k_mx_accumulate<NR>, GU, BU, 0, st, o, s++, L.x32, L.x);
-            hipLaunchKernelGGL(k_mx_residual<NR>, GS, BS, 0, st, o, s++, 0, L.val64, L.dg64, L.inv32, L.b, L.x, L.r32,
+            hipLaunchKernelGGL((k_mx_residual<NR, TB>), GS, BS, 0, st, o, s++, 0, L.val64, L.dg64, L.inv32, L.b, L.x, L.r32,
                                L.z32, L.x32);
             hipLaunchKernelGGL(k_mx_refine<NR>, G1, BU, 0, st, o, s++, 0, o.gs, o.gu);
             first_prev = 1;
@@ -1388,6 +1406,8 @@ extern "C" int gf_krylov_solve(const gf_mesh *m, const gf_krylov_cfg *cfg, int32
     GF_REQUIRE(cfg->method == 0 || cfg->method == 1, "unknown Krylov method");
     GF_REQUIRE(!(cfg->method == 1 && cfg->mixed), "mixed precision is implemented for cg only");
     GF_REQUIRE(!(cfg->mixed && nrhs != 1), "mixed precision is implemented for one right-hand side");
+    GF_REQUIRE(!cfg->mixed || (cfg->inner_rtol > 0.0 && cfg->inner_rtol < 1.0),
+               "mixed precision needs 0 < inner_rtol < 1 (refinement trigger)");
     GF_REQUIRE(cfg->max_iter >= 1, "max_iter must be >= 1");
     GF_REQUIRE(m->n_cells >= 1, "empty grid");
     hipStream_t s = S(stream);
@@ -1419,6 +1439,7 @@ extern "C" int gf_krylov_solve(const gf_mesh *m, const gf_krylov_cfg *cfg, int32
     o.C = C;
     o.nb = solver_workgroups(C);
     o.nbs = spmv_workgroups(C);
+    o.tb = pcg_tile();
     o.gs = pcg_spmv_workgroups(C);
     o.gu = pcg_vector_workgroups(C);
     o.pstride = partial_stride(C);
@@ -1427,11 +1448,24 @@ extern "C" int gf_krylov_solve(const gf_mesh *m, const gf_krylov_cfg *cfg, int32
     o.max_restart = cfg->max_restart;
     o.atol = cfg->atol;
     o.rtol = cfg->rtol;
+    o.reliable = cfg->inner_rtol;
     o.st = L.st;
     o.partials = L.partials;
     int rc;
-    if (cfg->method == 0)
-        rc = nrhs == 1 ? run_cg<1>(o, L, *cfg, s, stats_out) : run_cg<3>(o, L, *cfg, s, stats_out);
+    if (cfg->method == 0) {
+        auto cg = [&](auto nr, auto tb) { return run_cg<decltype(nr)::value, decltype(tb)::value>(o, L, *cfg, s, stats_out); };
+        using I1 = std::integral_constant<int, 1>;
+        using I3 = std::integral_constant<int, 3>;
+        using T256 = std::integral_constant<int, 256>;
+        using T512 = std::integral_constant<int, 512>;
+        using T1024 = std::integral_constant<int, 1024>;
+        if (o.tb == 1024)
+            rc = nrhs == 1 ? cg(I1{}, T1024{}) : cg(I3{}, T1024{});
+        else if (o.tb == 512)
+            rc = nrhs == 1 ? cg(I1{}, T512{}) : cg(I3{}, T512{});
+        else
+            rc = nrhs == 1 ? cg(I1{}, T256{}) : cg(I3{}, T256{});
+    }
     else
         rc = nrhs == 1 ? run_bicgstab<1>(o, L, *cfg, s, stats_out) : run_bicgstab<3>(o, L, *cfg, s, stats_out);
     if (rc != GF_OK) return rc;
@@ -1481,6 +1515,7 @@ extern "C" int gf_sell_spmv(const gf_mesh *m, const void *diag, const void *val,
     o.C = m->n_cells;
     o.nb = solver_workgroups(o.C);
     o.nbs = spmv_workgroups(o.C);
+    o.tb = pcg_tile();
     o.gs = pcg_spmv_workgroups(o.C);
     o.gu = pcg_vector_workgroups(o.C);
     o.pstride = partial_stride(o.C);
@@ -1493,15 +1528,25 @@ extern "C" int gf_sell_spmv(const gf_mesh *m, const void *diag, const void *val,
     hipLaunchKernelGGL(k_spmv_bench_state, dim3(1), dim3(64), 0, s, o.st, 2 * kMaxRhs);
     // Every launch uses sequence number 1: it reads state[0] (never written
     // here) and writes state[1] and partials[1].
-    for (int k = 0; k < reps; ++k) {
-        if (fp32)  // mixed-precision operator: FP32 values and vectors, FP64 diagonal and accumulation
-            hipLaunchKernelGGL((k_cg_dir<float, 1, true>), dim3(o.gs), dim3(SB), 0, s, o, 1, 0, (const float *)val,
-                               (const double *)diag, (const float *)x, (float *)p0, (float *)p1, (float *)y);
-        else
-            hipLaunchKernelGGL((k_cg_dir<double, 1, false>), dim3(o.gs), dim3(SB), 0, s, o, 1, 0,
-                               (const double *)val, (const double *)diag, (const double *)x, (double *)p0,
-                               (double *)p1, (double *)y);
-    }
+    auto launch = [&](auto tbc) {
+        constexpr int TB = decltype(tbc)::value;
+        for (int k = 0; k < reps; ++k) {
+            if (fp32)  // mixed-precision operator: FP32 values and vectors, FP64 diagonal and accumulation
+                hipLaunchKernelGGL((k_cg_dir<float, 1, true, TB>), dim3(o.gs), dim3(TB), 0, s, o, 1, 0, 0,
+                                   (const float *)val, (const double *)diag, (const float *)x, (float *)p0,
+                                   (float *)p1, (float *)y);
+            else
+                hipLaunchKernelGGL((k_cg_dir<double, 1, false, TB>), dim3(o.gs), dim3(TB), 0, s, o, 1, 0, 0,
+                                   (const double *)val, (const double *)diag, (const double *)x, (double *)p0,
+                                   (double *)p1, (double *)y);
+        }
+    };
+    if (o.tb == 1024)
+        launch(std::integral_constant<int, 1024>{});
+    else if (o.tb == 512)
+        launch(std::integral_constant<int, 512>{});
+    else
+        launch(std::integral_constant<int, 256>{});
     GF_CHECK_LAUNCH();
     return GF_OK;
 }

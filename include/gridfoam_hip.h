@@ -227,7 +227,8 @@ typedef struct gf_krylov_cfg {
     int32_t max_iter, max_restart;
     double atol, rtol;
     int32_t mixed;         /* cg only: FP32 inner solve inside FP64 refinement */
-    double inner_rtol;     /* mixed: inner relative reduction per sweep */
+    double inner_rtol;     /* mixed: the FP64 refinement runs once the FP32 recursive residual
+                              is below inner_rtol x the last true residual (0 < inner_rtol < 1) */
     int32_t check_every;   /* host polls the device state every k iterations */
 } gf_krylov_cfg;
 /* workspace bytes needed for (C, nrhs, cfg); caller allocates once */
