@@ -97,6 +97,8 @@ _SIGS: dict[str, tuple[Any, list[Any]]] = {
     "gf_sell_spmv_scratch_bytes": (_i64, [_P(GfMesh)]),
     "gf_sell_spmv": (_i32, [_P(GfMesh), _vp, _vp, _vp, _vp, _i32, _i32, _vp, _vp]),
     "gf_relax_momentum": (_i32, [_i32, _f64, _vp, _vp, _vp, _vp]),
+    "gf_ldu_h1": (_i32, [_P(GfMesh), _vp, _vp, _vp, _vp]),
+    "gf_simplec_rautu": (_i32, [_i32, _vp, _vp, _i32, _vp, _vp]),
     "gf_predictor_source": (_i32, [_i32, _i32, _vp, _vp, _vp, _vp, _vp]),
     "gf_relax_field": (_i32, [_i64, _f64, _vp, _vp, _vp]),
     "gf_stream_copy": (_i32, [_vp, _vp, _i64, _vp]),

@@ -1,7 +1,8 @@
-"""SIMPLE / PISO pressure-velocity drivers on the HIP path.
+"""SIMPLE / PISO / PIMPLE pressure-velocity drivers on the HIP path.
 
 The drivers stay host Python, as in the reference (src/gridfoam/algorithms/
-simple.py:287-408, piso.py:227-326, utils/pressure_correction.py:68-363),
+simple.py:287-408, piso.py:227-326, pimple.py:290-454,
+utils/pressure_correction.py:68-363),
 and call only native kernels for the per-step work: fused momentum
 assembly, batched 3-component BiCGSTAB, rAU/HbyA, interpolation and flux
 assembly, adjustPhi, pressure assembly with the hanging-face correction,
@@ -18,9 +19,9 @@ from typing import Dict, Optional, Tuple
 import torch
 
 from . import _native as nat
-from .config import PISOAlgorithm, SIMPLEAlgorithm
+from .config import PIMPLEAlgorithm, PISOAlgorithm, SIMPLEAlgorithm
 from .fields import CellField, DirichletBC, FaceField
-from .fv import ddt_coefficients, fvc, fvm
+from .fv import ddt_coefficients, fvc, fvm, sn_grad_corrected
 from .fvmatrix import Equation, FvMatrix
 from .grid import OctreeGrid, create_grid
 from .solvers import LinearSolver, SolveStats, create_solver, resolve_solver
@@ -138,11 +139,71 @@ def correct_phi(phi: FaceField, phi_hbya: FaceField, p_eqn: FvMatrix, p: CellFie
              nat.ptr(phi.single_buffer), nat.ptr(phi.bnd_buffer), nat.stream())
 
 
+def simplec_rAtU(UEqn: FvMatrix, rAU: CellField, *, bounded: bool = True) -> torch.Tensor:
+    """pressure_correction.py:163-193: 1/max(1/rAU - H1, 0.1/rAU)."""
+    h1 = UEqn.H1()
+    out = torch.empty_like(rAU.data)
+    nat.call("gf_simplec_rautu", ctypes.c_int32(rAU.grid.num_cells), nat.ptr(rAU.data), nat.ptr(h1),
+             ctypes.c_int32(int(bounded)), nat.ptr(out), nat.stream())
+    return out
+
+
+def apply_simplec(phi_hbya: FaceField, HbyA: CellField, p: CellField, rAU: CellField, rAtU: CellField) -> None:
+    """pressure_correction.py:222-267: phiHbyA += (rAtU-rAU)_f |Sf| snGrad(p) on
+    every face (boundary faces with the pressure condition's normal
+    gradient) and HbyA -= (rAU - rAtU) grad(p).  The face term is the flux
+    of laplacian(rAtU - rAU, p), added by the same kernel that corrects phi."""
+    g = p.grid
+    d_r = CellField(g, "rAtU-rAU")
+    d_r.data = rAtU.data - rAU.data
+    m = fvm.assemble(p, lap=1.0, gamma=d_r.data, corrected=sn_grad_corrected(g, p.name))
+    neg = CellField(g, "rAU-rAtU")
+    neg.data = -d_r.data
+    correct_phi(phi_hbya, phi_hbya, m, p, neg)
+    gp = fvc.grad(p)
+    nat.call("gf_correct_velocity", ctypes.c_int32(g.num_cells), nat.ptr(HbyA.data), nat.ptr(neg.data), nat.ptr(gp),
+             nat.ptr(HbyA.data), nat.stream())
+
+
 def correct_velocity(U: CellField, HbyA: CellField, rAtU: CellField, p: CellField) -> None:
     """pressure_correction.py:338-363."""
     gp = fvc.grad(p)
     nat.call("gf_correct_velocity", ctypes.c_int32(U.grid.num_cells), nat.ptr(HbyA.data), nat.ptr(rAtU.data),
              nat.ptr(gp), nat.ptr(U.data), nat.stream())
+
+
+def continuity_residual(phi: FaceField) -> float:
+    """algorithms/utils/residual.py:39-53: |div(phi)|_2."""
+    return float(torch.linalg.vector_norm(fvc.div(phi)))
+
+
+def residual_satisfied(residual: float, tolerance: float, rel_tolerance: float, initial: float) -> bool:
+    """algorithms/utils/residual.py:56-87."""
+    return residual < tolerance or (rel_tolerance > 0.0 and residual < rel_tolerance * initial)
+
+
+def normalize_residual_control(value: dict) -> Dict[str, Tuple[float, float]]:
+    """meta/config.py:364-388: a float is an absolute tolerance."""
+    out = {}
+    for name, entry in (value or {}).items():
+        if isinstance(entry, (int, float)):
+            out[name] = (float(entry), 0.0)
+        else:
+            out[name] = (float(entry["tolerance"]), float(entry.get("rel_tolerance", 0.0)))
+    return out
+
+
+def _predictor(UEqn: FvMatrix, U: CellField, grad_p: torch.Tensor) -> FvMatrix:
+    """UEqn.with_source(source - V grad p): shares the LDU arrays (fvmatrix.py:213-240)."""
+    g = U.grid
+    pred = FvMatrix.__new__(FvMatrix)
+    pred.field, pred.grid = U, g
+    pred._diag, pred._upper, pred._lower = UEqn._diag, UEqn._upper, UEqn._lower
+    pred.face_flux_correction = None
+    pred._source = torch.empty_like(UEqn._source)
+    nat.call("gf_predictor_source", ctypes.c_int32(g.num_cells), ctypes.c_int32(3), nat.ptr(UEqn._source),
+             nat.ptr(g.cell_volumes), nat.ptr(grad_p), nat.ptr(pred._source), nat.stream())
+    return pred
 
 
 class AlgorithmBase:
@@ -157,7 +218,8 @@ class AlgorithmBase:
         self.phi = FaceField(grid, "phi")
         self.phi_hbya = FaceField(grid, "phiHbyA")
         self.rAU = CellField(grid, "rAU")
-        self.rAtU = self.rAU
+        self.rAtU = self.rAU  # a field of its own with the SIMPLEC (consistent) formulation
+        self.consistent = False
         self.HbyA = CellField(grid, "HbyA", (3,))
         self._solvers: Dict[str, LinearSolver] = {k: create_solver(v) for k, v in cfg.fvSolution.solvers.items()}
         self.nu = float(cfg.properties.transport.nu)
@@ -200,6 +262,9 @@ class AlgorithmBase:
             compute_phi_hbya(self.phi_hbya, self.HbyA, self.U)
         if self.adjust_phi_enabled:
             adjust_phi(self.phi_hbya, self.U, self.p)
+        if self.consistent:  # SIMPLEC: rAtU = 1/max(1/rAU - H1, 0.1/rAU), consistent phiHbyA / HbyA
+            self.rAtU.data = simplec_rAtU(UEqn, self.rAU)
+            apply_simplec(self.phi_hbya, self.HbyA, self.p, self.rAU, self.rAtU)
         div_phi_hbya = fvc.div(self.phi_hbya)
         p_solver = resolve_solver(self.solvers, self.p.name, is_final=final) if transient else self.solvers["p"]
         res = solve_pressure_poisson(self.p, self.rAtU, div_phi_hbya, self.solvers["p"],
@@ -214,14 +279,15 @@ class AlgorithmBase:
 
 
 class SIMPLE(AlgorithmBase):
-    """algorithms/simple.py:141-408 (steady; consistent=False)."""
+    """algorithms/simple.py:141-408 (steady; SIMPLEC with consistent: true)."""
 
     def __init__(self, grid: OctreeGrid):
         super().__init__(grid, transient=False)
         alg = grid.sim_config.fvSolution.algorithm
         assert isinstance(alg, SIMPLEAlgorithm)
         if alg.consistent:
-            raise ValueError("SIMPLEC (consistent: true) is not part of this build")
+            self.consistent = True
+            self.rAtU = CellField(grid, "rAtU")
         self.alpha_U = alg.relaxationFactors.equations["U"]
         self.alpha_p = alg.relaxationFactors.equations["p"]
         correct_flux(self.phi, self.U, update_internal=True)
@@ -285,6 +351,85 @@ class PISO(AlgorithmBase):
         self.phi.update_history()
 
 
+class PIMPLE(AlgorithmBase):
+    """algorithms/pimple.py:136-454 (laminar): outer correctors around the
+    PISO loop; the outer loop stops one (final) pass after residualControl
+    is met; SIMPLEC with consistent: true."""
+
+    def __init__(self, grid: OctreeGrid):
+        super().__init__(grid, transient=True)
+        alg = grid.sim_config.fvSolution.algorithm
+        assert isinstance(alg, PIMPLEAlgorithm)
+        self.n_outer_correctors = alg.nOuterCorrectors
+        self.n_correctors = alg.nCorrectors
+        if alg.consistent:
+            self.consistent = True
+            self.rAtU = CellField(grid, "rAtU")
+        self._residual_control = normalize_residual_control(alg.residualControl)
+        self._initial_residuals: Dict[str, float] = {}
+        self._current_residuals: Dict[str, float] = {}
+        self._outer_converged = False
+        self.outer_iterations = 0
+        correct_flux(self.phi, self.U, update_internal=True)
+        self.U.update_history(reset=True)
+        self.phi.update_history(reset=True)
+
+    def has_converged(self) -> bool:
+        """pimple.py:246-257."""
+        return self._outer_converged
+
+    def _record_residual(self, name: str, residual) -> None:
+        r = float(residual)
+        self._initial_residuals.setdefault(name, r)
+        self._current_residuals[name] = r
+
+    def _check_outer_convergence(self, *, allow_relative: bool = True) -> bool:
+        """pimple.py:269-287."""
+        if not self._residual_control:
+            return False
+        for name, (tol, rtol) in self._residual_control.items():
+            res, init = self._current_residuals.get(name), self._initial_residuals.get(name)
+            if res is None or init is None:
+                return False
+            if not residual_satisfied(res, tol, rtol if allow_relative else 0.0, init):
+                return False
+        return True
+
+    def step(self) -> None:
+        g, U, p = self.grid, self.U, self.p
+        abc = ddt_coefficients(U)
+        self._outer_converged = False
+        self._initial_residuals.clear()
+        self._current_residuals.clear()
+        for outer in range(self.n_outer_correctors):
+            final_after = 0 < outer < self.n_outer_correctors - 1 and \
+                self._check_outer_convergence(allow_relative=outer > 1)
+            if final_after:
+                self._outer_converged = True
+            UEqn = fvm.assemble(U, ddt=1.0, div=1.0, lap=-1.0, phi=self.phi, gamma=self.nu, abc=abc)
+            pred = _predictor(UEqn, U, fvc.grad(p))
+            if U.name in self._residual_control:
+                self._record_residual(U.name, field_initial_residual(pred, U))
+            res = self.solvers["U"].solve(Equation(U, pred))
+            U.data = res.solution
+            self.solve_stats[U.name] = res.stats
+            for i in range(self.n_correctors):
+                pres = self._pressure_corrector(UEqn, transient=True, abc=abc, final=i == self.n_correctors - 1)
+                if p.name in self._residual_control:
+                    if i == 0:
+                        self._record_residual(p.name, pres.initial_residual)
+                    self._current_residuals[p.name] = float(pres.last_initial_residual)
+                correct_phi(self.phi, self.phi_hbya, pres.matrix, p, self.rAtU)
+                correct_velocity(U, self.HbyA, self.rAtU, p)
+            if "phi" in self._residual_control:
+                self._record_residual("phi", continuity_residual(self.phi))
+            self.outer_iterations = outer + 1
+            if final_after:
+                break
+        U.update_history()
+        self.phi.update_history()
+
+
 def create_algorithm(grid: OctreeGrid):
     """algorithms/factory.py:9-38."""
     alg = grid.sim_config.fvSolution.algorithm
@@ -292,6 +437,8 @@ def create_algorithm(grid: OctreeGrid):
         return SIMPLE(grid)
     if isinstance(alg, PISOAlgorithm):
         return PISO(grid)
+    if isinstance(alg, PIMPLEAlgorithm):
+        return PIMPLE(grid)
     raise ValueError(f"unsupported algorithm {alg}")
 
 

@@ -187,7 +187,7 @@ class SolverConfig(BaseModel, frozen=True):  # meta/config.py:269-316
     max_restart: int = 5
     log_interval: int = 25
     mixed_precision: bool = False
-    inner_rel_tolerance: float = 0.3  # mixed: refine after this drop of the FP32 residual
+    inner_rel_tolerance: float = 0.5  # mixed: refine after this drop of the FP32 residual
 
 
 class RelaxationFactorsConfig(BaseModel, frozen=True):
@@ -212,8 +212,19 @@ class PISOAlgorithm(BaseModel, frozen=True):
     pRefValue: Optional[float] = None
 
 
+class PIMPLEAlgorithm(BaseModel, frozen=True):  # meta/config.py:467-515
+    type: AlgorithmType = AlgorithmType.PIMPLE
+    nNonOrthogonalCorrectors: int = Field(default=0, ge=0)
+    nCorrectors: int = Field(default=2, ge=1)
+    nOuterCorrectors: int = Field(default=1, ge=1)
+    residualControl: Dict[str, Any] = Field(default_factory=dict)
+    consistent: bool = False
+    pRefCell: Optional[int] = Field(default=None, ge=0)
+    pRefValue: Optional[float] = None
+
+
 class fvSolutionConfig(BaseModel, frozen=True):
-    algorithm: Union[SIMPLEAlgorithm, PISOAlgorithm]
+    algorithm: Union[SIMPLEAlgorithm, PISOAlgorithm, PIMPLEAlgorithm]
     solvers: Dict[str, SolverConfig] = Field(default_factory=dict)
     adjustPhi: bool = True
 
@@ -226,6 +237,8 @@ class fvSolutionConfig(BaseModel, frozen=True):
                 return SIMPLEAlgorithm(**v)
             if kind == "piso":
                 return PISOAlgorithm(**v)
+            if kind == "pimple":
+                return PIMPLEAlgorithm(**v)
             raise ValueError(f"unsupported algorithm type {kind!r}")
         return v
 

@@ -499,6 +499,30 @@ __global__ void k_correct_velocity(int C, const double *HbyA, const double *rAtU
     for (int q = 0; q < 3; ++q) U[3 * c + q] = HbyA[3 * c + q] - r * gp[3 * c + q];
 }
 
+// H1 = -(sum of the row's off-diagonals) / V, gathered per cell over its face
+// lists in a fixed order (fvmatrix.py:250-264).
+__global__ void k_ldu_h1(const gf_mesh m, const double *__restrict__ upper, const double *__restrict__ lower,
+                         double *__restrict__ h1) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.n_cells) return;
+    double a = 0.0;
+    for (int f = m.own_row[c]; f < m.own_row[c + 1]; ++f) a -= upper[f];
+    for (int k = m.nbr_row[c]; k < m.nbr_row[c + 1]; ++k) a -= lower[m.nbr_face[k]];
+    h1[c] = a / m.volumes[c];
+}
+
+// SIMPLEC coefficient rAtU = 1 / (1/rAU - H1), bounded by 0.1/rAU
+// (pressure_correction.py:163-193).
+__global__ void k_simplec_rautu(int C, const double *__restrict__ rAU, const double *__restrict__ h1, int bounded,
+                                double *__restrict__ rAtU) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    const double ir = 1.0 / rAU[c];
+    double inv = ir - h1[c];
+    if (bounded) inv = fmax(inv, 0.1 * ir);
+    rAtU[c] = 1.0 / inv;
+}
+
 __global__ void k_set_reference(double *diag, double *source, int cell, double value) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const double rd = diag[cell];
@@ -804,6 +828,23 @@ extern "C" int gf_correct_velocity(int32_t n_cells, const double *HbyA, const do
     GF_REQUIRE(HbyA && rAtU && grad_p && U, "null argument");
     hipLaunchKernelGGL(k_correct_velocity, dim3(grid_for(n_cells)), dim3(kBlock), 0, S(stream), n_cells, HbyA, rAtU,
                        grad_p, U);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
+extern "C" int gf_ldu_h1(const gf_mesh *m, const double *upper, const double *lower, double *h1,
+                         gf_stream_t stream) {
+    GF_REQUIRE(m && upper && lower && h1, "null argument");
+    hipLaunchKernelGGL(k_ldu_h1, dim3(grid_for(m->n_cells)), dim3(kBlock), 0, S(stream), *m, upper, lower, h1);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
+extern "C" int gf_simplec_rautu(int32_t n_cells, const double *rAU, const double *h1, int32_t bounded, double *rAtU,
+                                gf_stream_t stream) {
+    GF_REQUIRE(rAU && h1 && rAtU, "null argument");
+    hipLaunchKernelGGL(k_simplec_rautu, dim3(grid_for(n_cells)), dim3(kBlock), 0, S(stream), n_cells, rAU, h1,
+                       bounded, rAtU);
     GF_CHECK_LAUNCH();
     return GF_OK;
 }

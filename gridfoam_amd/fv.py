@@ -17,7 +17,7 @@ from typing import Optional, Tuple, Union
 import torch
 
 from . import _native as nat
-from .config import DdtScheme, DivScheme, GradScheme, LaplacianScheme
+from .config import DdtScheme, DivScheme, GradScheme, LaplacianScheme, SnGradScheme
 from .fields import CellField, FaceField
 from .fvmatrix import FvMatrix
 
@@ -43,6 +43,13 @@ def div_scheme(grid, phi_name: str, field_name: str) -> DivScheme:
     if s not in (DivScheme.UPWIND, DivScheme.LINEAR):
         raise ValueError(f"div scheme {s} is not supported by this build")
     return s
+
+
+def sn_grad_corrected(grid, field_name: str) -> bool:
+    """fv/schemes/selection.py:94-102 (default: corrected)."""
+    s = _lookup(grid.sim_config.fvSchemes.snGradSchemes if grid.sim_config else None,
+                f"snGrad({field_name})", SnGradScheme.CORRECTED)
+    return SnGradScheme(s) == SnGradScheme.CORRECTED
 
 
 def laplacian_corrected(grid, field_name: str) -> bool:

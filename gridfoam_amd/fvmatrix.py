@@ -144,11 +144,11 @@ class FvMatrix:
         return (self._source - self.multiply(x) + d * x) / V
 
     def H1(self) -> torch.Tensor:
-        g = self.grid
-        h1 = torch.zeros_like(self._diag)
-        h1.index_add_(0, g.owner, -self.upper)
-        h1.index_add_(0, g.neighbour, -self.lower)
-        return h1 / g.cell_volumes
+        """fvmatrix.py:250-264 -- deterministic per-cell gather."""
+        h1 = torch.empty_like(self._diag)
+        nat.call("gf_ldu_h1", self.grid.mesh_ref, nat.ptr(self._upper), nat.ptr(self._lower), nat.ptr(h1),
+                 nat.stream())
+        return h1
 
     def flux(self, psi: torch.Tensor) -> torch.Tensor:
         """fvmatrix.py:424-468."""

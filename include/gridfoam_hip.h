@@ -191,6 +191,12 @@ int gf_correct_phi(const gf_mesh *m, const double *phi_hbya_s, const double *phi
 /* U = HbyA - rAtU grad p */
 int gf_correct_velocity(int32_t n_cells, const double *HbyA, const double *rAtU,
                         const double *grad_p, double *U, gf_stream_t stream);
+/* H1 = -(sum of off-diagonals of the row)/V  (fvmatrix.py:250-264) */
+int gf_ldu_h1(const gf_mesh *m, const double *upper, const double *lower, double *h1, gf_stream_t stream);
+/* SIMPLEC rAtU = 1/(1/rAU - H1), bounded by 0.1/rAU when bounded != 0
+ * (pressure_correction.py:163-193) */
+int gf_simplec_rautu(int32_t n_cells, const double *rAU, const double *h1, int32_t bounded, double *rAtU,
+                     gf_stream_t stream);
 /* diag[ref] += diag[ref]; source[ref] += diag_old*value (reference_value.py:33-51) */
 int gf_set_reference(double *diag, double *source, int32_t cell, double value,
                      gf_stream_t stream);

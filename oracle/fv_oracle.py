@@ -791,8 +791,9 @@ def needs_reference(p: Field) -> bool:  # algorithms/utils/reference_value.py:14
     return not any(bc.kind == "dirichlet" for bc in p.bcs.values())
 
 
-def solve_pressure(g, p, rAtU, div_hbya, cfg_p, cfg_final, n_nonorth, ref, corrected=True):
-    """pressure_correction.py:68-160 + reference_value.py:33-51."""
+def solve_pressure(g, p, rAtU, div_hbya, cfg_p, cfg_final, n_nonorth, ref, corrected=True, last=None):
+    """pressure_correction.py:68-160 + reference_value.py:33-51.  ``last``
+    (a list) receives the initial residual of the last pass."""
     div_src = div_hbya * g.cell_volumes
     first = None
     for corr in range(n_nonorth + 1):
@@ -806,6 +807,8 @@ def solve_pressure(g, p, rAtU, div_hbya, cfg_p, cfg_final, n_nonorth, ref, corre
         res0 = initial_residual(mat, p.data)
         if first is None:
             first = res0
+        if last is not None:
+            last[:] = [res0]
         cfg = cfg_final if (cfg_final is not None and corr == n_nonorth) else cfg_p
         p.data, st = solve(mat, p.data, cfg)
     return mat, st, first
@@ -823,8 +826,48 @@ def correct_phi(g, phi_hbya: FaceValues, pmat: Matrix, p: Field, rAtU: np.ndarra
     return out
 
 
+def simplec_rAtU(UEqn: Matrix, rAU: np.ndarray, bounded: bool = True) -> np.ndarray:
+    """pressure_correction.py:163-193."""
+    inv = 1.0 / rAU - UEqn.H1()
+    if bounded:
+        inv = np.maximum(inv, 0.1 / rAU)
+    return 1.0 / inv
+
+
+def apply_simplec(g: Grid, phi_h: FaceValues, HbyA: np.ndarray, p: Field, rAU: np.ndarray, rAtU: np.ndarray,
+                  corrected: bool = True):
+    """pressure_correction.py:222-267: phiHbyA += (rAtU-rAU)_f |Sf| snGrad(p)
+    (boundary faces: the condition's normal gradient); HbyA -= (rAU-rAtU) grad p."""
+    d_r = rAtU - rAU
+    w = g.geo.w
+    d_r_f = w * d_r[g.owner] + (1.0 - w) * d_r[g.neighbour]
+    out = FaceValues(phi_h.single + d_r_f * g.geo.mag_Sf * sn_grad(g, p, corrected), phi_h.bnd.copy())
+    for b in batches(g, p):
+        f, rv, rg, _ = bc_state(g, p, b)
+        coeff = value_grad_coeff(b, f)
+        sng = coeff * (rv - p.data[b.cells]) + (1.0 - f) * rg
+        out.bnd[b.mask] = out.bnd[b.mask] + d_r[b.cells] * b.mag_Sf * sng
+    return out, HbyA - (rAU - rAtU)[:, None] * lsq_grad(g, p)
+
+
+def _residual_control(rc: dict) -> dict:
+    """meta/config.py:364-388: a float is an absolute tolerance."""
+    out = {}
+    for k, v in (rc or {}).items():
+        if isinstance(v, (int, float)):
+            out[k] = (float(v), 0.0)
+        else:
+            out[k] = (float(v["tolerance"]), float(v.get("rel_tolerance", 0.0)))
+    return out
+
+
+def _residual_satisfied(res: float, tol: float, rtol: float, init: float) -> bool:
+    """algorithms/utils/residual.py:56-87."""
+    return res < tol or (rtol > 0.0 and res < rtol * init)
+
+
 # ---------------------------------------------------------------------------
-# Case description and drivers  (algorithms/simple.py, piso.py)
+# Case description and drivers  (algorithms/simple.py, piso.py, pimple.py)
 # ---------------------------------------------------------------------------
 def _bc_from_cfg(entry: dict, cshape) -> BC:
     kind = entry["type"]
@@ -886,6 +929,10 @@ class Case:
         self.lap_u_corr = lap_u not in ("uncorrected", "Gauss linear uncorrected")
         self.lap_p_corr = lap_p not in ("uncorrected", "Gauss linear uncorrected")
         self.ddt_scheme = look("ddtSchemes", "ddt(U)", "euler")
+        self.consistent = bool(self.alg.get("consistent", False)) and self.kind in ("simple", "pimple")
+        self.sn_p_corr = look("snGradSchemes", "snGrad(p)", "corrected") != "uncorrected"
+        self.rc = _residual_control(self.alg.get("residualControl", {}))
+        self.outer_converged = False
         self.ref = None
         if needs_reference(self.p):
             self.ref = (int(self.alg["pRefCell"]), float(self.alg["pRefValue"]))
@@ -919,15 +966,23 @@ class Case:
                 phi_h.bnd[b.mask] = np.sum(bc_state(g, U, b)[3] * b.Sf, axis=1)
         if self.adjust:
             adjust_phi(g, phi_h, U, p)
+        if self.consistent:
+            rAtU = simplec_rAtU(UEqn, rAU)
+            phi_h, HbyA_d = apply_simplec(g, phi_h, HbyA.data, p, rAU, rAtU, self.sn_p_corr)
+            HbyA = Field("HbyA", HbyA_d)
+            rAU = rAtU
         div_h = fvc_div(g, phi_h)
         cfg_final = None
         if final:
             cfg_final = self.solvers.get("pFinal", self.solvers.get("p_final"))
+        last = []
         pmat, st, res0 = solve_pressure(g, p, rAU, div_h, self.solvers["p"], cfg_final,
                                         int(self.alg.get("nNonOrthogonalCorrectors", 0)), self.ref,
-                                        self.lap_p_corr)
+                                        self.lap_p_corr, last)
+        self._last_res0 = last[0]
         self.stats["p"] = st
         self.stats["p_initial_residual"] = res0
+        self.stats["p_last_initial_residual"] = self._last_res0
         return HbyA, rAU, phi_h, pmat
 
     def step_simple(self):
@@ -968,11 +1023,65 @@ class Case:
         self.phi_older, self.phi_old = self.phi_old, self.phi.single.copy()
         self.prev_dt = g.dt
 
+    def step_pimple(self):
+        """algorithms/pimple.py:290-454 (laminar): outer correctors around the
+        PISO loop, residualControl exit after one more (final) outer pass."""
+        g, U, p = self.g, self.U, self.p
+        abc = ddt_coefficients(self.ddt_scheme, U.older is not None, g.dt, self.prev_dt)
+        n_outer = int(self.alg.get("nOuterCorrectors", 1))
+        n_corr = int(self.alg.get("nCorrectors", 2))
+        init, cur = {}, {}
+        self.outer_converged = False
+        self.stats["outer"] = 0
+
+        def record(name, r):
+            init.setdefault(name, r)
+            cur[name] = r
+
+        def converged(allow_relative):
+            if not self.rc:
+                return False
+            for name, (tol, rtol) in self.rc.items():
+                if name not in cur or name not in init:
+                    return False
+                if not _residual_satisfied(cur[name], tol, rtol if allow_relative else 0.0, init[name]):
+                    return False
+            return True
+
+        for outer in range(n_outer):
+            final_after = 0 < outer < n_outer - 1 and converged(outer > 1)
+            if final_after:
+                self.outer_converged = True
+            UEqn = self._momentum(True)
+            pred = UEqn.copy()
+            pred.source = UEqn.source - g.cell_volumes[:, None] * lsq_grad(g, p)
+            if "U" in self.rc:
+                record("U", initial_residual(pred, U.data))
+            U.data, self.stats["U"] = solve(pred, U.data, self.solvers["U"])
+            for i in range(n_corr):
+                HbyA, rAtU, phi_h, pmat = self._pressure_pass(UEqn, True, i == n_corr - 1, abc)
+                if "p" in self.rc:
+                    if i == 0:
+                        record("p", self.stats["p_initial_residual"])
+                    cur["p"] = self.stats["p_last_initial_residual"]
+                self.phi = correct_phi(g, phi_h, pmat, p, rAtU)
+                U.data = HbyA.data - rAtU[:, None] * lsq_grad(g, p)
+            if "phi" in self.rc:
+                record("phi", float(np.linalg.norm(fvc_div(g, self.phi))))
+            self.stats["outer"] = outer + 1
+            if final_after:
+                break
+        U.older, U.old = U.old, U.data.copy()
+        self.phi_older, self.phi_old = self.phi_old, self.phi.single.copy()
+        self.prev_dt = g.dt
+
     def step(self):
         if self.kind == "simple":
             self.step_simple()
         elif self.kind == "piso":
             self.step_piso()
+        elif self.kind == "pimple":
+            self.step_pimple()
         else:
             raise ValueError(f"unsupported algorithm {self.kind}")
 

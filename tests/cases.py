@@ -51,3 +51,30 @@ def fluxel_block(name: str) -> dict:
         "n_leaf_refinement": f["n_leaf"],
         "refinement_regions": [{"min": list(r[0]), "max": list(r[1]), "level": r[2]} for r in f["regions"]],
     }
+
+
+def small_cavity3d_pimple(root=8, steps=None, n_outer=2, residual_control=None, consistent=False,
+                          cfl=2.0) -> dict:
+    """The octree cavity with PIMPLE at a larger time step (outer correctors)."""
+    cfg = small_cavity3d(root, steps)
+    h_min = 1.0 / (root * 4)
+    cfg["simulator"]["control"]["deltaT"] = cfl * h_min
+    cfg["simulator"]["control"]["endTime"] = cfl * h_min * (steps or 10)
+    alg = cfg["simulator"]["fvSolution"]["algorithm"]
+    alg["type"] = "pimple"
+    alg["nOuterCorrectors"] = n_outer
+    alg["consistent"] = consistent
+    if residual_control is not None:
+        alg["residualControl"] = residual_control
+    return cfg
+
+
+def small_cavity2d(n=16, consistent=False, alpha_p=0.3) -> dict:
+    """The 2-D SIMPLE cavity on an n x n grid."""
+    cfg = copy.deepcopy(case_dict("cavity2d_simple.yaml"))
+    cfg["fluxel"]["root_resolution"] = [n, n, 1]
+    cfg["fluxel"]["domain"]["upper"] = [1.0, 1.0, 1.0 / n]
+    alg = cfg["simulator"]["fvSolution"]["algorithm"]
+    alg["consistent"] = consistent
+    alg["relaxationFactors"]["equations"]["p"] = alpha_p
+    return cfg

@@ -512,3 +512,29 @@ def test_adjoint_gradient_matches_central_differences(method):
                 analytic = float((gk * d).sum())
             fd = float(J(p_plus) - J(p_minus)) / (2 * h)
         assert abs(analytic - fd) <= 1e-5 * max(abs(fd), 1e-30), (k, analytic, fd)
+
+
+# ----------------------------------------------------------------- PIMPLE / SIMPLEC
+@pytest.mark.parametrize("consistent", [False, True])
+def test_pimple_octree_cavity_matches_oracle(consistent):
+    """PIMPLE, 2 outer x 2 inner correctors at CFL 2, residualControl on
+    p and U (pimple.py:290-454), 3 steps; SIMPLEC variant with consistent."""
+    from tests.cases import small_cavity3d_pimple
+
+    cfg = small_cavity3d_pimple(root=6, steps=3, n_outer=3, consistent=consistent,
+                                residual_control={"p": {"tolerance": 1e-30, "rel_tolerance": 0.5}, "U": 1e-30})
+    _say(f"pimple cavity 6^3 consistent={consistent}")
+    alg, case = _run_both(cfg, 3)
+    assert alg.outer_iterations == case.stats["outer"]
+    assert _rel(_np(alg.U.data), case.U.data) < 1e-10
+    assert _rel(_np(alg.p.data), case.p.data) < 1e-10
+
+
+def test_simplec_cavity2d_matches_oracle():
+    from tests.cases import small_cavity2d
+
+    cfg = small_cavity2d(32, consistent=True)
+    _say("simplec cavity2d 32x32, 30 its")
+    alg, case = _run_both(cfg, 30)
+    assert _rel(_np(alg.U.data), case.U.data) < 1e-10
+    assert _rel(_np(alg.p.data), case.p.data) < 1e-10

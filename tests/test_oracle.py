@@ -370,3 +370,78 @@ def test_ldu_multiply_transpose_and_h_operators():
     off = A - np.diag(np.diag(A))
     np.testing.assert_allclose(m.H(x), (m.source - off @ x) / V, rtol=1e-13, atol=1e-12)
     np.testing.assert_allclose(m.H1(), -off.sum(axis=1) / V, rtol=1e-13, atol=1e-12)
+
+
+# ----------------------------------------------------------------- PIMPLE / SIMPLEC
+def test_pimple_with_one_outer_corrector_is_piso():
+    """pimple.py:290-454 with nOuterCorrectors 1 and no residualControl runs
+    exactly the PISO step (piso.py:227-326)."""
+    from tests.cases import small_cavity3d, small_cavity3d_pimple
+
+    a = fo.Case(small_cavity3d(root=4, steps=3))
+    b = fo.Case(small_cavity3d_pimple(root=4, steps=3, n_outer=1, cfl=0.5))
+    for _ in range(3):
+        a.step()
+        b.step()
+    np.testing.assert_array_equal(a.U.data, b.U.data)
+    np.testing.assert_array_equal(a.p.data, b.p.data)
+
+
+def test_pimple_residual_control_exit():
+    """The outer loop runs one final pass after residualControl is met
+    (pimple.py:326-335): a tolerance every residual meets stops after two
+    outer passes; an unreachable one runs all of them."""
+    from tests.cases import small_cavity3d_pimple
+
+    easy = fo.Case(small_cavity3d_pimple(root=4, steps=1, n_outer=5, residual_control={"p": 1e30, "U": 1e30}))
+    easy.step()
+    assert easy.stats["outer"] == 2 and easy.outer_converged
+    hard = fo.Case(small_cavity3d_pimple(root=4, steps=1, n_outer=4, residual_control={"p": 0.0}))
+    hard.step()
+    assert hard.stats["outer"] == 4 and not hard.outer_converged
+
+
+def test_pimple_outer_correctors_conserve_mass():
+    """After every PIMPLE step the corrected flux is divergence-free to the
+    pressure tolerance (test_pressure_coupling.py:66-197 checks PISO/SIMPLE)."""
+    from tests.cases import small_cavity3d_pimple
+
+    c = fo.Case(small_cavity3d_pimple(root=4, steps=2, n_outer=3))
+    for _ in range(2):
+        c.step()
+        assert fo.continuity_max(c.g, c.phi) < 1e-8
+
+
+def test_simplec_rAtU_closed_form_and_bound():
+    """pressure_correction.py:163-193 on a hand-made matrix: 1/(1/rAU - H1),
+    floored at 0.1/rAU."""
+    g = _grid("refined2d")
+    m = fo.Matrix(g)
+    m.diag = np.full(g.num_cells, 4.0)
+    m.upper = np.full(g.num_faces, -0.5)
+    m.lower = np.full(g.num_faces, -0.5)
+    rAU = g.cell_volumes / m.diag
+    nnb = np.bincount(g.owner, minlength=g.num_cells) + np.bincount(g.neighbour, minlength=g.num_cells)
+    h1 = 0.5 * nnb / g.cell_volumes
+    np.testing.assert_allclose(fo.simplec_rAtU(m, rAU, bounded=False), 1.0 / (1.0 / rAU - h1), rtol=1e-14)
+    big = m.copy()
+    big.upper = np.full(g.num_faces, -10.0)
+    big.lower = big.upper.copy()
+    out = fo.simplec_rAtU(big, rAU)
+    np.testing.assert_allclose(out, rAU / 0.1, rtol=1e-14)  # 1/rAU - H1 < 0 everywhere: the floor wins
+
+
+def test_simplec_reaches_the_simple_fixed_point():
+    """SIMPLE and SIMPLEC solve the same discrete steady equations, so both
+    converge to the same state (16x16 cavity, Re 100)."""
+    from tests.cases import small_cavity2d
+
+    a = fo.Case(small_cavity2d(16, consistent=False))
+    b = fo.Case(small_cavity2d(16, consistent=True, alpha_p=0.3))
+    for _ in range(600):
+        a.step()
+        b.step()
+    rel = lambda x, y: np.linalg.norm(x - y) / np.linalg.norm(y)  # noqa: E731
+    assert rel(b.U.data, a.U.data) < 1e-6
+    assert rel(b.p.data, a.p.data) < 1e-6
+    assert fo.continuity_max(b.g, b.phi) < 1e-8
