@@ -98,6 +98,7 @@ _SIGS: dict[str, tuple[Any, list[Any]]] = {
     "gf_sell_spmv": (_i32, [_P(GfMesh), _vp, _vp, _vp, _vp, _i32, _i32, _vp, _vp]),
     "gf_relax_momentum": (_i32, [_i32, _f64, _vp, _vp, _vp, _vp]),
     "gf_ldu_h1": (_i32, [_P(GfMesh), _vp, _vp, _vp, _vp]),
+    "gf_fixed_values": (_i32, [_P(GfMesh), _i32, _i32] + [_vp] * 13 + [_vp]),
     "gf_simplec_rautu": (_i32, [_i32, _vp, _vp, _i32, _vp, _vp]),
     "gf_predictor_source": (_i32, [_i32, _i32, _vp, _vp, _vp, _vp, _vp]),
     "gf_relax_field": (_i32, [_i64, _f64, _vp, _vp, _vp]),

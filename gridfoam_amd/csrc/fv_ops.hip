@@ -499,6 +499,66 @@ __global__ void k_correct_velocity(int C, const double *HbyA, const double *rAtU
     for (int q = 0; q < 3; ++q) U[3 * c + q] = HbyA[3 * c + q] - r * gp[3 * c + q];
 }
 
+// ---- symmetric elimination of prescribed cell values (fvmatrix.py:325-394)
+__global__ void k_fix_mark(int K, int nc, const int32_t *__restrict__ cells, const double *__restrict__ values,
+                           uint8_t *__restrict__ fixed, double *__restrict__ presc) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const int c = cells[k];
+    fixed[c] = 1;
+    for (int q = 0; q < nc; ++q) presc[(size_t)c * nc + q] = values[(size_t)k * nc + q];
+}
+
+// Sum of the diagonal by one workgroup in a fixed order (the sign picks the
+// unit diagonal of rows that have none).
+__global__ __launch_bounds__(kBlock) void k_diag_sum(int C, const double *__restrict__ diag, double *__restrict__ out) {
+    double a = 0.0;
+    for (int i = threadIdx.x; i < C; i += kBlock) a += diag[i];
+    double v[1] = {a};
+    const bool mx[1] = {false};
+    block_reduce<1, kBlock>(v, mx);
+    if (threadIdx.x == 0) out[0] = v[0];
+}
+
+__global__ void k_fix_faces(const gf_mesh m, const uint8_t *__restrict__ fixed, const double *__restrict__ upper,
+                            const double *__restrict__ lower, double *__restrict__ out_upper,
+                            double *__restrict__ out_lower) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.n_faces) return;
+    const bool conn = fixed[m.owner[f]] | fixed[m.neighbour[f]];
+    out_upper[f] = conn ? 0.0 : upper[f];
+    out_lower[f] = conn ? 0.0 : lower[f];
+}
+
+// Fixed rows: constraint diagonal (the row's own, or +-1 when it is zero)
+// and source = diag * value.  Other rows: prescribed neighbours move to the
+// right-hand side.  Gathered per cell over its face lists (no atomics).
+__global__ void k_fix_cells(const gf_mesh m, int nc, const uint8_t *__restrict__ fixed,
+                            const double *__restrict__ presc, const double *__restrict__ diag,
+                            const double *__restrict__ upper, const double *__restrict__ lower,
+                            const double *__restrict__ source, const double *__restrict__ dsum,
+                            double *__restrict__ out_diag, double *__restrict__ out_source) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.n_cells) return;
+    const double d = diag[c];
+    if (fixed[c]) {
+        const double cd = d != 0.0 ? d : (dsum[0] < 0.0 ? -1.0 : 1.0);
+        out_diag[c] = cd;
+        for (int q = 0; q < nc; ++q) out_source[(size_t)c * nc + q] = cd * presc[(size_t)c * nc + q];
+        return;
+    }
+    out_diag[c] = d;
+    for (int q = 0; q < nc; ++q) {
+        double s = source[(size_t)c * nc + q];
+        for (int f = m.own_row[c]; f < m.own_row[c + 1]; ++f) s -= upper[f] * presc[(size_t)m.neighbour[f] * nc + q];
+        for (int k = m.nbr_row[c]; k < m.nbr_row[c + 1]; ++k) {
+            const int f = m.nbr_face[k];
+            s -= lower[f] * presc[(size_t)m.owner[f] * nc + q];
+        }
+        out_source[(size_t)c * nc + q] = s;
+    }
+}
+
 // H1 = -(sum of the row's off-diagonals) / V, gathered per cell over its face
 // lists in a fixed order (fvmatrix.py:250-264).
 __global__ void k_ldu_h1(const gf_mesh m, const double *__restrict__ upper, const double *__restrict__ lower,
@@ -828,6 +888,30 @@ extern "C" int gf_correct_velocity(int32_t n_cells, const double *HbyA, const do
     GF_REQUIRE(HbyA && rAtU && grad_p && U, "null argument");
     hipLaunchKernelGGL(k_correct_velocity, dim3(grid_for(n_cells)), dim3(kBlock), 0, S(stream), n_cells, HbyA, rAtU,
                        grad_p, U);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
+extern "C" int gf_fixed_values(const gf_mesh *m, int32_t nc, int32_t n_fixed, const int32_t *cells,
+                               const double *values, const double *diag, const double *upper, const double *lower,
+                               const double *source, uint8_t *fixed, double *presc, double *dsum, double *out_diag,
+                               double *out_upper, double *out_lower, double *out_source, gf_stream_t stream) {
+    GF_REQUIRE(m && cells && values && diag && upper && lower && source && fixed && presc && dsum && out_diag &&
+                   out_upper && out_lower && out_source,
+               "null argument");
+    GF_REQUIRE(nc >= 1 && n_fixed >= 1, "bad fixed-value arguments");
+    hipStream_t s = S(stream);
+    const int C = m->n_cells;
+    GF_CHECK_HIP(hipMemsetAsync(fixed, 0, (size_t)C, s));
+    GF_CHECK_HIP(hipMemsetAsync(presc, 0, sizeof(double) * (size_t)C * nc, s));
+    hipLaunchKernelGGL(k_fix_mark, dim3(grid_for(n_fixed)), dim3(kBlock), 0, s, n_fixed, nc, cells, values, fixed,
+                       presc);
+    hipLaunchKernelGGL(k_diag_sum, dim3(1), dim3(kBlock), 0, s, C, diag, dsum);
+    if (m->n_faces)
+        hipLaunchKernelGGL(k_fix_faces, dim3(grid_for(m->n_faces)), dim3(kBlock), 0, s, *m, fixed, upper, lower,
+                           out_upper, out_lower);
+    hipLaunchKernelGGL(k_fix_cells, dim3(grid_for(C)), dim3(kBlock), 0, s, *m, nc, fixed, presc, diag, upper, lower,
+                       source, dsum, out_diag, out_source);
     GF_CHECK_LAUNCH();
     return GF_OK;
 }

@@ -162,11 +162,65 @@ class FvMatrix:
         return out[:F]
 
     def with_fixed_values(self, cells: torch.Tensor, values: torch.Tensor) -> "FvMatrix":
-        """fvmatrix.py:325-394; cell constraints only arise from immersed
-        Dirichlet snapping, which this build does not produce."""
+        """fvmatrix.py:325-394: symmetric elimination of prescribed cell values
+        into a separate solve matrix (no face-flux correction); ``self`` is
+        unchanged.  Gradients flow to ``values`` and to the input source;
+        gradients of the input LDU entries through the elimination are not
+        provided (an input LDU that requires grad raises)."""
+        _require(values, (cells.numel(), *self.field.component_shape), "fixed values")
         if cells.numel() == 0:
             return self
-        raise NotImplementedError("immersed Dirichlet cell constraints are not part of this build")
+        if any(t.requires_grad for t in (self._diag, self._upper, self._lower)):
+            raise NotImplementedError("gradients of LDU entries through with_fixed_values are not supported")
+        g = self.grid
+        nc = self.field.nc
+        C = g.num_cells
+        cells32 = cells.to(device=g.device, dtype=torch.int32).contiguous()
+        if torch.unique(cells32).numel() != cells32.numel():
+            raise ValueError("fixed-value cells must be unique")
+        vals = values.detach().to(torch.float64).reshape(cells.numel(), nc).contiguous()
+        src = self._source.detach().contiguous()
+        fixed = torch.empty(C, dtype=torch.uint8, device=g.device)
+        presc = torch.empty(C * nc, dtype=torch.float64, device=g.device)
+        dsum = torch.empty(1, dtype=torch.float64, device=g.device)
+        out = FvMatrix(self.field)
+        nat.call("gf_fixed_values", g.mesh_ref, ctypes.c_int32(nc), ctypes.c_int32(cells.numel()), nat.ptr(cells32),
+                 nat.ptr(vals), nat.ptr(self._diag), nat.ptr(self._upper), nat.ptr(self._lower), nat.ptr(src),
+                 nat.ptr(fixed), nat.ptr(presc), nat.ptr(dsum), nat.ptr(out._diag), nat.ptr(out._upper),
+                 nat.ptr(out._lower), nat.ptr(out._source), nat.stream())
+        if values.requires_grad or self._source.requires_grad:
+            out._source = _FixedSource.apply(values, self._source, out._source, self, out, cells32, fixed)
+        return out
+
+
+class _FixedSource(torch.autograd.Function):
+    """Source of the eliminated system as a function of the prescribed values
+    and the input source: fixed rows cd*value, free rows source - A_off presc."""
+
+    @staticmethod
+    def forward(ctx, values, source, out_source, base, out, cells32, fixed):  # noqa: D401
+        ctx.base, ctx.out, ctx.cells = base, out, cells32
+        ctx.save_for_backward(fixed)
+        ctx.vshape = values.shape
+        return out_source.clone()
+
+    @staticmethod
+    def backward(ctx, grad):
+        (fixed,) = ctx.saved_tensors
+        base, out, cells = ctx.base, ctx.out, ctx.cells.long()
+        free = (fixed == 0).to(grad.dtype)
+        g_free = grad * (free if grad.dim() == 1 else free[:, None])
+        # off-diagonal part of A^T applied to the free rows' gradient
+        offT = FvMatrix.__new__(FvMatrix)
+        offT.field, offT.grid = base.field, base.grid
+        offT._diag = torch.zeros_like(base._diag)
+        offT._upper, offT._lower = base._lower, base._upper
+        offT._source = base._source
+        offT.face_flux_correction = None
+        at = offT.multiply(g_free.contiguous())
+        cd = out._diag[cells]
+        gv = (cd if grad.dim() == 1 else cd[:, None]) * grad[cells] - at[cells]
+        return gv.reshape(ctx.vshape), g_free, None, None, None, None, None
 
 
 class Equation:

@@ -191,6 +191,16 @@ int gf_correct_phi(const gf_mesh *m, const double *phi_hbya_s, const double *phi
 /* U = HbyA - rAtU grad p */
 int gf_correct_velocity(int32_t n_cells, const double *HbyA, const double *rAtU,
                         const double *grad_p, double *U, gf_stream_t stream);
+/* Symmetric elimination of prescribed cell values (fvmatrix.py:325-394):
+ * couplings of fixed cells zeroed, their neighbours' contributions moved to the
+ * right-hand side, fixed rows diag*x = diag*value (zero diagonal -> +-1 by the
+ * sign of sum(diag)).  cells [K] unique, values [K,nc]; fixed [C] (uint8),
+ * presc [C,nc] and dsum [1] are caller scratch. */
+int gf_fixed_values(const gf_mesh *m, int32_t nc, int32_t n_fixed, const int32_t *cells,
+                    const double *values, const double *diag, const double *upper,
+                    const double *lower, const double *source, uint8_t *fixed, double *presc,
+                    double *dsum, double *out_diag, double *out_upper, double *out_lower,
+                    double *out_source, gf_stream_t stream);
 /* H1 = -(sum of off-diagonals of the row)/V  (fvmatrix.py:250-264) */
 int gf_ldu_h1(const gf_mesh *m, const double *upper, const double *lower, double *h1, gf_stream_t stream);
 /* SIMPLEC rAtU = 1/(1/rAU - H1), bounded by 0.1/rAU when bounded != 0

@@ -538,3 +538,56 @@ def test_simplec_cavity2d_matches_oracle():
     alg, case = _run_both(cfg, 30)
     assert _rel(_np(alg.U.data), case.U.data) < 1e-10
     assert _rel(_np(alg.p.data), case.p.data) < 1e-10
+
+
+# ----------------------------------------------------------------- fixed values
+@pytest.mark.parametrize("shape", [(), (3,)])
+@pytest.mark.parametrize("sign", [1.0, -1.0])
+def test_fixed_values_match_dense_system_and_gradients(shape, sign):
+    """tests/unit/core/test_fixed_values.py:15-89: the eliminated system keeps
+    upper == lower, leaves the input untouched, solves to the prescribed values
+    and to the reduced dense system (1e-11), and its value gradient matches
+    the dense one (1e-10)."""
+    from gridfoam_amd.config import SolverConfig
+    from gridfoam_amd.fields import CellField
+    from gridfoam_amd.fvmatrix import Equation, FvMatrix
+    from gridfoam_amd.solvers import create_solver
+
+    g, _ = _grid("refined2d")
+    _say(f"fixed values shape={shape} sign={sign}")
+    field = CellField(g, "fixed_test", shape)
+    m = FvMatrix(field)
+    m.diag = torch.full_like(m.diag, 10.0 * sign)
+    m.upper = torch.full((g.num_internal_faces,), -sign, dtype=torch.float64, device=g.device)
+    m.lower = m.upper.clone()
+    gen = torch.Generator().manual_seed(3)
+    m.source = torch.randn(m.source.shape, dtype=torch.float64, generator=gen).to(g.device)
+    src_before = m.source.clone()
+    cells = torch.tensor([0, 1, g.num_cells - 1], device=g.device)
+    values = torch.randn((3, *shape), dtype=torch.float64, generator=gen).to(g.device).requires_grad_(True)
+    con = m.with_fixed_values(cells, values)
+    torch.testing.assert_close(con.upper, con.lower)
+    torch.testing.assert_close(m.source, src_before)
+    solver = create_solver(SolverConfig(method="cg", tolerance=1e-12, rel_tolerance=0.0, max_iter=100))
+    res = solver.solve(Equation(field, con))
+    assert all(s.converged for s in res.stats)
+    torch.testing.assert_close(res.solution[cells], values, atol=1e-12, rtol=0)
+    # independent reduced dense system on the CPU
+    C = g.num_cells
+    dense = torch.diag(m.diag.detach().cpu())
+    o, n = g.owner.cpu(), g.neighbour.cpu()
+    dense.index_put_((o, n), m.upper.detach().cpu(), accumulate=True)
+    dense.index_put_((n, o), m.lower.detach().cpu(), accumulate=True)
+    free = torch.ones(C, dtype=torch.bool)
+    free[cells.cpu()] = False
+    vc = values.cpu()
+    flat = vc.reshape(3, -1)
+    rhs = m.source.detach().cpu().reshape(C, -1)[free] - dense[free][:, cells.cpu()] @ flat
+    fs = torch.linalg.solve(dense[free][:, free], rhs)
+    exp = torch.zeros(C, flat.shape[1], dtype=torch.float64)
+    exp[free] = fs
+    exp = exp.index_copy(0, cells.cpu(), flat).reshape(m.source.shape)
+    torch.testing.assert_close(res.solution.detach().cpu(), exp.detach(), atol=1e-11, rtol=0)
+    ga = torch.autograd.grad(res.solution.square().sum(), values)[0].cpu()
+    ge = torch.autograd.grad(exp.square().sum(), vc)[0]
+    torch.testing.assert_close(ga, ge, atol=1e-10, rtol=0)
