@@ -7,12 +7,13 @@ is visible, every native call raises.
 from __future__ import annotations
 
 import ctypes
+import os
 import pathlib
 from typing import Any
 
 import torch
 
-LIB_PATH = pathlib.Path(__file__).resolve().parent / "libgridfoam_hip.so"
+LIB_PATH = pathlib.Path(os.environ.get("GRIDFOAM_HIP_LIB", pathlib.Path(__file__).resolve().parent / "libgridfoam_hip.so"))
 
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32

@@ -198,14 +198,23 @@ __device__ __forceinline__ void reduce_prev(const Ops &o, int s, int nb_prev, co
     __shared__ double res[NR * NQ];
     const double *P = o.partials + (size_t)((s - 1) & 1) * kMaxRhs * kNQ * o.pstride;
     const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+    double acc[NR][NQ] = {};
+    for (int j = threadIdx.x; j < nb_prev; j += BS) {  // all quantities' loads issued together
+        double x[NR][NQ];
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) x[r][q] = P[((size_t)r * kNQ + q) * o.pstride + j];
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) acc[r][q] = is_max[q] ? fmax(acc[r][q], x[r][q]) : acc[r][q] + x[r][q];
+    }
 #pragma unroll
     for (int r = 0; r < NR; ++r)
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
-            const double *Pq = P + ((size_t)r * kNQ + q) * o.pstride;
-            double v = 0.0;
-            for (int j = threadIdx.x; j < nb_prev; j += BS) v = is_max[q] ? fmax(v, Pq[j]) : v + Pq[j];
-            v = is_max[q] ? wave_max(v) : wave_sum(v);
+            const double v = is_max[q] ? wave_max(acc[r][q]) : wave_sum(acc[r][q]);
             if (lane == 0) lds[r * NQ + q][wid] = v;
         }
     lds_sync();
@@ -367,7 +376,7 @@ __global__ __launch_bounds__(TB) void k_cg_dir(Ops o, int s, int nb_prev, int ma
         pnew[r] = (sh[r].pcur ? p0 : p1) + (size_t)r * o.C;
     }
     // ---- loads issued ahead of the reduction
-    const int sl = ic >> 6, lane = ic & 63;
+    const int sl = __builtin_amdgcn_readfirstlane(ic >> 6), lane = ic & 63;
     const int base = o.sell_ptr[sl];
     const int W = (o.sell_ptr[sl + 1] - base) >> 6;
     const double d = dg[ic];
@@ -1155,17 +1164,6 @@ __global__ __launch_bounds__(RB) void k_bi_p(Ops o, int s, const double *r_, con
             p_[k] = r_[k] + sh[r].beta * (p_[k] - sh[r].omega * v_[k]);
         }
     store_state<NR>(o, s, sh);
-}
-
-// ---------------------------------------------------------------- plain SpMV entry
-template <typename VT>
-__global__ __launch_bounds__(kBlock) void k_sell_spmv(const int32_t *sell_ptr, const int32_t *sell_col, int C,
-                                                      const VT *val, const VT *dg, const VT *x, VT *y) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= C) return;
-    VT acc[1];
-    sell_row<VT, 1>(sell_ptr, sell_col, val, dg[i], i, x, C, acc);
-    y[i] = acc[0];
 }
 
 // ---------------------------------------------------------------- workspace
