@@ -220,6 +220,42 @@ def cpu_baseline(gpu_p_iters_per_step: float) -> dict:
                        f"{gpu_p_iters_per_step:.0f} CG iterations per step as on the GPU; numpy FP64, 1 thread")}
 
 
+def serve_measure(n_requests: int, slots: int, iterations: int = 5, seed: int = 0) -> dict:
+    """Serving stream after the build plan: random lid-driven cases (base n^3,
+    n log-uniform in [16, 96], 0-2 refinement levels around 1-4 boxes,
+    Re U(100, 1000)), each request = `iterations` SIMPLE iterations.  Measured:
+    one-at-a-time plain FP64 solves/s; the server's capacity with every
+    request queued at t = 0 (mixed precision); p50/p99 latency with Poisson
+    arrivals at 80% of that capacity."""
+    import numpy as np
+    import torch
+
+    from gridfoam_amd.serving import Request, SolveServer, latency_stats, random_case, run_case
+
+    rng = np.random.default_rng(seed)
+    cases = [random_case(rng) for _ in range(n_requests)]
+    plain = [copy.deepcopy(c) for c in cases]
+    for c in plain:
+        c["simulator"]["fvSolution"]["solvers"]["p"]["mixed_precision"] = False
+    run_case(plain[0], 1)  # warm-up (library load, kernels)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for c in plain:
+        run_case(c, iterations)
+    t_seq = time.perf_counter() - t0
+    server = SolveServer(slots=slots)
+    sat = latency_stats(server.run([Request(k, c, 0.0, iterations) for k, c in enumerate(cases)]))
+    rate = 0.8 * sat["solves_per_s"]
+    arr = np.cumsum(rng.exponential(1.0 / rate, size=n_requests))
+    poi = latency_stats(server.run([Request(k, c, float(a), iterations) for k, (c, a) in enumerate(zip(cases, arr))]))
+    return {"requests": n_requests, "slots": slots, "simple_iterations_per_request": iterations,
+            "one_at_a_time_plain_solves_per_s": round(n_requests / t_seq, 3),
+            "served_capacity_solves_per_s": round(sat["solves_per_s"], 3),
+            "served_vs_one_at_a_time_plain": round(sat["solves_per_s"] * t_seq / n_requests, 3),
+            "poisson_load": 0.8, "poisson_solves_per_s": round(poi["solves_per_s"], 3),
+            "p50_ms": round(poi["p50_ms"], 1), "p99_ms": round(poi["p99_ms"], 1)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -230,6 +266,9 @@ def main() -> int:
     ap.add_argument("--no-large", action="store_true", help="skip the 256^3 SpMV measurement")
     ap.add_argument("--spmv-only", choices=("fp32", "fp64"), default=None,
                     help="only launch the SpMV kernel on the cavity mesh (for counter runs) and exit")
+    ap.add_argument("--serve-requests", type=int, default=48, help="requests in the serving measurement (0: skip)")
+    ap.add_argument("--serve-slots", type=int, default=4, help="cases in flight at once in the serving measurement")
+    ap.add_argument("--serve-only", action="store_true", help="only run the serving measurement")
     ap.add_argument("--inner-rtol", type=float, default=None,
                     help="refinement trigger of the mixed solve (default: the solver config's inner_rel_tolerance)")
     a = ap.parse_args()
@@ -242,6 +281,9 @@ def main() -> int:
     import torch
 
     torch.cuda.set_device(0)
+    if a.serve_only:
+        print(json.dumps({"serving": serve_measure(a.serve_requests, a.serve_slots)}), flush=True)
+        return 0
     if a.spmv_only:
         from gridfoam_amd.algorithms import create_case
         from gridfoam_amd.config import load_config
@@ -311,6 +353,9 @@ def main() -> int:
         line["roofline_256cubed"] = {"fp32": spmv_roofline(big, pm, fp32=True, launches=50),
                                      "fp64": spmv_roofline(big, pm, fp32=False, launches=50)}
         del big, pm
+    if a.serve_requests > 0:
+        _log(f"serving: {a.serve_requests} requests, {a.serve_slots} slots")
+        line["serving"] = serve_measure(a.serve_requests, a.serve_slots)
     if rel is not None:
         line["mixed_vs_fp64_rel_l2"] = rel
         line["speedup_mixed_vs_fp64"] = round(out["plain"]["dt"] / out["mixed"]["dt"], 3)

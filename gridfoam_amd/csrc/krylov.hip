@@ -1271,8 +1271,8 @@ struct Poller {
 template <int NR>
 int finish(const Layout &L, int s, const gf_krylov_cfg &cfg, hipStream_t st, double *stats) {
     KState *h = host_stage();
+    GF_CHECK_HIP(hipMemcpyAsync(h, L.st + (s & 1) * kMaxRhs, sizeof(KState) * NR, hipMemcpyDeviceToHost, st));
     GF_CHECK_HIP(hipStreamSynchronize(st));
-    GF_CHECK_HIP(hipMemcpy(h, L.st + (s & 1) * kMaxRhs, sizeof(KState) * NR, hipMemcpyDeviceToHost));
     for (int r = 0; r < NR; ++r) {
         stats[r * 4 + 0] = h[r].norm0;
         stats[r * 4 + 1] = h[r].final_res;

@@ -193,7 +193,7 @@ class OctreeGrid:
         self.sell_code = _i32(nnz, dev)
         m.sell_ptr, m.sell_col, m.sell_code = (nat.ptr(t) for t in (self.sell_ptr, self.sell_col, self.sell_code))
         nat.call("gf_sell_fill", mp, nat.ptr(self.sell_col), nat.ptr(self.sell_code), st)
-        torch.cuda.synchronize()
+        torch.cuda.current_stream().synchronize()
 
     # ------------------------------------------------------- reference contract
     @property

@@ -591,3 +591,22 @@ def test_fixed_values_match_dense_system_and_gradients(shape, sign):
     ga = torch.autograd.grad(res.solution.square().sum(), values)[0].cpu()
     ge = torch.autograd.grad(exp.square().sum(), vc)[0]
     torch.testing.assert_close(ga, ge, atol=1e-10, rtol=0)
+
+
+# ----------------------------------------------------------------- serving
+def test_served_results_equal_the_same_cases_alone():
+    """Cases served concurrently (3 slots, every request admitted while others
+    run) give bit-identical U and p to the same cases run one at a time."""
+    from gridfoam_amd.serving import Request, SolveServer, random_case, run_case
+
+    rng = np.random.default_rng(5)
+    cfgs = [random_case(rng, n_range=(8, 16)) for _ in range(5)]
+    _say("serving: 5 cases alone")
+    alone = [run_case(c, 2) for c in cfgs]
+    _say("serving: same 5 cases through the server")
+    res = SolveServer(slots=3).run([Request(k, c, arrival=0.01 * k, iterations=2) for k, c in enumerate(cfgs)])
+    assert sorted(r.rid for r in res) == list(range(5))
+    for r in res:
+        cells, U, p, its = alone[r.rid]
+        assert r.cells == cells and r.p_iterations == its
+        assert torch.equal(r.U, U) and torch.equal(r.p, p)
