@@ -243,11 +243,12 @@ def serve_measure(n_requests: int, slots: int, iterations: int = 5, seed: int = 
     for c in plain:
         run_case(c, iterations)
     t_seq = time.perf_counter() - t0
-    server = SolveServer(slots=slots)
-    sat = latency_stats(server.run([Request(k, c, 0.0, iterations) for k, c in enumerate(cases)]))
-    rate = 0.8 * sat["solves_per_s"]
-    arr = np.cumsum(rng.exponential(1.0 / rate, size=n_requests))
-    poi = latency_stats(server.run([Request(k, c, float(a), iterations) for k, (c, a) in enumerate(zip(cases, arr))]))
+    with SolveServer(slots=slots) as server:  # slot processes start (and warm up) before the clock
+        sat = latency_stats(server.run([Request(k, c, 0.0, iterations) for k, c in enumerate(cases)]))
+        rate = 0.8 * sat["solves_per_s"]
+        arr = np.cumsum(rng.exponential(1.0 / rate, size=n_requests))
+        poi = latency_stats(server.run([Request(k, c, float(a), iterations)
+                                        for k, (c, a) in enumerate(zip(cases, arr))]))
     return {"requests": n_requests, "slots": slots, "simple_iterations_per_request": iterations,
             "one_at_a_time_plain_solves_per_s": round(n_requests / t_seq, 3),
             "served_capacity_solves_per_s": round(sat["solves_per_s"], 3),

@@ -93,6 +93,8 @@ _SIGS: dict[str, tuple[Any, list[Any]]] = {
     "gf_initial_residual": (_i32, [_P(GfMesh), _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gf_krylov_workspace_bytes": (_i64, [_P(GfMesh), _i32, _P(GfKrylovCfg)]),
     "gf_krylov_solve": (_i32, [_P(GfMesh), _P(GfKrylovCfg), _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "gf_krylov_use_graphs": (_i32, [_i32]),
+    "gf_krylov_release": (_i32, [_vp]),
     "gf_ldu_grads": (_i32, [_P(GfMesh), _i32, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gf_sell_pack": (_i32, [_P(GfMesh), _i32, _vp, _vp, _vp, _i32, _vp]),
     "gf_sell_spmv_scratch_bytes": (_i64, [_P(GfMesh)]),

@@ -19,6 +19,12 @@
 // for all components; values are repacked once per solve into slice-of-64
 // ELL order (one slice = one wavefront, coalesced column/value loads).
 #include <algorithm>
+#include <atomic>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <utility>
+#include <vector>
 #include <type_traits>
 #include <cstdlib>
 #include <cmath>
@@ -1282,53 +1288,134 @@ int finish(const Layout &L, int s, const gf_krylov_cfg &cfg, hipStream_t st, dou
     return GF_OK;
 }
 
+// ---- launch-sequence graphs --------------------------------------------
+// A chunk of `every` PCG iterations (plus the refinement trio in mixed
+// precision) is captured once into a HIP graph and replayed: one host call
+// per chunk instead of 2*every+3 launches.  Launches read only the parity of
+// their sequence number, so a graph captured at sequence s0 replays any chunk
+// that starts at s0 + 2k.  Graphs are cached per workspace (they embed its
+// pointers) and dropped by gf_krylov_release.
+struct GraphCache {
+    std::mutex mu;
+    std::unordered_map<const void *, std::vector<std::pair<std::string, hipGraphExec_t>>> by_ws;
+};
+GraphCache &graph_cache() {
+    static GraphCache c;
+    return c;
+}
+std::atomic<int> &graphs_flag() {
+    static std::atomic<int> on{[] {
+        const char *e = getenv("GRIDFOAM_KRYLOV_GRAPHS");
+        return (e && atoi(e) == 0) ? 0 : 1;
+    }()};
+    return on;
+}
+bool graphs_enabled() { return graphs_flag().load() != 0; }
+template <typename T>
+void key_add(std::string &k, const T &v) {
+    k.append(reinterpret_cast<const char *>(&v), sizeof(T));
+}
+
 template <int NR, int TB>
-int run_cg(Ops o, const Layout &L, const gf_krylov_cfg &cfg, hipStream_t st, double *stats) {
+void enqueue_cg_chunk(const Ops &o, const Layout &L, bool mixed, int every, int s, int first_prev, hipStream_t st) {
+    const dim3 GS(o.gs), BS(TB), GU(o.gu), BU(RB), G1(1);
+    for (int j = 0; j < every; ++j) {
+        const int nb_prev = j == 0 ? first_prev : o.gu;
+        if (!mixed) {
+            hipLaunchKernelGGL((k_cg_dir<double, NR, false, TB>), GS, BS, 0, st, o, s++, nb_prev, 0, L.val64, L.dg64,
+                               L.z, L.p, L.p1, L.ap);
+            hipLaunchKernelGGL((k_cg_upd<double, NR>), GU, BU, 0, st, o, s++, L.inv64, L.x, L.r, L.z, L.p, L.p1, L.ap);
+        } else {
+            hipLaunchKernelGGL((k_cg_dir<float, NR, true, TB>), GS, BS, 0, st, o, s++, nb_prev, j == every - 1,
+                               L.val32, L.dgc, L.z32, L.p32, L.p32b, L.ap32);
+            hipLaunchKernelGGL((k_cg_upd<float, NR>), GU, BU, 0, st, o, s++, L.inv32, L.x32, L.r32, L.z32, L.p32,
+                               L.p32b, L.ap32);
+        }
+    }
+    if (mixed) {  // FP64 refinement (no-ops unless a component asked for it)
+        hipLaunchKernelGGL(k_mx_accumulate<NR>, GU, BU, 0, st, o, s++, L.x32, L.x);
+        hipLaunchKernelGGL((k_mx_residual<NR, TB>), GS, BS, 0, st, o, s++, 0, L.val64, L.dg64, L.inv32, L.b, L.x,
+                           L.r32, L.z32, L.x32);
+        hipLaunchKernelGGL(k_mx_refine<NR>, G1, BU, 0, st, o, s++, 0, o.gs, o.gu);
+    }
+}
+
+// Enqueue one chunk starting at sequence s, through a cached graph when possible.
+template <int NR, int TB>
+int launch_cg_chunk(const void *ws, const Ops &o, const Layout &L, bool mixed, int every, int s, int first_prev,
+                    hipStream_t st) {
+    if (!graphs_enabled()) {
+        enqueue_cg_chunk<NR, TB>(o, L, mixed, every, s, first_prev, st);
+        GF_CHECK_LAUNCH();
+        return GF_OK;
+    }
+    std::string key;
+    key_add(key, o);
+    key_add(key, L);
+    for (int v : {NR, TB, (int)mixed, every, s & 1, first_prev}) key_add(key, v);
+    GraphCache &c = graph_cache();
+    hipGraphExec_t exec = nullptr;
+    {
+        std::lock_guard<std::mutex> g(c.mu);
+        for (auto &kv : c.by_ws[ws])
+            if (kv.first == key) exec = kv.second;
+    }
+    if (!exec) {
+        // Capture on a side stream of this thread (the caller's stream may be
+        // the legacy default stream, which cannot be captured); the graph is
+        // then launched on the caller's stream.
+        static thread_local hipStream_t cap = nullptr;
+        if (!cap) GF_CHECK_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        hipGraph_t graph = nullptr;
+        GF_CHECK_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+        enqueue_cg_chunk<NR, TB>(o, L, mixed, every, s, first_prev, cap);
+        const hipError_t le = hipGetLastError();
+        const hipError_t ce = hipStreamEndCapture(cap, &graph);
+        if (le != hipSuccess || ce != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            set_error(std::string("cg chunk capture failed: ") + hipGetErrorString(le != hipSuccess ? le : ce));
+            return GF_EHIP;
+        }
+        const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ie != hipSuccess) {
+            set_error(std::string("cg chunk graph instantiation failed: ") + hipGetErrorString(ie));
+            return GF_EHIP;
+        }
+        std::lock_guard<std::mutex> g(c.mu);
+        c.by_ws[ws].emplace_back(key, exec);
+    }
+    GF_CHECK_HIP(hipGraphLaunch(exec, st));
+    return GF_OK;
+}
+
+template <int NR, int TB>
+int run_cg(const void *ws, Ops o, const Layout &L, const gf_krylov_cfg &cfg, hipStream_t st, double *stats) {
     Poller P;
     if (int rc = P.init()) return rc;
     const int every = cfg.check_every > 0 ? cfg.check_every : 8;
     int s = 1;  // launch sequence; state[0] holds the zeroed initial state
-    const dim3 GS(o.gs), BS(TB), GU(o.gu), BU(RB), G1(1);
-    int first_prev;  // partial sets left for the first k_cg_dir of a chunk
+    const dim3 GS(o.gs), BS(TB), BU(RB), G1(1);
+    // Partial sets the first k_cg_dir of a chunk reduces: the refinement's one
+    // reduced set in mixed precision; the update's in FP64 (the very first
+    // direction ignores them, so every chunk is the same launch sequence).
+    const int first_prev = cfg.mixed ? 1 : o.gu;
     if (!cfg.mixed) {
         hipLaunchKernelGGL((k_cg_init<NR, TB>), GS, BS, 0, st, o, s++, L.val64, L.dg64, L.inv64, L.b, L.x, L.r, L.z);
         hipLaunchKernelGGL(k_cg_init_fin<NR>, G1, BU, 0, st, o, s++, o.gs);
-        first_prev = 0;  // the first direction needs no partials
     } else {
         hipLaunchKernelGGL((k_mx_residual<NR, TB>), GS, BS, 0, st, o, s++, 1, L.val64, L.dg64, L.inv32, L.b, L.x, L.r32,
                            L.z32, L.x32);
         hipLaunchKernelGGL(k_mx_refine<NR>, G1, BU, 0, st, o, s++, 1, o.gs, o.gu);
-        first_prev = 1;
     }
     GF_CHECK_LAUNCH();
+    const int per_chunk = 2 * every + (cfg.mixed ? 3 : 0);
     int k = 1;
     const long cap = (long)cfg.max_iter + 4L * every;
     for (;;) {
-        for (int j = 0; j < every; ++j, ++k) {
-            const int nb_prev = j == 0 ? first_prev : o.gu;
-            if (!cfg.mixed) {
-                hipLaunchKernelGGL((k_cg_dir<double, NR, false, TB>), GS, BS, 0, st, o, s++, nb_prev, 0, L.val64, L.dg64,
-                                   L.z, L.p, L.p1, L.ap);
-                hipLaunchKernelGGL((k_cg_upd<double, NR>), GU, BU, 0, st, o, s++, L.inv64, L.x, L.r, L.z, L.p, L.p1,
-                                   L.ap);
-            } else {
-                hipLaunchKernelGGL((k_cg_dir<float, NR, true, TB>), GS, BS, 0, st, o, s++, nb_prev, j == every - 1,
-                                   L.val32, L.dgc,
-                                   L.z32, L.p32, L.p32b, L.ap32);
-                hipLaunchKernelGGL((k_cg_upd<float, NR>), GU, BU, 0, st, o, s++, L.inv32, L.x32, L.r32, L.z32, L.p32,
-                                   L.p32b, L.ap32);
-            }
-        }
-        if (cfg.mixed) {  // FP64 refinement (no-ops unless a component asked for it)
-            hipLaunchKernelGGL(k_mx_accumulate<NR>, GU, BU, 0, st, o, s++, L.x32, L.x);
-            hipLaunchKernelGGL((k_mx_residual<NR, TB>), GS, BS, 0, st, o, s++, 0, L.val64, L.dg64, L.inv32, L.b, L.x, L.r32,
-                               L.z32, L.x32);
-            hipLaunchKernelGGL(k_mx_refine<NR>, G1, BU, 0, st, o, s++, 0, o.gs, o.gu);
-            first_prev = 1;
-        } else {
-            first_prev = o.gu;
-        }
-        GF_CHECK_LAUNCH();
+        if (int rc = launch_cg_chunk<NR, TB>(ws, o, L, cfg.mixed != 0, every, s, first_prev, st)) return rc;
+        s += per_chunk;
+        k += every;
         bool done = false;
         auto all_done = [&](const KState *h) {
             bool a = true;
@@ -1451,7 +1538,9 @@ extern "C" int gf_krylov_solve(const gf_mesh *m, const gf_krylov_cfg *cfg, int32
     o.partials = L.partials;
     int rc;
     if (cfg->method == 0) {
-        auto cg = [&](auto nr, auto tb) { return run_cg<decltype(nr)::value, decltype(tb)::value>(o, L, *cfg, s, stats_out); };
+        auto cg = [&](auto nr, auto tb) {
+            return run_cg<decltype(nr)::value, decltype(tb)::value>(workspace, o, L, *cfg, s, stats_out);
+        };
         using I1 = std::integral_constant<int, 1>;
         using I3 = std::integral_constant<int, 3>;
         using T256 = std::integral_constant<int, 256>;
@@ -1473,6 +1562,25 @@ extern "C" int gf_krylov_solve(const gf_mesh *m, const gf_krylov_cfg *cfg, int32
         hipLaunchKernelGGL(k_from_soa, dim3(grid_for(C)), dim3(kBlock), 0, s, C, nrhs, L.x, x);
         GF_CHECK_LAUNCH();
     }
+    return GF_OK;
+}
+
+// Replay PCG chunks through cached graphs (1, the default) or launch them one
+// by one (0).  Returns the previous setting.
+extern "C" int gf_krylov_use_graphs(int32_t on) { return graphs_flag().exchange(on ? 1 : 0); }
+
+// Drop the launch graphs cached for a workspace (call before freeing it).
+extern "C" int gf_krylov_release(const void *workspace) {
+    GraphCache &c = graph_cache();
+    std::vector<std::pair<std::string, hipGraphExec_t>> v;
+    {
+        std::lock_guard<std::mutex> g(c.mu);
+        auto it = c.by_ws.find(workspace);
+        if (it == c.by_ws.end()) return GF_OK;
+        v.swap(it->second);
+        c.by_ws.erase(it);
+    }
+    for (auto &kv : v) GF_CHECK_HIP(hipGraphExecDestroy(kv.second));
     return GF_OK;
 }
 

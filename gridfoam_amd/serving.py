@@ -10,15 +10,18 @@ memory free up: there is no batch boundary.  The kernels are deterministic
 and never read another case's memory, so a result is bit-identical whether
 its case ran alone or in the mix.
 
-Each slot is a host thread (ctypes releases the GIL inside every native
-call, including the Krylov solves' device-state polls) with its own HIP
-stream; streams keep the default priority.
+Slots are worker processes by default (one Python interpreter, HIP context
+and queue per slot, so neither the interpreter lock nor one queue serialises
+the cases' host work and launches); a thread per slot with its own stream is
+the lighter alternative.  Streams keep the default priority.
 """
 
 from __future__ import annotations
 
 import copy
 import math
+import multiprocessing as mp
+import queue as queue_mod
 import threading
 import time
 from dataclasses import dataclass
@@ -127,20 +130,119 @@ def run_case(cfg: dict, iterations: int) -> tuple:
     return grid.num_cells, U, p, p_its
 
 
-class SolveServer:
-    """Admits requests by free slot and HBM footprint and runs each on its own stream."""
+def _worker_main(wid: int, req_q, res_q) -> None:
+    """Slot process: owns its HIP context; runs one request at a time."""
+    torch.cuda.set_device(0)
+    warm = random_case(np.random.default_rng(0), n_range=(8, 8))
+    run_case(warm, 1)
+    res_q.put(("ready", wid))
+    while True:
+        item = req_q.get()
+        if item is None:
+            break
+        rid, cfg, iterations = item
+        start = time.time()
+        try:
+            cells, U, p, its = run_case(cfg, iterations)
+        except BaseException as exc:  # report and keep serving
+            res_q.put(("error", wid, rid, repr(exc)))
+            continue
+        res_q.put(("done", wid, rid, cells, U.numpy(), p.numpy(), its, start, time.time()))
 
-    def __init__(self, slots: int = 4, hbm_budget: Optional[int] = None):
+
+class SolveServer:
+    """Admits requests by free slot and HBM footprint; each admitted case runs
+    in a slot of its own (a worker process by default, or a thread with its
+    own stream)."""
+
+    def __init__(self, slots: int = 4, hbm_budget: Optional[int] = None, mode: str = "process"):
         if slots < 1:
             raise ValueError("slots must be >= 1")
+        if mode not in ("process", "thread"):
+            raise ValueError("mode must be 'process' or 'thread'")
         self.slots = slots
+        self.mode = mode
         if hbm_budget is None:
             free, _ = torch.cuda.mem_get_info(0)
             hbm_budget = int(0.6 * free)
         self.hbm_budget = int(hbm_budget)
-        self._streams = [torch.cuda.Stream(device=0) for _ in range(slots)]
+        self._streams = [torch.cuda.Stream(device=0) for _ in range(slots)] if mode == "thread" else []
+        self._procs: List = []
+        if mode == "process":
+            ctx = mp.get_context("spawn")
+            self._res_q = ctx.Queue()
+            self._req_qs = [ctx.Queue() for _ in range(slots)]
+            self._procs = [ctx.Process(target=_worker_main, args=(w, self._req_qs[w], self._res_q), daemon=True)
+                           for w in range(slots)]
+            for pr in self._procs:
+                pr.start()
+            ready = 0
+            while ready < slots:
+                msg = self._res_q.get(timeout=600)
+                if msg[0] == "ready":
+                    ready += 1
+
+    def close(self) -> None:
+        for q in getattr(self, "_req_qs", []):
+            q.put(None)
+        for pr in self._procs:
+            pr.join(timeout=60)
+        self._procs = []
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
     def run(self, requests: Sequence[Request]) -> List[Result]:
+        """Serve requests that arrive at their ``arrival`` offsets (seconds
+        after this call); returns the results in completion order."""
+        if self.mode == "process":
+            return self._run_processes(requests)
+        return self._run_threads(requests)
+
+    def _run_processes(self, requests: Sequence[Request]) -> List[Result]:
+        order = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        t0 = time.time()
+        pending: List[Request] = []
+        free = list(range(self.slots))
+        in_use, running, nxt = 0, {}, 0
+        results: List[Result] = []
+        by_id = {r.rid: r for r in order}
+        while len(results) < len(order):
+            now = time.time() - t0
+            while nxt < len(order) and order[nxt].arrival <= now:
+                pending.append(order[nxt])
+                nxt += 1
+            while free and pending:
+                pick = next((r for r in pending
+                             if not running or in_use + footprint_bytes(r.config) <= self.hbm_budget), None)
+                if pick is None:
+                    break
+                pending.remove(pick)
+                w = free.pop()
+                need = footprint_bytes(pick.config)
+                in_use += need
+                running[pick.rid] = (w, need)
+                self._req_qs[w].put((pick.rid, pick.config, pick.iterations))
+            wait = 0.05 if nxt >= len(order) else max(0.0, min(0.05, order[nxt].arrival - (time.time() - t0)))
+            try:
+                msg = self._res_q.get(timeout=wait) if wait > 0 else self._res_q.get_nowait()
+            except queue_mod.Empty:
+                continue
+            if msg[0] == "error":
+                raise RuntimeError(f"request {msg[2]} failed in slot {msg[1]}: {msg[3]}")
+            _, w, rid, cells, U, p, its, start, end = msg
+            _, need = running.pop(rid)
+            in_use -= need
+            free.append(w)
+            req = by_id[rid]
+            results.append(Result(rid, cells, torch.from_numpy(U), torch.from_numpy(p), its, req.arrival,
+                                  start - t0, end - t0))
+        return results
+
+    def _run_threads(self, requests: Sequence[Request]) -> List[Result]:
         """Serve requests that arrive at their ``arrival`` offsets (seconds
         after this call); returns the results in completion order."""
         order = sorted(requests, key=lambda r: (r.arrival, r.rid))

@@ -46,7 +46,8 @@ def residual_threshold(atol: float, rtol: float, ref_norm: torch.Tensor) -> torc
 
 
 class _Workspace:
-    """Krylov workspace per (grid, nrhs), allocated once and reused."""
+    """Krylov workspace per (grid, nrhs), allocated once and reused.  The
+    library caches launch graphs per workspace; they are released with it."""
 
     def __init__(self):
         self._bufs: Dict[tuple, torch.Tensor] = {}
@@ -58,9 +59,22 @@ class _Workspace:
         key = (id(grid), nrhs)
         buf = self._bufs.get(key)
         if buf is None or buf.numel() < nbytes:
+            if buf is not None:
+                _release(buf)
             buf = torch.empty(nbytes, dtype=torch.uint8, device=grid.device)
             self._bufs[key] = buf
         return buf
+
+    def __del__(self):
+        for buf in self._bufs.values():
+            _release(buf)
+
+
+def _release(buf: torch.Tensor) -> None:
+    try:
+        nat.lib().gf_krylov_release(ctypes.c_void_p(buf.data_ptr()))
+    except Exception:  # interpreter shutdown: the library may already be gone
+        pass
 
 
 class LinearSolver:

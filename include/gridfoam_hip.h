@@ -253,6 +253,12 @@ int gf_krylov_solve(const gf_mesh *m, const gf_krylov_cfg *cfg, int32_t nrhs, in
                     const double *diag, const double *upper, const double *lower,
                     const double *b /* [C,nrhs] */, double *x /* [C,nrhs], in/out */,
                     void *workspace, double *stats_out, gf_stream_t stream);
+/* PCG chunks of check_every iterations are replayed through HIP graphs cached
+ * per workspace (no counterpart in the reference: launch-overhead control).
+ * gf_krylov_use_graphs(0) launches them one by one; returns the previous
+ * setting.  gf_krylov_release drops a workspace's graphs before it is freed. */
+int gf_krylov_use_graphs(int32_t on);
+int gf_krylov_release(const void *workspace);
 /* ldu_grads.py:11-53: gd = -sum lambda x, gu = -sum lambda_O x_N, gl = -sum lambda_N x_O */
 int gf_ldu_grads(const gf_mesh *m, int32_t nc, const double *lambda, const double *x,
                  double *g_diag, double *g_upper, double *g_lower, gf_stream_t stream);

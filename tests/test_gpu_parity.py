@@ -604,9 +604,48 @@ def test_served_results_equal_the_same_cases_alone():
     _say("serving: 5 cases alone")
     alone = [run_case(c, 2) for c in cfgs]
     _say("serving: same 5 cases through the server")
-    res = SolveServer(slots=3).run([Request(k, c, arrival=0.01 * k, iterations=2) for k, c in enumerate(cfgs)])
+    with SolveServer(slots=3) as srv:
+        res = srv.run([Request(k, c, arrival=0.01 * k, iterations=2) for k, c in enumerate(cfgs)])
+    _say("serving: same 5 cases through the threaded server")
+    res_t = SolveServer(slots=3, mode="thread").run([Request(k, c, 0.0, 2) for k, c in enumerate(cfgs)])
+    for r in res_t:
+        assert torch.equal(r.U, alone[r.rid][1]) and torch.equal(r.p, alone[r.rid][2])
     assert sorted(r.rid for r in res) == list(range(5))
     for r in res:
         cells, U, p, its = alone[r.rid]
         assert r.cells == cells and r.p_iterations == its
         assert torch.equal(r.U, U) and torch.equal(r.p, p)
+
+
+# ----------------------------------------------------------------- launch graphs
+@pytest.mark.parametrize("mixed", [False, True])
+def test_graph_replay_matches_eager_launches(mixed):
+    """PCG chunks replayed through cached HIP graphs give bit-identical
+    solutions to the same launches made one by one, on every replay, with
+    eager work and synchronisations between the solves."""
+    from gridfoam_amd import _native as nat
+    from gridfoam_amd.config import SolverConfig
+    from gridfoam_amd.fv import fvm
+    from gridfoam_amd.fvmatrix import Equation
+    from gridfoam_amd.solvers import create_solver
+
+    g, o = _grid("walls8")
+    P, oP, gamma, b = _poisson(g, o)
+    pm = fvm.assemble(P, lap=-1.0, gamma=torch.tensor(gamma, device=g.device), corrected=False)
+    pm.diag[0] = 2 * pm.diag[0]
+    pm.source = torch.tensor(b, device=g.device)
+    cfg = SolverConfig(method="cg", preconditioner="jacobi", tolerance=0.0, rel_tolerance=1e-10, max_iter=2000,
+                       mixed_precision=mixed)
+    _say(f"graphs vs eager mixed={mixed}")
+    prev = nat.lib().gf_krylov_use_graphs(0)
+    try:
+        ref = create_solver(cfg).solve(Equation(P, pm))
+        x_ref, it_ref = ref.solution.clone(), ref.stats[0].iterations
+        nat.lib().gf_krylov_use_graphs(1)
+        solver = create_solver(cfg)
+        for k in range(3):
+            res = solver.solve(Equation(P, pm))
+            assert torch.equal(res.solution, x_ref) and res.stats[0].iterations == it_ref, k
+            _ = (res.solution * 2.0).sum().item()  # eager work and a synchronisation between replays
+    finally:
+        nat.lib().gf_krylov_use_graphs(prev)
