@@ -10,10 +10,10 @@ memory free up: there is no batch boundary.  The kernels are deterministic
 and never read another case's memory, so a result is bit-identical whether
 its case ran alone or in the mix.
 
-Slots are worker processes by default (one Python interpreter, HIP context
-and queue per slot, so neither the interpreter lock nor one queue serialises
-the cases' host work and launches); a thread per slot with its own stream is
-the lighter alternative.  Streams keep the default priority.
+Slots are threads with a HIP stream each by default (ctypes releases the
+interpreter lock inside every native call, including the Krylov solves'
+device-state polls); worker processes (one interpreter and HIP context per
+slot) are the alternative.  Streams keep the default priority.
 """
 
 from __future__ import annotations
@@ -152,10 +152,10 @@ def _worker_main(wid: int, req_q, res_q) -> None:
 
 class SolveServer:
     """Admits requests by free slot and HBM footprint; each admitted case runs
-    in a slot of its own (a worker process by default, or a thread with its
-    own stream)."""
+    in a slot of its own (a thread with its own stream by default, or a
+    worker process)."""
 
-    def __init__(self, slots: int = 4, hbm_budget: Optional[int] = None, mode: str = "process"):
+    def __init__(self, slots: int = 4, hbm_budget: Optional[int] = None, mode: str = "thread"):
         if slots < 1:
             raise ValueError("slots must be >= 1")
         if mode not in ("process", "thread"):

@@ -606,8 +606,9 @@ def test_served_results_equal_the_same_cases_alone():
     _say("serving: same 5 cases through the server")
     with SolveServer(slots=3) as srv:
         res = srv.run([Request(k, c, arrival=0.01 * k, iterations=2) for k, c in enumerate(cfgs)])
-    _say("serving: same 5 cases through the threaded server")
-    res_t = SolveServer(slots=3, mode="thread").run([Request(k, c, 0.0, 2) for k, c in enumerate(cfgs)])
+    _say("serving: same 5 cases through slot processes")
+    with SolveServer(slots=3, mode="process") as srv:
+        res_t = srv.run([Request(k, c, 0.0, 2) for k, c in enumerate(cfgs)])
     for r in res_t:
         assert torch.equal(r.U, alone[r.rid][1]) and torch.equal(r.p, alone[r.rid][2])
     assert sorted(r.rid for r in res) == list(range(5))
