@@ -303,15 +303,21 @@ def main() -> int:
     for mode in (("plain", "mixed") if a.mode == "both" else (a.mode,)):
         _log(f"{mode}: {a.warmup} warmup + {a.steps} timed steps")
         grid, alg, dt, iters = run_gpu(mode == "mixed", a.steps, a.warmup)
+        from gridfoam_amd.algorithms import compute_continuity_error
+        from gridfoam_amd.fv import fvc
+
         out[mode] = dict(grid=grid, alg=alg, dt=dt, iters=iters,
-                         U=alg.U.data.detach().cpu().numpy(), p=alg.p.data.detach().cpu().numpy())
+                         U=alg.U.data.detach().cpu().numpy(), p=alg.p.data.detach().cpu().numpy(),
+                         gradp=fvc.grad(alg.p).cpu().numpy(),
+                         continuity=compute_continuity_error(alg.phi, grid.dt)[0])
         _log(f"{mode}: {dt:.2f} s, {dt / a.steps * 1e3:.1f} ms/step, {iters / a.steps:.0f} p-CG iterations/step")
     head = "mixed" if "mixed" in out else "plain"
     rel = None
     if "mixed" in out and "plain" in out:
         P, M = out["plain"], out["mixed"]
         rel = {"U": float(np.linalg.norm(M["U"] - P["U"]) / np.linalg.norm(P["U"])),
-               "p": float(np.linalg.norm(M["p"] - P["p"]) / np.linalg.norm(P["p"]))}
+               "p": float(np.linalg.norm(M["p"] - P["p"]) / np.linalg.norm(P["p"])),
+               "grad_p": float(np.linalg.norm(M["gradp"] - P["gradp"]) / np.linalg.norm(P["gradp"]))}
         if max(rel.values()) > 1e-6:
             _log(f"mixed precision error {rel} exceeds 1e-6: headline falls back to plain FP64")
             head = "plain"
@@ -338,14 +344,12 @@ def main() -> int:
                    "cells": C, "internal_faces": grid.num_internal_faces, "parallelism": "single GPU"},
         "roofline": roof,
     }
-    if "plain" in out:
-        line["plain_fp64"] = {"value": round(C * a.steps / out["plain"]["dt"] / 1e6, 3),
-                              "ms_per_step": round(out["plain"]["dt"] / a.steps * 1e3, 3),
-                              "p_cg_iters_per_step": round(out["plain"]["iters"] / a.steps, 1)}
-    if "mixed" in out:
-        line["mixed"] = {"value": round(C * a.steps / out["mixed"]["dt"] / 1e6, 3),
-                         "ms_per_step": round(out["mixed"]["dt"] / a.steps * 1e3, 3),
-                         "p_cg_iters_per_step": round(out["mixed"]["iters"] / a.steps, 1)}
+    for mode, key in (("plain", "plain_fp64"), ("mixed", "mixed")):
+        if mode in out:
+            line[key] = {"value": round(C * a.steps / out[mode]["dt"] / 1e6, 3),
+                         "ms_per_step": round(out[mode]["dt"] / a.steps * 1e3, 3),
+                         "p_cg_iters_per_step": round(out[mode]["iters"] / a.steps, 1),
+                         "continuity_error_local_fp64": out[mode]["continuity"]}
     roof["copy_peak_GBs"] = copy_peak()
     if not a.no_large:
         _log("SpMV roofline on a uniform 256^3 mesh (working set above the 256 MB MALL)")

@@ -172,6 +172,18 @@ def correct_velocity(U: CellField, HbyA: CellField, rAtU: CellField, p: CellFiel
              nat.ptr(gp), nat.ptr(U.data), nat.stream())
 
 
+def compute_continuity_error(phi: FaceField, delta_t: float) -> Tuple[float, float]:
+    """post/diagnostics/continuity.py:12-49: (local, global) volume-weighted
+    continuity errors of the face flux, computed in FP64 on the GPU."""
+    g = phi.grid
+    div = torch.empty(g.num_cells, dtype=torch.float64, device=g.device)
+    out = torch.empty(2, dtype=torch.float64, device=g.device)
+    nat.call("gf_continuity_error", g.mesh_ref, nat.ptr(phi.single_buffer), nat.ptr(phi.bnd_buffer),
+             ctypes.c_double(delta_t), nat.ptr(div), nat.ptr(out), nat.stream())
+    local, glob = out.tolist()
+    return local, glob
+
+
 def continuity_residual(phi: FaceField) -> float:
     """algorithms/utils/residual.py:39-53: |div(phi)|_2."""
     return float(torch.linalg.vector_norm(fvc.div(phi)))

@@ -765,6 +765,37 @@ extern "C" int gf_fvc_div(const gf_mesh *m, int32_t nc, const double *phi_s, con
     return GF_OK;
 }
 
+// Volume-weighted continuity errors of a face flux, FP64, one workgroup in a
+// fixed order (post/diagnostics/continuity.py:12-49): out = {dt sum|div| V / sum V,
+// dt sum div V / sum V}.
+__global__ __launch_bounds__(1024) void k_continuity(int C, const double *__restrict__ div,
+                                                     const double *__restrict__ vol, double dt,
+                                                     double *__restrict__ out) {
+    double a[3] = {0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < C; i += 1024) {
+        const double v = vol[i], d = div[i];
+        a[0] += fabs(d) * v;
+        a[1] += d * v;
+        a[2] += v;
+    }
+    const bool mx[3] = {false, false, false};
+    block_reduce<3, 1024>(a, mx);
+    if (threadIdx.x == 0) {
+        out[0] = dt * a[0] / a[2];
+        out[1] = dt * a[1] / a[2];
+    }
+}
+
+extern "C" int gf_continuity_error(const gf_mesh *m, const double *phi_s, const double *phi_b, double dt,
+                                   double *div_scratch, double *out, gf_stream_t stream) {
+    GF_REQUIRE(m && phi_s && phi_b && div_scratch && out, "null argument");
+    hipStream_t s = S(stream);
+    GF_NC_DISPATCH(1, k_fvc_div, grid_for(m->n_cells), *m, phi_s, phi_b, div_scratch);
+    hipLaunchKernelGGL(k_continuity, dim3(1), dim3(1024), 0, s, m->n_cells, div_scratch, m->volumes, dt, out);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
 extern "C" int gf_sn_grad_hang(const gf_mesh *m, int32_t nc, const double *grad, double *corr, gf_stream_t stream) {
     GF_REQUIRE(m && grad && corr, "null argument");
     hipStream_t s = S(stream);

@@ -127,6 +127,11 @@ int gf_interpolate(const gf_mesh *m, int32_t nc, const double *psi, const double
 /* fvc/div.py:15-64 */
 int gf_fvc_div(const gf_mesh *m, int32_t nc, const double *phi_s, const double *phi_b,
                double *out, gf_stream_t stream);
+/* FP64 mass-conservation check of a face flux (post/diagnostics/continuity.py:12-49):
+ * out[0] = dt sum(|div phi| V)/sum V (local), out[1] = dt sum(div phi V)/sum V (global);
+ * div_scratch [C] receives div(phi) */
+int gf_continuity_error(const gf_mesh *m, const double *phi_s, const double *phi_b, double dt,
+                        double *div_scratch, double *out, gf_stream_t stream);
 /* fvc/sn_grad + face_interpolation.py:200-246: hanging snGrad correction [H,nc] */
 int gf_sn_grad_hang(const gf_mesh *m, int32_t nc, const double *grad, double *corr,
                     gf_stream_t stream);

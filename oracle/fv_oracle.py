@@ -1086,6 +1086,14 @@ class Case:
             raise ValueError(f"unsupported algorithm {self.kind}")
 
 
+def continuity_error(g: Grid, phi: FaceValues, dt: float) -> tuple[float, float]:
+    """post/diagnostics/continuity.py:12-49: dt * volume-weighted mean of
+    |div phi| (local) and of div phi (global)."""
+    d = fvc_div(g, phi)
+    V = g.cell_volumes
+    return float(dt * np.sum(np.abs(d) * V) / np.sum(V)), float(dt * np.sum(d * V) / np.sum(V))
+
+
 def continuity_max(g: Grid, phi: FaceValues) -> float:
     return float(np.max(np.abs(fvc_div(g, phi))))
 

@@ -650,3 +650,29 @@ def test_graph_replay_matches_eager_launches(mixed):
             _ = (res.solution * 2.0).sum().item()  # eager work and a synchronisation between replays
     finally:
         nat.lib().gf_krylov_use_graphs(prev)
+
+
+def test_continuity_error_matches_oracle():
+    """FP64 mass-conservation check on the GPU (continuity.py:12-49) on a flux
+    with divergence (the predicted phiHbyA) and on the corrected flux."""
+    from gridfoam_amd.algorithms import compute_continuity_error, create_case
+    from gridfoam_amd.config import load_config
+
+    cfg = small_cavity3d(root=6, steps=2)
+    grid, alg = create_case(load_config(cfg))
+    alg.step()
+    _say("continuity error")
+    o = fo.Grid(**{k: v for k, v in zip(("lower", "upper", "root", "regions", "n_leaf"),
+                                       (cfg["fluxel"]["domain"]["lower"], cfg["fluxel"]["domain"]["upper"],
+                                        cfg["fluxel"]["root_resolution"],
+                                        [(r["min"], r["max"], r["level"]) for r in cfg["fluxel"]["refinement_regions"]],
+                                        0))}, dt=grid.dt)
+    for phi in (alg.phi_hbya, alg.phi):
+        F, B = grid.num_internal_faces, grid.num_domain_bnd_faces
+        ophi = fo.FaceValues(_np(phi.single_buffer)[:F].copy(), _np(phi.bnd_buffer)[:B].copy())
+        loc, glo = compute_continuity_error(phi, grid.dt)
+        oloc, oglo = fo.continuity_error(o, ophi, grid.dt)
+        assert abs(loc - oloc) <= 1e-12 * max(abs(oloc), 1e-300) + 1e-300
+        assert abs(glo - oglo) <= 1e-12 * max(abs(oloc), 1e-300) + 1e-300
+    # the pressure solve (rel. tolerance 1e-8) removes the divergence of the predicted flux
+    assert compute_continuity_error(alg.phi, grid.dt)[0] < 1e-6 * compute_continuity_error(alg.phi_hbya, grid.dt)[0]

@@ -245,11 +245,15 @@ def test_div_of_linear_velocity_is_one():
         U.bcs[patch] = fo.BC("neumann", np.array([(2 * (d % 2) - 1) * (d // 2 == 0), 0.0, 0.0], float))
     phi = fo.flux_from_face_velocity(g, fo.interpolate(g, U), U)
     np.testing.assert_allclose(fo.fvc_div(g, phi), 1.0, atol=1e-12)
+    # continuity errors (post/diagnostics/continuity.py:12-49): dt * 1 for a unit divergence
+    loc, glo = fo.continuity_error(g, phi, 0.25)
+    assert abs(loc - 0.25) < 1e-12 and abs(glo - 0.25) < 1e-12
     U.data = np.ones_like(U.data)
     for patch in fo.PATCHES:
         U.bcs[patch] = fo.BC("dirichlet", np.ones(3))
     phi = fo.flux_from_face_velocity(g, fo.interpolate(g, U), U)
     np.testing.assert_allclose(fo.fvc_div(g, phi), 0.0, atol=1e-12)
+    assert max(abs(v) for v in fo.continuity_error(g, phi, 0.25)) < 1e-12
 
 
 def test_ddt_weights():
