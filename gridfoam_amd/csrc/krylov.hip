@@ -196,16 +196,72 @@ __device__ __forceinline__ void store_state(const Ops &o, int s, const KState *s
     if (blockIdx.x == 0 && threadIdx.x < NR) o.st[(s & 1) * kMaxRhs + threadIdx.x] = sh[threadIdx.x];
 }
 
-// Reduce the previous launch's partials: tot[r][q] in every thread.
+// State of the previous launch read with scalar loads: it is uniform across
+// the grid, lands in SGPRs of every wave, and needs no LDS broadcast (the
+// scalar cache is invalidated at every kernel start).
+typedef const __attribute__((address_space(4))) KState ConstKState;
+__device__ __forceinline__ KState state_in(const Ops &o, int s, int r) {
+    ConstKState *c = (ConstKState *)(o.st + ((s - 1) & 1) * kMaxRhs + r);
+    KState v;
+    v.thresh = c->thresh;
+    v.norm0 = c->norm0;
+    v.final_res = c->final_res;
+    v.rz_old = c->rz_old;
+    v.alpha = c->alpha;
+    v.beta = c->beta;
+    v.rho_old = c->rho_old;
+    v.omega = c->omega;
+    v.rhat2 = c->rhat2;
+    v.scale = c->scale;
+    v.iter = c->iter;
+    v.phase = c->phase;
+    v.converged = c->converged;
+    v.restarts = c->restarts;
+    v.fresh = c->fresh;
+    v.pcur = c->pcur;
+    return v;
+}
+// Workgroup 0 writes this launch's state (every wave holds the same copy).
+template <int NR>
+__device__ __forceinline__ void state_out(const Ops &o, int s, const KState (&S)[NR]) {
+    if (blockIdx.x != 0) return;
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+        if (threadIdx.x == r) o.st[(s & 1) * kMaxRhs + r] = S[r];
+}
+
+// First share of the previous launch's partials (index threadIdx.x), loaded
+// ahead of everything that waits on the solver state.
+template <int NR, int NQ>
+__device__ __forceinline__ void prefetch_prev(const Ops &o, int s, int nb_prev, double (&x)[NR][NQ]) {
+    const double *P = o.partials + (size_t)((s - 1) & 1) * kMaxRhs * kNQ * o.pstride;
+    const int j = threadIdx.x;
+#pragma unroll
+    for (int r = 0; r < NR; ++r)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) x[r][q] = j < nb_prev ? P[((size_t)r * kNQ + q) * o.pstride + j] : 0.0;
+}
+
+// Reduce the previous launch's partials: tot[r][q] in every thread.  With
+// `pre`, the first share was prefetched (same summation order either way).
 template <int NR, int NQ, int BS = RB>
 __device__ __forceinline__ void reduce_prev(const Ops &o, int s, int nb_prev, const bool (&is_max)[NQ],
-                                            double (&tot)[NR][NQ]) {
+                                            double (&tot)[NR][NQ], const double (*pre)[NQ] = nullptr) {
     __shared__ double lds[NR * NQ][BS / kWave];
     __shared__ double res[NR * NQ];
     const double *P = o.partials + (size_t)((s - 1) & 1) * kMaxRhs * kNQ * o.pstride;
     const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
     double acc[NR][NQ] = {};
-    for (int j = threadIdx.x; j < nb_prev; j += BS) {  // all quantities' loads issued together
+    if (pre) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const double x = pre[r][q];
+                acc[r][q] = is_max[q] ? fmax(acc[r][q], x) : acc[r][q] + x;
+            }
+    }
+    for (int j = threadIdx.x + (pre ? BS : 0); j < nb_prev; j += BS) {  // all quantities' loads issued together
         double x[NR][NQ];
 #pragma unroll
         for (int r = 0; r < NR; ++r)
@@ -358,40 +414,26 @@ constexpr int kHead = 6;
 // Mixed precision (MX): the test only decides whether the FP64 refinement is due;
 // the reliable-update trigger is only taken where the host queued a
 // refinement right after this iteration (may_refine), so no launches idle.
-// The row's matrix entries, its own operands and the first batch of gathered
-// neighbours are loaded before the workgroup reduces the previous partials,
-// so those loads are in flight during the reduction.
+// Everything that does not depend on the solver state (the previous
+// partials, the row's matrix entries, z at the row and at its first
+// neighbours) is loaded before the state is read; p_old follows once the
+// state names its buffer, and all of it is in flight during the reduction.
+// Every wave applies the state transition itself (same inputs, same result).
 template <typename VT, int NR, bool MX, int TB>
 __global__ __launch_bounds__(TB) void k_cg_dir(Ops o, int s, int nb_prev, int may_refine, const VT *__restrict__ val,
                                                const double *__restrict__ dg, const VT *__restrict__ z,
                                                VT *__restrict__ p0, VT *__restrict__ p1, VT *__restrict__ ap) {
-    __shared__ KState sh[NR];
-    __shared__ int start_s[NR];
-    load_state<NR>(o, s, sh);
-    if (!any_phase<NR>(sh, PH_ACTIVE)) {
-        store_state<NR>(o, s, sh);
-        return;
-    }
+    double pre[NR][2];
+    prefetch_prev<NR, 2>(o, s, nb_prev, pre);
     const int i = xcd_tile(blockIdx.x, gridDim.x) * TB + threadIdx.x;
     const int ic = i < o.C ? i : o.C - 1;
-    const VT *pold[NR];
-    VT *pnew[NR];
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {  // current direction in buffer pcur; the new one goes to the other
-        pold[r] = (sh[r].pcur ? p1 : p0) + (size_t)r * o.C;
-        pnew[r] = (sh[r].pcur ? p0 : p1) + (size_t)r * o.C;
-    }
-    // ---- loads issued ahead of the reduction
     const int sl = __builtin_amdgcn_readfirstlane(ic >> 6), lane = ic & 63;
     const int base = o.sell_ptr[sl];
     const int W = (o.sell_ptr[sl + 1] - base) >> 6;
     const double d = dg[ic];
-    VT zi[NR], poi[NR];
+    VT zi[NR];
 #pragma unroll
-    for (int r = 0; r < NR; ++r) {
-        zi[r] = z[(size_t)r * o.C + ic];
-        poi[r] = pold[r][ic];
-    }
+    for (int r = 0; r < NR; ++r) zi[r] = z[(size_t)r * o.C + ic];
     int col[kHead];
     VT v[kHead];
 #pragma unroll
@@ -404,61 +446,82 @@ __global__ __launch_bounds__(TB) void k_cg_dir(Ops o, int s, int nb_prev, int ma
             v[u] = __builtin_nontemporal_load(val + e);
         }
     }
-    VT zj[kHead][NR], pj[kHead][NR];
+    VT zj[kHead][NR];
 #pragma unroll
     for (int u = 0; u < kHead; ++u)
 #pragma unroll
-        for (int r = 0; r < NR; ++r) {
-            zj[u][r] = z[(size_t)r * o.C + col[u]];
-            pj[u][r] = pold[r][col[u]];
-        }
+        for (int r = 0; r < NR; ++r) zj[u][r] = z[(size_t)r * o.C + col[u]];
+    asm volatile("" ::: "memory");  // keep the loads above issued before the state test
+    KState S[NR];
+    bool any = false;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        S[r] = state_in(o, s, r);
+        any |= S[r].phase == PH_ACTIVE;
+    }
+    if (!any) {
+        state_out<NR>(o, s, S);
+        return;
+    }
+    const VT *pold[NR];
+    VT *pnew[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {  // current direction in buffer pcur; the new one goes to the other
+        pold[r] = (S[r].pcur ? p1 : p0) + (size_t)r * o.C;
+        pnew[r] = (S[r].pcur ? p0 : p1) + (size_t)r * o.C;
+    }
+    VT poi[NR], pj[kHead][NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) poi[r] = pold[r][ic];
+#pragma unroll
+    for (int u = 0; u < kHead; ++u)
+#pragma unroll
+        for (int r = 0; r < NR; ++r) pj[u][r] = pold[r][col[u]];
     // ---- previous partials: |r|, r.z
     const bool mx[2] = {(bool)o.norm_inf, false};
     double tot[NR][2];
-    reduce_prev<NR, 2, TB>(o, s, nb_prev, mx, tot);
-    if (threadIdx.x == 0) {
-        for (int r = 0; r < NR; ++r) {
-            KState &S = sh[r];
-            start_s[r] = S.fresh == 1;
-            if (S.phase != PH_ACTIVE) continue;
-            if (S.fresh == 0) {
-                const double nr = nrm_fin(tot[r][0], o.norm_inf);
-                S.iter += 1;
-                if (!MX) {
-                    S.final_res = nr;
-                    if (nr < S.thresh) {
-                        S.converged = 1;
-                        S.phase = PH_DONE;
-                        continue;
-                    }
-                    if (S.iter >= o.max_iter) {
-                        S.phase = PH_DONE;
-                        continue;
-                    }
-                } else if (nr < S.thresh || S.iter >= o.max_iter || (may_refine && nr < o.reliable * S.scale)) {
-                    S.phase = PH_RESTART;  // FP64 refinement due
-                    continue;
-                }
-            }
-            if (S.fresh != 1) {
-                S.beta = tot[r][1] / S.rz_old;
-                S.rz_old = tot[r][1];
-            } else {
-                S.beta = 0.0;
-            }
-            S.fresh = 0;
-            S.pcur ^= 1;
-        }
-    }
-    lds_sync();
+    reduce_prev<NR, 2, TB>(o, s, nb_prev, mx, tot, pre);
     bool act[NR], start[NR];
-    VT be[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
-        act[r] = sh[r].phase == PH_ACTIVE;
-        start[r] = start_s[r] != 0;
-        be[r] = (VT)sh[r].beta;
+        KState &St = S[r];
+        start[r] = St.fresh == 1;
+        if (St.phase == PH_ACTIVE) {
+            bool go = true;
+            if (St.fresh == 0) {
+                const double nr = nrm_fin(tot[r][0], o.norm_inf);
+                St.iter += 1;
+                if (!MX) {
+                    St.final_res = nr;
+                    if (nr < St.thresh) {
+                        St.converged = 1;
+                        St.phase = PH_DONE;
+                        go = false;
+                    } else if (St.iter >= o.max_iter) {
+                        St.phase = PH_DONE;
+                        go = false;
+                    }
+                } else if (nr < St.thresh || St.iter >= o.max_iter || (may_refine && nr < o.reliable * St.scale)) {
+                    St.phase = PH_RESTART;  // FP64 refinement due
+                    go = false;
+                }
+            }
+            if (go) {
+                if (St.fresh != 1) {
+                    St.beta = tot[r][1] / St.rz_old;
+                    St.rz_old = tot[r][1];
+                } else {
+                    St.beta = 0.0;
+                }
+                St.fresh = 0;
+                St.pcur ^= 1;
+            }
+        }
+        act[r] = St.phase == PH_ACTIVE;
     }
+    VT be[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) be[r] = (VT)S[r].beta;
     double acc[NR];
     VT pi[NR];
 #pragma unroll
@@ -510,7 +573,7 @@ __global__ __launch_bounds__(TB) void k_cg_dir(Ops o, int s, int nb_prev, int ma
     }
     const bool mx1[1] = {false};
     write_partials<NR, 1, TB>(o, s, pv, mx1);
-    store_state<NR>(o, s, sh);
+    state_out<NR>(o, s, S);
 }
 
 // alpha = rz_old / p.Ap ; x += alpha p ; r -= alpha Ap ; z = M^-1 r ; partials |r|, r.z.
@@ -522,17 +585,10 @@ __global__ __launch_bounds__(RB) void k_cg_upd(Ops o, int s, const VT *__restric
                                                VT *__restrict__ r_, VT *__restrict__ z_, const VT *__restrict__ p0,
                                                const VT *__restrict__ p1, const VT *__restrict__ ap_) {
     constexpr int kHoist = (sizeof(VT) == 4 ? 4 : 2) / NR > 0 ? (sizeof(VT) == 4 ? 4 : 2) / NR : 1;
-    __shared__ KState sh[NR];
-    load_state<NR>(o, s, sh);
-    if (!any_phase<NR>(sh, PH_ACTIVE)) {
-        store_state<NR>(o, s, sh);
-        return;
-    }
+    double pre[NR][1];
+    prefetch_prev<NR, 1>(o, s, o.gs, pre);
     int lo, hi;
     chunk_range(o.C, gridDim.x, xcd_tile(blockIdx.x, gridDim.x), lo, hi);
-    const VT *p_[NR];
-#pragma unroll
-    for (int r = 0; r < NR; ++r) p_[r] = (sh[r].pcur ? p1 : p0) + (size_t)r * o.C;
     VT hx[kHoist][NR], hr[kHoist][NR], hp[kHoist][NR], ha[kHoist][NR], hid[kHoist];
 #pragma unroll
     for (int h = 0; h < kHoist; ++h) {
@@ -544,24 +600,42 @@ __global__ __launch_bounds__(RB) void k_cg_upd(Ops o, int s, const VT *__restric
             const size_t k = (size_t)r * o.C + ic;
             hx[h][r] = x_[k];
             hr[h][r] = r_[k];
-            hp[h][r] = p_[r][ic];
             ha[h][r] = ap_[k];
         }
     }
+    asm volatile("" ::: "memory");  // keep the loads above issued before the state test
+    KState S[NR];
+    bool any = false;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        S[r] = state_in(o, s, r);
+        any |= S[r].phase == PH_ACTIVE;
+    }
+    if (!any) {
+        state_out<NR>(o, s, S);
+        return;
+    }
+    const VT *p_[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) p_[r] = (S[r].pcur ? p1 : p0) + (size_t)r * o.C;
+#pragma unroll
+    for (int h = 0; h < kHoist; ++h) {
+        const int i = lo + threadIdx.x + h * RB;
+        const int ic = i < hi ? i : o.C - 1;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) hp[h][r] = p_[r][ic];
+    }
     const bool mx1[1] = {false};
     double pap[NR][1];
-    reduce_prev<NR, 1, RB>(o, s, o.gs, mx1, pap);
-    if (threadIdx.x == 0)
-        for (int r = 0; r < NR; ++r)
-            if (sh[r].phase == PH_ACTIVE) sh[r].alpha = sh[r].rz_old / pap[r][0];
-    lds_sync();
+    reduce_prev<NR, 1, RB>(o, s, o.gs, mx1, pap, pre);
     double v[NR][2] = {};
     bool act[NR];
     VT a[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
-        act[r] = sh[r].phase == PH_ACTIVE;
-        a[r] = (VT)sh[r].alpha;
+        if (S[r].phase == PH_ACTIVE) S[r].alpha = S[r].rz_old / pap[r][0];
+        act[r] = S[r].phase == PH_ACTIVE;
+        a[r] = (VT)S[r].alpha;
     }
     auto row = [&](int i, VT id, const VT (&xv)[NR], const VT (&rv)[NR], const VT (&pv)[NR], const VT (&av)[NR]) {
 #pragma unroll
@@ -597,7 +671,7 @@ __global__ __launch_bounds__(RB) void k_cg_upd(Ops o, int s, const VT *__restric
     }
     const bool mx[2] = {(bool)o.norm_inf, false};
     write_partials<NR, 2, RB>(o, s, v, mx);
-    store_state<NR>(o, s, sh);
+    state_out<NR>(o, s, S);
 }
 
 // r = b - A x ; z = M^-1 r ; partials |r|, r.z  (cg.py:88-118)

@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import ctypes
 import logging
+import os
 from dataclasses import dataclass
 from typing import Dict, Optional, Tuple
 
@@ -23,6 +24,11 @@ from .config import NormType, PreconditionerType, SolverConfig, SolverType
 from .fvmatrix import Equation, FvMatrix
 
 logger = logging.getLogger(__name__)
+
+# Iterations queued between two polls of the device state (one launch graph
+# per chunk); GRIDFOAM_CHECK_EVERY_CG / _BICGSTAB override them for tuning.
+_CHECK_EVERY_CG = int(os.environ.get("GRIDFOAM_CHECK_EVERY_CG", "16"))
+_CHECK_EVERY_BICGSTAB = int(os.environ.get("GRIDFOAM_CHECK_EVERY_BICGSTAB", "4"))
 
 
 @dataclass(frozen=True)
@@ -109,7 +115,7 @@ class LinearSolver:
         k.rtol = c.rel_tolerance
         k.mixed = int(c.mixed_precision if mixed is None else mixed)
         k.inner_rtol = c.inner_rel_tolerance
-        k.check_every = 8 if k.method == 0 else 4
+        k.check_every = _CHECK_EVERY_CG if k.method == 0 else _CHECK_EVERY_BICGSTAB
         return k
 
     def _native_solve(self, A: FvMatrix, b: torch.Tensor, x0: torch.Tensor, transpose: bool,
