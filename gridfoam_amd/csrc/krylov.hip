@@ -406,9 +406,16 @@ __device__ __forceinline__ float dir_fma<float>(float beta, float p_old, float z
     return __builtin_fmaf(beta, p_old, z);
 }
 
-// Entries per row loaded ahead of the partial reduction: six covers every
-// row of a uniform 3-D region and keeps the FP32 kernel within 64 VGPRs.
-constexpr int kHead = 6;
+// Entries per row loaded ahead of the partial reduction.  Six cover every
+// row of a uniform 3-D region; nine also cover slices that hold a 2:1
+// refinement interface (a fifth of the slices of a wall-refined octree) and
+// keep the FP32 kernel at 68 VGPRs (seven waves per SIMD).  The FP64 kernel
+// keeps six: with nine it needs 90 VGPRs (five waves per SIMD) and is no
+// faster.
+template <typename VT>
+constexpr int head_entries() {
+    return sizeof(VT) == 4 ? 9 : 6;
+}
 
 // Convergence test on the previous update's residual, beta and p = z + beta p_old ; Ap ; partial p.Ap.
 // Mixed precision (MX): the test only decides whether the FP64 refinement is due;
@@ -423,6 +430,7 @@ template <typename VT, int NR, bool MX, int TB>
 __global__ __launch_bounds__(TB) void k_cg_dir(Ops o, int s, int nb_prev, int may_refine, const VT *__restrict__ val,
                                                const double *__restrict__ dg, const VT *__restrict__ z,
                                                VT *__restrict__ p0, VT *__restrict__ p1, VT *__restrict__ ap) {
+    constexpr int kHead = head_entries<VT>();
     double pre[NR][2];
     prefetch_prev<NR, 2>(o, s, nb_prev, pre);
     const int i = xcd_tile(blockIdx.x, gridDim.x) * TB + threadIdx.x;
