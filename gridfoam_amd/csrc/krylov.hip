@@ -230,38 +230,50 @@ __device__ __forceinline__ void state_out(const Ops &o, int s, const KState (&S)
         if (threadIdx.x == r) o.st[(s & 1) * kMaxRhs + r] = S[r];
 }
 
-// First share of the previous launch's partials (index threadIdx.x), loaded
-// ahead of everything that waits on the solver state.
-template <int NR, int NQ>
-__device__ __forceinline__ void prefetch_prev(const Ops &o, int s, int nb_prev, double (&x)[NR][NQ]) {
+// The first KP shares of the previous launch's partials (indices
+// threadIdx.x + k BS), loaded ahead of everything that waits on the solver
+// state; zero beyond nb_prev.
+template <int NR, int NQ, int KP, int BS>
+struct Prefetched {
+    double x[KP][NR][NQ];
+};
+template <int NR, int NQ, int KP, int BS>
+__device__ __forceinline__ void prefetch_prev(const Ops &o, int s, int nb_prev, Prefetched<NR, NQ, KP, BS> &pf) {
     const double *P = o.partials + (size_t)((s - 1) & 1) * kMaxRhs * kNQ * o.pstride;
-    const int j = threadIdx.x;
 #pragma unroll
-    for (int r = 0; r < NR; ++r)
+    for (int k = 0; k < KP; ++k) {
+        const int j = threadIdx.x + k * BS;
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) x[r][q] = j < nb_prev ? P[((size_t)r * kNQ + q) * o.pstride + j] : 0.0;
+        for (int r = 0; r < NR; ++r)
+#pragma unroll
+            for (int q = 0; q < NQ; ++q)
+                pf.x[k][r][q] = j < nb_prev ? P[((size_t)r * kNQ + q) * o.pstride + j] : 0.0;
+    }
 }
 
 // Reduce the previous launch's partials: tot[r][q] in every thread.  With
-// `pre`, the first share was prefetched (same summation order either way).
-template <int NR, int NQ, int BS = RB>
+// `pf`, the first KP shares were prefetched (same summation order either way).
+template <int NR, int NQ, int BS = RB, int KP = 1>
 __device__ __forceinline__ void reduce_prev(const Ops &o, int s, int nb_prev, const bool (&is_max)[NQ],
-                                            double (&tot)[NR][NQ], const double (*pre)[NQ] = nullptr) {
+                                            double (&tot)[NR][NQ],
+                                            const Prefetched<NR, NQ, KP, BS> *pf = nullptr) {
     __shared__ double lds[NR * NQ][BS / kWave];
     __shared__ double res[NR * NQ];
     const double *P = o.partials + (size_t)((s - 1) & 1) * kMaxRhs * kNQ * o.pstride;
     const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
     double acc[NR][NQ] = {};
-    if (pre) {
+    if (pf) {
 #pragma unroll
-        for (int r = 0; r < NR; ++r)
+        for (int k = 0; k < KP; ++k)
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-                const double x = pre[r][q];
-                acc[r][q] = is_max[q] ? fmax(acc[r][q], x) : acc[r][q] + x;
-            }
+            for (int r = 0; r < NR; ++r)
+#pragma unroll
+                for (int q = 0; q < NQ; ++q) {
+                    const double x = pf->x[k][r][q];
+                    acc[r][q] = is_max[q] ? fmax(acc[r][q], x) : acc[r][q] + x;
+                }
     }
-    for (int j = threadIdx.x + (pre ? BS : 0); j < nb_prev; j += BS) {  // all quantities' loads issued together
+    for (int j = threadIdx.x + (pf ? KP * BS : 0); j < nb_prev; j += BS) {  // all quantities' loads issued together
         double x[NR][NQ];
 #pragma unroll
         for (int r = 0; r < NR; ++r)
@@ -431,8 +443,8 @@ __global__ __launch_bounds__(TB) void k_cg_dir(Ops o, int s, int nb_prev, int ma
                                                const double *__restrict__ dg, const VT *__restrict__ z,
                                                VT *__restrict__ p0, VT *__restrict__ p1, VT *__restrict__ ap) {
     constexpr int kHead = head_entries<VT>();
-    double pre[NR][2];
-    prefetch_prev<NR, 2>(o, s, nb_prev, pre);
+    Prefetched<NR, 2, 1, TB> pre;
+    prefetch_prev(o, s, nb_prev, pre);
     const int i = xcd_tile(blockIdx.x, gridDim.x) * TB + threadIdx.x;
     const int ic = i < o.C ? i : o.C - 1;
     const int sl = __builtin_amdgcn_readfirstlane(ic >> 6), lane = ic & 63;
@@ -488,7 +500,7 @@ __global__ __launch_bounds__(TB) void k_cg_dir(Ops o, int s, int nb_prev, int ma
     // ---- previous partials: |r|, r.z
     const bool mx[2] = {(bool)o.norm_inf, false};
     double tot[NR][2];
-    reduce_prev<NR, 2, TB>(o, s, nb_prev, mx, tot, pre);
+    reduce_prev<NR, 2, TB, 1>(o, s, nb_prev, mx, tot, &pre);
     bool act[NR], start[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
@@ -585,16 +597,17 @@ __global__ __launch_bounds__(TB) void k_cg_dir(Ops o, int s, int nb_prev, int ma
 }
 
 // alpha = rz_old / p.Ap ; x += alpha p ; r -= alpha Ap ; z = M^-1 r ; partials |r|, r.z.
-// Each thread's first rows (kHoist of them) are loaded before the partial
-// reduction that yields alpha; kHoist keeps the FP64 kernel within 64 VGPRs
-// (two 1024-thread workgroups per CU).
+// The grid has at most one 1024-thread workgroup per CU, so the kernel may
+// use up to 128 VGPRs: each thread's first rows (kHoist of them; all rows of
+// a 1.1M-cell mesh) and the first five shares of the direction kernel's
+// partials are loaded before the state is read.
 template <typename VT, int NR>
 __global__ __launch_bounds__(RB) void k_cg_upd(Ops o, int s, const VT *__restrict__ inv_d, VT *__restrict__ x_,
                                                VT *__restrict__ r_, VT *__restrict__ z_, const VT *__restrict__ p0,
                                                const VT *__restrict__ p1, const VT *__restrict__ ap_) {
-    constexpr int kHoist = (sizeof(VT) == 4 ? 4 : 2) / NR > 0 ? (sizeof(VT) == 4 ? 4 : 2) / NR : 1;
-    double pre[NR][1];
-    prefetch_prev<NR, 1>(o, s, o.gs, pre);
+    constexpr int kHoist = (sizeof(VT) == 4 ? 6 : 5) / NR > 0 ? (sizeof(VT) == 4 ? 6 : 5) / NR : 1;
+    Prefetched<NR, 1, 5, RB> pre;
+    prefetch_prev(o, s, o.gs, pre);
     int lo, hi;
     chunk_range(o.C, gridDim.x, xcd_tile(blockIdx.x, gridDim.x), lo, hi);
     VT hx[kHoist][NR], hr[kHoist][NR], hp[kHoist][NR], ha[kHoist][NR], hid[kHoist];
@@ -635,7 +648,7 @@ __global__ __launch_bounds__(RB) void k_cg_upd(Ops o, int s, const VT *__restric
     }
     const bool mx1[1] = {false};
     double pap[NR][1];
-    reduce_prev<NR, 1, RB>(o, s, o.gs, mx1, pap, pre);
+    reduce_prev<NR, 1, RB, 5>(o, s, o.gs, mx1, pap, &pre);
     double v[NR][2] = {};
     bool act[NR];
     VT a[NR];
