@@ -404,8 +404,12 @@ __global__ void k_zero_state(KState *st, int n) {
 //              is A times the stored p ; partial p.Ap.
 //   k_cg_upd : alpha = r.z_old / p.Ap ; x += alpha p ; r -= alpha Ap ;
 //              z = M^-1 r ; partials |r|, r.z.
-// The current p buffer is tracked in the solver state (pcur), so launches
-// that turn into no-ops never desynchronise it.
+// The two p buffers alternate by the iteration's position in its chunk (an
+// even number of iterations): iteration j reads p_old from buffer j & 1 and
+// writes p to the other, so every gather is issued before the solver state
+// is read.  The state still records the buffer of the current direction
+// (pcur); when the mixed solve stops a chunk after an odd number of
+// iterations, the refinement moves the direction back to buffer 0.
 
 template <typename VT>
 __device__ __forceinline__ VT dir_fma(VT beta, VT p_old, VT z);
@@ -439,9 +443,10 @@ constexpr int head_entries() {
 // state names its buffer, and all of it is in flight during the reduction.
 // Every wave applies the state transition itself (same inputs, same result).
 template <typename VT, int NR, bool MX, int TB>
-__global__ __launch_bounds__(TB) void k_cg_dir(Ops o, int s, int nb_prev, int may_refine, const VT *__restrict__ val,
-                                               const double *__restrict__ dg, const VT *__restrict__ z,
-                                               VT *__restrict__ p0, VT *__restrict__ p1, VT *__restrict__ ap) {
+__global__ __launch_bounds__(TB) void k_cg_dir(Ops o, int s, int nb_prev, int may_refine, int pb,
+                                               const VT *__restrict__ val, const double *__restrict__ dg,
+                                               const VT *__restrict__ z, VT *__restrict__ p0, VT *__restrict__ p1,
+                                               VT *__restrict__ ap) {
     constexpr int kHead = head_entries<VT>();
     Prefetched<NR, 2, 1, TB> pre;
     prefetch_prev(o, s, nb_prev, pre);
@@ -466,11 +471,23 @@ __global__ __launch_bounds__(TB) void k_cg_dir(Ops o, int s, int nb_prev, int ma
             v[u] = __builtin_nontemporal_load(val + e);
         }
     }
-    VT zj[kHead][NR];
+    const VT *pold[NR];
+    VT *pnew[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {  // direction of the previous iteration in buffer pb; the new one goes to the other
+        pold[r] = (pb ? p1 : p0) + (size_t)r * o.C;
+        pnew[r] = (pb ? p0 : p1) + (size_t)r * o.C;
+    }
+    VT poi[NR], zj[kHead][NR], pj[kHead][NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) poi[r] = pold[r][ic];
 #pragma unroll
     for (int u = 0; u < kHead; ++u)
 #pragma unroll
-        for (int r = 0; r < NR; ++r) zj[u][r] = z[(size_t)r * o.C + col[u]];
+        for (int r = 0; r < NR; ++r) {
+            zj[u][r] = z[(size_t)r * o.C + col[u]];
+            pj[u][r] = pold[r][col[u]];
+        }
     asm volatile("" ::: "memory");  // keep the loads above issued before the state test
     KState S[NR];
     bool any = false;
@@ -483,20 +500,6 @@ __global__ __launch_bounds__(TB) void k_cg_dir(Ops o, int s, int nb_prev, int ma
         state_out<NR>(o, s, S);
         return;
     }
-    const VT *pold[NR];
-    VT *pnew[NR];
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {  // current direction in buffer pcur; the new one goes to the other
-        pold[r] = (S[r].pcur ? p1 : p0) + (size_t)r * o.C;
-        pnew[r] = (S[r].pcur ? p0 : p1) + (size_t)r * o.C;
-    }
-    VT poi[NR], pj[kHead][NR];
-#pragma unroll
-    for (int r = 0; r < NR; ++r) poi[r] = pold[r][ic];
-#pragma unroll
-    for (int u = 0; u < kHead; ++u)
-#pragma unroll
-        for (int r = 0; r < NR; ++r) pj[u][r] = pold[r][col[u]];
     // ---- previous partials: |r|, r.z
     const bool mx[2] = {(bool)o.norm_inf, false};
     double tot[NR][2];
@@ -602,14 +605,18 @@ __global__ __launch_bounds__(TB) void k_cg_dir(Ops o, int s, int nb_prev, int ma
 // a 1.1M-cell mesh) and the first five shares of the direction kernel's
 // partials are loaded before the state is read.
 template <typename VT, int NR>
-__global__ __launch_bounds__(RB) void k_cg_upd(Ops o, int s, const VT *__restrict__ inv_d, VT *__restrict__ x_,
-                                               VT *__restrict__ r_, VT *__restrict__ z_, const VT *__restrict__ p0,
-                                               const VT *__restrict__ p1, const VT *__restrict__ ap_) {
+__global__ __launch_bounds__(RB) void k_cg_upd(Ops o, int s, int pb, const VT *__restrict__ inv_d,
+                                               VT *__restrict__ x_, VT *__restrict__ r_, VT *__restrict__ z_,
+                                               const VT *__restrict__ p0, const VT *__restrict__ p1,
+                                               const VT *__restrict__ ap_) {
     constexpr int kHoist = (sizeof(VT) == 4 ? 6 : 5) / NR > 0 ? (sizeof(VT) == 4 ? 6 : 5) / NR : 1;
     Prefetched<NR, 1, 5, RB> pre;
     prefetch_prev(o, s, o.gs, pre);
     int lo, hi;
     chunk_range(o.C, gridDim.x, xcd_tile(blockIdx.x, gridDim.x), lo, hi);
+    const VT *p_[NR];  // this iteration's direction: buffer pb
+#pragma unroll
+    for (int r = 0; r < NR; ++r) p_[r] = (pb ? p1 : p0) + (size_t)r * o.C;
     VT hx[kHoist][NR], hr[kHoist][NR], hp[kHoist][NR], ha[kHoist][NR], hid[kHoist];
 #pragma unroll
     for (int h = 0; h < kHoist; ++h) {
@@ -621,6 +628,7 @@ __global__ __launch_bounds__(RB) void k_cg_upd(Ops o, int s, const VT *__restric
             const size_t k = (size_t)r * o.C + ic;
             hx[h][r] = x_[k];
             hr[h][r] = r_[k];
+            hp[h][r] = p_[r][ic];
             ha[h][r] = ap_[k];
         }
     }
@@ -635,16 +643,6 @@ __global__ __launch_bounds__(RB) void k_cg_upd(Ops o, int s, const VT *__restric
     if (!any) {
         state_out<NR>(o, s, S);
         return;
-    }
-    const VT *p_[NR];
-#pragma unroll
-    for (int r = 0; r < NR; ++r) p_[r] = (S[r].pcur ? p1 : p0) + (size_t)r * o.C;
-#pragma unroll
-    for (int h = 0; h < kHoist; ++h) {
-        const int i = lo + threadIdx.x + h * RB;
-        const int ic = i < hi ? i : o.C - 1;
-#pragma unroll
-        for (int r = 0; r < NR; ++r) hp[h][r] = p_[r][ic];
     }
     const bool mx1[1] = {false};
     double pap[NR][1];
@@ -784,23 +782,36 @@ __global__ void k_mx_prep(int C, const int32_t *sell_ptr, const double *val64, c
     dgc[i] = diag[i] + corr;
 }
 
-// Refinement, part 1: x64 += x32 ; x32 = 0
+// Refinement, part 1: x64 += x32 ; x32 = 0 ; the current direction moves
+// to buffer 0 if the chunk stopped after an odd number of iterations (the
+// next chunk's first iteration reads p_old from buffer 0).
 template <int NR>
 __global__ __launch_bounds__(RB) void k_mx_accumulate(Ops o, int s, float *__restrict__ x32,
-                                                      double *__restrict__ x64) {
+                                                      double *__restrict__ x64, float *__restrict__ p0,
+                                                      const float *__restrict__ p1) {
     __shared__ KState sh[NR];
     load_state<NR>(o, s, sh);
     if (any_phase<NR>(sh, PH_RESTART)) {
+        bool rs[NR], mv[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            rs[r] = sh[r].phase == PH_RESTART;
+            mv[r] = rs[r] && sh[r].pcur != 0;
+        }
         int lo, hi;
         chunk_range(o.C, gridDim.x, xcd_tile(blockIdx.x, gridDim.x), lo, hi);
         for (int i = lo + threadIdx.x; i < hi; i += RB)
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
-                if (sh[r].phase != PH_RESTART) continue;
+                if (!rs[r]) continue;
                 const size_t k = (size_t)r * o.C + i;
                 x64[k] += (double)x32[k];
                 x32[k] = 0.0f;
+                if (mv[r]) p0[k] = p1[k];
             }
+        if (threadIdx.x == 0)
+            for (int r = 0; r < NR; ++r)
+                if (mv[r]) sh[r].pcur = 0;
     }
     store_state<NR>(o, s, sh);
 }
@@ -1416,19 +1427,21 @@ void enqueue_cg_chunk(const Ops &o, const Layout &L, bool mixed, int every, int 
     const dim3 GS(o.gs), BS(TB), GU(o.gu), BU(RB), G1(1);
     for (int j = 0; j < every; ++j) {
         const int nb_prev = j == 0 ? first_prev : o.gu;
+        const int pb = j & 1;  // buffer of p_old; `every` is even
         if (!mixed) {
-            hipLaunchKernelGGL((k_cg_dir<double, NR, false, TB>), GS, BS, 0, st, o, s++, nb_prev, 0, L.val64, L.dg64,
-                               L.z, L.p, L.p1, L.ap);
-            hipLaunchKernelGGL((k_cg_upd<double, NR>), GU, BU, 0, st, o, s++, L.inv64, L.x, L.r, L.z, L.p, L.p1, L.ap);
+            hipLaunchKernelGGL((k_cg_dir<double, NR, false, TB>), GS, BS, 0, st, o, s++, nb_prev, 0, pb, L.val64,
+                               L.dg64, L.z, L.p, L.p1, L.ap);
+            hipLaunchKernelGGL((k_cg_upd<double, NR>), GU, BU, 0, st, o, s++, pb ^ 1, L.inv64, L.x, L.r, L.z, L.p,
+                               L.p1, L.ap);
         } else {
-            hipLaunchKernelGGL((k_cg_dir<float, NR, true, TB>), GS, BS, 0, st, o, s++, nb_prev, j == every - 1,
+            hipLaunchKernelGGL((k_cg_dir<float, NR, true, TB>), GS, BS, 0, st, o, s++, nb_prev, j == every - 1, pb,
                                L.val32, L.dgc, L.z32, L.p32, L.p32b, L.ap32);
-            hipLaunchKernelGGL((k_cg_upd<float, NR>), GU, BU, 0, st, o, s++, L.inv32, L.x32, L.r32, L.z32, L.p32,
-                               L.p32b, L.ap32);
+            hipLaunchKernelGGL((k_cg_upd<float, NR>), GU, BU, 0, st, o, s++, pb ^ 1, L.inv32, L.x32, L.r32, L.z32,
+                               L.p32, L.p32b, L.ap32);
         }
     }
     if (mixed) {  // FP64 refinement (no-ops unless a component asked for it)
-        hipLaunchKernelGGL(k_mx_accumulate<NR>, GU, BU, 0, st, o, s++, L.x32, L.x);
+        hipLaunchKernelGGL(k_mx_accumulate<NR>, GU, BU, 0, st, o, s++, L.x32, L.x, L.p32, L.p32b);
         hipLaunchKernelGGL((k_mx_residual<NR, TB>), GS, BS, 0, st, o, s++, 0, L.val64, L.dg64, L.inv32, L.b, L.x,
                            L.r32, L.z32, L.x32);
         hipLaunchKernelGGL(k_mx_refine<NR>, G1, BU, 0, st, o, s++, 0, o.gs, o.gu);
@@ -1488,7 +1501,8 @@ template <int NR, int TB>
 int run_cg(const void *ws, Ops o, const Layout &L, const gf_krylov_cfg &cfg, hipStream_t st, double *stats) {
     Poller P;
     if (int rc = P.init()) return rc;
-    const int every = cfg.check_every > 0 ? cfg.check_every : 8;
+    // iterations per chunk: even, so every chunk starts with p_old in buffer 0
+    const int every = cfg.check_every > 1 ? (cfg.check_every + 1) & ~1 : 2;
     int s = 1;  // launch sequence; state[0] holds the zeroed initial state
     const dim3 GS(o.gs), BS(TB), BU(RB), G1(1);
     // Partial sets the first k_cg_dir of a chunk reduces: the refinement's one
@@ -1733,11 +1747,11 @@ extern "C" int gf_sell_spmv(const gf_mesh *m, const void *diag, const void *val,
         constexpr int TB = decltype(tbc)::value;
         for (int k = 0; k < reps; ++k) {
             if (fp32)  // mixed-precision operator: FP32 values and vectors, FP64 diagonal and accumulation
-                hipLaunchKernelGGL((k_cg_dir<float, 1, true, TB>), dim3(o.gs), dim3(TB), 0, s, o, 1, 0, 0,
+                hipLaunchKernelGGL((k_cg_dir<float, 1, true, TB>), dim3(o.gs), dim3(TB), 0, s, o, 1, 0, 0, 0,
                                    (const float *)val, (const double *)diag, (const float *)x, (float *)p0,
                                    (float *)p1, (float *)y);
             else
-                hipLaunchKernelGGL((k_cg_dir<double, 1, false, TB>), dim3(o.gs), dim3(TB), 0, s, o, 1, 0, 0,
+                hipLaunchKernelGGL((k_cg_dir<double, 1, false, TB>), dim3(o.gs), dim3(TB), 0, s, o, 1, 0, 0, 0,
                                    (const double *)val, (const double *)diag, (const double *)x, (double *)p0,
                                    (double *)p1, (double *)y);
         }

@@ -250,7 +250,7 @@ typedef struct gf_krylov_cfg {
     int32_t mixed;         /* cg only: FP32 inner solve inside FP64 refinement */
     double inner_rtol;     /* mixed: the FP64 refinement runs once the FP32 recursive residual
                               is below inner_rtol x the last true residual (0 < inner_rtol < 1) */
-    int32_t check_every;   /* host polls the device state every k iterations */
+    int32_t check_every;   /* host polls the device state every k iterations (PCG: k rounded up to even) */
 } gf_krylov_cfg;
 /* workspace bytes needed for (C, nrhs, cfg); caller allocates once */
 int64_t gf_krylov_workspace_bytes(const gf_mesh *m, int32_t nrhs, const gf_krylov_cfg *cfg);
