@@ -248,6 +248,8 @@ class AlgorithmBase:
         else:
             self.p_ref_cell = self.p_ref_value = None
         self.solve_stats: Dict[str, Tuple[SolveStats, ...]] = {}
+        self._diagnostics = None
+        self._diagnostics_step = 0
         self.last_p_residual: Optional[torch.Tensor] = None
         self.last_p_matrix: Optional[FvMatrix] = None
         self.p_solve_history: list = []
@@ -259,6 +261,21 @@ class AlgorithmBase:
     @property
     def solvers(self) -> Dict[str, LinearSolver]:
         return self._solvers
+
+    def has_simulation_converged(self) -> bool:
+        """algorithms/base.py:62-64: only a steady SIMPLE run ends early."""
+        return False
+
+    def attach_diagnostics(self, diagnostics) -> None:
+        """algorithms/base.py:66-75: a DiagnosticsCollector recording every step."""
+        self._diagnostics = diagnostics
+
+    def _finalize_diagnostics(self) -> None:
+        """algorithms/base.py:77-100: one record per completed step."""
+        if self._diagnostics is None:
+            return
+        self._diagnostics_step += 1
+        self._diagnostics.record_step(self._diagnostics_step, self.phi, self.solve_stats)
 
     def _grad_p_into(self) -> torch.Tensor:
         return fvc.grad(self.p)
@@ -302,7 +319,33 @@ class SIMPLE(AlgorithmBase):
             self.rAtU = CellField(grid, "rAtU")
         self.alpha_U = alg.relaxationFactors.equations["U"]
         self.alpha_p = alg.relaxationFactors.equations["p"]
+        self._residual_control = normalize_residual_control(alg.residualControl)
+        self._initial_residuals: Dict[str, float] = {}
+        self._current_residuals: Dict[str, float] = {}
         correct_flux(self.phi, self.U, update_internal=True)
+
+    def _record_residual(self, name: str, residual) -> None:
+        """simple.py:277-285: the first recorded value stays the initial one."""
+        r = float(residual)
+        self._initial_residuals.setdefault(name, r)
+        self._current_residuals[name] = r
+
+    def has_converged(self) -> bool:
+        """simple.py:241-271: every residualControl entry below its absolute
+        tolerance (rel_tolerance is ignored); False when none is set."""
+        if not self._residual_control:
+            return False
+        for name, (tol, _rtol) in self._residual_control.items():
+            res, init = self._current_residuals.get(name), self._initial_residuals.get(name)
+            if res is None or init is None:
+                return False
+            if not residual_satisfied(res, tol, 0.0, init):
+                return False
+        return True
+
+    def has_simulation_converged(self) -> bool:
+        """simple.py:273-275: a converged steady iteration ends the run."""
+        return self.has_converged()
 
     def step(self) -> None:
         g, U, p = self.grid, self.U, self.p
@@ -317,15 +360,22 @@ class SIMPLE(AlgorithmBase):
         pred._source = torch.empty_like(UEqn._source)
         nat.call("gf_predictor_source", ctypes.c_int32(g.num_cells), ctypes.c_int32(3), nat.ptr(UEqn._source),
                  nat.ptr(g.cell_volumes), nat.ptr(gp), nat.ptr(pred._source), nat.stream())
+        if U.name in self._residual_control:
+            self._record_residual(U.name, field_initial_residual(pred, U))
         res = self.solvers["U"].solve(Equation(U, pred))
         U.data = res.solution
         self.solve_stats[U.name] = res.stats
         p_old = p.data.clone()
         pres = self._pressure_corrector(UEqn, transient=False, abc=None, final=False)
+        if p.name in self._residual_control:
+            self._record_residual(p.name, pres.initial_residual)
         correct_phi(self.phi, self.phi_hbya, pres.matrix, p, self.rAtU)
         nat.call("gf_relax_field", ctypes.c_int64(g.num_cells), ctypes.c_double(self.alpha_p), nat.ptr(p_old),
                  nat.ptr(p.data), nat.stream())
         correct_velocity(U, self.HbyA, self.rAtU, p)
+        if "phi" in self._residual_control:
+            self._record_residual("phi", continuity_residual(self.phi))
+        self._finalize_diagnostics()
 
 
 class PISO(AlgorithmBase):
@@ -361,6 +411,7 @@ class PISO(AlgorithmBase):
             correct_velocity(U, self.HbyA, self.rAtU, p)
         U.update_history()
         self.phi.update_history()
+        self._finalize_diagnostics()
 
 
 class PIMPLE(AlgorithmBase):
@@ -440,6 +491,7 @@ class PIMPLE(AlgorithmBase):
                 break
         U.update_history()
         self.phi.update_history()
+        self._finalize_diagnostics()
 
 
 def create_algorithm(grid: OctreeGrid):

@@ -270,12 +270,29 @@ class PropertiesConfig(BaseModel, frozen=True):
     turbulence: TurbulenceConfig = Field(default_factory=TurbulenceConfig)
 
 
+class ContinuityErrorConfig(BaseModel, frozen=True):  # meta/config.py:767-778
+    phi: str = "phi"
+    writeInterval: int = Field(default=1, ge=1)
+
+
+class SolverInfoConfig(BaseModel, frozen=True):  # meta/config.py:781-792
+    fields: Optional[List[str]] = None
+    writeInterval: int = Field(default=1, ge=1)
+
+
+class PostProcessingConfig(BaseModel, frozen=True):  # meta/config.py:794-810
+    forceCoeff: Optional[Dict[str, Any]] = None  # accepted; force evaluation needs immersed bodies
+    continuityError: Optional[ContinuityErrorConfig] = None
+    solverInfo: Optional[SolverInfoConfig] = None
+
+
 class SimulatorConfig(BaseModel, frozen=True):
     control: ControlConfig
     fvSchemes: fvSchemesConfig = Field(default_factory=fvSchemesConfig)
     fvSolution: fvSolutionConfig
     conditions: Dict[str, ConditionConfig] = Field(default_factory=dict)
     properties: PropertiesConfig
+    post_processing: Optional[PostProcessingConfig] = None
     device: str = "cuda"
 
     @model_validator(mode="after")

@@ -113,6 +113,7 @@ class CellField:
         self.transient = transient
         self.data = torch.zeros((grid.num_cells, *self.component_shape), dtype=torch.float64, device=grid.device)
         self.bcs: Dict[DomainBoundaryPatch, BoundaryCondition] = {}
+        self.export = condition.export if condition is not None else True  # meta/config.py:757-761
         if condition is not None:
             self.data[:] = torch.as_tensor(condition.internal, dtype=torch.float64)
             for bc_cfg in condition.boundary.values():

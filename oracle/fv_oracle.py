@@ -933,6 +933,8 @@ class Case:
         self.sn_p_corr = look("snGradSchemes", "snGrad(p)", "corrected") != "uncorrected"
         self.rc = _residual_control(self.alg.get("residualControl", {}))
         self.outer_converged = False
+        self.simple_init: dict[str, float] = {}  # simple.py:277-285: first recorded value is kept
+        self.simple_cur: dict[str, float] = {}
         self.ref = None
         if needs_reference(self.p):
             self.ref = (int(self.alg["pRefCell"]), float(self.alg["pRefValue"]))
@@ -997,12 +999,34 @@ class Case:
         gp = lsq_grad(g, p)
         pred = UEqn.copy()
         pred.source = UEqn.source - g.cell_volumes[:, None] * gp
+        if "U" in self.rc:  # simple.py:319-323
+            self._simple_record("U", initial_residual(pred, U.data))
         U.data, self.stats["U"] = solve(pred, U.data, self.solvers["U"])
         p_old = p.data.copy()
         HbyA, rAU, phi_h, pmat = self._pressure_pass(UEqn, False, False)
+        if "p" in self.rc:  # simple.py:379-380
+            self._simple_record("p", self.stats["p_initial_residual"])
         self.phi = correct_phi(g, phi_h, pmat, p, rAU)
         p.data = p_old + ap * (p.data - p_old)
         U.data = HbyA.data - rAU[:, None] * lsq_grad(g, p)
+        if "phi" in self.rc:  # simple.py:394-395
+            self._simple_record("phi", float(np.linalg.norm(fvc_div(g, self.phi))))
+
+    def _simple_record(self, name: str, r: float) -> None:
+        self.simple_init.setdefault(name, float(r))
+        self.simple_cur[name] = float(r)
+
+    def simple_converged(self) -> bool:
+        """simple.py:241-271: every residualControl entry below its absolute
+        tolerance (relative tolerance ignored); False when none is set."""
+        if not self.rc:
+            return False
+        for name, (tol, _rtol) in self.rc.items():
+            if name not in self.simple_cur or name not in self.simple_init:
+                return False
+            if not _residual_satisfied(self.simple_cur[name], tol, 0.0, self.simple_init[name]):
+                return False
+        return True
 
     def step_piso(self):
         """algorithms/piso.py:227-326."""

@@ -449,3 +449,34 @@ def test_simplec_reaches_the_simple_fixed_point():
     assert rel(b.U.data, a.U.data) < 1e-6
     assert rel(b.p.data, a.p.data) < 1e-6
     assert fo.continuity_max(b.g, b.phi) < 1e-8
+
+
+def test_simple_residual_control_semantics():
+    """simple.py:241-285: the first recorded residual stays the initial one,
+    convergence needs every entry below its absolute tolerance (rel_tolerance
+    ignored) and is false without entries; the phi entry is the L2 norm of the
+    corrected flux's divergence, which the pressure solve drives down."""
+    from tests.cases import small_cavity2d
+
+    def run(rc, steps):
+        cfg = small_cavity2d(8)
+        cfg["simulator"]["fvSolution"]["algorithm"]["residualControl"] = rc
+        case = fo.Case(cfg)
+        hist = []
+        for _ in range(steps):
+            case.step()
+            hist.append((dict(case.simple_cur), case.simple_converged()))
+        return case, hist
+
+    case, hist = run({"p": 0.0, "U": 0.0, "phi": 0.0}, 3)
+    assert set(case.simple_init) == {"p", "U", "phi"}
+    assert case.simple_init == hist[0][0]  # kept from the first step
+    assert hist[2][0] != hist[0][0]
+    assert not any(c for _, c in hist)
+    assert case.simple_cur["phi"] == pytest.approx(float(np.linalg.norm(fo.fvc_div(case.g, case.phi))), rel=0, abs=0)
+    _, hist = run({"p": 1e30, "U": {"tolerance": 1e30, "rel_tolerance": 0.0}, "phi": 1e30}, 1)
+    assert hist[0][1]
+    _, hist = run({"p": 1e30, "U": {"tolerance": 0.0, "rel_tolerance": 0.99}}, 2)
+    assert not hist[1][1]  # the relative tolerance does not count for SIMPLE
+    _, hist = run({}, 1)
+    assert not hist[0][1]
