@@ -147,7 +147,7 @@ def spmv_roofline(grid, pmat, fp32: bool, launches: int = 200) -> dict:
     B = sd * C + sv * F + 8 * F + 4 * sx * C
     achieved = B / avg_s / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _traffic(fp32),
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _traffic(fp32, C),
             "kernel": "k_cg_dir<float, 1, true>" if fp32 else "k_cg_dir<double, 1, false>",
             "cells": C, "bytes_per_launch": int(B), "avg_us": round(avg_s * 1e6, 2), "rel_err_vs_fp64": err}
 
@@ -178,16 +178,21 @@ def copy_peak(nbytes: int = 1 << 30, launches: int = 50) -> float:
     return round(nbytes * launches / (e0.elapsed_time(e1) / 1e3) / 1e9, 1)
 
 
-def _traffic(fp32: bool):
-    """HBM bytes per SpMV launch from the committed rocprofv3 counter runs."""
+def _traffic(fp32: bool, cells: int):
+    """HBM bytes per SpMV launch from the committed rocprofv3 counter runs,
+    for the mesh those runs measured (None for any other mesh)."""
     p = ROOT / "profiles" / "spmv_traffic.json"
     if not p.exists():
         return None
     try:
         d = json.loads(p.read_text())
-        return d.get("fp32" if fp32 else "fp64", {}).get("bytes_per_launch")
     except (ValueError, OSError):
         return None
+    prec = "fp32" if fp32 else "fp64"
+    for key, rec in d.items():
+        if key.split("_")[0] == prec and rec.get("cells") == cells:
+            return rec.get("bytes_per_launch")
+    return None
 
 
 def cpu_baseline(gpu_p_iters_per_step: float) -> dict:
@@ -266,7 +271,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-large", action="store_true", help="skip the 256^3 SpMV measurement")
     ap.add_argument("--spmv-only", choices=("fp32", "fp64"), default=None,
-                    help="only launch the SpMV kernel on the cavity mesh (for counter runs) and exit")
+                    help="only launch the SpMV kernel (for counter runs) and exit")
+    ap.add_argument("--spmv-mesh", choices=("cavity", "uniform256"), default="cavity",
+                    help="mesh of --spmv-only: the cavity octree or the uniform 256^3 grid")
     ap.add_argument("--serve-requests", type=int, default=48, help="requests in the serving measurement (0: skip)")
     ap.add_argument("--serve-slots", type=int, default=4, help="cases in flight at once in the serving measurement")
     ap.add_argument("--serve-only", action="store_true", help="only run the serving measurement")
@@ -289,7 +296,10 @@ def main() -> int:
         from gridfoam_amd.algorithms import create_case
         from gridfoam_amd.config import load_config
 
-        grid, _ = create_case(load_config(_cfg(False)), device="cuda:0")
+        if a.spmv_mesh == "uniform256":
+            grid = uniform_grid(256)
+        else:
+            grid, _ = create_case(load_config(_cfg(False)), device="cuda:0")
         r = spmv_roofline(grid, laplacian_operator(grid), fp32=a.spmv_only == "fp32", launches=20)
         # calibration for the HBM counters: k_stream_copy reads 256 MiB and writes 256 MiB
         x = torch.ones(64 << 20, dtype=torch.float32, device="cuda:0")
