@@ -79,6 +79,7 @@ struct Ops {
     double reliable;   // mixed PCG: refine once |r| < reliable * (last true residual)
     KState *st;        // [2][kMaxRhs]
     double *partials;  // [2][kMaxRhs][kNQ][pstride]
+    int *poll;         // last launch of a chunk: pinned host word <- (s << 1) | all components done
 };
 
 inline int round8(int n) { return (n + 7) / 8 * 8; }
@@ -190,10 +191,24 @@ __device__ __forceinline__ void load_state(const Ops &o, int s, KState *sh) {
     if (threadIdx.x < NR) sh[threadIdx.x] = o.st[((s - 1) & 1) * kMaxRhs + threadIdx.x];
     lds_sync();
 }
+// The last launch of a chunk also posts one word to pinned host memory for
+// the host's poll: a single aligned 32-bit system-scope atomic store (no
+// copy of it stays in the GPU's caches), read by the host only after an
+// event recorded behind the launch has completed.
+template <int NR>
+__device__ __forceinline__ void post_poll(const Ops &o, int s, bool all_done) {
+    if (o.poll && blockIdx.x == 0 && threadIdx.x == 0)
+        __hip_atomic_store(o.poll, (s << 1) | (all_done ? 1 : 0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
 template <int NR>
 __device__ __forceinline__ void store_state(const Ops &o, int s, const KState *sh) {
     lds_sync();
     if (blockIdx.x == 0 && threadIdx.x < NR) o.st[(s & 1) * kMaxRhs + threadIdx.x] = sh[threadIdx.x];
+    if (o.poll) {
+        bool done = true;
+        for (int r = 0; r < NR; ++r) done &= sh[r].phase == PH_DONE;
+        post_poll<NR>(o, s, done);
+    }
 }
 
 // State of the previous launch read with scalar loads: it is uniform across
@@ -228,6 +243,12 @@ __device__ __forceinline__ void state_out(const Ops &o, int s, const KState (&S)
 #pragma unroll
     for (int r = 0; r < NR; ++r)
         if (threadIdx.x == r) o.st[(s & 1) * kMaxRhs + r] = S[r];
+    if (o.poll) {
+        bool done = true;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) done &= S[r].phase == PH_DONE;
+        post_poll<NR>(o, s, done);
+    }
 }
 
 // The first KP shares of the previous launch's partials (indices
@@ -1339,19 +1360,41 @@ Layout layout(char *base, int64_t C, int64_t nnz, int nrhs, const gf_krylov_cfg 
     return L;
 }
 
-// Pinned staging for the polled device state.
-KState *host_stage() {
-    static thread_local KState *buf = nullptr;
-    if (!buf) {
-        if (hipHostMalloc((void **)&buf, sizeof(KState) * kMaxRhs * 2, hipHostMallocDefault) != hipSuccess)
-            buf = nullptr;
+// Pinned host memory of this thread: the final solver state, and two poll
+// words the last launch of each chunk writes directly (no copy on the stream).
+struct HostStage {
+    KState *state = nullptr;  // [kMaxRhs]
+    int *words = nullptr;     // [2] host view
+    int *dev_words = nullptr; // [2] device view
+};
+HostStage *host_stage() {
+    static thread_local HostStage hs;
+    if (!hs.state) {
+        if (hipHostMalloc((void **)&hs.state, sizeof(KState) * kMaxRhs, hipHostMallocDefault) != hipSuccess)
+            return nullptr;
+        // coherent: the GPU's store reaches host memory uncached and snooped,
+        // so the host never reads a stale cached copy of the word
+        if (hipHostMalloc((void **)&hs.words, 2 * sizeof(int), hipHostMallocCoherent | hipHostMallocMapped) !=
+                hipSuccess ||
+            hipHostGetDevicePointer((void **)&hs.dev_words, hs.words, 0) != hipSuccess) {
+            (void)hipHostFree(hs.state);
+            hs.state = nullptr;
+            return nullptr;
+        }
     }
-    return buf;
+    return &hs;
 }
 
+// Chunk k's last launch posts to word k & 1; the host queues chunk k + 1
+// before it waits for chunk k (an event behind it) and reads that word.  The
+// host clears the word to kNoPost before it queues the chunk (the chunk that
+// used it last has completed), so a word that still reads kNoPost after the
+// event means the post is missing.  (Replayed graphs carry the sequence
+// numbers of their capture, so the posted value cannot identify the chunk.)
+constexpr int kNoPost = -1;
 struct Poller {
     hipEvent_t ev[2] = {nullptr, nullptr};
-    KState *h = nullptr;
+    HostStage *h = nullptr;
     int chunk = 0;
     int init() {
         h = host_stage();
@@ -1364,28 +1407,40 @@ struct Poller {
         if (ev[0]) (void)hipEventDestroy(ev[0]);
         if (ev[1]) (void)hipEventDestroy(ev[1]);
     }
-    // queue a copy of state[s&1]; returns 1 when the previous chunk's copy says all done
-    template <typename F>
-    int poll(const Layout &L, int s, int nrhs, hipStream_t st, F all_done, bool *done) {
-        KState *slot = h + (chunk & 1) * kMaxRhs;
-        GF_CHECK_HIP(hipMemcpyAsync(slot, L.st + (s & 1) * kMaxRhs, sizeof(KState) * nrhs, hipMemcpyDeviceToHost, st));
+    // cleared device address of the word the next chunk posts to
+    int *next_word() const {
+        __atomic_store_n(h->words + (chunk & 1), kNoPost, __ATOMIC_RELEASE);
+        return h->dev_words + (chunk & 1);
+    }
+    // after queueing a chunk: records its event, waits for the previous chunk
+    // and reports whether it finished
+    int poll(hipStream_t st, bool *done) {
         GF_CHECK_HIP(hipEventRecord(ev[chunk & 1], st));
         *done = false;
         if (chunk > 0) {
-            GF_CHECK_HIP(hipEventSynchronize(ev[(chunk - 1) & 1]));
-            *done = all_done(h + ((chunk - 1) & 1) * kMaxRhs);
+            const int k = (chunk - 1) & 1;
+            GF_CHECK_HIP(hipEventSynchronize(ev[k]));
+            const int w = __atomic_load_n(h->words + k, __ATOMIC_ACQUIRE);
+            GF_REQUIRE(w != kNoPost, "solver chunk completed without posting its state");
+            *done = (w & 1) != 0;
         }
         ++chunk;
         return GF_OK;
     }
 };
 
+// Queue the copy of the final state; read it with finish_read after the
+// stream has been synchronised.
 template <int NR>
-int finish(const Layout &L, int s, const gf_krylov_cfg &cfg, hipStream_t st, double *stats) {
-    KState *h = host_stage();
-    GF_CHECK_HIP(hipMemcpyAsync(h, L.st + (s & 1) * kMaxRhs, sizeof(KState) * NR, hipMemcpyDeviceToHost, st));
-    GF_CHECK_HIP(hipStreamSynchronize(st));
-    for (int r = 0; r < NR; ++r) {
+int finish_enqueue(const Layout &L, int s, hipStream_t st) {
+    HostStage *h = host_stage();
+    GF_REQUIRE(h != nullptr, "pinned staging allocation failed");
+    GF_CHECK_HIP(hipMemcpyAsync(h->state, L.st + (s & 1) * kMaxRhs, sizeof(KState) * NR, hipMemcpyDeviceToHost, st));
+    return GF_OK;
+}
+int finish_read(int nr, double *stats) {
+    const KState *h = host_stage()->state;
+    for (int r = 0; r < nr; ++r) {
         stats[r * 4 + 0] = h[r].norm0;
         stats[r * 4 + 1] = h[r].final_res;
         stats[r * 4 + 2] = h[r].iter;
@@ -1423,16 +1478,19 @@ void key_add(std::string &k, const T &v) {
 }
 
 template <int NR, int TB>
-void enqueue_cg_chunk(const Ops &o, const Layout &L, bool mixed, int every, int s, int first_prev, hipStream_t st) {
+void enqueue_cg_chunk(const Ops &o, const Layout &L, bool mixed, int every, int s, int first_prev, int *poll,
+                      hipStream_t st) {
     const dim3 GS(o.gs), BS(TB), GU(o.gu), BU(RB), G1(1);
+    Ops ol = o;  // the chunk's last launch posts the poll word
+    ol.poll = poll;
     for (int j = 0; j < every; ++j) {
         const int nb_prev = j == 0 ? first_prev : o.gu;
         const int pb = j & 1;  // buffer of p_old; `every` is even
         if (!mixed) {
             hipLaunchKernelGGL((k_cg_dir<double, NR, false, TB>), GS, BS, 0, st, o, s++, nb_prev, 0, pb, L.val64,
                                L.dg64, L.z, L.p, L.p1, L.ap);
-            hipLaunchKernelGGL((k_cg_upd<double, NR>), GU, BU, 0, st, o, s++, pb ^ 1, L.inv64, L.x, L.r, L.z, L.p,
-                               L.p1, L.ap);
+            hipLaunchKernelGGL((k_cg_upd<double, NR>), GU, BU, 0, st, j == every - 1 ? ol : o, s++, pb ^ 1, L.inv64,
+                               L.x, L.r, L.z, L.p, L.p1, L.ap);
         } else {
             hipLaunchKernelGGL((k_cg_dir<float, NR, true, TB>), GS, BS, 0, st, o, s++, nb_prev, j == every - 1, pb,
                                L.val32, L.dgc, L.z32, L.p32, L.p32b, L.ap32);
@@ -1444,16 +1502,16 @@ void enqueue_cg_chunk(const Ops &o, const Layout &L, bool mixed, int every, int 
         hipLaunchKernelGGL(k_mx_accumulate<NR>, GU, BU, 0, st, o, s++, L.x32, L.x, L.p32, L.p32b);
         hipLaunchKernelGGL((k_mx_residual<NR, TB>), GS, BS, 0, st, o, s++, 0, L.val64, L.dg64, L.inv32, L.b, L.x,
                            L.r32, L.z32, L.x32);
-        hipLaunchKernelGGL(k_mx_refine<NR>, G1, BU, 0, st, o, s++, 0, o.gs, o.gu);
+        hipLaunchKernelGGL(k_mx_refine<NR>, G1, BU, 0, st, ol, s++, 0, o.gs, o.gu);
     }
 }
 
 // Enqueue one chunk starting at sequence s, through a cached graph when possible.
 template <int NR, int TB>
 int launch_cg_chunk(const void *ws, const Ops &o, const Layout &L, bool mixed, int every, int s, int first_prev,
-                    hipStream_t st) {
+                    int *poll, hipStream_t st) {
     if (!graphs_enabled()) {
-        enqueue_cg_chunk<NR, TB>(o, L, mixed, every, s, first_prev, st);
+        enqueue_cg_chunk<NR, TB>(o, L, mixed, every, s, first_prev, poll, st);
         GF_CHECK_LAUNCH();
         return GF_OK;
     }
@@ -1461,6 +1519,7 @@ int launch_cg_chunk(const void *ws, const Ops &o, const Layout &L, bool mixed, i
     key_add(key, o);
     key_add(key, L);
     for (int v : {NR, TB, (int)mixed, every, s & 1, first_prev}) key_add(key, v);
+    key_add(key, poll);
     GraphCache &c = graph_cache();
     hipGraphExec_t exec = nullptr;
     {
@@ -1476,7 +1535,7 @@ int launch_cg_chunk(const void *ws, const Ops &o, const Layout &L, bool mixed, i
         if (!cap) GF_CHECK_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
         hipGraph_t graph = nullptr;
         GF_CHECK_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-        enqueue_cg_chunk<NR, TB>(o, L, mixed, every, s, first_prev, cap);
+        enqueue_cg_chunk<NR, TB>(o, L, mixed, every, s, first_prev, poll, cap);
         const hipError_t le = hipGetLastError();
         const hipError_t ce = hipStreamEndCapture(cap, &graph);
         if (le != hipSuccess || ce != hipSuccess) {
@@ -1522,23 +1581,19 @@ int run_cg(const void *ws, Ops o, const Layout &L, const gf_krylov_cfg &cfg, hip
     int k = 1;
     const long cap = (long)cfg.max_iter + 4L * every;
     for (;;) {
-        if (int rc = launch_cg_chunk<NR, TB>(ws, o, L, cfg.mixed != 0, every, s, first_prev, st)) return rc;
+        if (int rc = launch_cg_chunk<NR, TB>(ws, o, L, cfg.mixed != 0, every, s, first_prev, P.next_word(), st))
+            return rc;
         s += per_chunk;
         k += every;
         bool done = false;
-        auto all_done = [&](const KState *h) {
-            bool a = true;
-            for (int r = 0; r < NR; ++r) a &= h[r].phase == PH_DONE;
-            return a;
-        };
-        if (int rc = P.poll(L, s - 1, NR, st, all_done, &done)) return rc;
+        if (int rc = P.poll(st, &done)) return rc;
         if (done) break;
         if (k > cap) {
             set_error("cg: solver state machine did not finish");
             return GF_EINVAL;
         }
     }
-    return finish<NR>(L, s - 1, cfg, st, stats);
+    return finish_enqueue<NR>(L, s - 1, st);
 }
 
 template <int NR>
@@ -1553,7 +1608,9 @@ int run_bicgstab(Ops o, const Layout &L, const gf_krylov_cfg &cfg, hipStream_t s
     hipLaunchKernelGGL(k_bi_init_fin<NR>, G1, B, 0, st, o, s++, o.nbs);
     GF_CHECK_LAUNCH();
     int k = 0;
+    Ops ol = o;  // the chunk's last launch posts the poll word
     for (;;) {
+        ol.poll = P.next_word();
         for (int j = 0; j < every; ++j, ++k) {
             hipLaunchKernelGGL(k_bi_restart<NR>, G, B, 0, st, o, s++, L.r, L.rh, L.p);
             hipLaunchKernelGGL(k_bi_y<NR>, G, B, 0, st, o, s++, L.inv64, L.p, L.y);
@@ -1562,23 +1619,18 @@ int run_bicgstab(Ops o, const Layout &L, const gf_krylov_cfg &cfg, hipStream_t s
             hipLaunchKernelGGL(k_bi_z<NR>, G, B, 0, st, o, s++, L.inv64, L.y, L.s, L.x, L.z);
             hipLaunchKernelGGL(k_bi_t<NR>, GS, BS, 0, st, o, s++, L.val64, L.dg64, L.z, L.s, L.t);
             hipLaunchKernelGGL(k_bi_update<NR>, G, B, 0, st, o, s++, L.y, L.z, L.s, L.t, L.rh, L.x, L.r);
-            hipLaunchKernelGGL(k_bi_p<NR>, G, B, 0, st, o, s++, L.r, L.v, L.p);
+            hipLaunchKernelGGL(k_bi_p<NR>, G, B, 0, st, j == every - 1 ? ol : o, s++, L.r, L.v, L.p);
         }
         GF_CHECK_LAUNCH();
         bool done = false;
-        auto all_done = [&](const KState *h) {
-            bool a = true;
-            for (int r = 0; r < NR; ++r) a &= h[r].phase == PH_DONE;
-            return a;
-        };
-        if (int rc = P.poll(L, s - 1, NR, st, all_done, &done)) return rc;
+        if (int rc = P.poll(st, &done)) return rc;
         if (done) break;
         if (k > cfg.max_iter + 4 * every + 8) {
             set_error("bicgstab: solver state machine did not finish");
             return GF_EINVAL;
         }
     }
-    return finish<NR>(L, s - 1, cfg, st, stats);
+    return finish_enqueue<NR>(L, s - 1, st);
 }
 
 }  // namespace
@@ -1671,7 +1723,8 @@ extern "C" int gf_krylov_solve(const gf_mesh *m, const gf_krylov_cfg *cfg, int32
         hipLaunchKernelGGL(k_from_soa, dim3(grid_for(C)), dim3(kBlock), 0, s, C, nrhs, L.x, x);
         GF_CHECK_LAUNCH();
     }
-    return GF_OK;
+    GF_CHECK_HIP(hipStreamSynchronize(s));  // the final state copy and the solution, in one wait
+    return finish_read(nrhs, stats_out);
 }
 
 // Replay PCG chunks through cached graphs (1, the default) or launch them one

@@ -4,7 +4,9 @@
 counts and durations from the ``*_kernel_trace.csv`` files under DIR;
 ``python -m gridfoam_amd.profiling counters DIR`` prints, per kernel, the
 mean value per dispatch of every counter in the ``*_counter_collection.csv``
-files under DIR (run rocprofv3 with ``--output-format csv``).
+files under DIR (run rocprofv3 with ``--output-format csv``);
+``python -m gridfoam_amd.profiling gaps DIR [N]`` sums the GPU's idle time
+between consecutive kernels per kernel transition.
 """
 
 from __future__ import annotations
@@ -36,6 +38,36 @@ def kernels(root: str, top: int = 25) -> str:
     return "\n".join(lines)
 
 
+def gaps(root: str, top: int = 20, max_gap_us: float = 5000.0) -> str:
+    """Idle time of the GPU between consecutive kernels (all queues merged),
+    summed per (previous kernel -> next kernel) transition; gaps longer than
+    max_gap_us (set-up, compilation, host pauses between phases of the
+    measurement) are left out."""
+    ev = []
+    for f in pathlib.Path(root).rglob("*_kernel_trace.csv"):
+        with open(f, newline="") as fh:
+            for row in csv.DictReader(fh):
+                ev.append((int(row["Start_Timestamp"]), int(row["End_Timestamp"]), _short(row["Kernel_Name"])))
+    ev.sort()
+    agg = collections.defaultdict(list)
+    busy_end, prev = None, None
+    busy = idle = 0
+    for st, en, name in ev:
+        if busy_end is not None and st > busy_end:
+            g = st - busy_end
+            if g <= max_gap_us * 1e3:
+                agg[(prev, name)].append(g)
+                idle += g
+        busy += en - st
+        if busy_end is None or en > busy_end:
+            busy_end, prev = en, name
+    lines = [f"kernel time {busy / 1e6:.2f} ms, idle gaps <= {max_gap_us:.0f} us: {idle / 1e6:.2f} ms",
+             f"{'previous kernel -> next kernel':100s} {'gaps':>6s} {'total_ms':>9s} {'avg_us':>8s}"]
+    for (a, b), v in sorted(agg.items(), key=lambda kv: -sum(kv[1]))[:top]:
+        lines.append(f"{(a or '')[:48]:48s} -> {b[:48]:48s} {len(v):6d} {sum(v) / 1e6:9.2f} {sum(v) / len(v) / 1e3:8.2f}")
+    return "\n".join(lines)
+
+
 def counters(root: str) -> str:
     agg = collections.defaultdict(list)
     for f in pathlib.Path(root).rglob("*_counter_collection.csv"):
@@ -54,5 +86,7 @@ if __name__ == "__main__":
         print(kernels(root, int(sys.argv[3]) if len(sys.argv) > 3 else 25))
     elif mode == "counters":
         print(counters(root))
+    elif mode == "gaps":
+        print(gaps(root, int(sys.argv[3]) if len(sys.argv) > 3 else 20))
     else:
         raise SystemExit(f"unknown mode {mode!r}")
