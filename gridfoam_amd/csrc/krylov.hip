@@ -386,15 +386,30 @@ __global__ void k_diag_prep(int C, const double *__restrict__ diag, int precond,
     inv_d[i] = (VT)inv;
 }
 
-__global__ void k_to_soa(int C, int nrhs, const double *__restrict__ in, double *__restrict__ out) {
+// Solve entry: b and x ([C, nrhs]) into the workspace's component-major copies.
+__global__ void k_solve_in(int C, int nrhs, const double *__restrict__ b, const double *__restrict__ x,
+                           double *__restrict__ wb, double *__restrict__ wx) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= C) return;
-    for (int r = 0; r < nrhs; ++r) out[(size_t)r * C + i] = in[(size_t)i * nrhs + r];
+    for (int r = 0; r < nrhs; ++r) {
+        wb[(size_t)r * C + i] = b[(size_t)i * nrhs + r];
+        wx[(size_t)r * C + i] = x[(size_t)i * nrhs + r];
+    }
 }
-__global__ void k_from_soa(int C, int nrhs, const double *__restrict__ in, double *__restrict__ out) {
+
+// Solve exit: the solution back to [C, nrhs], and the final solver state of
+// `nr` components into coherent pinned host memory (system-scope stores).
+__global__ void k_solve_out(int C, int nrhs, const double *__restrict__ wx, double *__restrict__ x,
+                            const KState *__restrict__ st, KState *host_state) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= C) return;
-    for (int r = 0; r < nrhs; ++r) out[(size_t)i * nrhs + r] = in[(size_t)r * C + i];
+    if (i < C)
+        for (int r = 0; r < nrhs; ++r) x[(size_t)i * nrhs + r] = wx[(size_t)r * C + i];
+    constexpr int kWords = (int)(sizeof(KState) / sizeof(uint32_t));
+    if (blockIdx.x == 0 && threadIdx.x < nrhs * kWords) {
+        const uint32_t *src = reinterpret_cast<const uint32_t *>(st);
+        uint32_t *dst = reinterpret_cast<uint32_t *>(host_state);
+        __hip_atomic_store(dst + threadIdx.x, src[threadIdx.x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
 }
 
 // Steady PCG state with beta = 0 for the SpMV benchmark entry.
@@ -1363,24 +1378,26 @@ Layout layout(char *base, int64_t C, int64_t nnz, int nrhs, const gf_krylov_cfg 
 // Pinned host memory of this thread: the final solver state, and two poll
 // words the last launch of each chunk writes directly (no copy on the stream).
 struct HostStage {
-    KState *state = nullptr;  // [kMaxRhs]
-    int *words = nullptr;     // [2] host view
-    int *dev_words = nullptr; // [2] device view
+    KState *state = nullptr;      // [kMaxRhs] host view
+    KState *dev_state = nullptr;  // device view
+    int *words = nullptr;         // [2] host view
+    int *dev_words = nullptr;     // [2] device view
 };
 HostStage *host_stage() {
     static thread_local HostStage hs;
     if (!hs.state) {
-        if (hipHostMalloc((void **)&hs.state, sizeof(KState) * kMaxRhs, hipHostMallocDefault) != hipSuccess)
-            return nullptr;
-        // coherent: the GPU's store reaches host memory uncached and snooped,
-        // so the host never reads a stale cached copy of the word
-        if (hipHostMalloc((void **)&hs.words, 2 * sizeof(int), hipHostMallocCoherent | hipHostMallocMapped) !=
-                hipSuccess ||
+        // coherent: the GPU's stores reach host memory uncached and snooped,
+        // so the host never reads a stale cached copy
+        const unsigned fl = hipHostMallocCoherent | hipHostMallocMapped;
+        KState *st = nullptr;
+        if (hipHostMalloc((void **)&st, sizeof(KState) * kMaxRhs, fl) != hipSuccess) return nullptr;
+        if (hipHostGetDevicePointer((void **)&hs.dev_state, st, 0) != hipSuccess ||
+            hipHostMalloc((void **)&hs.words, 2 * sizeof(int), fl) != hipSuccess ||
             hipHostGetDevicePointer((void **)&hs.dev_words, hs.words, 0) != hipSuccess) {
-            (void)hipHostFree(hs.state);
-            hs.state = nullptr;
+            (void)hipHostFree(st);
             return nullptr;
         }
+        hs.state = st;
     }
     return &hs;
 }
@@ -1429,15 +1446,7 @@ struct Poller {
     }
 };
 
-// Queue the copy of the final state; read it with finish_read after the
-// stream has been synchronised.
-template <int NR>
-int finish_enqueue(const Layout &L, int s, hipStream_t st) {
-    HostStage *h = host_stage();
-    GF_REQUIRE(h != nullptr, "pinned staging allocation failed");
-    GF_CHECK_HIP(hipMemcpyAsync(h->state, L.st + (s & 1) * kMaxRhs, sizeof(KState) * NR, hipMemcpyDeviceToHost, st));
-    return GF_OK;
-}
+// Statistics from the final state k_solve_out wrote (after the stream wait).
 int finish_read(int nr, double *stats) {
     const KState *h = host_stage()->state;
     for (int r = 0; r < nr; ++r) {
@@ -1557,7 +1566,7 @@ int launch_cg_chunk(const void *ws, const Ops &o, const Layout &L, bool mixed, i
 }
 
 template <int NR, int TB>
-int run_cg(const void *ws, Ops o, const Layout &L, const gf_krylov_cfg &cfg, hipStream_t st, double *stats) {
+int run_cg(const void *ws, Ops o, const Layout &L, const gf_krylov_cfg &cfg, hipStream_t st, int *s_last) {
     Poller P;
     if (int rc = P.init()) return rc;
     // iterations per chunk: even, so every chunk starts with p_old in buffer 0
@@ -1593,11 +1602,12 @@ int run_cg(const void *ws, Ops o, const Layout &L, const gf_krylov_cfg &cfg, hip
             return GF_EINVAL;
         }
     }
-    return finish_enqueue<NR>(L, s - 1, st);
+    *s_last = s - 1;
+    return GF_OK;
 }
 
 template <int NR>
-int run_bicgstab(Ops o, const Layout &L, const gf_krylov_cfg &cfg, hipStream_t st, double *stats) {
+int run_bicgstab(Ops o, const Layout &L, const gf_krylov_cfg &cfg, hipStream_t st, int *s_last) {
     const int nb = o.nb;
     Poller P;
     if (int rc = P.init()) return rc;
@@ -1630,7 +1640,8 @@ int run_bicgstab(Ops o, const Layout &L, const gf_krylov_cfg &cfg, hipStream_t s
             return GF_EINVAL;
         }
     }
-    return finish_enqueue<NR>(L, s - 1, st);
+    *s_last = s - 1;
+    return GF_OK;
 }
 
 }  // namespace
@@ -1671,14 +1682,10 @@ extern "C" int gf_krylov_solve(const gf_mesh *m, const gf_krylov_cfg *cfg, int32
                            L.inv32);
     }
     hipLaunchKernelGGL(k_zero_state, dim3(1), dim3(64), 0, s, L.st, 2 * kMaxRhs);
-    if (nrhs == 1) {
-        GF_CHECK_HIP(hipMemcpyAsync(L.b, b, sizeof(double) * C, hipMemcpyDeviceToDevice, s));
-        GF_CHECK_HIP(hipMemcpyAsync(L.x, x, sizeof(double) * C, hipMemcpyDeviceToDevice, s));
-    } else {
-        hipLaunchKernelGGL(k_to_soa, dim3(grid_for(C)), dim3(kBlock), 0, s, C, nrhs, b, L.b);
-        hipLaunchKernelGGL(k_to_soa, dim3(grid_for(C)), dim3(kBlock), 0, s, C, nrhs, x, L.x);
-    }
+    hipLaunchKernelGGL(k_solve_in, dim3(grid_for(C)), dim3(kBlock), 0, s, C, nrhs, b, x, L.b, L.x);
     GF_CHECK_LAUNCH();
+    HostStage *hs = host_stage();
+    GF_REQUIRE(hs != nullptr, "pinned staging allocation failed");
     Ops o{};
     o.sell_ptr = m->sell_ptr;
     o.sell_col = m->sell_col;
@@ -1698,9 +1705,10 @@ extern "C" int gf_krylov_solve(const gf_mesh *m, const gf_krylov_cfg *cfg, int32
     o.st = L.st;
     o.partials = L.partials;
     int rc;
+    int s_last = 0;
     if (cfg->method == 0) {
         auto cg = [&](auto nr, auto tb) {
-            return run_cg<decltype(nr)::value, decltype(tb)::value>(workspace, o, L, *cfg, s, stats_out);
+            return run_cg<decltype(nr)::value, decltype(tb)::value>(workspace, o, L, *cfg, s, &s_last);
         };
         using I1 = std::integral_constant<int, 1>;
         using I3 = std::integral_constant<int, 3>;
@@ -1715,15 +1723,12 @@ extern "C" int gf_krylov_solve(const gf_mesh *m, const gf_krylov_cfg *cfg, int32
             rc = nrhs == 1 ? cg(I1{}, T256{}) : cg(I3{}, T256{});
     }
     else
-        rc = nrhs == 1 ? run_bicgstab<1>(o, L, *cfg, s, stats_out) : run_bicgstab<3>(o, L, *cfg, s, stats_out);
+        rc = nrhs == 1 ? run_bicgstab<1>(o, L, *cfg, s, &s_last) : run_bicgstab<3>(o, L, *cfg, s, &s_last);
     if (rc != GF_OK) return rc;
-    if (nrhs == 1) {
-        GF_CHECK_HIP(hipMemcpyAsync(x, L.x, sizeof(double) * C, hipMemcpyDeviceToDevice, s));
-    } else {
-        hipLaunchKernelGGL(k_from_soa, dim3(grid_for(C)), dim3(kBlock), 0, s, C, nrhs, L.x, x);
-        GF_CHECK_LAUNCH();
-    }
-    GF_CHECK_HIP(hipStreamSynchronize(s));  // the final state copy and the solution, in one wait
+    hipLaunchKernelGGL(k_solve_out, dim3(grid_for(C)), dim3(kBlock), 0, s, C, nrhs, L.x, x,
+                       L.st + (s_last & 1) * kMaxRhs, hs->dev_state);
+    GF_CHECK_LAUNCH();
+    GF_CHECK_HIP(hipStreamSynchronize(s));  // the solution and the final state, in one wait
     return finish_read(nrhs, stats_out);
 }
 

@@ -38,17 +38,22 @@ def kernels(root: str, top: int = 25) -> str:
     return "\n".join(lines)
 
 
-def gaps(root: str, top: int = 20, max_gap_us: float = 5000.0) -> str:
+def gaps(root: str, top: int = 20, max_gap_us: float = 5000.0, between: str = "") -> str:
     """Idle time of the GPU between consecutive kernels (all queues merged),
     summed per (previous kernel -> next kernel) transition; gaps longer than
     max_gap_us (set-up, compilation, host pauses between phases of the
-    measurement) are left out."""
+    measurement) are left out.  With `between`, only the span from the
+    second to the last launch of kernels whose name contains it counts."""
     ev = []
     for f in pathlib.Path(root).rglob("*_kernel_trace.csv"):
         with open(f, newline="") as fh:
             for row in csv.DictReader(fh):
                 ev.append((int(row["Start_Timestamp"]), int(row["End_Timestamp"]), _short(row["Kernel_Name"])))
     ev.sort()
+    if between:
+        marks = [e[0] for e in ev if between in e[2]]
+        if len(marks) >= 3:
+            ev = [e for e in ev if marks[1] <= e[0] <= marks[-1]]
     agg = collections.defaultdict(list)
     busy_end, prev = None, None
     busy = idle = 0
@@ -61,7 +66,9 @@ def gaps(root: str, top: int = 20, max_gap_us: float = 5000.0) -> str:
         busy += en - st
         if busy_end is None or en > busy_end:
             busy_end, prev = en, name
-    lines = [f"kernel time {busy / 1e6:.2f} ms, idle gaps <= {max_gap_us:.0f} us: {idle / 1e6:.2f} ms",
+    span = (ev[-1][1] - ev[0][0]) if ev else 0
+    lines = [f"span {span / 1e6:.2f} ms, kernel time {busy / 1e6:.2f} ms, idle gaps <= {max_gap_us:.0f} us: "
+             f"{idle / 1e6:.2f} ms",
              f"{'previous kernel -> next kernel':100s} {'gaps':>6s} {'total_ms':>9s} {'avg_us':>8s}"]
     for (a, b), v in sorted(agg.items(), key=lambda kv: -sum(kv[1]))[:top]:
         lines.append(f"{(a or '')[:48]:48s} -> {b[:48]:48s} {len(v):6d} {sum(v) / 1e6:9.2f} {sum(v) / len(v) / 1e3:8.2f}")
@@ -87,6 +94,7 @@ if __name__ == "__main__":
     elif mode == "counters":
         print(counters(root))
     elif mode == "gaps":
-        print(gaps(root, int(sys.argv[3]) if len(sys.argv) > 3 else 20))
+        print(gaps(root, int(sys.argv[3]) if len(sys.argv) > 3 else 20,
+                   between=sys.argv[4] if len(sys.argv) > 4 else ""))
     else:
         raise SystemExit(f"unknown mode {mode!r}")
