@@ -22,6 +22,9 @@ FLUXEL = {
                    regions=[([0, 0, 0], [0.1, 1, 1], 2), ([0.9, 0, 0], [1, 1, 1], 2),
                             ([0, 0, 0], [1, 0.1, 1], 2), ([0, 0.9, 0], [1, 1, 1], 2),
                             ([0, 0, 0], [1, 1, 0.1], 2), ([0, 0, 0.9], [1, 1, 1], 2)], n_leaf=0),
+    # two cells (one internal face) and a 3^3 block: one partial slice of 64 rows
+    "pair": dict(lower=[0, 0, 0], upper=[2, 1, 1], root=[2, 1, 1], regions=[], n_leaf=0),
+    "cube3": dict(lower=[0, 0, 0], upper=[1, 1, 1], root=[3, 3, 3], regions=[], n_leaf=0),
     # odd root block counts, nested regions and a leaf refinement
     "odd": dict(lower=[-1, 0, 2], upper=[2, 1, 3.5], root=[3, 5, 2],
                 regions=[([0.1, 0.1, 2.1], [0.7, 0.4, 2.6], 2), ([-0.5, 0.6, 3.0], [0.3, 0.9, 3.4], 1)], n_leaf=1),
