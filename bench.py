@@ -275,7 +275,7 @@ def main() -> int:
     ap.add_argument("--spmv-mesh", choices=("cavity", "uniform256"), default="cavity",
                     help="mesh of --spmv-only: the cavity octree or the uniform 256^3 grid")
     ap.add_argument("--serve-requests", type=int, default=48, help="requests in the serving measurement (0: skip)")
-    ap.add_argument("--serve-slots", type=int, default=4, help="cases in flight at once in the serving measurement")
+    ap.add_argument("--serve-slots", type=int, default=8, help="cases in flight at once in the serving measurement")
     ap.add_argument("--serve-only", action="store_true", help="only run the serving measurement")
     ap.add_argument("--inner-rtol", type=float, default=None,
                     help="refinement trigger of the mixed solve (default: the solver config's inner_rel_tolerance)")
