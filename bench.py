@@ -226,7 +226,7 @@ def cpu_baseline(gpu_p_iters_per_step: float) -> dict:
 
 
 def serve_measure(n_requests: int, slots: int, iterations: int = 5, seed: int = 0) -> dict:
-    """Serving stream after the build plan: random lid-driven cases (base n^3,
+    """Serving stream of random lid-driven cases (base n^3,
     n log-uniform in [16, 96], 0-2 refinement levels around 1-4 boxes,
     Re U(100, 1000)), each request = `iterations` SIMPLE iterations.  Measured:
     one-at-a-time plain FP64 solves/s; the server's capacity with every

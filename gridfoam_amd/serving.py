@@ -81,7 +81,7 @@ def footprint_bytes(cfg: dict) -> int:
 
 
 def random_case(rng: np.random.Generator, *, n_range=(16, 96), mixed: bool = True) -> dict:
-    """A serving case after the stream in the build plan: base grid n^3 with n
+    """A serving request: base grid n^3 with n
     log-uniform in n_range, 0-2 refinement levels around 1-4 random boxes
     (side U(0.05, 0.2), centre uniform in the interior), Re U(100, 1000),
     lid-driven SIMPLE, FP32-in-FP64 pressure solves.  The boxes only trigger

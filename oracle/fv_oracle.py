@@ -1,13 +1,13 @@
 """ORACLE -- test infrastructure only.
 
 FP64 CPU (numpy) restatement of gridfoam's pressure-velocity path, used by
-``tests/`` as the checker, by ``__graft_entry__.smoke()`` and by the
+``tests/`` as the expected answer, by ``__graft_entry__.smoke()`` and by the
 ``cpu_baseline`` leg of ``bench.py``.  Nothing in ``gridfoam_amd`` imports
 it; the product runs only on the HIP library.
 
 Scatters use ``np.bincount`` (sequential, deterministic summation order),
 never atomics.  Each function cites the reference file:line it restates.
-Every function is pinned in ``tests/test_oracle_*.py`` by the reference's own
+Every function is pinned in ``tests/test_oracle.py`` by the reference's own
 known-answer tests, closed forms and invariants.
 """
 
