@@ -173,3 +173,40 @@ def test_three_component_cg_matches_per_component_oracle():
         assert res.stats[r].converged
         assert abs(res.stats[r].iterations - ost[r].iterations) <= 1
         assert _rel(x[:, r], ox[:, r]) < 1e-8
+
+
+@pytest.mark.parametrize("name", ["odd", "walls8"])
+def test_sliced_ell_layout(name):
+    """The operator's sliced-ELL arrays, bit for bit: each slice of 64
+    Morton-ordered rows is as wide as its longest row; row i holds the cells
+    of its face neighbours in ascending order with the face codes (f: upper,
+    -f-1: lower), then padding (its own row, code INT32_MIN)."""
+    g, o = _grid(name)
+    _say(f"operator layout {name}")
+    C = g.num_cells
+    own, nbr = o.owner, o.neighbour
+    length = np.bincount(own, minlength=C) + np.bincount(nbr, minlength=C)
+    ns = (C + 63) // 64
+    lens = np.zeros(ns * 64, np.int64)
+    lens[:C] = length
+    widths = lens.reshape(ns, 64).max(axis=1)
+    ptr = _np(g.sell_ptr).astype(np.int64)
+    np.testing.assert_array_equal(np.diff(ptr), 64 * widths)
+    col = _np(g.sell_col).astype(np.int64)
+    code = _np(g.sell_code).astype(np.int64)
+    assert g.mesh.nnz_pad == ptr[-1] == col.size
+    nb_of = [[] for _ in range(C)]
+    for f, (a, b) in enumerate(zip(own, nbr)):
+        nb_of[a].append((b, f))
+        nb_of[b].append((a, -f - 1))
+    for s in range(ns):
+        W = widths[s]
+        for lane in range(64):
+            i = s * 64 + lane
+            if i >= C:
+                continue
+            ent = sorted(nb_of[i])
+            e = ptr[s] + np.arange(W) * 64 + lane
+            got = list(zip(col[e], code[e]))
+            assert got[:len(ent)] == ent, (i, got, ent)
+            assert all(c == i and k == np.iinfo(np.int32).min for c, k in got[len(ent):]), (i, got)
