@@ -1383,22 +1383,52 @@ struct HostStage {
     int *words = nullptr;         // [2] host view
     int *dev_words = nullptr;     // [2] device view
 };
-HostStage *host_stage() {
-    static thread_local HostStage hs;
-    if (!hs.state) {
-        // coherent: the GPU's stores reach host memory uncached and snooped,
-        // so the host never reads a stale cached copy
-        const unsigned fl = hipHostMallocCoherent | hipHostMallocMapped;
-        KState *st = nullptr;
-        if (hipHostMalloc((void **)&st, sizeof(KState) * kMaxRhs, fl) != hipSuccess) return nullptr;
-        if (hipHostGetDevicePointer((void **)&hs.dev_state, st, 0) != hipSuccess ||
-            hipHostMalloc((void **)&hs.words, 2 * sizeof(int), fl) != hipSuccess ||
-            hipHostGetDevicePointer((void **)&hs.dev_words, hs.words, 0) != hipSuccess) {
-            (void)hipHostFree(st);
-            return nullptr;
-        }
-        hs.state = st;
+// One stage per thread that solves; a thread that ends returns its stage to
+// a pool the next new thread takes it from, so short-lived worker threads do
+// not accumulate pinned allocations (and no HIP call runs at thread exit).
+struct StagePool {
+    std::mutex mu;
+    std::vector<HostStage> free;
+};
+StagePool &stage_pool() {
+    static StagePool p;
+    return p;
+}
+struct StageHolder {
+    HostStage hs;
+    ~StageHolder() {
+        if (!hs.state) return;
+        StagePool &p = stage_pool();
+        std::lock_guard<std::mutex> g(p.mu);
+        p.free.push_back(hs);
     }
+};
+HostStage *host_stage() {
+    static thread_local StageHolder holder;
+    HostStage &hs = holder.hs;
+    if (hs.state) return &hs;
+    {
+        StagePool &p = stage_pool();
+        std::lock_guard<std::mutex> g(p.mu);
+        if (!p.free.empty()) {
+            hs = p.free.back();
+            p.free.pop_back();
+            return &hs;
+        }
+    }
+    // coherent: the GPU's stores reach host memory uncached and snooped,
+    // so the host never reads a stale cached copy
+    const unsigned fl = hipHostMallocCoherent | hipHostMallocMapped;
+    HostStage n;
+    if (hipHostMalloc((void **)&n.state, sizeof(KState) * kMaxRhs, fl) != hipSuccess) return nullptr;
+    if (hipHostGetDevicePointer((void **)&n.dev_state, n.state, 0) != hipSuccess ||
+        hipHostMalloc((void **)&n.words, 2 * sizeof(int), fl) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&n.dev_words, n.words, 0) != hipSuccess) {
+        (void)hipHostFree(n.state);
+        if (n.words) (void)hipHostFree(n.words);
+        return nullptr;
+    }
+    hs = n;
     return &hs;
 }
 
