@@ -159,7 +159,20 @@ class fvm:
                  scheme: Optional[DivScheme] = None, corrected: Optional[bool] = None,
                  abc: Optional[Tuple[float, float, float]] = None,
                  grad: Optional[torch.Tensor] = None) -> FvMatrix:
-        """Fused ``ddt*fvm.ddt + div*fvm.div(phi) + lap*fvm.laplacian(gamma)``."""
+        """Fused ``ddt*fvm.ddt + div*fvm.div(phi) + lap*fvm.laplacian(gamma)``.
+
+        The assembly is native and records no autograd graph: inputs that
+        require gradients (diffusivity, flux, field values) raise instead of
+        silently losing them.  Gradients reach the assembled LDU arrays and
+        the source through the solve's implicit adjoint."""
+        if torch.is_grad_enabled():
+            ins = [gamma if isinstance(gamma, torch.Tensor) else None, grad, field.data, field.old_data,
+                   field.older_data]
+            if phi is not None:
+                ins += [phi.single_buffer, phi.bnd_buffer]
+            if any(t is not None and t.requires_grad for t in ins):
+                raise NotImplementedError("gradients through the native FV assembly are not supported; "
+                                          "differentiate with respect to the assembled matrix and source")
         g = field.grid
         mat = FvMatrix(field)
         st = field.boundary_state()

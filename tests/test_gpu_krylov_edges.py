@@ -210,3 +210,22 @@ def test_sliced_ell_layout(name):
             got = list(zip(col[e], code[e]))
             assert got[:len(ent)] == ent, (i, got, ent)
             assert all(c == i and k == np.iinfo(np.int32).min for c, k in got[len(ent):]), (i, got)
+
+
+def test_assembly_refuses_inputs_that_require_grad():
+    """The native assembly records no autograd graph, so a diffusivity that
+    requires gradients raises instead of losing them silently; without
+    gradients (or under no_grad) the same call assembles."""
+    from gridfoam_amd.fields import CellField
+    from gridfoam_amd.fv import fvm
+
+    g, _ = _grid("refined3d")
+    P = CellField(g, "q_grad", ())
+    gamma = torch.ones(g.num_cells, dtype=torch.float64, device=g.device, requires_grad=True)
+    _say("assembly grad guard")
+    with pytest.raises(NotImplementedError):
+        fvm.assemble(P, lap=-1.0, gamma=gamma)
+    with torch.no_grad():
+        m = fvm.assemble(P, lap=-1.0, gamma=gamma)
+    m2 = fvm.assemble(P, lap=-1.0, gamma=gamma.detach())
+    assert torch.equal(m.diag, m2.diag) and torch.equal(m.upper, m2.upper)
