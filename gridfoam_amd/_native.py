@@ -41,7 +41,7 @@ class GfAssembly(ctypes.Structure):
         ("ddt_a", _f64), ("ddt_b", _f64), ("ddt_c", _f64), ("dt", _f64),
         ("psi_old", _vp), ("psi_older", _vp),
         ("div_scheme", _i32), ("phi_s", _vp), ("phi_b", _vp),
-        ("gamma", _f64), ("gamma_cell", _vp), ("lap_corrected", _i32),
+        ("gamma", _f64), ("gamma_cell", _vp), ("lap_corrected", _i32), ("lap_harmonic", _i32),
         ("psi", _vp), ("grad", _vp),
         ("frac", _vp), ("ref_v", _vp), ("ref_g", _vp), ("psi_b", _vp), ("covered", _vp),
     ]

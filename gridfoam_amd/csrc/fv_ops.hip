@@ -190,10 +190,18 @@ __global__ void k_fvc_div(const gf_mesh m, const double *phi_s, const double *ph
 }
 
 // ---------------------------------------------------------------- assembly
+// Face diffusivity (laplacian.py:38-73): linear interpolation, or the
+// distance-weighted harmonic mean 1 / ((1-w)/gO + w/gN) evaluated scaled by
+// max(gO, gN) (no overflow; a zero side blocks the face with a finite value).
 __device__ __forceinline__ double gamma_face(const gf_assembly &a, const gf_mesh &m, int f) {
     if (!a.gamma_cell) return a.gamma;
     const double w = m.w[f];
-    return w * a.gamma_cell[m.owner[f]] + (1.0 - w) * a.gamma_cell[m.neighbour[f]];
+    const double go = a.gamma_cell[m.owner[f]], gn = a.gamma_cell[m.neighbour[f]];
+    if (!a.lap_harmonic) return w * go + (1.0 - w) * gn;
+    const double scale = fmax(go, gn);
+    const double safe = scale > 0.0 ? scale : 1.0;
+    const double den = w * (go / safe) + (1.0 - w) * (gn / safe);
+    return fmin(go, gn) / (den > 0.0 ? den : 1.0);
 }
 
 template <int NC>

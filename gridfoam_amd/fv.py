@@ -26,6 +26,10 @@ _LAP_ALIASES = {
     LaplacianScheme.GAUSS_LINEAR_CORRECTED: LaplacianScheme.CORRECTED,
     LaplacianScheme.GAUSS_LINEAR_UNCORRECTED: LaplacianScheme.UNCORRECTED,
 }
+_LAP_HARMONIC = {  # schemes/laplacian.py: harmonic coefficient interpolation -> correction
+    LaplacianScheme.GAUSS_HARMONIC_CORRECTED: LaplacianScheme.CORRECTED,
+    LaplacianScheme.GAUSS_HARMONIC_UNCORRECTED: LaplacianScheme.UNCORRECTED,
+}
 
 
 def _lookup(table, key, default):
@@ -52,13 +56,19 @@ def sn_grad_corrected(grid, field_name: str) -> bool:
     return SnGradScheme(s) == SnGradScheme.CORRECTED
 
 
-def laplacian_corrected(grid, field_name: str) -> bool:
+def laplacian_scheme(grid, field_name: str) -> Tuple[bool, bool]:
+    """(corrected, harmonic) of the laplacian scheme configured for the field."""
     s = _lookup(grid.sim_config.fvSchemes.laplacianSchemes if grid.sim_config else None,
                 f"laplacian({field_name})", LaplacianScheme.CORRECTED)
-    s = _LAP_ALIASES.get(s, s)
+    harmonic = s in _LAP_HARMONIC
+    s = _LAP_HARMONIC.get(s, _LAP_ALIASES.get(s, s))
     if s not in (LaplacianScheme.CORRECTED, LaplacianScheme.UNCORRECTED):
         raise ValueError(f"laplacian scheme {s} is not supported by this build")
-    return s == LaplacianScheme.CORRECTED
+    return s == LaplacianScheme.CORRECTED, harmonic
+
+
+def laplacian_corrected(grid, field_name: str) -> bool:
+    return laplacian_scheme(grid, field_name)[0]
 
 
 def grad_scheme(grid, field_name: str) -> GradScheme:
@@ -158,7 +168,7 @@ class fvm:
                  phi: Optional[FaceField] = None, gamma: Union[float, torch.Tensor, None] = None,
                  scheme: Optional[DivScheme] = None, corrected: Optional[bool] = None,
                  abc: Optional[Tuple[float, float, float]] = None,
-                 grad: Optional[torch.Tensor] = None) -> FvMatrix:
+                 grad: Optional[torch.Tensor] = None, harmonic: Optional[bool] = None) -> FvMatrix:
         """Fused ``ddt*fvm.ddt + div*fvm.div(phi) + lap*fvm.laplacian(gamma)``.
 
         The assembly is native and records no autograd graph: inputs that
@@ -191,8 +201,11 @@ class fvm:
             need_grad |= a.div_scheme == 1
         ffc = None
         if lap:
-            corr = laplacian_corrected(g, field.name) if corrected is None else corrected
+            corr_cfg, harm_cfg = laplacian_scheme(g, field.name)
+            corr = corr_cfg if corrected is None else corrected
+            harm = harm_cfg if harmonic is None else harmonic
             a.lap_corrected = int(corr)
+            a.lap_harmonic = int(harm)
             if isinstance(gamma, torch.Tensor):
                 if tuple(gamma.shape) != (g.num_cells,):
                     raise ValueError("diffusivity must have shape [C]")
@@ -200,6 +213,11 @@ class fvm:
                 a.gamma_cell = nat.ptr(gamma_c)
             else:
                 a.gamma = float(gamma)
+            if harm:  # laplacian.py:45-54
+                ok = (bool(torch.all(torch.isfinite(gamma_c) & (gamma_c >= 0))) if isinstance(gamma, torch.Tensor)
+                      else math.isfinite(float(gamma)) and float(gamma) >= 0)
+                if not ok:
+                    raise ValueError("Harmonic diffusivity must be finite and nonnegative")
             need_grad |= corr
             if corr and g.num_hanging_faces:
                 ffc = torch.zeros((max(g.num_internal_faces, 1), *field.component_shape), dtype=torch.float64,

@@ -151,10 +151,12 @@ typedef struct gf_assembly {
     /* div: scheme 0 upwind, 1 linear; phi_s [F], phi_b [B] */
     int32_t div_scheme;
     const double *phi_s, *phi_b;
-    /* laplacian */
+    /* laplacian: face diffusivity from gamma_cell by linear interpolation, or
+     * harmonic (series resistances, laplacian.py:38-73) with lap_harmonic */
     double gamma;
     const double *gamma_cell;
     int32_t lap_corrected;
+    int32_t lap_harmonic;
     /* field + boundary state */
     const double *psi, *grad;          /* [C,nc], [C,nc,3] (hang cells) */
     const double *frac, *ref_v, *ref_g, *psi_b; /* boundary state */

@@ -524,14 +524,30 @@ def ddt_corr(g: Grid, U: Field, phi_old: np.ndarray, abc, phi_older=None) -> np.
 # ---------------------------------------------------------------------------
 # Implicit operators  (fv/fvm/*.py, fv/schemes/div.py)
 # ---------------------------------------------------------------------------
-def fvm_laplacian(g: Grid, gamma, fld: Field, corrected: bool = True) -> Matrix:
-    """fv/fvm/laplacian.py:101-201 (linear gamma interpolation)."""
+def interpolate_gamma(g: Grid, gamma, harmonic: bool = False):
+    """fv/fvm/laplacian.py:38-73: linear, or distance-weighted harmonic
+    (series resistances 1 / ((1-w)/gamma_O + w/gamma_N), evaluated scaled by
+    max(gamma_O, gamma_N); a zero diffusivity blocks the face)."""
+    if harmonic:
+        ok = np.all(np.isfinite(gamma)) and np.all(np.asarray(gamma) >= 0)
+        if not ok:
+            raise ValueError("Harmonic diffusivity must be finite and nonnegative")
+    if np.ndim(gamma) == 0:
+        return float(gamma)
+    go, gn, w = gamma[g.owner], gamma[g.neighbour], g.geo.w
+    if not harmonic:
+        return w * go + (1.0 - w) * gn
+    scale = np.maximum(go, gn)
+    safe = np.where(scale > 0, scale, 1.0)
+    den = w * (go / safe) + (1.0 - w) * (gn / safe)
+    return np.minimum(go, gn) / np.where(den > 0, den, 1.0)
+
+
+def fvm_laplacian(g: Grid, gamma, fld: Field, corrected: bool = True, harmonic: bool = False) -> Matrix:
+    """fv/fvm/laplacian.py:101-201."""
     geo = g.geo
     m = Matrix(g, fld.cshape)
-    if np.ndim(gamma) == 0:
-        gamma_f = float(gamma)
-    else:
-        gamma_f = geo.w * gamma[g.owner] + (1.0 - geo.w) * gamma[g.neighbour]
+    gamma_f = interpolate_gamma(g, gamma, harmonic)
     coeff = gamma_f * geo.mag_Sf * geo.delta
     coeff = np.broadcast_to(coeff, (g.num_faces,)).copy()
     m.upper = coeff.copy()
@@ -791,13 +807,14 @@ def needs_reference(p: Field) -> bool:  # algorithms/utils/reference_value.py:14
     return not any(bc.kind == "dirichlet" for bc in p.bcs.values())
 
 
-def solve_pressure(g, p, rAtU, div_hbya, cfg_p, cfg_final, n_nonorth, ref, corrected=True, last=None):
+def solve_pressure(g, p, rAtU, div_hbya, cfg_p, cfg_final, n_nonorth, ref, corrected=True, last=None,
+                   harmonic=False):
     """pressure_correction.py:68-160 + reference_value.py:33-51.  ``last``
     (a list) receives the initial residual of the last pass."""
     div_src = div_hbya * g.cell_volumes
     first = None
     for corr in range(n_nonorth + 1):
-        mat = -fvm_laplacian(g, rAtU, p, corrected)
+        mat = -fvm_laplacian(g, rAtU, p, corrected, harmonic)
         mat.source = mat.source - div_src
         if ref is not None:
             cell, val = ref
@@ -926,8 +943,11 @@ class Case:
         self.div_scheme = look("divSchemes", "div(phi, U)", "upwind")
         lap_u = look("laplacianSchemes", "laplacian(U)", "corrected")
         lap_p = look("laplacianSchemes", "laplacian(p)", "corrected")
-        self.lap_u_corr = lap_u not in ("uncorrected", "Gauss linear uncorrected")
-        self.lap_p_corr = lap_p not in ("uncorrected", "Gauss linear uncorrected")
+        uncorr = ("uncorrected", "Gauss linear uncorrected", "Gauss harmonic uncorrected")
+        self.lap_u_corr = lap_u not in uncorr
+        self.lap_p_corr = lap_p not in uncorr
+        self.lap_u_harm = lap_u.startswith("Gauss harmonic")
+        self.lap_p_harm = lap_p.startswith("Gauss harmonic")
         self.ddt_scheme = look("ddtSchemes", "ddt(U)", "euler")
         self.consistent = bool(self.alg.get("consistent", False)) and self.kind in ("simple", "pimple")
         self.sn_p_corr = look("snGradSchemes", "snGrad(p)", "corrected") != "uncorrected"
@@ -948,7 +968,7 @@ class Case:
 
     def _momentum(self, transient: bool):
         g, U = self.g, self.U
-        m = fvm_div(g, self.phi, U, self.div_scheme) - fvm_laplacian(g, self.nu, U, self.lap_u_corr)
+        m = fvm_div(g, self.phi, U, self.div_scheme) - fvm_laplacian(g, self.nu, U, self.lap_u_corr, self.lap_u_harm)
         if transient:
             abc = ddt_coefficients(self.ddt_scheme, U.older is not None, g.dt, self.prev_dt)
             m = fvm_ddt(g, U, abc) + m
@@ -980,7 +1000,7 @@ class Case:
         last = []
         pmat, st, res0 = solve_pressure(g, p, rAU, div_h, self.solvers["p"], cfg_final,
                                         int(self.alg.get("nNonOrthogonalCorrectors", 0)), self.ref,
-                                        self.lap_p_corr, last)
+                                        self.lap_p_corr, last, self.lap_p_harm)
         self._last_res0 = last[0]
         self.stats["p"] = st
         self.stats["p_initial_residual"] = res0

@@ -480,3 +480,30 @@ def test_simple_residual_control_semantics():
     assert not hist[1][1]  # the relative tolerance does not count for SIMPLE
     _, hist = run({}, 1)
     assert not hist[0][1]
+
+
+def test_harmonic_diffusivity_interpolation():
+    """laplacian.py:38-73: the harmonic face value is the series resistance
+    1/((1-w)/gO + w/gN) (w = the owner weight of linear interpolation), equals
+    the linear value where gO = gN, lies between min and max (and below the
+    linear mean on faces between equal cells, w = 1/2), is 0 where either
+    side is 0, and rejects negative or non-finite input."""
+    g = _grid("refined3d")
+    rng = np.random.default_rng(5)
+    gamma = 0.1 + rng.random(g.num_cells)
+    h = fo.interpolate_gamma(g, gamma, harmonic=True)
+    go, gn, w = gamma[g.owner], gamma[g.neighbour], g.geo.w
+    np.testing.assert_allclose(h, 1.0 / ((1.0 - w) / go + w / gn), rtol=1e-14)
+    lin = fo.interpolate_gamma(g, gamma)
+    assert np.all(h <= np.maximum(go, gn) * (1 + 1e-14)) and np.all(h >= np.minimum(go, gn) * (1 - 1e-14))
+    half = np.abs(w - 0.5) < 1e-12
+    assert half.any() and np.all(h[half] <= lin[half] * (1 + 1e-14))
+    flat = np.full(g.num_cells, 0.7)
+    np.testing.assert_allclose(fo.interpolate_gamma(g, flat, harmonic=True), 0.7, rtol=1e-15)
+    z = gamma.copy()
+    z[g.owner[0]] = 0.0
+    hz = fo.interpolate_gamma(g, z, harmonic=True)
+    assert hz[0] == 0.0 and np.all(np.isfinite(hz))
+    with pytest.raises(ValueError):
+        fo.interpolate_gamma(g, -gamma, harmonic=True)
+    assert fo.interpolate_gamma(g, 0.3, harmonic=True) == 0.3
