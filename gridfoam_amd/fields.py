@@ -8,6 +8,10 @@ boundary face, ``(fraction, ref_value, ref_grad)``.  Here those are dense
 per-face device arrays over all domain faces plus a ``covered`` mask (a
 face is covered when a non-empty BC owns it), which is what the HIP
 kernels read.  The boundary face value is evaluated by ``gf_bc_state``.
+Conditions that depend on other registered fields (inletOutlet reads the
+flux ``phi``, fixedFluxPressure the predicted flux and the pressure
+coefficient) refresh their faces on every ``boundary_state()`` call, as the
+reference re-evaluates them at each use.
 """
 
 from __future__ import annotations
@@ -28,9 +32,14 @@ class BoundaryCondition:
 
     type: BoundaryConditionType
     assignable: bool = True
+    dynamic: bool = False  # True: refresh() runs on every boundary_state() call
 
     def fill(self, grid: OctreeGrid, mask: torch.Tensor, frac, ref_v, ref_g) -> None:
         raise NotImplementedError
+
+    def refresh(self, field: "CellField", patch: DomainBoundaryPatch, idx: torch.Tensor,
+                state: "BoundaryState") -> None:
+        """Rewrite (frac, ref_v, ref_g) on the faces ``idx`` from current data."""
 
 
 class DirichletBC(BoundaryCondition):  # boundaries/basic/dirichlet.py:17-73
@@ -75,8 +84,94 @@ class SlipBC(BoundaryCondition):  # boundaries/basic/slip.py:17-101
         ref_g[mask] = 0.0
 
 
+class InletOutletBC(BoundaryCondition):
+    """boundaries/derived/inlet_outlet.py:27-123: zero gradient where the
+    registered flux ``phi`` leaves the domain or is zero (phi >= 0), the
+    inlet value where it enters (phi < 0).  Assignable, and adjustable by
+    adjustPhi (fv/adjust_phi.py:14-41)."""
+
+    type = BoundaryConditionType.INLET_OUTLET
+    dynamic = True
+
+    def __init__(self, inlet_value, phi_name: str = "phi"):
+        self.inlet_value = torch.as_tensor(inlet_value, dtype=torch.float64)
+        self.phi_name = phi_name
+
+    def fill(self, grid, mask, frac, ref_v, ref_g):
+        frac[mask] = 0.0
+        ref_v[mask] = 0.0
+        ref_g[mask] = 0.0
+
+    def refresh(self, field, patch, idx, state):
+        if tuple(self.inlet_value.shape) != field.component_shape:
+            raise ValueError(f"inlet value shape {tuple(self.inlet_value.shape)} does not match "
+                             f"field shape {field.component_shape}")
+        phi = field.grid.get_facefield(self.phi_name)
+        if phi is None:
+            raise ValueError(f"Field {self.phi_name} is not found.")
+        inflow = phi.domain_bnd_data.index_select(0, idx) < 0.0
+        state.frac.index_copy_(0, idx, inflow.to(torch.float64))
+        val = self.inlet_value.to(state.ref_v.device).reshape(1, -1)
+        state.ref_v.index_copy_(0, idx, torch.where(inflow[:, None], val, torch.zeros_like(val)))
+
+
+class FixedFluxPressureBC(BoundaryCondition):
+    """boundaries/derived/fixed_flux_pressure.py:124-252: a normal gradient
+    that makes the corrected boundary flux equal the velocity boundary flux,
+    ``snGrad(p) = (phiHbyA_b - U_b . Sf) / (rAtU_P |Sf|)`` (zero where the
+    denominator is not positive).  The coefficient is ``rAtU``, else
+    ``rAU``; without ``phiHbyA`` the predicted flux is ``HbyA_P . Sf``;
+    without a velocity condition on the patch the target is the boundary
+    ``phi``.  Until the coupling fields are registered the gradient is zero.
+    Scalar fields only."""
+
+    type = BoundaryConditionType.FIXED_FLUX_PRESSURE
+    dynamic = True
+
+    def __init__(self, phase_suffix: str = ""):
+        sfx = phase_suffix
+        self.phi_name, self.phi_hbya_name, self.HbyA_name = "phi" + sfx, "phiHbyA" + sfx, "HbyA" + sfx
+        self.rAU_name, self.rAtU_name, self.U_name = "rAU" + sfx, "rAtU" + sfx, "U" + sfx
+
+    def fill(self, grid, mask, frac, ref_v, ref_g):
+        frac[mask] = 0.0
+        ref_v[mask] = 0.0
+        ref_g[mask] = 0.0
+
+    def refresh(self, field, patch, idx, state):
+        if field.component_shape != ():
+            raise ValueError("FixedFluxPressure requires a scalar field")
+        g = field.grid
+        phi, phi_hbya = g.get_facefield(self.phi_name), g.get_facefield(self.phi_hbya_name)
+        HbyA, U = g.get_cellfield(self.HbyA_name), g.get_cellfield(self.U_name)
+        coeff = g.get_cellfield(self.rAtU_name)
+        if coeff is None:
+            coeff = g.get_cellfield(self.rAU_name)
+        if coeff is None or (phi_hbya is None and HbyA is None):
+            state.ref_g.index_fill_(0, idx, 0.0)
+            return
+        cells = g.domain_bnd_owner.index_select(0, idx)
+        Sf = g.domain_bnd_Sf.index_select(0, idx)
+        if phi_hbya is not None:
+            pred = phi_hbya.domain_bnd_data.index_select(0, idx)
+        else:
+            pred = (HbyA.data.index_select(0, cells) * Sf).sum(dim=1)
+        if U is not None and patch in U.bcs:
+            target = (U.boundary_state().psi_b.index_select(0, idx) * Sf).sum(dim=1)
+        elif phi is not None:
+            target = phi.domain_bnd_data.index_select(0, idx)
+        else:
+            state.ref_g.index_fill_(0, idx, 0.0)
+            return
+        denom = coeff.data.index_select(0, cells) * torch.linalg.vector_norm(Sf, dim=1)
+        pos = denom > 0.0
+        grad = torch.where(pos, (pred - target) / torch.where(pos, denom, torch.ones_like(denom)),
+                           torch.zeros_like(denom))
+        state.ref_g.index_copy_(0, idx, grad[:, None])
+
+
 def create_boundary_condition(kind: BoundaryConditionType, value, cshape) -> BoundaryCondition:
-    """boundaries/factory.py:25-69 (subset)."""
+    """boundaries/factory.py:25-69."""
     if kind == BoundaryConditionType.DIRICHLET:
         return DirichletBC(value if value is not None else 0.0)
     if kind == BoundaryConditionType.NEUMANN:
@@ -85,6 +180,12 @@ def create_boundary_condition(kind: BoundaryConditionType, value, cshape) -> Bou
         return EmptyBC()
     if kind == BoundaryConditionType.SLIP:
         return SlipBC()
+    if kind == BoundaryConditionType.INLET_OUTLET:
+        if value is None:
+            raise ValueError("inletOutlet requires an inlet value")
+        return InletOutletBC(torch.broadcast_to(torch.as_tensor(value, dtype=torch.float64), cshape))
+    if kind == BoundaryConditionType.FIXED_FLUX_PRESSURE:
+        return FixedFluxPressureBC()
     raise ValueError(f"boundary condition {kind} is not supported by this build")
 
 
@@ -121,6 +222,8 @@ class CellField:
                 for patch in bc_cfg.patches:
                     self.bcs[DomainBoundaryPatch(patch)] = bc
         self._state: Optional[BoundaryState] = None
+        self._dynamic: list = []
+        grid.register_cellfield(self)
         self.old_data: Optional[torch.Tensor] = None
         self.older_data: Optional[torch.Tensor] = None
         self.previous_dt: Optional[float] = None
@@ -159,6 +262,7 @@ class CellField:
             kind = torch.zeros(max(B, 1), dtype=torch.int8, device=dev)
             slip = torch.zeros(max(B, 1), dtype=torch.uint8, device=dev)
             dirs = g.domain_bnd_dir_id
+            dynamic = []
             for patch, bc in self.bcs.items():
                 mask = torch.zeros(max(B, 1), dtype=torch.bool, device=dev)
                 mask[:B] = dirs == patch.to_direction()
@@ -171,11 +275,18 @@ class CellField:
                     nonassign[mask] = 1
                 if isinstance(bc, SlipBC):
                     slip[mask] = 1
+                if bc.dynamic:
+                    idx = torch.nonzero(mask[:B]).flatten()
+                    if idx.numel():
+                        dynamic.append((patch, bc, idx))
             self._state = BoundaryState(frac, ref_v, ref_g, covered, None, nonassign, kind,
                                         slip if bool(slip.any()) else None)
+            self._dynamic = dynamic
         st = self._state
         if st.psi_b is None:
             st.psi_b = torch.zeros((max(B, 1), nc), dtype=torch.float64, device=dev)
+        for patch, bc, idx in self._dynamic:
+            bc.refresh(self, patch, idx, st)
         if st.slip is not None and nc == 3:
             nat.call("gf_bc_slip", g.mesh_ref, nat.ptr(self.data), nat.ptr(st.slip), nat.ptr(st.ref_v), nat.stream())
         nat.call("gf_bc_state", g.mesh_ref, ctypes.c_int32(nc), nat.ptr(self.data), nat.ptr(st.frac),
@@ -195,6 +306,7 @@ class FaceField:
         shp = (lambda n: (max(n, 1), 3)) if nc == 3 else (lambda n: (max(n, 1),))
         self._single = torch.zeros(shp(F), dtype=torch.float64, device=grid.device)
         self._bnd = torch.zeros(shp(B), dtype=torch.float64, device=grid.device)
+        grid.register_facefield(self)
         self.old_single_data: Optional[torch.Tensor] = None
         self.older_single_data: Optional[torch.Tensor] = None
         self.previous_dt: Optional[float] = None

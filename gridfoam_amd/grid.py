@@ -19,6 +19,7 @@ single-sided (``ap_is_immersed_faces`` is all False).
 from __future__ import annotations
 
 import ctypes
+import weakref
 from functools import cached_property
 
 import torch
@@ -53,6 +54,10 @@ class OctreeGrid:
         self.dt = float(dt)
         self.fluxel = fluxel
         self.sim_config = None
+        # field registry by name (axis_projected.py:78-79, :370-386): fields
+        # register on construction; conditions look up the fields they read
+        self._cellfields: "weakref.WeakValueDictionary" = weakref.WeakValueDictionary()
+        self._facefields: "weakref.WeakValueDictionary" = weakref.WeakValueDictionary()
         dev = self.device
         lo = (ctypes.c_double * 3)(*fluxel.domain.lower)
         hi = (ctypes.c_double * 3)(*fluxel.domain.upper)
@@ -232,6 +237,18 @@ class OctreeGrid:
         from .config import DomainBoundaryPatch
 
         return self.domain_bnd_dir_id == DomainBoundaryPatch(patch).to_direction()
+
+    def register_cellfield(self, field) -> None:
+        self._cellfields[field.name] = field
+
+    def register_facefield(self, field) -> None:
+        self._facefields[field.name] = field
+
+    def get_cellfield(self, name: str):
+        return self._cellfields.get(name)
+
+    def get_facefield(self, name: str):
+        return self._facefields.get(name)
 
     @property
     def num_hanging_faces(self) -> int:

@@ -153,6 +153,10 @@ class Grid:
         self.bnd_Sf = np.zeros((self.num_bnd, 3))
         self.bnd_Sf[bi, baxis] = barea
         self.geo = FaceGeometry(self)
+        # core/grid/axis_projected.py:370-386 -- registered fields by name;
+        # each entry is a zero-argument callable returning the current value
+        # (U, p: Field; phi, phiHbyA: FaceValues; HbyA, rAU, rAtU: arrays)
+        self.registry: dict[str, Any] = {}
 
     def patch_mask(self, patch: str) -> np.ndarray:
         return self.bnd_dir == PATCHES.index(patch)
@@ -207,7 +211,7 @@ def _lsq_outer(d: np.ndarray) -> np.ndarray:
 # ---------------------------------------------------------------------------
 @dataclass
 class BC:
-    kind: str  # dirichlet | neumann | empty | slip
+    kind: str  # dirichlet | neumann | empty | slip | inlet_outlet | fixed_flux_pressure
     value: np.ndarray | float = 0.0
 
     @property
@@ -274,7 +278,64 @@ def bc_evaluate(g: Grid, fld: Field, b: Batch):
         psi = fld.data[b.cells]
         un = np.sum(psi * nvec, axis=-1)
         return np.ones(n), psi - un[:, None] * nvec, np.zeros((n,) + cs)
+    if bc.kind == "inlet_outlet":
+        return _inlet_outlet(g, fld, b, bc)
+    if bc.kind == "fixed_flux_pressure":
+        return _fixed_flux_pressure(g, fld, b)
     raise ValueError(f"unsupported bc {bc.kind}")
+
+
+def _inlet_outlet(g: Grid, fld: Field, b: Batch, bc: BC):
+    """boundaries/derived/inlet_outlet.py:61-123: the inlet value where the
+    registered boundary flux phi is negative (inflow), zero gradient where it
+    is zero or positive."""
+    cs = fld.cshape
+    v = np.asarray(bc.value, np.float64)
+    if v.shape != cs:
+        raise ValueError(f"inlet value shape {v.shape} does not match field shape {cs}")
+    if "phi" not in g.registry:
+        raise ValueError("Field phi is not found.")
+    inflow = g.registry["phi"]().bnd[b.mask] < 0.0
+    n = len(b.cells)
+    rv = np.zeros((n,) + cs)
+    rv[inflow] = v
+    return inflow.astype(np.float64), rv, np.zeros((n,) + cs)
+
+
+def _fixed_flux_pressure(g: Grid, fld: Field, b: Batch):
+    """boundaries/derived/fixed_flux_pressure.py:185-252: fraction 0 and the
+    normal gradient (phiHbyA_b - U_b . Sf) / (rAtU_P |Sf|), zero where that
+    denominator is not positive; rAU stands in for an unregistered rAtU,
+    HbyA_P . Sf for an unregistered phiHbyA, the boundary phi for a velocity
+    without a condition on the patch; zero gradient while the coefficient or
+    both predicted-flux fields are unregistered."""
+    if fld.cshape != ():
+        raise ValueError("FixedFluxPressure requires a scalar field")
+    reg = g.registry
+    n = len(b.cells)
+    zero = np.zeros(n)
+    coeff = reg.get("rAtU", reg.get("rAU"))
+    if coeff is None or ("phiHbyA" not in reg and "HbyA" not in reg):
+        return zero, zero.copy(), zero.copy()
+    if "phiHbyA" in reg:
+        pred = reg["phiHbyA"]().bnd[b.mask]
+    else:
+        pred = np.sum(reg["HbyA"]()[b.cells] * b.Sf, axis=1)
+    U = reg["U"]() if "U" in reg else None
+    if U is not None and b.patch in U.bcs:
+        if U.bcs[b.patch].kind == "empty":  # empty.py:31-51: zero-gradient face value
+            ub = U.data[b.cells]
+        else:
+            ub = bc_state(g, U, b)[3]
+        target = np.sum(ub * b.Sf, axis=1)
+    elif "phi" in reg:
+        target = reg["phi"]().bnd[b.mask]
+    else:
+        return zero, zero.copy(), zero.copy()
+    denom = coeff()[b.cells] * b.mag_Sf
+    pos = denom > 0.0
+    grad = np.where(pos, (pred - target) / np.where(pos, denom, 1.0), 0.0)
+    return zero, zero.copy(), grad
 
 
 def bc_state(g: Grid, fld: Field, b: Batch):
@@ -852,9 +913,12 @@ def simplec_rAtU(UEqn: Matrix, rAU: np.ndarray, bounded: bool = True) -> np.ndar
 
 
 def apply_simplec(g: Grid, phi_h: FaceValues, HbyA: np.ndarray, p: Field, rAU: np.ndarray, rAtU: np.ndarray,
-                  corrected: bool = True):
+                  corrected: bool = True, publish=None):
     """pressure_correction.py:222-267: phiHbyA += (rAtU-rAU)_f |Sf| snGrad(p)
-    (boundary faces: the condition's normal gradient); HbyA -= (rAU-rAtU) grad p."""
+    (boundary faces: the condition's normal gradient); HbyA -= (rAU-rAtU) grad p.
+    The reference updates the registered phiHbyA in place before grad(p), so
+    conditions that read it see the updated flux there: ``publish(out)``
+    registers the new flux at that point."""
     d_r = rAtU - rAU
     w = g.geo.w
     d_r_f = w * d_r[g.owner] + (1.0 - w) * d_r[g.neighbour]
@@ -864,6 +928,8 @@ def apply_simplec(g: Grid, phi_h: FaceValues, HbyA: np.ndarray, p: Field, rAU: n
         coeff = value_grad_coeff(b, f)
         sng = coeff * (rv - p.data[b.cells]) + (1.0 - f) * rg
         out.bnd[b.mask] = out.bnd[b.mask] + d_r[b.cells] * b.mag_Sf * sng
+    if publish is not None:
+        publish(out)
     return out, HbyA - (rAU - rAtU)[:, None] * lsq_grad(g, p)
 
 
@@ -887,11 +953,16 @@ def _residual_satisfied(res: float, tol: float, rtol: float, init: float) -> boo
 # Case description and drivers  (algorithms/simple.py, piso.py, pimple.py)
 # ---------------------------------------------------------------------------
 def _bc_from_cfg(entry: dict, cshape) -> BC:
+    """boundaries/factory.py:25-69."""
     kind = entry["type"]
     if kind in ("dirichlet", "neumann"):
         v = entry.get("value", 0.0)
         return BC(kind, np.asarray(v, np.float64))
-    if kind in ("empty", "slip"):
+    if kind == "inlet_outlet":
+        if entry.get("value") is None:
+            raise ValueError("inletOutlet requires an inlet value")
+        return BC(kind, np.broadcast_to(np.asarray(entry["value"], np.float64), tuple(cshape)).copy())
+    if kind in ("empty", "slip", "fixed_flux_pressure"):
         return BC(kind)
     raise ValueError(f"unsupported boundary type {kind}")
 
@@ -958,6 +1029,14 @@ class Case:
         self.ref = None
         if needs_reference(self.p):
             self.ref = (int(self.alg["pRefCell"]), float(self.alg["pRefValue"]))
+        # the algorithms register U, p, phi, phiHbyA, rAU, rAtU and HbyA at
+        # construction, zero-initialised (piso.py:143-176, simple.py:155-190)
+        C = g.num_cells
+        self.phi = FaceValues(np.zeros(g.num_faces), np.zeros(g.num_bnd))
+        self.phi_h = FaceValues(np.zeros(g.num_faces), np.zeros(g.num_bnd))
+        self.HbyA_d, self.rAU_d, self.rAtU_d = np.zeros((C, 3)), np.zeros(C), np.zeros(C)
+        g.registry.update(U=lambda: self.U, p=lambda: self.p, phi=lambda: self.phi, phiHbyA=lambda: self.phi_h,
+                          HbyA=lambda: self.HbyA_d, rAU=lambda: self.rAU_d, rAtU=lambda: self.rAtU_d)
         self.phi = correct_flux(g, self.U)
         self.U.old = self.U.data.copy()
         self.U.older = None
@@ -978,6 +1057,7 @@ class Case:
         g, U, p = self.g, self.U, self.p
         rAU = 1.0 / UEqn.A()
         HbyA = Field("HbyA", rAU[:, None] * UEqn.H(U.data))
+        self.rAU_d, self.HbyA_d = rAU, HbyA.data
         Hf = interpolate(g, HbyA)
         phi_h = flux_from_face_velocity(g, Hf, U)
         if transient:
@@ -988,11 +1068,19 @@ class Case:
                 phi_h.bnd[b.mask] = np.sum(bc_state(g, U, b)[3] * b.Sf, axis=1)
         if self.adjust:
             adjust_phi(g, phi_h, U, p)
+        self.phi_h = phi_h
         if self.consistent:
             rAtU = simplec_rAtU(UEqn, rAU)
-            phi_h, HbyA_d = apply_simplec(g, phi_h, HbyA.data, p, rAU, rAtU, self.sn_p_corr)
+            self.rAtU_d = rAtU
+
+            def publish(out):
+                self.phi_h = out
+            phi_h, HbyA_d = apply_simplec(g, phi_h, HbyA.data, p, rAU, rAtU, self.sn_p_corr, publish)
             HbyA = Field("HbyA", HbyA_d)
+            self.HbyA_d = HbyA_d
             rAU = rAtU
+        else:
+            self.rAtU_d = rAU
         div_h = fvc_div(g, phi_h)
         cfg_final = None
         if final:

@@ -81,3 +81,59 @@ def small_cavity2d(n=16, consistent=False, alpha_p=0.3) -> dict:
     alg["consistent"] = consistent
     alg["relaxationFactors"]["equations"]["p"] = alpha_p
     return cfg
+
+
+def channel_flow(algorithm: str = "piso", *, p_outlet: str = "dirichlet", walls: str = "slip",
+                 refined: bool = True, consistent: bool = False, u_internal=(1.0, 0.0, 0.0),
+                 ffp_walls: bool = False) -> dict:
+    """Inlet/outlet channel of the reference's pressure-coupling tests
+    (tests/helpers/configs.py:335-461): [0,2]x[0,0.5]x[0,0.1] on an 8x2x1
+    root with an optional centred level-1 block; U Dirichlet (1,0,0) at
+    x_minus, inletOutlet (inlet value 0) at x_plus, slip or Dirichlet-0
+    walls at y; p fixedFluxPressure at x_minus, zero gradient at the walls,
+    Dirichlet 0 or zero gradient (pRefCell 0) at x_plus; z faces carry no
+    condition.  ``ffp_walls`` moves fixedFluxPressure onto the walls with a
+    zero-gradient U there (the corrected wall flux then follows U), so that
+    its normal gradient is not zero."""
+    steady = algorithm == "simple"
+    if algorithm == "simple":
+        alg = {"type": "simple", "consistent": consistent, "pRefCell": 0, "pRefValue": 0.0,
+               "relaxationFactors": {"equations": {"U": 0.9 if consistent else 0.7, "p": 0.3}}}
+    elif algorithm == "piso":
+        alg = {"type": "piso", "nCorrectors": 2, "pRefCell": 0, "pRefValue": 0.0}
+    else:
+        alg = {"type": "pimple", "nCorrectors": 2, "nOuterCorrectors": 2, "consistent": consistent,
+               "pRefCell": 0, "pRefValue": 0.0}
+    wall_patches = ["y_minus", "y_plus"]
+    u_walls = ({"type": "neumann", "patches": wall_patches, "value": [0.0, 0.0, 0.0]} if ffp_walls else
+               {"type": walls, "patches": wall_patches, "value": [0.0, 0.0, 0.0]})
+    p_bcs = {
+        "fixedFluxWalls": {"type": "fixed_flux_pressure", "patches": ["x_minus"]},
+        "walls": {"type": "neumann", "patches": wall_patches, "value": 0.0},
+        "outlet": {"type": p_outlet, "patches": ["x_plus"], "value": 0.0},
+    }
+    if ffp_walls:
+        p_bcs["fixedFluxWalls"]["patches"] = ["x_minus", *wall_patches]
+        del p_bcs["walls"]
+    regions = [{"name": "center", "min": [0.75, 0.0, 0.0], "max": [1.25, 0.5, 0.1], "level": 1}] if refined else []
+    solver = {"preconditioner": "jacobi", "rel_tolerance": 0.0, "tolerance": 1e-12, "max_iter": 1000,
+              "norm_type": "l_2"}
+    return {
+        "version": "0.3.0",
+        "fluxel": {"domain": {"lower": [0.0, 0.0, 0.0], "upper": [2.0, 0.5, 0.1]}, "root_resolution": [8, 2, 1],
+                   "target_level": 0, "n_leaf_refinement": 0, "refinement_regions": regions},
+        "simulator": {
+            "control": {"deltaT": 1.0 if steady else 0.05, "endTime": 10.0 if steady else 1.0, "writeInterval": 1,
+                        "precision": "float64"},
+            "fvSolution": {"adjustPhi": True, "algorithm": alg,
+                           "solvers": {"U": {"method": "bicgstab", **solver}, "p": {"method": "cg", **solver}}},
+            "conditions": {
+                "U": {"internal": list(u_internal), "boundary": {
+                    "inlet": {"type": "dirichlet", "patches": ["x_minus"], "value": [1.0, 0.0, 0.0]},
+                    "outlet": {"type": "inlet_outlet", "patches": ["x_plus"], "value": [0.0, 0.0, 0.0]},
+                    "walls": u_walls}},
+                "p": {"internal": 0.0, "boundary": p_bcs},
+            },
+            "properties": {"transport": {"type": "newtonian", "nu": 0.01}, "turbulence": {"type": "laminar"}},
+        },
+    }
