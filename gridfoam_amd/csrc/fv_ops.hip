@@ -342,6 +342,184 @@ __global__ void k_asm_hang(const gf_mesh m, const gf_assembly a, double *source,
     for (int q = 0; q < NC; ++q) source[c * NC + q] += add[q];
 }
 
+// ---------------------------------------------------------------- assembly VJP
+// Reverse mode of k_asm_faces / k_asm_cells / k_asm_hang: the gradients of a
+// scalar with respect to (diag, upper, lower, source, ffc) are carried back to
+// the diffusivity, the face flux, the old field levels and the boundary
+// reference values, following what autograd computes through the reference's
+// torch assembly (fvm/laplacian.py:38-201, fvm/div.py:15-73,
+// schemes/div.py:36-142, fvm/ddt.py:7-40).  Each output element is written by
+// exactly one thread (face outputs by the face, boundary outputs by the
+// owning cell), so the result is deterministic.  The field values enter only
+// through the hanging-face corrections, which are held fixed.
+
+// d gamma_f / d (gamma_O, gamma_N).  Linear: (w, 1-w).  Harmonic
+// gamma_f = gO gN / (w gO + (1-w) gN): ((1-w) gN^2, w gO^2) / D^2, evaluated
+// scaled by max(gO, gN); where both sides are zero the tie of min() splits
+// the gradient in halves, as torch.minimum's backward does.
+__device__ __forceinline__ void gamma_face_partials(const gf_assembly &a, const gf_mesh &m, int f, double &dgo,
+                                                    double &dgn) {
+    const double w = m.w[f];
+    if (!a.lap_harmonic) {
+        dgo = w;
+        dgn = 1.0 - w;
+        return;
+    }
+    const double go = a.gamma_cell[m.owner[f]], gn = a.gamma_cell[m.neighbour[f]];
+    const double scale = fmax(go, gn);
+    if (!(scale > 0.0)) {
+        dgo = 0.5;
+        dgn = 0.5;
+        return;
+    }
+    const double gos = go / scale, gns = gn / scale;
+    const double den = w * gos + (1.0 - w) * gns;
+    const double ro = gns / den, rn = gos / den;
+    dgo = (1.0 - w) * (ro * ro);
+    dgn = w * (rn * rn);
+}
+
+// Per face: d/d gamma_f into work[f] and d/d phi_s[f] (upwind follows
+// torch.clamp's backward, which passes the gradient at phi = 0 to both the
+// min(phi, 0) and the max(phi, 0) branches).
+template <int NC>
+__global__ void k_asmv_faces(const gf_mesh m, const gf_assembly a, const double *__restrict__ gd,
+                             const double *__restrict__ gu, const double *__restrict__ gl, double *work,
+                             double *g_phi_s) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.n_faces) return;
+    const int o = m.owner[f], n = m.neighbour[f];
+    if (g_phi_s) {
+        double gp = 0.0;
+        if (a.s_div != 0.0) {
+            const double ph = a.phi_s[f];
+            if (a.div_scheme == 0) {
+                const double neg = ph <= 0.0 ? gu[f] - gd[n] : 0.0;
+                const double pos = ph >= 0.0 ? gd[o] - gl[f] : 0.0;
+                gp = neg + pos;
+            } else {
+                const double w = m.w[f];
+                gp = ((1.0 - w) * gu[f] - w * gl[f]) + (w * gd[o] - (1.0 - w) * gd[n]);
+            }
+            gp *= a.s_div;
+        }
+        g_phi_s[f] = gp;
+    }
+    if (work) {
+        double gg = 0.0;
+        if (a.s_lap != 0.0) gg = a.s_lap * (m.mag_sf[f] * m.delta[f]) * ((gu[f] + gl[f]) - (gd[o] + gd[n]));
+        work[f] = gg;
+    }
+}
+
+// Hanging faces: the explicit corrections' share (one thread per hanging face,
+// after k_asmv_faces).
+template <int NC>
+__global__ void k_asmv_hang(const gf_mesh m, const gf_assembly a, const double *__restrict__ gs,
+                            const double *__restrict__ gffc, double *work, double *g_phi_s) {
+    int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= m.n_hang) return;
+    const int f = m.hang_face[h];
+    const int o = m.owner[f], n = m.neighbour[f];
+    if (g_phi_s && a.s_div != 0.0 && a.div_scheme == 1) {
+        const double w = m.w[f];
+        double acc = 0.0;
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+            const double base = w * a.psi[o * NC + q] + (1.0 - w) * a.psi[n * NC + q];
+            const double corrected = base + hang_interp_corr<NC>(m, a.grad, h, q);
+            acc += -(corrected - base) * (gs[o * NC + q] - gs[n * NC + q]);
+        }
+        g_phi_s[f] += a.s_div * acc;
+    }
+    if (work && a.s_lap != 0.0 && a.lap_corrected) {
+        double acc = 0.0;
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+            const double sc = hang_sn_corr<NC>(m, a.grad, h, q);
+            const double gf = gffc ? gffc[f * NC + q] : 0.0;
+            acc += sc * ((gs[n * NC + q] - gs[o * NC + q]) + gf);
+        }
+        work[f] += a.s_lap * m.mag_sf[f] * acc;
+    }
+}
+
+// Per cell: old levels, the diffusivity (cell values, or per-cell shares of a
+// uniform diffusivity whose sum is its gradient: each face counted at its
+// owner) and the cell's boundary faces (flux, reference value and gradient).
+template <int NC>
+__global__ void k_asmv_cells(const gf_mesh m, const gf_assembly a, const double *__restrict__ gd,
+                             const double *__restrict__ gs, const double *__restrict__ work, double *g_gamma,
+                             double *g_phi_b, double *g_psi_old, double *g_psi_older, double *g_ref_v,
+                             double *g_ref_g) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.n_cells) return;
+    const double vdt = a.s_ddt != 0.0 ? m.volumes[c] / a.dt : 0.0;
+    if (g_psi_old)
+#pragma unroll
+        for (int q = 0; q < NC; ++q) g_psi_old[c * NC + q] = a.s_ddt * (vdt * a.ddt_b) * gs[c * NC + q];
+    if (g_psi_older)
+#pragma unroll
+        for (int q = 0; q < NC; ++q) g_psi_older[c * NC + q] = -(a.s_ddt * (vdt * a.ddt_c)) * gs[c * NC + q];
+    const double gdc = gd[c];
+    double gsc[NC];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) gsc[q] = gs[c * NC + q];
+    double ggam = 0.0;
+    if (g_gamma && a.s_lap != 0.0) {
+        for (int f = m.own_row[c]; f < m.own_row[c + 1]; ++f) {
+            double dgo = 1.0, dgn = 0.0;
+            if (a.gamma_cell) gamma_face_partials(a, m, f, dgo, dgn);
+            ggam += dgo * work[f];
+        }
+        if (a.gamma_cell)
+            for (int k = m.nbr_row[c]; k < m.nbr_row[c + 1]; ++k) {
+                const int f = m.nbr_face[k];
+                double dgo, dgn;
+                gamma_face_partials(a, m, f, dgo, dgn);
+                ggam += dgn * work[f];
+            }
+    }
+    const double gb = a.gamma_cell ? a.gamma_cell[c] : a.gamma;
+    for (int k = m.bnd_row[c]; k < m.bnd_row[c + 1]; ++k) {
+        double gphib = 0.0, grv[NC] = {}, grg[NC] = {};
+        if (a.covered[k]) {
+            const double fr = a.frac[k], dd = m.bnd_mag_d[k];
+            if (a.s_div != 0.0) {
+                const double flux = a.phi_b[k];
+                double s = 0.0;
+#pragma unroll
+                for (int q = 0; q < NC; ++q) {
+                    s += gsc[q] * (fr * a.ref_v[k * NC + q] + (1.0 - fr) * dd * a.ref_g[k * NC + q]);
+                    grv[q] += -(a.s_div * flux * fr) * gsc[q];
+                    grg[q] += -(a.s_div * flux * (1.0 - fr) * dd) * gsc[q];
+                }
+                gphib = a.s_div * (gdc * (1.0 - fr) - s);
+            }
+            if (a.s_lap != 0.0) {
+                const double coef = fr != 0.0 ? fr / m.bnd_mag_d[k] : 0.0;
+                const double cv = gb * m.bnd_mag_sf[k] * coef, cg = gb * m.bnd_mag_sf[k] * (1.0 - fr);
+                double s = -coef * gdc;
+#pragma unroll
+                for (int q = 0; q < NC; ++q) {
+                    s -= (coef * a.ref_v[k * NC + q] + (1.0 - fr) * a.ref_g[k * NC + q]) * gsc[q];
+                    grv[q] += -(a.s_lap * cv) * gsc[q];
+                    grg[q] += -(a.s_lap * cg) * gsc[q];
+                }
+                ggam += a.s_lap * m.bnd_mag_sf[k] * s;
+            }
+        }
+        if (g_phi_b) g_phi_b[k] = gphib;
+        if (g_ref_v)
+#pragma unroll
+            for (int q = 0; q < NC; ++q) g_ref_v[k * NC + q] = grv[q];
+        if (g_ref_g)
+#pragma unroll
+            for (int q = 0; q < NC; ++q) g_ref_g[k * NC + q] = grg[q];
+    }
+    if (g_gamma) g_gamma[c] = ggam;
+}
+
 // ---------------------------------------------------------------- LDU algebra
 template <int NC>
 __global__ void k_ldu_multiply(const gf_mesh m, const double *diag, const double *upper, const double *lower,
@@ -827,6 +1005,31 @@ extern "C" int gf_assemble(const gf_mesh *m, const gf_assembly *a, double *diag,
     GF_NC_DISPATCH(a->nc, k_asm_cells, grid_for(m->n_cells), *m, *a, diag, source);
     if (need_grad && m->n_hang_cells > 0)
         GF_NC_DISPATCH(a->nc, k_asm_hang, grid_for(m->n_hang_cells), *m, *a, source, ffc);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
+extern "C" int gf_assemble_vjp(const gf_mesh *m, const gf_assembly *a, const double *g_diag, const double *g_upper,
+                               const double *g_lower, const double *g_source, const double *g_ffc, double *work,
+                               double *g_gamma, double *g_phi_s, double *g_phi_b, double *g_psi_old,
+                               double *g_psi_older, double *g_ref_v, double *g_ref_g, gf_stream_t stream) {
+    GF_REQUIRE(m && a && g_diag && g_upper && g_lower && g_source, "null argument");
+    GF_REQUIRE(a->covered && a->frac && a->ref_v && a->ref_g, "boundary state missing");
+    GF_REQUIRE(!g_gamma || work, "the diffusivity gradient needs a work array of one double per face");
+    GF_REQUIRE(a->s_div == 0.0 || (a->phi_s && a->phi_b), "div needs face flux");
+    GF_REQUIRE(a->s_ddt == 0.0 || a->dt > 0.0, "ddt needs dt > 0");
+    const bool hang_div = a->s_div != 0.0 && a->div_scheme == 1 && g_phi_s;
+    const bool hang_lap = a->s_lap != 0.0 && a->lap_corrected && g_gamma;
+    const bool need_grad = m->n_hang > 0 && (hang_div || hang_lap);
+    GF_REQUIRE(!need_grad || (a->grad && a->psi), "hanging faces need psi and its gradient");
+    hipStream_t s = S(stream);
+    double *wk = g_gamma ? work : nullptr;
+    if (m->n_faces > 0 && (wk || g_phi_s))
+        GF_NC_DISPATCH(a->nc, k_asmv_faces, grid_for(m->n_faces), *m, *a, g_diag, g_upper, g_lower, wk, g_phi_s);
+    if (need_grad)
+        GF_NC_DISPATCH(a->nc, k_asmv_hang, grid_for(m->n_hang), *m, *a, g_source, g_ffc, wk, g_phi_s);
+    GF_NC_DISPATCH(a->nc, k_asmv_cells, grid_for(m->n_cells), *m, *a, g_diag, g_source, wk, g_gamma, g_phi_b,
+                   g_psi_old, g_psi_older, g_ref_v, g_ref_g);
     GF_CHECK_LAUNCH();
     return GF_OK;
 }

@@ -1390,9 +1390,12 @@ struct StagePool {
     std::mutex mu;
     std::vector<HostStage> free;
 };
+// Never destroyed: threads that outlive static destruction (the autograd
+// engine's device threads run the backward solves and end after it at
+// process exit) still return their stage from ~StageHolder.
 StagePool &stage_pool() {
-    static StagePool p;
-    return p;
+    static StagePool *p = new StagePool;
+    return *p;
 }
 struct StageHolder {
     HostStage hs;
@@ -1500,8 +1503,8 @@ struct GraphCache {
     std::unordered_map<const void *, std::vector<std::pair<std::string, hipGraphExec_t>>> by_ws;
 };
 GraphCache &graph_cache() {
-    static GraphCache c;
-    return c;
+    static GraphCache *c = new GraphCache;  // never destroyed, as the stage pool
+    return *c;
 }
 std::atomic<int> &graphs_flag() {
     static std::atomic<int> on{[] {

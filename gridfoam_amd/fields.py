@@ -51,7 +51,7 @@ class DirichletBC(BoundaryCondition):  # boundaries/basic/dirichlet.py:17-73
 
     def fill(self, grid, mask, frac, ref_v, ref_g):
         frac[mask] = 1.0
-        ref_v[mask] = self.value.to(ref_v.device)
+        ref_v[mask] = self.value.detach().to(ref_v.device)
         ref_g[mask] = 0.0
 
 
@@ -64,7 +64,7 @@ class NeumannBC(BoundaryCondition):  # boundaries/basic/neumann.py:17-73
     def fill(self, grid, mask, frac, ref_v, ref_g):
         frac[mask] = 0.0
         ref_v[mask] = 0.0
-        ref_g[mask] = self.grad_value.to(ref_g.device)
+        ref_g[mask] = self.grad_value.detach().to(ref_g.device)
 
 
 class EmptyBC(BoundaryCondition):  # boundaries/basic/empty.py:13-51 (excluded from batches)
@@ -111,7 +111,7 @@ class InletOutletBC(BoundaryCondition):
             raise ValueError(f"Field {self.phi_name} is not found.")
         inflow = phi.domain_bnd_data.index_select(0, idx) < 0.0
         state.frac.index_copy_(0, idx, inflow.to(torch.float64))
-        val = self.inlet_value.to(state.ref_v.device).reshape(1, -1)
+        val = self.inlet_value.detach().to(state.ref_v.device).reshape(1, -1)
         state.ref_v.index_copy_(0, idx, torch.where(inflow[:, None], val, torch.zeros_like(val)))
 
 
@@ -199,6 +199,10 @@ class BoundaryState:
     nonassignable: torch.Tensor  # [B] uint8 (constrainHbyA faces)
     adjust_kind: torch.Tensor  # [B] int8: 0 none, 1 fixed value, 2 adjustable
     slip: Optional[torch.Tensor]  # [B] uint8 or None
+    # autograd-connected copies of ref_v / ref_g, rebuilt on every call when a
+    # Dirichlet value or Neumann gradient requires grad (else None)
+    ref_v_ad: Optional[torch.Tensor] = None
+    ref_g_ad: Optional[torch.Tensor] = None
 
 
 class CellField:
@@ -223,6 +227,7 @@ class CellField:
                     self.bcs[DomainBoundaryPatch(patch)] = bc
         self._state: Optional[BoundaryState] = None
         self._dynamic: list = []
+        self._valued: list = []
         grid.register_cellfield(self)
         self.old_data: Optional[torch.Tensor] = None
         self.older_data: Optional[torch.Tensor] = None
@@ -254,6 +259,7 @@ class CellField:
         g = self.grid
         B, nc, dev = g.num_domain_bnd_faces, self.nc, g.device
         if self._state is None:
+            self._valued = []
             frac = torch.zeros(max(B, 1), dtype=torch.float64, device=dev)
             ref_v = torch.zeros((max(B, 1), nc), dtype=torch.float64, device=dev)
             ref_g = torch.zeros((max(B, 1), nc), dtype=torch.float64, device=dev)
@@ -279,6 +285,8 @@ class CellField:
                     idx = torch.nonzero(mask[:B]).flatten()
                     if idx.numel():
                         dynamic.append((patch, bc, idx))
+                if isinstance(bc, (DirichletBC, NeumannBC)):
+                    self._valued.append((bc, mask))
             self._state = BoundaryState(frac, ref_v, ref_g, covered, None, nonassign, kind,
                                         slip if bool(slip.any()) else None)
             self._dynamic = dynamic
@@ -289,6 +297,20 @@ class CellField:
             bc.refresh(self, patch, idx, st)
         if st.slip is not None and nc == 3:
             nat.call("gf_bc_slip", g.mesh_ref, nat.ptr(self.data), nat.ptr(st.slip), nat.ptr(st.ref_v), nat.stream())
+        st.ref_v_ad = st.ref_g_ad = None
+        if torch.is_grad_enabled() and any(
+                (bc.value if isinstance(bc, DirichletBC) else bc.grad_value).requires_grad for bc, _ in self._valued):
+            # values that require grad are re-read on every call: the device
+            # arrays get their current values, the copies carry the graph
+            rv, rg = st.ref_v.detach().clone(), st.ref_g.detach().clone()
+            for bc, mask in self._valued:
+                if isinstance(bc, DirichletBC):
+                    rv = torch.where(mask[:, None], bc.value.to(dev).reshape(1, -1), rv)
+                else:
+                    rg = torch.where(mask[:, None], bc.grad_value.to(dev).reshape(1, -1), rg)
+            st.ref_v.copy_(rv.detach())
+            st.ref_g.copy_(rg.detach())
+            st.ref_v_ad, st.ref_g_ad = rv, rg
         nat.call("gf_bc_state", g.mesh_ref, ctypes.c_int32(nc), nat.ptr(self.data), nat.ptr(st.frac),
                  nat.ptr(st.ref_v), nat.ptr(st.ref_g), nat.ptr(st.covered), nat.ptr(st.psi_b), nat.stream())
         return st

@@ -162,6 +162,120 @@ def _assembly(field: CellField, st) -> nat.GfAssembly:
     return a
 
 
+class _Assembly:
+    """The launch description of one gf_assemble call, kept for its VJP."""
+
+    def __init__(self, field: CellField, st, ddt, div, lap, phi, gamma, scheme, corrected, abc, grad, harmonic):
+        g = field.grid
+        self.field, self.st, self.grid = field, st, g
+        a = _assembly(field, st)
+        a.s_ddt, a.s_div, a.s_lap = float(ddt), float(div), float(lap)
+        self.keep = [field.data]
+        if ddt:
+            aa, bb, cc = abc if abc is not None else ddt_coefficients(field)
+            a.ddt_a, a.ddt_b, a.ddt_c, a.dt = aa, bb, cc, g.dt
+            a.psi_old = nat.ptr(field.old_data)
+            a.psi_older = nat.ptr(field.older_data) if cc != 0.0 else None
+            self.keep += [field.old_data, field.older_data]
+        self.older_used = bool(ddt) and a.ddt_c != 0.0
+        need_grad = False
+        if div:
+            sch = scheme if scheme is not None else div_scheme(g, phi.name, field.name)
+            a.div_scheme = 1 if sch == DivScheme.LINEAR else 0
+            a.phi_s, a.phi_b = nat.ptr(phi.single_buffer), nat.ptr(phi.bnd_buffer)
+            self.keep += [phi.single_buffer, phi.bnd_buffer]
+            need_grad |= a.div_scheme == 1
+        self.gamma_t = None
+        self.has_ffc = False
+        if lap:
+            corr_cfg, harm_cfg = laplacian_scheme(g, field.name)
+            corr = corr_cfg if corrected is None else corrected
+            harm = harm_cfg if harmonic is None else harmonic
+            a.lap_corrected = int(corr)
+            a.lap_harmonic = int(harm)
+            if isinstance(gamma, torch.Tensor) and gamma.dim() > 0:
+                if tuple(gamma.shape) != (g.num_cells,):
+                    raise ValueError("diffusivity must have shape [C]")
+                gamma_c = gamma.detach().contiguous()  # kept alive with the description
+                self.gamma_t = gamma_c
+                a.gamma_cell = nat.ptr(gamma_c)
+            else:
+                a.gamma = float(gamma.detach()) if isinstance(gamma, torch.Tensor) else float(gamma)
+            if harm:  # laplacian.py:45-54
+                ok = (bool(torch.all(torch.isfinite(self.gamma_t) & (self.gamma_t >= 0))) if self.gamma_t is not None
+                      else math.isfinite(a.gamma) and a.gamma >= 0)
+                if not ok:
+                    raise ValueError("Harmonic diffusivity must be finite and nonnegative")
+            need_grad |= corr
+            self.has_ffc = bool(corr and g.num_hanging_faces)
+        self.hang_terms = bool(need_grad and g.num_hanging_faces)
+        if self.hang_terms:
+            if grad is None:
+                grad = fvc.grad(field, hang_only=True)
+            a.grad = nat.ptr(grad)
+            self.keep.append(grad)
+        self.a = a
+
+    def run(self):
+        g, field = self.grid, self.field
+        C, F = g.num_cells, g.num_internal_faces
+        dev, cs = g.device, field.component_shape
+        diag = torch.empty(C, dtype=torch.float64, device=dev)
+        upper = torch.empty(max(F, 1), dtype=torch.float64, device=dev)
+        lower = torch.empty(max(F, 1), dtype=torch.float64, device=dev)
+        source = torch.empty((C, *cs), dtype=torch.float64, device=dev)
+        if F == 0:
+            upper.zero_()
+            lower.zero_()
+        ffc = torch.zeros((max(F, 1), *cs), dtype=torch.float64, device=dev) if self.has_ffc else None
+        nat.call("gf_assemble", g.mesh_ref, ctypes.byref(self.a), nat.ptr(diag), nat.ptr(upper), nat.ptr(lower),
+                 nat.ptr(source), nat.ptr(ffc), nat.stream())
+        return diag, upper, lower, source, ffc
+
+
+class _AssembleFn(torch.autograd.Function):
+    """gf_assemble as an autograd node; the backward is gf_assemble_vjp."""
+
+    @staticmethod
+    def forward(ctx, spec, gamma, phi_s, phi_b, psi_old, psi_older, ref_v, ref_g):  # noqa: D401
+        ctx.spec = spec
+        ctx.gamma_shape = None if gamma is None else tuple(gamma.shape)
+        ctx.like = (phi_s, phi_b, psi_old, psi_older, ref_v, ref_g)
+        diag, upper, lower, source, ffc = spec.run()
+        if ffc is None:
+            ffc = torch.zeros(0, dtype=torch.float64, device=diag.device)
+            ctx.mark_non_differentiable(ffc)
+        return diag, upper, lower, source, ffc
+
+    @staticmethod
+    def backward(ctx, g_diag, g_upper, g_lower, g_source, g_ffc):
+        spec: _Assembly = ctx.spec
+        g = spec.grid
+        need = ctx.needs_input_grad
+        F, C = g.num_internal_faces, g.num_cells
+        dev = g.device
+        f64 = dict(dtype=torch.float64, device=dev)
+
+        def upstream(t, shape):
+            return torch.zeros(shape, **f64) if t is None else t.contiguous()
+
+        gd = upstream(g_diag, (C,))
+        gu = upstream(g_upper, (max(F, 1),))
+        gl = upstream(g_lower, (max(F, 1),))
+        gs = upstream(g_source, (C, *spec.field.component_shape))
+        gf = g_ffc.contiguous() if (g_ffc is not None and spec.has_ffc) else None
+        work = torch.zeros(max(F, 1), **f64) if need[1] else None
+        g_gamma = torch.zeros(C, **f64) if need[1] else None
+        outs = [torch.zeros_like(t) if (need[k + 2] and t is not None) else None for k, t in enumerate(ctx.like)]
+        g_phi_s, g_phi_b, g_old, g_older, g_rv, g_rg = outs
+        nat.call("gf_assemble_vjp", g.mesh_ref, ctypes.byref(spec.a), nat.ptr(gd), nat.ptr(gu), nat.ptr(gl),
+                 nat.ptr(gs), nat.ptr(gf), nat.ptr(work), nat.ptr(g_gamma), nat.ptr(g_phi_s), nat.ptr(g_phi_b),
+                 nat.ptr(g_old), nat.ptr(g_older), nat.ptr(g_rv), nat.ptr(g_rg), nat.stream())
+        if g_gamma is not None and ctx.gamma_shape != (C,):
+            g_gamma = g_gamma.sum().reshape(ctx.gamma_shape)
+        return None, g_gamma, g_phi_s, g_phi_b, g_old, g_older, g_rv, g_rg
+
+
 class fvm:
     @staticmethod
     def assemble(field: CellField, *, ddt: float = 0.0, div: float = 0.0, lap: float = 0.0,
@@ -171,64 +285,42 @@ class fvm:
                  grad: Optional[torch.Tensor] = None, harmonic: Optional[bool] = None) -> FvMatrix:
         """Fused ``ddt*fvm.ddt + div*fvm.div(phi) + lap*fvm.laplacian(gamma)``.
 
-        The assembly is native and records no autograd graph: inputs that
-        require gradients (diffusivity, flux, field values) raise instead of
-        silently losing them.  Gradients reach the assembled LDU arrays and
-        the source through the solve's implicit adjoint."""
-        if torch.is_grad_enabled():
-            ins = [gamma if isinstance(gamma, torch.Tensor) else None, grad, field.data, field.old_data,
-                   field.older_data]
-            if phi is not None:
-                ins += [phi.single_buffer, phi.bnd_buffer]
-            if any(t is not None and t.requires_grad for t in ins):
-                raise NotImplementedError("gradients through the native FV assembly are not supported; "
-                                          "differentiate with respect to the assembled matrix and source")
+        Differentiable (when autograd is on and an input requires grad) with
+        respect to the diffusivity (a [C] tensor or a 0-d tensor), the face
+        flux, the old and older field levels and the Dirichlet values and
+        Neumann gradients of the field's conditions: the backward is the
+        native VJP ``gf_assemble_vjp``.  The current field values enter only
+        through the hanging-face corrections and slip conditions; a field
+        that requires grad there raises, as does an explicit ``grad``
+        argument that requires grad."""
         g = field.grid
-        mat = FvMatrix(field)
         st = field.boundary_state()
-        a = _assembly(field, st)
-        a.s_ddt, a.s_div, a.s_lap = float(ddt), float(div), float(lap)
-        if ddt:
-            aa, bb, cc = abc if abc is not None else ddt_coefficients(field)
-            a.ddt_a, a.ddt_b, a.ddt_c, a.dt = aa, bb, cc, g.dt
-            a.psi_old = nat.ptr(field.old_data)
-            a.psi_older = nat.ptr(field.older_data) if cc != 0.0 else None
-        need_grad = False
-        if div:
-            sch = scheme if scheme is not None else div_scheme(g, phi.name, field.name)
-            a.div_scheme = 1 if sch == DivScheme.LINEAR else 0
-            a.phi_s, a.phi_b = nat.ptr(phi.single_buffer), nat.ptr(phi.bnd_buffer)
-            need_grad |= a.div_scheme == 1
-        ffc = None
-        if lap:
-            corr_cfg, harm_cfg = laplacian_scheme(g, field.name)
-            corr = corr_cfg if corrected is None else corrected
-            harm = harm_cfg if harmonic is None else harmonic
-            a.lap_corrected = int(corr)
-            a.lap_harmonic = int(harm)
-            if isinstance(gamma, torch.Tensor):
-                if tuple(gamma.shape) != (g.num_cells,):
-                    raise ValueError("diffusivity must have shape [C]")
-                gamma_c = gamma.contiguous()  # keep alive until the launch below
-                a.gamma_cell = nat.ptr(gamma_c)
-            else:
-                a.gamma = float(gamma)
-            if harm:  # laplacian.py:45-54
-                ok = (bool(torch.all(torch.isfinite(gamma_c) & (gamma_c >= 0))) if isinstance(gamma, torch.Tensor)
-                      else math.isfinite(float(gamma)) and float(gamma) >= 0)
-                if not ok:
-                    raise ValueError("Harmonic diffusivity must be finite and nonnegative")
-            need_grad |= corr
-            if corr and g.num_hanging_faces:
-                ffc = torch.zeros((max(g.num_internal_faces, 1), *field.component_shape), dtype=torch.float64,
-                                  device=g.device)
-        if need_grad and g.num_hanging_faces:
-            if grad is None:
-                grad = fvc.grad(field, hang_only=True)
-            a.grad = nat.ptr(grad)
-        nat.call("gf_assemble", g.mesh_ref, ctypes.byref(a), nat.ptr(mat._diag), nat.ptr(mat._upper),
-                 nat.ptr(mat._lower), nat.ptr(mat._source), nat.ptr(ffc), nat.stream())
-        if ffc is not None:
+        spec = _Assembly(field, st, ddt, div, lap, phi, gamma, scheme, corrected, abc, grad, harmonic)
+        mat = FvMatrix.__new__(FvMatrix)
+        mat.field, mat.grid, mat.face_flux_correction = field, g, None
+        track = torch.is_grad_enabled()
+        gamma_in = gamma if (lap and isinstance(gamma, torch.Tensor)) else None
+        phi_s = phi.single_buffer if (div and phi is not None) else None
+        phi_b = phi.bnd_buffer if (div and phi is not None) else None
+        old = field.old_data if ddt else None
+        older = field.older_data if spec.older_used else None
+        rv = st.ref_v_ad if st.ref_v_ad is not None else st.ref_v
+        rg = st.ref_g_ad if st.ref_g_ad is not None else st.ref_g
+        ins = (gamma_in, phi_s, phi_b, old, older, rv, rg)
+        if track and any(t is not None and t.requires_grad for t in ins):
+            if field.data.requires_grad and (spec.hang_terms or (field.nc == 3 and st.slip is not None)):
+                raise NotImplementedError("gradients with respect to the field values through hanging-face "
+                                          "corrections or slip conditions are not supported")
+            if grad is not None and grad.requires_grad:
+                raise NotImplementedError("gradients through an explicit cell gradient are not supported")
+            diag, upper, lower, source, ffc = _AssembleFn.apply(spec, *ins)
+        else:
+            if track and field.data.requires_grad and spec.hang_terms:
+                raise NotImplementedError("gradients with respect to the field values through hanging-face "
+                                          "corrections are not supported")
+            diag, upper, lower, source, ffc = spec.run()
+        mat._diag, mat._upper, mat._lower, mat._source = diag, upper, lower, source
+        if spec.has_ffc:
             mat.face_flux_correction = ffc[: g.num_internal_faces]
         return mat
 

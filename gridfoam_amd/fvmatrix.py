@@ -79,6 +79,17 @@ class FvMatrix:
     # ---- algebra (fvmatrix.py:181-240) -------------------------------------
     def _combine(self, other: "FvMatrix", sign: float) -> "FvMatrix":
         res = FvMatrix(self.field)
+        if torch.is_grad_enabled() and any(t.requires_grad for m in (self, other) for t in (
+                m._diag, m._upper, m._lower, m._source)):
+            # differentiable term sum (the values equal the native axpby's)
+            res._diag = self._diag + sign * other._diag
+            res._upper = self._upper + sign * other._upper
+            res._lower = self._lower + sign * other._lower
+            res._source = self._source + sign * other._source
+            fa, fb = self.face_flux_correction, other.face_flux_correction
+            if fa is not None or fb is not None:
+                res.face_flux_correction = (fa if fa is not None else 0.0) + sign * (fb if fb is not None else 0.0)
+            return res
         st = nat.stream()
         for a, b, out in ((self._diag, other._diag, res._diag), (self._upper, other._upper, res._upper),
                           (self._lower, other._lower, res._lower), (self._source, other._source, res._source)):

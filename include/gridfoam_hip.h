@@ -164,6 +164,24 @@ typedef struct gf_assembly {
 } gf_assembly;
 int gf_assemble(const gf_mesh *m, const gf_assembly *a, double *diag, double *upper,
                 double *lower, double *source, double *ffc, gf_stream_t stream);
+/* Reverse mode of gf_assemble, for the assembly inputs that autograd
+ * differentiates in the reference's torch assembly (fv/fvm/laplacian.py:38-201,
+ * fv/fvm/div.py:15-73, fv/schemes/div.py:36-142, fv/fvm/ddt.py:7-40): given
+ * the gradients of a scalar with respect to diag [C], upper [F], lower [F],
+ * source [C,nc] and ffc [F,nc] (g_ffc may be NULL: zero), writes
+ *   g_gamma [C]        d/d gamma_cell; with a uniform gamma, per-cell shares
+ *                      whose sum is d/d gamma (needs work [F] scratch),
+ *   g_phi_s [F], g_phi_b [B], g_psi_old [C,nc], g_psi_older [C,nc],
+ *   g_ref_v [B,nc], g_ref_g [B,nc]   (boundary faces not covered: 0).
+ * Any output may be NULL.  The field values psi (hanging-face corrections)
+ * and the boundary fractions are held fixed.  Upwind follows torch.clamp's
+ * backward at phi = 0; the harmonic mean follows torch.minimum's tie rule
+ * where both diffusivities are zero. */
+int gf_assemble_vjp(const gf_mesh *m, const gf_assembly *a, const double *g_diag,
+                    const double *g_upper, const double *g_lower, const double *g_source,
+                    const double *g_ffc, double *work, double *g_gamma, double *g_phi_s,
+                    double *g_phi_b, double *g_psi_old, double *g_psi_older, double *g_ref_v,
+                    double *g_ref_g, gf_stream_t stream);
 
 /* ---- LDU algebra (fvmatrix.py:242-323, :424-468) ------------------------ */
 int gf_ldu_multiply(const gf_mesh *m, int32_t nc, const double *diag, const double *upper,

@@ -1,0 +1,286 @@
+"""GPU: the assembly's reverse mode (``gf_assemble_vjp`` behind
+``fvm.assemble``) and gradients through assembly + solve.
+
+Pinned by: the reference's own BDF2 source gradients 2V/dt and -0.5V/dt
+(tests/integration/fv/test_ddt_schemes.py:104-120); the harmonic face
+diffusivity's derivative against torch autograd of the reference's
+expression (fv/fvm/laplacian.py:38-73), including insulating (zero) and
+extreme diffusivities (tests/integration/fv/test_harmonic_diffusion.py:
+93-132); central differences of a random linear functional of every
+assembled array along random directions of each input (the assembly is
+linear in each of them away from phi = 0, so the differences are exact to
+rounding); and central differences of J = w.x through the solve (north-star
+bar 1e-5 relative), for PCG and BiCGSTAB."""
+
+from __future__ import annotations
+
+import sys
+
+import pytest
+import torch
+
+from tests.cases import fluxel_block
+
+pytestmark = pytest.mark.gpu
+
+PATCHES = ("x_minus", "x_plus", "y_minus", "y_plus", "z_minus", "z_plus")
+
+
+def _say(msg: str) -> None:
+    print(f"[gpu-case] {msg}", file=sys.stderr, flush=True)
+
+
+def _grid(name: str, dt: float = 0.01):
+    from gridfoam_amd.config import FluxelConfig
+    from gridfoam_amd.grid import OctreeGrid
+
+    _say(f"grid {name}")
+    return OctreeGrid(FluxelConfig.model_validate(fluxel_block(name)), dt=dt)
+
+
+def test_bdf2_source_gradients_match_the_reference():
+    """test_ddt_schemes.py:104-120: d source / d old = 2 V/dt and
+    d source / d older = -0.5 V/dt for equal steps."""
+    from gridfoam_amd.fields import CellField
+    from gridfoam_amd.fv import fvm
+
+    g = _grid("refined3d")
+    f = CellField(g, "q", (), transient=True)
+    older = torch.full_like(f.data, 1.0, requires_grad=True)
+    old = torch.full_like(f.data, 2.0, requires_grad=True)
+    f.data = older
+    f.update_history(reset=True)
+    f.data = old
+    f.update_history()
+    m = fvm.assemble(f, ddt=1.0, abc=(1.5, 2.0, 0.5))
+    g_old, g_older = torch.autograd.grad(m.source.sum(), (old, older))
+    V = g.cell_volumes
+    torch.testing.assert_close(g_old, 2.0 * V / g.dt, rtol=1e-14, atol=0)
+    torch.testing.assert_close(g_older, -0.5 * V / g.dt, rtol=1e-14, atol=0)
+
+
+def _torch_harmonic(g, gamma):
+    """fv/fvm/laplacian.py:38-73 written in torch (autograd is the check)."""
+    F = g.num_internal_faces
+    go, gn = gamma[g.owner], gamma[g.neighbour]
+    w = g.w[:F]
+    scale = torch.maximum(go, gn)
+    safe = torch.where(scale > 0, scale, torch.ones_like(scale))
+    den = w * (go / safe) + (1.0 - w) * (gn / safe)
+    safe_den = torch.where(den > 0, den, torch.ones_like(den))
+    return torch.minimum(go, gn) / safe_den
+
+
+@pytest.mark.parametrize("kind", ["positive", "insulating", "zero", "huge", "tiny", "ties"])
+def test_harmonic_diffusivity_gradient_matches_autograd(kind):
+    from gridfoam_amd.fields import CellField
+    from gridfoam_amd.fv import fvm
+
+    g = _grid("refined3d")
+    C, F = g.num_cells, g.num_internal_faces
+    x = g.cell_centers[:, 0]
+    gamma = {"positive": torch.linspace(1.0, 3.0, C, dtype=torch.float64, device=g.device),
+             "insulating": torch.where(x < 0.5, 0.0, 2.0).to(torch.float64),
+             "zero": torch.zeros(C, dtype=torch.float64, device=g.device),
+             "huge": torch.full((C,), 1e200, dtype=torch.float64, device=g.device),
+             "tiny": torch.full((C,), 1e-200, dtype=torch.float64, device=g.device),
+             "ties": torch.where(x < 0.5, 1.0, 2.0).to(torch.float64)}[kind]
+    _say(f"harmonic gradient {kind}")
+    P = CellField(g, "q_h", ())
+    wts = torch.randn(F, dtype=torch.float64, device=g.device, generator=torch.Generator(g.device).manual_seed(3))
+    gam = gamma.clone().requires_grad_()
+    m = fvm.assemble(P, lap=1.0, gamma=gam, corrected=False, harmonic=True)
+    (ours,) = torch.autograd.grad((m.upper * wts).sum(), gam)
+    gt = gamma.clone().requires_grad_()
+    coeff = _torch_harmonic(g, gt) * g.mag_Sf[:F] * g.delta_coeffs[:F]
+    (ref,) = torch.autograd.grad((coeff * wts).sum(), gt)
+    assert torch.isfinite(ours).all()
+    torch.testing.assert_close(ours, ref, rtol=1e-12, atol=1e-300)
+
+
+def _setup(name, nc, scheme, harmonic, seed=0):
+    """A field with Dirichlet, Neumann and zero-gradient patches and every
+    differentiable assembly input as a leaf tensor."""
+    from gridfoam_amd.fields import CellField, DirichletBC, FaceField, NeumannBC
+
+    g = _grid(name)
+    rng = torch.Generator(g.device).manual_seed(seed)
+    C, F, B = g.num_cells, g.num_internal_faces, g.num_domain_bnd_faces
+    cs = () if nc == 1 else (3,)
+    dev = g.device
+
+    def rnd(*shape, lo=None):
+        t = torch.randn(*shape, dtype=torch.float64, device=dev, generator=rng)
+        return t if lo is None else lo + t.abs()
+
+    P = CellField(g, f"q{nc}", cs, transient=True)
+    P.data = rnd(C, *cs)
+    P.update_history(reset=True)
+    P.data = rnd(C, *cs)
+    P.update_history()
+    P.data = rnd(C, *cs)
+    leaves = {
+        "gamma": rnd(C, lo=0.5),
+        "phi_s": rnd(F).sign() * (0.2 + rnd(F).abs()),  # away from phi = 0
+        "phi_b": rnd(B),
+        "old": P.old_data.detach().clone(),
+        "older": P.older_data.detach().clone(),
+        "dirichlet": rnd(*cs) if cs else rnd(()),
+        "neumann": rnd(*cs) if cs else rnd(()),
+    }
+    return g, P, leaves, cs, scheme, harmonic
+
+
+def _assemble(g, P, vals, cs, scheme, harmonic):
+    from gridfoam_amd.config import DivScheme
+    from gridfoam_amd.fields import DirichletBC, FaceField, NeumannBC
+    from gridfoam_amd.fv import fvm
+
+    P.add_boundary_conditions({"x_minus": DirichletBC(vals["dirichlet"]), "y_plus": DirichletBC(vals["dirichlet"]),
+                               "x_plus": NeumannBC(vals["neumann"]), "z_minus": NeumannBC(vals["neumann"]),
+                               "y_minus": NeumannBC(torch.zeros(cs, dtype=torch.float64))})
+    phi = FaceField(g, "phi_adj")
+    phi.single_data = vals["phi_s"]
+    phi.domain_bnd_data = vals["phi_b"]
+    P.old_data = vals["old"]
+    P.older_data = vals["older"]
+    return fvm.assemble(P, ddt=1.0, div=1.0, lap=-1.0, phi=phi, gamma=vals["gamma"], abc=(1.5, 2.0, 0.5),
+                        scheme=DivScheme.LINEAR if scheme == "linear" else DivScheme.UPWIND, corrected=True,
+                        harmonic=harmonic)
+
+
+def _functional(g, m, wts):
+    F = g.num_internal_faces
+    J = (m.diag * wts["d"]).sum() + (m.upper * wts["u"][:F]).sum() + (m.lower * wts["l"][:F]).sum()
+    J = J + (m.source * wts["s"]).sum()
+    if m.face_flux_correction is not None:
+        J = J + (m.face_flux_correction * wts["f"][:F]).sum()
+    return J
+
+
+@pytest.mark.parametrize("nc,scheme,harmonic", [(1, "upwind", False), (1, "linear", True), (3, "linear", False),
+                                                (3, "upwind", True)])
+def test_assembly_vjp_matches_central_differences(nc, scheme, harmonic):
+    """<grad J, v> from the VJP equals (J(x + hv) - J(x - hv)) / 2h for each
+    input on the refined 3-D grid (hanging faces, corrected Laplacian)."""
+    g, P, leaves, cs, scheme, harmonic = _setup("refined3d", nc, scheme, harmonic)
+    _say(f"assembly vjp nc={nc} {scheme} harmonic={harmonic}")
+    gen = torch.Generator(g.device).manual_seed(11)
+    C, F = g.num_cells, g.num_internal_faces
+    wts = {k: torch.randn(*shape, dtype=torch.float64, device=g.device, generator=gen)
+           for k, shape in (("d", (C,)), ("u", (max(F, 1),)), ("l", (max(F, 1),)), ("s", (C, *cs)),
+                            ("f", (max(F, 1), *cs)))}
+    params = {k: v.clone().requires_grad_() for k, v in leaves.items()}
+    m = _assemble(g, P, params, cs, scheme, harmonic)
+    J = _functional(g, m, wts)
+    grads = dict(zip(params, torch.autograd.grad(J, list(params.values()))))
+    with torch.no_grad():
+        for k, base in leaves.items():
+            v = torch.randn(base.shape, dtype=torch.float64, device=g.device, generator=gen)
+            h = 1e-4 * max(float(base.abs().max()), 1.0)
+
+            def J_at(t):
+                vals = dict(leaves)
+                vals[k] = base + t * v
+                return float(_functional(g, _assemble(g, P, vals, cs, scheme, harmonic), wts))
+            fd = (J_at(h) - J_at(-h)) / (2 * h)
+            an = float((grads[k] * v).sum())
+            assert abs(an - fd) <= 1e-6 * max(abs(fd), 1e-3), (k, an, fd)
+
+
+@pytest.mark.parametrize("method", ["cg", "bicgstab"])
+def test_gradients_through_assembly_and_solve_match_central_differences(method):
+    """J = w . x with A(theta) x = b(theta) assembled natively and solved on
+    the GPU; dJ/dtheta (solve adjoint + assembly VJP) against central
+    differences for the diffusivity, the flux (BiCGSTAB), the old levels and
+    the boundary values, within 1e-5 relative."""
+    from gridfoam_amd.config import SolverConfig
+    from gridfoam_amd.fvmatrix import Equation
+    from gridfoam_amd.solvers import create_solver
+
+    g, P, leaves, cs, scheme, harmonic = _setup("refined3d", 1, "upwind", False, seed=5)
+    _say(f"assembly + solve gradient {method}")
+    if method == "cg":  # symmetric: ddt + laplacian only
+        leaves["phi_s"] = torch.zeros_like(leaves["phi_s"])
+        leaves["phi_b"] = torch.zeros_like(leaves["phi_b"])
+    solver = create_solver(SolverConfig(method=method, preconditioner="jacobi", tolerance=0.0,
+                                        rel_tolerance=1e-14, max_iter=5000))
+    gen = torch.Generator(g.device).manual_seed(23)
+    w = torch.randn(g.num_cells, dtype=torch.float64, device=g.device, generator=gen)
+
+    def J_of(vals, div_on):
+        from gridfoam_amd.config import DivScheme
+        from gridfoam_amd.fields import DirichletBC, FaceField, NeumannBC
+        from gridfoam_amd.fv import fvm
+
+        P.add_boundary_conditions({"x_minus": DirichletBC(vals["dirichlet"]), "x_plus": NeumannBC(vals["neumann"])})
+        phi = FaceField(g, "phi_adj2")
+        phi.single_data = vals["phi_s"]
+        phi.domain_bnd_data = vals["phi_b"]
+        P.old_data, P.older_data = vals["old"], vals["older"]
+        m = fvm.assemble(P, ddt=1.0, div=1.0 if div_on else 0.0, lap=-1.0, phi=phi, gamma=vals["gamma"],
+                         abc=(1.5, 2.0, 0.5), scheme=DivScheme.UPWIND, corrected=True)
+        x = solver.solve(Equation(P, m)).solution
+        return (w * x).sum()
+
+    keys = ["gamma", "old", "older", "dirichlet", "neumann"] + (["phi_s", "phi_b"] if method == "bicgstab" else [])
+    params = {k: (v.clone().requires_grad_() if k in keys else v) for k, v in leaves.items()}
+    J = J_of(params, method == "bicgstab")
+    grads = dict(zip(keys, torch.autograd.grad(J, [params[k] for k in keys])))
+    for k in keys:
+        base = leaves[k]
+        v = torch.randn(base.shape, dtype=torch.float64, device=g.device, generator=gen)
+        h = 1e-5 * max(float(base.abs().max()), 1.0)
+        with torch.no_grad():
+            def J_at(t):
+                vals = dict(leaves)
+                vals[k] = base + t * v
+                return float(J_of(vals, method == "bicgstab"))
+            fd = (J_at(h) - J_at(-h)) / (2 * h)
+        an = float((grads[k] * v).sum())
+        assert abs(an - fd) <= 1e-5 * max(abs(fd), 1e-8), (k, an, fd)
+
+
+def test_uniform_diffusivity_tensor_gradient():
+    """A 0-d diffusivity (the transport model's nu) gets the sum of the
+    per-face and boundary shares: checked against central differences."""
+    from gridfoam_amd.fields import CellField, DirichletBC
+    from gridfoam_amd.fv import fvm
+
+    g = _grid("refined2d")
+    P = CellField(g, "q_nu", ())
+    P.add_boundary_conditions({p: DirichletBC(1.0) for p in ("x_minus", "y_plus")})
+    gen = torch.Generator(g.device).manual_seed(2)
+    P.data = torch.randn(g.num_cells, dtype=torch.float64, device=g.device, generator=gen)
+    wd = torch.randn(g.num_cells, dtype=torch.float64, device=g.device, generator=gen)
+    ws = torch.randn(g.num_cells, dtype=torch.float64, device=g.device, generator=gen)
+    _say("uniform diffusivity gradient")
+
+    def J(nu):
+        m = fvm.assemble(P, lap=-1.0, gamma=nu, corrected=True)
+        return (m.diag * wd).sum() + (m.source * ws).sum() + m.upper.sum() - 2.0 * m.lower.sum()
+
+    nu = torch.tensor(0.01, dtype=torch.float64, device=g.device, requires_grad=True)
+    (gn,) = torch.autograd.grad(J(nu), nu)
+    h = 1e-6
+    with torch.no_grad():
+        fd = (float(J(torch.tensor(0.01 + h, dtype=torch.float64, device=g.device)))
+              - float(J(torch.tensor(0.01 - h, dtype=torch.float64, device=g.device)))) / (2 * h)
+    assert abs(float(gn) - fd) <= 1e-8 * abs(fd)
+
+
+def test_field_values_through_hanging_corrections_raise():
+    """The field values enter the corrected Laplacian on hanging faces; a
+    field that requires grad there is refused rather than losing it."""
+    from gridfoam_amd.fields import CellField
+    from gridfoam_amd.fv import fvm
+
+    g = _grid("refined3d")
+    P = CellField(g, "q_raise", ())
+    P.data = torch.ones(g.num_cells, dtype=torch.float64, device=g.device, requires_grad=True)
+    _say("field-value grad guard")
+    with pytest.raises(NotImplementedError):
+        fvm.assemble(P, lap=-1.0, gamma=1.0, corrected=True)
+    with torch.no_grad():
+        m = fvm.assemble(P, lap=-1.0, gamma=1.0, corrected=True)
+    assert m.face_flux_correction is not None and not m.diag.requires_grad

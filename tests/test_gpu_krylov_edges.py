@@ -212,20 +212,18 @@ def test_sliced_ell_layout(name):
             assert all(c == i and k == np.iinfo(np.int32).min for c, k in got[len(ent):]), (i, got)
 
 
-def test_assembly_refuses_inputs_that_require_grad():
-    """The native assembly records no autograd graph, so a diffusivity that
-    requires gradients raises instead of losing them silently; without
-    gradients (or under no_grad) the same call assembles."""
+def test_assembly_with_a_grad_requiring_diffusivity_equals_the_plain_assembly():
+    """A diffusivity that requires gradients goes through the differentiable
+    assembly node; its arrays equal the plain assembly's bit for bit."""
     from gridfoam_amd.fields import CellField
     from gridfoam_amd.fv import fvm
 
     g, _ = _grid("refined3d")
     P = CellField(g, "q_grad", ())
-    gamma = torch.ones(g.num_cells, dtype=torch.float64, device=g.device, requires_grad=True)
-    _say("assembly grad guard")
-    with pytest.raises(NotImplementedError):
-        fvm.assemble(P, lap=-1.0, gamma=gamma)
-    with torch.no_grad():
-        m = fvm.assemble(P, lap=-1.0, gamma=gamma)
+    gamma = torch.linspace(0.5, 1.5, g.num_cells, dtype=torch.float64, device=g.device).requires_grad_()
+    _say("assembly with grad-requiring diffusivity")
+    m = fvm.assemble(P, lap=-1.0, gamma=gamma)
+    assert m.diag.requires_grad and m.upper.requires_grad
     m2 = fvm.assemble(P, lap=-1.0, gamma=gamma.detach())
-    assert torch.equal(m.diag, m2.diag) and torch.equal(m.upper, m2.upper)
+    assert torch.equal(m.diag, m2.diag) and torch.equal(m.upper, m2.upper) and torch.equal(m.source, m2.source)
+    assert torch.equal(m.face_flux_correction, m2.face_flux_correction)
