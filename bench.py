@@ -11,7 +11,8 @@ within 1e-6 relative L2 of the plain FP64 run.
 
 Also measured: the dominant kernel (the PCG SpMV) timed with HIP events on
 the solver's stream, against the HBM roofline, and the FP64 CPU oracle on
-the host cores as a bounded sample of the same step.
+the host cores as a bounded sample of the same step; the 2-D SIMPLE cavity
+(64x64, 500 iterations) on the GPU beside the oracle; the serving stream.
 """
 
 from __future__ import annotations
@@ -225,6 +226,51 @@ def cpu_baseline(gpu_p_iters_per_step: float) -> dict:
                        f"{gpu_p_iters_per_step:.0f} CG iterations per step as on the GPU; numpy FP64, 1 thread")}
 
 
+def config1_measure(cpu: bool, cpu_iters: int = 100) -> dict:
+    """The reference's CPU-runnable case: the 2-D lid-driven cavity, 64x64x1,
+    Re 100, SIMPLE, 500 iterations, FP64.  GPU: the 500 iterations timed
+    (synchronised at both ends), the final U and p against the oracle's
+    500-iteration state (the committed golden fixture).  CPU: the FP64 oracle
+    timed over the first `cpu_iters` iterations on the host."""
+    import numpy as np
+    import torch
+    import yaml
+
+    from gridfoam_amd.algorithms import create_case
+    from gridfoam_amd.config import CASES, load_config
+
+    cfg = yaml.safe_load((CASES / "cavity2d_simple.yaml").read_text())
+    n = int(round(cfg["simulator"]["control"]["endTime"] / cfg["simulator"]["control"]["deltaT"]))
+    grid, alg = create_case(load_config(cfg), device="cuda:0")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(n):
+        alg.step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    C = grid.num_cells
+    out = {"workload": "2-D lid-driven cavity 64x64x1, Re 100, SIMPLE, 500 iterations, FP64", "cells": C,
+           "iterations": n, "solve_s": round(dt, 3), "value": round(C * n / dt / 1e6, 4), "unit": "Mcell-steps/s"}
+    gold = ROOT / "tests" / "golden" / "cavity2d_simple_500.npz"
+    if gold.exists():
+        d = np.load(gold)
+        U, p = alg.U.data.cpu().numpy(), alg.p.data.cpu().numpy()
+        out["rel_l2_vs_oracle"] = {"U": float(np.linalg.norm(U - d["U"]) / np.linalg.norm(d["U"])),
+                                   "p": float(np.linalg.norm(p - d["p"]) / np.linalg.norm(d["p"]))}
+    if cpu:
+        from oracle import fv_oracle as fo
+
+        case = fo.Case(cfg)
+        t0 = time.perf_counter()
+        for _ in range(cpu_iters):
+            case.step()
+        tc = (time.perf_counter() - t0) / cpu_iters
+        out["cpu_baseline"] = {"value": round(C / tc / 1e6, 5), "unit": "Mcell-steps/s", "cores": 1, "kind": "port",
+                               "sample": f"the first {cpu_iters} SIMPLE iterations of the same case "
+                                         f"({tc * 1e3:.0f} ms each); numpy FP64, 1 thread"}
+    return out
+
+
 def serve_measure(n_requests: int, slots: int, iterations: int = 5, seed: int = 0) -> dict:
     """Serving stream of random lid-driven cases (base n^3,
     n log-uniform in [16, 96], 0-2 refinement levels around 1-4 boxes,
@@ -270,6 +316,7 @@ def main() -> int:
     ap.add_argument("--mode", choices=("both", "plain", "mixed"), default="both")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-large", action="store_true", help="skip the 256^3 SpMV measurement")
+    ap.add_argument("--no-config1", action="store_true", help="skip the 64x64 SIMPLE cavity measurement")
     ap.add_argument("--spmv-only", choices=("fp32", "fp64"), default=None,
                     help="only launch the SpMV kernel (for counter runs) and exit")
     ap.add_argument("--spmv-mesh", choices=("cavity", "uniform256"), default="cavity",
@@ -368,6 +415,9 @@ def main() -> int:
         line["roofline_256cubed"] = {"fp32": spmv_roofline(big, pm, fp32=True, launches=50),
                                      "fp64": spmv_roofline(big, pm, fp32=False, launches=50)}
         del big, pm
+    if not a.no_config1:
+        _log("2-D SIMPLE cavity 64x64, 500 iterations")
+        line["cavity2d_simple_64"] = config1_measure(cpu=not a.no_cpu)
     if a.serve_requests > 0:
         _log(f"serving: {a.serve_requests} requests, {a.serve_slots} slots")
         line["serving"] = serve_measure(a.serve_requests, a.serve_slots)
