@@ -204,6 +204,58 @@ __device__ __forceinline__ double gamma_face(const gf_assembly &a, const gf_mesh
     return fmin(go, gn) / (den > 0.0 ? den : 1.0);
 }
 
+// TVD convection (schemes/div.py:145-350): the upwind matrix plus, per face,
+// the deferred-correction source -phi psi(r) (psi_linear - psi_upwind).
+// Scheme codes 2..7: vanLeer, minmod, superbee, monotonized central,
+// limitedLinear, Koren.
+__device__ __forceinline__ double tvd_limiter(int scheme, double r) {
+    switch (scheme) {
+        case 2: return (r + fabs(r)) / (1.0 + fabs(r));
+        case 3: return fmax(fmin(1.0, r), 0.0);
+        case 4: return fmax(fmax(fmin(2.0 * r, 1.0), fmin(r, 2.0)), 0.0);
+        case 5: return fmax(fmin(fmin(2.0 * r, 0.5 * r + 0.5), 2.0), 0.0);
+        case 6: return fmax(fmin(2.0 * r, 1.0), 0.0);
+        default: return fmax(fmin(fmin(2.0 * r, (2.0 + r) / 3.0), 2.0), 0.0);
+    }
+}
+
+__device__ __forceinline__ double sgn(double v) { return (double)((v > 0.0) - (v < 0.0)); }
+
+// phi psi(r) of face f, with its linear (uncorrected) and upwind values.
+// r = 2 (grad_U . d) / (psi_N - psi_O) - 1 from the upwind cell's gradient
+// (vectors: component sums, NVDVTVDV); |num| >= 1000 |den| is steep.
+template <int NC>
+__device__ __forceinline__ double tvd_weight(const gf_mesh &m, const gf_assembly &a, int f, double (&base)[NC],
+                                             double (&up)[NC]) {
+    const int o = m.owner[f], n = m.neighbour[f];
+    const double ph = a.phi_s[f], w = m.w[f];
+    const bool own = ph > 0.0;
+    const int u = own ? o : n;
+    double d[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) d[j] = m.centers[3 * n + j] - m.centers[3 * o + j];
+    double num = 0.0, den = 0.0;
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+        const double po = a.psi[o * NC + q], pn = a.psi[n * NC + q];
+        base[q] = w * po + (1.0 - w) * pn;
+        up[q] = own ? po : pn;
+        const double *gu = a.grad + (u * NC + q) * 3;
+        const double gd = (gu[0] * d[0] + gu[1] * d[1]) + gu[2] * d[2];
+        const double dp = pn - po;
+        if (NC == 1) {
+            num = gd;
+            den = dp;
+        } else {
+            num += dp * gd;
+            den += dp * dp;
+        }
+    }
+    const bool steep = fabs(num) >= 1000.0 * fabs(den);
+    const double r = steep ? 2.0 * 1000.0 * sgn(num) * sgn(den) - 1.0 : 2.0 * num / den - 1.0;
+    return ph * tvd_limiter(a.div_scheme, r);
+}
+
 template <int NC>
 __global__ void k_asm_faces(const gf_mesh m, const gf_assembly a, double *upper, double *lower, double *ffc) {
     int f = blockIdx.x * blockDim.x + threadIdx.x;
@@ -212,7 +264,7 @@ __global__ void k_asm_faces(const gf_mesh m, const gf_assembly a, double *upper,
     if (a.s_div != 0.0) {
         const double ph = a.phi_s[f], w = m.w[f];
         double ud, ld;
-        if (a.div_scheme == 0) {
+        if (a.div_scheme != 1) {
             ud = fmin(ph, 0.0);
             ld = -fmax(ph, 0.0);
         } else {
@@ -253,15 +305,31 @@ __global__ void k_asm_cells(const gf_mesh m, const gf_assembly a, double *diag, 
     }
     if (a.s_div != 0.0) {
         double so = 0.0, sn = 0.0, sb = 0.0;
+        const bool tvd = a.div_scheme >= 2;
+        double ts[NC] = {};
         for (int f = m.own_row[c]; f < m.own_row[c + 1]; ++f) {
             const double ph = a.phi_s[f];
-            so += a.div_scheme == 0 ? fmax(ph, 0.0) : ph * m.w[f];
+            so += a.div_scheme != 1 ? fmax(ph, 0.0) : ph * m.w[f];
+            if (tvd) {
+                double base[NC], up[NC];
+                const double lw = tvd_weight<NC>(m, a, f, base, up);
+#pragma unroll
+                for (int q = 0; q < NC; ++q) ts[q] += -lw * (base[q] - up[q]);
+            }
         }
         for (int k = m.nbr_row[c]; k < m.nbr_row[c + 1]; ++k) {
             const int f = m.nbr_face[k];
             const double ph = a.phi_s[f];
-            sn += a.div_scheme == 0 ? -fmin(ph, 0.0) : -ph * (1.0 - m.w[f]);
+            sn += a.div_scheme != 1 ? -fmin(ph, 0.0) : -ph * (1.0 - m.w[f]);
+            if (tvd) {
+                double base[NC], up[NC];
+                const double lw = tvd_weight<NC>(m, a, f, base, up);
+#pragma unroll
+                for (int q = 0; q < NC; ++q) ts[q] -= -lw * (base[q] - up[q]);
+            }
         }
+#pragma unroll
+        for (int q = 0; q < NC; ++q) src[q] = src[q] + a.s_div * ts[q];
         double bs[NC] = {};
         for (int k = m.bnd_row[c]; k < m.bnd_row[c + 1]; ++k) {
             if (!a.covered[k]) continue;
@@ -312,7 +380,7 @@ __global__ void k_asm_hang(const gf_mesh m, const gf_assembly a, double *source,
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m.n_hang_cells) return;
     const int c = m.hang_cell[i];
-    const bool do_div = a.s_div != 0.0 && a.div_scheme == 1;
+    const bool do_div = a.s_div != 0.0 && a.div_scheme >= 1;
     const bool do_lap = a.s_lap != 0.0 && a.lap_corrected;
     if (!do_div && !do_lap) return;
     double add[NC] = {};
@@ -328,7 +396,12 @@ __global__ void k_asm_hang(const gf_mesh m, const gf_assembly a, double *source,
                 const double w = m.w[f];
                 const double base = w * a.psi[o * NC + q] + (1.0 - w) * a.psi[n * NC + q];
                 const double corrected = base + hang_interp_corr<NC>(m, a.grad, h, q);
-                const double sface = -a.phi_s[f] * (corrected - base);
+                double fl = a.phi_s[f];
+                if (a.div_scheme >= 2) {  // TVD: the limited flux
+                    double b2[NC], u2[NC];
+                    fl = tvd_weight<NC>(m, a, f, b2, u2);
+                }
+                const double sface = -fl * (corrected - base);
                 add[q] += a.s_div * (own ? sface : -sface);
             }
             if (do_lap) {
@@ -384,8 +457,8 @@ __device__ __forceinline__ void gamma_face_partials(const gf_assembly &a, const 
 // min(phi, 0) and the max(phi, 0) branches).
 template <int NC>
 __global__ void k_asmv_faces(const gf_mesh m, const gf_assembly a, const double *__restrict__ gd,
-                             const double *__restrict__ gu, const double *__restrict__ gl, double *work,
-                             double *g_phi_s) {
+                             const double *__restrict__ gu, const double *__restrict__ gl,
+                             const double *__restrict__ gs, double *work, double *g_phi_s) {
     int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= m.n_faces) return;
     const int o = m.owner[f], n = m.neighbour[f];
@@ -393,10 +466,16 @@ __global__ void k_asmv_faces(const gf_mesh m, const gf_assembly a, const double 
         double gp = 0.0;
         if (a.s_div != 0.0) {
             const double ph = a.phi_s[f];
-            if (a.div_scheme == 0) {
+            if (a.div_scheme != 1) {
                 const double neg = ph <= 0.0 ? gu[f] - gd[n] : 0.0;
                 const double pos = ph >= 0.0 ? gd[o] - gl[f] : 0.0;
                 gp = neg + pos;
+                if (a.div_scheme >= 2 && ph != 0.0) {  // d/dphi of -phi psi(r) (base - up), r held fixed
+                    double base[NC], up[NC];
+                    const double lim = tvd_weight<NC>(m, a, f, base, up) / ph;
+#pragma unroll
+                    for (int q = 0; q < NC; ++q) gp += -lim * (base[q] - up[q]) * (gs[o * NC + q] - gs[n * NC + q]);
+                }
             } else {
                 const double w = m.w[f];
                 gp = ((1.0 - w) * gu[f] - w * gl[f]) + (w * gd[o] - (1.0 - w) * gd[n]);
@@ -421,14 +500,20 @@ __global__ void k_asmv_hang(const gf_mesh m, const gf_assembly a, const double *
     if (h >= m.n_hang) return;
     const int f = m.hang_face[h];
     const int o = m.owner[f], n = m.neighbour[f];
-    if (g_phi_s && a.s_div != 0.0 && a.div_scheme == 1) {
+    if (g_phi_s && a.s_div != 0.0 && a.div_scheme >= 1) {
         const double w = m.w[f];
+        double lim = 1.0;
+        if (a.div_scheme >= 2) {  // TVD: the limiter, r held fixed
+            double b2[NC], u2[NC];
+            const double ph = a.phi_s[f];
+            lim = ph != 0.0 ? tvd_weight<NC>(m, a, f, b2, u2) / ph : 0.0;
+        }
         double acc = 0.0;
 #pragma unroll
         for (int q = 0; q < NC; ++q) {
             const double base = w * a.psi[o * NC + q] + (1.0 - w) * a.psi[n * NC + q];
             const double corrected = base + hang_interp_corr<NC>(m, a.grad, h, q);
-            acc += -(corrected - base) * (gs[o * NC + q] - gs[n * NC + q]);
+            acc += -lim * (corrected - base) * (gs[o * NC + q] - gs[n * NC + q]);
         }
         g_phi_s[f] += a.s_div * acc;
     }
@@ -998,8 +1083,11 @@ extern "C" int gf_assemble(const gf_mesh *m, const gf_assembly *a, double *diag,
     GF_REQUIRE(a->s_ddt == 0.0 || a->ddt_c == 0.0 || a->psi_older, "backward ddt needs older values");
     GF_REQUIRE(a->s_div == 0.0 || (a->phi_s && a->phi_b), "div needs face flux");
     GF_REQUIRE(a->covered && a->frac && a->ref_v && a->ref_g, "boundary state missing");
-    const bool need_grad = m->n_hang > 0 && ((a->s_div != 0.0 && a->div_scheme == 1) || (a->s_lap != 0.0 && a->lap_corrected));
-    GF_REQUIRE(!need_grad || (a->grad && a->psi), "hanging faces need psi and its gradient");
+    GF_REQUIRE(a->div_scheme >= 0 && a->div_scheme <= 7, "unknown div scheme");
+    const bool tvd = a->s_div != 0.0 && a->div_scheme >= 2;
+    const bool need_grad =
+        tvd || (m->n_hang > 0 && ((a->s_div != 0.0 && a->div_scheme == 1) || (a->s_lap != 0.0 && a->lap_corrected)));
+    GF_REQUIRE(!need_grad || (a->grad && a->psi), "hanging faces and TVD schemes need psi and its gradient");
     hipStream_t s = S(stream);
     GF_NC_DISPATCH(a->nc, k_asm_faces, grid_for(m->n_faces), *m, *a, upper, lower, ffc);
     GF_NC_DISPATCH(a->nc, k_asm_cells, grid_for(m->n_cells), *m, *a, diag, source);
@@ -1018,15 +1106,17 @@ extern "C" int gf_assemble_vjp(const gf_mesh *m, const gf_assembly *a, const dou
     GF_REQUIRE(!g_gamma || work, "the diffusivity gradient needs a work array of one double per face");
     GF_REQUIRE(a->s_div == 0.0 || (a->phi_s && a->phi_b), "div needs face flux");
     GF_REQUIRE(a->s_ddt == 0.0 || a->dt > 0.0, "ddt needs dt > 0");
-    const bool hang_div = a->s_div != 0.0 && a->div_scheme == 1 && g_phi_s;
+    const bool hang_div = a->s_div != 0.0 && a->div_scheme >= 1 && g_phi_s;
     const bool hang_lap = a->s_lap != 0.0 && a->lap_corrected && g_gamma;
-    const bool need_grad = m->n_hang > 0 && (hang_div || hang_lap);
-    GF_REQUIRE(!need_grad || (a->grad && a->psi), "hanging faces need psi and its gradient");
+    const bool tvd = a->s_div != 0.0 && a->div_scheme >= 2 && g_phi_s;
+    const bool need_grad = tvd || (m->n_hang > 0 && (hang_div || hang_lap));
+    GF_REQUIRE(!need_grad || (a->grad && a->psi), "hanging faces and TVD schemes need psi and its gradient");
     hipStream_t s = S(stream);
     double *wk = g_gamma ? work : nullptr;
     if (m->n_faces > 0 && (wk || g_phi_s))
-        GF_NC_DISPATCH(a->nc, k_asmv_faces, grid_for(m->n_faces), *m, *a, g_diag, g_upper, g_lower, wk, g_phi_s);
-    if (need_grad)
+        GF_NC_DISPATCH(a->nc, k_asmv_faces, grid_for(m->n_faces), *m, *a, g_diag, g_upper, g_lower, g_source, wk,
+                       g_phi_s);
+    if (m->n_hang > 0 && (hang_div || hang_lap))
         GF_NC_DISPATCH(a->nc, k_asmv_hang, grid_for(m->n_hang), *m, *a, g_source, g_ffc, wk, g_phi_s);
     GF_NC_DISPATCH(a->nc, k_asmv_cells, grid_for(m->n_cells), *m, *a, g_diag, g_source, wk, g_gamma, g_phi_b,
                    g_psi_old, g_psi_older, g_ref_v, g_ref_g);

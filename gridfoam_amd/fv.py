@@ -41,12 +41,18 @@ def _lookup(table, key, default):
     return default if v is None else v
 
 
+# schemes/div.py:449-456 -- the scheme codes of gf_assembly.div_scheme
+DIV_SCHEME_CODES = {
+    DivScheme.UPWIND: 0, DivScheme.LINEAR: 1, DivScheme.VANLEER: 2, DivScheme.MINMOD: 3, DivScheme.SUPERBEE: 4,
+    DivScheme.MONOTONIZED_CENTRAL: 5, DivScheme.LIMITEDLINEAR: 6, DivScheme.KOREN: 7,
+}
+
+
 def div_scheme(grid, phi_name: str, field_name: str) -> DivScheme:
+    """schemes/selection.py: the scheme for div(phi, field), default upwind."""
     s = _lookup(grid.sim_config.fvSchemes.divSchemes if grid.sim_config else None,
                 f"div({phi_name}, {field_name})", DivScheme.UPWIND)
-    if s not in (DivScheme.UPWIND, DivScheme.LINEAR):
-        raise ValueError(f"div scheme {s} is not supported by this build")
-    return s
+    return DivScheme(s)
 
 
 def sn_grad_corrected(grid, field_name: str) -> bool:
@@ -179,12 +185,16 @@ class _Assembly:
             self.keep += [field.old_data, field.older_data]
         self.older_used = bool(ddt) and a.ddt_c != 0.0
         need_grad = False
+        self.tvd = False
         if div:
-            sch = scheme if scheme is not None else div_scheme(g, phi.name, field.name)
-            a.div_scheme = 1 if sch == DivScheme.LINEAR else 0
+            if phi is None or phi.component_shape != ():
+                raise ValueError("convection requires scalar face flux")  # fvm/div.py:41-42
+            sch = DivScheme(scheme) if scheme is not None else div_scheme(g, phi.name, field.name)
+            a.div_scheme = DIV_SCHEME_CODES[sch]
             a.phi_s, a.phi_b = nat.ptr(phi.single_buffer), nat.ptr(phi.bnd_buffer)
             self.keep += [phi.single_buffer, phi.bnd_buffer]
             need_grad |= a.div_scheme == 1
+            self.tvd = a.div_scheme >= 2
         self.gamma_t = None
         self.has_ffc = False
         if lap:
@@ -208,12 +218,18 @@ class _Assembly:
                     raise ValueError("Harmonic diffusivity must be finite and nonnegative")
             need_grad |= corr
             self.has_ffc = bool(corr and g.num_hanging_faces)
-        self.hang_terms = bool(need_grad and g.num_hanging_faces)
-        if self.hang_terms:
+        self.hang_terms = bool((need_grad or self.tvd) and g.num_hanging_faces)
+        if self.tvd:  # r needs the gradient of every upwind cell (schemes/div.py:270-275)
+            grad = fvc.grad(field)
+            a.grad = nat.ptr(grad)
+            self.keep.append(grad)
+        elif self.hang_terms:
             if grad is None:
                 grad = fvc.grad(field, hang_only=True)
             a.grad = nat.ptr(grad)
             self.keep.append(grad)
+        # the field values enter the arrays (hanging corrections, TVD sources)
+        self.psi_terms = self.hang_terms or self.tvd
         self.a = a
 
     def run(self):
@@ -290,8 +306,10 @@ class fvm:
         flux, the old and older field levels and the Dirichlet values and
         Neumann gradients of the field's conditions: the backward is the
         native VJP ``gf_assemble_vjp``.  The current field values enter only
-        through the hanging-face corrections and slip conditions; a field
-        that requires grad there raises, as does an explicit ``grad``
+        through the hanging-face corrections, TVD sources and slip
+        conditions; a field that requires grad there raises, as do boundary
+        values that require grad when such terms are present (they reach
+        the arrays through the cell gradient too) and an explicit ``grad``
         argument that requires grad."""
         g = field.grid
         st = field.boundary_state()
@@ -308,16 +326,21 @@ class fvm:
         rg = st.ref_g_ad if st.ref_g_ad is not None else st.ref_g
         ins = (gamma_in, phi_s, phi_b, old, older, rv, rg)
         if track and any(t is not None and t.requires_grad for t in ins):
-            if field.data.requires_grad and (spec.hang_terms or (field.nc == 3 and st.slip is not None)):
+            if field.data.requires_grad and (spec.psi_terms or (field.nc == 3 and st.slip is not None)):
                 raise NotImplementedError("gradients with respect to the field values through hanging-face "
-                                          "corrections or slip conditions are not supported")
+                                          "corrections, TVD sources or slip conditions are not supported")
             if grad is not None and grad.requires_grad:
                 raise NotImplementedError("gradients through an explicit cell gradient are not supported")
+            if spec.psi_terms and (rv.requires_grad or rg.requires_grad):
+                # the boundary values also enter through the cell gradient of
+                # the hanging corrections / TVD ratios, which is held fixed
+                raise NotImplementedError("gradients of boundary values are not supported with hanging-face "
+                                          "corrections or TVD schemes")
             diag, upper, lower, source, ffc = _AssembleFn.apply(spec, *ins)
         else:
-            if track and field.data.requires_grad and spec.hang_terms:
+            if track and field.data.requires_grad and spec.psi_terms:
                 raise NotImplementedError("gradients with respect to the field values through hanging-face "
-                                          "corrections are not supported")
+                                          "corrections or TVD sources are not supported")
             diag, upper, lower, source, ffc = spec.run()
         mat._diag, mat._upper, mat._lower, mat._source = diag, upper, lower, source
         if spec.has_ffc:

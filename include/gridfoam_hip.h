@@ -148,7 +148,9 @@ typedef struct gf_assembly {
     /* ddt: a, b, c weights, dt; psi_old/psi_older [C,nc] (older may be NULL) */
     double ddt_a, ddt_b, ddt_c, dt;
     const double *psi_old, *psi_older;
-    /* div: scheme 0 upwind, 1 linear; phi_s [F], phi_b [B] */
+    /* div: scheme 0 upwind, 1 linear, 2..7 TVD deferred correction (vanLeer, minmod,
+     * superbee, monotonized central, limitedLinear, Koren; schemes/div.py:145-461);
+     * phi_s [F], phi_b [B] */
     int32_t div_scheme;
     const double *phi_s, *phi_b;
     /* laplacian: face diffusivity from gamma_cell by linear interpolation, or
@@ -158,7 +160,7 @@ typedef struct gf_assembly {
     int32_t lap_corrected;
     int32_t lap_harmonic;
     /* field + boundary state */
-    const double *psi, *grad;          /* [C,nc], [C,nc,3] (hang cells) */
+    const double *psi, *grad;          /* [C,nc], [C,nc,3] (hang cells; every cell for TVD) */
     const double *frac, *ref_v, *ref_g, *psi_b; /* boundary state */
     const uint8_t *covered;
 } gf_assembly;

@@ -635,15 +635,54 @@ def fvm_laplacian(g: Grid, gamma, fld: Field, corrected: bool = True, harmonic: 
     return m
 
 
+# schemes/div.py:145-196 -- TVD limiters psi(r)
+TVD_LIMITERS = {
+    "vanleer": lambda r: (r + np.abs(r)) / (1.0 + np.abs(r)),
+    "minmod": lambda r: np.maximum(np.minimum(1.0, r), 0.0),
+    "superbee": lambda r: np.maximum(np.maximum(np.minimum(2.0 * r, 1.0), np.minimum(r, 2.0)), 0.0),
+    "monotonized_central": lambda r: np.maximum(np.minimum(np.minimum(2.0 * r, 0.5 * r + 0.5), 2.0), 0.0),
+    "limitedlinear": lambda r: np.maximum(np.minimum(2.0 * r, 1.0), 0.0),
+    "koren": lambda r: np.maximum(np.minimum(np.minimum(2.0 * r, (2.0 + r) / 3.0), 2.0), 0.0),
+}
+
+
+def tvd_r(g: Grid, fld: Field, grad: np.ndarray, upwind_owner: np.ndarray) -> np.ndarray:
+    """schemes/div.py:199-340: r = 2 (grad_U . d) / (psi_N - psi_O) - 1 (for
+    vectors the component sums of (psi_N - psi_O) . (grad_U d) over
+    |psi_N - psi_O|^2), with r = 2000 sign(num) sign(den) - 1 where
+    |num| >= 1000 |den| (steep)."""
+    d = g.cell_centers[g.neighbour] - g.cell_centers[g.owner]
+    gU = np.where(_bcast(upwind_owner, grad[g.owner]), grad[g.owner], grad[g.neighbour])
+    dpsi = fld.data[g.neighbour] - fld.data[g.owner]
+    gd = np.einsum("n...j,nj->n...", gU, d)
+    if fld.cshape == ():
+        den, num = dpsi, gd
+    else:
+        den, num = np.sum(dpsi * dpsi, axis=1), np.sum(dpsi * gd, axis=1)
+    steep = np.abs(num) >= 1000.0 * np.abs(den)
+    safe = np.where(steep, 1.0, den)
+    return np.where(steep, 2.0 * 1000.0 * np.sign(num) * np.sign(den) - 1.0, 2.0 * num / safe - 1.0)
+
+
 def fvm_div(g: Grid, phi: FaceValues, fld: Field, scheme: str) -> Matrix:
-    """fv/fvm/div.py:15-73 with schemes/div.py:36-139 (upwind, linear)."""
+    """fv/fvm/div.py:15-73 with schemes/div.py:36-461 (upwind, linear and the
+    TVD schemes: upwind matrix plus the deferred-correction source
+    -phi psi(r) (psi_linear - psi_upwind) per face)."""
     geo = g.geo
     m = Matrix(g, fld.cshape)
     ph = phi.single
-    if scheme == "upwind":
+    if scheme == "upwind" or scheme in TVD_LIMITERS:
         pos, neg = np.maximum(ph, 0.0), np.minimum(ph, 0.0)
         upper, lower, dO, dN = neg, -pos, pos, -neg
         sface = np.zeros((g.num_faces,) + fld.cshape)
+        if scheme in TVD_LIMITERS:
+            grad = lsq_grad(g, fld)
+            base = linear_face_values(g, fld.data)
+            lin = correct_face_values(g, base, grad) if len(geo.hang_idx) else base
+            up_own = ph > 0
+            psi_up = np.where(_bcast(up_own, base), fld.data[g.owner], fld.data[g.neighbour])
+            lim = TVD_LIMITERS[scheme](tvd_r(g, fld, grad, up_own))
+            sface = -_bcast(ph * lim, lin) * (lin - psi_up)
     elif scheme == "linear":
         w = geo.w
         dO, upper = ph * w, ph * (1.0 - w)

@@ -147,9 +147,11 @@ def test_lsq_gradient_and_interpolation(name):
 
 
 @pytest.mark.parametrize("name", ["refined2d", "refined3d", "walls8"])
-@pytest.mark.parametrize("scheme", ["upwind", "linear"])
+@pytest.mark.parametrize("scheme", ["upwind", "linear", "vanleer", "minmod", "superbee", "monotonized_central",
+                                    "limitedlinear", "koren"])
 def test_momentum_assembly(name, scheme):
-    """ddt + div(phi, U) - laplacian(nu, U) and -laplacian(gamma, p)."""
+    """ddt + div(phi, U) - laplacian(nu, U) and -laplacian(gamma, p), with
+    upwind, linear and the TVD schemes (schemes/div.py:36-461)."""
     from gridfoam_amd.config import DivScheme
     from gridfoam_amd.fields import FaceField
     from gridfoam_amd.fv import fvm
