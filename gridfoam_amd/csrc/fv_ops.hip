@@ -167,6 +167,40 @@ __global__ void k_sn_grad_hang(const gf_mesh m, const double *grad, double *corr
     for (int q = 0; q < NC; ++q) corr[h * NC + q] = hang_sn_corr<NC>(m, grad, h, q);
 }
 
+// Green-Gauss cell gradient (fv/kernels/gauss_gradient.py:15-73): the sum
+// over a cell's faces of psi_f Sf (owner +, neighbour -, boundary +) over
+// its volume, gathered per cell (no atomics).  Sf lies along the face axis.
+template <int NC>
+__global__ void k_gauss_grad(const gf_mesh m, const double *psi_s, const double *psi_b, double *grad) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.n_cells) return;
+    double acc[NC][3] = {};
+    for (int f = m.own_row[c]; f < m.own_row[c + 1]; ++f) {
+        const int a = m.axis[f];
+        const double sf = m.mag_sf[f];
+#pragma unroll
+        for (int q = 0; q < NC; ++q) acc[q][a] += psi_s[f * NC + q] * sf;
+    }
+    for (int k = m.nbr_row[c]; k < m.nbr_row[c + 1]; ++k) {
+        const int f = m.nbr_face[k];
+        const int a = m.axis[f];
+        const double sf = m.mag_sf[f];
+#pragma unroll
+        for (int q = 0; q < NC; ++q) acc[q][a] -= psi_s[f * NC + q] * sf;
+    }
+    for (int k = m.bnd_row[c]; k < m.bnd_row[c + 1]; ++k) {
+        const int a = m.bnd_dir[k] / 2;
+        const double sf = m.bnd_sign[k] * m.bnd_mag_sf[k];
+#pragma unroll
+        for (int q = 0; q < NC; ++q) acc[q][a] += psi_b[k * NC + q] * sf;
+    }
+    const double V = m.volumes[c];
+#pragma unroll
+    for (int q = 0; q < NC; ++q)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) grad[(c * NC + q) * 3 + j] = acc[q][j] / V;
+}
+
 // ---------------------------------------------------------------- divergence
 template <int NC>
 __global__ void k_fvc_div(const gf_mesh m, const double *phi_s, const double *phi_b, double *out) {
@@ -1063,6 +1097,15 @@ extern "C" int gf_continuity_error(const gf_mesh *m, const double *phi_s, const 
     hipStream_t s = S(stream);
     GF_NC_DISPATCH(1, k_fvc_div, grid_for(m->n_cells), *m, phi_s, phi_b, div_scratch);
     hipLaunchKernelGGL(k_continuity, dim3(1), dim3(1024), 0, s, m->n_cells, div_scratch, m->volumes, dt, out);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
+extern "C" int gf_gauss_grad(const gf_mesh *m, int32_t nc, const double *psi_s, const double *psi_b, double *grad,
+                             gf_stream_t stream) {
+    GF_REQUIRE(m && psi_s && psi_b && grad, "null argument");
+    hipStream_t s = S(stream);
+    GF_NC_DISPATCH(nc, k_gauss_grad, grid_for(m->n_cells), *m, psi_s, psi_b, grad);
     GF_CHECK_LAUNCH();
     return GF_OK;
 }

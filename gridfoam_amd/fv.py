@@ -78,11 +78,10 @@ def laplacian_corrected(grid, field_name: str) -> bool:
 
 
 def grad_scheme(grid, field_name: str) -> GradScheme:
+    """schemes/selection.py: the cell gradient of grad(field), default least squares."""
     s = _lookup(grid.sim_config.fvSchemes.gradSchemes if grid.sim_config else None,
                 f"grad({field_name})", GradScheme.LEASTSQUARE)
-    if s != GradScheme.LEASTSQUARE:
-        raise ValueError(f"grad scheme {s} is not supported by this build")
-    return s
+    return GradScheme(s)
 
 
 def ddt_scheme(grid, field_name: str) -> DdtScheme:
@@ -108,13 +107,22 @@ def ddt_coefficients(field: CellField) -> Tuple[float, float, float]:
 class fvc:
     @staticmethod
     def grad(field: CellField, *, hang_only: bool = False, out: Optional[torch.Tensor] = None) -> torch.Tensor:
-        """least_squares.py:29-101 -> [C, *cs, 3] (hang_only: valid on hang cells)."""
+        """The configured cell gradient [C, *cs, 3] (schemes/grad.py:82-101):
+        least squares (least_squares.py:29-101) or Green-Gauss from the
+        linear face values (grad.py:21-44).  hang_only: the least-squares
+        gradient valid on the cells next to hanging faces, whatever the
+        configuration (the hanging-face corrections always use it)."""
         g = field.grid
-        grad_scheme(g, field.name)
-        st = field.boundary_state()
+        scheme = grad_scheme(g, field.name)
         shape = (g.num_cells, *field.component_shape, 3)
         if out is None:
             out = torch.zeros(shape, dtype=torch.float64, device=g.device)
+        if not hang_only and scheme == GradScheme.LINEAR:
+            fi = fvc.interpolate(field)
+            nat.call("gf_gauss_grad", g.mesh_ref, ctypes.c_int32(field.nc), nat.ptr(fi.single_buffer),
+                     nat.ptr(fi.bnd_buffer), nat.ptr(out), nat.stream())
+            return out
+        st = field.boundary_state()
         nat.call("gf_lsq_grad", g.mesh_ref, ctypes.c_int32(field.nc), nat.ptr(field.data), nat.ptr(st.psi_b),
                  nat.ptr(st.covered), ctypes.c_int32(int(hang_only)), nat.ptr(out), nat.stream())
         return out

@@ -121,6 +121,11 @@ int gf_bc_slip(const gf_mesh *m, const double *psi, const uint8_t *slip_mask, do
 /* least_squares.py:29-101; hang_only evaluates only cells next to hanging faces */
 int gf_lsq_grad(const gf_mesh *m, int32_t nc, const double *psi, const double *psi_b,
                 const uint8_t *covered, int32_t hang_only, double *grad, gf_stream_t stream);
+/* Green-Gauss gradient (schemes/grad.py:21-44, gauss_gradient.py:15-73) from
+ * face values psi_s [F,nc] and psi_b [B,nc] (gf_interpolate + boundary state):
+ * grad [C,nc,3] = sum psi_f Sf / V */
+int gf_gauss_grad(const gf_mesh *m, int32_t nc, const double *psi_s, const double *psi_b,
+                  double *grad, gf_stream_t stream);
 /* interpolate.py:20-63 + face_interpolation.py:114-197; grad valid on hang cells */
 int gf_interpolate(const gf_mesh *m, int32_t nc, const double *psi, const double *grad,
                    double *psi_f, gf_stream_t stream);

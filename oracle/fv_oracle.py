@@ -464,6 +464,28 @@ def lsq_grad(g: Grid, fld: Field) -> np.ndarray:
     return np.einsum("nij,n...j->n...i", g.geo.lsq_ata_inv, atb)
 
 
+def gauss_grad(g: Grid, fld: Field) -> np.ndarray:
+    """fv/schemes/grad.py:21-44 + fv/kernels/gauss_gradient.py:15-73:
+    Green-Gauss, sum over faces of psi_f Sf / V, with the linear face values
+    (hanging faces corrected by the least-squares gradient) and the
+    boundary condition's face values (zero gradient where none)."""
+    fv = interpolate(g, fld)
+    flux = fv.single[..., None] * g.Sf.reshape((g.num_faces,) + (1,) * len(fld.cshape) + (3,))
+    bflux = fv.bnd[..., None] * g.bnd_Sf.reshape((g.num_bnd,) + (1,) * len(fld.cshape) + (3,))
+    out = _scatter(g.owner, flux, g.num_cells) - _scatter(g.neighbour, flux, g.num_cells)
+    out = out + _scatter(g.bnd_owner, bflux, g.num_cells)
+    return out / g.cell_volumes.reshape((g.num_cells,) + (1,) * (out.ndim - 1))
+
+
+def cell_grad(g: Grid, fld: Field, scheme: str = "leastsquare") -> np.ndarray:
+    """fv/schemes/grad.py:82-101: the configured cell gradient."""
+    if scheme == "linear":
+        return gauss_grad(g, fld)
+    if scheme != "leastsquare":
+        raise ValueError(f"unsupported grad scheme {scheme}")
+    return lsq_grad(g, fld)
+
+
 def linear_face_values(g: Grid, psi: np.ndarray) -> np.ndarray:
     """face_interpolation.py:114-138."""
     w = _bcast(g.geo.w, psi[g.owner])
@@ -664,7 +686,7 @@ def tvd_r(g: Grid, fld: Field, grad: np.ndarray, upwind_owner: np.ndarray) -> np
     return np.where(steep, 2.0 * 1000.0 * np.sign(num) * np.sign(den) - 1.0, 2.0 * num / safe - 1.0)
 
 
-def fvm_div(g: Grid, phi: FaceValues, fld: Field, scheme: str) -> Matrix:
+def fvm_div(g: Grid, phi: FaceValues, fld: Field, scheme: str, grad_scheme: str = "leastsquare") -> Matrix:
     """fv/fvm/div.py:15-73 with schemes/div.py:36-461 (upwind, linear and the
     TVD schemes: upwind matrix plus the deferred-correction source
     -phi psi(r) (psi_linear - psi_upwind) per face)."""
@@ -676,7 +698,7 @@ def fvm_div(g: Grid, phi: FaceValues, fld: Field, scheme: str) -> Matrix:
         upper, lower, dO, dN = neg, -pos, pos, -neg
         sface = np.zeros((g.num_faces,) + fld.cshape)
         if scheme in TVD_LIMITERS:
-            grad = lsq_grad(g, fld)
+            grad = cell_grad(g, fld, grad_scheme)  # also corrects the hanging faces' linear values
             base = linear_face_values(g, fld.data)
             lin = correct_face_values(g, base, grad) if len(geo.hang_idx) else base
             up_own = ph > 0
@@ -952,7 +974,7 @@ def simplec_rAtU(UEqn: Matrix, rAU: np.ndarray, bounded: bool = True) -> np.ndar
 
 
 def apply_simplec(g: Grid, phi_h: FaceValues, HbyA: np.ndarray, p: Field, rAU: np.ndarray, rAtU: np.ndarray,
-                  corrected: bool = True, publish=None):
+                  corrected: bool = True, publish=None, grad_scheme: str = "leastsquare"):
     """pressure_correction.py:222-267: phiHbyA += (rAtU-rAU)_f |Sf| snGrad(p)
     (boundary faces: the condition's normal gradient); HbyA -= (rAU-rAtU) grad p.
     The reference updates the registered phiHbyA in place before grad(p), so
@@ -969,7 +991,7 @@ def apply_simplec(g: Grid, phi_h: FaceValues, HbyA: np.ndarray, p: Field, rAU: n
         out.bnd[b.mask] = out.bnd[b.mask] + d_r[b.cells] * b.mag_Sf * sng
     if publish is not None:
         publish(out)
-    return out, HbyA - (rAU - rAtU)[:, None] * lsq_grad(g, p)
+    return out, HbyA - (rAU - rAtU)[:, None] * cell_grad(g, p, grad_scheme)
 
 
 def _residual_control(rc: dict) -> dict:
@@ -1051,6 +1073,8 @@ class Case:
             return d.get(key, d.get("default", default))
 
         self.div_scheme = look("divSchemes", "div(phi, U)", "upwind")
+        self.grad_p = look("gradSchemes", "grad(p)", "leastsquare")
+        self.grad_U = look("gradSchemes", "grad(U)", "leastsquare")
         lap_u = look("laplacianSchemes", "laplacian(U)", "corrected")
         lap_p = look("laplacianSchemes", "laplacian(p)", "corrected")
         uncorr = ("uncorrected", "Gauss linear uncorrected", "Gauss harmonic uncorrected")
@@ -1086,7 +1110,8 @@ class Case:
 
     def _momentum(self, transient: bool):
         g, U = self.g, self.U
-        m = fvm_div(g, self.phi, U, self.div_scheme) - fvm_laplacian(g, self.nu, U, self.lap_u_corr, self.lap_u_harm)
+        m = (fvm_div(g, self.phi, U, self.div_scheme, self.grad_U)
+             - fvm_laplacian(g, self.nu, U, self.lap_u_corr, self.lap_u_harm))
         if transient:
             abc = ddt_coefficients(self.ddt_scheme, U.older is not None, g.dt, self.prev_dt)
             m = fvm_ddt(g, U, abc) + m
@@ -1114,7 +1139,7 @@ class Case:
 
             def publish(out):
                 self.phi_h = out
-            phi_h, HbyA_d = apply_simplec(g, phi_h, HbyA.data, p, rAU, rAtU, self.sn_p_corr, publish)
+            phi_h, HbyA_d = apply_simplec(g, phi_h, HbyA.data, p, rAU, rAtU, self.sn_p_corr, publish, self.grad_p)
             HbyA = Field("HbyA", HbyA_d)
             self.HbyA_d = HbyA_d
             rAU = rAtU
@@ -1143,7 +1168,7 @@ class Case:
         A_old = UEqn.diag.copy()
         UEqn.diag = A_old / aU
         UEqn.source = UEqn.source + (((1.0 - aU) / aU) * A_old)[:, None] * U.data
-        gp = lsq_grad(g, p)
+        gp = cell_grad(g, p, self.grad_p)
         pred = UEqn.copy()
         pred.source = UEqn.source - g.cell_volumes[:, None] * gp
         if "U" in self.rc:  # simple.py:319-323
@@ -1155,7 +1180,7 @@ class Case:
             self._simple_record("p", self.stats["p_initial_residual"])
         self.phi = correct_phi(g, phi_h, pmat, p, rAU)
         p.data = p_old + ap * (p.data - p_old)
-        U.data = HbyA.data - rAU[:, None] * lsq_grad(g, p)
+        U.data = HbyA.data - rAU[:, None] * cell_grad(g, p, self.grad_p)
         if "phi" in self.rc:  # simple.py:394-395
             self._simple_record("phi", float(np.linalg.norm(fvc_div(g, self.phi))))
 
@@ -1180,7 +1205,7 @@ class Case:
         g, U, p = self.g, self.U, self.p
         abc = ddt_coefficients(self.ddt_scheme, U.older is not None, g.dt, self.prev_dt)
         UEqn = self._momentum(True)
-        gp = lsq_grad(g, p)
+        gp = cell_grad(g, p, self.grad_p)
         pred = UEqn.copy()
         pred.source = UEqn.source - g.cell_volumes[:, None] * gp
         U.data, self.stats["U"] = solve(pred, U.data, self.solvers["U"])
@@ -1188,7 +1213,7 @@ class Case:
         for i in range(n_corr):
             HbyA, rAU, phi_h, pmat = self._pressure_pass(UEqn, True, i == n_corr - 1, abc)
             self.phi = correct_phi(g, phi_h, pmat, p, rAU)
-            U.data = HbyA.data - rAU[:, None] * lsq_grad(g, p)
+            U.data = HbyA.data - rAU[:, None] * cell_grad(g, p, self.grad_p)
         # field.py:241-255 -- history advance once per step
         U.older, U.old = U.old, U.data.copy()
         self.phi_older, self.phi_old = self.phi_old, self.phi.single.copy()
@@ -1225,7 +1250,7 @@ class Case:
                 self.outer_converged = True
             UEqn = self._momentum(True)
             pred = UEqn.copy()
-            pred.source = UEqn.source - g.cell_volumes[:, None] * lsq_grad(g, p)
+            pred.source = UEqn.source - g.cell_volumes[:, None] * cell_grad(g, p, self.grad_p)
             if "U" in self.rc:
                 record("U", initial_residual(pred, U.data))
             U.data, self.stats["U"] = solve(pred, U.data, self.solvers["U"])
@@ -1236,7 +1261,7 @@ class Case:
                         record("p", self.stats["p_initial_residual"])
                     cur["p"] = self.stats["p_last_initial_residual"]
                 self.phi = correct_phi(g, phi_h, pmat, p, rAtU)
-                U.data = HbyA.data - rAtU[:, None] * lsq_grad(g, p)
+                U.data = HbyA.data - rAtU[:, None] * cell_grad(g, p, self.grad_p)
             if "phi" in self.rc:
                 record("phi", float(np.linalg.norm(fvc_div(g, self.phi))))
             self.stats["outer"] = outer + 1

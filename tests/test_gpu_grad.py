@@ -1,0 +1,115 @@
+"""GPU: the configured cell gradient -- Green-Gauss from the linear face
+values (fv/schemes/grad.py:21-44, gauss_gradient.py:15-73) beside least
+squares -- against the FP64 oracle, the reference's own gradient tests
+(tests/integration/fv/test_grad_schemes.py: linear exactness with analytic
+normal derivatives, constant fields, the vector layout) and SIMPLE / PISO
+runs configured with the Green-Gauss gradient."""
+
+from __future__ import annotations
+
+import copy
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import fv_oracle as fo
+from tests.cases import FLUXEL, fluxel_block, small_cavity2d, small_cavity3d
+
+pytestmark = pytest.mark.gpu
+
+G = np.array([2.0, -3.0, 5.0])
+J = np.array([[1.0, 2.0, -3.0], [4.0, -2.0, 1.0], [0.5, 3.0, 2.0]])
+
+
+def _say(msg: str) -> None:
+    print(f"[gpu-case] {msg}", file=sys.stderr, flush=True)
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _rel(a, b) -> float:
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _grids(name, scheme):
+    from gridfoam_amd.config import load_config
+    from gridfoam_amd.grid import create_grid
+
+    cfg = copy.deepcopy(small_cavity2d(8))
+    cfg["fluxel"] = {**cfg["fluxel"], **fluxel_block(name)}
+    cfg["simulator"]["fvSchemes"]["gradSchemes"] = {"default": scheme}
+    g = create_grid(load_config(cfg))
+    f = FLUXEL[name]
+    return g, fo.Grid(f["lower"], f["upper"], f["root"], f["regions"], f["n_leaf"])
+
+
+def _normal(col):
+    out = {}
+    for k, p in enumerate(fo.PATCHES):
+        axis, sign = k // 2, 2.0 * (k % 2) - 1.0
+        out[p] = sign * col(axis)
+    return out
+
+
+@pytest.mark.parametrize("name", ["refined2d", "refined3d", "walls8", "odd"])
+@pytest.mark.parametrize("scheme", ["linear", "leastsquare"])
+def test_cell_gradient_matches_oracle_and_is_exact_for_linear_fields(name, scheme):
+    from gridfoam_amd.fields import CellField, DirichletBC, NeumannBC
+    from gridfoam_amd.fv import fvc
+
+    g, o = _grids(name, scheme)
+    _say(f"cell gradient {scheme} {name}")
+    rng = np.random.default_rng(2)
+    q = rng.standard_normal(g.num_cells)
+    Q = CellField(g, "q", ())
+    Q.data = torch.tensor(q, device=g.device)
+    Q.add_boundary_conditions({"x_minus": DirichletBC(0.5), "x_plus": NeumannBC(0.2), "y_plus": DirichletBC(-1.0)})
+    oQ = fo.Field("q", q.copy(), {"x_minus": fo.BC("dirichlet", 0.5), "x_plus": fo.BC("neumann", 0.2),
+                                  "y_plus": fo.BC("dirichlet", -1.0)})
+    assert _rel(_np(fvc.grad(Q)), fo.cell_grad(o, oQ, scheme)) < 1e-13
+    # linear scalar and vector fields with the analytic normal derivative
+    L = CellField(g, "lin", ())
+    L.data = g.cell_centers @ torch.tensor(G, device=g.device) + 7.0
+    L.add_boundary_conditions({p: NeumannBC(v) for p, v in _normal(lambda a: float(G[a])).items()})
+    gl = _np(fvc.grad(L))
+    np.testing.assert_allclose(gl, np.broadcast_to(G, gl.shape), rtol=0, atol=1e-11)
+    U = CellField(g, "Ulin", (3,))
+    U.data = g.cell_centers @ torch.tensor(J.T, device=g.device)
+    U.add_boundary_conditions({p: NeumannBC(v.tolist()) for p, v in _normal(lambda a: J[:, a]).items()})
+    gu = _np(fvc.grad(U))
+    assert gu.shape == (g.num_cells, 3, 3)
+    np.testing.assert_allclose(gu, np.broadcast_to(J, gu.shape), rtol=0, atol=1e-11)
+
+
+def test_constant_field_has_zero_gauss_gradient():
+    from gridfoam_amd.fields import CellField, NeumannBC
+    from gridfoam_amd.fv import fvc
+
+    g, _ = _grids("refined2d", "linear")
+    _say("gauss gradient of a constant")
+    Q = CellField(g, "const", ())
+    Q.data = torch.full((g.num_cells,), 3.0, dtype=torch.float64, device=g.device)
+    Q.add_boundary_conditions({p: NeumannBC(0.0) for p in fo.PATCHES})
+    torch.testing.assert_close(fvc.grad(Q), torch.zeros(g.num_cells, 3, dtype=torch.float64, device=g.device),
+                               atol=1e-12, rtol=0)
+
+
+def test_simple_and_piso_with_gauss_gradient_match_oracle():
+    from gridfoam_amd.algorithms import create_case
+    from gridfoam_amd.config import load_config
+
+    for cfg, steps, label in ((small_cavity2d(16), 15, "simple cavity2d 16"),
+                              (small_cavity3d(root=8, steps=3), 3, "piso cavity 8^3 octree")):
+        cfg["simulator"]["fvSchemes"]["gradSchemes"] = {"default": "linear"}
+        _say(f"{label} gauss gradient")
+        grid, alg = create_case(load_config(cfg))
+        case = fo.Case(cfg)
+        for _ in range(steps):
+            alg.step()
+            case.step()
+        assert _rel(_np(alg.U.data), case.U.data) < 1e-10, label
+        assert _rel(_np(alg.p.data), case.p.data) < 1e-10, label
