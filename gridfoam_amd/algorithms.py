@@ -19,7 +19,7 @@ from typing import Dict, Optional, Tuple
 import torch
 
 from . import _native as nat
-from .config import PIMPLEAlgorithm, PISOAlgorithm, SIMPLEAlgorithm
+from .config import PIMPLEAlgorithm, PISOAlgorithm, PrecisionType, SIMPLEAlgorithm
 from .fields import CellField, DirichletBC, FaceField
 from .fv import ddt_coefficients, fvc, fvm, sn_grad_corrected
 from .fvmatrix import Equation, FvMatrix
@@ -496,6 +496,11 @@ class PIMPLE(AlgorithmBase):
 
 def create_algorithm(grid: OctreeGrid):
     """algorithms/factory.py:9-38."""
+    if grid.sim_config.control.precision != PrecisionType.FLOAT64:
+        # the fields, assembly and true residuals are FP64 here; an FP32 inner
+        # pressure solve is the solver's mixed_precision option
+        raise ValueError("control.precision float32 is not supported: fields and assembly run in FP64 "
+                         "(use mixed_precision on the pressure solver for FP32 inner solves)")
     alg = grid.sim_config.fvSolution.algorithm
     if isinstance(alg, SIMPLEAlgorithm):
         return SIMPLE(grid)

@@ -201,6 +201,33 @@ __global__ void k_gauss_grad(const gf_mesh m, const double *psi_s, const double 
         for (int j = 0; j < 3; ++j) grad[(c * NC + q) * 3 + j] = acc[q][j] / V;
 }
 
+// fvc.reconstruct (fvc/reconstruct.py:27-107): U_P = (sum |Sf| n n^T)^-1
+// sum n phi_f over all faces of the cell, both sides of an internal face
+// with the face's own normal; diagonal for axis-aligned faces.
+__global__ void k_reconstruct(const gf_mesh m, const double *phi_s, const double *phi_b, double *U) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.n_cells) return;
+    double num[3] = {}, den[3] = {};
+    for (int f = m.own_row[c]; f < m.own_row[c + 1]; ++f) {
+        const int a = m.axis[f];
+        num[a] += phi_s[f];
+        den[a] += m.mag_sf[f];
+    }
+    for (int k = m.nbr_row[c]; k < m.nbr_row[c + 1]; ++k) {
+        const int f = m.nbr_face[k];
+        const int a = m.axis[f];
+        num[a] += phi_s[f];
+        den[a] += m.mag_sf[f];
+    }
+    for (int k = m.bnd_row[c]; k < m.bnd_row[c + 1]; ++k) {
+        const int a = m.bnd_dir[k] / 2;
+        num[a] += m.bnd_sign[k] * phi_b[k];
+        den[a] += m.bnd_mag_sf[k];
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) U[3 * c + j] = num[j] / den[j];
+}
+
 // ---------------------------------------------------------------- divergence
 template <int NC>
 __global__ void k_fvc_div(const gf_mesh m, const double *phi_s, const double *phi_b, double *out) {
@@ -1097,6 +1124,14 @@ extern "C" int gf_continuity_error(const gf_mesh *m, const double *phi_s, const 
     hipStream_t s = S(stream);
     GF_NC_DISPATCH(1, k_fvc_div, grid_for(m->n_cells), *m, phi_s, phi_b, div_scratch);
     hipLaunchKernelGGL(k_continuity, dim3(1), dim3(1024), 0, s, m->n_cells, div_scratch, m->volumes, dt, out);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
+extern "C" int gf_reconstruct(const gf_mesh *m, const double *phi_s, const double *phi_b, double *U,
+                              gf_stream_t stream) {
+    GF_REQUIRE(m && phi_s && phi_b && U, "null argument");
+    hipLaunchKernelGGL(k_reconstruct, dim3(grid_for(m->n_cells)), dim3(kBlock), 0, S(stream), *m, phi_s, phi_b, U);
     GF_CHECK_LAUNCH();
     return GF_OK;
 }

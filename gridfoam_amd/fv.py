@@ -150,6 +150,24 @@ class fvc:
         return out
 
     @staticmethod
+    def reconstruct(phi: FaceField, U: CellField) -> torch.Tensor:
+        """fvc/reconstruct.py:27-107: the cell velocity of a face flux (per
+        axis, the area-weighted average of the face-normal velocities); U
+        receives the data and it is returned."""
+        if phi.component_shape != ():
+            raise ValueError("phi must be a scalar face flux field.")
+        if U.component_shape != (3,):
+            raise ValueError("U must be a 3-component velocity field.")
+        if phi.grid is not U.grid:
+            raise ValueError("phi and U must share the same grid.")
+        g = phi.grid
+        out = torch.empty((g.num_cells, 3), dtype=torch.float64, device=g.device)
+        nat.call("gf_reconstruct", g.mesh_ref, nat.ptr(phi.single_buffer), nat.ptr(phi.bnd_buffer), nat.ptr(out),
+                 nat.stream())
+        U.data = out
+        return out
+
+    @staticmethod
     def sn_grad(field: CellField) -> torch.Tensor:
         """fv/schemes/sn_grad.py:109-181 (corrected)."""
         g = field.grid

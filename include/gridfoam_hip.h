@@ -126,6 +126,10 @@ int gf_lsq_grad(const gf_mesh *m, int32_t nc, const double *psi, const double *p
  * grad [C,nc,3] = sum psi_f Sf / V */
 int gf_gauss_grad(const gf_mesh *m, int32_t nc, const double *psi_s, const double *psi_b,
                   double *grad, gf_stream_t stream);
+/* fvc/reconstruct.py:27-107: U [C,3] from the face flux phi_s [F], phi_b [B]:
+ * per axis, the area-weighted average of the face-normal velocities */
+int gf_reconstruct(const gf_mesh *m, const double *phi_s, const double *phi_b, double *U,
+                   gf_stream_t stream);
 /* interpolate.py:20-63 + face_interpolation.py:114-197; grad valid on hang cells */
 int gf_interpolate(const gf_mesh *m, int32_t nc, const double *psi, const double *grad,
                    double *psi_f, gf_stream_t stream);

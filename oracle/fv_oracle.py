@@ -556,6 +556,23 @@ def fvc_div(g: Grid, phi: FaceValues) -> np.ndarray:
     return d / _bcast(g.cell_volumes, d)
 
 
+def reconstruct(g: Grid, phi: FaceValues) -> np.ndarray:
+    """fv/fvc/reconstruct.py:10-107: U_P = (sum_f |Sf| n n^T)^-1 sum_f n phi_f
+    over every face of the cell (owner and neighbour with the same sign);
+    the tensor is diagonal on axis-aligned faces."""
+    def acc(cells, Sf, flux):
+        mag = np.sqrt(np.sum(Sf * Sf, axis=1))
+        n = Sf / mag[:, None]
+        return _scatter(cells, flux[:, None] * n, g.num_cells), _scatter(cells, mag[:, None] * n * n, g.num_cells)
+    num = np.zeros((g.num_cells, 3))
+    den = np.zeros((g.num_cells, 3))
+    for cells, Sf, flux in ((g.owner, g.Sf, phi.single), (g.neighbour, g.Sf, phi.single),
+                            (g.bnd_owner, g.bnd_Sf, phi.bnd)):
+        a, b = acc(cells, Sf, flux)
+        num, den = num + a, den + b
+    return num / den
+
+
 def covered_domain(g: Grid, fld: Field) -> np.ndarray:
     cov = np.zeros(g.num_bnd, bool)
     for b in batches(g, fld):

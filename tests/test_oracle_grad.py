@@ -62,3 +62,17 @@ def test_constant_field_has_zero_gauss_gradient():
     g = _grid("refined2d")
     q = fo.Field("q", np.full(g.num_cells, 3.0), {p: fo.BC("neumann", 0.0) for p in fo.PATCHES})
     np.testing.assert_allclose(fo.gauss_grad(g, q), 0.0, atol=1e-12)
+
+
+@pytest.mark.parametrize("name", ["refined2d", "refined3d", "odd"])
+def test_reconstruct_recovers_uniform_and_linear_velocity(name):
+    """tests/integration/fv/test_reconstruct.py:30-69: a uniform velocity and
+    U = (x, 2y, -3z) (face fluxes from face-centre values) are recovered in
+    every cell, hanging sub-faces included."""
+    g = _grid(name)
+    u = np.array([1.0, 0.5, -0.25])
+    phi = fo.FaceValues(g.Sf @ u, g.bnd_Sf @ u)
+    np.testing.assert_allclose(fo.reconstruct(g, phi), np.broadcast_to(u, (g.num_cells, 3)), rtol=1e-12, atol=1e-12)
+    s = np.array([1.0, 2.0, -3.0])
+    phi = fo.FaceValues(np.sum(g.face_centers * s * g.Sf, axis=1), np.sum(g.bnd_face_centers * s * g.bnd_Sf, axis=1))
+    np.testing.assert_allclose(fo.reconstruct(g, phi), g.cell_centers * s, rtol=1e-12, atol=1e-12)
