@@ -82,7 +82,9 @@ _SIGS: dict[str, tuple[Any, list[Any]]] = {
     "gf_continuity_error": (_i32, [_P(GfMesh), _vp, _vp, _f64, _vp, _vp, _vp]),
     "gf_sn_grad_hang": (_i32, [_P(GfMesh), _i32, _vp, _vp, _vp]),
     "gf_assemble": (_i32, [_P(GfMesh), _P(GfAssembly), _vp, _vp, _vp, _vp, _vp, _vp]),
-    "gf_assemble_vjp": (_i32, [_P(GfMesh), _P(GfAssembly)] + [_vp] * 14),
+    "gf_assemble_vjp": (_i32, [_P(GfMesh), _P(GfAssembly)] + [_vp] * 15),
+    "gf_lsq_grad_vjp": (_i32, [_P(GfMesh), _i32, _vp, _vp, _i32, _vp, _vp, _vp, _vp]),
+    "gf_bc_state_vjp": (_i32, [_P(GfMesh), _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gf_ldu_multiply": (_i32, [_P(GfMesh), _i32, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gf_rau_hbya": (_i32, [_P(GfMesh), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gf_ldu_flux": (_i32, [_P(GfMesh), _i32, _f64, _vp, _vp, _vp, _vp, _vp, _vp]),

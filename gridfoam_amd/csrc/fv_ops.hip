@@ -116,6 +116,124 @@ __global__ void k_lsq_grad(const gf_mesh m, const double *psi, const double *psi
     }
 }
 
+// Reverse mode of k_lsq_grad: with v_c = A_c^T gbar_c (A = ata_inv) on the
+// cells the forward evaluated, face f = (o, n) adds wd_f . (v_o + v_n) to
+// psi_bar_n and subtracts it from psi_bar_o (both cells used wd_f (psi_n -
+// psi_o)); a covered boundary face k of cell c gets psi_b_bar = wd_k . v_c,
+// and c loses the same.  Gathered per cell (no atomics).
+template <int NC>
+__global__ void k_lsq_v(const gf_mesh m, const double *gbar, const int32_t *cell_list, int n_list, double *v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_list) return;
+    const int c = cell_list ? cell_list[i] : i;
+    const double *inv = m.ata_inv + 9 * c;
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+        const double *g = gbar + (c * NC + q) * 3;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) v[(c * NC + q) * 3 + j] = (inv[j] * g[0] + inv[3 + j] * g[1]) + inv[6 + j] * g[2];
+    }
+}
+
+template <int NC>
+__global__ void k_lsq_vjp(const gf_mesh m, const double *v, const uint8_t *covered, double *g_psi, double *g_psi_b) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.n_cells) return;
+    const double *cc = m.centers + 3 * c;
+    double acc[NC] = {};
+    for (int f = m.own_row[c]; f < m.own_row[c + 1]; ++f) {
+        const int n = m.neighbour[f];
+        const double *cn = m.centers + 3 * n;
+        double wd[3];
+        lsq_wd(cn[0] - cc[0], cn[1] - cc[1], cn[2] - cc[2], wd);
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+            const double *vc = v + (c * NC + q) * 3, *vn = v + (n * NC + q) * 3;
+            acc[q] -= (wd[0] * (vc[0] + vn[0]) + wd[1] * (vc[1] + vn[1])) + wd[2] * (vc[2] + vn[2]);
+        }
+    }
+    for (int k = m.nbr_row[c]; k < m.nbr_row[c + 1]; ++k) {
+        const int o = m.owner[m.nbr_face[k]];
+        const double *co = m.centers + 3 * o;
+        double wd[3];
+        lsq_wd(cc[0] - co[0], cc[1] - co[1], cc[2] - co[2], wd);
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+            const double *vc = v + (c * NC + q) * 3, *vo = v + (o * NC + q) * 3;
+            acc[q] += (wd[0] * (vo[0] + vc[0]) + wd[1] * (vo[1] + vc[1])) + wd[2] * (vo[2] + vc[2]);
+        }
+    }
+    for (int k = m.bnd_row[c]; k < m.bnd_row[c + 1]; ++k) {
+        double wd[3];
+        lsq_wd(m.bnd_d[3 * k + 0], m.bnd_d[3 * k + 1], m.bnd_d[3 * k + 2], wd);
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+            const double *vc = v + (c * NC + q) * 3;
+            const double t = covered[k] ? (wd[0] * vc[0] + wd[1] * vc[1]) + wd[2] * vc[2] : 0.0;
+            acc[q] -= t;
+            g_psi_b[k * NC + q] = t;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NC; ++q) g_psi[c * NC + q] = acc[q];
+}
+
+// Reverse mode of k_bc_state (and of k_bc_slip on slip faces), per cell over
+// its boundary faces: psi_b = f ref_v + (1 - f)(psi_P + |d| ref_g) where
+// covered, psi_P elsewhere; slip ref_v = psi_P - (psi_P . n) n.  Adds to
+// g_psi and to the reference-value gradients g_ref_v / g_ref_g (which may
+// hold other consumers' shares); on slip faces the whole ref_v gradient is
+// carried on to psi_P and its entry is left 0.
+template <int NC>
+__global__ void k_bc_state_vjp(const gf_mesh m, const double *frac, const uint8_t *covered, const uint8_t *slip,
+                               const double *g_psi_b, double *g_psi, double *g_ref_v, double *g_ref_g) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.n_cells) return;
+    double acc[NC] = {};
+    for (int k = m.bnd_row[c]; k < m.bnd_row[c + 1]; ++k) {
+        double rv[NC], rg[NC];
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+            rv[q] = g_ref_v ? g_ref_v[k * NC + q] : 0.0;
+            rg[q] = g_ref_g ? g_ref_g[k * NC + q] : 0.0;
+        }
+        if (!covered[k]) {
+#pragma unroll
+            for (int q = 0; q < NC; ++q) acc[q] += g_psi_b[k * NC + q];
+        } else {
+            const double f = frac[k], d = m.bnd_mag_d[k];
+#pragma unroll
+            for (int q = 0; q < NC; ++q) {
+                const double gb = g_psi_b[k * NC + q];
+                acc[q] += (1.0 - f) * gb;
+                rv[q] += f * gb;
+                rg[q] += (1.0 - f) * d * gb;
+            }
+            if constexpr (NC == 3) {
+                if (slip && slip[k]) {  // ref_v is the tangential part of psi_P
+                    const int a = m.bnd_dir[k] / 2;
+                    double n[3] = {0.0, 0.0, 0.0};
+                    n[a] = m.bnd_sign[k];
+                    const double rn = (rv[0] * n[0] + rv[1] * n[1]) + rv[2] * n[2];
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        acc[q] += rv[q] - rn * n[q];
+                        rv[q] = 0.0;
+                    }
+                }
+            }
+        }
+        if (g_ref_v)
+#pragma unroll
+            for (int q = 0; q < NC; ++q) g_ref_v[k * NC + q] = rv[q];
+        if (g_ref_g)
+#pragma unroll
+            for (int q = 0; q < NC; ++q) g_ref_g[k * NC + q] = rg[q];
+    }
+#pragma unroll
+    for (int q = 0; q < NC; ++q) g_psi[c * NC + q] += acc[q];
+}
+
 // ---------------------------------------------------------------- interpolation
 template <int NC>
 __global__ void k_interp_linear(const gf_mesh m, const double *psi, double *psi_f) {
@@ -588,6 +706,49 @@ __global__ void k_asmv_hang(const gf_mesh m, const gf_assembly a, const double *
         }
         work[f] += a.s_lap * m.mag_sf[f] * acc;
     }
+}
+
+// The hanging corrections are linear in the (hang-cell) least-squares
+// gradient: gather d J / d grad per hang cell over its hanging faces
+// (owner side: w, -mo; neighbour side: 1 - w, +mn).  TVD sources are not
+// covered (their ratio depends on the gradient nonlinearly).
+template <int NC>
+__global__ void k_asmv_hang_grad(const gf_mesh m, const gf_assembly a, const double *__restrict__ gs,
+                                 const double *__restrict__ gffc, double *g_grad) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.n_hang_cells) return;
+    const int c = m.hang_cell[i];
+    const bool do_div = a.s_div != 0.0 && a.div_scheme == 1;
+    const bool do_lap = a.s_lap != 0.0 && a.lap_corrected;
+    double acc[NC][3] = {};
+    for (int k = m.hc_row[i]; k < m.hc_row[i + 1]; ++k) {
+        const int code = m.hc_code[k];
+        const bool own = code >= 0;
+        const int h = own ? code : -code - 1;
+        const int f = m.hang_face[h];
+        const int o = m.owner[f], n = m.neighbour[f];
+        const double w = m.w[f];
+        const double *off = m.hang_offset + 3 * h, *mo = m.hang_move_o + 3 * h, *mn = m.hang_move_n + 3 * h;
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+            if (do_div) {
+                const double fac = -a.s_div * a.phi_s[f] * (gs[o * NC + q] - gs[n * NC + q]) * (own ? w : 1.0 - w);
+#pragma unroll
+                for (int j = 0; j < 3; ++j) acc[q][j] += fac * off[j];
+            }
+            if (do_lap) {
+                const double gf = gffc ? gffc[f * NC + q] : 0.0;
+                const double fac = a.s_lap * (gamma_face(a, m, f) * m.mag_sf[f]) * m.delta[f] *
+                                   ((gs[n * NC + q] - gs[o * NC + q]) + gf);
+#pragma unroll
+                for (int j = 0; j < 3; ++j) acc[q][j] += fac * (own ? -mo[j] : mn[j]);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NC; ++q)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) g_grad[(c * NC + q) * 3 + j] = acc[q][j];
 }
 
 // Per cell: old levels, the diffusivity (cell values, or per-cell shares of a
@@ -1074,6 +1235,29 @@ extern "C" int gf_lsq_grad(const gf_mesh *m, int32_t nc, const double *psi, cons
     return GF_OK;
 }
 
+extern "C" int gf_lsq_grad_vjp(const gf_mesh *m, int32_t nc, const double *g_grad, const uint8_t *covered,
+                               int32_t hang_only, double *work, double *g_psi, double *g_psi_b, gf_stream_t stream) {
+    GF_REQUIRE(m && g_grad && covered && work && g_psi && g_psi_b, "null argument");
+    hipStream_t s = S(stream);
+    const int32_t *list = hang_only ? m->hang_cell : nullptr;
+    const int n = hang_only ? m->n_hang_cells : m->n_cells;
+    if (n > 0) GF_NC_DISPATCH(nc, k_lsq_v, grid_for(n), *m, g_grad, list, n, work);  // work arrives zeroed
+    GF_NC_DISPATCH(nc, k_lsq_vjp, grid_for(m->n_cells), *m, work, covered, g_psi, g_psi_b);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
+extern "C" int gf_bc_state_vjp(const gf_mesh *m, int32_t nc, const double *frac, const uint8_t *covered,
+                               const uint8_t *slip_mask, const double *g_psi_b, double *g_psi, double *g_ref_v,
+                               double *g_ref_g, gf_stream_t stream) {
+    GF_REQUIRE(m && frac && covered && g_psi_b && g_psi, "null argument");
+    hipStream_t s = S(stream);
+    GF_NC_DISPATCH(nc, k_bc_state_vjp, grid_for(m->n_cells), *m, frac, covered, slip_mask, g_psi_b, g_psi, g_ref_v,
+                   g_ref_g);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
 extern "C" int gf_interpolate(const gf_mesh *m, int32_t nc, const double *psi, const double *grad, double *psi_f,
                               gf_stream_t stream) {
     GF_REQUIRE(m && psi && psi_f, "null argument");
@@ -1178,7 +1362,8 @@ extern "C" int gf_assemble(const gf_mesh *m, const gf_assembly *a, double *diag,
 extern "C" int gf_assemble_vjp(const gf_mesh *m, const gf_assembly *a, const double *g_diag, const double *g_upper,
                                const double *g_lower, const double *g_source, const double *g_ffc, double *work,
                                double *g_gamma, double *g_phi_s, double *g_phi_b, double *g_psi_old,
-                               double *g_psi_older, double *g_ref_v, double *g_ref_g, gf_stream_t stream) {
+                               double *g_psi_older, double *g_ref_v, double *g_ref_g, double *g_grad,
+                               gf_stream_t stream) {
     GF_REQUIRE(m && a && g_diag && g_upper && g_lower && g_source, "null argument");
     GF_REQUIRE(a->covered && a->frac && a->ref_v && a->ref_g, "boundary state missing");
     GF_REQUIRE(!g_gamma || work, "the diffusivity gradient needs a work array of one double per face");
@@ -1198,6 +1383,10 @@ extern "C" int gf_assemble_vjp(const gf_mesh *m, const gf_assembly *a, const dou
         GF_NC_DISPATCH(a->nc, k_asmv_hang, grid_for(m->n_hang), *m, *a, g_source, g_ffc, wk, g_phi_s);
     GF_NC_DISPATCH(a->nc, k_asmv_cells, grid_for(m->n_cells), *m, *a, g_diag, g_source, wk, g_gamma, g_phi_b,
                    g_psi_old, g_psi_older, g_ref_v, g_ref_g);
+    if (g_grad && m->n_hang_cells > 0) {
+        GF_REQUIRE(!(a->s_div != 0.0 && a->div_scheme >= 2), "the gradient cotangent of TVD sources is not provided");
+        GF_NC_DISPATCH(a->nc, k_asmv_hang_grad, grid_for(m->n_hang_cells), *m, *a, g_source, g_ffc, g_grad);
+    }
     GF_CHECK_LAUNCH();
     return GF_OK;
 }

@@ -104,6 +104,50 @@ def ddt_coefficients(field: CellField) -> Tuple[float, float, float]:
     return a, a + c, c
 
 
+def _grad_pullback(field: CellField, st, g_grad: Optional[torch.Tensor], hang_only: bool, g_rv: torch.Tensor,
+                   g_rg: torch.Tensor) -> torch.Tensor:
+    """Carry a cotangent of the least-squares gradient (or None) and of the
+    boundary reference values back to the field values: gf_lsq_grad_vjp,
+    then gf_bc_state_vjp (which also moves slip faces' ref_v cotangent onto
+    the cell values).  g_rv / g_rg are added to in place; returns d/d psi."""
+    g = field.grid
+    C, B, nc = g.num_cells, g.num_domain_bnd_faces, field.nc
+    f64 = dict(dtype=torch.float64, device=g.device)
+    g_psi = torch.zeros((C, nc), **f64)
+    g_psi_b = torch.zeros((max(B, 1), nc), **f64)
+    if g_grad is not None:
+        work = torch.zeros((C, nc, 3), **f64)
+        nat.call("gf_lsq_grad_vjp", g.mesh_ref, ctypes.c_int32(nc), nat.ptr(g_grad.contiguous()),
+                 nat.ptr(st.covered), ctypes.c_int32(int(hang_only)), nat.ptr(work), nat.ptr(g_psi),
+                 nat.ptr(g_psi_b), nat.stream())
+    nat.call("gf_bc_state_vjp", g.mesh_ref, ctypes.c_int32(nc), nat.ptr(st.frac), nat.ptr(st.covered),
+             nat.ptr(st.slip), nat.ptr(g_psi_b), nat.ptr(g_psi), nat.ptr(g_rv), nat.ptr(g_rg), nat.stream())
+    return g_psi.reshape(C, *field.component_shape)
+
+
+class _LsqGradFn(torch.autograd.Function):
+    """The least-squares gradient as an autograd node (boundary state
+    included): the backward is gf_lsq_grad_vjp + gf_bc_state_vjp."""
+
+    @staticmethod
+    def forward(ctx, field, st, hang_only, psi, ref_v, ref_g):  # noqa: D401
+        ctx.field, ctx.st, ctx.hang_only = field, st, hang_only
+        g = field.grid
+        out = torch.zeros((g.num_cells, *field.component_shape, 3), dtype=torch.float64, device=g.device)
+        nat.call("gf_lsq_grad", g.mesh_ref, ctypes.c_int32(field.nc), nat.ptr(field.data), nat.ptr(st.psi_b),
+                 nat.ptr(st.covered), ctypes.c_int32(int(hang_only)), nat.ptr(out), nat.stream())
+        return out
+
+    @staticmethod
+    def backward(ctx, g_out):
+        st, field = ctx.st, ctx.field
+        g_rv, g_rg = torch.zeros_like(st.ref_v), torch.zeros_like(st.ref_g)
+        g_psi = _grad_pullback(field, st, g_out, ctx.hang_only, g_rv, g_rg)
+        need = ctx.needs_input_grad
+        return (None, None, None, g_psi if need[3] else None, g_rv if need[4] else None,
+                g_rg if need[5] else None)
+
+
 class fvc:
     @staticmethod
     def grad(field: CellField, *, hang_only: bool = False, out: Optional[torch.Tensor] = None) -> torch.Tensor:
@@ -117,12 +161,20 @@ class fvc:
         shape = (g.num_cells, *field.component_shape, 3)
         if out is None:
             out = torch.zeros(shape, dtype=torch.float64, device=g.device)
+        st = field.boundary_state()
+        rv = st.ref_v_ad if st.ref_v_ad is not None else st.ref_v
+        rg = st.ref_g_ad if st.ref_g_ad is not None else st.ref_g
+        track = torch.is_grad_enabled() and (field.data.requires_grad or rv.requires_grad or rg.requires_grad)
         if not hang_only and scheme == GradScheme.LINEAR:
+            if track:
+                raise NotImplementedError("gradients through the Green-Gauss cell gradient are not supported; "
+                                          "use the least-squares scheme")
             fi = fvc.interpolate(field)
             nat.call("gf_gauss_grad", g.mesh_ref, ctypes.c_int32(field.nc), nat.ptr(fi.single_buffer),
                      nat.ptr(fi.bnd_buffer), nat.ptr(out), nat.stream())
             return out
-        st = field.boundary_state()
+        if track:  # a new tensor carrying the graph (``out`` is not written)
+            return _LsqGradFn.apply(field, st, hang_only, field.data, rv, rg)
         nat.call("gf_lsq_grad", g.mesh_ref, ctypes.c_int32(field.nc), nat.ptr(field.data), nat.ptr(st.psi_b),
                  nat.ptr(st.covered), ctypes.c_int32(int(hang_only)), nat.ptr(out), nat.stream())
         return out
@@ -246,12 +298,14 @@ class _Assembly:
             self.has_ffc = bool(corr and g.num_hanging_faces)
         self.hang_terms = bool((need_grad or self.tvd) and g.num_hanging_faces)
         if self.tvd:  # r needs the gradient of every upwind cell (schemes/div.py:270-275)
-            grad = fvc.grad(field)
+            with torch.no_grad():  # the assembly's backward chains through it itself
+                grad = fvc.grad(field)
             a.grad = nat.ptr(grad)
             self.keep.append(grad)
         elif self.hang_terms:
             if grad is None:
-                grad = fvc.grad(field, hang_only=True)
+                with torch.no_grad():
+                    grad = fvc.grad(field, hang_only=True)
             a.grad = nat.ptr(grad)
             self.keep.append(grad)
         # the field values enter the arrays (hanging corrections, TVD sources)
@@ -279,7 +333,7 @@ class _AssembleFn(torch.autograd.Function):
     """gf_assemble as an autograd node; the backward is gf_assemble_vjp."""
 
     @staticmethod
-    def forward(ctx, spec, gamma, phi_s, phi_b, psi_old, psi_older, ref_v, ref_g):  # noqa: D401
+    def forward(ctx, spec, gamma, phi_s, phi_b, psi_old, psi_older, ref_v, ref_g, psi):  # noqa: D401
         ctx.spec = spec
         ctx.gamma_shape = None if gamma is None else tuple(gamma.shape)
         ctx.like = (phi_s, phi_b, psi_old, psi_older, ref_v, ref_g)
@@ -310,12 +364,24 @@ class _AssembleFn(torch.autograd.Function):
         g_gamma = torch.zeros(C, **f64) if need[1] else None
         outs = [torch.zeros_like(t) if (need[k + 2] and t is not None) else None for k, t in enumerate(ctx.like)]
         g_phi_s, g_phi_b, g_old, g_older, g_rv, g_rg = outs
+        # the field values and the reference values also reach the arrays through
+        # the hang-cell gradient (hanging corrections) and the slip condition
+        need_state = need[6] or need[7] or need[8]
+        st = spec.st
+        g_grad = None
+        if need_state:
+            g_rv = torch.zeros_like(st.ref_v) if g_rv is None else g_rv
+            g_rg = torch.zeros_like(st.ref_g) if g_rg is None else g_rg
+            if spec.hang_terms:
+                g_grad = torch.zeros((C, spec.field.nc, 3), **f64)
         nat.call("gf_assemble_vjp", g.mesh_ref, ctypes.byref(spec.a), nat.ptr(gd), nat.ptr(gu), nat.ptr(gl),
                  nat.ptr(gs), nat.ptr(gf), nat.ptr(work), nat.ptr(g_gamma), nat.ptr(g_phi_s), nat.ptr(g_phi_b),
-                 nat.ptr(g_old), nat.ptr(g_older), nat.ptr(g_rv), nat.ptr(g_rg), nat.stream())
+                 nat.ptr(g_old), nat.ptr(g_older), nat.ptr(g_rv), nat.ptr(g_rg), nat.ptr(g_grad), nat.stream())
+        g_psi = _grad_pullback(spec.field, st, g_grad, True, g_rv, g_rg) if need_state else None
         if g_gamma is not None and ctx.gamma_shape != (C,):
             g_gamma = g_gamma.sum().reshape(ctx.gamma_shape)
-        return None, g_gamma, g_phi_s, g_phi_b, g_old, g_older, g_rv, g_rg
+        return (None, g_gamma, g_phi_s, g_phi_b, g_old, g_older, g_rv if need[6] else None,
+                g_rg if need[7] else None, g_psi if need[8] else None)
 
 
 class fvm:
@@ -331,14 +397,19 @@ class fvm:
         respect to the diffusivity (a [C] tensor or a 0-d tensor), the face
         flux, the old and older field levels and the Dirichlet values and
         Neumann gradients of the field's conditions: the backward is the
-        native VJP ``gf_assemble_vjp``.  The current field values enter only
-        through the hanging-face corrections, TVD sources and slip
-        conditions; a field that requires grad there raises, as do boundary
-        values that require grad when such terms are present (they reach
-        the arrays through the cell gradient too) and an explicit ``grad``
-        argument that requires grad."""
+        native VJP ``gf_assemble_vjp``.  The current field values (and the
+        boundary values a second time) enter through the hanging-face
+        corrections, via the hang-cell least-squares gradient, and through
+        slip conditions; both are differentiated (gf_lsq_grad_vjp,
+        gf_bc_state_vjp).  Through TVD sources, whose ratio is nonlinear in
+        them, field and boundary values that require grad raise, as does an
+        explicit ``grad`` argument that requires grad."""
         g = field.grid
         st = field.boundary_state()
+        # the autograd-connected reference values of this call (later
+        # boundary_state() calls, e.g. for the cell gradient, rebuild them)
+        rv = st.ref_v_ad if st.ref_v_ad is not None else st.ref_v
+        rg = st.ref_g_ad if st.ref_g_ad is not None else st.ref_g
         spec = _Assembly(field, st, ddt, div, lap, phi, gamma, scheme, corrected, abc, grad, harmonic)
         mat = FvMatrix.__new__(FvMatrix)
         mat.field, mat.grid, mat.face_flux_correction = field, g, None
@@ -348,25 +419,16 @@ class fvm:
         phi_b = phi.bnd_buffer if (div and phi is not None) else None
         old = field.old_data if ddt else None
         older = field.older_data if spec.older_used else None
-        rv = st.ref_v_ad if st.ref_v_ad is not None else st.ref_v
-        rg = st.ref_g_ad if st.ref_g_ad is not None else st.ref_g
-        ins = (gamma_in, phi_s, phi_b, old, older, rv, rg)
+        ins = (gamma_in, phi_s, phi_b, old, older, rv, rg, field.data)
         if track and any(t is not None and t.requires_grad for t in ins):
-            if field.data.requires_grad and (spec.psi_terms or (field.nc == 3 and st.slip is not None)):
-                raise NotImplementedError("gradients with respect to the field values through hanging-face "
-                                          "corrections, TVD sources or slip conditions are not supported")
+            if spec.tvd and (field.data.requires_grad or rv.requires_grad or rg.requires_grad):
+                # the TVD ratio depends on the field values and its gradient nonlinearly
+                raise NotImplementedError("gradients with respect to the field values or boundary values "
+                                          "through TVD sources are not supported")
             if grad is not None and grad.requires_grad:
                 raise NotImplementedError("gradients through an explicit cell gradient are not supported")
-            if spec.psi_terms and (rv.requires_grad or rg.requires_grad):
-                # the boundary values also enter through the cell gradient of
-                # the hanging corrections / TVD ratios, which is held fixed
-                raise NotImplementedError("gradients of boundary values are not supported with hanging-face "
-                                          "corrections or TVD schemes")
             diag, upper, lower, source, ffc = _AssembleFn.apply(spec, *ins)
         else:
-            if track and field.data.requires_grad and spec.psi_terms:
-                raise NotImplementedError("gradients with respect to the field values through hanging-face "
-                                          "corrections or TVD sources are not supported")
             diag, upper, lower, source, ffc = spec.run()
         mat._diag, mat._upper, mat._lower, mat._source = diag, upper, lower, source
         if spec.has_ffc:

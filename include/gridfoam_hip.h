@@ -192,7 +192,25 @@ int gf_assemble_vjp(const gf_mesh *m, const gf_assembly *a, const double *g_diag
                     const double *g_upper, const double *g_lower, const double *g_source,
                     const double *g_ffc, double *work, double *g_gamma, double *g_phi_s,
                     double *g_phi_b, double *g_psi_old, double *g_psi_older, double *g_ref_v,
-                    double *g_ref_g, gf_stream_t stream);
+                    double *g_ref_g, double *g_grad, gf_stream_t stream);
+/* g_grad [C,nc,3] (may be NULL): d/d of the hang-cell least-squares gradient
+ * that the hanging-face corrections read (written on hang cells only; zero
+ * it first); chained with gf_lsq_grad_vjp and gf_bc_state_vjp it carries the
+ * field values and boundary values through those corrections.  Not for TVD
+ * sources (GF_EINVAL). */
+/* Reverse mode of gf_lsq_grad (least_squares.py:29-101): g_grad [C,nc,3] ->
+ * g_psi [C,nc], g_psi_b [B,nc] (covered faces; 0 elsewhere); hang_only as in
+ * the forward; work [C,nc,3] must be zero on entry. */
+int gf_lsq_grad_vjp(const gf_mesh *m, int32_t nc, const double *g_grad, const uint8_t *covered,
+                    int32_t hang_only, double *work, double *g_psi, double *g_psi_b,
+                    gf_stream_t stream);
+/* Reverse mode of gf_bc_state (+ gf_bc_slip on slip_mask faces, may be NULL):
+ * g_psi_b [B,nc] -> g_psi [C,nc], g_ref_v, g_ref_g [B,nc] (all added to; the
+ * reference arrays may be NULL); on slip faces the ref_v gradient is carried
+ * on to the cell values and left 0 */
+int gf_bc_state_vjp(const gf_mesh *m, int32_t nc, const double *frac, const uint8_t *covered,
+                    const uint8_t *slip_mask, const double *g_psi_b, double *g_psi,
+                    double *g_ref_v, double *g_ref_g, gf_stream_t stream);
 
 /* ---- LDU algebra (fvmatrix.py:242-323, :424-468) ------------------------ */
 int gf_ldu_multiply(const gf_mesh *m, int32_t nc, const double *diag, const double *upper,

@@ -127,6 +127,7 @@ def _setup(name, nc, scheme, harmonic, seed=0):
         "older": P.older_data.detach().clone(),
         "dirichlet": rnd(*cs) if cs else rnd(()),
         "neumann": rnd(*cs) if cs else rnd(()),
+        "psi": P.data.detach().clone(),
     }
     return g, P, leaves, cs, scheme, harmonic
 
@@ -144,6 +145,7 @@ def _assemble(g, P, vals, cs, scheme, harmonic, corrected=True):
     phi.domain_bnd_data = vals["phi_b"]
     P.old_data = vals["old"]
     P.older_data = vals["older"]
+    P.data = vals["psi"]
     return fvm.assemble(P, ddt=1.0, div=1.0, lap=-1.0, phi=phi, gamma=vals["gamma"], abc=(1.5, 2.0, 0.5),
                         scheme=DivScheme(scheme), corrected=corrected, harmonic=harmonic)
 
@@ -162,10 +164,10 @@ def _functional(g, m, wts):
     (1, "vanleer", False, False), (3, "koren", True, True)])
 def test_assembly_vjp_matches_central_differences(nc, scheme, harmonic, corrected):
     """<grad J, v> from the VJP equals (J(x + hv) - J(x - hv)) / 2h for each
-    input on the refined 3-D grid (hanging faces, corrected Laplacian).
-    Where the field values enter (hanging corrections, TVD sources) the
-    boundary values also reach the arrays through the cell gradient, and
-    asking for their gradient raises instead."""
+    input on the refined 3-D grid (hanging faces, corrected Laplacian),
+    the field values and boundary values included (through the hang-cell
+    gradient of the corrections); through TVD sources, asking for those
+    raises instead."""
     g, P, leaves, cs, scheme, harmonic = _setup("refined3d", nc, scheme, harmonic)
     _say(f"assembly vjp nc={nc} {scheme} harmonic={harmonic} corrected={corrected}")
     gen = torch.Generator(g.device).manual_seed(11)
@@ -173,12 +175,13 @@ def test_assembly_vjp_matches_central_differences(nc, scheme, harmonic, correcte
     wts = {k: torch.randn(*shape, dtype=torch.float64, device=g.device, generator=gen)
            for k, shape in (("d", (C,)), ("u", (max(F, 1),)), ("l", (max(F, 1),)), ("s", (C, *cs)),
                             ("f", (max(F, 1), *cs)))}
-    psi_terms = scheme not in ("upwind",) or corrected
-    keys = [k for k in leaves if not (psi_terms and k in ("dirichlet", "neumann"))]
-    if psi_terms:
-        bad = {k: (v.clone().requires_grad_() if k == "dirichlet" else v) for k, v in leaves.items()}
-        with pytest.raises(NotImplementedError):
-            _assemble(g, P, bad, cs, scheme, harmonic, corrected)
+    tvd = scheme not in ("upwind", "linear")
+    keys = [k for k in leaves if not (tvd and k in ("dirichlet", "neumann", "psi"))]
+    if tvd:
+        for k in ("dirichlet", "psi"):
+            bad = {kk: (v.clone().requires_grad_() if kk == k else v) for kk, v in leaves.items()}
+            with pytest.raises(NotImplementedError):
+                _assemble(g, P, bad, cs, scheme, harmonic, corrected)
     params = {k: (v.clone().requires_grad_() if k in keys else v) for k, v in leaves.items()}
     m = _assemble(g, P, params, cs, scheme, harmonic, corrected)
     J = _functional(g, m, wts)
@@ -279,18 +282,57 @@ def test_uniform_diffusivity_tensor_gradient():
     assert abs(float(gn) - fd) <= 1e-8 * abs(fd)
 
 
-def test_field_values_through_hanging_corrections_raise():
-    """The field values enter the corrected Laplacian on hanging faces; a
-    field that requires grad there is refused rather than losing it."""
-    from gridfoam_amd.fields import CellField
+def test_least_squares_gradient_vjp_matches_central_differences():
+    """fvc.grad (least squares, boundary state included) differentiated with
+    respect to the field values and the Dirichlet value / Neumann gradient
+    of its conditions, scalar and vector (with a slip wall), against
+    central differences of a random functional of the gradient."""
+    from gridfoam_amd.fields import CellField, DirichletBC, NeumannBC, SlipBC
+    from gridfoam_amd.fv import fvc
+
+    g = _grid("refined3d")
+    gen = torch.Generator(g.device).manual_seed(31)
+    _say("least-squares gradient vjp")
+    for cs in ((), (3,)):
+        P = CellField(g, f"lsq{len(cs)}", cs)
+        base = {"psi": torch.randn(g.num_cells, *cs, dtype=torch.float64, device=g.device, generator=gen),
+                "dir": torch.randn(cs, dtype=torch.float64, device=g.device, generator=gen),
+                "neu": torch.randn(cs, dtype=torch.float64, device=g.device, generator=gen)}
+        W = torch.randn(g.num_cells, *cs, 3, dtype=torch.float64, device=g.device, generator=gen)
+
+        def J(vals):
+            bcs = {"x_minus": DirichletBC(vals["dir"]), "y_plus": NeumannBC(vals["neu"])}
+            if cs:
+                bcs["z_plus"] = SlipBC()
+            P.add_boundary_conditions(bcs)
+            P.data = vals["psi"]
+            return (fvc.grad(P) * W).sum()
+
+        params = {k: v.clone().requires_grad_() for k, v in base.items()}
+        grads = dict(zip(params, torch.autograd.grad(J(params), list(params.values()))))
+        with torch.no_grad():
+            for k, b in base.items():
+                v = torch.randn(b.shape, dtype=torch.float64, device=g.device, generator=gen)
+                h = 1e-4 * max(float(b.abs().max()), 1.0)
+                fd = (float(J({**base, k: b + h * v})) - float(J({**base, k: b - h * v}))) / (2 * h)
+                an = float((grads[k] * v).sum())
+                assert abs(an - fd) <= 1e-7 * max(abs(fd), 1e-3), (cs, k, an, fd)
+
+
+def test_tvd_field_values_raise():
+    """Through TVD sources (a ratio nonlinear in the field and its
+    gradient) a field that requires grad is refused rather than losing it."""
+    from gridfoam_amd.config import DivScheme
+    from gridfoam_amd.fields import CellField, FaceField
     from gridfoam_amd.fv import fvm
 
     g = _grid("refined3d")
     P = CellField(g, "q_raise", ())
     P.data = torch.ones(g.num_cells, dtype=torch.float64, device=g.device, requires_grad=True)
-    _say("field-value grad guard")
+    phi = FaceField(g, "phi_raise")
+    _say("tvd field-value grad guard")
     with pytest.raises(NotImplementedError):
-        fvm.assemble(P, lap=-1.0, gamma=1.0, corrected=True)
+        fvm.assemble(P, div=1.0, phi=phi, scheme=DivScheme.VANLEER)
     with torch.no_grad():
-        m = fvm.assemble(P, lap=-1.0, gamma=1.0, corrected=True)
-    assert m.face_flux_correction is not None and not m.diag.requires_grad
+        m = fvm.assemble(P, div=1.0, phi=phi, scheme=DivScheme.VANLEER)
+    assert not m.diag.requires_grad
