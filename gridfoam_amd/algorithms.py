@@ -19,7 +19,7 @@ from typing import Dict, Optional, Tuple
 import torch
 
 from . import _native as nat
-from .config import PIMPLEAlgorithm, PISOAlgorithm, PrecisionType, SIMPLEAlgorithm
+from .config import ManualAlgorithm, PIMPLEAlgorithm, PISOAlgorithm, PrecisionType, SIMPLEAlgorithm
 from .fields import CellField, DirichletBC, FaceField
 from .fv import ddt_coefficients, fvc, fvm, sn_grad_corrected
 from .fvmatrix import Equation, FvMatrix
@@ -508,6 +508,8 @@ def create_algorithm(grid: OctreeGrid):
         return PISO(grid)
     if isinstance(alg, PIMPLEAlgorithm):
         return PIMPLE(grid)
+    if isinstance(alg, ManualAlgorithm):
+        raise ValueError("Manual algorithm has no step sequence")  # algorithms/factory.py:37-38
     raise ValueError(f"unsupported algorithm {alg}")
 
 

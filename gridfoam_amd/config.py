@@ -223,8 +223,12 @@ class PIMPLEAlgorithm(BaseModel, frozen=True):  # meta/config.py:467-515
     pRefValue: Optional[float] = None
 
 
+class ManualAlgorithm(BaseModel, frozen=True):  # meta/config.py:515-516: equations driven by the user
+    type: AlgorithmType = AlgorithmType.MANUAL
+
+
 class fvSolutionConfig(BaseModel, frozen=True):
-    algorithm: Union[SIMPLEAlgorithm, PISOAlgorithm, PIMPLEAlgorithm]
+    algorithm: Union[SIMPLEAlgorithm, PISOAlgorithm, PIMPLEAlgorithm, ManualAlgorithm]
     solvers: Dict[str, SolverConfig] = Field(default_factory=dict)
     adjustPhi: bool = True
 
@@ -239,6 +243,8 @@ class fvSolutionConfig(BaseModel, frozen=True):
                 return PISOAlgorithm(**v)
             if kind == "pimple":
                 return PIMPLEAlgorithm(**v)
+            if kind == "manual":
+                return ManualAlgorithm(**v)
             raise ValueError(f"unsupported algorithm type {kind!r}")
         return v
 

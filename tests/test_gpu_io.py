@@ -160,3 +160,21 @@ def test_all_run_stops_on_converged_simple(tmp_path):
     assert taken == stop + 1
     assert (tmp_path / f"cavity_{stop + 1:04d}.vtu").exists()
     assert not (tmp_path / f"cavity_{stop + 2:04d}.vtu").exists()
+
+
+def test_manual_algorithm_gives_a_grid_and_no_step_sequence():
+    """runner.py:46-96, algorithms/factory.py:37-38: a ``manual`` algorithm
+    loads, manual_run builds its grid, and asking for a step sequence raises."""
+    from gridfoam_amd.algorithms import create_case
+    from gridfoam_amd.config import load_config
+    from gridfoam_amd.runner import manual_run, manual_step
+
+    cfg = small_cavity2d(8)
+    cfg["simulator"]["fvSolution"]["algorithm"] = {"type": "manual"}
+    _say("manual algorithm")
+    grid = manual_run(cfg)
+    assert grid.num_cells == 64 and grid.sim_config.fvSolution.algorithm.type == "manual"
+    with pytest.raises(ValueError, match="no step sequence"):
+        create_case(load_config(cfg))
+    with pytest.raises(ValueError, match="no step sequence"):
+        next(iter(manual_step(cfg)))

@@ -142,3 +142,13 @@ def test_vtu_reuses_grid_and_round_trips_dtypes(tmp_path):
     assert m.cell_data["level"].dtype == np.int32
     np.testing.assert_array_equal(m.cell_data["level"], [0, 1, 2])
     np.testing.assert_array_equal(m.points, ug.points)
+
+
+def test_manual_algorithm_config_loads():
+    """meta/config.py:515-521: the manual algorithm type validates."""
+    from gridfoam_amd.config import ManualAlgorithm, load_config
+    from tests.cases import small_cavity2d
+
+    cfg = small_cavity2d(8)
+    cfg["simulator"]["fvSolution"]["algorithm"] = {"type": "manual"}
+    assert isinstance(load_config(cfg).simulator.fvSolution.algorithm, ManualAlgorithm)
