@@ -9,8 +9,8 @@ extreme diffusivities (tests/integration/fv/test_harmonic_diffusion.py:
 93-132); central differences of a random linear functional of every
 assembled array along random directions of each input (the assembly is
 linear in each of them away from phi = 0, so the differences are exact to
-rounding); and central differences of J = w.x through the solve (north-star
-bar 1e-5 relative), for PCG and BiCGSTAB."""
+rounding); and central differences of J = w.x through the solve (to 1e-5
+relative), for PCG and BiCGSTAB."""
 
 from __future__ import annotations
 
