@@ -368,6 +368,79 @@ __global__ void k_fvc_div(const gf_mesh m, const double *phi_s, const double *ph
     for (int q = 0; q < NC; ++q) out[c * NC + q] = ((so[q] + sn[q]) + sb[q]) / V;
 }
 
+// Reverse mode of k_fvc_div: a face flux enters its owner's sum with + and
+// its neighbour's with -, each divided by that cell's volume.
+template <int NC>
+__global__ void k_fvc_div_vjp_faces(const gf_mesh m, const double *g_out, double *g_phi_s) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.n_faces) return;
+    const int o = m.owner[f], n = m.neighbour[f];
+    const double vo = m.volumes[o], vn = m.volumes[n];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) g_phi_s[f * NC + q] = g_out[o * NC + q] / vo - g_out[n * NC + q] / vn;
+}
+
+template <int NC>
+__global__ void k_fvc_div_vjp_bnd(const gf_mesh m, const double *g_out, double *g_phi_b) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m.n_bnd) return;
+    const int c = m.bnd_owner[k];
+    const double v = m.volumes[c];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) g_phi_b[k * NC + q] = g_out[c * NC + q] / v;
+}
+
+// Reverse mode of k_interp_linear: each cell gathers w g_f over the faces it
+// owns and (1 - w) g_f over the faces it neighbours, added to g_psi.
+template <int NC>
+__global__ void k_interp_vjp_cells(const gf_mesh m, const double *g_f, double *g_psi) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.n_cells) return;
+    double acc[NC] = {};
+    for (int f = m.own_row[c]; f < m.own_row[c + 1]; ++f) {
+        const double w = m.w[f];
+#pragma unroll
+        for (int q = 0; q < NC; ++q) acc[q] += w * g_f[f * NC + q];
+    }
+    for (int k = m.nbr_row[c]; k < m.nbr_row[c + 1]; ++k) {
+        const int f = m.nbr_face[k];
+        const double w1 = 1.0 - m.w[f];
+#pragma unroll
+        for (int q = 0; q < NC; ++q) acc[q] += w1 * g_f[f * NC + q];
+    }
+#pragma unroll
+    for (int q = 0; q < NC; ++q) g_psi[c * NC + q] += acc[q];
+}
+
+// Reverse mode of k_interp_hang: the correction (w grad_O + (1 - w) grad_N)
+// . offset of hanging face h sends w offset g_f to the owner's gradient and
+// (1 - w) offset g_f to the neighbour's; one thread per hang cell gathers its
+// hanging faces (hc_code h: owner side, -h-1: neighbour side) and adds.
+template <int NC>
+__global__ void k_interp_vjp_hang(const gf_mesh m, const double *g_f, double *g_grad) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.n_hang_cells) return;
+    const int c = m.hang_cell[i];
+    double acc[NC][3] = {};
+    for (int k = m.hc_row[i]; k < m.hc_row[i + 1]; ++k) {
+        const int code = m.hc_code[k];
+        const int h = code >= 0 ? code : -code - 1;
+        const int f = m.hang_face[h];
+        const double s = code >= 0 ? m.w[f] : 1.0 - m.w[f];
+        const double *off = m.hang_offset + 3 * h;
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+            const double gs = s * g_f[f * NC + q];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) acc[q][j] += gs * off[j];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NC; ++q)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) g_grad[(c * NC + q) * 3 + j] += acc[q][j];
+}
+
 // ---------------------------------------------------------------- assembly
 // Face diffusivity (laplacian.py:38-73): linear interpolation, or the
 // distance-weighted harmonic mean 1 / ((1-w)/gO + w/gN) evaluated scaled by
@@ -1277,6 +1350,30 @@ extern "C" int gf_fvc_div(const gf_mesh *m, int32_t nc, const double *phi_s, con
     GF_REQUIRE(m && phi_s && phi_b && out, "null argument");
     hipStream_t s = S(stream);
     GF_NC_DISPATCH(nc, k_fvc_div, grid_for(m->n_cells), *m, phi_s, phi_b, out);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
+extern "C" int gf_fvc_div_vjp(const gf_mesh *m, int32_t nc, const double *g_out, double *g_phi_s, double *g_phi_b,
+                              gf_stream_t stream) {
+    GF_REQUIRE(m && g_out && g_phi_s && g_phi_b, "null argument");
+    hipStream_t s = S(stream);
+    if (m->n_faces > 0) GF_NC_DISPATCH(nc, k_fvc_div_vjp_faces, grid_for(m->n_faces), *m, g_out, g_phi_s);
+    if (m->n_bnd > 0) GF_NC_DISPATCH(nc, k_fvc_div_vjp_bnd, grid_for(m->n_bnd), *m, g_out, g_phi_b);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
+extern "C" int gf_interpolate_vjp(const gf_mesh *m, int32_t nc, const double *g_psi_f, double *g_psi,
+                                  double *g_grad, gf_stream_t stream) {
+    GF_REQUIRE(m && g_psi_f && g_psi, "null argument");
+    hipStream_t s = S(stream);
+    if (m->n_faces == 0) return GF_OK;
+    GF_NC_DISPATCH(nc, k_interp_vjp_cells, grid_for(m->n_cells), *m, g_psi_f, g_psi);
+    if (m->n_hang > 0) {
+        GF_REQUIRE(g_grad, "hanging faces need a gradient cotangent");
+        GF_NC_DISPATCH(nc, k_interp_vjp_hang, grid_for(m->n_hang_cells), *m, g_psi_f, g_grad);
+    }
     GF_CHECK_LAUNCH();
     return GF_OK;
 }

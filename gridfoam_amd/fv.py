@@ -148,6 +148,86 @@ class _LsqGradFn(torch.autograd.Function):
                 g_rg if need[5] else None)
 
 
+class _BndValueFn(torch.autograd.Function):
+    """A field's boundary face values (the value-fraction state,
+    boundaries/base.py:71-106) as an autograd node: the backward is
+    gf_bc_state_vjp (slip faces carry their cotangent on to the cell
+    values)."""
+
+    @staticmethod
+    def forward(ctx, field, st, psi, ref_v, ref_g):  # noqa: D401
+        ctx.field, ctx.st = field, st
+        return st.psi_b.clone()
+
+    @staticmethod
+    def backward(ctx, g_out):
+        st, field = ctx.st, ctx.field
+        g = field.grid
+        g_rv, g_rg = torch.zeros_like(st.ref_v), torch.zeros_like(st.ref_g)
+        g_psi = torch.zeros((g.num_cells, field.nc), dtype=torch.float64, device=g.device)
+        nat.call("gf_bc_state_vjp", g.mesh_ref, ctypes.c_int32(field.nc), nat.ptr(st.frac), nat.ptr(st.covered),
+                 nat.ptr(st.slip), nat.ptr(g_out.contiguous()), nat.ptr(g_psi), nat.ptr(g_rv), nat.ptr(g_rg),
+                 nat.stream())
+        need = ctx.needs_input_grad
+        return (None, None, g_psi.reshape(g.num_cells, *field.component_shape) if need[2] else None,
+                g_rv if need[3] else None, g_rg if need[4] else None)
+
+
+class _InterpFn(torch.autograd.Function):
+    """Linear face values of the internal faces (with the hanging-face
+    gradient corrections) as an autograd node: the backward is
+    gf_interpolate_vjp; the gradient's own inputs follow through its node."""
+
+    @staticmethod
+    def forward(ctx, field, psi, grad):  # noqa: D401
+        ctx.field, ctx.has_grad = field, grad is not None
+        g = field.grid
+        out = torch.zeros((max(g.num_internal_faces, 1), *field.component_shape), dtype=torch.float64,
+                          device=g.device)
+        nat.call("gf_interpolate", g.mesh_ref, ctypes.c_int32(field.nc), nat.ptr(psi.contiguous()),
+                 nat.ptr(None if grad is None else grad.contiguous()), nat.ptr(out), nat.stream())
+        return out
+
+    @staticmethod
+    def backward(ctx, g_out):
+        field = ctx.field
+        g = field.grid
+        f64 = dict(dtype=torch.float64, device=g.device)
+        g_psi = torch.zeros((g.num_cells, *field.component_shape), **f64)
+        g_grad = torch.zeros((g.num_cells, *field.component_shape, 3), **f64) if ctx.has_grad else None
+        nat.call("gf_interpolate_vjp", g.mesh_ref, ctypes.c_int32(field.nc), nat.ptr(g_out.contiguous()),
+                 nat.ptr(g_psi), nat.ptr(g_grad), nat.stream())
+        need = ctx.needs_input_grad
+        return None, g_psi if need[1] else None, g_grad if need[2] else None
+
+
+class _DivFn(torch.autograd.Function):
+    """fvc.div of a face flux as an autograd node: the backward is
+    gf_fvc_div_vjp."""
+
+    @staticmethod
+    def forward(ctx, grid, nc, cshape, phi_s, phi_b):  # noqa: D401
+        ctx.grid, ctx.nc = grid, nc
+        out = torch.empty((grid.num_cells, *cshape), dtype=torch.float64, device=grid.device)
+        nat.call("gf_fvc_div", grid.mesh_ref, ctypes.c_int32(nc), nat.ptr(phi_s.contiguous()),
+                 nat.ptr(phi_b.contiguous()), nat.ptr(out), nat.stream())
+        return out
+
+    @staticmethod
+    def backward(ctx, g_out):
+        return (None, None, None) + _div_pullback(ctx.grid, ctx.nc, g_out, ctx.needs_input_grad[3:])
+
+
+def _div_pullback(g, nc: int, g_out: torch.Tensor, need) -> Tuple[Optional[torch.Tensor], Optional[torch.Tensor]]:
+    cs = tuple(g_out.shape[1:])
+    f64 = dict(dtype=torch.float64, device=g.device)
+    g_s = torch.zeros((max(g.num_internal_faces, 1), *cs), **f64)
+    g_b = torch.zeros((max(g.num_domain_bnd_faces, 1), *cs), **f64)
+    nat.call("gf_fvc_div_vjp", g.mesh_ref, ctypes.c_int32(nc), nat.ptr(g_out.contiguous()), nat.ptr(g_s),
+             nat.ptr(g_b), nat.stream())
+    return (g_s if need[0] else None), (g_b if need[1] else None)
+
+
 class fvc:
     @staticmethod
     def grad(field: CellField, *, hang_only: bool = False, out: Optional[torch.Tensor] = None) -> torch.Tensor:
@@ -181,11 +261,21 @@ class fvc:
 
     @staticmethod
     def interpolate(field: CellField, name: Optional[str] = None) -> FaceField:
-        """fv/schemes/interpolate.py:20-63."""
+        """fv/schemes/interpolate.py:20-63.  Differentiable with respect to
+        the field values and its boundary values when either requires
+        grad."""
         g = field.grid
         out = FaceField(g, name or f"{field.name}_f", field.component_shape)
         st = field.boundary_state()
+        rv = st.ref_v_ad if st.ref_v_ad is not None else st.ref_v
+        rg = st.ref_g_ad if st.ref_g_ad is not None else st.ref_g
+        track = torch.is_grad_enabled() and (field.data.requires_grad or rv.requires_grad or rg.requires_grad)
         grad = fvc.grad(field, hang_only=True) if g.num_hanging_faces else None
+        if track:  # face values carrying the graph (reverse mode: gf_interpolate_vjp, gf_bc_state_vjp)
+            F, B = g.num_internal_faces, g.num_domain_bnd_faces
+            out.single_data = _InterpFn.apply(field, field.data, grad)[:F]
+            out.domain_bnd_data = _BndValueFn.apply(field, st, field.data, rv, rg).view(out.bnd_buffer.shape)[:B]
+            return out
         nat.call("gf_interpolate", g.mesh_ref, ctypes.c_int32(field.nc), nat.ptr(field.data), nat.ptr(grad),
                  nat.ptr(out.single_buffer), nat.stream())
         out.bnd_buffer.copy_(st.psi_b.view(out.bnd_buffer.shape))
@@ -193,9 +283,12 @@ class fvc:
 
     @staticmethod
     def div(phi: FaceField) -> torch.Tensor:
-        """fv/fvc/div.py:15-64 (face sum / V)."""
+        """fv/fvc/div.py:15-64 (face sum / V); differentiable with respect to
+        the face values when they carry a graph."""
         g = phi.grid
         nc = 3 if phi.component_shape == (3,) else 1
+        if torch.is_grad_enabled() and (phi.single_buffer.requires_grad or phi.bnd_buffer.requires_grad):
+            return _DivFn.apply(g, nc, phi.component_shape, phi.single_buffer, phi.bnd_buffer)
         out = torch.empty((g.num_cells, *phi.component_shape), dtype=torch.float64, device=g.device)
         nat.call("gf_fvc_div", g.mesh_ref, ctypes.c_int32(nc), nat.ptr(phi.single_buffer), nat.ptr(phi.bnd_buffer),
                  nat.ptr(out), nat.stream())

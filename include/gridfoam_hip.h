@@ -136,6 +136,16 @@ int gf_interpolate(const gf_mesh *m, int32_t nc, const double *psi, const double
 /* fvc/div.py:15-64 */
 int gf_fvc_div(const gf_mesh *m, int32_t nc, const double *phi_s, const double *phi_b,
                double *out, gf_stream_t stream);
+/* Reverse mode of gf_fvc_div (autograd through fv/fvc/div.py:15-64):
+ * g_out [C,nc] -> g_phi_s [F,nc], g_phi_b [B,nc] (overwritten) */
+int gf_fvc_div_vjp(const gf_mesh *m, int32_t nc, const double *g_out, double *g_phi_s,
+                   double *g_phi_b, gf_stream_t stream);
+/* Reverse mode of gf_interpolate (autograd through interpolate.py:20-63 and
+ * face_interpolation.py:114-197): g_psi_f [F,nc] -> g_psi [C,nc] and, on
+ * grids with hanging faces, g_grad [C,nc,3] on the cells next to them (both
+ * added to); chained with gf_lsq_grad_vjp for the gradient's own inputs */
+int gf_interpolate_vjp(const gf_mesh *m, int32_t nc, const double *g_psi_f, double *g_psi,
+                       double *g_grad, gf_stream_t stream);
 /* FP64 mass-conservation check of a face flux (post/diagnostics/continuity.py:12-49):
  * out[0] = dt sum(|div phi| V)/sum V (local), out[1] = dt sum(div phi V)/sum V (global);
  * div_scratch [C] receives div(phi) */
