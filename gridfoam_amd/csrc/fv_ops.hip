@@ -319,6 +319,30 @@ __global__ void k_gauss_grad(const gf_mesh m, const double *psi_s, const double 
         for (int j = 0; j < 3; ++j) grad[(c * NC + q) * 3 + j] = acc[q][j] / V;
 }
 
+// Reverse mode of k_gauss_grad: face value f enters its owner's gradient
+// component axis[f] with +|Sf|/V_O and its neighbour's with -|Sf|/V_N; a
+// boundary value enters its cell's with sign |Sf|/V (overwrites).
+template <int NC>
+__global__ void k_gauss_grad_vjp_faces(const gf_mesh m, const double *g_grad, double *g_psi_s) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.n_faces) return;
+    const int o = m.owner[f], n = m.neighbour[f], a = m.axis[f];
+    const double sf = m.mag_sf[f], vo = m.volumes[o], vn = m.volumes[n];
+#pragma unroll
+    for (int q = 0; q < NC; ++q)
+        g_psi_s[f * NC + q] = sf * (g_grad[(o * NC + q) * 3 + a] / vo - g_grad[(n * NC + q) * 3 + a] / vn);
+}
+
+template <int NC>
+__global__ void k_gauss_grad_vjp_bnd(const gf_mesh m, const double *g_grad, double *g_psi_b) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m.n_bnd) return;
+    const int c = m.bnd_owner[k], a = m.bnd_dir[k] / 2;
+    const double sf = m.bnd_sign[k] * m.bnd_mag_sf[k], v = m.volumes[c];
+#pragma unroll
+    for (int q = 0; q < NC; ++q) g_psi_b[k * NC + q] = sf * g_grad[(c * NC + q) * 3 + a] / v;
+}
+
 // fvc.reconstruct (fvc/reconstruct.py:27-107): U_P = (sum |Sf| n n^T)^-1
 // sum n phi_f over all faces of the cell, both sides of an internal face
 // with the face's own normal; diagonal for axis-aligned faces.
@@ -1422,6 +1446,16 @@ extern "C" int gf_gauss_grad(const gf_mesh *m, int32_t nc, const double *psi_s, 
     GF_REQUIRE(m && psi_s && psi_b && grad, "null argument");
     hipStream_t s = S(stream);
     GF_NC_DISPATCH(nc, k_gauss_grad, grid_for(m->n_cells), *m, psi_s, psi_b, grad);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
+extern "C" int gf_gauss_grad_vjp(const gf_mesh *m, int32_t nc, const double *g_grad, double *g_psi_s,
+                                 double *g_psi_b, gf_stream_t stream) {
+    GF_REQUIRE(m && g_grad && g_psi_s && g_psi_b, "null argument");
+    hipStream_t s = S(stream);
+    if (m->n_faces > 0) GF_NC_DISPATCH(nc, k_gauss_grad_vjp_faces, grid_for(m->n_faces), *m, g_grad, g_psi_s);
+    if (m->n_bnd > 0) GF_NC_DISPATCH(nc, k_gauss_grad_vjp_bnd, grid_for(m->n_bnd), *m, g_grad, g_psi_b);
     GF_CHECK_LAUNCH();
     return GF_OK;
 }

@@ -218,6 +218,30 @@ class _DivFn(torch.autograd.Function):
         return (None, None, None) + _div_pullback(ctx.grid, ctx.nc, g_out, ctx.needs_input_grad[3:])
 
 
+class _GaussGradFn(torch.autograd.Function):
+    """The Green-Gauss cell gradient of given face values as an autograd
+    node: the backward is gf_gauss_grad_vjp."""
+
+    @staticmethod
+    def forward(ctx, grid, nc, cshape, psi_s, psi_b):  # noqa: D401
+        ctx.grid, ctx.nc = grid, nc
+        out = torch.empty((grid.num_cells, *cshape, 3), dtype=torch.float64, device=grid.device)
+        nat.call("gf_gauss_grad", grid.mesh_ref, ctypes.c_int32(nc), nat.ptr(psi_s.contiguous()),
+                 nat.ptr(psi_b.contiguous()), nat.ptr(out), nat.stream())
+        return out
+
+    @staticmethod
+    def backward(ctx, g_out):
+        g, need = ctx.grid, ctx.needs_input_grad
+        cs = tuple(g_out.shape[1:-1])
+        f64 = dict(dtype=torch.float64, device=g.device)
+        g_s = torch.zeros((max(g.num_internal_faces, 1), *cs), **f64)
+        g_b = torch.zeros((max(g.num_domain_bnd_faces, 1), *cs), **f64)
+        nat.call("gf_gauss_grad_vjp", g.mesh_ref, ctypes.c_int32(ctx.nc), nat.ptr(g_out.contiguous()),
+                 nat.ptr(g_s), nat.ptr(g_b), nat.stream())
+        return None, None, None, (g_s if need[3] else None), (g_b if need[4] else None)
+
+
 def _div_pullback(g, nc: int, g_out: torch.Tensor, need) -> Tuple[Optional[torch.Tensor], Optional[torch.Tensor]]:
     cs = tuple(g_out.shape[1:])
     f64 = dict(dtype=torch.float64, device=g.device)
@@ -246,10 +270,9 @@ class fvc:
         rg = st.ref_g_ad if st.ref_g_ad is not None else st.ref_g
         track = torch.is_grad_enabled() and (field.data.requires_grad or rv.requires_grad or rg.requires_grad)
         if not hang_only and scheme == GradScheme.LINEAR:
-            if track:
-                raise NotImplementedError("gradients through the Green-Gauss cell gradient are not supported; "
-                                          "use the least-squares scheme")
             fi = fvc.interpolate(field)
+            if track:  # through the linear face values (gf_gauss_grad_vjp, then the interpolation's node)
+                return _GaussGradFn.apply(g, field.nc, field.component_shape, fi.single_buffer, fi.bnd_buffer)
             nat.call("gf_gauss_grad", g.mesh_ref, ctypes.c_int32(field.nc), nat.ptr(fi.single_buffer),
                      nat.ptr(fi.bnd_buffer), nat.ptr(out), nat.stream())
             return out

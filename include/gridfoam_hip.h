@@ -126,6 +126,11 @@ int gf_lsq_grad(const gf_mesh *m, int32_t nc, const double *psi, const double *p
  * grad [C,nc,3] = sum psi_f Sf / V */
 int gf_gauss_grad(const gf_mesh *m, int32_t nc, const double *psi_s, const double *psi_b,
                   double *grad, gf_stream_t stream);
+/* Reverse mode of gf_gauss_grad (autograd through fv/schemes/grad.py:21-44):
+ * g_grad [C,nc,3] -> g_psi_s [F,nc], g_psi_b [B,nc] (overwritten); chained
+ * with gf_interpolate_vjp / gf_bc_state_vjp for the face values' inputs */
+int gf_gauss_grad_vjp(const gf_mesh *m, int32_t nc, const double *g_grad, double *g_psi_s,
+                      double *g_psi_b, gf_stream_t stream);
 /* fvc/reconstruct.py:27-107: U [C,3] from the face flux phi_s [F], phi_b [B]:
  * per axis, the area-weighted average of the face-normal velocities */
 int gf_reconstruct(const gf_mesh *m, const double *phi_s, const double *phi_b, double *U,

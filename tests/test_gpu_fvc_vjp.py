@@ -1,7 +1,8 @@
 """GPU: reverse mode of the explicit face operators -- fvc.interpolate
 (fv/schemes/interpolate.py:20-63 with the hanging-face corrections of
 fv/kernels/face_interpolation.py:114-197, boundary values from the
-conditions) and fvc.div (fv/fvc/div.py:15-64) -- against central
+conditions), fvc.div (fv/fvc/div.py:15-64) and the Green-Gauss gradient
+(fv/schemes/grad.py:21-44) -- against central
 differences.  Both maps are linear in the field values and the boundary
 values, so the differences are exact to rounding; the transport step built
 on an interpolated flux is differentiable away from phi = 0 (upwind), and
@@ -170,3 +171,41 @@ def test_transport_by_an_interpolated_velocity_is_differentiable(bdf2):
         smallest = min(float(phi.single_data.abs().min()), float(phi.domain_bnd_data.abs().min()))
     assert smallest > 1e-6  # no face flux near upwind's kink at phi = 0 for the steps below
     _check(J, base, gen, dev, tol=1e-6)
+
+
+@pytest.mark.parametrize("name", ["refined2d", "refined3d"])
+@pytest.mark.parametrize("cs", [(), (3,)])
+def test_green_gauss_gradient_vjp_matches_central_differences(name, cs):
+    """The Green-Gauss gradient (gradSchemes linear, fv/schemes/grad.py:21-44)
+    of the linear face values with respect to the cell values, a Dirichlet
+    value and a Neumann gradient; the tracked gradient equals the untracked
+    one."""
+    from gridfoam_amd.config import load_config
+    from gridfoam_amd.fields import CellField, DirichletBC, NeumannBC
+    from gridfoam_amd.fv import fvc
+    from gridfoam_amd.grid import create_grid
+
+    cfg = copy.deepcopy(small_cavity2d(8))
+    cfg["fluxel"] = {**cfg["fluxel"], **fluxel_block(name)}
+    cfg["simulator"]["fvSchemes"]["gradSchemes"] = {"default": "linear"}
+    g = create_grid(load_config(cfg))
+    dev = g.device
+    gen = torch.Generator(dev).manual_seed(17)
+    _say(f"green-gauss vjp {name} cs={cs}")
+    P = CellField(g, f"gg{len(cs)}", cs)
+    base = {"psi": torch.randn(g.num_cells, *cs, dtype=torch.float64, device=dev, generator=gen),
+            "dir": torch.randn(cs, dtype=torch.float64, device=dev, generator=gen),
+            "neu": torch.randn(cs, dtype=torch.float64, device=dev, generator=gen)}
+    W = torch.randn(g.num_cells, *cs, 3, dtype=torch.float64, device=dev, generator=gen)
+
+    def J(vals):
+        P.add_boundary_conditions({"x_plus": DirichletBC(vals["dir"]), "y_minus": NeumannBC(vals["neu"])})
+        P.data = vals["psi"]
+        return (fvc.grad(P) * W).sum()
+
+    _check(J, base, gen, dev)
+    with torch.no_grad():
+        P.data = base["psi"]
+        ref = fvc.grad(P).clone()
+    P.data = base["psi"].clone().requires_grad_()
+    assert torch.equal(fvc.grad(P).detach(), ref)
