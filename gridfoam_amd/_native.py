@@ -80,6 +80,7 @@ _SIGS: dict[str, tuple[Any, list[Any]]] = {
     "gf_reconstruct": (_i32, [_P(GfMesh), _vp, _vp, _vp, _vp]),
     "gf_fvc_div": (_i32, [_P(GfMesh), _i32, _vp, _vp, _vp, _vp]),
     "gf_fvc_div_vjp": (_i32, [_P(GfMesh), _i32, _vp, _vp, _vp, _vp]),
+    "gf_reconstruct_vjp": (_i32, [_P(GfMesh), _vp, _vp, _vp, _vp, _vp]),
     "gf_gauss_grad_vjp": (_i32, [_P(GfMesh), _i32, _vp, _vp, _vp, _vp]),
     "gf_interpolate_vjp": (_i32, [_P(GfMesh), _i32, _vp, _vp, _vp, _vp]),
     "gf_continuity_error": (_i32, [_P(GfMesh), _vp, _vp, _f64, _vp, _vp, _vp]),

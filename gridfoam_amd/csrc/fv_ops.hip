@@ -370,6 +370,35 @@ __global__ void k_reconstruct(const gf_mesh m, const double *phi_s, const double
     for (int j = 0; j < 3; ++j) U[3 * c + j] = num[j] / den[j];
 }
 
+// Reverse mode of k_reconstruct: per cell, s[c][a] = g_U[c][a] / den_c[a]
+// (the same area sums as the forward); then every face value gathers s of
+// the cells it enters (both sides of an internal face with +, a boundary
+// face with its sign).
+__global__ void k_reconstruct_vjp_cells(const gf_mesh m, const double *g_U, double *s) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.n_cells) return;
+    double den[3] = {};
+    for (int f = m.own_row[c]; f < m.own_row[c + 1]; ++f) den[m.axis[f]] += m.mag_sf[f];
+    for (int k = m.nbr_row[c]; k < m.nbr_row[c + 1]; ++k) {
+        const int f = m.nbr_face[k];
+        den[m.axis[f]] += m.mag_sf[f];
+    }
+    for (int k = m.bnd_row[c]; k < m.bnd_row[c + 1]; ++k) den[m.bnd_dir[k] / 2] += m.bnd_mag_sf[k];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) s[3 * c + j] = g_U[3 * c + j] / den[j];
+}
+
+__global__ void k_reconstruct_vjp_faces(const gf_mesh m, const double *s, double *g_phi_s, double *g_phi_b) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m.n_faces) {
+        const int a = m.axis[i];
+        g_phi_s[i] = s[3 * m.owner[i] + a] + s[3 * m.neighbour[i] + a];
+    } else if (i < m.n_faces + m.n_bnd) {
+        const int k = i - m.n_faces;
+        g_phi_b[k] = m.bnd_sign[k] * s[3 * m.bnd_owner[k] + m.bnd_dir[k] / 2];
+    }
+}
+
 // ---------------------------------------------------------------- divergence
 template <int NC>
 __global__ void k_fvc_div(const gf_mesh m, const double *phi_s, const double *phi_b, double *out) {
@@ -1374,6 +1403,19 @@ extern "C" int gf_fvc_div(const gf_mesh *m, int32_t nc, const double *phi_s, con
     GF_REQUIRE(m && phi_s && phi_b && out, "null argument");
     hipStream_t s = S(stream);
     GF_NC_DISPATCH(nc, k_fvc_div, grid_for(m->n_cells), *m, phi_s, phi_b, out);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
+extern "C" int gf_reconstruct_vjp(const gf_mesh *m, const double *g_U, double *work, double *g_phi_s,
+                                  double *g_phi_b, gf_stream_t stream) {
+    GF_REQUIRE(m && g_U && work && g_phi_s && g_phi_b, "null argument");
+    hipStream_t s = S(stream);
+    if (m->n_cells == 0) return GF_OK;
+    hipLaunchKernelGGL(k_reconstruct_vjp_cells, dim3(grid_for(m->n_cells)), dim3(kBlock), 0, s, *m, g_U, work);
+    if (m->n_faces + m->n_bnd > 0)
+        hipLaunchKernelGGL(k_reconstruct_vjp_faces, dim3(grid_for(m->n_faces + m->n_bnd)), dim3(kBlock), 0, s, *m, work,
+                           g_phi_s, g_phi_b);
     GF_CHECK_LAUNCH();
     return GF_OK;
 }

@@ -242,6 +242,30 @@ class _GaussGradFn(torch.autograd.Function):
         return None, None, None, (g_s if need[3] else None), (g_b if need[4] else None)
 
 
+class _ReconstructFn(torch.autograd.Function):
+    """fvc.reconstruct of a face flux as an autograd node: the backward is
+    gf_reconstruct_vjp."""
+
+    @staticmethod
+    def forward(ctx, grid, phi_s, phi_b):  # noqa: D401
+        ctx.grid = grid
+        out = torch.empty((grid.num_cells, 3), dtype=torch.float64, device=grid.device)
+        nat.call("gf_reconstruct", grid.mesh_ref, nat.ptr(phi_s.contiguous()), nat.ptr(phi_b.contiguous()),
+                 nat.ptr(out), nat.stream())
+        return out
+
+    @staticmethod
+    def backward(ctx, g_out):
+        g, need = ctx.grid, ctx.needs_input_grad
+        f64 = dict(dtype=torch.float64, device=g.device)
+        work = torch.empty((g.num_cells, 3), **f64)
+        g_s = torch.zeros(max(g.num_internal_faces, 1), **f64)
+        g_b = torch.zeros(max(g.num_domain_bnd_faces, 1), **f64)
+        nat.call("gf_reconstruct_vjp", g.mesh_ref, nat.ptr(g_out.contiguous()), nat.ptr(work), nat.ptr(g_s),
+                 nat.ptr(g_b), nat.stream())
+        return None, (g_s if need[1] else None), (g_b if need[2] else None)
+
+
 def _div_pullback(g, nc: int, g_out: torch.Tensor, need) -> Tuple[Optional[torch.Tensor], Optional[torch.Tensor]]:
     cs = tuple(g_out.shape[1:])
     f64 = dict(dtype=torch.float64, device=g.device)
@@ -329,6 +353,10 @@ class fvc:
         if phi.grid is not U.grid:
             raise ValueError("phi and U must share the same grid.")
         g = phi.grid
+        if torch.is_grad_enabled() and (phi.single_buffer.requires_grad or phi.bnd_buffer.requires_grad):
+            out = _ReconstructFn.apply(g, phi.single_buffer, phi.bnd_buffer)
+            U.data = out
+            return out
         out = torch.empty((g.num_cells, 3), dtype=torch.float64, device=g.device)
         nat.call("gf_reconstruct", g.mesh_ref, nat.ptr(phi.single_buffer), nat.ptr(phi.bnd_buffer), nat.ptr(out),
                  nat.stream())

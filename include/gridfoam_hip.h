@@ -135,6 +135,10 @@ int gf_gauss_grad_vjp(const gf_mesh *m, int32_t nc, const double *g_grad, double
  * per axis, the area-weighted average of the face-normal velocities */
 int gf_reconstruct(const gf_mesh *m, const double *phi_s, const double *phi_b, double *U,
                    gf_stream_t stream);
+/* Reverse mode of gf_reconstruct (autograd through fvc/reconstruct.py:27-107):
+ * g_U [C,3] -> g_phi_s [F], g_phi_b [B] (overwritten); work [C,3] scratch */
+int gf_reconstruct_vjp(const gf_mesh *m, const double *g_U, double *work, double *g_phi_s,
+                       double *g_phi_b, gf_stream_t stream);
 /* interpolate.py:20-63 + face_interpolation.py:114-197; grad valid on hang cells */
 int gf_interpolate(const gf_mesh *m, int32_t nc, const double *psi, const double *grad,
                    double *psi_f, gf_stream_t stream);

@@ -1,7 +1,8 @@
 """GPU: reverse mode of the explicit face operators -- fvc.interpolate
 (fv/schemes/interpolate.py:20-63 with the hanging-face corrections of
 fv/kernels/face_interpolation.py:114-197, boundary values from the
-conditions), fvc.div (fv/fvc/div.py:15-64) and the Green-Gauss gradient
+conditions), fvc.div (fv/fvc/div.py:15-64), fvc.reconstruct
+(fv/fvc/reconstruct.py:27-107) and the Green-Gauss gradient
 (fv/schemes/grad.py:21-44) -- against central
 differences.  Both maps are linear in the field values and the boundary
 values, so the differences are exact to rounding; the transport step built
@@ -209,3 +210,38 @@ def test_green_gauss_gradient_vjp_matches_central_differences(name, cs):
         ref = fvc.grad(P).clone()
     P.data = base["psi"].clone().requires_grad_()
     assert torch.equal(fvc.grad(P).detach(), ref)
+
+
+@pytest.mark.parametrize("name", ["refined2d", "refined3d"])
+def test_reconstruct_vjp_matches_central_differences(name):
+    """fvc.reconstruct (fvc/reconstruct.py:27-107) of a face flux with
+    respect to its internal and boundary values; the tracked result equals
+    the untracked one."""
+    from gridfoam_amd.fields import CellField, FaceField
+    from gridfoam_amd.fv import fvc
+
+    g = _grid(name)
+    dev = g.device
+    gen = torch.Generator(dev).manual_seed(23)
+    _say(f"reconstruct vjp {name}")
+    F, B = g.num_internal_faces, g.num_domain_bnd_faces
+    base = {"s": torch.randn(F, dtype=torch.float64, device=dev, generator=gen),
+            "b": torch.randn(B, dtype=torch.float64, device=dev, generator=gen)}
+    W = torch.randn(g.num_cells, 3, dtype=torch.float64, device=dev, generator=gen)
+    U = CellField(g, "U_rec", (3,))
+
+    def J(vals):
+        phi = FaceField(g, "phi_rec")
+        phi.single_data = vals["s"]
+        phi.domain_bnd_data = vals["b"]
+        return (fvc.reconstruct(phi, U) * W).sum()
+
+    _check(J, base, gen, dev)
+    with torch.no_grad():
+        phi = FaceField(g, "phi_rec_ref")
+        phi.single_data, phi.domain_bnd_data = base["s"], base["b"]
+        ref = fvc.reconstruct(phi, U).clone()
+    phi = FaceField(g, "phi_rec_got")
+    phi.single_data = base["s"].clone().requires_grad_()
+    phi.domain_bnd_data = base["b"]
+    assert torch.equal(fvc.reconstruct(phi, U).detach(), ref)
