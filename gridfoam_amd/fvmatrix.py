@@ -138,7 +138,11 @@ class FvMatrix:
 
     # ---- operators ---------------------------------------------------------
     def multiply(self, x: torch.Tensor) -> torch.Tensor:
-        """fvmatrix.py:292-323 -- deterministic gather SpMV."""
+        """fvmatrix.py:292-323 -- deterministic gather SpMV; differentiable
+        with respect to x and the LDU entries when any of them requires grad
+        (then also H)."""
+        if torch.is_grad_enabled() and any(t.requires_grad for t in (x, self._diag, self._upper, self._lower)):
+            return _MultiplyFn.apply(self, self._diag, self.upper, self.lower, x)
         nc = 1 if x.dim() == 1 else x.shape[1]
         xc = x.contiguous()
         y = torch.empty_like(xc)
@@ -245,3 +249,43 @@ class Equation:
 
 def equation(target: CellField, fv_matrix: FvMatrix) -> Equation:
     return Equation(target, fv_matrix)
+
+
+class _MultiplyFn(torch.autograd.Function):
+    """y = A x as an autograd node.  Backward: x gets A^T g (the same SpMV
+    with upper and lower exchanged); the LDU entries get g_diag = g x,
+    g_upper[f] = g_O x_N, g_lower[f] = g_N x_O (gf_ldu_grads with lambda =
+    -g, summed over components)."""
+
+    @staticmethod
+    def forward(ctx, mat, diag, upper, lower, x):  # noqa: D401
+        ctx.mat = mat
+        ctx.save_for_backward(x)
+        nc = 1 if x.dim() == 1 else x.shape[1]
+        xc = x.contiguous()
+        y = torch.empty_like(xc)
+        nat.call("gf_ldu_multiply", mat.grid.mesh_ref, ctypes.c_int32(nc), nat.ptr(mat._diag), nat.ptr(mat._upper),
+                 nat.ptr(mat._lower), nat.ptr(xc), nat.ptr(y), nat.stream())
+        return y
+
+    @staticmethod
+    def backward(ctx, g_y):
+        (x,) = ctx.saved_tensors
+        mat, need = ctx.mat, ctx.needs_input_grad
+        g = mat.grid
+        nc = 1 if x.dim() == 1 else x.shape[1]
+        gy = g_y.contiguous()
+        g_x = None
+        if need[4]:
+            g_x = torch.empty_like(gy)
+            nat.call("gf_ldu_multiply", g.mesh_ref, ctypes.c_int32(nc), nat.ptr(mat._diag), nat.ptr(mat._lower),
+                     nat.ptr(mat._upper), nat.ptr(gy), nat.ptr(g_x), nat.stream())
+        gd = gu = gl = None
+        if need[1] or need[2] or need[3]:
+            gd, gu, gl = (torch.empty_like(t) for t in (mat._diag, mat._upper, mat._lower))
+            lam = (-gy).contiguous()
+            nat.call("gf_ldu_grads", g.mesh_ref, ctypes.c_int32(nc), nat.ptr(lam), nat.ptr(x.contiguous()),
+                     nat.ptr(gd), nat.ptr(gu), nat.ptr(gl), nat.stream())
+            F = g.num_internal_faces
+            gu, gl = gu[:F], gl[:F]
+        return None, gd, gu, gl, g_x

@@ -245,3 +245,42 @@ def test_reconstruct_vjp_matches_central_differences(name):
     phi.single_data = base["s"].clone().requires_grad_()
     phi.domain_bnd_data = base["b"]
     assert torch.equal(fvc.reconstruct(phi, U).detach(), ref)
+
+
+@pytest.mark.parametrize("cs", [(), (3,)])
+def test_matrix_h_and_a_vjp_match_central_differences(cs):
+    """FvMatrix.H(x) and A() (core/fvmatrix.py:242-290) of an assembled
+    ddt + div - laplacian matrix, with respect to the diffusivity (through
+    the assembly's reverse mode) and x: the SpMV's reverse mode is the
+    transposed SpMV and the LDU outer products."""
+    from gridfoam_amd.fields import CellField, DirichletBC, FaceField, NeumannBC
+    from gridfoam_amd.fv import fvm
+
+    g = _grid("refined3d")
+    dev = g.device
+    gen = torch.Generator(dev).manual_seed(29)
+    _say(f"matrix H/A vjp cs={cs}")
+    F, B = g.num_internal_faces, g.num_domain_bnd_faces
+    q = CellField(g, f"qH{len(cs)}", cs)
+    q.add_boundary_conditions({"x_minus": DirichletBC(torch.ones(cs, dtype=torch.float64, device=dev)),
+                               "x_plus": NeumannBC(torch.zeros(cs, dtype=torch.float64, device=dev))})
+    q.data = torch.randn(g.num_cells, *cs, dtype=torch.float64, device=dev, generator=gen)
+    q.update_history(reset=True)
+    phi = FaceField(g, "phi_H")
+    phi.single_data = torch.randn(F, dtype=torch.float64, device=dev, generator=gen) * 0.01
+    phi.domain_bnd_data = torch.zeros(B, dtype=torch.float64, device=dev)
+    base = {"gamma": torch.tensor(0.05, dtype=torch.float64, device=dev),
+            "x": torch.randn(g.num_cells, *cs, dtype=torch.float64, device=dev, generator=gen)}
+    W1 = torch.randn(g.num_cells, *cs, dtype=torch.float64, device=dev, generator=gen)
+    W2 = torch.randn(g.num_cells, dtype=torch.float64, device=dev, generator=gen)
+
+    def J(vals):
+        m = fvm.assemble(q, ddt=1.0, div=1.0, lap=-1.0, phi=phi, gamma=vals["gamma"])
+        return (m.H(vals["x"]) * W1).sum() + (m.A() * W2).sum()
+
+    _check(J, base, gen, dev)
+    with torch.no_grad():
+        m = fvm.assemble(q, ddt=1.0, div=1.0, lap=-1.0, phi=phi, gamma=base["gamma"])
+        ref = m.H(base["x"]).clone()
+    got = m.H(base["x"].clone().requires_grad_())
+    assert torch.equal(got.detach(), ref)
