@@ -196,34 +196,54 @@ def _traffic(fp32: bool, cells: int):
     return None
 
 
-def cpu_baseline(gpu_p_iters_per_step: float) -> dict:
-    """FP64 oracle on the host: one PISO step of the same case with the
-    pressure CG capped at 0 iterations, plus 20 timed CG iterations on that
-    step's pressure matrix, scaled to the GPU run's CG iterations per step."""
-    import numpy as np
+def _cpu_model() -> str:
+    try:
+        for line in pathlib.Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(steps: int) -> dict:
+    """The FP64 oracle (the CPU restatement of the reference's path) on the
+    host: `steps` complete PISO steps of the same case from the same initial
+    state, no iteration cap and no scaling; setup (mesh + algorithm) timed
+    apart from the step loop, as solve_s of tests/profile/case.py:149-205.
+    The Krylov multiplies run through a CSR copy on N = len(sched_getaffinity)
+    torch threads; numpy's elementwise work is single-threaded (BLAS limited
+    to one thread so it does not compete with torch's pool)."""
+    import threadpoolctl
+    import torch
 
     from oracle import fv_oracle as fo
 
-    cfg = _cfg(False)
-    for k in ("p", "pFinal"):
-        cfg["simulator"]["fvSolution"]["solvers"][k]["max_iter"] = 0
-    case = fo.Case(cfg)
-    t0 = time.perf_counter()
-    case.step()
-    t_step = time.perf_counter() - t0
-    g = case.g
-    rAU = 1.0 / case._momentum(True).A()
-    pm = -fo.fvm_laplacian(g, rAU, case.p, True)
-    pm.source = np.random.default_rng(0).standard_normal(g.num_cells)
-    scfg = fo.SolverCfg("cg", "jacobi", 0.0, 1e-30, 20)
-    t0 = time.perf_counter()
-    fo.solve(pm, np.zeros(g.num_cells), scfg)
-    t_it = (time.perf_counter() - t0) / 20
-    t_est = t_step + gpu_p_iters_per_step * t_it
-    return {"value": round(g.num_cells / t_est / 1e6, 4), "unit": "Mcell-steps/s", "cores": 1, "kind": "port",
-            "sample": (f"one PISO step of the same {g.num_cells}-cell case (pressure CG capped at 0 iterations, "
-                       f"{t_step:.1f} s) + 20 timed CG iterations ({t_it * 1e3:.0f} ms each), scaled to "
-                       f"{gpu_p_iters_per_step:.0f} CG iterations per step as on the GPU; numpy FP64, 1 thread")}
+    n = len(os.sched_getaffinity(0))
+    torch.set_num_threads(n)
+    fo.use_threaded_spmv(n)
+    try:
+        with threadpoolctl.threadpool_limits(1, user_api="blas"):
+            t0 = time.perf_counter()
+            case = fo.Case(_cfg(False))
+            setup_s = time.perf_counter() - t0
+            its = []
+            t0 = time.perf_counter()
+            for k in range(steps):
+                case.step()
+                its.append(int(case.stats["p"][0].iterations))
+                _log(f"cpu baseline: oracle step {k + 1}/{steps} done")
+            solve_s = time.perf_counter() - t0
+    finally:
+        fo.use_threaded_spmv(0)
+    C = case.g.num_cells
+    return {"value": round(C * steps / solve_s / 1e6, 5), "unit": "Mcell-steps/s", "cores": n, "kind": "port",
+            "cpu_count": os.cpu_count(), "cpu_model": _cpu_model(), "steps": steps,
+            "setup_s": round(setup_s, 2), "solve_s": round(solve_s, 2),
+            "sample": (f"{steps} complete PISO steps of the same {C}-cell case from the same initial state "
+                       f"(no iteration cap, no scaling; last pressure solve per step {its} CG iterations); "
+                       f"FP64 numpy oracle, Krylov SpMV on {n} torch threads (CSR), numpy elementwise "
+                       f"single-threaded")}
 
 
 def config1_measure(cpu: bool, cpu_iters: int = 100) -> dict:
@@ -315,6 +335,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--mode", choices=("both", "plain", "mixed"), default="both")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--cpu-steps", type=int, default=2, help="complete oracle PISO steps timed for the CPU baseline")
     ap.add_argument("--no-large", action="store_true", help="skip the 256^3 SpMV measurement")
     ap.add_argument("--no-config1", action="store_true", help="skip the 64x64 SIMPLE cavity measurement")
     ap.add_argument("--spmv-only", choices=("fp32", "fp64"), default=None,
@@ -424,10 +445,9 @@ def main() -> int:
     if rel is not None:
         line["mixed_vs_fp64_rel_l2"] = rel
         line["speedup_mixed_vs_fp64"] = round(out["plain"]["dt"] / out["mixed"]["dt"], 3)
-    if not a.no_cpu:
-        ref_iters = out["plain"]["iters"] / a.steps if "plain" in out else H["iters"] / a.steps
-        _log("cpu baseline (FP64 oracle, bounded sample)")
-        line["cpu_baseline"] = cpu_baseline(ref_iters)
+    if not a.no_cpu and a.cpu_steps > 0:
+        _log(f"cpu baseline: FP64 oracle, {a.cpu_steps} complete PISO steps")
+        line["cpu_baseline"] = cpu_baseline(a.cpu_steps)
     print(json.dumps(line), flush=True)
     return 0
 

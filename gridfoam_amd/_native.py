@@ -65,6 +65,7 @@ _SIGS: dict[str, tuple[Any, list[Any]]] = {
     "gf_grid_geometry": (_i32, [_i32, _i32, _i32] + [_vp] * 17 + [_vp]),
     "gf_face_geometry": (_i32, [_P(GfMesh), _vp, _vp, _vp, _vp, _vp, _f64, _vp]),
     "gf_row_counts": (_i32, [_P(GfMesh), _vp, _vp, _vp, _vp]),
+    "gf_mesh_validate": (_i32, [_P(GfMesh), _P(ctypes.c_int32), _vp]),
     "gf_exclusive_scan_i32": (_i32, [_vp, _vp, _i32, _vp]),
     "gf_fill_nbr_lists": (_i32, [_P(GfMesh), _vp, _vp]),
     "gf_sell_widths": (_i32, [_P(GfMesh), _vp, _vp]),

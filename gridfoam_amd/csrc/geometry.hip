@@ -108,6 +108,39 @@ __global__ void k_row_counts(const gf_mesh m, int32_t *own_cnt, int32_t *nbr_cnt
     if (f < m.n_bnd) atomicAdd(bnd_cnt + m.bnd_owner[f], 1);
 }
 
+// Face-order contract of the per-cell face lists (one bit per violated rule):
+//   1 an index out of range (cell, axis or direction) or owner == neighbour
+//   2 internal faces not grouped by owner in ascending order
+//   4 domain faces not grouped by owner in ascending order
+//   8 own_row / bnd_row do not bound each face's owner group
+//  16 the neighbour does not lie on the +axis side of the owner
+// Every face is checked by one thread; flags are OR-ed into one word.
+enum : int { kBadIndex = 1, kOwnerOrder = 2, kBndOrder = 4, kRowMismatch = 8, kNbrSide = 16 };
+__global__ void k_mesh_validate(const gf_mesh m, int32_t *flags) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    int bad = 0;
+    if (f < m.n_faces) {
+        const int o = m.owner[f], n = m.neighbour[f], a = m.axis[f];
+        if (o < 0 || o >= m.n_cells || n < 0 || n >= m.n_cells || o == n || a < 0 || a > 2) {
+            bad |= kBadIndex;
+        } else {
+            if (f > 0 && m.owner[f - 1] > o) bad |= kOwnerOrder;
+            if (m.own_row && (f < m.own_row[o] || f >= m.own_row[o + 1])) bad |= kRowMismatch;
+            if (m.centers && !(m.centers[3 * n + a] > m.centers[3 * o + a])) bad |= kNbrSide;
+        }
+    }
+    if (f < m.n_bnd) {
+        const int o = m.bnd_owner[f], d = m.bnd_dir[f];
+        if (o < 0 || o >= m.n_cells || d < 0 || d > 5) {
+            bad |= kBadIndex;
+        } else {
+            if (f > 0 && m.bnd_owner[f - 1] > o) bad |= kBndOrder;
+            if (m.bnd_row && (f < m.bnd_row[o] || f >= m.bnd_row[o + 1])) bad |= kRowMismatch;
+        }
+    }
+    if (bad) atomicOr(flags, bad);
+}
+
 // faces seen from their neighbour, grouped by cell; order fixed afterwards
 __global__ void k_nbr_scatter(const gf_mesh m, int32_t *cursor, int32_t *nbr_face) {
     int f = blockIdx.x * blockDim.x + threadIdx.x;
@@ -367,6 +400,42 @@ extern "C" int gf_row_counts(const gf_mesh *m, int32_t *own_cnt, int32_t *nbr_cn
     hipLaunchKernelGGL(k_row_counts, dim3(grid_for(n)), dim3(kBlock), 0, S(stream), *m, own_cnt, nbr_cnt,
                        bnd_cnt);
     GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
+extern "C" int gf_mesh_validate(const gf_mesh *m, int32_t *flags_out, gf_stream_t stream) {
+    GF_REQUIRE(m && flags_out, "null argument");
+    GF_REQUIRE(m->n_cells >= 1 && m->n_faces >= 0 && m->n_bnd >= 0, "bad mesh sizes");
+    GF_REQUIRE(m->owner && m->neighbour && m->axis && m->bnd_owner && m->bnd_dir, "mesh topology missing");
+    hipStream_t s = S(stream);
+    int32_t *flags = nullptr;
+    GF_CHECK_HIP(hipMalloc(&flags, sizeof(int32_t)));
+    int32_t h = 0;
+    hipError_t e = hipMemsetAsync(flags, 0, sizeof(int32_t), s);
+    const int n = m->n_faces > m->n_bnd ? m->n_faces : m->n_bnd;
+    if (e == hipSuccess && n > 0) {
+        hipLaunchKernelGGL(k_mesh_validate, dim3(grid_for(n)), dim3(kBlock), 0, s, *m, flags);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, flags, sizeof(int32_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    const hipError_t fe = hipFree(flags);
+    if (e == hipSuccess) e = fe;
+    if (e != hipSuccess) {
+        set_error(std::string("gf_mesh_validate: ") + hipGetErrorString(e));
+        return GF_EHIP;
+    }
+    *flags_out = h;
+    if (h) {
+        std::string msg = "gf_mesh violates the face-order contract:";
+        if (h & kBadIndex) msg += " index out of range or owner == neighbour;";
+        if (h & kOwnerOrder) msg += " internal faces not grouped by owner in ascending order;";
+        if (h & kBndOrder) msg += " domain faces not grouped by owner in ascending order;";
+        if (h & kRowMismatch) msg += " own_row/bnd_row do not match the face owners;";
+        if (h & kNbrSide) msg += " neighbour not on the +axis side of the owner;";
+        set_error(msg);
+        return GF_EINVAL;
+    }
     return GF_OK;
 }
 

@@ -1453,9 +1453,16 @@ struct Poller {
         GF_CHECK_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
         return GF_OK;
     }
+    // The last chunk's event may still be pending when a solve ends (the
+    // loop waits for the chunk before it); the owner keeps the Poller alive
+    // until the stream has been synchronised, and an early error exit waits
+    // here, so no event is destroyed while the GPU may still complete it.
     ~Poller() {
-        if (ev[0]) (void)hipEventDestroy(ev[0]);
-        if (ev[1]) (void)hipEventDestroy(ev[1]);
+        for (hipEvent_t e : ev)
+            if (e) {
+                (void)hipEventSynchronize(e);
+                (void)hipEventDestroy(e);
+            }
     }
     // cleared device address of the word the next chunk posts to
     int *next_word() const {
@@ -1498,6 +1505,11 @@ int finish_read(int nr, double *stats) {
 // their sequence number, so a graph captured at sequence s0 replays any chunk
 // that starts at s0 + 2k.  Graphs are cached per workspace (they embed its
 // pointers) and dropped by gf_krylov_release.
+// One mutex guards the cache AND every capture/instantiation: the host
+// threads of a serving run solve concurrently, and capture plus
+// instantiation are done by one thread at a time (a miss is captured while
+// the lock is held, so no two threads capture the same chunk either).
+// Replays (hipGraphLaunch) run outside the lock.
 struct GraphCache {
     std::mutex mu;
     std::unordered_map<const void *, std::vector<std::pair<std::string, hipGraphExec_t>>> by_ws;
@@ -1564,11 +1576,9 @@ int launch_cg_chunk(const void *ws, const Ops &o, const Layout &L, bool mixed, i
     key_add(key, poll);
     GraphCache &c = graph_cache();
     hipGraphExec_t exec = nullptr;
-    {
-        std::lock_guard<std::mutex> g(c.mu);
-        for (auto &kv : c.by_ws[ws])
-            if (kv.first == key) exec = kv.second;
-    }
+    std::unique_lock<std::mutex> lock(c.mu);
+    for (auto &kv : c.by_ws[ws])
+        if (kv.first == key) exec = kv.second;
     if (!exec) {
         // Capture on a side stream of this thread (the caller's stream may be
         // the legacy default stream, which cannot be captured); the graph is
@@ -1591,16 +1601,16 @@ int launch_cg_chunk(const void *ws, const Ops &o, const Layout &L, bool mixed, i
             set_error(std::string("cg chunk graph instantiation failed: ") + hipGetErrorString(ie));
             return GF_EHIP;
         }
-        std::lock_guard<std::mutex> g(c.mu);
         c.by_ws[ws].emplace_back(key, exec);
     }
+    lock.unlock();
     GF_CHECK_HIP(hipGraphLaunch(exec, st));
     return GF_OK;
 }
 
 template <int NR, int TB>
-int run_cg(const void *ws, Ops o, const Layout &L, const gf_krylov_cfg &cfg, hipStream_t st, int *s_last) {
-    Poller P;
+int run_cg(const void *ws, Ops o, const Layout &L, const gf_krylov_cfg &cfg, hipStream_t st, Poller &P,
+           int *s_last) {
     if (int rc = P.init()) return rc;
     // iterations per chunk: even, so every chunk starts with p_old in buffer 0
     const int every = cfg.check_every > 1 ? (cfg.check_every + 1) & ~1 : 2;
@@ -1640,9 +1650,8 @@ int run_cg(const void *ws, Ops o, const Layout &L, const gf_krylov_cfg &cfg, hip
 }
 
 template <int NR>
-int run_bicgstab(Ops o, const Layout &L, const gf_krylov_cfg &cfg, hipStream_t st, int *s_last) {
+int run_bicgstab(Ops o, const Layout &L, const gf_krylov_cfg &cfg, hipStream_t st, Poller &P, int *s_last) {
     const int nb = o.nb;
-    Poller P;
     if (int rc = P.init()) return rc;
     const int every = cfg.check_every > 0 ? cfg.check_every : 4;
     int s = 1;
@@ -1739,9 +1748,10 @@ extern "C" int gf_krylov_solve(const gf_mesh *m, const gf_krylov_cfg *cfg, int32
     o.partials = L.partials;
     int rc;
     int s_last = 0;
+    Poller P;  // outlives the final stream synchronisation below
     if (cfg->method == 0) {
         auto cg = [&](auto nr, auto tb) {
-            return run_cg<decltype(nr)::value, decltype(tb)::value>(workspace, o, L, *cfg, s, &s_last);
+            return run_cg<decltype(nr)::value, decltype(tb)::value>(workspace, o, L, *cfg, s, P, &s_last);
         };
         using I1 = std::integral_constant<int, 1>;
         using I3 = std::integral_constant<int, 3>;
@@ -1756,7 +1766,7 @@ extern "C" int gf_krylov_solve(const gf_mesh *m, const gf_krylov_cfg *cfg, int32
             rc = nrhs == 1 ? cg(I1{}, T256{}) : cg(I3{}, T256{});
     }
     else
-        rc = nrhs == 1 ? run_bicgstab<1>(o, L, *cfg, s, &s_last) : run_bicgstab<3>(o, L, *cfg, s, &s_last);
+        rc = nrhs == 1 ? run_bicgstab<1>(o, L, *cfg, s, P, &s_last) : run_bicgstab<3>(o, L, *cfg, s, P, &s_last);
     if (rc != GF_OK) return rc;
     hipLaunchKernelGGL(k_solve_out, dim3(grid_for(C)), dim3(kBlock), 0, s, C, nrhs, L.x, x,
                        L.st + (s_last & 1) * kMaxRhs, hs->dev_state);

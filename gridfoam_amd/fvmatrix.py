@@ -142,7 +142,7 @@ class FvMatrix:
         with respect to x and the LDU entries when any of them requires grad
         (then also H)."""
         if torch.is_grad_enabled() and any(t.requires_grad for t in (x, self._diag, self._upper, self._lower)):
-            return _MultiplyFn.apply(self, self._diag, self.upper, self.lower, x)
+            return _MultiplyFn.apply(self.grid, self._diag, self._upper, self._lower, x)
         nc = 1 if x.dim() == 1 else x.shape[1]
         xc = x.contiguous()
         y = torch.empty_like(xc)
@@ -258,34 +258,34 @@ class _MultiplyFn(torch.autograd.Function):
     -g, summed over components)."""
 
     @staticmethod
-    def forward(ctx, mat, diag, upper, lower, x):  # noqa: D401
-        ctx.mat = mat
-        ctx.save_for_backward(x)
-        nc = 1 if x.dim() == 1 else x.shape[1]
+    def forward(ctx, grid, diag, upper, lower, x):  # noqa: D401
+        # upper/lower are the matrix's full storage ([max(F, 1)]); they are
+        # saved, so an in-place change before backward raises in autograd
+        # instead of giving a silently wrong A^T g
+        ctx.grid = grid
         xc = x.contiguous()
+        ctx.save_for_backward(diag, upper, lower, xc)
+        nc = 1 if x.dim() == 1 else x.shape[1]
         y = torch.empty_like(xc)
-        nat.call("gf_ldu_multiply", mat.grid.mesh_ref, ctypes.c_int32(nc), nat.ptr(mat._diag), nat.ptr(mat._upper),
-                 nat.ptr(mat._lower), nat.ptr(xc), nat.ptr(y), nat.stream())
+        nat.call("gf_ldu_multiply", grid.mesh_ref, ctypes.c_int32(nc), nat.ptr(diag), nat.ptr(upper),
+                 nat.ptr(lower), nat.ptr(xc), nat.ptr(y), nat.stream())
         return y
 
     @staticmethod
     def backward(ctx, g_y):
-        (x,) = ctx.saved_tensors
-        mat, need = ctx.mat, ctx.needs_input_grad
-        g = mat.grid
+        diag, upper, lower, x = ctx.saved_tensors
+        need, g = ctx.needs_input_grad, ctx.grid
         nc = 1 if x.dim() == 1 else x.shape[1]
         gy = g_y.contiguous()
         g_x = None
         if need[4]:
             g_x = torch.empty_like(gy)
-            nat.call("gf_ldu_multiply", g.mesh_ref, ctypes.c_int32(nc), nat.ptr(mat._diag), nat.ptr(mat._lower),
-                     nat.ptr(mat._upper), nat.ptr(gy), nat.ptr(g_x), nat.stream())
+            nat.call("gf_ldu_multiply", g.mesh_ref, ctypes.c_int32(nc), nat.ptr(diag), nat.ptr(lower),
+                     nat.ptr(upper), nat.ptr(gy), nat.ptr(g_x), nat.stream())
         gd = gu = gl = None
         if need[1] or need[2] or need[3]:
-            gd, gu, gl = (torch.empty_like(t) for t in (mat._diag, mat._upper, mat._lower))
+            gd, gu, gl = torch.empty_like(diag), torch.zeros_like(upper), torch.zeros_like(lower)
             lam = (-gy).contiguous()
-            nat.call("gf_ldu_grads", g.mesh_ref, ctypes.c_int32(nc), nat.ptr(lam), nat.ptr(x.contiguous()),
+            nat.call("gf_ldu_grads", g.mesh_ref, ctypes.c_int32(nc), nat.ptr(lam), nat.ptr(x),
                      nat.ptr(gd), nat.ptr(gu), nat.ptr(gl), nat.stream())
-            F = g.num_internal_faces
-            gu, gl = gu[:F], gl[:F]
         return None, gd, gu, gl, g_x

@@ -47,7 +47,14 @@ def _scan(cnt: torch.Tensor, n: int) -> torch.Tensor:
 class OctreeGrid:
     """Device-resident octree grid (no immersed boundary)."""
 
-    def __init__(self, fluxel: FluxelConfig, *, dt: float = 1.0, device: str | torch.device = "cuda"):
+    def __init__(self, fluxel: FluxelConfig, *, dt: float = 1.0, device: str | torch.device = "cuda",
+                 topology: dict | None = None):
+        """Build the octree on the GPU from the fluxel block, or adopt a
+        given integer topology (``topology``: level [C], ijk [C,3], owner,
+        neighbour, axis [F], bnd_owner, bnd_dir [B]; e.g. a fluxel mesh),
+        whose faces may come in any order: they are brought into the
+        face-order contract by a stable sort by owner, and ``face_order`` /
+        ``bnd_face_order`` record the given index of every face."""
         nat.require_gpu()
         self.device = torch.device(device)
         self.dtype = torch.float64
@@ -58,10 +65,19 @@ class OctreeGrid:
         # register on construction; conditions look up the fields they read
         self._cellfields: "weakref.WeakValueDictionary" = weakref.WeakValueDictionary()
         self._facefields: "weakref.WeakValueDictionary" = weakref.WeakValueDictionary()
-        dev = self.device
         lo = (ctypes.c_double * 3)(*fluxel.domain.lower)
         hi = (ctypes.c_double * 3)(*fluxel.domain.upper)
         root = (ctypes.c_int32 * 3)(*fluxel.root_resolution)
+        if topology is None:
+            self._build_topology(fluxel, lo, hi, root)
+        else:
+            self._adopt_topology(topology)
+        self.num_immersed_faces = 0
+        self._build_geometry(lo, hi, root)
+        self._build_operator()
+
+    def _build_topology(self, fluxel: FluxelConfig, lo, hi, root) -> None:
+        dev = self.device
         regs = fluxel.refinement_regions
         nr = len(regs)
         rmin = (ctypes.c_double * (3 * nr + 3))(*[v for r in regs for v in r.min])
@@ -73,7 +89,6 @@ class OctreeGrid:
                  ctypes.c_int32(fluxel.n_leaf_refinement), ctypes.byref(handle), counts, nat.stream())
         C, F, B, lmax = (int(v) for v in counts)
         self.num_cells, self.num_internal_faces, self.num_domain_bnd_faces, self.lmax = C, F, B, lmax
-        self.num_immersed_faces = 0
         self.level = _i32(C, dev)
         self.ijk = torch.zeros((max(C, 1), 3), dtype=torch.int32, device=dev)
         self.owner_i32 = _i32(F, dev)
@@ -87,8 +102,31 @@ class OctreeGrid:
                 self.bnd_owner_i32, self.bnd_dir_i32)), nat.stream())
         finally:
             nat.lib().gf_octree_free(handle)
-        self._build_geometry(lo, hi, root)
-        self._build_operator()
+        self.face_order = None
+        self.bnd_face_order = None
+
+    def _adopt_topology(self, t: dict) -> None:
+        dev = self.device
+
+        def i32(x, n, shape=None):
+            x = torch.as_tensor(x).to(device=dev, dtype=torch.int32)
+            out = _i32(n, dev) if shape is None else torch.zeros(shape, dtype=torch.int32, device=dev)
+            out[: x.shape[0]] = x
+            return out
+
+        level = torch.as_tensor(t["level"])
+        C, F, B = int(level.shape[0]), int(len(t["owner"])), int(len(t["bnd_owner"]))
+        self.num_cells, self.num_internal_faces, self.num_domain_bnd_faces = C, F, B
+        self.lmax = int(level.max()) if C else 0
+        order, bnd_order = canonical_face_order(torch.as_tensor(t["owner"]), torch.as_tensor(t["bnd_owner"]))
+        self.face_order, self.bnd_face_order = order.to(dev), bnd_order.to(dev)
+        self.level = i32(level, C)
+        self.ijk = i32(t["ijk"], C, (max(C, 1), 3))
+        self.owner_i32 = i32(torch.as_tensor(t["owner"])[order], F)
+        self.neighbour_i32 = i32(torch.as_tensor(t["neighbour"])[order], F)
+        self.axis_i32 = i32(torch.as_tensor(t["axis"])[order], F)
+        self.bnd_owner_i32 = i32(torch.as_tensor(t["bnd_owner"])[bnd_order], B)
+        self.bnd_dir_i32 = i32(torch.as_tensor(t["bnd_dir"])[bnd_order], B)
 
     # ------------------------------------------------------------------ build
     def _build_geometry(self, lo, hi, root) -> None:
@@ -134,6 +172,8 @@ class OctreeGrid:
         self.nbr_row = _scan(nbr_cnt, C + 1)
         self.bnd_row = _scan(bnd_cnt, C + 1)
         m.own_row, m.nbr_row, m.bnd_row = (nat.ptr(t) for t in (self.own_row, self.nbr_row, self.bnd_row))
+        flags = ctypes.c_int32(0)
+        nat.call("gf_mesh_validate", mp, ctypes.byref(flags), st)  # raises on a face-order violation
         self.nbr_face = _i32(F, dev)
         m.nbr_face = nat.ptr(self.nbr_face)
         nat.call("gf_fill_nbr_lists", mp, nat.ptr(self.nbr_face), st)
@@ -266,6 +306,15 @@ class OctreeGrid:
             "bnd_owner": self.bnd_owner_i32[:B].cpu().numpy(),
             "bnd_dir": self.bnd_dir_i32[:B].cpu().numpy(),
         }
+
+
+def canonical_face_order(owner: torch.Tensor, bnd_owner: torch.Tensor) -> tuple[torch.Tensor, torch.Tensor]:
+    """Permutations (new index -> given index) that group internal and domain
+    faces by owner in ascending order, keeping the given order inside a
+    group (stable sort): the face-order contract of gf_mesh."""
+    o = torch.as_tensor(owner).long().cpu()
+    b = torch.as_tensor(bnd_owner).long().cpu()
+    return torch.sort(o, stable=True).indices, torch.sort(b, stable=True).indices
 
 
 def create_grid(config: GridfoamConfig, device: str | torch.device = "cuda") -> OctreeGrid:

@@ -33,10 +33,17 @@ typedef struct ihipStream_t *gf_stream_t; /* == hipStream_t */
 #define GF_ENOMEM (-3)
 
 /* Device-resident face-neighbour operator.  Filled by the host from
- * buffers it owns (gridfoam_amd/core/grid.py); passed by pointer to every
+ * buffers it owns (gridfoam_amd/grid.py); passed by pointer to every
  * call.  Replaces the index/geometry tensors of GridBase
  * (src/gridfoam/core/grid/base.py:19-275) and the cached FaceGeometry
- * (src/gridfoam/fv/kernels/face_geometry.py:20-72). */
+ * (src/gridfoam/fv/kernels/face_geometry.py:20-72).
+ *
+ * Face-order contract: internal faces are grouped by owner in ascending
+ * owner order (own_row[c] .. own_row[c+1] are cell c's owned faces), and
+ * domain faces likewise by bnd_owner (bnd_row).  The neighbour lies on the
+ * +axis side of the owner.  Faces in another order (fluxel's order is not
+ * specified) are brought into this order by a stable sort by owner, e.g.
+ * gridfoam_amd.grid.canonical_face_order; gf_mesh_validate checks it. */
 typedef struct gf_mesh {
     int32_t n_cells, n_faces, n_bnd, n_hang, n_hang_cells, n_slices;
     int64_t nnz_pad; /* padded entries of the sliced-ELL operator */
@@ -97,6 +104,13 @@ int gf_face_geometry(const gf_mesh *m, const double *face_centers, double *w, do
                      double *delta, uint8_t *is_hang, double h_min, gf_stream_t stream);
 int gf_row_counts(const gf_mesh *m, int32_t *own_cnt, int32_t *nbr_cnt, int32_t *bnd_cnt,
                   gf_stream_t stream);
+/* Checks the face-order contract above (index ranges; internal and domain
+ * faces grouped by owner, ascending; own_row/bnd_row when set; neighbour on
+ * the +axis side when centers is set).  flags_out (host) = OR of 1 index,
+ * 2 internal order, 4 domain order, 8 row lists, 16 neighbour side;
+ * GF_EINVAL with a message when any is set.  Guards the boundary that
+ * replaces GridBase's face lists (core/grid/base.py:19-275). */
+int gf_mesh_validate(const gf_mesh *m, int32_t *flags_out, gf_stream_t stream);
 int gf_exclusive_scan_i32(const int32_t *in, int32_t *out, int32_t n, gf_stream_t stream);
 int gf_fill_nbr_lists(const gf_mesh *m, int32_t *nbr_face, gf_stream_t stream);
 int gf_sell_widths(const gf_mesh *m, int32_t *slice_len, gf_stream_t stream);

@@ -796,9 +796,42 @@ def _inv_diag(m: Matrix, kind: str) -> np.ndarray:
     return np.where(np.abs(m.diag) < eps, 1.0, 1.0 / safe)
 
 
-def cg_single(m: Matrix, b: np.ndarray, x: np.ndarray, cfg: SolverCfg, inv_d: np.ndarray):
+# Host-timing aid: with threads > 0 the Krylov solves multiply through a CSR
+# copy of the LDU matrix (built once per solve) on that many torch CPU
+# threads.  Same products; each row is summed in column order instead of
+# face order, so results agree with the LDU multiply to rounding.  Off (0)
+# for every parity use: the expected answers use the LDU multiply.
+_SPMV_THREADS = 0
+
+
+def use_threaded_spmv(threads: int) -> None:
+    global _SPMV_THREADS
+    _SPMV_THREADS = int(threads)
+
+
+def _operator(m: Matrix):
+    """The multiply of a Krylov solve: the LDU multiply, or its CSR copy."""
+    if _SPMV_THREADS <= 0:
+        return m.multiply
+    import torch
+
+    torch.set_num_threads(_SPMV_THREADS)
+    g, C = m.g, m.g.num_cells
+    rows = np.concatenate([np.arange(C), g.owner, g.neighbour])
+    cols = np.concatenate([np.arange(C), g.neighbour, g.owner])
+    vals = np.concatenate([m.diag, m.upper, m.lower])
+    order = np.lexsort((cols, rows))
+    indptr = np.zeros(C + 1, np.int64)
+    np.cumsum(np.bincount(rows, minlength=C), out=indptr[1:])
+    A = torch.sparse_csr_tensor(torch.from_numpy(indptr), torch.from_numpy(cols[order].astype(np.int64)),
+                                torch.from_numpy(vals[order]), size=(C, C))
+    return lambda x: (A @ torch.from_numpy(np.ascontiguousarray(x))).numpy()
+
+
+def cg_single(m: Matrix, b: np.ndarray, x: np.ndarray, cfg: SolverCfg, inv_d: np.ndarray, op=None):
     """solvers/cg.py:75-174."""
-    r = b - m.multiply(x)
+    mul = op or m.multiply
+    r = b - mul(x)
     nr0 = _norm(r, cfg.norm_type)
     thresh = max(cfg.tolerance, cfg.rel_tolerance * nr0)
     if nr0 < thresh:
@@ -807,25 +840,29 @@ def cg_single(m: Matrix, b: np.ndarray, x: np.ndarray, cfg: SolverCfg, inv_d: np
     p = z.copy()
     rz_old = np.dot(r, z)
     final, conv, it = nr0, False, 0
+    tmp = np.empty_like(p)
+    # in-place updates: the same IEEE operations as x + alpha p, r - alpha Ap,
+    # z + p beta, without a temporary allocation per operation
     for it in range(1, cfg.max_iter + 1):
-        Ap = m.multiply(p)
+        Ap = mul(p)
         alpha = rz_old / np.dot(p, Ap)
-        x = x + alpha * p
-        r = r - alpha * Ap
+        np.add(x, np.multiply(p, alpha, out=tmp), out=x)
+        np.subtract(r, np.multiply(Ap, alpha, out=tmp), out=r)
         final = _norm(r, cfg.norm_type)
         if final < thresh:
             conv = True
             break
-        z = inv_d * r
+        np.multiply(inv_d, r, out=z)
         rz_new = np.dot(r, z)
-        p = z + p * (rz_new / rz_old)
+        np.add(z, np.multiply(p, rz_new / rz_old, out=p), out=p)
         rz_old = rz_new
     return x, Stats("cg", nr0, final, it, conv)
 
 
-def bicgstab_single(m: Matrix, b: np.ndarray, x: np.ndarray, cfg: SolverCfg, inv_d: np.ndarray):
+def bicgstab_single(m: Matrix, b: np.ndarray, x: np.ndarray, cfg: SolverCfg, inv_d: np.ndarray, op=None):
     """solvers/bicgstab.py:76-272."""
-    r = b - m.multiply(x)
+    mul = op or m.multiply
+    r = b - mul(x)
     nr0 = _norm(r, cfg.norm_type)
     thresh = max(cfg.tolerance, cfg.rel_tolerance * nr0)
     if nr0 < thresh:
@@ -841,7 +878,7 @@ def bicgstab_single(m: Matrix, b: np.ndarray, x: np.ndarray, cfg: SolverCfg, inv
     restarts, final, conv, it = 0, nr0, False, 0
     for it in range(1, cfg.max_iter + 1):
         y = inv_d * p
-        v = m.multiply(y)
+        v = mul(y)
         den = np.dot(r_hat, v)
         if abs(den) <= 1e-6 * l2(r_hat) * l2(v):
             if restarts >= cfg.max_restart:
@@ -857,7 +894,7 @@ def bicgstab_single(m: Matrix, b: np.ndarray, x: np.ndarray, cfg: SolverCfg, inv
             conv = True
             break
         z = inv_d * s
-        t = m.multiply(z)
+        t = mul(z)
         dom = np.dot(t, t)
         if abs(dom) <= 1e-6 * l2(t) * l2(t):
             if restarts >= cfg.max_restart:
@@ -895,12 +932,13 @@ def solve(m: Matrix, x0: np.ndarray, cfg: SolverCfg):
     """solvers/krylov.py:34-82 -- one shared preconditioner, per component."""
     inv_d = _inv_diag(m, cfg.preconditioner)
     single = cg_single if cfg.method == "cg" else bicgstab_single
+    op = _operator(m)
     if x0.ndim == 1:
-        x, st = single(m, m.source, x0.copy(), cfg, inv_d)
+        x, st = single(m, m.source, x0.copy(), cfg, inv_d, op)
         return x, (st,)
     xs, sts = [], []
     for k in range(x0.shape[1]):
-        x, st = single(m, m.source[:, k], x0[:, k].copy(), cfg, inv_d)
+        x, st = single(m, m.source[:, k], x0[:, k].copy(), cfg, inv_d, op)
         xs.append(x)
         sts.append(st)
     return np.stack(xs, axis=1), tuple(sts)
