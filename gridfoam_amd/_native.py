@@ -102,6 +102,7 @@ _SIGS: dict[str, tuple[Any, list[Any]]] = {
     "gf_set_reference": (_i32, [_vp, _vp, _i32, _f64, _vp]),
     "gf_axpby": (_i32, [_i64, _f64, _vp, _f64, _vp, _vp, _vp]),
     "gf_initial_residual": (_i32, [_P(GfMesh), _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "gf_initial_residual_noalloc": (_i32, [_P(GfMesh), _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gf_krylov_workspace_bytes": (_i64, [_P(GfMesh), _i32, _P(GfKrylovCfg)]),
     "gf_krylov_solve": (_i32, [_P(GfMesh), _P(GfKrylovCfg), _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gf_krylov_use_graphs": (_i32, [_i32]),

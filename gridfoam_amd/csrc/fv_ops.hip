@@ -8,6 +8,7 @@
 // then neighbour-side sum, then boundary terms).
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "gf_common.h"
@@ -1717,6 +1718,58 @@ extern "C" int gf_axpby(int64_t n, double a, const double *x, double b, const do
     GF_REQUIRE(x && y && out, "null argument");
     if (n == 0) return GF_OK;
     hipLaunchKernelGGL(k_axpby, dim3(grid_for(n)), dim3(kBlock), 0, S(stream), n, a, x, b, y, out);
+    GF_CHECK_LAUNCH();
+    return GF_OK;
+}
+
+// ---- allocation-free initial residual -------------------------------------
+// The per-workgroup partials of k_init_res live in a file-scope device array
+// (one copy per GPU, held until the process ends) instead of a caller
+// buffer: 2 x 2^17 doubles (2 MiB), enough for 2^17 workgroups of 256 cells,
+// i.e. up to 33,554,432 cells.  Every workgroup writes its two partials
+// before k_init_res_fin reads exactly those, so nothing needs clearing; calls
+// that overlap on one GPU share the array and race.
+constexpr int kNoAllocBlocks = 1 << 17;
+static __device__ __align__(256) double scratch[2 * kNoAllocBlocks];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device: looked up once per
+// device with that device current (the array's name is only a host-side
+// stand-in and must never be passed to a kernel).
+int scratch_address(double **out) {
+    int dev = 0;
+    GF_CHECK_HIP(hipGetDevice(&dev));
+    GF_REQUIRE(dev >= 0 && dev < kMaxDevices, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        GF_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        GF_REQUIRE(p != nullptr, "scratch symbol has no device address");
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return GF_OK;
+}
+}  // namespace
+
+extern "C" int gf_initial_residual_noalloc(const gf_mesh *m, int32_t nc, const double *diag, const double *upper,
+                                           const double *lower, const double *source, const double *psi,
+                                           double *out, gf_stream_t stream) {
+    GF_REQUIRE(m && diag && upper && lower && source && psi && out, "null argument");
+    GF_REQUIRE(nc == 1 || nc == 3, "nc must be 1 or 3");
+    GF_REQUIRE(m->n_cells >= 1, "empty grid");
+    const int nb = grid_for(m->n_cells);
+    GF_REQUIRE(nb <= kNoAllocBlocks,
+               "gf_initial_residual_noalloc: n_cells exceeds the static scratch capacity (33,554,432 cells)");
+    double *sp = nullptr;
+    if (int rc = scratch_address(&sp)) return rc;
+    hipStream_t s = S(stream);
+    GF_NC_DISPATCH(nc, k_init_res, nb, *m, diag, upper, lower, source, psi, sp);
+    hipLaunchKernelGGL(k_init_res_fin, dim3(1), dim3(1024), 0, s, sp, nb, out);
     GF_CHECK_LAUNCH();
     return GF_OK;
 }

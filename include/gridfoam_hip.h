@@ -317,6 +317,15 @@ int gf_axpby(int64_t n, double a, const double *x, double b, const double *y, do
 int gf_initial_residual(const gf_mesh *m, int32_t nc, const double *diag, const double *upper,
                         const double *lower, const double *source, const double *psi,
                         double *scratch, double *out, gf_stream_t stream);
+/* gf_initial_residual without the scratch argument and with no allocation of
+ * any kind: its per-workgroup partials live in a static device array of the
+ * library (2 MiB per GPU, held until the process ends), so n_cells is limited
+ * to 33,554,432 (GF_EINVAL beyond).  No counterpart in the reference: it is
+ * for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops).  Calls that overlap on one GPU share the array and race. */
+int gf_initial_residual_noalloc(const gf_mesh *m, int32_t nc, const double *diag, const double *upper,
+                                const double *lower, const double *source, const double *psi,
+                                double *out, gf_stream_t stream);
 
 /* ---- Krylov solves (solvers/cg.py:75-174, bicgstab.py:76-272,
  * preconditioners.py:76-96, krylov.py:34-82, base.py:140-162) -------------
