@@ -104,40 +104,38 @@ def uniform_grid(n: int):
 
 def spmv_roofline(grid, pmat, fp32: bool, launches: int = 200) -> dict:
     """Average duration of the PCG direction/SpMV kernel k_cg_dir (the
-    solver's own kernel, launched on its own by gf_sell_spmv), timed with
-    HIP events on the stream it is launched on."""
+    solver's own kernel on the solver's layout, launched on its own by
+    gf_pcg_dir_bench), timed with HIP events on the stream it is launched on."""
     import ctypes
 
     import torch
 
     from gridfoam_amd import _native as nat
 
-    dt = torch.float32 if fp32 else torch.float64
-    val = torch.empty(grid.mesh.nnz_pad, dtype=dt, device=grid.device)
-    nat.call("gf_sell_pack", grid.mesh_ref, ctypes.c_int32(0), nat.ptr(pmat._upper), nat.ptr(pmat._lower),
-             nat.ptr(val), ctypes.c_int32(int(fp32)), nat.stream())
-    diag = pmat._diag.contiguous()  # FP64 in both operators
     gen = torch.Generator(device="cpu").manual_seed(1)
-    x = torch.randn(grid.num_cells, generator=gen, dtype=torch.float64).to(device=grid.device, dtype=dt)
+    x = torch.randn(grid.num_cells, generator=gen, dtype=torch.float64).to(device=grid.device)
     y = torch.empty_like(x)
-    scratch = torch.empty(int(nat.lib().gf_sell_spmv_scratch_bytes(grid.mesh_ref)), dtype=torch.uint8,
+    scratch = torch.empty(int(nat.lib().gf_pcg_dir_bench_bytes(grid.mesh_ref, int(fp32))), dtype=torch.uint8,
                           device=grid.device)
 
-    def run(reps: int) -> None:
-        nat.call("gf_sell_spmv", grid.mesh_ref, nat.ptr(diag), nat.ptr(val), nat.ptr(x), nat.ptr(y),
-                 ctypes.c_int32(int(fp32)), ctypes.c_int32(reps), nat.ptr(scratch), nat.stream())
+    def bench(phase: int, reps: int = 1) -> None:
+        nat.call("gf_pcg_dir_bench", grid.mesh_ref, ctypes.c_int32(phase), ctypes.c_int32(int(fp32)),
+                 ctypes.c_int32(reps), nat.ptr(pmat._diag), nat.ptr(pmat._upper), nat.ptr(pmat._lower), nat.ptr(x),
+                 nat.ptr(y), nat.ptr(scratch), nat.stream())
 
-    run(10)
-    s = torch.cuda.current_stream()  # the stream gf_sell_spmv launches on
+    bench(0)
+    bench(1, 10)
+    s = torch.cuda.current_stream()  # the stream gf_pcg_dir_bench launches on
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(s)
-    run(launches)
+    bench(1, launches)
     e1.record(s)
     torch.cuda.synchronize()
-    avg_s = e0.elapsed_time(e1) / 1e3 / launches  # + one tiny state set-up kernel per `launches`
+    avg_s = e0.elapsed_time(e1) / 1e3 / launches
+    bench(2)
     # check: y = A x against the matrix's own FP64 multiply
-    ref = pmat.multiply(x.double())
-    err = float(torch.linalg.norm(y.double() - ref) / torch.linalg.norm(ref))
+    ref = pmat.multiply(x)
+    err = float(torch.linalg.norm(y - ref) / torch.linalg.norm(ref))
     C, F = grid.num_cells, grid.num_internal_faces
     # Algorithmic bytes of one k_cg_dir launch: the symmetric LDU SpMV
     # (diagonal s_d C, one off-diagonal value per face s_v F, two int32
@@ -150,7 +148,8 @@ def spmv_roofline(grid, pmat, fp32: bool, launches: int = 200) -> dict:
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _traffic(fp32, C),
             "kernel": "k_cg_dir<float, 1, true>" if fp32 else "k_cg_dir<double, 1, false>",
-            "cells": C, "bytes_per_launch": int(B), "avg_us": round(avg_s * 1e6, 2), "rel_err_vs_fp64": err}
+            "cells": C, "bricks": int(grid.mesh.n_bricks), "brick_cells_frac": round(64 * grid.mesh.n_bricks / C, 4),
+            "bytes_per_launch": int(B), "avg_us": round(avg_s * 1e6, 2), "rel_err_vs_fp64": err}
 
 
 def stream_copy(a, b) -> None:

@@ -32,6 +32,9 @@ class GfMesh(ctypes.Structure):
         ("bnd_d", _vp), ("bnd_mag_d", _vp), ("bnd_mag_sf", _vp), ("bnd_sign", _vp), ("ata_inv", _vp),
         ("hang_face", _vp), ("hang_offset", _vp), ("hang_move_o", _vp), ("hang_move_n", _vp),
         ("hang_cell", _vp), ("hc_row", _vp), ("hc_code", _vp),
+        ("n_bricks", _i32), ("n_expl", _i32), ("pnnz_pad", _i64),
+        ("row_perm", _vp), ("row_iperm", _vp), ("brick_nbr", _vp), ("brick_vp", _vp), ("brick_vm", _vp),
+        ("expl_col", _vp), ("expl_code", _vp), ("psell_ptr", _vp), ("psell_col", _vp), ("psell_code", _vp),
     ]
 
 
@@ -51,6 +54,7 @@ class GfKrylovCfg(ctypes.Structure):
     _fields_ = [
         ("method", _i32), ("precond", _i32), ("norm", _i32), ("max_iter", _i32), ("max_restart", _i32),
         ("atol", _f64), ("rtol", _f64), ("mixed", _i32), ("inner_rtol", _f64), ("check_every", _i32),
+        ("symmetric", _i32),
     ]
 
 
@@ -108,9 +112,8 @@ _SIGS: dict[str, tuple[Any, list[Any]]] = {
     "gf_krylov_use_graphs": (_i32, [_i32]),
     "gf_krylov_release": (_i32, [_vp]),
     "gf_ldu_grads": (_i32, [_P(GfMesh), _i32, _vp, _vp, _vp, _vp, _vp, _vp]),
-    "gf_sell_pack": (_i32, [_P(GfMesh), _i32, _vp, _vp, _vp, _i32, _vp]),
-    "gf_sell_spmv_scratch_bytes": (_i64, [_P(GfMesh)]),
-    "gf_sell_spmv": (_i32, [_P(GfMesh), _vp, _vp, _vp, _vp, _i32, _i32, _vp, _vp]),
+    "gf_pcg_dir_bench_bytes": (_i64, [_P(GfMesh), _i32]),
+    "gf_pcg_dir_bench": (_i32, [_P(GfMesh), _i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gf_relax_momentum": (_i32, [_i32, _f64, _vp, _vp, _vp, _vp]),
     "gf_ldu_h1": (_i32, [_P(GfMesh), _vp, _vp, _vp, _vp]),
     "gf_fixed_values": (_i32, [_P(GfMesh), _i32, _i32] + [_vp] * 13 + [_vp]),

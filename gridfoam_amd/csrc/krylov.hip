@@ -67,11 +67,15 @@ constexpr int SB = 256;  // threads per workgroup in SpMV kernels (one row per t
 
 struct Ops {
     const int32_t *sell_ptr, *sell_col;
+    const int32_t *bnbr;  // brick neighbours [nbs*6] (solver row order), null without bricks
+    const int32_t *ecol;  // explicit-block columns [n_expl*64]
+    int nbrick;           // brick slices: solver rows [0, 64 nbrick) are bricks (0: none)
     int C;
     int nb;       // BiCGSTAB vector kernels: workgroups (<= NBMAX)
     int nbs;      // BiCGSTAB SpMV kernels: workgroups (ceil(C / SB))
     int tb;       // PCG SpMV-type kernels: rows (= threads) per workgroup, 256, 512 or 1024
     int gs;       // PCG SpMV-type kernels: workgroups (ceil(C / tb) rounded up to a multiple of 8)
+    int gd;       // PCG direction kernel: workgroups (its waves carry bricks or sliced-ELL slices)
     int gu;       // PCG vector kernels: workgroups (a multiple of 8)
     int pstride;  // partials per (rhs, quantity): the largest workgroup count
     int norm_inf, max_iter, max_restart;
@@ -94,6 +98,8 @@ int pcg_tile() {
     return tb;
 }
 inline int pcg_spmv_workgroups(int C) { return round8((C + pcg_tile() - 1) / pcg_tile()); }
+// Direction kernel: one wave per kBricks bricks, then one wave per remaining slice of 64 rows.
+int pcg_dir_workgroups(int C, int nbrick);
 inline int pcg_vector_workgroups(int C) { return round8(solver_workgroups(C)); }
 inline int partial_stride(int C) {
     int m = solver_workgroups(C);
@@ -373,37 +379,41 @@ __global__ void k_sell_pack(int64_t n, const int32_t *__restrict__ code, const d
 }
 
 template <typename VT>
-__global__ void k_diag_prep(int C, const double *__restrict__ diag, int precond, VT *__restrict__ dg,
-                            VT *__restrict__ inv_d) {
+__global__ void k_diag_prep(int C, const double *__restrict__ diag, const int32_t *__restrict__ perm, int precond,
+                            VT *__restrict__ dg, VT *__restrict__ inv_d) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= C) return;
-    double d = diag[i];
+    double d = diag[perm ? perm[i] : i];  // solver row i is cell perm[i]
     dg[i] = (VT)d;
     // preconditioners.py:83-90: 1/diag, or 1 where |diag| < eps(dtype)
     const double eps = sizeof(VT) == 4 ? 1.1920928955078125e-07 : 2.220446049250313e-16;
     double inv = 1.0;
     if (precond) inv = fabs(d) < eps ? 1.0 : 1.0 / d;
-    inv_d[i] = (VT)inv;
+    if (inv_d) inv_d[i] = (VT)inv;
 }
 
-// Solve entry: b and x ([C, nrhs]) into the workspace's component-major copies.
+// Solve entry: b and x ([C, nrhs]) into the workspace's component-major
+// copies, in solver row order (row i is cell perm[i]; identity without perm).
 __global__ void k_solve_in(int C, int nrhs, const double *__restrict__ b, const double *__restrict__ x,
-                           double *__restrict__ wb, double *__restrict__ wx) {
+                           const int32_t *__restrict__ perm, double *__restrict__ wb, double *__restrict__ wx) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= C) return;
+    const int c = perm ? perm[i] : i;
     for (int r = 0; r < nrhs; ++r) {
-        wb[(size_t)r * C + i] = b[(size_t)i * nrhs + r];
-        wx[(size_t)r * C + i] = x[(size_t)i * nrhs + r];
+        wb[(size_t)r * C + i] = b[(size_t)c * nrhs + r];
+        wx[(size_t)r * C + i] = x[(size_t)c * nrhs + r];
     }
 }
 
 // Solve exit: the solution back to [C, nrhs], and the final solver state of
 // `nr` components into coherent pinned host memory (system-scope stores).
-__global__ void k_solve_out(int C, int nrhs, const double *__restrict__ wx, double *__restrict__ x,
-                            const KState *__restrict__ st, KState *host_state) {
+__global__ void k_solve_out(int C, int nrhs, const double *__restrict__ wx, const int32_t *__restrict__ perm,
+                            double *__restrict__ x, const KState *__restrict__ st, KState *host_state) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < C)
-        for (int r = 0; r < nrhs; ++r) x[(size_t)i * nrhs + r] = wx[(size_t)r * C + i];
+    if (i < C) {
+        const int c = perm ? perm[i] : i;
+        for (int r = 0; r < nrhs; ++r) x[(size_t)c * nrhs + r] = wx[(size_t)r * C + i];
+    }
     constexpr int kWords = (int)(sizeof(KState) / sizeof(uint32_t));
     if (blockIdx.x == 0 && threadIdx.x < nrhs * kWords) {
         const uint32_t *src = reinterpret_cast<const uint32_t *>(st);
@@ -469,78 +479,166 @@ constexpr int head_entries() {
     return sizeof(VT) == 4 ? 9 : 6;
 }
 
-// Convergence test on the previous update's residual, beta and p = z + beta p_old ; Ap ; partial p.Ap.
-// Mixed precision (MX): the test only decides whether the FP64 refinement is due;
-// the reliable-update trigger is only taken where the host queued a
-// refinement right after this iteration (may_refine), so no launches idle.
-// Everything that does not depend on the solver state (the previous
-// partials, the row's matrix entries, z at the row and at its first
-// neighbours) is loaded before the state is read; p_old follows once the
-// state names its buffer, and all of it is in flight during the reduction.
-// Every wave applies the state transition itself (same inputs, same result).
-template <typename VT, int NR, bool MX, int TB>
-__global__ __launch_bounds__(TB) void k_cg_dir(Ops o, int s, int nb_prev, int may_refine, int pb,
-                                               const VT *__restrict__ val, const double *__restrict__ dg,
-                                               const VT *__restrict__ z, VT *__restrict__ p0, VT *__restrict__ p1,
-                                               VT *__restrict__ ap) {
-    constexpr int kHead = head_entries<VT>();
-    Prefetched<NR, 2, 1, TB> pre;
-    prefetch_prev(o, s, nb_prev, pre);
-    const int i = xcd_tile(blockIdx.x, gridDim.x) * TB + threadIdx.x;
-    const int ic = i < o.C ? i : o.C - 1;
-    const int sl = __builtin_amdgcn_readfirstlane(ic >> 6), lane = ic & 63;
-    const int base = o.sell_ptr[sl];
-    const int W = (o.sell_ptr[sl + 1] - base) >> 6;
-    const double d = dg[ic];
-    VT zi[NR];
+// ---- brick slices of the solver row order ----------------------------------
+// A brick is 64 leaves of one level filling an aligned 4x4x4 block; lane
+// l = x + 4 y + 16 z holds the cell with in-brick coordinates (x, y, z), so
+// its neighbours inside the brick are lanes l +- 1, +- 4, +- 16 and their
+// values come from cross-lane permutes.  Each lane keeps the value of its
+// +axis face (symmetric operator: the -axis value is the -neighbour's +axis
+// value).  A side toward a brick of the same level reads that brick's mirror
+// lane (l ^ 3, l ^ 12, l ^ 48) and its -axis values are stored per side (16
+// per axis); a side toward anything else is an explicit block of 64 slots
+// (16 boundary cells x 4 faces, slot = 4 q + k).
+__device__ __forceinline__ int bcoord(int l, int a) { return (l >> (2 * a)) & 3; }
+// position q of a boundary cell on an axis-a side: c_b + 4 c_c, b = (a+1) % 3, c = (a+2) % 3
+__device__ __forceinline__ int bface(int l, int a) {
+    return a == 0 ? l >> 2 : (a == 2 ? l & 15 : (l >> 4) + 4 * (l & 3));
+}
+
+typedef const __attribute__((address_space(4))) int32_t ConstI32;
+
+// Bricks per wave in the direction kernel: each wave carries kBricks
+// bricks, all of whose loads are issued before the solver state is read, so
+// the dependent neighbour-brick gathers of several bricks overlap.
+constexpr int kBricks = 2;
+
+// Waves that carry bricks: whole workgroups only (bricks left over go
+// through the sliced ELL with the other rows).
+__host__ __device__ __forceinline__ int brick_waves(int nbrick, int tb) {
+    const int per = tb / 64;
+    return nbrick / (kBricks * per * 8) * per * 8;
+}
+// The grid is [brick workgroups | sliced-ELL workgroups], each part a
+// multiple of 8 so that both spread evenly over the 8 XCDs (each XCD gets a
+// contiguous range of either part).
+__host__ __device__ __forceinline__ int brick_workgroups(int nbrick, int tb) {
+    return brick_waves(nbrick, tb) / (tb / 64);  // whole workgroups, a multiple of 8 by construction
+}
+int pcg_dir_workgroups(int C, int nbrick) {
+    const int nsl = (C + 63) / 64, nbw = brick_waves(nbrick, pcg_tile()), per = pcg_tile() / 64;
+    return brick_workgroups(nbrick, pcg_tile()) + round8((nsl - nbw * kBricks + per - 1) / per);
+}
+
+// Everything a brick row needs that does not depend on the solver state.
+template <typename VT>
+struct BrickRow {
+    double d;
+    VT z, po, vp[3], vm[3], zx[3], px[3];
+    int nb[6];
+    int d0, d1;  // first two explicit sides (uniform), -1 none
+    VT e0v, e0z, e0p, e1v, e1z, e1p;
+};
+
+template <typename VT>
+__device__ __forceinline__ void brick_load(const Ops &o, int sl, int lane, const double *__restrict__ dg,
+                                           const VT *__restrict__ z, const VT *__restrict__ pold,
+                                           const VT *__restrict__ bv, const VT *__restrict__ bm,
+                                           const VT *__restrict__ ev, BrickRow<VT> &B) {
+    const int i = sl * 64 + lane;
+    ConstI32 *nbp = (ConstI32 *)(o.bnbr + 6 * sl);
 #pragma unroll
-    for (int r = 0; r < NR; ++r) zi[r] = z[(size_t)r * o.C + ic];
-    int col[kHead];
-    VT v[kHead];
+    for (int k = 0; k < 6; ++k) B.nb[k] = nbp[k];
+    B.d = dg[i];
+    B.z = z[i];
+    B.po = pold[i];
+    // every lane loads (its own row where it has no neighbour brick): no divergence
 #pragma unroll
-    for (int u = 0; u < kHead; ++u) {
-        col[u] = ic;
-        v[u] = (VT)0;
-        if (u < W) {
-            const int e = base + u * 64 + lane;
-            col[u] = __builtin_nontemporal_load(o.sell_col + e);
-            v[u] = __builtin_nontemporal_load(val + e);
+    for (int a = 0; a < 3; ++a) {
+        B.vp[a] = bv[((size_t)sl * 3 + a) * 64 + lane];
+        const int c = bcoord(lane, a);
+        const int nbr = c == 3 ? B.nb[2 * a + 1] : B.nb[2 * a];
+        const bool hx = (c == 0 || c == 3) && nbr >= 0;
+        const int e = hx ? nbr * 64 + (lane ^ (3 << (2 * a))) : i;
+        B.zx[a] = z[e];
+        B.px[a] = pold[e];
+        const VT m = bm[((size_t)sl * 3 + a) * 16 + bface(lane, a)];
+        B.vm[a] = (c == 0 && hx) ? m : (VT)0;
+    }
+    B.d0 = -1;
+    B.d1 = -1;
+    int b0 = 0, b1 = 0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        if (B.nb[k] <= -2) {
+            if (B.d0 < 0) {
+                B.d0 = k;
+                b0 = -2 - B.nb[k];
+            } else if (B.d1 < 0) {
+                B.d1 = k;
+                b1 = -2 - B.nb[k];
+            }
         }
     }
-    const VT *pold[NR];
-    VT *pnew[NR];
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {  // direction of the previous iteration in buffer pb; the new one goes to the other
-        pold[r] = (pb ? p1 : p0) + (size_t)r * o.C;
-        pnew[r] = (pb ? p0 : p1) + (size_t)r * o.C;
+    B.e0v = B.e0z = B.e0p = B.e1v = B.e1z = B.e1p = (VT)0;
+    if (B.d0 >= 0) {
+        const int e = b0 * 64 + lane;
+        const int c = o.ecol[e];
+        B.e0v = ev[e];
+        B.e0z = z[c];
+        B.e0p = pold[c];
     }
-    VT poi[NR], zj[kHead][NR], pj[kHead][NR];
+    if (B.d1 >= 0) {
+        const int e = b1 * 64 + lane;
+        const int c = o.ecol[e];
+        B.e1v = ev[e];
+        B.e1z = z[c];
+        B.e1p = pold[c];
+    }
+}
+
+// Ap of a brick row; p is this row's new direction value.  Every lane of the
+// wave takes part in the permutes (a lane masked off would read as zero).
+template <typename VT>
+__device__ __forceinline__ double brick_apply(const Ops &o, int lane, bool start, VT be, VT p,
+                                              const BrickRow<VT> &B, const VT *__restrict__ z,
+                                              const VT *__restrict__ pold, const VT *__restrict__ ev) {
+    double acc = B.d * (double)p;
 #pragma unroll
-    for (int r = 0; r < NR; ++r) poi[r] = pold[r][ic];
+    for (int a = 0; a < 3; ++a) {
+        const int st = 1 << (2 * a), c = bcoord(lane, a);
+        const VT pe = start ? B.zx[a] : dir_fma<VT>(be, B.px[a], B.zx[a]);
+        const VT sp = __shfl(p, lane + st);
+        const VT sm = __shfl(p, lane - st);
+        const VT sv = __shfl(B.vp[a], lane - st);
+        const VT qp = c == 3 ? pe : sp;  // a boundary lane takes its outside neighbour
+        const VT qm = c == 0 ? pe : sm;  // (zero value where there is none)
+        const VT vmn = c == 0 ? B.vm[a] : sv;
+        acc += (double)B.vp[a] * (double)qp;
+        acc += (double)vmn * (double)qm;
+    }
+    // explicit side dd: slot l = 4 q + k holds face k of boundary cell q; the
+    // four slots of a cell are summed across lanes, then read by the cell's lane
+    auto side = [&](int dd, VT vv, VT zz, VT pp) {
+        const VT pe = start ? zz : dir_fma<VT>(be, pp, zz);
+        double cl = (double)vv * (double)pe;
+        cl += __shfl_xor(cl, 1);
+        cl += __shfl_xor(cl, 2);
+        const int a = dd >> 1;
+        const double t = __shfl(cl, 4 * bface(lane, a));
+        if (bcoord(lane, a) == ((dd & 1) ? 3 : 0)) acc += t;
+    };
+    if (B.d0 >= 0) side(B.d0, B.e0v, B.e0z, B.e0p);
+    if (B.d1 >= 0) side(B.d1, B.e1v, B.e1z, B.e1p);
 #pragma unroll
-    for (int u = 0; u < kHead; ++u)
-#pragma unroll
-        for (int r = 0; r < NR; ++r) {
-            zj[u][r] = z[(size_t)r * o.C + col[u]];
-            pj[u][r] = pold[r][col[u]];
+    for (int dd = 0; dd < 6; ++dd) {  // sides beyond the first two (rare): loaded here
+        if (B.nb[dd] <= -2 && dd != B.d0 && dd != B.d1) {
+            const int e = (-2 - B.nb[dd]) * 64 + lane;
+            const int c = o.ecol[e];
+            side(dd, ev[e], z[c], pold[c]);
         }
-    asm volatile("" ::: "memory");  // keep the loads above issued before the state test
-    KState S[NR];
-    bool any = false;
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-        S[r] = state_in(o, s, r);
-        any |= S[r].phase == PH_ACTIVE;
     }
-    if (!any) {
-        state_out<NR>(o, s, S);
-        return;
-    }
-    // ---- previous partials: |r|, r.z
-    const bool mx[2] = {(bool)o.norm_inf, false};
-    double tot[NR][2];
-    reduce_prev<NR, 2, TB, 1>(o, s, nb_prev, mx, tot, &pre);
-    bool act[NR], start[NR];
+    return acc;
+}
+
+// The state transition of the direction kernel (cg.py:150-170), applied by
+// every wave to the same inputs: convergence test on the previous update's
+// residual, beta, the direction buffer flip.  Mixed precision (MX): the test
+// only decides whether the FP64 refinement is due; the reliable-update
+// trigger is only taken where the host queued a refinement right after this
+// iteration (may_refine), so no launches idle.
+template <int NR, bool MX>
+__device__ __forceinline__ void dir_transition(const Ops &o, int may_refine, const double (&tot)[NR][2],
+                                               KState (&S)[NR], bool (&act)[NR], bool (&start)[NR]) {
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
         KState &St = S[r];
@@ -578,6 +676,119 @@ __global__ __launch_bounds__(TB) void k_cg_dir(Ops o, int s, int nb_prev, int ma
         }
         act[r] = St.phase == PH_ACTIVE;
     }
+}
+
+// Direction work of a wave that carries kBricks bricks (one rhs, symmetric
+// operator).  Everything that does not depend on the solver state is loaded
+// before the state is read and stays in flight during the reduction.
+template <typename VT, bool MX, int TB>
+__device__ __forceinline__ void dir_bricks(const Ops &o, int s, int nb_prev, int may_refine, int pb, int sl,
+                                           const Prefetched<1, 2, 1, TB> &pre, const double *__restrict__ dg,
+                                           const VT *__restrict__ z, VT *__restrict__ p0, VT *__restrict__ p1,
+                                           VT *__restrict__ ap, const VT *__restrict__ bv,
+                                           const VT *__restrict__ bm, const VT *__restrict__ ev) {
+    const int lane = threadIdx.x & 63;
+    const VT *pold = pb ? p1 : p0;
+    VT *pnew = pb ? p0 : p1;
+    BrickRow<VT> Bk[kBricks];
+#pragma unroll
+    for (int k = 0; k < kBricks; ++k) brick_load<VT>(o, sl + k, lane, dg, z, pold, bv, bm, ev, Bk[k]);
+    asm volatile("" ::: "memory");  // keep the loads above issued before the state test
+    KState S[1];
+    S[0] = state_in(o, s, 0);
+    if (S[0].phase != PH_ACTIVE) {
+        state_out<1>(o, s, S);
+        return;
+    }
+    const bool mx[2] = {(bool)o.norm_inf, false};
+    double tot[1][2];
+    reduce_prev<1, 2, TB, 1>(o, s, nb_prev, mx, tot, &pre);
+    bool act[1], start[1];
+    dir_transition<1, MX>(o, may_refine, tot, S, act, start);
+    const VT be = (VT)S[0].beta;
+    double pv[1][1] = {{0.0}};
+#pragma unroll
+    for (int k = 0; k < kBricks; ++k) {
+        const VT p = start[0] ? Bk[k].z : dir_fma<VT>(be, Bk[k].po, Bk[k].z);
+        const double acc = brick_apply<VT>(o, lane, start[0], be, p, Bk[k], z, pold, ev);
+        if (act[0]) {
+            const int ik = (sl + k) * 64 + lane;
+            const VT a = (VT)acc;
+            pnew[ik] = p;
+            ap[ik] = a;
+            pv[0][0] += (double)p * (double)a;
+        }
+    }
+    const bool mx1[1] = {false};
+    write_partials<1, 1, TB>(o, s, pv, mx1);
+    state_out<1>(o, s, S);
+}
+
+// Direction work of a wave that carries one slice of the sliced ELL (slice
+// sl0; past the last slice the wave only takes part in the reductions).
+template <typename VT, int NR, bool MX, int TB>
+__device__ __forceinline__ void dir_sell(const Ops &o, int s, int nb_prev, int may_refine, int pb, int sl0,
+                                         const Prefetched<NR, 2, 1, TB> &pre, const VT *__restrict__ val,
+                                         const double *__restrict__ dg, const VT *__restrict__ z,
+                                         VT *__restrict__ p0, VT *__restrict__ p1, VT *__restrict__ ap) {
+    constexpr int kHead = head_entries<VT>();
+    const int lane = threadIdx.x & 63;
+    const int nsl = (o.C + 63) >> 6;
+    const int sl = sl0 < nsl ? sl0 : nsl - 1;
+    const int i = sl0 * 64 + lane;
+    const int ic = i < o.C ? i : o.C - 1;
+    const VT *pold[NR];
+    VT *pnew[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {  // direction of the previous iteration in buffer pb; the new one goes to the other
+        pold[r] = (pb ? p1 : p0) + (size_t)r * o.C;
+        pnew[r] = (pb ? p0 : p1) + (size_t)r * o.C;
+    }
+    const double d = dg[ic];
+    VT zi[NR], poi[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        zi[r] = z[(size_t)r * o.C + ic];
+        poi[r] = pold[r][ic];
+    }
+    const int base = o.sell_ptr[sl];
+    const int W = sl0 < nsl ? (o.sell_ptr[sl + 1] - base) >> 6 : 0;
+    int col[kHead];
+    VT v[kHead], zj[kHead][NR], pj[kHead][NR];
+#pragma unroll
+    for (int u = 0; u < kHead; ++u) {
+        col[u] = ic;
+        v[u] = (VT)0;
+        if (u < W) {
+            const int e = base + u * 64 + lane;
+            col[u] = __builtin_nontemporal_load(o.sell_col + e);
+            v[u] = __builtin_nontemporal_load(val + e);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kHead; ++u)
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            zj[u][r] = z[(size_t)r * o.C + col[u]];
+            pj[u][r] = pold[r][col[u]];
+        }
+    asm volatile("" ::: "memory");  // keep the loads above issued before the state test
+    KState S[NR];
+    bool any = false;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        S[r] = state_in(o, s, r);
+        any |= S[r].phase == PH_ACTIVE;
+    }
+    if (!any) {
+        state_out<NR>(o, s, S);
+        return;
+    }
+    const bool mx[2] = {(bool)o.norm_inf, false};
+    double tot[NR][2];
+    reduce_prev<NR, 2, TB, 1>(o, s, nb_prev, mx, tot, &pre);
+    bool act[NR], start[NR];
+    dir_transition<NR, MX>(o, may_refine, tot, S, act, start);
     VT be[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) be[r] = (VT)S[r].beta;
@@ -635,6 +846,34 @@ __global__ __launch_bounds__(TB) void k_cg_dir(Ops o, int s, int nb_prev, int ma
     state_out<NR>(o, s, S);
 }
 
+// Convergence test on the previous update's residual, beta and p = z + beta
+// p_old ; Ap ; partial p.Ap (cg.py:120-170 in two launches, see above).
+// Work per wave: the first workgroups (a multiple of 8) carry kBricks
+// bricks per wave (values bv: +axis faces, bm: -axis sides toward bricks,
+// ev: explicit blocks); every wave of the following workgroups one slice of
+// the sliced ELL, from slice nbw * kBricks on.  A workgroup is all-brick or
+// all-ELL, so both paths meet the same barriers.
+template <typename VT, int NR, bool MX, int TB>
+__global__ __launch_bounds__(TB) void k_cg_dir(Ops o, int s, int nb_prev, int may_refine, int pb,
+                                               const VT *__restrict__ val, const double *__restrict__ dg,
+                                               const VT *__restrict__ z, VT *__restrict__ p0, VT *__restrict__ p1,
+                                               VT *__restrict__ ap, const VT *__restrict__ bv,
+                                               const VT *__restrict__ bm, const VT *__restrict__ ev) {
+    Prefetched<NR, 2, 1, TB> pre;
+    prefetch_prev(o, s, nb_prev, pre);
+    const int nbw = brick_waves(o.nbrick, TB), gb = brick_workgroups(o.nbrick, TB);
+    const int wid = threadIdx.x >> 6;
+    if constexpr (NR == 1) {
+        if ((int)blockIdx.x < gb) {
+            const int wv = __builtin_amdgcn_readfirstlane(xcd_tile(blockIdx.x, gb) * (TB / 64) + wid);
+            dir_bricks<VT, MX, TB>(o, s, nb_prev, may_refine, pb, wv * kBricks, pre, dg, z, p0, p1, ap, bv, bm, ev);
+            return;
+        }
+    }
+    const int ws = __builtin_amdgcn_readfirstlane(xcd_tile(blockIdx.x - gb, gridDim.x - gb) * (TB / 64) + wid);
+    dir_sell<VT, NR, MX, TB>(o, s, nb_prev, may_refine, pb, nbw * kBricks + ws, pre, val, dg, z, p0, p1, ap);
+}
+
 // alpha = rz_old / p.Ap ; x += alpha p ; r -= alpha Ap ; z = M^-1 r ; partials |r|, r.z.
 // The grid has at most one 1024-thread workgroup per CU, so the kernel may
 // use up to 128 VGPRs: each thread's first rows (kHoist of them; all rows of
@@ -647,7 +886,7 @@ __global__ __launch_bounds__(RB) void k_cg_upd(Ops o, int s, int pb, const VT *_
                                                const VT *__restrict__ ap_) {
     constexpr int kHoist = (sizeof(VT) == 4 ? 6 : 5) / NR > 0 ? (sizeof(VT) == 4 ? 6 : 5) / NR : 1;
     Prefetched<NR, 1, 5, RB> pre;
-    prefetch_prev(o, s, o.gs, pre);
+    prefetch_prev(o, s, o.gd, pre);
     int lo, hi;
     chunk_range(o.C, gridDim.x, xcd_tile(blockIdx.x, gridDim.x), lo, hi);
     const VT *p_[NR];  // this iteration's direction: buffer pb
@@ -682,7 +921,7 @@ __global__ __launch_bounds__(RB) void k_cg_upd(Ops o, int s, int pb, const VT *_
     }
     const bool mx1[1] = {false};
     double pap[NR][1];
-    reduce_prev<NR, 1, RB, 5>(o, s, o.gs, mx1, pap, &pre);
+    reduce_prev<NR, 1, RB, 5>(o, s, o.gd, mx1, pap, &pre);
     double v[NR][2] = {};
     bool act[NR];
     VT a[NR];
@@ -800,8 +1039,8 @@ __global__ __launch_bounds__(RB) void k_cg_init_fin(Ops o, int s, int nb_prev) {
 // convergence is decided on that true residual only.
 
 // val32 = (float) val64 ; dgc = diag + sum_row(val64 - val32)
-__global__ void k_mx_prep(int C, const int32_t *sell_ptr, const double *val64, const double *diag, float *val32,
-                          double *dgc) {
+__global__ void k_mx_prep(int C, const int32_t *sell_ptr, const double *val64, const double *diag,
+                          const int32_t *perm, float *val32, double *dgc) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= C) return;
     const int sl = i >> 6, lane = i & 63;
@@ -815,7 +1054,7 @@ __global__ void k_mx_prep(int C, const int32_t *sell_ptr, const double *val64, c
         val32[e] = v32;
         corr += v - (double)v32;
     }
-    dgc[i] = diag[i] + corr;
+    dgc[i] = diag[perm ? perm[i] : i] + corr;
 }
 
 // Refinement, part 1: x64 += x32 ; x32 = 0 ; the current direction moves
@@ -1330,12 +1569,30 @@ struct WS {
 struct Layout {
     double *val64, *dg64, *inv64, *b, *x, *r, *z, *p, *p1, *ap, *y, *s, *t, *rh, *v;
     float *val32, *dg32, *inv32, *x32, *r32, *z32, *p32, *p32b, *ap32;
+    double *bv64, *bm64, *ev64;  // brick values (plain PCG), null without bricks
+    float *bv32, *bm32, *ev32;   // brick values (mixed PCG)
     double *partials, *dgc;
     KState *st;
     size_t bytes;
 };
 
-Layout layout(char *base, int64_t C, int64_t nnz, int nrhs, const gf_krylov_cfg &cfg) {
+// Brick layout of the solve: the PCG of a symmetric operator with one rhs on
+// a mesh whose solver layout was built.
+struct BrickPlan {
+    bool on = false;
+    int nbs = 0, ne = 0;
+};
+BrickPlan brick_plan(const gf_mesh *m, int nrhs, const gf_krylov_cfg &cfg) {
+    BrickPlan b;
+    b.on = cfg.method == 0 && nrhs == 1 && cfg.symmetric && m->n_bricks > 0 && m->psell_ptr && m->row_perm;
+    if (b.on) {
+        b.nbs = m->n_bricks;
+        b.ne = m->n_expl;
+    }
+    return b;
+}
+
+Layout layout(char *base, int64_t C, int64_t nnz, int nrhs, const gf_krylov_cfg &cfg, const BrickPlan &bp) {
     WS w{base, 0};
     Layout L{};
     const size_t n = (size_t)C * nrhs;
@@ -1367,6 +1624,18 @@ Layout layout(char *base, int64_t C, int64_t nnz, int nrhs, const gf_krylov_cfg 
             L.t = w.take<double>(n);
             L.rh = w.take<double>(n);
             L.v = L.ap;
+        }
+    }
+    if (bp.on) {
+        const size_t nv = (size_t)bp.nbs * 192, nm = (size_t)bp.nbs * 48, ne = (size_t)bp.ne * 64 + 1;
+        if (cfg.mixed) {
+            L.bv32 = w.take<float>(nv);
+            L.bm32 = w.take<float>(nm);
+            L.ev32 = w.take<float>(ne);
+        } else {
+            L.bv64 = w.take<double>(nv);
+            L.bm64 = w.take<double>(nm);
+            L.ev64 = w.take<double>(ne);
         }
     }
     L.partials = w.take<double>((size_t)2 * kMaxRhs * kNQ * partial_stride((int)C));
@@ -1534,20 +1803,20 @@ void key_add(std::string &k, const T &v) {
 template <int NR, int TB>
 void enqueue_cg_chunk(const Ops &o, const Layout &L, bool mixed, int every, int s, int first_prev, int *poll,
                       hipStream_t st) {
-    const dim3 GS(o.gs), BS(TB), GU(o.gu), BU(RB), G1(1);
+    const dim3 GS(o.gs), GD(o.gd), BS(TB), GU(o.gu), BU(RB), G1(1);
     Ops ol = o;  // the chunk's last launch posts the poll word
     ol.poll = poll;
     for (int j = 0; j < every; ++j) {
         const int nb_prev = j == 0 ? first_prev : o.gu;
         const int pb = j & 1;  // buffer of p_old; `every` is even
         if (!mixed) {
-            hipLaunchKernelGGL((k_cg_dir<double, NR, false, TB>), GS, BS, 0, st, o, s++, nb_prev, 0, pb, L.val64,
-                               L.dg64, L.z, L.p, L.p1, L.ap);
+            hipLaunchKernelGGL((k_cg_dir<double, NR, false, TB>), GD, BS, 0, st, o, s++, nb_prev, 0, pb, L.val64,
+                               L.dg64, L.z, L.p, L.p1, L.ap, L.bv64, L.bm64, L.ev64);
             hipLaunchKernelGGL((k_cg_upd<double, NR>), GU, BU, 0, st, j == every - 1 ? ol : o, s++, pb ^ 1, L.inv64,
                                L.x, L.r, L.z, L.p, L.p1, L.ap);
         } else {
-            hipLaunchKernelGGL((k_cg_dir<float, NR, true, TB>), GS, BS, 0, st, o, s++, nb_prev, j == every - 1, pb,
-                               L.val32, L.dgc, L.z32, L.p32, L.p32b, L.ap32);
+            hipLaunchKernelGGL((k_cg_dir<float, NR, true, TB>), GD, BS, 0, st, o, s++, nb_prev, j == every - 1, pb,
+                               L.val32, L.dgc, L.z32, L.p32, L.p32b, L.ap32, L.bv32, L.bm32, L.ev32);
             hipLaunchKernelGGL((k_cg_upd<float, NR>), GU, BU, 0, st, o, s++, pb ^ 1, L.inv32, L.x32, L.r32, L.z32,
                                L.p32, L.p32b, L.ap32);
         }
@@ -1693,7 +1962,8 @@ using namespace gf;
 
 extern "C" int64_t gf_krylov_workspace_bytes(const gf_mesh *m, int32_t nrhs, const gf_krylov_cfg *cfg) {
     if (!m || !cfg || nrhs < 1 || nrhs > kMaxRhs) return -1;
-    Layout L = layout(nullptr, m->n_cells, m->nnz_pad, nrhs, *cfg);
+    const BrickPlan bp = brick_plan(m, nrhs, *cfg);
+    Layout L = layout(nullptr, m->n_cells, bp.on ? m->pnnz_pad : m->nnz_pad, nrhs, *cfg, bp);
     return (int64_t)L.bytes;
 }
 
@@ -1711,31 +1981,64 @@ extern "C" int gf_krylov_solve(const gf_mesh *m, const gf_krylov_cfg *cfg, int32
     GF_REQUIRE(m->n_cells >= 1, "empty grid");
     hipStream_t s = S(stream);
     const int C = m->n_cells;
-    Layout L = layout(reinterpret_cast<char *>(workspace), C, m->nnz_pad, nrhs, *cfg);
-    const int64_t nnz = m->nnz_pad;
-    hipLaunchKernelGGL(k_sell_pack<double>, dim3(grid_for(nnz)), dim3(kBlock), 0, s, nnz, m->sell_code, upper, lower,
+    // PCG of a symmetric operator with one rhs runs on the solver row order
+    // (bricks first); everything else on the grid's order
+    const BrickPlan bp = brick_plan(m, nrhs, *cfg);
+    const int32_t *perm = bp.on ? m->row_perm : nullptr;
+    const int32_t *sptr = bp.on ? m->psell_ptr : m->sell_ptr;
+    const int32_t *scol = bp.on ? m->psell_col : m->sell_col;
+    const int32_t *scode = bp.on ? m->psell_code : m->sell_code;
+    const int64_t nnz = bp.on ? m->pnnz_pad : m->nnz_pad;
+    Layout L = layout(reinterpret_cast<char *>(workspace), C, nnz, nrhs, *cfg, bp);
+    hipLaunchKernelGGL(k_sell_pack<double>, dim3(grid_for(nnz)), dim3(kBlock), 0, s, nnz, scode, upper, lower,
                        transpose, L.val64);
-    hipLaunchKernelGGL(k_diag_prep<double>, dim3(grid_for(C)), dim3(kBlock), 0, s, C, diag, cfg->precond, L.dg64,
-                       L.inv64);
+    hipLaunchKernelGGL(k_diag_prep<double>, dim3(grid_for(C)), dim3(kBlock), 0, s, C, diag, perm, cfg->precond,
+                       L.dg64, L.inv64);
     if (cfg->method == 0 && cfg->mixed) {
-        hipLaunchKernelGGL(k_mx_prep, dim3(grid_for(C)), dim3(kBlock), 0, s, C, m->sell_ptr, L.val64, diag, L.val32,
+        hipLaunchKernelGGL(k_mx_prep, dim3(grid_for(C)), dim3(kBlock), 0, s, C, sptr, L.val64, diag, perm, L.val32,
                            L.dgc);
-        hipLaunchKernelGGL(k_diag_prep<float>, dim3(grid_for(C)), dim3(kBlock), 0, s, C, diag, cfg->precond, L.dg32,
-                           L.inv32);
+        hipLaunchKernelGGL(k_diag_prep<float>, dim3(grid_for(C)), dim3(kBlock), 0, s, C, diag, perm, cfg->precond,
+                           L.dg32, L.inv32);
+    }
+    if (bp.on) {  // brick values, in the precision of the direction kernel
+        const int64_t nv = (int64_t)bp.nbs * 192, nm = (int64_t)bp.nbs * 48, ne = (int64_t)bp.ne * 64;
+        if (cfg->mixed) {
+            hipLaunchKernelGGL(k_sell_pack<float>, dim3(grid_for(nv)), dim3(kBlock), 0, s, nv, m->brick_vp, upper,
+                               lower, transpose, L.bv32);
+            hipLaunchKernelGGL(k_sell_pack<float>, dim3(grid_for(nm)), dim3(kBlock), 0, s, nm, m->brick_vm, upper,
+                               lower, transpose, L.bm32);
+            if (ne)
+                hipLaunchKernelGGL(k_sell_pack<float>, dim3(grid_for(ne)), dim3(kBlock), 0, s, ne, m->expl_code,
+                                   upper, lower, transpose, L.ev32);
+        } else {
+            hipLaunchKernelGGL(k_sell_pack<double>, dim3(grid_for(nv)), dim3(kBlock), 0, s, nv, m->brick_vp, upper,
+                               lower, transpose, L.bv64);
+            hipLaunchKernelGGL(k_sell_pack<double>, dim3(grid_for(nm)), dim3(kBlock), 0, s, nm, m->brick_vm, upper,
+                               lower, transpose, L.bm64);
+            if (ne)
+                hipLaunchKernelGGL(k_sell_pack<double>, dim3(grid_for(ne)), dim3(kBlock), 0, s, ne, m->expl_code,
+                                   upper, lower, transpose, L.ev64);
+        }
     }
     hipLaunchKernelGGL(k_zero_state, dim3(1), dim3(64), 0, s, L.st, 2 * kMaxRhs);
-    hipLaunchKernelGGL(k_solve_in, dim3(grid_for(C)), dim3(kBlock), 0, s, C, nrhs, b, x, L.b, L.x);
+    hipLaunchKernelGGL(k_solve_in, dim3(grid_for(C)), dim3(kBlock), 0, s, C, nrhs, b, x, perm, L.b, L.x);
     GF_CHECK_LAUNCH();
     HostStage *hs = host_stage();
     GF_REQUIRE(hs != nullptr, "pinned staging allocation failed");
     Ops o{};
-    o.sell_ptr = m->sell_ptr;
-    o.sell_col = m->sell_col;
+    o.sell_ptr = sptr;
+    o.sell_col = scol;
+    if (bp.on) {
+        o.bnbr = m->brick_nbr;
+        o.ecol = m->expl_col;
+        o.nbrick = bp.nbs;
+    }
     o.C = C;
     o.nb = solver_workgroups(C);
     o.nbs = spmv_workgroups(C);
     o.tb = pcg_tile();
     o.gs = pcg_spmv_workgroups(C);
+    o.gd = pcg_dir_workgroups(C, o.nbrick);
     o.gu = pcg_vector_workgroups(C);
     o.pstride = partial_stride(C);
     o.norm_inf = cfg->norm;
@@ -1768,7 +2071,7 @@ extern "C" int gf_krylov_solve(const gf_mesh *m, const gf_krylov_cfg *cfg, int32
     else
         rc = nrhs == 1 ? run_bicgstab<1>(o, L, *cfg, s, P, &s_last) : run_bicgstab<3>(o, L, *cfg, s, P, &s_last);
     if (rc != GF_OK) return rc;
-    hipLaunchKernelGGL(k_solve_out, dim3(grid_for(C)), dim3(kBlock), 0, s, C, nrhs, L.x, x,
+    hipLaunchKernelGGL(k_solve_out, dim3(grid_for(C)), dim3(kBlock), 0, s, C, nrhs, L.x, perm, x,
                        L.st + (s_last & 1) * kMaxRhs, hs->dev_state);
     GF_CHECK_LAUNCH();
     GF_CHECK_HIP(hipStreamSynchronize(s));  // the solution and the final state, in one wait
@@ -1794,75 +2097,159 @@ extern "C" int gf_krylov_release(const void *workspace) {
     return GF_OK;
 }
 
-extern "C" int gf_sell_pack(const gf_mesh *m, int32_t transpose, const double *upper, const double *lower, void *val,
-                            int32_t fp32, gf_stream_t stream) {
-    GF_REQUIRE(m && upper && lower && val, "null argument");
-    const int64_t nnz = m->nnz_pad;
-    if (fp32)
-        hipLaunchKernelGGL(k_sell_pack<float>, dim3(grid_for(nnz)), dim3(kBlock), 0, S(stream), nnz, m->sell_code,
-                           upper, lower, transpose, (float *)val);
-    else
-        hipLaunchKernelGGL(k_sell_pack<double>, dim3(grid_for(nnz)), dim3(kBlock), 0, S(stream), nnz, m->sell_code,
-                           upper, lower, transpose, (double *)val);
-    GF_CHECK_LAUNCH();
-    return GF_OK;
-}
-
-extern "C" int64_t gf_sell_spmv_scratch_bytes(const gf_mesh *m) {
-    if (!m || m->n_cells < 1) return 0;
-    return (int64_t)(sizeof(double) * 2 * kMaxRhs * kNQ * partial_stride(m->n_cells) + sizeof(KState) * 2 * kMaxRhs +
-                     2 * sizeof(double) * (size_t)m->n_cells + 4 * 256);
-}
-
-// Benchmark entry: the PCG direction/SpMV kernel in its steady state with
-// beta = 0, so p = z = x and y = A x; it also writes p to scratch, reads the
+// ---- benchmark of the direction kernel on its own -------------------------
+// The PCG direction/SpMV kernel in its steady state with beta = 0, so
+// p = z = x and Ap = A x, on the layout a symmetric one-rhs PCG solve uses
+// (bricks first when the mesh has them).  It also writes p, reads the
 // previous direction (zeros) and reduces the (empty) previous partials, as
-// inside the solver.  Launched `reps` times after one state set-up.
-extern "C" int gf_sell_spmv(const gf_mesh *m, const void *diag, const void *val, const void *x, void *y, int32_t fp32,
-                            int32_t reps, void *scratch, gf_stream_t stream) {
-    GF_REQUIRE(m && diag && val && x && y && scratch, "null argument");
-    GF_REQUIRE(m->n_cells >= 1, "empty grid");
-    GF_REQUIRE(reps >= 1, "reps must be >= 1");
-    hipStream_t s = S(stream);
-    WS w{reinterpret_cast<char *>(scratch), 0};
-    Ops o{};
-    o.sell_ptr = m->sell_ptr;
-    o.sell_col = m->sell_col;
-    o.C = m->n_cells;
-    o.nb = solver_workgroups(o.C);
-    o.nbs = spmv_workgroups(o.C);
+// inside the solver.
+namespace gf {
+namespace {
+template <typename VT>
+__global__ void k_bench_in(int C, const double *__restrict__ x, const int32_t *__restrict__ perm,
+                           VT *__restrict__ z, VT *__restrict__ p0, VT *__restrict__ p1) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= C) return;
+    z[i] = (VT)x[perm ? perm[i] : i];
+    p0[i] = (VT)0;
+    p1[i] = (VT)0;
+}
+template <typename VT>
+__global__ void k_bench_out(int C, const VT *__restrict__ ap, const int32_t *__restrict__ perm,
+                            double *__restrict__ y) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= C) return;
+    y[perm ? perm[i] : i] = (double)ap[i];
+}
+
+struct BenchLayout {
+    Ops o;
+    BrickPlan bp;
+    const int32_t *perm, *scode;
+    int64_t nnz;
+    void *val, *bv, *bm, *ev, *z, *p0, *p1, *ap;
+    double *dg;
+    size_t bytes;
+};
+BenchLayout bench_layout(const gf_mesh *m, int fp32, char *base) {
+    gf_krylov_cfg cfg{};
+    cfg.method = 0;
+    cfg.symmetric = 1;
+    BenchLayout B{};
+    B.bp = brick_plan(m, 1, cfg);
+    B.perm = B.bp.on ? m->row_perm : nullptr;
+    B.scode = B.bp.on ? m->psell_code : m->sell_code;
+    B.nnz = B.bp.on ? m->pnnz_pad : m->nnz_pad;
+    WS w{base, 0};
+    const size_t vb = fp32 ? sizeof(float) : sizeof(double);
+    const int C = m->n_cells;
+    Ops &o = B.o;
+    o.sell_ptr = B.bp.on ? m->psell_ptr : m->sell_ptr;
+    o.sell_col = B.bp.on ? m->psell_col : m->sell_col;
+    if (B.bp.on) {
+        o.bnbr = m->brick_nbr;
+        o.ecol = m->expl_col;
+        o.nbrick = B.bp.nbs;
+    }
+    o.C = C;
+    o.nb = solver_workgroups(C);
+    o.nbs = spmv_workgroups(C);
     o.tb = pcg_tile();
-    o.gs = pcg_spmv_workgroups(o.C);
-    o.gu = pcg_vector_workgroups(o.C);
-    o.pstride = partial_stride(o.C);
+    o.gs = pcg_spmv_workgroups(C);
+    o.gd = pcg_dir_workgroups(C, o.nbrick);
+    o.gu = pcg_vector_workgroups(C);
+    o.pstride = partial_stride(C);
     o.partials = w.take<double>((size_t)2 * kMaxRhs * kNQ * o.pstride);
     o.st = w.take<KState>(2 * kMaxRhs);
-    const size_t vb = fp32 ? sizeof(float) : sizeof(double);
-    char *p0 = reinterpret_cast<char *>(w.take<double>((size_t)o.C));
-    char *p1 = p0 + vb * o.C;
-    GF_CHECK_HIP(hipMemsetAsync(p0, 0, vb * o.C, s));
-    hipLaunchKernelGGL(k_spmv_bench_state, dim3(1), dim3(64), 0, s, o.st, 2 * kMaxRhs);
-    // Every launch uses sequence number 1: it reads state[0] (never written
-    // here) and writes state[1] and partials[1].
+    B.val = w.take<char>(vb * B.nnz);
+    B.bv = w.take<char>(vb * ((size_t)B.bp.nbs * 192 + 1));
+    B.bm = w.take<char>(vb * ((size_t)B.bp.nbs * 48 + 1));
+    B.ev = w.take<char>(vb * ((size_t)B.bp.ne * 64 + 1));
+    B.z = w.take<char>(vb * C);
+    B.p0 = w.take<char>(vb * C);
+    B.p1 = w.take<char>(vb * C);
+    B.ap = w.take<char>(vb * C);
+    B.dg = w.take<double>(C);
+    B.bytes = w.off + 256;
+    return B;
+}
+template <typename VT>
+void bench_pack(const gf_mesh *m, const BenchLayout &B, const double *diag, const double *upper,
+                const double *lower, const double *x, hipStream_t s) {
+    const int C = m->n_cells;
+    hipLaunchKernelGGL(k_sell_pack<VT>, dim3(grid_for(B.nnz)), dim3(kBlock), 0, s, B.nnz, B.scode, upper, lower, 0,
+                       (VT *)B.val);
+    if (B.bp.on) {
+        const int64_t nv = (int64_t)B.bp.nbs * 192, nm = (int64_t)B.bp.nbs * 48, ne = (int64_t)B.bp.ne * 64;
+        hipLaunchKernelGGL(k_sell_pack<VT>, dim3(grid_for(nv)), dim3(kBlock), 0, s, nv, m->brick_vp, upper, lower, 0,
+                           (VT *)B.bv);
+        hipLaunchKernelGGL(k_sell_pack<VT>, dim3(grid_for(nm)), dim3(kBlock), 0, s, nm, m->brick_vm, upper, lower, 0,
+                           (VT *)B.bm);
+        if (ne)
+            hipLaunchKernelGGL(k_sell_pack<VT>, dim3(grid_for(ne)), dim3(kBlock), 0, s, ne, m->expl_code, upper,
+                               lower, 0, (VT *)B.ev);
+    }
+    hipLaunchKernelGGL(k_diag_prep<double>, dim3(grid_for(C)), dim3(kBlock), 0, s, C, diag, B.perm, 0, B.dg,
+                       (double *)nullptr);
+    hipLaunchKernelGGL(k_bench_in<VT>, dim3(grid_for(C)), dim3(kBlock), 0, s, C, x, B.perm, (VT *)B.z, (VT *)B.p0,
+                       (VT *)B.p1);
+    hipLaunchKernelGGL(k_spmv_bench_state, dim3(1), dim3(64), 0, s, B.o.st, 2 * kMaxRhs);
+}
+template <typename VT, bool MX>
+void bench_run(const BenchLayout &B, int reps, hipStream_t s) {
     auto launch = [&](auto tbc) {
         constexpr int TB = decltype(tbc)::value;
-        for (int k = 0; k < reps; ++k) {
-            if (fp32)  // mixed-precision operator: FP32 values and vectors, FP64 diagonal and accumulation
-                hipLaunchKernelGGL((k_cg_dir<float, 1, true, TB>), dim3(o.gs), dim3(TB), 0, s, o, 1, 0, 0, 0,
-                                   (const float *)val, (const double *)diag, (const float *)x, (float *)p0,
-                                   (float *)p1, (float *)y);
-            else
-                hipLaunchKernelGGL((k_cg_dir<double, 1, false, TB>), dim3(o.gs), dim3(TB), 0, s, o, 1, 0, 0, 0,
-                                   (const double *)val, (const double *)diag, (const double *)x, (double *)p0,
-                                   (double *)p1, (double *)y);
-        }
+        // every launch uses sequence number 1: it reads state[0] (never
+        // written here) and writes state[1] and partials[1]
+        for (int k = 0; k < reps; ++k)
+            hipLaunchKernelGGL((k_cg_dir<VT, 1, MX, TB>), dim3(B.o.gd), dim3(TB), 0, s, B.o, 1, 0, 0, 0,
+                               (const VT *)B.val, (const double *)B.dg, (const VT *)B.z, (VT *)B.p0, (VT *)B.p1,
+                               (VT *)B.ap, (const VT *)B.bv, (const VT *)B.bm, (const VT *)B.ev);
     };
-    if (o.tb == 1024)
+    if (B.o.tb == 1024)
         launch(std::integral_constant<int, 1024>{});
-    else if (o.tb == 512)
+    else if (B.o.tb == 512)
         launch(std::integral_constant<int, 512>{});
     else
         launch(std::integral_constant<int, 256>{});
+}
+}  // namespace
+}  // namespace gf
+
+extern "C" int64_t gf_pcg_dir_bench_bytes(const gf_mesh *m, int32_t fp32) {
+    if (!m || m->n_cells < 1) return -1;
+    return (int64_t)bench_layout(m, fp32, nullptr).bytes;
+}
+
+extern "C" int gf_pcg_dir_bench(const gf_mesh *m, int32_t phase, int32_t fp32, int32_t reps, const double *diag,
+                                const double *upper, const double *lower, const double *x, double *y, void *scratch,
+                                gf_stream_t stream) {
+    GF_REQUIRE(m && scratch, "null argument");
+    GF_REQUIRE(m->n_cells >= 1, "empty grid");
+    GF_REQUIRE(phase >= 0 && phase <= 2, "phase must be 0 (prepare), 1 (launch) or 2 (result)");
+    hipStream_t s = S(stream);
+    const BenchLayout B = bench_layout(m, fp32, reinterpret_cast<char *>(scratch));
+    if (phase == 0) {
+        GF_REQUIRE(diag && upper && lower && x, "null argument");
+        if (fp32)
+            bench_pack<float>(m, B, diag, upper, lower, x, s);
+        else
+            bench_pack<double>(m, B, diag, upper, lower, x, s);
+    } else if (phase == 1) {
+        GF_REQUIRE(reps >= 1, "reps must be >= 1");
+        if (fp32)  // mixed-precision operator: FP32 values and vectors, FP64 diagonal and accumulation
+            bench_run<float, true>(B, reps, s);
+        else
+            bench_run<double, false>(B, reps, s);
+    } else {
+        GF_REQUIRE(y, "null argument");
+        if (fp32)
+            hipLaunchKernelGGL(k_bench_out<float>, dim3(grid_for(m->n_cells)), dim3(kBlock), 0, s, m->n_cells,
+                               (const float *)B.ap, B.perm, y);
+        else
+            hipLaunchKernelGGL(k_bench_out<double>, dim3(grid_for(m->n_cells)), dim3(kBlock), 0, s, m->n_cells,
+                               (const double *)B.ap, B.perm, y);
+    }
     GF_CHECK_LAUNCH();
     return GF_OK;
 }

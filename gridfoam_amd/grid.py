@@ -26,6 +26,7 @@ import torch
 
 from . import _native as nat
 from .config import FluxelConfig, GridfoamConfig
+from .solver_layout import build_solver_layout
 
 
 def _i32(n: int, dev) -> torch.Tensor:
@@ -238,7 +239,24 @@ class OctreeGrid:
         self.sell_code = _i32(nnz, dev)
         m.sell_ptr, m.sell_col, m.sell_code = (nat.ptr(t) for t in (self.sell_ptr, self.sell_col, self.sell_code))
         nat.call("gf_sell_fill", mp, nat.ptr(self.sell_col), nat.ptr(self.sell_code), st)
+        self._build_solver_layout()
         torch.cuda.current_stream().synchronize()
+
+    def _build_solver_layout(self) -> None:
+        """Solver row order and brick layout of the symmetric PCG operator
+        (solver_layout.py), kept on the device with the rest of the mesh."""
+        F = self.num_internal_faces
+        lay = build_solver_layout(self.level[: self.num_cells], self.ijk[: self.num_cells], self.owner_i32[:F],
+                                  self.neighbour_i32[:F], self.axis_i32[:F], list(self.fluxel.root_resolution))
+        self.solver_layout = lay
+        m = self._mesh
+        m.n_bricks, m.n_expl, m.pnnz_pad = lay.n_bricks, lay.n_expl, int(lay.psell_col.numel())
+        keep = lambda t: t if t.numel() else torch.zeros(1, dtype=torch.int32, device=self.device)  # noqa: E731
+        self._layout_keep = [keep(t) for t in (lay.row_perm, lay.row_iperm, lay.brick_nbr, lay.brick_vp, lay.brick_vm,
+                                               lay.expl_col, lay.expl_code, lay.psell_ptr, lay.psell_col,
+                                               lay.psell_code)]
+        (m.row_perm, m.row_iperm, m.brick_nbr, m.brick_vp, m.brick_vm, m.expl_col, m.expl_code, m.psell_ptr,
+         m.psell_col, m.psell_code) = (nat.ptr(t) for t in self._layout_keep)
 
     # ------------------------------------------------------- reference contract
     @property

@@ -125,6 +125,9 @@ class LinearSolver:
         if nrhs > 1:
             mixed = False  # the FP32 inner solve takes one right-hand side; components solve in FP64
         cfg = self._cfg(mixed)
+        # a symmetric operator (upper == lower, e.g. the pressure Laplacian)
+        # lets a one-rhs PCG run on the brick layout of the solver row order
+        cfg.symmetric = int(cfg.method == 0 and nrhs == 1 and torch.equal(A._upper, A._lower))
         ws = self._ws.get(g, nrhs, cfg)
         x = x0.detach().clone().contiguous()
         bc = b.detach().contiguous()

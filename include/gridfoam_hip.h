@@ -66,6 +66,18 @@ typedef struct gf_mesh {
     const double *hang_offset, *hang_move_o, *hang_move_n; /* [H,3] */
     const int32_t *hang_cell;                /* [HC] cells adjacent to a hanging face */
     const int32_t *hc_row, *hc_code;         /* [HC+1], h (owner side) or -h-1 (neighbour side) */
+    /* solver row order of the symmetric one-rhs PCG (gridfoam_amd/solver_layout.py;
+     * n_bricks = 0 / NULL: not built, the grid's order is used): 64-cell
+     * same-level 4x4x4 bricks first (lane = in-brick Morton index), then the
+     * remaining cells in the grid's order */
+    int32_t n_bricks, n_expl;
+    int64_t pnnz_pad;                        /* padded entries of the solver-order sliced ELL */
+    const int32_t *row_perm, *row_iperm;     /* [C] solver row -> cell, cell -> solver row */
+    const int32_t *brick_nbr;                /* [nb*6] brick across side 2a+s: >=0 brick, -1 none, -2-e explicit block e */
+    const int32_t *brick_vp;                 /* [nb*3*64] +axis face code per lane (f: upper[f]; INT32_MIN none) */
+    const int32_t *brick_vm;                 /* [nb*3*16] -axis face code toward a neighbour brick (-f-1: lower[f]) */
+    const int32_t *expl_col, *expl_code;     /* [ne*64] explicit 2:1 slots: solver column, face code */
+    const int32_t *psell_ptr, *psell_col, *psell_code; /* sliced ELL over solver rows */
 } gf_mesh;
 
 const char *gf_last_error(void);
@@ -342,6 +354,7 @@ typedef struct gf_krylov_cfg {
     double inner_rtol;     /* mixed: the FP64 refinement runs once the FP32 recursive residual
                               is below inner_rtol x the last true residual (0 < inner_rtol < 1) */
     int32_t check_every;   /* host polls the device state every k iterations (PCG: k rounded up to even) */
+    int32_t symmetric;     /* caller guarantees upper == lower: a one-rhs PCG then runs on the brick layout */
 } gf_krylov_cfg;
 /* workspace bytes needed for (C, nrhs, cfg); caller allocates once */
 int64_t gf_krylov_workspace_bytes(const gf_mesh *m, int32_t nrhs, const gf_krylov_cfg *cfg);
@@ -358,15 +371,18 @@ int gf_krylov_release(const void *workspace);
 /* ldu_grads.py:11-53: gd = -sum lambda x, gu = -sum lambda_O x_N, gl = -sum lambda_N x_O */
 int gf_ldu_grads(const gf_mesh *m, int32_t nc, const double *lambda, const double *x,
                  double *g_diag, double *g_upper, double *g_lower, gf_stream_t stream);
-/* Pack LDU off-diagonals into sliced-ELL order (FP64 or FP32 values). */
-int gf_sell_pack(const gf_mesh *m, int32_t transpose, const double *upper, const double *lower,
-                 void *val, int32_t fp32, gf_stream_t stream);
-/* y = A x with the solver's own SpMV kernel, `reps` times in a row (benchmark
- * entry: FP64 operator, or FP32 values/vectors with an FP64 diagonal and FP64
- * accumulation); scratch of gf_sell_spmv_scratch_bytes(m) bytes */
-int64_t gf_sell_spmv_scratch_bytes(const gf_mesh *m);
-int gf_sell_spmv(const gf_mesh *m, const void *diag, const void *val, const void *x, void *y,
-                 int32_t fp32, int32_t reps, void *scratch, gf_stream_t stream);
+/* Benchmark of the PCG direction/SpMV kernel on its own (beta = 0, so p = z =
+ * x and Ap = A x), on the layout a symmetric one-rhs PCG solve uses (bricks
+ * when the mesh has them); no counterpart in the reference (measurement).
+ * phase 0 packs the operator from diag/upper/lower and copies x [C] (FP64,
+ * cell order) in; phase 1 launches the kernel reps times; phase 2 writes
+ * y = A x [C] (FP64, cell order).  fp32 = 1: the mixed-precision operator
+ * (FP32 values and vectors, FP64 diagonal and accumulation).  Scratch of
+ * gf_pcg_dir_bench_bytes(m, fp32) bytes, kept between the phases. */
+int64_t gf_pcg_dir_bench_bytes(const gf_mesh *m, int32_t fp32);
+int gf_pcg_dir_bench(const gf_mesh *m, int32_t phase, int32_t fp32, int32_t reps, const double *diag,
+                     const double *upper, const double *lower, const double *x, double *y, void *scratch,
+                     gf_stream_t stream);
 
 #ifdef __cplusplus
 }

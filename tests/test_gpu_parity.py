@@ -397,31 +397,41 @@ def test_simple_cavity2d_500_iterations_match_golden():
 
 
 # ----------------------------------------------------------------- SpMV benchmark entry
-@pytest.mark.parametrize("fp32", [False, True])
-def test_sell_spmv_entry_matches_ldu_multiply(fp32):
-    """gf_sell_spmv runs the solver's direction/SpMV kernel with beta = 0: y = A x."""
+def pcg_dir_spmv(g, pm, x, fp32: bool, reps: int = 3):
+    """y = A x through gf_pcg_dir_bench (the solver's direction kernel with beta = 0)."""
     import ctypes
 
     from gridfoam_amd import _native as nat
 
-    g, o = _grid("walls8")
-    P, oP, gamma, b = _poisson(g, o)
+    scratch = torch.empty(int(nat.lib().gf_pcg_dir_bench_bytes(g.mesh_ref, int(fp32))), dtype=torch.uint8,
+                          device=g.device)
+    y = torch.full_like(x, float("nan"))
+    args = (nat.ptr(pm._diag), nat.ptr(pm._upper), nat.ptr(pm._lower))
+    for phase in (0, 1, 2):
+        nat.call("gf_pcg_dir_bench", g.mesh_ref, ctypes.c_int32(phase), ctypes.c_int32(int(fp32)),
+                 ctypes.c_int32(reps), *args, nat.ptr(x), nat.ptr(y), nat.ptr(scratch), nat.stream())
+    return y
+
+
+@pytest.mark.parametrize("name", ["walls8", "refined3d", "odd", "cube3"])
+@pytest.mark.parametrize("fp32", [False, True])
+def test_pcg_direction_kernel_matches_ldu_multiply(name, fp32):
+    """The direction/SpMV kernel with beta = 0 on the solver layout (brick
+    slices through lanes, permutes, neighbour bricks and explicit 2:1
+    blocks; the other rows through the sliced ELL): y = A x of the LDU
+    multiply, for a symmetric operator with hanging faces."""
     from gridfoam_amd.fv import fvm
 
+    g, o = _grid(name)
+    P, oP, gamma, b = _poisson(g, o)
     pm = fvm.assemble(P, lap=-1.0, gamma=torch.tensor(gamma, device=g.device), corrected=False)
-    dt = torch.float32 if fp32 else torch.float64
-    _say(f"sell_spmv fp32={fp32}")
-    val = torch.empty(g.mesh.nnz_pad, dtype=dt, device=g.device)
-    nat.call("gf_sell_pack", g.mesh_ref, ctypes.c_int32(0), nat.ptr(pm._upper), nat.ptr(pm._lower), nat.ptr(val),
-             ctypes.c_int32(int(fp32)), nat.stream())
-    x = torch.tensor(np.random.default_rng(1).standard_normal(g.num_cells), device=g.device).to(dt)
-    y = torch.empty_like(x)
-    scratch = torch.empty(int(nat.lib().gf_sell_spmv_scratch_bytes(g.mesh_ref)), dtype=torch.uint8, device=g.device)
-    diag = pm._diag.contiguous()
-    nat.call("gf_sell_spmv", g.mesh_ref, nat.ptr(diag), nat.ptr(val), nat.ptr(x), nat.ptr(y), ctypes.c_int32(int(fp32)),
-             ctypes.c_int32(3), nat.ptr(scratch), nat.stream())
-    ref = _np(pm.multiply(x.double()))
+    _say(f"pcg direction kernel {name} fp32={fp32} bricks={g.mesh.n_bricks} explicit={g.mesh.n_expl}")
+    x = torch.tensor(np.random.default_rng(1).standard_normal(g.num_cells), device=g.device)
+    y = pcg_dir_spmv(g, pm, x, fp32)
+    ref = _np(pm.multiply(x))
     assert _rel(_np(y), ref) < (1e-6 if fp32 else 1e-14)
+    if name == "walls8":
+        assert g.mesh.n_bricks > 0 and g.mesh.n_expl > 0
 
 
 # ----------------------------------------------------------------- implicit adjoint
