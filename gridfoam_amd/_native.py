@@ -35,7 +35,12 @@ class GfMesh(ctypes.Structure):
         ("n_bricks", _i32), ("n_expl", _i32), ("pnnz_pad", _i64),
         ("row_perm", _vp), ("row_iperm", _vp), ("brick_nbr", _vp), ("brick_vp", _vp), ("brick_vm", _vp),
         ("expl_col", _vp), ("expl_code", _vp), ("psell_ptr", _vp), ("psell_col", _vp), ("psell_code", _vp),
+        ("bnd_face", _vp), ("bnd_snap", _vp),
     ]
+
+
+class GfShape(ctypes.Structure):
+    _fields_ = [("kind", _i32), ("axis", _i32), ("patch", _i32), ("pad", _i32), ("p", _f64 * 20)]
 
 
 class GfAssembly(ctypes.Structure):
@@ -72,6 +77,8 @@ _SIGS: dict[str, tuple[Any, list[Any]]] = {
     "gf_mesh_validate": (_i32, [_P(GfMesh), _P(ctypes.c_int32), _vp]),
     "gf_exclusive_scan_i32": (_i32, [_vp, _vp, _i32, _vp]),
     "gf_fill_nbr_lists": (_i32, [_P(GfMesh), _vp, _vp]),
+    "gf_fill_bnd_lists": (_i32, [_P(GfMesh), _vp, _vp]),
+    "gf_ap_cut": (_i32, [_i32] + [_vp] * 6 + [_i32, _P(GfShape), _f64] + [_vp] * 7 + [_vp]),
     "gf_sell_widths": (_i32, [_P(GfMesh), _vp, _vp]),
     "gf_sell_fill": (_i32, [_P(GfMesh), _vp, _vp, _vp]),
     "gf_lsq_ata_inv": (_i32, [_P(GfMesh), _vp, _vp]),

@@ -24,6 +24,7 @@ from .fields import CellField, DirichletBC, FaceField
 from .fv import ddt_coefficients, fvc, fvm, sn_grad_corrected
 from .fvmatrix import Equation, FvMatrix
 from .grid import OctreeGrid, create_grid
+from .immersed import apply_immersed_dirichlet_values, close_fixed_cell_flux
 from .solvers import LinearSolver, SolveStats, create_solver, resolve_solver
 
 logger = logging.getLogger(__name__)
@@ -66,7 +67,7 @@ def correct_flux(phi: FaceField, U: CellField, update_internal: bool = False) ->
     st = U.boundary_state()
     if update_internal:
         Uf = fvc.interpolate(U)
-        nat.call("gf_face_flux", g.mesh_ref, nat.ptr(Uf.single_buffer), nat.ptr(Uf.bnd_buffer), nat.ptr(st.covered),
+        nat.call("gf_face_flux", g.mesh_ref, nat.ptr(Uf.single_buffer), nat.ptr(Uf.bnd_buffer), nat.ptr(st.flux_keep),
                  nat.ptr(phi.single_buffer), nat.ptr(phi.bnd_buffer), nat.stream())
     nat.call("gf_constrain_bnd_flux", g.mesh_ref, nat.ptr(st.psi_b), nat.ptr(st.covered), nat.ptr(phi.bnd_buffer),
              nat.stream())
@@ -78,7 +79,7 @@ def compute_phi_hbya(phi_hbya: FaceField, HbyA: CellField, U: CellField, *, rAU:
     g = U.grid
     Hf = fvc.interpolate(HbyA)
     st = U.boundary_state()
-    nat.call("gf_face_flux", g.mesh_ref, nat.ptr(Hf.single_buffer), nat.ptr(Hf.bnd_buffer), nat.ptr(st.covered),
+    nat.call("gf_face_flux", g.mesh_ref, nat.ptr(Hf.single_buffer), nat.ptr(Hf.bnd_buffer), nat.ptr(st.flux_keep),
              nat.ptr(phi_hbya.single_buffer), nat.ptr(phi_hbya.bnd_buffer), nat.stream())
     if rAU is not None:
         _, b, c = abc
@@ -119,7 +120,7 @@ def solve_pressure_poisson(p: CellField, rAtU: CellField, div_phi_hbya: torch.Te
             set_reference_value(mat, p_ref_cell, p_ref_value)
         eq = Equation(p, mat)
         if compute_residual:
-            last = field_initial_residual(mat, p)
+            last = field_initial_residual(eq.fv_matrix, p)
             if first is None:
                 first = last
         s = final_solver if (final_solver is not None and corr == n_non_orthogonal_correctors) else solver
@@ -130,7 +131,17 @@ def solve_pressure_poisson(p: CellField, rAtU: CellField, div_phi_hbya: torch.Te
 
 
 def correct_phi(phi: FaceField, phi_hbya: FaceField, p_eqn: FvMatrix, p: CellField, rAtU: CellField) -> None:
-    """pressure_correction.py:270-305 (no snapped immersed cells to close)."""
+    """pressure_correction.py:270-335: phi = phiHbyA - pEqn.flux(p) with the
+    boundary normal-gradient flux, then the mass-balance closure of cells
+    fixed by immersed snapping."""
+    _face_flux_correction(phi, phi_hbya, p_eqn, p, rAtU)
+    close_fixed_cell_flux(phi, p)
+
+
+def _face_flux_correction(phi: FaceField, phi_hbya: FaceField, p_eqn: FvMatrix, p: CellField,
+                          rAtU: CellField) -> None:
+    """phi = phiHbyA - pEqn.flux(p) on internal faces; boundary faces
+    phiHbyA_b - rAtU_P |Sf| snGrad(p)_b (pressure_correction.py:197-217, :297-304)."""
     g = p.grid
     st = p.boundary_state()
     nat.call("gf_correct_phi", g.mesh_ref, nat.ptr(phi_hbya.single_buffer), nat.ptr(phi_hbya.bnd_buffer),
@@ -159,17 +170,19 @@ def apply_simplec(phi_hbya: FaceField, HbyA: CellField, p: CellField, rAU: CellF
     m = fvm.assemble(p, lap=1.0, gamma=d_r.data, corrected=sn_grad_corrected(g, p.name))
     neg = CellField(g, "rAU-rAtU")
     neg.data = -d_r.data
-    correct_phi(phi_hbya, phi_hbya, m, p, neg)
+    _face_flux_correction(phi_hbya, phi_hbya, m, p, neg)
     gp = fvc.grad(p)
     nat.call("gf_correct_velocity", ctypes.c_int32(g.num_cells), nat.ptr(HbyA.data), nat.ptr(neg.data), nat.ptr(gp),
              nat.ptr(HbyA.data), nat.stream())
 
 
 def correct_velocity(U: CellField, HbyA: CellField, rAtU: CellField, p: CellField) -> None:
-    """pressure_correction.py:338-363."""
+    """pressure_correction.py:338-363: U = HbyA - rAtU grad(p), then the
+    prescribed values of snapped immersed cells."""
     gp = fvc.grad(p)
     nat.call("gf_correct_velocity", ctypes.c_int32(U.grid.num_cells), nat.ptr(HbyA.data), nat.ptr(rAtU.data),
              nat.ptr(gp), nat.ptr(U.data), nat.stream())
+    apply_immersed_dirichlet_values(U)
 
 
 def compute_continuity_error(phi: FaceField, delta_t: float) -> Tuple[float, float]:
@@ -360,11 +373,13 @@ class SIMPLE(AlgorithmBase):
         pred._source = torch.empty_like(UEqn._source)
         nat.call("gf_predictor_source", ctypes.c_int32(g.num_cells), ctypes.c_int32(3), nat.ptr(UEqn._source),
                  nat.ptr(g.cell_volumes), nat.ptr(gp), nat.ptr(pred._source), nat.stream())
+        u_eq = Equation(U, pred)
         if U.name in self._residual_control:
-            self._record_residual(U.name, field_initial_residual(pred, U))
-        res = self.solvers["U"].solve(Equation(U, pred))
+            self._record_residual(U.name, field_initial_residual(u_eq.fv_matrix, U))
+        res = self.solvers["U"].solve(u_eq)
         U.data = res.solution
         self.solve_stats[U.name] = res.stats
+        UEqn = Equation(U, UEqn).fv_matrix  # simple.py:335: constrained pressure-free equation for H/A
         p_old = p.data.clone()
         pres = self._pressure_corrector(UEqn, transient=False, abc=None, final=False)
         if p.name in self._residual_control:
@@ -372,6 +387,7 @@ class SIMPLE(AlgorithmBase):
         correct_phi(self.phi, self.phi_hbya, pres.matrix, p, self.rAtU)
         nat.call("gf_relax_field", ctypes.c_int64(g.num_cells), ctypes.c_double(self.alpha_p), nat.ptr(p_old),
                  nat.ptr(p.data), nat.stream())
+        apply_immersed_dirichlet_values(p)
         correct_velocity(U, self.HbyA, self.rAtU, p)
         if "phi" in self._residual_control:
             self._record_residual("phi", continuity_residual(self.phi))
@@ -405,6 +421,7 @@ class PISO(AlgorithmBase):
         res = self.solvers["U"].solve(Equation(U, pred))
         U.data = res.solution
         self.solve_stats[U.name] = res.stats
+        UEqn = Equation(U, UEqn).fv_matrix  # piso.py:257
         for i in range(self.n_correctors):
             pres = self._pressure_corrector(UEqn, transient=True, abc=abc, final=i == self.n_correctors - 1)
             correct_phi(self.phi, self.phi_hbya, pres.matrix, p, self.rAtU)
@@ -470,12 +487,13 @@ class PIMPLE(AlgorithmBase):
             if final_after:
                 self._outer_converged = True
             UEqn = fvm.assemble(U, ddt=1.0, div=1.0, lap=-1.0, phi=self.phi, gamma=self.nu, abc=abc)
-            pred = _predictor(UEqn, U, fvc.grad(p))
+            u_eq = Equation(U, _predictor(UEqn, U, fvc.grad(p)))
             if U.name in self._residual_control:
-                self._record_residual(U.name, field_initial_residual(pred, U))
-            res = self.solvers["U"].solve(Equation(U, pred))
+                self._record_residual(U.name, field_initial_residual(u_eq.fv_matrix, U))
+            res = self.solvers["U"].solve(u_eq)
             U.data = res.solution
             self.solve_stats[U.name] = res.stats
+            UEqn = Equation(U, UEqn).fv_matrix  # pimple.py:346
             for i in range(self.n_correctors):
                 pres = self._pressure_corrector(UEqn, transient=True, abc=abc, final=i == self.n_correctors - 1)
                 if p.name in self._residual_control:

@@ -144,6 +144,40 @@ class RefinementRegionConfig(BaseModel, frozen=True):  # meta/config.py:47-87
         return self
 
 
+class ImmersedBodyConfig(BaseModel, frozen=True):
+    """An analytic immersed body; it stands in for the surface mesh
+    (`mesh_path`) the reference hands to fluxel's AP-IBM builder, and its
+    `name` is the patch both sides of every cut face carry.
+    kind "plane": normal to `axis` at `offset` (an open surface);
+    "box": axis-aligned, `min` - `max`;
+    "fourier": prism along z over r(theta) = radius (1 + sum_k a_k cos k theta
+    + b_k sin k theta), k = 1..len(a), around `center` (x, y)."""
+
+    name: str
+    kind: str
+    axis: int = 0
+    offset: float = 0.0
+    min: Optional[List[float]] = None
+    max: Optional[List[float]] = None
+    center: Optional[List[float]] = None
+    radius: float = 0.0
+    a: List[float] = Field(default_factory=list)
+    b: List[float] = Field(default_factory=list)
+
+    @model_validator(mode="after")
+    def _check(self) -> "ImmersedBodyConfig":
+        if self.kind not in ("plane", "box", "fourier"):
+            raise ValueError(f"unknown immersed body kind {self.kind!r}")
+        if self.name in {p.value for p in DomainBoundaryPatch}:
+            raise ValueError(f"immersed body name {self.name!r} is a domain patch name")
+        if self.kind == "box" and (self.min is None or self.max is None or len(self.min) != 3 or len(self.max) != 3):
+            raise ValueError("a box needs min and max (3 values each)")
+        if self.kind == "fourier" and (self.center is None or len(self.center) < 2 or len(self.a) > 8
+                                       or len(self.b) > 8 or self.radius <= 0.0):
+            raise ValueError("a fourier body needs center (x, y), radius > 0 and at most 8 a_k and b_k")
+        return self
+
+
 class FluxelConfig(BaseModel, frozen=True):  # meta/config.py:90-145
     domain: DomainConfig
     root_resolution: List[int]
@@ -153,6 +187,7 @@ class FluxelConfig(BaseModel, frozen=True):  # meta/config.py:90-145
     mesh_path: Optional[pathlib.Path] = None
     ibm_type: str = "axis_projected"
     motion: str = "static"
+    immersed_bodies: List[ImmersedBodyConfig] = Field(default_factory=list)
 
     @model_validator(mode="after")
     def _no_surface(self) -> "FluxelConfig":
@@ -251,7 +286,7 @@ class fvSolutionConfig(BaseModel, frozen=True):
 
 class BoundaryConditionConfig(BaseModel, frozen=True):  # meta/config.py:552-585
     type: BoundaryConditionType
-    patches: List[DomainBoundaryPatch]
+    patches: List[str]  # domain patches (DomainBoundaryPatch values) or immersed body names
     value: Optional[Union[float, List[float]]] = None
 
 

@@ -94,7 +94,7 @@ __global__ void k_lsq_grad(const gf_mesh m, const double *psi, const double *psi
             for (int j = 0; j < 3; ++j) an[q][j] += dp * wd[j];
         }
     }
-    for (int k = m.bnd_row[c]; k < m.bnd_row[c + 1]; ++k) {
+    GF_FOR_BND(m, c, k) {
         if (!covered[k]) continue;
         double wd[3];
         lsq_wd(m.bnd_d[3 * k + 0], m.bnd_d[3 * k + 1], m.bnd_d[3 * k + 2], wd);
@@ -164,7 +164,7 @@ __global__ void k_lsq_vjp(const gf_mesh m, const double *v, const uint8_t *cover
             acc[q] += (wd[0] * (vo[0] + vc[0]) + wd[1] * (vo[1] + vc[1])) + wd[2] * (vo[2] + vc[2]);
         }
     }
-    for (int k = m.bnd_row[c]; k < m.bnd_row[c + 1]; ++k) {
+    GF_FOR_BND(m, c, k) {
         double wd[3];
         lsq_wd(m.bnd_d[3 * k + 0], m.bnd_d[3 * k + 1], m.bnd_d[3 * k + 2], wd);
 #pragma unroll
@@ -191,7 +191,7 @@ __global__ void k_bc_state_vjp(const gf_mesh m, const double *frac, const uint8_
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= m.n_cells) return;
     double acc[NC] = {};
-    for (int k = m.bnd_row[c]; k < m.bnd_row[c + 1]; ++k) {
+    GF_FOR_BND(m, c, k) {
         double rv[NC], rg[NC];
 #pragma unroll
         for (int q = 0; q < NC; ++q) {
@@ -307,7 +307,7 @@ __global__ void k_gauss_grad(const gf_mesh m, const double *psi_s, const double 
 #pragma unroll
         for (int q = 0; q < NC; ++q) acc[q][a] -= psi_s[f * NC + q] * sf;
     }
-    for (int k = m.bnd_row[c]; k < m.bnd_row[c + 1]; ++k) {
+    GF_FOR_BND(m, c, k) {
         const int a = m.bnd_dir[k] / 2;
         const double sf = m.bnd_sign[k] * m.bnd_mag_sf[k];
 #pragma unroll
@@ -362,7 +362,7 @@ __global__ void k_reconstruct(const gf_mesh m, const double *phi_s, const double
         num[a] += phi_s[f];
         den[a] += m.mag_sf[f];
     }
-    for (int k = m.bnd_row[c]; k < m.bnd_row[c + 1]; ++k) {
+    GF_FOR_BND(m, c, k) {
         const int a = m.bnd_dir[k] / 2;
         num[a] += m.bnd_sign[k] * phi_b[k];
         den[a] += m.bnd_mag_sf[k];
@@ -384,7 +384,7 @@ __global__ void k_reconstruct_vjp_cells(const gf_mesh m, const double *g_U, doub
         const int f = m.nbr_face[k];
         den[m.axis[f]] += m.mag_sf[f];
     }
-    for (int k = m.bnd_row[c]; k < m.bnd_row[c + 1]; ++k) den[m.bnd_dir[k] / 2] += m.bnd_mag_sf[k];
+    GF_FOR_BND(m, c, k) den[m.bnd_dir[k] / 2] += m.bnd_mag_sf[k];
 #pragma unroll
     for (int j = 0; j < 3; ++j) s[3 * c + j] = g_U[3 * c + j] / den[j];
 }
@@ -414,7 +414,7 @@ __global__ void k_fvc_div(const gf_mesh m, const double *phi_s, const double *ph
 #pragma unroll
         for (int q = 0; q < NC; ++q) sn[q] += -phi_s[f * NC + q];
     }
-    for (int k = m.bnd_row[c]; k < m.bnd_row[c + 1]; ++k)
+    GF_FOR_BND(m, c, k)
 #pragma unroll
         for (int q = 0; q < NC; ++q) sb[q] += phi_b[k * NC + q];
     const double V = m.volumes[c];
@@ -637,7 +637,7 @@ __global__ void k_asm_cells(const gf_mesh m, const gf_assembly a, double *diag, 
 #pragma unroll
         for (int q = 0; q < NC; ++q) src[q] = src[q] + a.s_div * ts[q];
         double bs[NC] = {};
-        for (int k = m.bnd_row[c]; k < m.bnd_row[c + 1]; ++k) {
+        GF_FOR_BND(m, c, k) {
             if (!a.covered[k]) continue;
             const double fr = a.frac[k], flux = a.phi_b[k], dd = m.bnd_mag_d[k];
             sb += flux * (1.0 - fr);
@@ -660,11 +660,11 @@ __global__ void k_asm_cells(const gf_mesh m, const gf_assembly a, double *diag, 
         }
         double bs[NC] = {};
         const double gb = a.gamma_cell ? a.gamma_cell[c] : a.gamma;
-        for (int k = m.bnd_row[c]; k < m.bnd_row[c + 1]; ++k) {
+        GF_FOR_BND(m, c, k) {
             if (!a.covered[k]) continue;
             const double fr = a.frac[k];
             const double cond = gb * m.bnd_mag_sf[k];
-            const double coef = fr != 0.0 ? fr / m.bnd_mag_d[k] : 0.0;
+            const double coef = bnd_value_coef(m, k, fr);
             const double cv = cond * coef, cg = cond * (1.0 - fr);
             sb += -cv;
 #pragma unroll
@@ -915,7 +915,7 @@ __global__ void k_asmv_cells(const gf_mesh m, const gf_assembly a, const double 
             }
     }
     const double gb = a.gamma_cell ? a.gamma_cell[c] : a.gamma;
-    for (int k = m.bnd_row[c]; k < m.bnd_row[c + 1]; ++k) {
+    GF_FOR_BND(m, c, k) {
         double gphib = 0.0, grv[NC] = {}, grg[NC] = {};
         if (a.covered[k]) {
             const double fr = a.frac[k], dd = m.bnd_mag_d[k];
@@ -931,7 +931,7 @@ __global__ void k_asmv_cells(const gf_mesh m, const gf_assembly a, const double 
                 gphib = a.s_div * (gdc * (1.0 - fr) - s);
             }
             if (a.s_lap != 0.0) {
-                const double coef = fr != 0.0 ? fr / m.bnd_mag_d[k] : 0.0;
+                const double coef = bnd_value_coef(m, k, fr);
                 const double cv = gb * m.bnd_mag_sf[k] * coef, cg = gb * m.bnd_mag_sf[k] * (1.0 - fr);
                 double s = -coef * gdc;
 #pragma unroll
@@ -1104,7 +1104,7 @@ __global__ void k_correct_phi_bnd(const gf_mesh m, const double *phi_hb, const d
     if (covered[k]) {
         const int c = m.bnd_owner[k];
         const double fr = frac[k];
-        const double coef = fr != 0.0 ? fr / m.bnd_mag_d[k] : 0.0;
+        const double coef = bnd_value_coef(m, k, fr);
         const double sng = coef * (rv[k] - p[c]) + (1.0 - fr) * rg[k];
         out = out - rAtU[c] * m.bnd_mag_sf[k] * sng;
     }

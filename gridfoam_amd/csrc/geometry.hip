@@ -111,11 +111,11 @@ __global__ void k_row_counts(const gf_mesh m, int32_t *own_cnt, int32_t *nbr_cnt
 // Face-order contract of the per-cell face lists (one bit per violated rule):
 //   1 an index out of range (cell, axis or direction) or owner == neighbour
 //   2 internal faces not grouped by owner in ascending order
-//   4 domain faces not grouped by owner in ascending order
-//   8 own_row / bnd_row do not bound each face's owner group
+//   8 own_row does not bound each face's owner group, or a boundary face is
+//     missing from its owner's bnd_row / bnd_face list
 //  16 the neighbour does not lie on the +axis side of the owner
 // Every face is checked by one thread; flags are OR-ed into one word.
-enum : int { kBadIndex = 1, kOwnerOrder = 2, kBndOrder = 4, kRowMismatch = 8, kNbrSide = 16 };
+enum : int { kBadIndex = 1, kOwnerOrder = 2, kRowMismatch = 8, kNbrSide = 16 };
 __global__ void k_mesh_validate(const gf_mesh m, int32_t *flags) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     int bad = 0;
@@ -133,12 +133,36 @@ __global__ void k_mesh_validate(const gf_mesh m, int32_t *flags) {
         const int o = m.bnd_owner[f], d = m.bnd_dir[f];
         if (o < 0 || o >= m.n_cells || d < 0 || d > 5) {
             bad |= kBadIndex;
-        } else {
-            if (f > 0 && m.bnd_owner[f - 1] > o) bad |= kBndOrder;
-            if (m.bnd_row && (f < m.bnd_row[o] || f >= m.bnd_row[o + 1])) bad |= kRowMismatch;
+        } else if (m.bnd_row && m.bnd_face) {
+            bool listed = false;
+            for (int k = m.bnd_row[o]; k < m.bnd_row[o + 1]; ++k) listed |= m.bnd_face[k] == f;
+            if (!listed) bad |= kRowMismatch;
         }
     }
     if (bad) atomicOr(flags, bad);
+}
+
+// boundary faces grouped by owner; order fixed afterwards (ascending face index)
+__global__ void k_bnd_scatter(const gf_mesh m, int32_t *cursor, int32_t *bnd_face) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m.n_bnd) return;
+    const int c = m.bnd_owner[k];
+    const int slot = atomicAdd(cursor + c, 1);
+    bnd_face[m.bnd_row[c] + slot] = k;
+}
+__global__ void k_bnd_sort(const gf_mesh m, int32_t *bnd_face) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.n_cells) return;
+    const int b = m.bnd_row[c], e = m.bnd_row[c + 1];
+    for (int i = b + 1; i < e; ++i) {
+        int32_t x = bnd_face[i];
+        int j = i - 1;
+        while (j >= b && bnd_face[j] > x) {
+            bnd_face[j + 1] = bnd_face[j];
+            --j;
+        }
+        bnd_face[j + 1] = x;
+    }
 }
 
 // faces seen from their neighbour, grouped by cell; order fixed afterwards
@@ -306,7 +330,7 @@ __global__ void k_ata_inv(const gf_mesh m, double *out) {
         const double *co = m.centers + 3 * m.owner[m.nbr_face[k]];
         add_outer(a, cc[0] - co[0], cc[1] - co[1], cc[2] - co[2]);
     }
-    for (int k = m.bnd_row[c]; k < m.bnd_row[c + 1]; ++k)
+    GF_FOR_BND(m, c, k)
         add_outer(a, m.bnd_d[3 * k + 0], m.bnd_d[3 * k + 1], m.bnd_d[3 * k + 2]);
     double inv[9];
     pinv_sym3(a, inv);
@@ -430,8 +454,7 @@ extern "C" int gf_mesh_validate(const gf_mesh *m, int32_t *flags_out, gf_stream_
         std::string msg = "gf_mesh violates the face-order contract:";
         if (h & kBadIndex) msg += " index out of range or owner == neighbour;";
         if (h & kOwnerOrder) msg += " internal faces not grouped by owner in ascending order;";
-        if (h & kBndOrder) msg += " domain faces not grouped by owner in ascending order;";
-        if (h & kRowMismatch) msg += " own_row/bnd_row do not match the face owners;";
+        if (h & kRowMismatch) msg += " own_row/bnd_row/bnd_face do not match the face owners;";
         if (h & kNbrSide) msg += " neighbour not on the +axis side of the owner;";
         set_error(msg);
         return GF_EINVAL;
@@ -448,6 +471,21 @@ extern "C" int gf_fill_nbr_lists(const gf_mesh *m, int32_t *nbr_face, gf_stream_
     if (m->n_faces > 0)
         hipLaunchKernelGGL(k_nbr_scatter, dim3(grid_for(m->n_faces)), dim3(kBlock), 0, s, *m, cursor, nbr_face);
     hipLaunchKernelGGL(k_nbr_sort, dim3(grid_for(m->n_cells)), dim3(kBlock), 0, s, *m, nbr_face);
+    GF_CHECK_LAUNCH();
+    GF_CHECK_HIP(hipStreamSynchronize(s));
+    GF_CHECK_HIP(hipFree(cursor));
+    return GF_OK;
+}
+
+extern "C" int gf_fill_bnd_lists(const gf_mesh *m, int32_t *bnd_face, gf_stream_t stream) {
+    GF_REQUIRE(m && bnd_face, "null argument");
+    hipStream_t s = S(stream);
+    int32_t *cursor = nullptr;
+    GF_CHECK_HIP(hipMalloc(&cursor, sizeof(int32_t) * (m->n_cells + 1)));
+    GF_CHECK_HIP(hipMemsetAsync(cursor, 0, sizeof(int32_t) * (m->n_cells + 1), s));
+    if (m->n_bnd > 0)
+        hipLaunchKernelGGL(k_bnd_scatter, dim3(grid_for(m->n_bnd)), dim3(kBlock), 0, s, *m, cursor, bnd_face);
+    hipLaunchKernelGGL(k_bnd_sort, dim3(grid_for(m->n_cells)), dim3(kBlock), 0, s, *m, bnd_face);
     GF_CHECK_LAUNCH();
     GF_CHECK_HIP(hipStreamSynchronize(s));
     GF_CHECK_HIP(hipFree(cursor));

@@ -37,6 +37,22 @@ inline hipStream_t S(gf_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
         }                              \
     } while (0)
 
+// The boundary faces k of cell c (per-cell lists bnd_row / bnd_face; any
+// boundary-face storage order).
+#define GF_FOR_BND(m, c, k)                                                       \
+    for (int k##_i = (m).bnd_row[c], k = 0;                                       \
+         k##_i < (m).bnd_row[(c) + 1] && ((k = (m).bnd_face[k##_i]), true); ++k##_i)
+
+// Coefficient of (ref_v - psi_P) in the boundary normal gradient
+// (fv/boundary_ops.py:372-392): fraction / |d|; zero on pure Neumann faces
+// and on snapped Dirichlet faces (an immersed side within the Gibou distance
+// with fraction 1: its cell carries the prescribed value instead).
+__device__ __forceinline__ double bnd_value_coef(const gf_mesh &m, int k, double fr) {
+    if (fr == 0.0) return 0.0;
+    if (m.bnd_snap && m.bnd_snap[k] && fr == 1.0) return 0.0;
+    return fr / m.bnd_mag_d[k];
+}
+
 inline int grid_for(int64_t n, int block = kBlock) {
     int64_t g = (n + block - 1) / block;
     return (int)(g > 0 ? g : 1);

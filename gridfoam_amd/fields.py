@@ -109,7 +109,7 @@ class InletOutletBC(BoundaryCondition):
         phi = field.grid.get_facefield(self.phi_name)
         if phi is None:
             raise ValueError(f"Field {self.phi_name} is not found.")
-        inflow = phi.domain_bnd_data.index_select(0, idx) < 0.0
+        inflow = phi.bnd_data.index_select(0, idx) < 0.0
         state.frac.index_copy_(0, idx, inflow.to(torch.float64))
         val = self.inlet_value.detach().to(state.ref_v.device).reshape(1, -1)
         state.ref_v.index_copy_(0, idx, torch.where(inflow[:, None], val, torch.zeros_like(val)))
@@ -150,16 +150,16 @@ class FixedFluxPressureBC(BoundaryCondition):
         if coeff is None or (phi_hbya is None and HbyA is None):
             state.ref_g.index_fill_(0, idx, 0.0)
             return
-        cells = g.domain_bnd_owner.index_select(0, idx)
-        Sf = g.domain_bnd_Sf.index_select(0, idx)
+        cells = g.bnd_owner.index_select(0, idx)
+        Sf = g.bnd_Sf.index_select(0, idx)
         if phi_hbya is not None:
-            pred = phi_hbya.domain_bnd_data.index_select(0, idx)
+            pred = phi_hbya.bnd_data.index_select(0, idx)
         else:
             pred = (HbyA.data.index_select(0, cells) * Sf).sum(dim=1)
         if U is not None and patch in U.bcs:
             target = (U.boundary_state().psi_b.index_select(0, idx) * Sf).sum(dim=1)
         elif phi is not None:
-            target = phi.domain_bnd_data.index_select(0, idx)
+            target = phi.bnd_data.index_select(0, idx)
         else:
             state.ref_g.index_fill_(0, idx, 0.0)
             return
@@ -168,6 +168,14 @@ class FixedFluxPressureBC(BoundaryCondition):
         grad = torch.where(pos, (pred - target) / torch.where(pos, denom, torch.ones_like(denom)),
                            torch.zeros_like(denom))
         state.ref_g.index_copy_(0, idx, grad[:, None])
+
+
+def patch_key(patch) -> Union[DomainBoundaryPatch, str]:
+    """A condition's patch: a domain patch, or the name of an immersed body."""
+    if isinstance(patch, DomainBoundaryPatch):
+        return patch
+    name = str(patch)
+    return DomainBoundaryPatch(name) if name in {p.value for p in DomainBoundaryPatch} else name
 
 
 def create_boundary_condition(kind: BoundaryConditionType, value, cshape) -> BoundaryCondition:
@@ -199,6 +207,9 @@ class BoundaryState:
     nonassignable: torch.Tensor  # [B] uint8 (constrainHbyA faces)
     adjust_kind: torch.Tensor  # [B] int8: 0 none, 1 fixed value, 2 adjustable
     slip: Optional[torch.Tensor]  # [B] uint8 or None
+    # faces whose flux U_f . Sf is kept by flux_from_face_velocity: covered
+    # domain faces and every immersed side (fv/flux.py:59-71)
+    flux_keep: Optional[torch.Tensor] = None
     # autograd-connected copies of ref_v / ref_g, rebuilt on every call when a
     # Dirichlet value or Neumann gradient requires grad (else None)
     ref_v_ad: Optional[torch.Tensor] = None
@@ -217,14 +228,14 @@ class CellField:
             raise ValueError("component_shape must be () or (3,)")
         self.transient = transient
         self.data = torch.zeros((grid.num_cells, *self.component_shape), dtype=torch.float64, device=grid.device)
-        self.bcs: Dict[DomainBoundaryPatch, BoundaryCondition] = {}
+        self.bcs: Dict[Union[DomainBoundaryPatch, str], BoundaryCondition] = {}
         self.export = condition.export if condition is not None else True  # meta/config.py:757-761
         if condition is not None:
             self.data[:] = torch.as_tensor(condition.internal, dtype=torch.float64)
             for bc_cfg in condition.boundary.values():
                 bc = create_boundary_condition(bc_cfg.type, bc_cfg.value, self.component_shape)
                 for patch in bc_cfg.patches:
-                    self.bcs[DomainBoundaryPatch(patch)] = bc
+                    self.bcs[patch_key(patch)] = bc
         self._state: Optional[BoundaryState] = None
         self._dynamic: list = []
         self._valued: list = []
@@ -240,7 +251,7 @@ class CellField:
 
     def add_boundary_conditions(self, bcs: Dict[Union[str, DomainBoundaryPatch], BoundaryCondition]) -> None:
         for k, v in bcs.items():
-            self.bcs[DomainBoundaryPatch(k)] = v
+            self.bcs[patch_key(k)] = v
         self._state = None
 
     def update_history(self, *, reset: bool = False) -> None:
@@ -257,7 +268,7 @@ class CellField:
     def boundary_state(self) -> BoundaryState:
         """Per-face value-fraction arrays, psi_b refreshed from current data."""
         g = self.grid
-        B, nc, dev = g.num_domain_bnd_faces, self.nc, g.device
+        B, nc, dev = g.num_bnd_faces, self.nc, g.device
         if self._state is None:
             self._valued = []
             frac = torch.zeros(max(B, 1), dtype=torch.float64, device=dev)
@@ -267,12 +278,12 @@ class CellField:
             nonassign = torch.zeros(max(B, 1), dtype=torch.uint8, device=dev)
             kind = torch.zeros(max(B, 1), dtype=torch.int8, device=dev)
             slip = torch.zeros(max(B, 1), dtype=torch.uint8, device=dev)
-            dirs = g.domain_bnd_dir_id
             dynamic = []
             for patch, bc in self.bcs.items():
                 mask = torch.zeros(max(B, 1), dtype=torch.bool, device=dev)
-                mask[:B] = dirs == patch.to_direction()
-                kind[mask] = 1 if isinstance(bc, DirichletBC) else 2
+                mask[:B] = g.bnd_patch_mask(patch)
+                if isinstance(patch, DomainBoundaryPatch):  # adjustPhi balances domain patches only
+                    kind[mask] = 1 if isinstance(bc, DirichletBC) else 2
                 if bc.type == BoundaryConditionType.EMPTY:
                     continue
                 bc.fill(g, mask, frac, ref_v, ref_g)
@@ -287,8 +298,10 @@ class CellField:
                         dynamic.append((patch, bc, idx))
                 if isinstance(bc, (DirichletBC, NeumannBC)):
                     self._valued.append((bc, mask))
+            flux_keep = covered.clone()
+            flux_keep[g.num_domain_bnd_faces:B] = 1
             self._state = BoundaryState(frac, ref_v, ref_g, covered, None, nonassign, kind,
-                                        slip if bool(slip.any()) else None)
+                                        slip if bool(slip.any()) else None, flux_keep)
             self._dynamic = dynamic
         st = self._state
         if st.psi_b is None:
@@ -311,6 +324,10 @@ class CellField:
             st.ref_v.copy_(rv.detach())
             st.ref_g.copy_(rg.detach())
             st.ref_v_ad, st.ref_g_ad = rv, rg
+        if g.bnd_zero_dist.numel() and not torch.cuda.is_current_stream_capturing():
+            from .immersed import check_zero_distance  # noqa: PLC0415 (immersed imports this module)
+
+            check_zero_distance(self, st)
         nat.call("gf_bc_state", g.mesh_ref, ctypes.c_int32(nc), nat.ptr(self.data), nat.ptr(st.frac),
                  nat.ptr(st.ref_v), nat.ptr(st.ref_g), nat.ptr(st.covered), nat.ptr(st.psi_b), nat.stream())
         return st
@@ -324,7 +341,7 @@ class FaceField:
         self.name = name
         self.component_shape = tuple(component_shape)
         nc = 3 if self.component_shape == (3,) else 1
-        F, B = grid.num_internal_faces, grid.num_domain_bnd_faces
+        F, B = grid.num_internal_faces, grid.num_bnd_faces
         shp = (lambda n: (max(n, 1), 3)) if nc == 3 else (lambda n: (max(n, 1),))
         self._single = torch.zeros(shp(F), dtype=torch.float64, device=grid.device)
         self._bnd = torch.zeros(shp(B), dtype=torch.float64, device=grid.device)
@@ -353,6 +370,37 @@ class FaceField:
     @domain_bnd_data.setter
     def domain_bnd_data(self, v: torch.Tensor) -> None:
         self._bnd[: self.grid.num_domain_bnd_faces] = v
+
+    @property
+    def bnd_data(self) -> torch.Tensor:
+        """Every boundary face: [domain | immersed upper | immersed lower]."""
+        return self._bnd[: self.grid.num_bnd_faces]
+
+    @bnd_data.setter
+    def bnd_data(self, v: torch.Tensor) -> None:
+        self._bnd[: self.grid.num_bnd_faces] = v
+
+    @property
+    def immersed_upper(self) -> torch.Tensor:
+        """core/field.py:455-470: owner sides of the immersed faces (along +Sf)."""
+        B, Fi = self.grid.num_domain_bnd_faces, self.grid.num_immersed_faces
+        return self._bnd[B: B + Fi]
+
+    @immersed_upper.setter
+    def immersed_upper(self, v: torch.Tensor) -> None:
+        B, Fi = self.grid.num_domain_bnd_faces, self.grid.num_immersed_faces
+        self._bnd[B: B + Fi] = v
+
+    @property
+    def immersed_lower(self) -> torch.Tensor:
+        """Neighbour sides of the immersed faces (along -Sf)."""
+        B, Fi = self.grid.num_domain_bnd_faces, self.grid.num_immersed_faces
+        return self._bnd[B + Fi: B + 2 * Fi]
+
+    @immersed_lower.setter
+    def immersed_lower(self, v: torch.Tensor) -> None:
+        B, Fi = self.grid.num_domain_bnd_faces, self.grid.num_immersed_faces
+        self._bnd[B + Fi: B + 2 * Fi] = v
 
     @property
     def single_buffer(self) -> torch.Tensor:

@@ -111,7 +111,7 @@ def _grad_pullback(field: CellField, st, g_grad: Optional[torch.Tensor], hang_on
     then gf_bc_state_vjp (which also moves slip faces' ref_v cotangent onto
     the cell values).  g_rv / g_rg are added to in place; returns d/d psi."""
     g = field.grid
-    C, B, nc = g.num_cells, g.num_domain_bnd_faces, field.nc
+    C, B, nc = g.num_cells, g.num_bnd_faces, field.nc
     f64 = dict(dtype=torch.float64, device=g.device)
     g_psi = torch.zeros((C, nc), **f64)
     g_psi_b = torch.zeros((max(B, 1), nc), **f64)
@@ -236,7 +236,7 @@ class _GaussGradFn(torch.autograd.Function):
         cs = tuple(g_out.shape[1:-1])
         f64 = dict(dtype=torch.float64, device=g.device)
         g_s = torch.zeros((max(g.num_internal_faces, 1), *cs), **f64)
-        g_b = torch.zeros((max(g.num_domain_bnd_faces, 1), *cs), **f64)
+        g_b = torch.zeros((max(g.num_bnd_faces, 1), *cs), **f64)
         nat.call("gf_gauss_grad_vjp", g.mesh_ref, ctypes.c_int32(ctx.nc), nat.ptr(g_out.contiguous()),
                  nat.ptr(g_s), nat.ptr(g_b), nat.stream())
         return None, None, None, (g_s if need[3] else None), (g_b if need[4] else None)
@@ -260,7 +260,7 @@ class _ReconstructFn(torch.autograd.Function):
         f64 = dict(dtype=torch.float64, device=g.device)
         work = torch.empty((g.num_cells, 3), **f64)
         g_s = torch.zeros(max(g.num_internal_faces, 1), **f64)
-        g_b = torch.zeros(max(g.num_domain_bnd_faces, 1), **f64)
+        g_b = torch.zeros(max(g.num_bnd_faces, 1), **f64)
         nat.call("gf_reconstruct_vjp", g.mesh_ref, nat.ptr(g_out.contiguous()), nat.ptr(work), nat.ptr(g_s),
                  nat.ptr(g_b), nat.stream())
         return None, (g_s if need[1] else None), (g_b if need[2] else None)
@@ -270,7 +270,7 @@ def _div_pullback(g, nc: int, g_out: torch.Tensor, need) -> Tuple[Optional[torch
     cs = tuple(g_out.shape[1:])
     f64 = dict(dtype=torch.float64, device=g.device)
     g_s = torch.zeros((max(g.num_internal_faces, 1), *cs), **f64)
-    g_b = torch.zeros((max(g.num_domain_bnd_faces, 1), *cs), **f64)
+    g_b = torch.zeros((max(g.num_bnd_faces, 1), *cs), **f64)
     nat.call("gf_fvc_div_vjp", g.mesh_ref, ctypes.c_int32(nc), nat.ptr(g_out.contiguous()), nat.ptr(g_s),
              nat.ptr(g_b), nat.stream())
     return (g_s if need[0] else None), (g_b if need[1] else None)
@@ -319,9 +319,9 @@ class fvc:
         track = torch.is_grad_enabled() and (field.data.requires_grad or rv.requires_grad or rg.requires_grad)
         grad = fvc.grad(field, hang_only=True) if g.num_hanging_faces else None
         if track:  # face values carrying the graph (reverse mode: gf_interpolate_vjp, gf_bc_state_vjp)
-            F, B = g.num_internal_faces, g.num_domain_bnd_faces
+            F, B = g.num_internal_faces, g.num_bnd_faces
             out.single_data = _InterpFn.apply(field, field.data, grad)[:F]
-            out.domain_bnd_data = _BndValueFn.apply(field, st, field.data, rv, rg).view(out.bnd_buffer.shape)[:B]
+            out.bnd_data = _BndValueFn.apply(field, st, field.data, rv, rg).view(out.bnd_buffer.shape)[:B]
             return out
         nat.call("gf_interpolate", g.mesh_ref, ctypes.c_int32(field.nc), nat.ptr(field.data), nat.ptr(grad),
                  nat.ptr(out.single_buffer), nat.stream())
@@ -362,6 +362,25 @@ class fvc:
                  nat.stream())
         U.data = out
         return out
+
+    @staticmethod
+    def sn_grad_boundary(field: CellField) -> torch.Tensor:
+        """fv/fvc/sn_grad.py:46-52: the boundary blocks of snGrad, per
+        boundary face (domain, immersed upper, immersed lower): the mixed
+        normal gradient f/|d| (ref_v - psi_P) + (1 - f) ref_g of
+        boundary_ops.py:425-437 on faces with a condition, with the value
+        term zero on snapped Dirichlet faces (a placeholder there, not an
+        imposed gradient) and zero on faces without a condition."""
+        g = field.grid
+        B, nc = g.num_bnd_faces, field.nc
+        st = field.boundary_state()
+        f = st.frac[:B]
+        active = (f != 0) & ~(g.bnd_snap[:B].bool() & (f == 1))
+        coef = torch.where(active, f / torch.where(active, g.bnd_mag_d[:B], torch.ones_like(f)), torch.zeros_like(f))
+        psi = field.data.reshape(g.num_cells, nc)[g.bnd_owner[:B]]
+        out = coef[:, None] * (st.ref_v[:B] - psi) + (1.0 - f)[:, None] * st.ref_g[:B]
+        out = torch.where(st.covered[:B].bool()[:, None], out, torch.zeros_like(out))
+        return out.reshape((B, *field.component_shape))
 
     @staticmethod
     def sn_grad(field: CellField) -> torch.Tensor:

@@ -176,22 +176,22 @@ class FvMatrix:
                  nat.ptr(self._lower), nat.ptr(ffc), nat.ptr(pc), nat.ptr(out), nat.stream())
         return out[:F]
 
-    def with_fixed_values(self, cells: torch.Tensor, values: torch.Tensor) -> "FvMatrix":
+    def with_fixed_values(self, cells: torch.Tensor, values: torch.Tensor, *, unique: bool = False) -> "FvMatrix":
         """fvmatrix.py:325-394: symmetric elimination of prescribed cell values
         into a separate solve matrix (no face-flux correction); ``self`` is
-        unchanged.  Gradients flow to ``values`` and to the input source;
-        gradients of the input LDU entries through the elimination are not
-        provided (an input LDU that requires grad raises)."""
+        unchanged.  Gradients flow to ``values`` and to the input source, and,
+        when the input LDU entries carry a graph, to them as well (the
+        elimination is then written as differentiable tensor operations)."""
         _require(values, (cells.numel(), *self.field.component_shape), "fixed values")
         if cells.numel() == 0:
             return self
-        if any(t.requires_grad for t in (self._diag, self._upper, self._lower)):
-            raise NotImplementedError("gradients of LDU entries through with_fixed_values are not supported")
+        if torch.is_grad_enabled() and any(t.requires_grad for t in (self._diag, self._upper, self._lower)):
+            return self._with_fixed_values_graph(cells, values)
         g = self.grid
         nc = self.field.nc
         C = g.num_cells
         cells32 = cells.to(device=g.device, dtype=torch.int32).contiguous()
-        if torch.unique(cells32).numel() != cells32.numel():
+        if not unique and torch.unique(cells32).numel() != cells32.numel():  # unique=True: the caller vouches
             raise ValueError("fixed-value cells must be unique")
         vals = values.detach().to(torch.float64).reshape(cells.numel(), nc).contiguous()
         src = self._source.detach().contiguous()
@@ -205,6 +205,35 @@ class FvMatrix:
                  nat.ptr(out._lower), nat.ptr(out._source), nat.stream())
         if values.requires_grad or self._source.requires_grad:
             out._source = _FixedSource.apply(values, self._source, out._source, self, out, cells32, fixed)
+        return out
+
+
+    def _with_fixed_values_graph(self, cells: torch.Tensor, values: torch.Tensor) -> "FvMatrix":
+        """The elimination of fvmatrix.py:365-394 as tensor operations, so
+        that autograd carries gradients to the LDU entries too."""
+        g = self.grid
+        C, F = g.num_cells, g.num_internal_faces
+        cells = cells.to(device=g.device, dtype=torch.long)
+        fixed = torch.zeros(C, dtype=torch.bool, device=g.device)
+        fixed[cells] = True
+        presc = torch.zeros_like(self._source).index_copy(0, cells, values.to(torch.float64))
+        o, n = g.owner, g.neighbour
+        diag, upper, lower = self._diag, self._upper[:F], self._lower[:F]
+        unit = torch.where(diag.detach().sum() < 0, -torch.ones_like(diag), torch.ones_like(diag))
+        cdiag = torch.where(diag != 0, diag, unit)
+        conn = fixed[o] | fixed[n]
+        out = FvMatrix(self.field)
+        out._diag = torch.where(fixed, cdiag, diag)
+        pad = self._upper.shape[0] - F
+        zero = torch.zeros(pad, dtype=torch.float64, device=g.device)
+        out._upper = torch.cat([torch.where(conn, torch.zeros_like(upper), upper), zero])
+        out._lower = torch.cat([torch.where(conn, torch.zeros_like(lower), lower), zero])
+
+        def bc(c, like):
+            return c if like.dim() == 1 else c[:, None]
+        src = self._source.index_add(0, o, -bc(upper, presc[n]) * presc[n])
+        src = src.index_add(0, n, -bc(lower, presc[o]) * presc[o])
+        out._source = src.index_copy(0, cells, bc(cdiag[cells], values) * values.to(torch.float64))
         return out
 
 
@@ -239,11 +268,18 @@ class _FixedSource(torch.autograd.Function):
 
 
 class Equation:
-    """core/equation.py:6-68 (no immersed constraints: solve matrix = input)."""
+    """core/equation.py:6-68: the target field and its solve matrix, which
+    carries the immersed Dirichlet cell constraints (Gibou snapping) through
+    ``with_fixed_values``; the input matrix itself when there are none."""
 
     def __init__(self, target: CellField, fv_matrix: FvMatrix):
         self.name = target.name
         self.target = target
+        if target.grid.num_immersed_faces:
+            from .immersed import immersed_dirichlet_constraints  # noqa: PLC0415 (immersed imports fields)
+
+            cells, values = immersed_dirichlet_constraints(target)  # one cell at most once
+            fv_matrix = fv_matrix.with_fixed_values(cells, values, unique=True)
         self.fv_matrix = fv_matrix
 
 

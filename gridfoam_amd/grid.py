@@ -10,10 +10,15 @@ centres, Sf -- built entirely by the HIP library:
 2. ``gf_grid_geometry`` restates axis_projected.py:130-161, :715-811;
 3. ``gf_face_geometry`` / ``gf_lsq_ata_inv`` / ``gf_hang_geometry`` restate
    fv/kernels/face_geometry.py:107-262;
-4. the per-cell CSR face lists and the slice-of-64 ELL operator are the
-   device layout the assembly and Krylov kernels read.
-Immersed boundaries are not part of this build: every internal face is
-single-sided (``ap_is_immersed_faces`` is all False).
+4. ``gf_ap_cut`` cuts the internal faces by analytic immersed bodies
+   (immersed.py): a cut face becomes two boundary faces, so the internal
+   list holds the single-sided faces and the boundary list is
+   [domain | immersed upper sides | immersed lower sides];
+5. the per-cell CSR face lists, the slice-of-64 ELL operator and the PCG
+   solver layout are the device layout the assembly and Krylov kernels read.
+``num_internal_faces`` counts the single-sided faces (the LDU arrays cover
+them); ``num_all_internal_faces`` and ``ap_is_immersed_faces`` describe the
+faces before the cut.
 """
 
 from __future__ import annotations
@@ -73,8 +78,8 @@ class OctreeGrid:
             self._build_topology(fluxel, lo, hi, root)
         else:
             self._adopt_topology(topology)
-        self.num_immersed_faces = 0
         self._build_geometry(lo, hi, root)
+        self._apply_immersed(list(fluxel.immersed_bodies))
         self._build_operator()
 
     def _build_topology(self, fluxel: FluxelConfig, lo, hi, root) -> None:
@@ -147,14 +152,98 @@ class OctreeGrid:
         # boundary vectors from the owner centre (boundary_ops.py:128-146)
         cb = self.cell_centers[self.bnd_owner_i32.long()]
         self.bnd_d = (self.domain_bnd_face_centers[:B] - cb[:B]).contiguous()
-        self.bnd_mag_d = torch.linalg.vector_norm(self.bnd_d, dim=1).contiguous()
-        self.bnd_mag_sf = torch.linalg.vector_norm(self.domain_bnd_Sf[:B], dim=1).contiguous()
-        self.bnd_sign = (2.0 * (self.bnd_dir_i32[:B] % 2).double() - 1.0).contiguous()
-        if B and bool((self.bnd_mag_d == 0).any()):
+        if B and bool((torch.linalg.vector_norm(self.bnd_d, dim=1) == 0).any()):
             raise ValueError("zero cell-to-boundary distance")
 
+    def _apply_immersed(self, bodies: list) -> None:
+        """Cut the internal faces by the immersed bodies (gf_ap_cut) and turn
+        every cut face into two boundary faces: the internal list keeps the
+        single-sided faces in ascending order, the boundary list becomes
+        [domain | upper sides (owner, +Sf) | lower sides (neighbour, -Sf)]
+        (fv/boundary_ops.py:150-186, fv/kernels/face_geometry.py:128-131,
+        boundary_lsq_vectors :198-243)."""
+        from .immersed import N_DOMAIN_PATCHES, ap_cut, patch_ids
+
+        dev = self.device
+        F_all, B_d = self.num_internal_faces, self.num_domain_bnd_faces
+        self.num_all_internal_faces = F_all
+        self.immersed_bodies = bodies
+        self.patch_name_to_id = patch_ids(bodies)
+        dom_dir = self.bnd_dir_i32[:B_d].long()
+        bnd_sf = self.domain_bnd_Sf[:B_d]
+        self.ap_is_immersed_faces = torch.zeros(F_all, dtype=torch.bool, device=dev)
+        if bodies and F_all:
+            lo, hi = self.fluxel.domain.lower, self.fluxel.domain.upper
+            L = max(h - l for l, h in zip(lo, hi))
+            cut = ap_cut(self.owner_i32[:F_all], self.neighbour_i32[:F_all], self.axis_i32[:F_all],
+                         self.cell_centers, self.cell_sizes, self.face_centers[:F_all], bodies, L)
+            self.ap_is_immersed_faces = cut.is_immersed
+        F_i = int(self.ap_is_immersed_faces.sum())
+        self.num_immersed_faces = F_i
+        if F_i == 0:
+            self.bnd_patch_id = dom_dir.contiguous()
+            self.bnd_mag_d = torch.linalg.vector_norm(self.bnd_d, dim=1).contiguous()
+            self.bnd_mag_sf = torch.linalg.vector_norm(bnd_sf, dim=1).contiguous()
+            self.bnd_sign = (2.0 * (dom_dir % 2).double() - 1.0).contiguous()
+            self.bnd_Sf = bnd_sf.contiguous()
+            self.bnd_snap = torch.zeros(max(B_d, 1), dtype=torch.uint8, device=dev)
+            self.bnd_zero_dist = torch.zeros(0, dtype=torch.long, device=dev)
+            self.num_bnd_faces = B_d
+            self.ap_dist_owner_to_bnd = self.ap_dist_neighbour_to_bnd = torch.zeros(0, dtype=torch.float64,
+                                                                                    device=dev)
+            self.ap_owner_near_boundary = self.ap_neighbour_near_boundary = torch.zeros(0, dtype=torch.bool,
+                                                                                      device=dev)
+            self.ap_owner_bnd_patch_id = self.ap_neighbour_bnd_patch_id = torch.zeros(0, dtype=torch.long,
+                                                                                    device=dev)
+            return
+        imm = self.ap_is_immersed_faces
+        single = ~imm
+        self.ap_dist_owner_to_bnd, self.ap_dist_neighbour_to_bnd = cut.dist_owner, cut.dist_neighbour
+        self.ap_owner_near_boundary, self.ap_neighbour_near_boundary = cut.owner_near, cut.neighbour_near
+        self.ap_owner_bnd_patch_id, self.ap_neighbour_bnd_patch_id = cut.owner_patch, cut.neighbour_patch
+        o_all, n_all, a_all = (t[:F_all].long() for t in (self.owner_i32, self.neighbour_i32, self.axis_i32))
+        io, inb, ia = o_all[imm], n_all[imm], a_all[imm]
+        Sf_i = self.Sf[:F_all][imm]
+        mag_sf_i = torch.linalg.vector_norm(Sf_i, dim=1)
+        e = torch.zeros((F_i, 3), dtype=torch.float64, device=dev)
+        e[torch.arange(F_i, device=dev), ia] = 1.0
+        # single-sided internal faces (order kept: still grouped by owner)
+        F_s = F_all - F_i
+        self.num_internal_faces = F_s
+
+        def keep_single(t, shape):
+            out = torch.zeros(shape, dtype=t.dtype, device=dev)
+            out[:F_s] = t[:F_all][single]
+            return out
+
+        self.owner_i32 = keep_single(self.owner_i32, max(F_s, 1))
+        self.neighbour_i32 = keep_single(self.neighbour_i32, max(F_s, 1))
+        self.axis_i32 = keep_single(self.axis_i32, max(F_s, 1))
+        self.face_centers = keep_single(self.face_centers, (max(F_s, 1), 3))
+        self.Sf = keep_single(self.Sf, (max(F_s, 1), 3))
+        # boundary list [domain | upper | lower]
+        d_up = cut.dist_owner[:, None] * e
+        d_lo = -cut.dist_neighbour[:, None] * e
+        B = B_d + 2 * F_i
+        self.num_bnd_faces = B
+        self.bnd_owner_i32 = torch.cat([self.bnd_owner_i32[:B_d], io.int(), inb.int()]).contiguous()
+        self.bnd_dir_i32 = torch.cat([self.bnd_dir_i32[:B_d], (2 * ia + 1).int(), (2 * ia).int()]).contiguous()
+        self.bnd_patch_id = torch.cat([dom_dir, N_DOMAIN_PATCHES + cut.owner_patch,
+                                       N_DOMAIN_PATCHES + cut.neighbour_patch]).contiguous()
+        self.bnd_d = torch.cat([self.bnd_d[:B_d], d_up, d_lo]).contiguous()
+        self.bnd_mag_d = torch.cat([torch.linalg.vector_norm(self.bnd_d[:B_d], dim=1), cut.dist_owner,
+                                    cut.dist_neighbour]).contiguous()
+        self.bnd_mag_sf = torch.cat([torch.linalg.vector_norm(bnd_sf, dim=1), mag_sf_i, mag_sf_i]).contiguous()
+        self.bnd_sign = torch.cat([2.0 * (dom_dir % 2).double() - 1.0,
+                                   torch.ones(F_i, dtype=torch.float64, device=dev),
+                                   -torch.ones(F_i, dtype=torch.float64, device=dev)]).contiguous()
+        self.bnd_Sf = torch.cat([bnd_sf, Sf_i, -Sf_i]).contiguous()
+        self.bnd_snap = torch.cat([torch.zeros(B_d, dtype=torch.uint8, device=dev),
+                                   cut.owner_near.to(torch.uint8), cut.neighbour_near.to(torch.uint8)]).contiguous()
+        self.bnd_zero_dist = torch.nonzero(self.bnd_mag_d == 0).flatten()
+
     def _build_operator(self) -> None:
-        C, F, B, dev = self.num_cells, self.num_internal_faces, self.num_domain_bnd_faces, self.device
+        C, F, B, dev = self.num_cells, self.num_internal_faces, self.num_bnd_faces, self.device
         m = nat.GfMesh()
         self._mesh = m
         self._keep: list[torch.Tensor] = []
@@ -173,6 +262,10 @@ class OctreeGrid:
         self.nbr_row = _scan(nbr_cnt, C + 1)
         self.bnd_row = _scan(bnd_cnt, C + 1)
         m.own_row, m.nbr_row, m.bnd_row = (nat.ptr(t) for t in (self.own_row, self.nbr_row, self.bnd_row))
+        self.bnd_face = _i32(B, dev)
+        m.bnd_face = nat.ptr(self.bnd_face)
+        nat.call("gf_fill_bnd_lists", mp, nat.ptr(self.bnd_face), st)
+        m.bnd_snap = nat.ptr(self.bnd_snap)
         flags = ctypes.c_int32(0)
         nat.call("gf_mesh_validate", mp, ctypes.byref(flags), st)  # raises on a face-order violation
         self.nbr_face = _i32(F, dev)
@@ -287,14 +380,34 @@ class OctreeGrid:
     def domain_bnd_dir_id(self) -> torch.Tensor:
         return self.bnd_dir_i32[: self.num_domain_bnd_faces].long()
 
-    @cached_property
-    def ap_is_immersed_faces(self) -> torch.Tensor:
-        return torch.zeros(self.num_internal_faces, dtype=torch.bool, device=self.device)
-
     def get_domain_bnd_mask(self, patch) -> torch.Tensor:
         from .config import DomainBoundaryPatch
 
         return self.domain_bnd_dir_id == DomainBoundaryPatch(patch).to_direction()
+
+    def patch_id(self, patch) -> int:
+        """Boundary patch id: domain patches 0..5 (their direction), immersed
+        bodies 6 + their index; -1 for an unknown name."""
+        from .config import DomainBoundaryPatch
+
+        key = patch.value if isinstance(patch, DomainBoundaryPatch) else str(patch)
+        if key in {p.value for p in DomainBoundaryPatch}:
+            return DomainBoundaryPatch(key).to_direction()
+        return self.patch_name_to_id.get(key, -1)
+
+    def bnd_patch_mask(self, patch) -> torch.Tensor:
+        """Mask over all boundary faces (domain and immersed sides) of a patch."""
+        return self.bnd_patch_id[: self.num_bnd_faces] == self.patch_id(patch)
+
+    def ap_get_patch_mask(self, patch_name: str):
+        """axis_projected.py:394-410: (upper, lower) masks over the immersed faces."""
+        pid = self.patch_name_to_id.get(str(patch_name), -1) - 6
+        return self.ap_owner_bnd_patch_id == pid, self.ap_neighbour_bnd_patch_id == pid
+
+    @property
+    def bnd_owner(self) -> torch.Tensor:
+        """Owner cell of every boundary face (domain and immersed sides)."""
+        return self.bnd_owner_i32[: self.num_bnd_faces].long()
 
     def register_cellfield(self, field) -> None:
         self._cellfields[field.name] = field

@@ -39,9 +39,9 @@ typedef struct ihipStream_t *gf_stream_t; /* == hipStream_t */
  * (src/gridfoam/fv/kernels/face_geometry.py:20-72).
  *
  * Face-order contract: internal faces are grouped by owner in ascending
- * owner order (own_row[c] .. own_row[c+1] are cell c's owned faces), and
- * domain faces likewise by bnd_owner (bnd_row).  The neighbour lies on the
- * +axis side of the owner.  Faces in another order (fluxel's order is not
+ * owner order (own_row[c] .. own_row[c+1] are cell c's owned faces).  The
+ * neighbour lies on the +axis side of the owner.  Boundary faces may come
+ * in any order (per-cell lists bnd_row / bnd_face).  Faces in another order (fluxel's order is not
  * specified) are brought into this order by a stable sort by owner, e.g.
  * gridfoam_amd.grid.canonical_face_order; gf_mesh_validate checks it. */
 typedef struct gf_mesh {
@@ -78,6 +78,10 @@ typedef struct gf_mesh {
     const int32_t *brick_vm;                 /* [nb*3*16] -axis face code toward a neighbour brick (-f-1: lower[f]) */
     const int32_t *expl_col, *expl_code;     /* [ne*64] explicit 2:1 slots: solver column, face code */
     const int32_t *psell_ptr, *psell_col, *psell_code; /* sliced ELL over solver rows */
+    /* boundary faces of each cell: bnd_face[bnd_row[c] .. bnd_row[c+1]) (any
+     * boundary-face order: domain faces, then the two sides of immersed faces) */
+    const int32_t *bnd_face;                 /* [B] */
+    const uint8_t *bnd_snap;                 /* [B] immersed side within the Gibou distance (NULL: none) */
 } gf_mesh;
 
 const char *gf_last_error(void);
@@ -85,6 +89,26 @@ int gf_device_count(int *count);
 /* sizeof(gf_mesh), sizeof(gf_assembly), sizeof(gf_krylov_cfg): lets a
  * binding check its struct layouts against the library (no GPU needed) */
 int gf_abi_sizes(int64_t out[3]);
+
+/* ---- axis-projected immersed boundary (replaces the AP payload fluxel's
+ * builder hands to AxisProjectedGrid, src/gridfoam/core/grid/axis_projected.py:163-211;
+ * contract in docs/source/contributor_guide/architecture/data_contracts.rst:181-231)
+ * for analytic bodies.  kind 0: plane normal to `axis` at p[0] (open surface);
+ * 1: box p[0..2] - p[3..5]; 2: prism along z over the outline
+ * r(theta) = p[2] (1 + sum_k p[2+k] cos k theta + p[10+k] sin k theta), k = 1..8,
+ * centred at (p[0], p[1]).  Both sides of a cut face carry the body's patch. */
+typedef struct gf_shape {
+    int32_t kind, axis, patch, pad;
+    double p[20];
+} gf_shape;
+/* Per internal face: cut or not, axis distances owner/neighbour centre ->
+ * body, near-boundary flags (distance / dx <= dx / ref_length), patch ids
+ * (-1 where uncut); at most 16 bodies (host array). */
+int gf_ap_cut(int32_t n_faces, const int32_t *owner, const int32_t *neighbour, const int32_t *axis,
+              const double *centers, const double *sizes, const double *face_centers, int32_t n_shapes,
+              const gf_shape *shapes, double ref_length, uint8_t *is_immersed, double *dist_owner,
+              double *dist_neighbour, uint8_t *near_owner, uint8_t *near_neighbour, int32_t *patch_owner,
+              int32_t *patch_neighbour, gf_stream_t stream);
 
 /* ---- octree face-neighbour operator (replaces fluxel-cfd's
  * FluxelManager.build_axis_projected_mesh, src/gridfoam/core/grid/factory.py:32-65,
@@ -116,15 +140,19 @@ int gf_face_geometry(const gf_mesh *m, const double *face_centers, double *w, do
                      double *delta, uint8_t *is_hang, double h_min, gf_stream_t stream);
 int gf_row_counts(const gf_mesh *m, int32_t *own_cnt, int32_t *nbr_cnt, int32_t *bnd_cnt,
                   gf_stream_t stream);
-/* Checks the face-order contract above (index ranges; internal and domain
- * faces grouped by owner, ascending; own_row/bnd_row when set; neighbour on
- * the +axis side when centers is set).  flags_out (host) = OR of 1 index,
- * 2 internal order, 4 domain order, 8 row lists, 16 neighbour side;
+/* Checks the face-order contract above (index ranges; internal faces
+ * grouped by owner, ascending; own_row when set; every boundary face once in
+ * the bnd_row/bnd_face lists of its owner when set; neighbour on the +axis
+ * side when centers is set).  flags_out (host) = OR of 1 index, 2 internal
+ * order, 8 row lists, 16 neighbour side;
  * GF_EINVAL with a message when any is set.  Guards the boundary that
  * replaces GridBase's face lists (core/grid/base.py:19-275). */
 int gf_mesh_validate(const gf_mesh *m, int32_t *flags_out, gf_stream_t stream);
 int gf_exclusive_scan_i32(const int32_t *in, int32_t *out, int32_t n, gf_stream_t stream);
 int gf_fill_nbr_lists(const gf_mesh *m, int32_t *nbr_face, gf_stream_t stream);
+/* boundary faces grouped by owner (ascending face index in a group) into
+ * bnd_face, after gf_row_counts / scan gave bnd_row */
+int gf_fill_bnd_lists(const gf_mesh *m, int32_t *bnd_face, gf_stream_t stream);
 int gf_sell_widths(const gf_mesh *m, int32_t *slice_len, gf_stream_t stream);
 int gf_sell_fill(const gf_mesh *m, int32_t *sell_col, int32_t *sell_code, gf_stream_t stream);
 int gf_lsq_ata_inv(const gf_mesh *m, double *ata_inv, gf_stream_t stream);

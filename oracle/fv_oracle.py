@@ -105,10 +105,120 @@ def _scatter(idx: np.ndarray, vals: np.ndarray, n: int) -> np.ndarray:
     return out.reshape((n,) + vals.shape[1:])
 
 
-class Grid:
-    """Restates AxisProjectedGrid's topology + geometry for grids without IB."""
+# ---------------------------------------------------------------------------
+# Axis-projected immersed-boundary payload for analytic bodies
+# ---------------------------------------------------------------------------
+N_DOMAIN_PATCHES = 6
 
-    def __init__(self, lower, upper, root, regions=(), n_leaf=0, dt=1.0):
+
+def _fourier_outline(body: dict, x: np.ndarray, y: np.ndarray) -> np.ndarray:
+    """|P - c| - r(theta), r = radius (1 + sum_k a_k cos k theta + b_k sin k theta):
+    negative inside the outline."""
+    a = list(body.get("a", [])) + [0.0] * (8 - len(body.get("a", [])))
+    b = list(body.get("b", [])) + [0.0] * (8 - len(body.get("b", [])))
+    cx, cy = body["center"][0], body["center"][1]
+    dx, dy = x - cx, y - cy
+    rho = np.sqrt(dx * dx + dy * dy)
+    th = np.arctan2(dy, dx)
+    r = np.ones_like(th)
+    for k in range(1, 9):
+        r = r + (a[k - 1] * np.cos(k * th) + b[k - 1] * np.sin(k * th))
+    return rho - body["radius"] * r
+
+
+def _body_hits(body: dict, ax: np.ndarray, P: np.ndarray, s0: np.ndarray, s1: np.ndarray):
+    """Hit coordinate along each face's axis ray [s0, s1] (through the face
+    centre P, endpoints included) seen from the s0 end and from the s1 end;
+    NaN where the ray misses the body."""
+    nan = np.full(len(ax), np.nan)
+    kind = body["kind"]
+    if kind == "plane":
+        s = float(body.get("offset", 0.0))
+        hit = (ax == int(body.get("axis", 0))) & (s0 <= s) & (s <= s1)
+        h = np.where(hit, s, nan)
+        return h, h.copy()
+    if kind == "box":
+        lo3, hi3 = np.asarray(body["min"], np.float64), np.asarray(body["max"], np.float64)
+        inside = np.ones(len(ax), bool)
+        for j in range(3):
+            inside &= (ax == j) | ((P[:, j] >= lo3[j]) & (P[:, j] <= hi3[j]))
+        lo, hi = lo3[ax], hi3[ax]
+        in_lo = inside & (lo >= s0) & (lo <= s1)
+        in_hi = inside & (hi >= s0) & (hi <= s1)
+        from_lo = np.where(in_lo, lo, np.where(in_hi, hi, nan))
+        from_hi = np.where(in_hi, hi, np.where(in_lo, lo, nan))
+        return from_lo, from_hi
+    if kind == "fourier":
+        def outline(s):
+            return _fourier_outline(body, np.where(ax == 0, s, P[:, 0]), np.where(ax == 1, s, P[:, 1]))
+        g0, g1 = outline(s0), outline(s1)
+        ok = (ax != 2) & ~(g0 * g1 > 0.0)
+        h = np.where(ok & (g0 == 0.0), s0, np.where(ok & (g1 == 0.0), s1, nan))
+        need = ok & (g0 != 0.0) & (g1 != 0.0)
+        lo, hi, glo = s0.copy(), s1.copy(), g0.copy()
+        active = need.copy()
+        for _ in range(64):  # bisection on the sign change
+            mid = 0.5 * (lo + hi)
+            active &= ~((mid <= lo) | (mid >= hi))
+            gm = outline(mid)
+            same = (gm <= 0.0) == (glo <= 0.0)
+            lo = np.where(active & same, mid, lo)
+            glo = np.where(active & same, gm, glo)
+            hi = np.where(active & ~same, mid, hi)
+        h = np.where(need, 0.5 * (lo + hi), h)
+        return h, h.copy()
+    raise ValueError(f"unknown immersed body kind {kind!r}")
+
+
+@dataclass
+class APCut:
+    is_immersed: np.ndarray  # [F] bool
+    dist_owner: np.ndarray   # [F_i]
+    dist_neighbour: np.ndarray
+    owner_near: np.ndarray   # [F_i] bool
+    neighbour_near: np.ndarray
+    owner_patch: np.ndarray  # [F_i] body index
+    neighbour_patch: np.ndarray
+
+
+def ap_cut(owner, neighbour, axis, centers, sizes, face_centers, bodies, ref_length: float) -> APCut:
+    """The AP-IBM payload fluxel hands to AxisProjectedGrid
+    (core/grid/axis_projected.py:163-211), defined by
+    docs/source/contributor_guide/architecture/data_contracts.rst:181-231, for
+    analytic bodies: a face is immersed when its axis ray from the owner's to
+    the neighbour's centre meets a body; the distances are ray distances from
+    each centre to the first hit on its side, the near flags Gibou's
+    distance / dx <= dx / L (dx the cell's width on the axis, L the largest
+    domain extent), and each side carries the patch of the body it hits (the
+    first body listed wins a tie)."""
+    F = len(owner)
+    s0 = centers[owner, axis]
+    s1 = centers[neighbour, axis]
+    best_o, best_n = np.full(F, np.inf), np.full(F, np.inf)
+    po, pn = np.full(F, -1), np.full(F, -1)
+    for k, body in enumerate(bodies):
+        h_lo, h_hi = _body_hits(body, axis, face_centers, s0, s1)
+        do, dn = h_lo - s0, s1 - h_hi
+        up_o = ~np.isnan(h_lo) & (do < best_o)
+        up_n = ~np.isnan(h_hi) & (dn < best_n)
+        best_o, po = np.where(up_o, do, best_o), np.where(up_o, k, po)
+        best_n, pn = np.where(up_n, dn, best_n), np.where(up_n, k, pn)
+    cut = po >= 0
+    dxo, dxn = sizes[owner, axis], sizes[neighbour, axis]
+    near_o = cut & (best_o / dxo <= dxo / ref_length)
+    near_n = cut & (best_n / dxn <= dxn / ref_length)
+    return APCut(cut, best_o[cut], best_n[cut], near_o[cut], near_n[cut], po[cut], pn[cut])
+
+
+class Grid:
+    """Restates AxisProjectedGrid's topology + geometry; with ``bodies`` the
+    internal faces cut by them become two boundary faces each (upper: owner
+    side, +Sf, distance to the body from the owner; lower: neighbour side,
+    -Sf), fv/boundary_ops.py:150-186.  The boundary arrays then hold
+    [domain faces | upper sides | lower sides]; the internal arrays the
+    single-sided faces in ascending order (face_geometry.py:128-131)."""
+
+    def __init__(self, lower, upper, root, regions=(), n_leaf=0, dt=1.0, bodies=()):
         topo = build_octree(lower, upper, root, regions, n_leaf)
         self.topo = topo
         self.lower = np.asarray(lower, np.float64)
@@ -152,18 +262,74 @@ class Grid:
         barea = sign * (sb[:, 0] * sb[:, 1] * sb[:, 2]) / sb[bi, baxis]
         self.bnd_Sf = np.zeros((self.num_bnd, 3))
         self.bnd_Sf[bi, baxis] = barea
+        # cell-to-boundary vectors (boundary_ops.py:128-146) and patch ids
+        self.bnd_d = self.bnd_face_centers - cb
+        self.bnd_mag_d = np.sqrt(np.sum(self.bnd_d * self.bnd_d, axis=1))
+        self.bnd_patch = self.bnd_dir.copy()
+        self.bnd_snap = np.zeros(self.num_bnd, bool)
+        self.num_domain_bnd = self.num_bnd
+        self.num_all_faces = self.num_faces
+        self.bodies = [dict(b) for b in bodies]
+        self.patch_ids = {b["name"]: N_DOMAIN_PATCHES + k for k, b in enumerate(self.bodies)}
+        self.is_immersed = np.zeros(self.num_faces, bool)
+        self.num_immersed = 0
+        if self.bodies:
+            self._cut(float(np.max(ext)))
         self.geo = FaceGeometry(self)
         # core/grid/axis_projected.py:370-386 -- registered fields by name;
         # each entry is a zero-argument callable returning the current value
         # (U, p: Field; phi, phiHbyA: FaceValues; HbyA, rAU, rAtU: arrays)
         self.registry: dict[str, Any] = {}
 
+    def _cut(self, L: float) -> None:
+        cut = ap_cut(self.owner, self.neighbour, self.axis, self.cell_centers, self.cell_sizes, self.face_centers,
+                     self.bodies, L)
+        self.ap = cut
+        imm = cut.is_immersed
+        Fi = int(imm.sum())
+        self.is_immersed, self.num_immersed = imm, Fi
+        if Fi == 0:
+            return
+        Sf_i = self.Sf[imm]
+        mag = np.sqrt(np.sum(Sf_i * Sf_i, axis=1))
+        n_hat = Sf_i / mag[:, None]
+        ax_i = self.axis[imm]
+        d_up = cut.dist_owner[:, None] * n_hat
+        d_lo = cut.dist_neighbour[:, None] * (-n_hat)
+        own_i, nbr_i = self.owner[imm], self.neighbour[imm]
+        self.bnd_owner = np.concatenate([self.bnd_owner, own_i, nbr_i])
+        self.bnd_dir = np.concatenate([self.bnd_dir, 2 * ax_i + 1, 2 * ax_i])
+        self.bnd_patch = np.concatenate([self.bnd_patch, N_DOMAIN_PATCHES + cut.owner_patch,
+                                         N_DOMAIN_PATCHES + cut.neighbour_patch])
+        self.bnd_d = np.concatenate([self.bnd_d, d_up, d_lo])
+        self.bnd_mag_d = np.concatenate([self.bnd_mag_d, cut.dist_owner, cut.dist_neighbour])
+        self.bnd_Sf = np.concatenate([self.bnd_Sf, Sf_i, -Sf_i])
+        self.bnd_face_centers = np.concatenate([self.bnd_face_centers, self.cell_centers[own_i] + d_up,
+                                                self.cell_centers[nbr_i] + d_lo])
+        self.bnd_snap = np.concatenate([self.bnd_snap, cut.owner_near, cut.neighbour_near])
+        self.num_bnd = len(self.bnd_owner)
+        single = ~imm
+        self.owner, self.neighbour, self.axis = self.owner[single], self.neighbour[single], self.axis[single]
+        self.face_centers, self.Sf = self.face_centers[single], self.Sf[single]
+        self.num_faces = len(self.owner)
+
+    def patch_id(self, patch: str) -> int:
+        if patch in PATCHES:
+            return PATCHES.index(patch)
+        return self.patch_ids.get(patch, -1)
+
     def patch_mask(self, patch: str) -> np.ndarray:
-        return self.bnd_dir == PATCHES.index(patch)
+        return self.bnd_patch == self.patch_id(patch)
+
+    @property
+    def domain_mask(self) -> np.ndarray:
+        return np.arange(self.num_bnd) < self.num_domain_bnd
 
 
 class FaceGeometry:
-    """fv/kernels/face_geometry.py:107-188 (all faces single-sided: no IB)."""
+    """fv/kernels/face_geometry.py:107-262 over the single-sided faces; the
+    least-squares normal matrix takes every boundary face's cell-to-boundary
+    vector (boundary_lsq_vectors, :198-243)."""
 
     def __init__(self, g: Grid):
         C = g.cell_centers
@@ -195,8 +361,7 @@ class FaceGeometry:
         outer = _lsq_outer(d_all)
         ata += _scatter(o, outer, g.num_cells)
         ata += _scatter(n, outer, g.num_cells)
-        d_b = g.bnd_face_centers - C[g.bnd_owner]
-        ata += _scatter(g.bnd_owner, _lsq_outer(d_b), g.num_cells)
+        ata += _scatter(g.bnd_owner, _lsq_outer(g.bnd_d), g.num_cells)
         self.lsq_ata_inv = np.linalg.pinv(ata, hermitian=True)
 
 
@@ -241,10 +406,14 @@ class Batch:
     mag_d: np.ndarray
     Sf: np.ndarray
     mag_Sf: np.ndarray
+    snap: np.ndarray  # immersed side within the Gibou distance
+    immersed: bool = False
 
 
 def batches(g: Grid, fld: Field) -> list[Batch]:
-    """boundary_ops.py:121-148 (domain patches, empty skipped)."""
+    """boundary_ops.py:121-186 in the order of the field's conditions, empty
+    skipped; an immersed patch is one batch of its upper sides followed by
+    its lower sides (the reference's upper batch, then its lower batch)."""
     out = []
     for patch, bc in fld.bcs.items():
         if bc.kind == "empty":
@@ -253,10 +422,9 @@ def batches(g: Grid, fld: Field) -> list[Batch]:
         if not m.any():
             continue
         cells = g.bnd_owner[m]
-        d = g.bnd_face_centers[m] - g.cell_centers[cells]
         Sf = g.bnd_Sf[m]
-        out.append(Batch(patch, m, cells, d, np.sqrt(np.sum(d * d, axis=1)), Sf,
-                         np.sqrt(np.sum(Sf * Sf, axis=1))))
+        out.append(Batch(patch, m, cells, g.bnd_d[m], g.bnd_mag_d[m], Sf, np.sqrt(np.sum(Sf * Sf, axis=1)),
+                         g.bnd_snap[m], patch not in PATCHES))
     return out
 
 
@@ -348,13 +516,90 @@ def bc_state(g: Grid, fld: Field, b: Batch):
     return f, rv, rg, psi_b
 
 
+def fixed_value_mask(b: Batch, f: np.ndarray) -> np.ndarray:
+    """boundary_ops.py:388-402: snapped immersed sides with fraction 1."""
+    return b.snap & (f == 1)
+
+
 def value_grad_coeff(b: Batch, f: np.ndarray) -> np.ndarray:
-    """boundary_ops.py:405-422 (no IB, so no snapped faces)."""
-    active = f != 0
+    """boundary_ops.py:405-422: f / |d|, zero on pure Neumann and snapped
+    Dirichlet faces."""
+    active = (f != 0) & ~fixed_value_mask(b, f)
     if np.any(active & (b.mag_d == 0)):
         raise ValueError("A mixed boundary at zero distance needs a cell constraint")
     dist = np.where(active, b.mag_d, 1.0)
     return np.where(active, f / dist, 0.0)
+
+
+def immersed_constraints(g: Grid, fld: Field) -> tuple[np.ndarray, np.ndarray]:
+    """boundary_ops.py:440-496: cells fixed by Gibou snapping and their
+    values.  Candidates are the snapped sides with fraction 1 of immersed
+    patches whose condition is not Neumann; a cell with several takes the
+    closest, ties in batch/face order."""
+    cells, dist, vals = [], [], []
+    for b in batches(g, fld):
+        if not b.immersed or fld.bcs[b.patch].kind == "neumann":
+            continue
+        f, rv, _ = bc_evaluate(g, fld, b)
+        fixed = fixed_value_mask(b, f)
+        if fixed.any():
+            cells.append(b.cells[fixed])
+            dist.append(b.mag_d[fixed])
+            vals.append(rv[fixed])
+    if not cells:
+        return np.zeros(0, np.int64), fld.data[:0].copy()
+    cell, d, v = np.concatenate(cells), np.concatenate(dist), np.concatenate(vals)
+    n = len(cell)
+    closest = np.full(g.num_cells, np.inf)
+    np.minimum.at(closest, cell, d)
+    cand = np.where(d == closest[cell], np.arange(n), n)
+    sel = np.full(g.num_cells, n)
+    np.minimum.at(sel, cell, cand)
+    ucells = np.nonzero(sel < n)[0]
+    return ucells, v[sel[ucells]]
+
+
+def apply_immersed_values(g: Grid, fld: Field) -> None:
+    """boundary_ops.py:538-542: restore prescribed values after an explicit update."""
+    cells, values = immersed_constraints(g, fld)
+    if len(cells):
+        fld.data = fld.data.copy()
+        fld.data[cells] = values
+
+
+def close_fixed_cell_flux(g: Grid, phi: FaceValues, p: Field) -> None:
+    """pressure_correction.py:308-335: on cells fixed by snapping, the cell's
+    flux imbalance is taken off its snapped Dirichlet sides in proportion
+    to their area."""
+    selected = []
+    area_sum = np.zeros(g.num_cells)
+    for b in batches(g, p):
+        f = bc_evaluate(g, p, b)[0]
+        fixed = fixed_value_mask(b, f)
+        if not fixed.any():
+            continue
+        area = np.where(fixed, b.mag_Sf, 0.0)
+        area_sum += _scatter(b.cells, area, g.num_cells)
+        selected.append((b, area))
+    if not selected:
+        return
+    imbalance = fvc_div(g, phi) * g.cell_volumes
+    for b, area in selected:
+        total = area_sum[b.cells]
+        frac = area / np.where(total > 0, total, 1.0)
+        phi.bnd[b.mask] = phi.bnd[b.mask] - frac * imbalance[b.cells]
+
+
+def sn_grad_boundary(g: Grid, fld: Field) -> np.ndarray:
+    """fv/fvc/sn_grad.py:40-60 boundary blocks: the mixed normal gradient
+    coeff (ref_v - psi_P) + (1 - f) ref_g of boundary_ops.py:425-437 on
+    covered faces (zero on snapped Dirichlet faces), zero elsewhere."""
+    out = np.zeros((g.num_bnd,) + fld.cshape)
+    for b in batches(g, fld):
+        f, rv, rg, _ = bc_state(g, fld, b)
+        c = value_grad_coeff(b, f)
+        out[b.mask] = _bcast(c, rv) * (rv - fld.data[b.cells]) + _bcast(1.0 - f, rg) * rg
+    return out
 
 
 def _bcast(v: np.ndarray, like: np.ndarray) -> np.ndarray:
@@ -432,6 +677,34 @@ class Matrix:
         if self.ffc is not None:
             out = out + self.ffc
         return out
+
+    def with_fixed_values(self, cells: np.ndarray, values: np.ndarray) -> "Matrix":
+        """fvmatrix.py:325-394: symmetric elimination of prescribed cell
+        values.  Prescribed columns move to the source, row and column
+        couplings of a fixed cell are zeroed, the fixed row keeps its
+        diagonal (a zero one becomes -1 when the diagonal sum is negative,
+        else +1) with source diag * value; no face-flux correction."""
+        if len(cells) == 0:
+            return self
+        g = self.g
+        fixed = np.zeros(g.num_cells, bool)
+        fixed[cells] = True
+        presc = np.zeros_like(self.source)
+        presc[cells] = values
+        unit = -1.0 if self.diag.sum() < 0 else 1.0
+        cdiag = np.where(self.diag != 0, self.diag, unit)
+        m = Matrix(g, self.cshape)
+        m.diag = np.where(fixed, cdiag, self.diag)
+        conn = fixed[g.owner] | fixed[g.neighbour]
+        m.upper = np.where(conn, 0.0, self.upper)
+        m.lower = np.where(conn, 0.0, self.lower)
+        pn, po = presc[g.neighbour], presc[g.owner]
+        src = self.source + _scatter(g.owner, -_bcast(self.upper, pn) * pn, g.num_cells)
+        src = src + _scatter(g.neighbour, -_bcast(self.lower, po) * po, g.num_cells)
+        src[cells] = _bcast(cdiag[cells], values) * values
+        m.source = src
+        m.ffc = None
+        return m
 
     def transpose(self) -> "Matrix":  # fvmatrix.py:396-412
         m = Matrix(self.g, self.cshape)
@@ -581,10 +854,11 @@ def covered_domain(g: Grid, fld: Field) -> np.ndarray:
 
 
 def flux_from_face_velocity(g: Grid, Uf: FaceValues, U: Field) -> FaceValues:
-    """fv/flux.py:32-71."""
+    """fv/flux.py:32-71: domain faces without a velocity condition carry no
+    flux; immersed sides always take U_f . Sf."""
     single = np.sum(Uf.single * g.Sf, axis=1)
     dom = np.sum(Uf.bnd * g.bnd_Sf, axis=1)
-    dom = np.where(~covered_domain(g, U), 0.0, dom)
+    dom = np.where(~covered_domain(g, U) & g.domain_mask, 0.0, dom)
     return FaceValues(single, dom)
 
 
@@ -959,6 +1233,8 @@ def adjust_phi(g: Grid, phi: FaceValues, U: Field, p: Field) -> bool:
         return False
     mass_in = fixed_out = adj_out = 0.0
     for patch, bc in U.bcs.items():
+        if patch not in PATCHES:  # immersed patches are not balanced
+            continue
         m = g.patch_mask(patch)
         if not m.any():
             continue
@@ -972,7 +1248,7 @@ def adjust_phi(g: Grid, phi: FaceValues, U: Field, p: Field) -> bool:
         return False
     corr = (mass_in - fixed_out) / adj_out
     for patch, bc in U.bcs.items():
-        if bc.kind == "dirichlet":
+        if bc.kind == "dirichlet" or patch not in PATCHES:
             continue
         m = g.patch_mask(patch)
         ph = phi.bnd[m]
@@ -998,18 +1274,20 @@ def solve_pressure(g, p, rAtU, div_hbya, cfg_p, cfg_final, n_nonorth, ref, corre
             rd = mat.diag[cell]
             mat.diag[cell] += rd
             mat.source[cell] += rd * val
-        res0 = initial_residual(mat, p.data)
+        smat = mat.with_fixed_values(*immersed_constraints(g, p))  # core/equation.py:42-47
+        res0 = initial_residual(smat, p.data)
         if first is None:
             first = res0
         if last is not None:
             last[:] = [res0]
         cfg = cfg_final if (cfg_final is not None and corr == n_nonorth) else cfg_p
-        p.data, st = solve(mat, p.data, cfg)
+        p.data, st = solve(smat, p.data, cfg)
     return mat, st, first
 
 
 def correct_phi(g, phi_hbya: FaceValues, pmat: Matrix, p: Field, rAtU: np.ndarray) -> FaceValues:
-    """pressure_correction.py:197-217, :270-305."""
+    """pressure_correction.py:197-217, :270-335 (snapped cells closed by
+    their mass balance)."""
     flux_p = (-pmat).flux(p.data)
     out = FaceValues(phi_hbya.single - flux_p, phi_hbya.bnd.copy())
     for b in batches(g, p):
@@ -1017,6 +1295,7 @@ def correct_phi(g, phi_hbya: FaceValues, pmat: Matrix, p: Field, rAtU: np.ndarra
         coeff = value_grad_coeff(b, f)
         sng = coeff * (rv - p.data[b.cells]) + (1.0 - f) * rg
         out.bnd[b.mask] = out.bnd[b.mask] - rAtU[b.cells] * b.mag_Sf * sng
+    close_fixed_cell_flux(g, out, p)
     return out
 
 
@@ -1098,7 +1377,8 @@ def grid_from_cfg(cfg: dict) -> Grid:
     fx = cfg["fluxel"]
     regions = [(r["min"], r["max"], r["level"]) for r in fx.get("refinement_regions", []) or []]
     return Grid(fx["domain"]["lower"], fx["domain"]["upper"], fx["root_resolution"], regions,
-                fx.get("n_leaf_refinement", 0), cfg["simulator"]["control"]["deltaT"])
+                fx.get("n_leaf_refinement", 0), cfg["simulator"]["control"]["deltaT"],
+                fx.get("immersed_bodies", []) or [])
 
 
 def _solver_cfg(d: dict) -> SolverCfg:
@@ -1226,16 +1506,20 @@ class Case:
         gp = cell_grad(g, p, self.grad_p)
         pred = UEqn.copy()
         pred.source = UEqn.source - g.cell_volumes[:, None] * gp
+        pred = pred.with_fixed_values(*immersed_constraints(g, U))
         if "U" in self.rc:  # simple.py:319-323
             self._simple_record("U", initial_residual(pred, U.data))
         U.data, self.stats["U"] = solve(pred, U.data, self.solvers["U"])
+        UEqn = UEqn.with_fixed_values(*immersed_constraints(g, U))  # simple.py:335
         p_old = p.data.copy()
         HbyA, rAU, phi_h, pmat = self._pressure_pass(UEqn, False, False)
         if "p" in self.rc:  # simple.py:379-380
             self._simple_record("p", self.stats["p_initial_residual"])
         self.phi = correct_phi(g, phi_h, pmat, p, rAU)
         p.data = p_old + ap * (p.data - p_old)
+        apply_immersed_values(g, p)
         U.data = HbyA.data - rAU[:, None] * cell_grad(g, p, self.grad_p)
+        apply_immersed_values(g, U)
         if "phi" in self.rc:  # simple.py:394-395
             self._simple_record("phi", float(np.linalg.norm(fvc_div(g, self.phi))))
 
@@ -1263,12 +1547,15 @@ class Case:
         gp = cell_grad(g, p, self.grad_p)
         pred = UEqn.copy()
         pred.source = UEqn.source - g.cell_volumes[:, None] * gp
+        pred = pred.with_fixed_values(*immersed_constraints(g, U))
         U.data, self.stats["U"] = solve(pred, U.data, self.solvers["U"])
+        UEqn = UEqn.with_fixed_values(*immersed_constraints(g, U))  # piso.py:257
         n_corr = int(self.alg.get("nCorrectors", 2))
         for i in range(n_corr):
             HbyA, rAU, phi_h, pmat = self._pressure_pass(UEqn, True, i == n_corr - 1, abc)
             self.phi = correct_phi(g, phi_h, pmat, p, rAU)
             U.data = HbyA.data - rAU[:, None] * cell_grad(g, p, self.grad_p)
+            apply_immersed_values(g, U)
         # field.py:241-255 -- history advance once per step
         U.older, U.old = U.old, U.data.copy()
         self.phi_older, self.phi_old = self.phi_old, self.phi.single.copy()
@@ -1306,9 +1593,11 @@ class Case:
             UEqn = self._momentum(True)
             pred = UEqn.copy()
             pred.source = UEqn.source - g.cell_volumes[:, None] * cell_grad(g, p, self.grad_p)
+            pred = pred.with_fixed_values(*immersed_constraints(g, U))
             if "U" in self.rc:
                 record("U", initial_residual(pred, U.data))
             U.data, self.stats["U"] = solve(pred, U.data, self.solvers["U"])
+            UEqn = UEqn.with_fixed_values(*immersed_constraints(g, U))  # pimple.py:346
             for i in range(n_corr):
                 HbyA, rAtU, phi_h, pmat = self._pressure_pass(UEqn, True, i == n_corr - 1, abc)
                 if "p" in self.rc:
@@ -1317,6 +1606,7 @@ class Case:
                     cur["p"] = self.stats["p_last_initial_residual"]
                 self.phi = correct_phi(g, phi_h, pmat, p, rAtU)
                 U.data = HbyA.data - rAtU[:, None] * cell_grad(g, p, self.grad_p)
+                apply_immersed_values(g, U)
             if "phi" in self.rc:
                 record("phi", float(np.linalg.norm(fvc_div(g, self.phi))))
             self.stats["outer"] = outer + 1

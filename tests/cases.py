@@ -137,3 +137,56 @@ def channel_flow(algorithm: str = "piso", *, p_outlet: str = "dirichlet", walls:
             "properties": {"transport": {"type": "newtonian", "nu": 0.01}, "turbulence": {"type": "laminar"}},
         },
     }
+
+
+def fourier_channel(root=(20, 8, 1), level: int = 1, algorithm: str = "pimple", dt: float = 0.05,
+                    consistent: bool = False) -> dict:
+    """The Fourier-body channel case on a coarser root grid and fewer levels
+    (same body, conditions and solvers)."""
+    cfg = copy.deepcopy(case_dict("channel2d_fourier_pimple.yaml"))
+    fx = cfg["fluxel"]
+    fx["root_resolution"] = list(root)
+    fx["domain"]["upper"][2] = 20.0 / root[0]
+    fx["refinement_regions"][0]["level"] = level
+    fx["refinement_regions"][0]["max"][2] = 20.0 / root[0]
+    sim = cfg["simulator"]
+    sim["control"]["deltaT"] = dt
+    sim["control"]["endTime"] = 10 * dt
+    alg = sim["fvSolution"]["algorithm"]
+    alg["type"] = algorithm
+    alg["consistent"] = consistent
+    alg.setdefault("relaxationFactors", {"equations": {"U": 0.7, "p": 0.3}})
+    for s in sim["fvSolution"]["solvers"].values():
+        s["tolerance"], s["rel_tolerance"] = 1e-11, 0.0
+    return cfg
+
+
+def box_cavity(n: int = 16, boxes=None, algorithm: str = "simple", depth: int = 1, consistent: bool = False) -> dict:
+    """The lid-driven cavity (lid u = 1, Re 100) with axis-aligned box
+    obstacles: no-slip, zero pressure gradient on the boxes; ``depth`` root
+    blocks in z (1: a 2-D slab with empty z faces)."""
+    cfg = small_cavity2d(n, consistent=consistent)
+    boxes = boxes if boxes is not None else [dict(name="block", kind="box", min=[0.28125, 0.25, -1.0],
+                                                  max=[0.53125, 0.5, 2.0])]
+    fx = cfg["fluxel"]
+    fx["immersed_bodies"] = boxes
+    fx["root_resolution"] = [n, n, depth]
+    fx["domain"]["upper"] = [1.0, 1.0, depth / n]
+    sim = cfg["simulator"]
+    sim["fvSolution"]["algorithm"]["type"] = algorithm
+    sim["fvSolution"]["algorithm"].setdefault("nCorrectors", 2)
+    sim["fvSolution"]["algorithm"].setdefault("nOuterCorrectors", 2)
+    sim["control"]["deltaT"] = 1.0 if algorithm == "simple" else 0.02
+    names = [b["name"] for b in boxes]
+    cond = sim["conditions"]
+    cond["U"]["boundary"]["obstacles"] = {"type": "dirichlet", "patches": names, "value": [0.0, 0.0, 0.0]}
+    cond["p"]["boundary"]["obstacles"] = {"type": "neumann", "patches": names, "value": 0.0}
+    if depth > 1:
+        for fld in ("U", "p"):
+            cond[fld]["boundary"].pop("empty")
+        cond["U"]["boundary"]["wall"]["patches"] = ["x_minus", "x_plus", "y_minus", "z_minus", "z_plus"]
+        cond["p"]["boundary"]["wall"]["patches"] = ["x_minus", "x_plus", "y_minus", "y_plus", "z_minus", "z_plus"]
+    for s in sim["fvSolution"]["solvers"].values():
+        s["tolerance"], s["rel_tolerance"] = 1e-11, 0.0
+        s["max_iter"] = 4000
+    return cfg

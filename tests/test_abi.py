@@ -58,10 +58,11 @@ def test_struct_layouts_match_library():
     assert sizes[0] == ctypes.sizeof(_native.GfMesh)
     assert sizes[1] == ctypes.sizeof(_native.GfAssembly)
     assert sizes[2] == ctypes.sizeof(_native.GfKrylovCfg)
-    # gf_mesh: six int32 counts + one int64, 30 pointers, then the solver
-    # layout: two int32 counts + one int64 and 10 pointers
+    # gf_mesh: six int32 counts + one int64, 30 pointers, the solver layout
+    # (two int32 counts + one int64 and 10 pointers), then bnd_face, bnd_snap
     assert _native.GfMesh.owner.offset == 32 and _native.GfMesh.n_bricks.offset == 32 + 30 * 8
-    assert ctypes.sizeof(_native.GfMesh) == 32 + 30 * 8 + 16 + 10 * 8
+    assert ctypes.sizeof(_native.GfMesh) == 32 + 30 * 8 + 16 + 10 * 8 + 2 * 8
+    assert ctypes.sizeof(_native.GfShape) == 16 + 20 * 8
 
 
 def test_product_path_has_no_cpu_fallback():
