@@ -15,6 +15,13 @@ cavity3d_octree_piso_2.npz  3-D lid-driven cavity on the octree (32^3 root,
                           numpy seed 0, the L2 norms of the whole U and p, and
                           the whole-field functionals sum(p r_p), sum(U r_U)
                           with r ~ N(0, 1) from numpy seed 1 (p first).
+channel2d_fourier_pimple_2.npz  2-D channel around the Fourier body (40x16x1
+                          root, four levels within one diameter, 205,600
+                          cells, Re 40, PIMPLE 2 outer x 2 inner correctors)
+                          after 2 steps in FP64: the same samples, norms and
+                          functionals as above, the AP payload (ray distances
+                          and near flags of the 2,144 immersed faces) and the
+                          corrected flux on their upper and lower sides.
 
 The reference itself cannot run in this environment (Python >= 3.12 source
 and the absent fluxel-cfd mesher), so the fixtures freeze the oracle, whose
@@ -64,6 +71,29 @@ def cavity3d_octree(steps: int = 2, n_sample: int = 20000) -> None:
                         p_iterations=np.int32(sum(st.iterations for st in case.stats["p"])))
 
 
+def channel2d_fourier(steps: int = 2, n_sample: int = 20000) -> None:
+    fo.use_threaded_spmv(8)
+    case = fo.Case(case_dict("channel2d_fourier_pimple.yaml"))
+    for _ in range(steps):
+        case.step()
+    g = case.g
+    C = g.num_cells
+    idx = np.sort(np.random.default_rng(0).choice(C, size=n_sample, replace=False)).astype(np.int32)
+    rng = np.random.default_rng(1)
+    r_p = rng.standard_normal(C)
+    r_U = rng.standard_normal((C, 3))
+    U, p = case.U.data, case.p.data
+    B = g.num_domain_bnd
+    np.savez_compressed(HERE / f"channel2d_fourier_pimple_{steps}.npz", idx=idx, U=U[idx], p=p[idx],
+                        U_norm=np.linalg.norm(U), p_norm=np.linalg.norm(p),
+                        f_p=float(np.dot(p, r_p)), f_U=float(np.sum(U * r_U)),
+                        cells=np.int64(C), steps=np.int32(steps), immersed=np.int64(g.num_immersed),
+                        dist_owner=g.ap.dist_owner, dist_neighbour=g.ap.dist_neighbour,
+                        owner_near=g.ap.owner_near, neighbour_near=g.ap.neighbour_near,
+                        phi_immersed=case.phi.bnd[B:], div_max=fo.continuity_max(g, case.phi),
+                        p_iterations=np.int32(sum(st.iterations for st in case.stats["p"])))
+
+
 def octree() -> None:
     f = FLUXEL["refined3d"]
     t = fo.build_octree(f["lower"], f["upper"], f["root"], f["regions"], f["n_leaf"])
@@ -72,10 +102,12 @@ def octree() -> None:
 
 
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["octree", "cavity2d", "cavity3d"]
+    which = sys.argv[1:] or ["octree", "cavity2d", "cavity3d", "channel2d"]
     if "octree" in which:
         octree()
     if "cavity2d" in which:
         cavity2d()
     if "cavity3d" in which:
         cavity3d_octree()  # about 3 minutes per step on 8 host cores
+    if "channel2d" in which:
+        channel2d_fourier()  # about 80 s per step on 8 host cores
