@@ -78,7 +78,7 @@ _SIGS: dict[str, tuple[Any, list[Any]]] = {
     "gf_exclusive_scan_i32": (_i32, [_vp, _vp, _i32, _vp]),
     "gf_fill_nbr_lists": (_i32, [_P(GfMesh), _vp, _vp]),
     "gf_fill_bnd_lists": (_i32, [_P(GfMesh), _vp, _vp]),
-    "gf_ap_cut": (_i32, [_i32] + [_vp] * 6 + [_i32, _P(GfShape), _f64] + [_vp] * 7 + [_vp]),
+    "gf_ap_cut": (_i32, [_i32] + [_vp] * 6 + [_i32, _P(GfShape), _f64] + [_vp] * 9 + [_vp]),
     "gf_sell_widths": (_i32, [_P(GfMesh), _vp, _vp]),
     "gf_sell_fill": (_i32, [_P(GfMesh), _vp, _vp, _vp]),
     "gf_lsq_ata_inv": (_i32, [_P(GfMesh), _vp, _vp]),

@@ -12,7 +12,7 @@ from __future__ import annotations
 
 import pathlib
 from enum import Enum
-from typing import Any, Dict, List, Optional, Union
+from typing import Any, Dict, List, Literal, Optional, Union
 
 import yaml
 from pydantic import BaseModel, Field, field_validator, model_validator
@@ -321,8 +321,61 @@ class SolverInfoConfig(BaseModel, frozen=True):  # meta/config.py:781-792
     writeInterval: int = Field(default=1, ge=1)
 
 
+class DragLiftCoord(BaseModel, frozen=True, extra="forbid"):  # meta/config.py:625-638
+    mode: Literal["drag_lift"] = "drag_lift"
+    drag_dir: List[float]
+    lift_dir: List[float]
+    center_of_rotation: List[float]
+
+    @model_validator(mode="after")
+    def _three(self) -> "DragLiftCoord":
+        if any(len(v) != 3 for v in (self.drag_dir, self.lift_dir, self.center_of_rotation)):
+            raise ValueError("drag_dir, lift_dir and center_of_rotation must have 3 values")
+        return self
+
+
+class DragPitchCoord(BaseModel, frozen=True, extra="forbid"):  # meta/config.py:641-656
+    mode: Literal["drag_pitch"] = "drag_pitch"
+    drag_dir: List[float]
+    pitch_axis: List[float]
+    center_of_rotation: List[float]
+
+    @model_validator(mode="after")
+    def _three(self) -> "DragPitchCoord":
+        if any(len(v) != 3 for v in (self.drag_dir, self.pitch_axis, self.center_of_rotation)):
+            raise ValueError("drag_dir, pitch_axis and center_of_rotation must have 3 values")
+        return self
+
+
+class ForceCoeffConfig(BaseModel, frozen=True):  # meta/config.py:665-740
+    local_coord: Union[DragLiftCoord, DragPitchCoord] = Field(discriminator="mode")
+    patches: List[str]
+    rho: float
+    magU_ref: float
+    A_ref: float
+    L_ref: float
+
+    @model_validator(mode="before")
+    @classmethod
+    def _infer_mode(cls, data: Any) -> Any:
+        """meta/config.py:710-740: the mode follows from lift_dir or pitch_axis."""
+        if not isinstance(data, dict):
+            return data
+        lc = data.get("local_coord")
+        if not isinstance(lc, dict):
+            raise ValueError("local_coord must be a dictionary")
+        if "mode" in lc:
+            return data
+        has_lift, has_pitch = "lift_dir" in lc, "pitch_axis" in lc
+        if has_lift and has_pitch:
+            raise ValueError("local_coord is ambiguous: provide either lift_dir or pitch_axis, not both.")
+        if not (has_lift or has_pitch):
+            raise ValueError("local_coord needs lift_dir or pitch_axis")
+        return {**data, "local_coord": {**lc, "mode": "drag_lift" if has_lift else "drag_pitch"}}
+
+
 class PostProcessingConfig(BaseModel, frozen=True):  # meta/config.py:794-810
-    forceCoeff: Optional[Dict[str, Any]] = None  # accepted; force evaluation needs immersed bodies
+    forceCoeff: Optional[ForceCoeffConfig] = None
     continuityError: Optional[ContinuityErrorConfig] = None
     solverInfo: Optional[SolverInfoConfig] = None
 

@@ -8,8 +8,10 @@
 // and the neighbour's centre meets the body surface (endpoints included),
 // the distances along the axis from each centre to the first hit on its side,
 // the near-boundary flags of Gibou et al. (distance / dx <= dx / L, dx the
-// cell's width along the axis, L the largest domain extent) and the patch of
-// the body hit on each side.  The ray runs through the face centre (its
+// cell's width along the axis, L the largest domain extent), the patch of
+// the body hit on each side and the body's outward unit normal at each hit
+// (it stands in for the surface-mesh face the reference anchors a side to,
+// post/forces/sampling.py:140-166).  The ray runs through the face centre (its
 // tangential coordinates), so a 2:1 face is cut where its finer side's
 // centre line meets the body.
 //
@@ -102,27 +104,56 @@ __device__ bool body_hit(const gf_shape &b, int a, const double P[3], double s0,
     return false;
 }
 
+// Body-outward unit normal of body b at the hit point (P with coordinate a
+// replaced by s); at_lo: a box hit on its lower face along the axis.
+__device__ void body_normal(const gf_shape &b, int a, const double P[3], double s, bool at_lo, double n[3]) {
+    n[0] = n[1] = n[2] = 0.0;
+    if (b.kind == 0) {  // the plane's own +axis normal
+        n[b.axis] = 1.0;
+    } else if (b.kind == 1) {
+        n[a] = at_lo ? -1.0 : 1.0;
+    } else {
+        // grad(|P - c| - R r(theta)) in the plane of the outline
+        double q[3] = {P[0], P[1], P[2]};
+        q[a] = s;
+        const double dx = q[0] - b.p[0], dy = q[1] - b.p[1];
+        const double rho2 = dx * dx + dy * dy, rho = sqrt(rho2);
+        const double th = atan2(dy, dx);
+        double dr = 0.0;  // r'(theta)
+        for (int k = 1; k <= 8; ++k) dr += k * (b.p[10 + k] * cos(k * th) - b.p[2 + k] * sin(k * th));
+        const double Rdr = b.p[2] * dr;
+        const double gx = dx / rho + Rdr * dy / rho2, gy = dy / rho - Rdr * dx / rho2;
+        const double gn = sqrt(gx * gx + gy * gy);
+        n[0] = gx / gn;
+        n[1] = gy / gn;
+    }
+}
+
 __global__ void k_ap_cut(int F, const int32_t *__restrict__ owner, const int32_t *__restrict__ neighbour,
                          const int32_t *__restrict__ axis, const double *__restrict__ centers,
                          const double *__restrict__ sizes, const double *__restrict__ face_centers, Shapes sh,
                          double L, uint8_t *is_imm, double *dist_o, double *dist_n, uint8_t *near_o,
-                         uint8_t *near_n, int32_t *patch_o, int32_t *patch_n) {
+                         uint8_t *near_n, int32_t *patch_o, int32_t *patch_n, double *nrm_o, double *nrm_n) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= F) return;
     const int o = owner[f], n = neighbour[f], a = axis[f];
     const double P[3] = {face_centers[3 * f + 0], face_centers[3 * f + 1], face_centers[3 * f + 2]};
     const double s0 = centers[3 * o + a], s1 = centers[3 * n + a];
-    double best_o = INFINITY, best_n = INFINITY;
-    int po = -1, pn = -1;
+    double best_o = INFINITY, best_n = INFINITY, hit_o = 0.0, hit_n = 0.0;
+    int po = -1, pn = -1, ko = 0, kn = 0;
     for (int k = 0; k < sh.n; ++k) {
         double h;
         if (body_hit(sh.s[k], a, P, s0, s1, true, &h) && h - s0 < best_o) {
             best_o = h - s0;
+            hit_o = h;
             po = sh.s[k].patch;
+            ko = k;
         }
         if (body_hit(sh.s[k], a, P, s0, s1, false, &h) && s1 - h < best_n) {
             best_n = s1 - h;
+            hit_n = h;
             pn = sh.s[k].patch;
+            kn = k;
         }
     }
     const bool cut = po >= 0;
@@ -134,6 +165,16 @@ __global__ void k_ap_cut(int F, const int32_t *__restrict__ owner, const int32_t
     const double dxo = sizes[3 * o + a], dxn = sizes[3 * n + a];
     near_o[f] = cut && best_o / dxo <= dxo / L;
     near_n[f] = cut && best_n / dxn <= dxn / L;
+    double no[3] = {0.0, 0.0, 0.0}, nn[3] = {0.0, 0.0, 0.0};
+    if (cut) {
+        const gf_shape &bo = sh.s[ko], &bn = sh.s[kn];
+        body_normal(bo, a, P, hit_o, bo.kind == 1 && hit_o == bo.p[a], no);
+        body_normal(bn, a, P, hit_n, bn.kind == 1 && hit_n == bn.p[a], nn);
+    }
+    for (int j = 0; j < 3; ++j) {
+        nrm_o[3 * f + j] = no[j];
+        nrm_n[3 * f + j] = nn[j];
+    }
 }
 
 }  // namespace
@@ -145,13 +186,14 @@ extern "C" int gf_ap_cut(int32_t n_faces, const int32_t *owner, const int32_t *n
                          const double *centers, const double *sizes, const double *face_centers, int32_t n_shapes,
                          const gf_shape *shapes, double ref_length, uint8_t *is_immersed, double *dist_owner,
                          double *dist_neighbour, uint8_t *near_owner, uint8_t *near_neighbour, int32_t *patch_owner,
-                         int32_t *patch_neighbour, gf_stream_t stream) {
+                         int32_t *patch_neighbour, double *normal_owner, double *normal_neighbour,
+                         gf_stream_t stream) {
     GF_REQUIRE(n_faces >= 0 && n_shapes >= 0 && n_shapes <= kMaxShapes, "bad sizes (at most 16 bodies)");
     GF_REQUIRE(ref_length > 0.0, "reference length must be positive");
     if (n_faces == 0) return GF_OK;
     GF_REQUIRE(owner && neighbour && axis && centers && sizes && face_centers && is_immersed && dist_owner &&
                    dist_neighbour && near_owner && near_neighbour && patch_owner && patch_neighbour &&
-                   (shapes || n_shapes == 0),
+                   normal_owner && normal_neighbour && (shapes || n_shapes == 0),
                "null argument");
     Shapes sh{};
     sh.n = n_shapes;
@@ -163,7 +205,7 @@ extern "C" int gf_ap_cut(int32_t n_faces, const int32_t *owner, const int32_t *n
     }
     hipLaunchKernelGGL(k_ap_cut, dim3(grid_for(n_faces)), dim3(kBlock), 0, S(stream), n_faces, owner, neighbour,
                        axis, centers, sizes, face_centers, sh, ref_length, is_immersed, dist_owner, dist_neighbour,
-                       near_owner, near_neighbour, patch_owner, patch_neighbour);
+                       near_owner, near_neighbour, patch_owner, patch_neighbour, normal_owner, normal_neighbour);
     GF_CHECK_LAUNCH();
     return GF_OK;
 }

@@ -195,12 +195,14 @@ class OctreeGrid:
                                                                                       device=dev)
             self.ap_owner_bnd_patch_id = self.ap_neighbour_bnd_patch_id = torch.zeros(0, dtype=torch.long,
                                                                                     device=dev)
+            self.ap_owner_normal = self.ap_neighbour_normal = torch.zeros((0, 3), dtype=torch.float64, device=dev)
             return
         imm = self.ap_is_immersed_faces
         single = ~imm
         self.ap_dist_owner_to_bnd, self.ap_dist_neighbour_to_bnd = cut.dist_owner, cut.dist_neighbour
         self.ap_owner_near_boundary, self.ap_neighbour_near_boundary = cut.owner_near, cut.neighbour_near
         self.ap_owner_bnd_patch_id, self.ap_neighbour_bnd_patch_id = cut.owner_patch, cut.neighbour_patch
+        self.ap_owner_normal, self.ap_neighbour_normal = cut.owner_normal, cut.neighbour_normal
         o_all, n_all, a_all = (t[:F_all].long() for t in (self.owner_i32, self.neighbour_i32, self.axis_i32))
         io, inb, ia = o_all[imm], n_all[imm], a_all[imm]
         Sf_i = self.Sf[:F_all][imm]

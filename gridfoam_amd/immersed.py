@@ -64,6 +64,8 @@ class APCut:
     neighbour_near: torch.Tensor
     owner_patch: torch.Tensor  # [F_i] body index
     neighbour_patch: torch.Tensor
+    owner_normal: torch.Tensor  # [F_i, 3] body-outward unit normal at the owner-side hit
+    neighbour_normal: torch.Tensor
 
 
 def ap_cut(owner, neighbour, axis, centers, sizes, face_centers, bodies: Sequence[ImmersedBodyConfig],
@@ -78,13 +80,15 @@ def ap_cut(owner, neighbour, axis, centers, sizes, face_centers, bodies: Sequenc
     nn = torch.zeros_like(imm)
     po = torch.zeros(max(F, 1), dtype=torch.int32, device=dev)
     pn = torch.zeros_like(po)
+    nrm_o = torch.zeros((max(F, 1), 3), dtype=torch.float64, device=dev)
+    nrm_n = torch.zeros_like(nrm_o)
     shapes = shapes_from_config(bodies)
     nat.call("gf_ap_cut", ctypes.c_int32(F), *map(nat.ptr, (owner, neighbour, axis, centers, sizes, face_centers)),
              ctypes.c_int32(len(bodies)), shapes, ctypes.c_double(ref_length),
-             *map(nat.ptr, (imm, do, dn, no, nn, po, pn)), nat.stream())
+             *map(nat.ptr, (imm, do, dn, no, nn, po, pn, nrm_o, nrm_n)), nat.stream())
     cut = imm[:F].bool()
     return APCut(cut, do[:F][cut].contiguous(), dn[:F][cut].contiguous(), no[:F][cut].bool(), nn[:F][cut].bool(),
-                 po[:F][cut].long(), pn[:F][cut].long())
+                 po[:F][cut].long(), pn[:F][cut].long(), nrm_o[:F][cut].contiguous(), nrm_n[:F][cut].contiguous())
 
 
 def _snap_candidates(field, st) -> torch.Tensor:

@@ -103,12 +103,15 @@ typedef struct gf_shape {
 } gf_shape;
 /* Per internal face: cut or not, axis distances owner/neighbour centre ->
  * body, near-boundary flags (distance / dx <= dx / ref_length), patch ids
- * (-1 where uncut); at most 16 bodies (host array). */
+ * (-1 where uncut) and the body-outward unit normals at the two hits
+ * ([3*F] each; they take the place of the surface-mesh anchor ids fluxel
+ * hands to the force sampling, post/forces/sampling.py:140-166); at most 16
+ * bodies (host array). */
 int gf_ap_cut(int32_t n_faces, const int32_t *owner, const int32_t *neighbour, const int32_t *axis,
               const double *centers, const double *sizes, const double *face_centers, int32_t n_shapes,
               const gf_shape *shapes, double ref_length, uint8_t *is_immersed, double *dist_owner,
               double *dist_neighbour, uint8_t *near_owner, uint8_t *near_neighbour, int32_t *patch_owner,
-              int32_t *patch_neighbour, gf_stream_t stream);
+              int32_t *patch_neighbour, double *normal_owner, double *normal_neighbour, gf_stream_t stream);
 
 /* ---- octree face-neighbour operator (replaces fluxel-cfd's
  * FluxelManager.build_axis_projected_mesh, src/gridfoam/core/grid/factory.py:32-65,

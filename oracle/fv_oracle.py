@@ -179,6 +179,38 @@ class APCut:
     neighbour_near: np.ndarray
     owner_patch: np.ndarray  # [F_i] body index
     neighbour_patch: np.ndarray
+    owner_normal: np.ndarray  # [F_i, 3] body-outward unit normal at the owner-side hit
+    neighbour_normal: np.ndarray
+
+
+def _body_normal(body: dict, ax: np.ndarray, P: np.ndarray, s: np.ndarray) -> np.ndarray:
+    """Body-outward unit normal at the hit s on each ray: a plane's own +axis
+    normal; a box's face normal (-axis on its lower face); the gradient of
+    |P - c| - R r(theta) for the Fourier outline."""
+    n = np.zeros((len(ax), 3))
+    kind = body["kind"]
+    if kind == "plane":
+        n[:, int(body.get("axis", 0))] = 1.0
+    elif kind == "box":
+        lo = np.asarray(body["min"], np.float64)[ax]
+        n[np.arange(len(ax)), ax] = np.where(s == lo, -1.0, 1.0)
+    else:
+        x = np.where(ax == 0, s, P[:, 0])
+        y = np.where(ax == 1, s, P[:, 1])
+        dx, dy = x - body["center"][0], y - body["center"][1]
+        rho2 = dx * dx + dy * dy
+        rho = np.sqrt(rho2)
+        th = np.arctan2(dy, dx)
+        a = list(body.get("a", [])) + [0.0] * (8 - len(body.get("a", [])))
+        b = list(body.get("b", [])) + [0.0] * (8 - len(body.get("b", [])))
+        dr = np.zeros_like(th)
+        for k in range(1, 9):
+            dr = dr + k * (b[k - 1] * np.cos(k * th) - a[k - 1] * np.sin(k * th))
+        Rdr = body["radius"] * dr
+        gx, gy = dx / rho + Rdr * dy / rho2, dy / rho - Rdr * dx / rho2
+        gn = np.sqrt(gx * gx + gy * gy)
+        n[:, 0], n[:, 1] = gx / gn, gy / gn
+    return n
 
 
 def ap_cut(owner, neighbour, axis, centers, sizes, face_centers, bodies, ref_length: float) -> APCut:
@@ -195,19 +227,26 @@ def ap_cut(owner, neighbour, axis, centers, sizes, face_centers, bodies, ref_len
     s0 = centers[owner, axis]
     s1 = centers[neighbour, axis]
     best_o, best_n = np.full(F, np.inf), np.full(F, np.inf)
+    hit_o, hit_n = np.zeros(F), np.zeros(F)
     po, pn = np.full(F, -1), np.full(F, -1)
     for k, body in enumerate(bodies):
         h_lo, h_hi = _body_hits(body, axis, face_centers, s0, s1)
         do, dn = h_lo - s0, s1 - h_hi
         up_o = ~np.isnan(h_lo) & (do < best_o)
         up_n = ~np.isnan(h_hi) & (dn < best_n)
-        best_o, po = np.where(up_o, do, best_o), np.where(up_o, k, po)
-        best_n, pn = np.where(up_n, dn, best_n), np.where(up_n, k, pn)
+        best_o, po, hit_o = np.where(up_o, do, best_o), np.where(up_o, k, po), np.where(up_o, h_lo, hit_o)
+        best_n, pn, hit_n = np.where(up_n, dn, best_n), np.where(up_n, k, pn), np.where(up_n, h_hi, hit_n)
     cut = po >= 0
     dxo, dxn = sizes[owner, axis], sizes[neighbour, axis]
     near_o = cut & (best_o / dxo <= dxo / ref_length)
     near_n = cut & (best_n / dxn <= dxn / ref_length)
-    return APCut(cut, best_o[cut], best_n[cut], near_o[cut], near_n[cut], po[cut], pn[cut])
+    nrm_o, nrm_n = np.zeros((F, 3)), np.zeros((F, 3))
+    for k, body in enumerate(bodies):
+        for pk, hit, nrm in ((po, hit_o, nrm_o), (pn, hit_n, nrm_n)):
+            sel = pk == k
+            if sel.any():
+                nrm[sel] = _body_normal(body, axis[sel], face_centers[sel], hit[sel])
+    return APCut(cut, best_o[cut], best_n[cut], near_o[cut], near_n[cut], po[cut], pn[cut], nrm_o[cut], nrm_n[cut])
 
 
 class Grid:
@@ -1625,6 +1664,90 @@ class Case:
             self.step_pimple()
         else:
             raise ValueError(f"unsupported algorithm {self.kind}")
+
+
+# ---------------------------------------------------------------------------
+# Force coefficients  (post/forces/*.py)
+# ---------------------------------------------------------------------------
+def _unit(v: np.ndarray) -> np.ndarray:
+    n = np.linalg.norm(v)
+    if n == 0:
+        raise ValueError("vector is zero vector")
+    return v / n
+
+
+def orthonormal_coord(drag_dir, lift_dir=None, pitch_axis=None):
+    """post/forces/coord.py:44-64 -> (e1, e2, e3)."""
+    e1 = _unit(np.asarray(drag_dir, np.float64))
+    if lift_dir is not None:
+        lift = np.asarray(lift_dir, np.float64)
+        e3 = _unit(lift - e1 * (lift @ e1))
+        return e1, np.cross(e3, e1), e3
+    pitch = np.asarray(pitch_axis, np.float64)
+    e2 = _unit(pitch - e1 * (pitch @ e1))
+    return e1, e2, np.cross(e1, e2)
+
+
+def force_coeffs(force, moment, coord, force_scale, moment_scale) -> dict:
+    """post/forces/coeffs.py:46-74."""
+    e1, e2, e3 = coord
+    Cd, Cs, Cl = (float(force @ e) / force_scale for e in (e1, e2, e3))
+    Cr, Cp, Cy = (float(moment @ e) / moment_scale for e in (e1, e2, e3))
+    return dict(Cd=Cd, Cd_f=0.5 * Cd + Cr, Cd_r=0.5 * Cd - Cr, Cl=Cl, Cl_f=0.5 * Cl + Cp, Cl_r=0.5 * Cl - Cp,
+                CmPitch=Cp, CmRoll=Cr, CmYaw=Cy, Cs=Cs, Cs_f=0.5 * Cs + Cy, Cs_r=0.5 * Cs - Cy)
+
+
+def _side_values(g: Grid, fld: Field, idx: np.ndarray) -> np.ndarray:
+    """sampling.py:28-65: the condition's boundary value on boundary faces
+    idx, the cell value where the patch has no condition."""
+    out = fld.data[g.bnd_owner[idx]].copy()
+    pos = {int(k): i for i, k in enumerate(idx)}
+    for b in batches(g, fld):
+        faces = np.nonzero(b.mask)[0]
+        hit = np.array([k in pos for k in faces], bool)
+        if hit.any():
+            psi_b = bc_state(g, fld, b)[3]
+            out[[pos[int(k)] for k in faces[hit]]] = psi_b[hit]
+    return out
+
+
+def patch_force(g: Grid, U: Field, p: Field, nu, patch: str, rho: float, cofr) -> tuple[np.ndarray, np.ndarray]:
+    """integration.py:61-187 + sampling.py:68-168: (force, moment) of one
+    immersed patch, summed over its kept axis-projected side samples."""
+    force, moment = np.zeros(3), np.zeros(3)
+    if g.num_immersed == 0:
+        return force, moment
+    Bd, Fi = g.num_domain_bnd, g.num_immersed
+    k_body = g.patch_ids[patch] - N_DOMAIN_PATCHES
+    nu_c = np.full(g.num_cells, float(nu)) if np.ndim(nu) == 0 else np.asarray(nu)
+    for side in (0, 1):
+        pid = g.ap.owner_patch if side == 0 else g.ap.neighbour_patch
+        sel = np.nonzero(pid == k_body)[0]
+        if len(sel) == 0:
+            continue
+        idx = Bd + side * Fi + sel
+        cells, Sf, mag_d = g.bnd_owner[idx], g.bnd_Sf[idx], g.bnd_mag_d[idx]
+        n_hat = Sf / np.linalg.norm(Sf, axis=1)[:, None]
+        n_out = (g.ap.owner_normal if side == 0 else g.ap.neighbour_normal)[sel]
+        n_s = -n_out
+        keep = np.sum(n_hat * n_s, axis=1) > 0.0
+        if not keep.any():
+            continue
+        p_b = _side_values(g, p, idx[keep])
+        U_b = _side_values(g, U, idx[keep])
+        U_c = U.data[cells[keep]]
+        ns = n_s[keep]
+        wall = mag_d[keep] * np.maximum(np.sum(n_hat[keep] * ns, axis=1), 1.0e-12)
+        dUdn = (U_b - U_c) / wall[:, None]
+        gradU = dUdn[:, :, None] * ns[:, None, :]
+        stress = rho * nu_c[cells[keep]][:, None, None] * (gradU + np.transpose(gradU, (0, 2, 1)))
+        fp = (rho * p_b)[:, None] * Sf[keep]
+        fv = -np.sum(stress * Sf[keep][:, None, :], axis=-1)
+        ff = fp + fv
+        pts = g.cell_centers[cells[keep]] + mag_d[keep][:, None] * n_hat[keep]
+        force += ff.sum(axis=0)
+        moment += np.cross(pts - np.asarray(cofr, np.float64), ff).sum(axis=0)
+    return force, moment
 
 
 def continuity_error(g: Grid, phi: FaceValues, dt: float) -> tuple[float, float]:
