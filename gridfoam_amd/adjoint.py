@@ -23,7 +23,7 @@ import torch
 from torch.utils.checkpoint import checkpoint
 
 from .config import DivScheme, SolverConfig
-from .fields import CellField, FaceField
+from .fields import CellField, DirichletBC, FaceField
 from .fv import fvm
 from .fvmatrix import Equation
 from .solvers import LinearSolver, create_solver
@@ -101,3 +101,214 @@ def rollout_scalar_transport(q: CellField, phi: FaceField, gamma, q0: torch.Tens
     step = ScalarTransport(q, phi, gamma, scheme=scheme, bdf2=bdf2, solver=solver)
     state: Sequence[torch.Tensor] = (q0, q0) if bdf2 else (q0,)
     return checkpointed_rollout(step, tuple(state), n_steps, every)[0]
+
+
+# --------------------------------------------------------------------------
+# Differentiable PISO / PIMPLE step
+# --------------------------------------------------------------------------
+def _pad(t: torch.Tensor, rows: int) -> torch.Tensor:
+    if t.shape[0] == rows:
+        return t
+    return torch.cat([t, torch.zeros((rows - t.shape[0],) + tuple(t.shape[1:]), dtype=t.dtype, device=t.device)])
+
+
+def _set_face(ff: FaceField, single: torch.Tensor, bnd: torch.Tensor) -> None:
+    """Point a face field's blocks at graph-carrying tensors."""
+    ff._single = _pad(single, ff._single.shape[0])
+    ff._bnd = _pad(bnd, ff._bnd.shape[0])
+
+
+class TransientStep:
+    """One PISO (algorithms/piso.py:227-326) or PIMPLE (pimple.py:290-454)
+    step of an existing driver, written with operations that carry the
+    autograd graph: native assembly with its VJP kernels (``gf_assemble`` /
+    ``gf_assemble_vjp``), Krylov solves with the transpose-solve adjoint, the
+    least-squares gradient, interpolation, boundary-value and divergence
+    nodes (``gf_lsq_grad_vjp``, ``gf_interpolate_vjp``, ``gf_bc_state_vjp``,
+    ``gf_fvc_div_vjp``), and tensor arithmetic for A, H, HbyA, the face
+    fluxes, ddtCorr, constrainHbyA, adjustPhi, the reference cell,
+    correct_phi and correct_velocity (pressure_correction.py:68-363,
+    fv/flux.py:32-129, fv/fvc/ddt_corr.py:13-78, fv/adjust_phi.py:44-127).
+    Gradients reach the initial state, the viscosity (a 0-d tensor given as
+    ``nu``) and boundary-condition values that require grad.
+
+    The state is ``(U, phi_single, phi_boundary, p, phiHbyA_single,
+    phiHbyA_boundary, rAU)`` at the start of a step (Euler time derivative:
+    U_old = U, phi_old = phi_single); the last three are the registered
+    fields that conditions such as fixedFluxPressure read, carried so that a
+    recomputed step sees what the original saw.  Those conditions' own
+    values (inletOutlet's switch, fixedFluxPressure's gradient) are
+    re-evaluated without a graph, so gradients do not run through them.
+    PIMPLE runs every outer corrector (no residual-control exit).  Not
+    covered: SIMPLEC (``consistent``), backward (BDF2) time derivatives, TVD
+    convection schemes and snapped immersed cells; those raise."""
+
+    def __init__(self, alg, *, nu: Union[float, torch.Tensor, None] = None):
+        from .algorithms import PIMPLE, PISO
+        from .config import DdtScheme
+        from .fv import ddt_scheme, div_scheme
+
+        if not isinstance(alg, (PISO, PIMPLE)):
+            raise TypeError("TransientStep runs a PISO or PIMPLE driver")
+        if alg.consistent:
+            raise NotImplementedError("SIMPLEC (consistent) steps are not differentiable here")
+        g = alg.grid
+        if ddt_scheme(g, "U") != DdtScheme.EULER:
+            raise NotImplementedError("only the Euler time derivative is differentiable here")
+        if div_scheme(g, "phi", "U") not in (DivScheme.UPWIND, DivScheme.LINEAR):
+            raise NotImplementedError("only upwind and linear convection are differentiable here")
+        self.alg, self.grid = alg, g
+        self.nu = alg.nu if nu is None else nu
+        self.n_outer = getattr(alg, "n_outer_correctors", 1)
+        self.n_corr = alg.n_correctors
+
+    # ---- pieces -------------------------------------------------------------
+    def _check_unconstrained(self, field) -> None:
+        g = self.grid
+        if g.num_immersed_faces:
+            from .immersed import immersed_dirichlet_constraints
+
+            if immersed_dirichlet_constraints(field)[0].numel():
+                raise NotImplementedError("snapped immersed cells are not differentiable here")
+
+    def _bnd_values(self, field) -> torch.Tensor:
+        from .fv import boundary_values
+
+        return boundary_values(field)[: self.grid.num_bnd_faces]
+
+    def _phi_hbya(self, HbyA: torch.Tensor, rAU: torch.Tensor, U_old: torch.Tensor, phi_old: torch.Tensor):
+        """compute_phi_hbya (fv/flux.py:74-129) with rAU_f ddtCorr."""
+        from .fv import fvc
+
+        g, alg = self.grid, self.alg
+        F, B = g.num_internal_faces, g.num_bnd_faces
+        U = alg.U
+        Hf_field = alg.HbyA
+        Hf_field.data = HbyA
+        Hf = fvc.interpolate(Hf_field)
+        Sf, bSf = g.Sf[:F], g.bnd_Sf[:B]
+        st = U.boundary_state()
+        keep = st.flux_keep[:B].to(torch.float64)
+        phi_s = (Hf.single_data * Sf).sum(dim=1)
+        phi_b = (Hf.bnd_data * bSf).sum(dim=1) * keep
+        # ddtCorr (Euler: b = 1, c = 0)
+        o, n = g.owner, g.neighbour
+        w = g.w[:F]
+        U0f = w[:, None] * U_old[o] + (1.0 - w)[:, None] * U_old[n]
+        corr = phi_old - (U0f * Sf).sum(dim=1)
+        tiny = torch.finfo(torch.float64).tiny
+        coupling = 1.0 - torch.clamp(corr.abs() / (phi_old.abs() + tiny), max=1.0)
+        rAUf = w * rAU[o] + (1.0 - w) * rAU[n]
+        phi_s = phi_s + rAUf * (coupling * corr / g.dt)
+        # constrainHbyA: the velocity condition's flux on non-assignable faces
+        nonassign = st.nonassignable[:B].bool()
+        Ub = self._bnd_values(U).reshape(B, 3)
+        phi_b = torch.where(nonassign, (Ub * bSf).sum(dim=1), phi_b)
+        if alg.adjust_phi_enabled and not any(isinstance(bc, DirichletBC) for bc in alg.p.bcs.values()):
+            phi_b = self._adjust_phi(phi_b, st.adjust_kind[:B])
+        return phi_s, phi_b
+
+    @staticmethod
+    def _adjust_phi(phi_b: torch.Tensor, kind: torch.Tensor) -> torch.Tensor:
+        """fv/adjust_phi.py:44-127 on the domain faces (kind 1 fixed value, 2 adjustable)."""
+        zero = torch.zeros_like(phi_b)
+        on = kind > 0
+        mass_in = -torch.where(on & (phi_b < 0), phi_b, zero).sum()
+        fixed_out = torch.where((kind == 1) & (phi_b > 0), phi_b, zero).sum()
+        adj_out = torch.where((kind == 2) & (phi_b > 0), phi_b, zero).sum()
+        if float(adj_out.detach()) <= 0.0:
+            return phi_b
+        corr = (mass_in - fixed_out) / adj_out
+        return torch.where((kind == 2) & (phi_b > 0), phi_b * corr, phi_b)
+
+    def _pressure(self, rAU: torch.Tensor, phi_s: torch.Tensor, phi_b: torch.Tensor, final: bool):
+        """solve_pressure_poisson (pressure_correction.py:68-160)."""
+        from .fv import fvc, fvm
+        from .solvers import resolve_solver
+
+        alg, g = self.alg, self.grid
+        _set_face(alg.phi_hbya, phi_s, phi_b)
+        div = fvc.div(alg.phi_hbya)
+        V = g.cell_volumes[: g.num_cells]
+        p = alg.p
+        self._check_unconstrained(p)
+        rf = alg.rAU
+        rf.data = rAU
+        mat = None
+        for corr in range(alg.n_non_orthogonal_correctors + 1):
+            mat = fvm.assemble(p, lap=-1.0, gamma=rAU)
+            src = mat.source - div * V
+            if alg.p_needs_ref:
+                c, v = alg.p_ref_cell, alg.p_ref_value
+                rd = mat.diag[c]
+                d = mat.diag.clone()
+                d[c] = d[c] + rd
+                src = src.clone()
+                src[c] = src[c] + rd * v
+                mat._diag = d
+            mat._source = src
+            last = corr == alg.n_non_orthogonal_correctors
+            solver = resolve_solver(alg.solvers, p.name, is_final=final) if last else alg.solvers["p"]
+            p.data = solver.solve(Equation(p, mat)).solution
+        return mat
+
+    def _correct_phi(self, phi_s, phi_b, pmat, rAU):
+        """correct_phi (pressure_correction.py:270-305): phiHbyA minus the
+        pressure flux; boundary faces with the condition's normal gradient."""
+        alg, g = self.alg, self.grid
+        F, B = g.num_internal_faces, g.num_bnd_faces
+        p = alg.p.data
+        o, n = g.owner, g.neighbour
+        flux = pmat.upper * p[n] - pmat.lower * p[o]
+        if pmat.face_flux_correction is not None:
+            flux = flux + pmat.face_flux_correction[:F]
+        phi_s = phi_s + flux  # pmat = -laplacian, so the laplacian's flux enters with a minus
+        st = alg.p.boundary_state()
+        f = st.frac[:B].clone()  # later refreshes rewrite the state arrays in place
+        rv = (st.ref_v_ad if st.ref_v_ad is not None else st.ref_v.clone())[:B, 0]
+        rg = (st.ref_g_ad if st.ref_g_ad is not None else st.ref_g.clone())[:B, 0]
+        active = (f != 0) & ~(g.bnd_snap[:B].bool() & (f == 1))
+        coef = torch.where(active, f / torch.where(active, g.bnd_mag_d[:B], torch.ones_like(f)), torch.zeros_like(f))
+        cells = g.bnd_owner[:B]
+        sng = coef * (rv - p[cells]) + (1.0 - f) * rg
+        cov = st.covered[:B].bool()
+        phi_b = torch.where(cov, phi_b - rAU[cells] * g.bnd_mag_sf[:B] * sng, phi_b)
+        return phi_s, phi_b
+
+    # ---- the step -------------------------------------------------------------
+    def __call__(self, state: State) -> State:
+        from .fv import fvc
+
+        U0, phi_s, phi_b, p0, ph_s, ph_b, rAU = state
+        alg, g = self.alg, self.grid
+        U, p = alg.U, alg.p
+        V = g.cell_volumes[: g.num_cells]
+        U.old_data, U.older_data = U0, None
+        U.data = U0
+        p.data = p0
+        _set_face(alg.phi_hbya, ph_s, ph_b)
+        alg.rAU.data = rAU
+        phi_old = phi_s
+        for outer in range(self.n_outer):
+            _set_face(alg.phi, phi_s, phi_b)
+            self._check_unconstrained(U)
+            UEqn = fvm.assemble(U, ddt=1.0, div=1.0, lap=-1.0, phi=alg.phi, gamma=self.nu, abc=(1.0, 1.0, 0.0))
+            pred = UEqn.with_source(UEqn.source - V[:, None] * fvc.grad(p))
+            U.data = alg.solvers["U"].solve(Equation(U, pred)).solution
+            for i in range(self.n_corr):
+                final = outer == self.n_outer - 1 and i == self.n_corr - 1
+                rAU = 1.0 / UEqn.A()
+                HbyA = rAU[:, None] * UEqn.H(U.data)
+                ph_s, ph_b = self._phi_hbya(HbyA, rAU, U0, phi_old)
+                pmat = self._pressure(rAU, ph_s, ph_b, final)
+                phi_s, phi_b = self._correct_phi(ph_s, ph_b, pmat, rAU)
+                _set_face(alg.phi, phi_s, phi_b)
+                U.data = HbyA - rAU[:, None] * fvc.grad(p)
+        return U.data, phi_s, phi_b, p.data, ph_s, ph_b, rAU
+
+
+def transient_state(alg) -> State:
+    """The (U, phi_single, phi_boundary, p, phiHbyA_single,
+    phiHbyA_boundary, rAU) state of a PISO / PIMPLE driver."""
+    return tuple(t.detach().clone() for t in (alg.U.data, alg.phi.single_data, alg.phi.bnd_data, alg.p.data,
+                                              alg.phi_hbya.single_data, alg.phi_hbya.bnd_data, alg.rAU.data))

@@ -17,6 +17,7 @@ reference re-evaluates them at each use.
 from __future__ import annotations
 
 import ctypes
+import dataclasses
 from dataclasses import dataclass
 from typing import Dict, Optional, Tuple, Union
 
@@ -40,6 +41,11 @@ class BoundaryCondition:
     def refresh(self, field: "CellField", patch: DomainBoundaryPatch, idx: torch.Tensor,
                 state: "BoundaryState") -> None:
         """Rewrite (frac, ref_v, ref_g) on the faces ``idx`` from current data."""
+
+    def graph_ref_g(self, field: "CellField", patch, idx: torch.Tensor) -> Optional[torch.Tensor]:
+        """The refreshed reference gradient on ``idx`` as a graph-carrying
+        tensor when its inputs require grad (else None)."""
+        return None
 
 
 class DirichletBC(BoundaryCondition):  # boundaries/basic/dirichlet.py:17-73
@@ -138,7 +144,8 @@ class FixedFluxPressureBC(BoundaryCondition):
         ref_v[mask] = 0.0
         ref_g[mask] = 0.0
 
-    def refresh(self, field, patch, idx, state):
+    def _gradient(self, field, patch, idx, graph: bool) -> Optional[torch.Tensor]:
+        """(phiHbyA_b - U_b . Sf) / (rAtU_P |Sf|) on ``idx``; None: zero gradient."""
         if field.component_shape != ():
             raise ValueError("FixedFluxPressure requires a scalar field")
         g = field.grid
@@ -148,8 +155,7 @@ class FixedFluxPressureBC(BoundaryCondition):
         if coeff is None:
             coeff = g.get_cellfield(self.rAU_name)
         if coeff is None or (phi_hbya is None and HbyA is None):
-            state.ref_g.index_fill_(0, idx, 0.0)
-            return
+            return None
         cells = g.bnd_owner.index_select(0, idx)
         Sf = g.bnd_Sf.index_select(0, idx)
         if phi_hbya is not None:
@@ -157,17 +163,39 @@ class FixedFluxPressureBC(BoundaryCondition):
         else:
             pred = (HbyA.data.index_select(0, cells) * Sf).sum(dim=1)
         if U is not None and patch in U.bcs:
-            target = (U.boundary_state().psi_b.index_select(0, idx) * Sf).sum(dim=1)
+            if graph:
+                from .fv import boundary_values  # noqa: PLC0415 (fv imports this module)
+
+                ub = boundary_values(U)
+            else:
+                ub = U.boundary_state().psi_b
+            target = (ub.index_select(0, idx) * Sf).sum(dim=1)
         elif phi is not None:
             target = phi.bnd_data.index_select(0, idx)
         else:
-            state.ref_g.index_fill_(0, idx, 0.0)
-            return
+            return None
         denom = coeff.data.index_select(0, cells) * torch.linalg.vector_norm(Sf, dim=1)
         pos = denom > 0.0
-        grad = torch.where(pos, (pred - target) / torch.where(pos, denom, torch.ones_like(denom)),
+        return torch.where(pos, (pred - target) / torch.where(pos, denom, torch.ones_like(denom)),
                            torch.zeros_like(denom))
-        state.ref_g.index_copy_(0, idx, grad[:, None])
+
+    def refresh(self, field, patch, idx, state):
+        grad = self._gradient(field, patch, idx, graph=False)
+        if grad is None:
+            state.ref_g.index_fill_(0, idx, 0.0)
+        else:
+            state.ref_g.index_copy_(0, idx, grad[:, None])
+
+    def graph_ref_g(self, field, patch, idx):
+        g = field.grid
+        ins = [g.get_facefield(self.phi_name), g.get_facefield(self.phi_hbya_name)]
+        cells = [g.get_cellfield(n) for n in (self.HbyA_name, self.U_name, self.rAtU_name, self.rAU_name)]
+        if not any(f is not None and f.bnd_buffer.requires_grad for f in ins) and not any(
+                c is not None and (c.data.requires_grad or any(
+                    getattr(b, "value", torch.zeros(())).requires_grad for b in c.bcs.values())) for c in cells):
+            return None
+        grad = self._gradient(field, patch, idx, graph=True)
+        return grad if grad is not None and grad.requires_grad else None
 
 
 def patch_key(patch) -> Union[DomainBoundaryPatch, str]:
@@ -214,6 +242,13 @@ class BoundaryState:
     # Dirichlet value or Neumann gradient requires grad (else None)
     ref_v_ad: Optional[torch.Tensor] = None
     ref_g_ad: Optional[torch.Tensor] = None
+
+    def frozen(self) -> "BoundaryState":
+        """A copy of the per-face values that later boundary_state() calls do
+        not rewrite (they refresh inletOutlet, fixedFluxPressure and slip
+        faces in place): what an autograd node keeps for its backward."""
+        return dataclasses.replace(self, frac=self.frac.clone(), ref_v=self.ref_v.clone(),
+                                   ref_g=self.ref_g.clone(), psi_b=self.psi_b.clone())
 
 
 class CellField:
@@ -265,6 +300,13 @@ class CellField:
             self.previous_dt = None
             self.old_data = self.data
 
+    @property
+    def has_dynamic_state(self) -> bool:
+        """Whether boundary_state() rewrites per-face values on each call."""
+        if self._state is None:
+            self.boundary_state()
+        return bool(self._dynamic) or self._state.slip is not None
+
     def boundary_state(self) -> BoundaryState:
         """Per-face value-fraction arrays, psi_b refreshed from current data."""
         g = self.grid
@@ -306,8 +348,9 @@ class CellField:
         st = self._state
         if st.psi_b is None:
             st.psi_b = torch.zeros((max(B, 1), nc), dtype=torch.float64, device=dev)
-        for patch, bc, idx in self._dynamic:
-            bc.refresh(self, patch, idx, st)
+        with torch.no_grad():  # the state arrays are device buffers; gradients do not run through these refreshes
+            for patch, bc, idx in self._dynamic:
+                bc.refresh(self, patch, idx, st)
         if st.slip is not None and nc == 3:
             nat.call("gf_bc_slip", g.mesh_ref, nat.ptr(self.data), nat.ptr(st.slip), nat.ptr(st.ref_v), nat.stream())
         st.ref_v_ad = st.ref_g_ad = None
@@ -324,6 +367,12 @@ class CellField:
             st.ref_v.copy_(rv.detach())
             st.ref_g.copy_(rg.detach())
             st.ref_v_ad, st.ref_g_ad = rv, rg
+        if torch.is_grad_enabled():  # conditions whose refreshed gradient depends on fields with a graph
+            for patch, bc, idx in self._dynamic:
+                gv = bc.graph_ref_g(self, patch, idx)
+                if gv is not None:
+                    base = st.ref_g_ad if st.ref_g_ad is not None else st.ref_g.detach().clone()
+                    st.ref_g_ad = base.index_copy(0, idx, gv.reshape(-1, *base.shape[1:]))
         if g.bnd_zero_dist.numel() and not torch.cuda.is_current_stream_capturing():
             from .immersed import check_zero_distance  # noqa: PLC0415 (immersed imports this module)
 

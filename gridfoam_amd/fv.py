@@ -173,6 +173,19 @@ class _BndValueFn(torch.autograd.Function):
                 g_rv if need[3] else None, g_rg if need[4] else None)
 
 
+def boundary_values(field: CellField) -> torch.Tensor:
+    """The field's boundary face values [B, nc] (psi_b), carrying the graph
+    of the field values and condition values when they require grad."""
+    st = field.boundary_state()
+    rv = st.ref_v_ad if st.ref_v_ad is not None else st.ref_v
+    rg = st.ref_g_ad if st.ref_g_ad is not None else st.ref_g
+    if not (torch.is_grad_enabled() and (field.data.requires_grad or rv.requires_grad or rg.requires_grad)):
+        return st.psi_b
+    if field.has_dynamic_state:
+        st = st.frozen()
+    return _BndValueFn.apply(field, st, field.data, rv, rg)
+
+
 class _InterpFn(torch.autograd.Function):
     """Linear face values of the internal faces (with the hanging-face
     gradient corrections) as an autograd node: the backward is
@@ -293,6 +306,8 @@ class fvc:
         rv = st.ref_v_ad if st.ref_v_ad is not None else st.ref_v
         rg = st.ref_g_ad if st.ref_g_ad is not None else st.ref_g
         track = torch.is_grad_enabled() and (field.data.requires_grad or rv.requires_grad or rg.requires_grad)
+        if track and field.has_dynamic_state:
+            st = st.frozen()
         if not hang_only and scheme == GradScheme.LINEAR:
             fi = fvc.interpolate(field)
             if track:  # through the linear face values (gf_gauss_grad_vjp, then the interpolation's node)
@@ -318,6 +333,8 @@ class fvc:
         rg = st.ref_g_ad if st.ref_g_ad is not None else st.ref_g
         track = torch.is_grad_enabled() and (field.data.requires_grad or rv.requires_grad or rg.requires_grad)
         grad = fvc.grad(field, hang_only=True) if g.num_hanging_faces else None
+        if track and field.has_dynamic_state:
+            st = st.frozen()
         if track:  # face values carrying the graph (reverse mode: gf_interpolate_vjp, gf_bc_state_vjp)
             F, B = g.num_internal_faces, g.num_bnd_faces
             out.single_data = _InterpFn.apply(field, field.data, grad)[:F]
@@ -573,6 +590,8 @@ class fvm:
         # boundary_state() calls, e.g. for the cell gradient, rebuild them)
         rv = st.ref_v_ad if st.ref_v_ad is not None else st.ref_v
         rg = st.ref_g_ad if st.ref_g_ad is not None else st.ref_g
+        if torch.is_grad_enabled() and field.has_dynamic_state:
+            st = st.frozen()  # the VJP reads these arrays through the launch description's pointers
         spec = _Assembly(field, st, ddt, div, lap, phi, gamma, scheme, corrected, abc, grad, harmonic)
         mat = FvMatrix.__new__(FvMatrix)
         mat.field, mat.grid, mat.face_flux_correction = field, g, None
