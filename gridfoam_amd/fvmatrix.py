@@ -282,6 +282,11 @@ class Equation:
             fv_matrix = fv_matrix.with_fixed_values(cells, values, unique=True)
         self.fv_matrix = fv_matrix
 
+    def _init_plain(self, target: CellField, fv_matrix: "FvMatrix") -> "Equation":
+        """An equation whose matrix already carries its constraints."""
+        self.name, self.target, self.fv_matrix = target.name, target, fv_matrix
+        return self
+
 
 def equation(target: CellField, fv_matrix: FvMatrix) -> Equation:
     return Equation(target, fv_matrix)

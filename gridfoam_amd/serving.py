@@ -80,12 +80,14 @@ def footprint_bytes(cfg: dict) -> int:
     return estimate_cells(cfg) * BYTES_PER_CELL
 
 
-def random_case(rng: np.random.Generator, *, n_range=(16, 96), mixed: bool = True) -> dict:
-    """A serving request: base grid n^3 with n
-    log-uniform in n_range, 0-2 refinement levels around 1-4 random boxes
-    (side U(0.05, 0.2), centre uniform in the interior), Re U(100, 1000),
-    lid-driven SIMPLE, FP32-in-FP64 pressure solves.  The boxes only trigger
-    refinement: immersed obstacles need the AP-IBM geometry builder."""
+def random_case(rng: np.random.Generator, *, n_range=(16, 96), mixed: bool = True, residual: float = 1e-6,
+                max_iterations: int = 2000, obstacles: bool = True) -> dict:
+    """A serving request: a lid-driven cube on a base grid n^3 with n
+    log-uniform in n_range, 1-4 axis-aligned box obstacles (side U(0.05,
+    0.2), centre uniform in the interior) with no-slip walls, 0-2 refinement
+    levels around them, Re U(100, 1000), steady SIMPLE until every
+    residualControl entry (U, p) is below ``residual`` or ``max_iterations``
+    outer iterations ran; FP32-in-FP64 pressure solves when ``mixed``."""
     cfg = copy.deepcopy(load_config_dict("cavity2d_simple.yaml"))
     n = int(math.floor(math.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1])))))
     n = max(n_range[0], min(n_range[1], n))
@@ -93,19 +95,31 @@ def random_case(rng: np.random.Generator, *, n_range=(16, 96), mixed: bool = Tru
     f["root_resolution"] = [n, n, n]
     f["domain"] = {"lower": [0.0, 0.0, 0.0], "upper": [1.0, 1.0, 1.0]}
     level = int(rng.integers(0, 3))
-    regions = []
+    regions, bodies = [], []
     for k in range(int(rng.integers(1, 5))):
         side = rng.uniform(0.05, 0.2)
         c = rng.uniform(side / 2, 1 - side / 2, size=3)
+        lo, hi = [float(v) for v in c - side / 2], [float(v) for v in c + side / 2]
         if level > 0:
-            regions.append({"name": f"box{k}", "min": list(c - side / 2), "max": list(c + side / 2), "level": level})
+            regions.append({"name": f"box{k}", "min": lo, "max": hi, "level": level})
+        if obstacles:
+            bodies.append({"name": f"box{k}", "kind": "box", "min": lo, "max": hi})
     f["refinement_regions"] = regions
+    f["immersed_bodies"] = bodies
     sim = cfg["simulator"]
     sim["properties"]["transport"]["nu"] = 1.0 / rng.uniform(100.0, 1000.0)
+    sim["control"]["endTime"] = float(max_iterations)
     cond = sim["conditions"]
     cond["U"]["boundary"]["wall"]["patches"] = ["x_minus", "x_plus", "y_minus", "z_minus", "z_plus"]
     cond["U"]["boundary"].pop("empty", None)
     cond["p"]["boundary"]["wall"]["patches"] = ["x_minus", "x_plus", "y_minus", "y_plus", "z_minus", "z_plus"]
+    cond["p"]["boundary"].pop("empty", None)
+    names = [b["name"] for b in bodies]
+    if names:
+        cond["U"]["boundary"]["obstacles"] = {"type": "dirichlet", "patches": names, "value": [0.0, 0.0, 0.0]}
+        cond["p"]["boundary"]["obstacles"] = {"type": "neumann", "patches": names, "value": 0.0}
+    alg = sim["fvSolution"]["algorithm"]
+    alg["residualControl"] = {"U": residual, "p": residual}
     sol = sim["fvSolution"]["solvers"]
     sol["p"]["mixed_precision"] = bool(mixed)
     sol["p"]["rel_tolerance"] = 1e-6

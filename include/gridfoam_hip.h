@@ -90,6 +90,23 @@ int gf_device_count(int *count);
  * binding check its struct layouts against the library (no GPU needed) */
 int gf_abi_sizes(int64_t out[3]);
 
+/* ---- segmented PCG for batched serving.  No counterpart in the reference
+ * (it has no serving mode): several cases concatenated into one mesh form a
+ * block-diagonal operator whose rows of case s are [seg_row[s], seg_row[s+1])
+ * (every segment at least 256 rows; seg_row on the device and on the host,
+ * row_seg [C] the segment of each row).  Each segment runs the Jacobi PCG of
+ * solvers/cg.py:75-174 with its own scalars and its own threshold
+ * max(atol, rtol |r0|) (solvers/base.py:140-162).  x [C] holds the initial
+ * guess and receives the solution; per segment: iterations, (|r0|, |r|),
+ * converged (host arrays).  workspace: device bytes from
+ * gf_seg_pcg_workspace; n_active_host: 2 n_seg pinned host words for the polls. */
+int gf_seg_pcg_workspace(int32_t n_cells, int32_t n_seg, int64_t nnz_pad, int64_t *bytes);
+int gf_seg_pcg(const gf_mesh *m, int32_t n_seg, const int32_t *seg_row, const int32_t *seg_row_host,
+               const int32_t *row_seg, const double *diag, const double *upper, const double *lower,
+               const double *b, double *x, int32_t precond, double atol, double rtol, int32_t max_iter,
+               int32_t check_every, void *workspace, int64_t workspace_bytes, int32_t *iters_out,
+               double *res_out, uint8_t *conv_out, int32_t *n_active_host, gf_stream_t stream);
+
 /* ---- axis-projected immersed boundary (replaces the AP payload fluxel's
  * builder hands to AxisProjectedGrid, src/gridfoam/core/grid/axis_projected.py:163-211;
  * contract in docs/source/contributor_guide/architecture/data_contracts.rst:181-231)
