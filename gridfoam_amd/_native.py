@@ -122,6 +122,9 @@ _SIGS: dict[str, tuple[Any, list[Any]]] = {
     "gf_pcg_dir_bench_bytes": (_i64, [_P(GfMesh), _i32]),
     "gf_pcg_dir_bench": (_i32, [_P(GfMesh), _i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gf_seg_pcg_workspace": (_i32, [_i32, _i32, _i64, _vp]),
+    "gf_seg_bicgstab_workspace": (_i32, [_i32, _i32, _i64, _vp]),
+    "gf_seg_bicgstab": (_i32, [_P(GfMesh), _i32, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _f64, _f64, _i32, _i32, _i32,
+                               _vp, _i64, _vp, _vp, _vp, _vp, _vp]),
     "gf_seg_pcg": (_i32, [_P(GfMesh), _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _f64, _f64, _i32, _i32,
                           _vp, _i64, _vp, _vp, _vp, _vp, _vp]),
     "gf_relax_momentum": (_i32, [_i32, _f64, _vp, _vp, _vp, _vp]),

@@ -8,8 +8,8 @@ boundary faces of case k follow those of case k-1; no face couples two
 cases), so assembly, gradients, fluxes and corrections run as one launch
 each for all of them, and the pressure equation is solved by the segmented
 PCG (``gf_seg_pcg``), whose per-case scalars and stopping tests make every
-case follow its own CG iteration.  The momentum BiCGSTAB runs per case on
-the case's own grid, from its slice of the combined LDU arrays.  Per-case
+case follow its own CG iteration; the momentum equations likewise go
+through the segmented BiCGSTAB (``gf_seg_bicgstab``).  Per-case
 quantities -- viscosity (Reynolds number), pressure reference cell,
 adjustPhi balance, initial residuals and residualControl -- are kept per
 segment, so a case's fields do not depend on which other cases share its
@@ -30,13 +30,12 @@ import torch
 from . import _native as nat
 from .algorithms import (compute_phi_hbya, correct_flux, correct_phi, correct_velocity, normalize_residual_control,
                          residual_satisfied)
-from .config import GridfoamConfig, PreconditionerType, SIMPLEAlgorithm
+from .config import GridfoamConfig, NormType, PreconditionerType, SIMPLEAlgorithm, SolverType
 from .fields import CellField, DirichletBC, FaceField
 from .fv import fvc, fvm
 from .fvmatrix import Equation, FvMatrix
 from .grid import OctreeGrid
 from .immersed import N_DOMAIN_PATCHES, apply_immersed_dirichlet_values
-from .solvers import create_solver
 
 SEG_MIN_ROWS = 256  # gf_seg_pcg: every case needs at least this many cells
 
@@ -229,6 +228,47 @@ class SegmentedCG:
                            [bool(c) for c in conv])
 
 
+class SegmentedBiCGSTAB:
+    """Jacobi right-preconditioned BiCGSTAB of a [C, 3] system over the
+    segments of a block-diagonal operator (``gf_seg_bicgstab``): every
+    (segment, component) has its own scalars, threshold, restarts and stop."""
+
+    def __init__(self, atol: float, rtol: float, max_iter: int, max_restart: int, precond: bool = True,
+                 check_every: int = 8):
+        self.atol, self.rtol, self.max_iter = float(atol), float(rtol), int(max_iter)
+        self.max_restart, self.precond, self.check_every = int(max_restart), bool(precond), int(check_every)
+        self._ws: Optional[torch.Tensor] = None
+        self._flag = torch.zeros(2, dtype=torch.int32).pin_memory()
+
+    def solve(self, A: FvMatrix, b: torch.Tensor, x0: torch.Tensor,
+              seg_row_host: List[int]) -> Tuple[torch.Tensor, SegStats]:
+        g = A.grid
+        n_seg = len(seg_row_host) - 1
+        if tuple(b.shape) != (g.num_cells, 3) or tuple(x0.shape) != (g.num_cells, 3):
+            raise ValueError("the segmented BiCGSTAB solves [C, 3] systems")
+        nbytes = ctypes.c_int64(0)
+        nat.call("gf_seg_bicgstab_workspace", ctypes.c_int32(g.num_cells), ctypes.c_int32(n_seg),
+                 ctypes.c_int64(int(g.mesh.nnz_pad)), ctypes.byref(nbytes))
+        if self._ws is None or self._ws.numel() < nbytes.value:
+            self._ws = torch.zeros(nbytes.value, dtype=torch.uint8, device=g.device)
+        if self._flag.numel() < 2 * n_seg:
+            self._flag = torch.zeros(2 * n_seg, dtype=torch.int32).pin_memory()
+        x = x0.detach().clone().contiguous()
+        host_rows = (ctypes.c_int32 * (n_seg + 1))(*seg_row_host)
+        n = 3 * n_seg
+        iters = (ctypes.c_int32 * n)()
+        res = (ctypes.c_double * (2 * n))()
+        conv = (ctypes.c_uint8 * n)()
+        nat.call("gf_seg_bicgstab", g.mesh_ref, ctypes.c_int32(n_seg), host_rows, nat.ptr(A._diag),
+                 nat.ptr(A._upper), nat.ptr(A._lower), nat.ptr(b.detach().contiguous()), nat.ptr(x),
+                 ctypes.c_int32(int(self.precond)), ctypes.c_double(self.atol), ctypes.c_double(self.rtol),
+                 ctypes.c_int32(self.max_iter), ctypes.c_int32(self.max_restart), ctypes.c_int32(self.check_every),
+                 nat.ptr(self._ws), ctypes.c_int64(self._ws.numel()), iters, res, conv,
+                 ctypes.c_void_p(self._flag.data_ptr()), nat.stream())
+        return x, SegStats(list(iters), [res[2 * k] for k in range(n)], [res[2 * k + 1] for k in range(n)],
+                           [bool(c) for c in conv])
+
+
 # ---------------------------------------------------------- batched SIMPLE
 @dataclass
 class CaseState:
@@ -348,11 +388,11 @@ class Batch:
         self.ref_cells = [(sl.cells[0] + c.ref[0], c.ref[1]) for c, sl in zip(self.cases, self.slices)
                           if c.ref is not None]
         sol = sim.fvSolution.solvers
-        self.solver_U = create_solver(sol["U"])
-        # each case's momentum is solved on its own grid from its slice of the
-        # combined LDU (cells and faces of a case are contiguous and in its
-        # own order), so the BiCGSTAB scalars stay per case
-        self._U_case = [CellField(c.grid, "U_batch", (3,)) for c in self.cases]
+        us = sol["U"]
+        if us.method != SolverType.BiCGSTAB or us.norm_type != NormType.L_2:
+            raise ValueError("the batched server solves momentum with the 2-norm BiCGSTAB")
+        self.solver_U = SegmentedBiCGSTAB(us.tolerance, us.rel_tolerance, us.max_iter, us.max_restart,
+                                          precond=us.preconditioner == PreconditionerType.JACOBI)
         ps = sol["p"]
         self.solver_p = SegmentedCG(ps.tolerance, ps.rel_tolerance, ps.max_iter,
                                     precond=ps.preconditioner == PreconditionerType.JACOBI)
@@ -461,24 +501,9 @@ class Batch:
         return converged
 
     def _solve_momentum(self, A: FvMatrix) -> torch.Tensor:
-        """BiCGSTAB of every case's momentum rows on its own grid."""
-        out = torch.empty_like(self.U.data)
-        for k, (c, sl) in enumerate(zip(self.cases, self.slices)):
-            a, b = sl.cells
-            f0, f1 = sl.faces
-            fk = self._U_case[k]
-            fk.data = self.U.data[a:b].contiguous()
-            m = FvMatrix.__new__(FvMatrix)
-            m.field, m.grid = fk, c.grid
-            m._diag = A._diag[a:b].contiguous()
-            pad = max(f1 - f0, 1)
-            m._upper = A._upper[f0:f0 + pad].contiguous() if f1 > f0 else torch.zeros(1, dtype=torch.float64,
-                                                                                        device=out.device)
-            m._lower = A._lower[f0:f0 + pad].contiguous() if f1 > f0 else torch.zeros_like(m._upper)
-            m._source = A._source[a:b].contiguous()
-            m.face_flux_correction = None
-            out[a:b] = self.solver_U.solve(Equation.__new__(Equation)._init_plain(fk, m)).solution
-        return out
+        """The segmented BiCGSTAB of every case's momentum rows."""
+        x, _ = self.solver_U.solve(A, A.source, self.U.data, self.seg_row_host)
+        return x
 
     def _case_converged(self, c: CaseState) -> bool:
         """simple.py:241-271: every residualControl entry below its absolute tolerance."""

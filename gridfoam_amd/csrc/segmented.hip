@@ -69,10 +69,12 @@ struct SegOps {
 };
 
 // Workgroup sum of NQ values, returned to every thread: wave sums by
-// shuffles, then the four wave sums in wave order (a fixed order).
+// shuffles, then the wave sums added in wave order (a fixed order) by the
+// first NQ threads and broadcast through LDS.
 template <int NQ>
 __device__ __forceinline__ void wg_sum(double (&v)[NQ]) {
     __shared__ double sh[NQ][TS / kWave];
+    __shared__ double res[NQ];
     const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
@@ -80,14 +82,14 @@ __device__ __forceinline__ void wg_sum(double (&v)[NQ]) {
         if (lane == 0) sh[q][wid] = x;
     }
     __syncthreads();
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        double a = sh[q][0];
-#pragma unroll
-        for (int k = 1; k < TS / kWave; ++k) a += sh[q][k];
-        v[q] = a;
+    if (threadIdx.x < NQ) {
+        double a = sh[threadIdx.x][0];
+        for (int k = 1; k < TS / kWave; ++k) a += sh[threadIdx.x][k];
+        res[threadIdx.x] = a;
     }
     __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) v[q] = res[q];
 }
 
 // The partials of a segment's W <= WMAX workgroups (one per thread),
@@ -508,3 +510,614 @@ extern "C" int gf_seg_pcg(const gf_mesh *m, int32_t n_seg, const int32_t *seg_ro
     return rc;
 }
 
+
+// ---------------------------------------------------------------------------
+// Segmented right-preconditioned BiCGSTAB for the three velocity components
+// (solvers/bicgstab.py:76-272 with solvers/preconditioners.py:76-96), one
+// scalar set per (segment, component).  Vectors are component-major [3][C].
+// One iteration is three launches, each reducing the previous launch's
+// partials of its segment in every workgroup:
+//  * V: the tests of the last update (|r| < thresh, rho breakdown), beta,
+//    pending restarts (r_hat = r, p = r, rho = r.r); p_new = r + beta (p -
+//    omega v) built on the fly for the rows and their gathered columns,
+//    y = M^-1 p_new, v = A y; partials r_hat.v, |r_hat|^2, |v|^2;
+//  * T: the iteration count, the alpha breakdown test, alpha; s = r - alpha
+//    v and z = M^-1 s on the fly, t = A z; partials |s|^2, t.t, t.s.  t is
+//    computed before |s| is known: when s meets the threshold it is unused;
+//  * U: |s| < thresh ? x += alpha y : the omega breakdown tests, x += alpha y
+//    + omega z, r = s - omega t; partials |r|^2, r_hat.r.
+// p and v are double buffered (their old values are gathered while the new
+// ones are written); state and partials alternate between two buffers by
+// launch parity.  Norms are 2-norms.
+namespace gf {
+namespace {
+
+constexpr int NC = 3;       // components
+constexpr int BQ = 3 * NC;  // partial quantities per launch at most
+enum BiPhase : int { BI_ACTIVE = 0, BI_RESTART = 1, BI_DONE = 3 };
+
+struct BiState {
+    double thresh, norm0, final_res, rho_old, alpha, omega, beta, rhat2, rr;
+    int iter, phase, converged, restarts;
+};
+
+struct BiOps {
+    int C, G;
+    const int32_t *sell_ptr, *sell_col;
+    const double *val, *dg, *inv_d;
+    const int4 *wg;
+    const int32_t *wg_end;
+    double *x, *r, *rh, *y, *s, *z, *t;
+    double *part;  // [2][BQ][G]
+    BiState *st;   // [2][n_seg][NC]
+    int32_t *active;
+    int n_seg;
+    double atol, rtol;
+    int max_iter, max_restart;
+};
+
+__device__ __forceinline__ void bi_break(BiState &S, const BiOps &o) {
+    // bicgstab.py:238-246: restart while the budget lasts, else stop
+    if (S.restarts >= o.max_restart || S.iter >= o.max_iter) {
+        S.phase = BI_DONE;
+    } else {
+        S.restarts += 1;
+        S.phase = BI_RESTART;
+    }
+}
+
+__device__ __forceinline__ const double *bi_part_in(const BiOps &o, int q) {
+    return o.part + (size_t)((q - 1) & 1) * BQ * o.G;
+}
+__device__ __forceinline__ double *bi_part_out(const BiOps &o, int q) {
+    return o.part + (size_t)(q & 1) * BQ * o.G;
+}
+
+// The segment's three states: read by thread 0 into LDS, written back by
+// the segment's first workgroup.
+__device__ __forceinline__ void bi_state_load(const BiOps &o, int q, int s, BiState *sh) {
+    if (threadIdx.x < NC) sh[threadIdx.x] = o.st[((size_t)((q - 1) & 1) * o.n_seg + s) * NC + threadIdx.x];
+}
+__device__ __forceinline__ void bi_state_store(const BiOps &o, int q, int s, int t, const BiState *sh) {
+    if (t == 0 && threadIdx.x < NC) o.st[((size_t)(q & 1) * o.n_seg + s) * NC + threadIdx.x] = sh[threadIdx.x];
+}
+
+template <int NQ>
+__device__ __forceinline__ void bi_partials_out(const BiOps &o, int q, int w, const double (&v)[NQ]) {
+    if (threadIdx.x == 0) {
+        double *P = bi_part_out(o, q);
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) P[(size_t)k * o.G + w] = v[k];
+    }
+}
+
+// r = b - A x ; partials |r|^2 per component
+__global__ __launch_bounds__(TS) void k_sbi_init(BiOps o, int q, const double *__restrict__ b) {
+    const int w = blockIdx.x;
+    const int4 d = o.wg[w];
+    const int t = w - d.y, W = d.z, r1 = o.wg_end[w];
+    double v[NC] = {};
+    for (int i = d.w + t * TS + (int)threadIdx.x; i < r1; i += W * TS) {
+        const int sl = i >> 6, lane = i & 63;
+        const int base = o.sell_ptr[sl];
+        const int Wl = (o.sell_ptr[sl + 1] - base) >> 6;
+        double ax[NC];
+        const double dgi = o.dg[i];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) ax[c] = dgi * o.x[(size_t)c * o.C + i];
+        for (int k = 0; k < Wl; ++k) {
+            const int e = base + k * 64 + lane;
+            const int col = o.sell_col[e];
+            const double a = o.val[e];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) ax[c] += a * o.x[(size_t)c * o.C + col];
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const double rr = b[(size_t)c * o.C + i] - ax[c];
+            o.r[(size_t)c * o.C + i] = rr;
+            v[c] += rr * rr;
+        }
+    }
+    wg_sum<NC>(v);
+    bi_partials_out<NC>(o, q, w, v);
+}
+
+// V (see above).  first: the launch after k_sbi_init (decides on |r0|).
+// po, vo: the current p and v; pn, vn: the buffers for the new ones
+__global__ __launch_bounds__(TS) void k_sbi_v(BiOps o, int q, int first, const double *__restrict__ po,
+                                              const double *__restrict__ vo, double *__restrict__ pn,
+                                              double *__restrict__ vn) {
+    __shared__ BiState sh[NC];
+    __shared__ int mode_s[NC];  // 0 idle, 1 continue (beta update), 2 restart (p = r, r_hat = r)
+    const int w = blockIdx.x;
+    const int4 d = o.wg[w];
+    const int s = d.x, w0 = d.y, t = w - w0, W = d.z, r1 = o.wg_end[w];
+    double tot[2 * NC];  // first: |r|^2 per component; else |r|^2, r_hat.r per component
+    load_partials<2 * NC>(bi_part_in(o, q), o.G, w0, W, tot);
+    bi_state_load(o, q, s, sh);
+    __syncthreads();
+    wg_sum<2 * NC>(tot);
+    if (threadIdx.x == 0) {
+        for (int c = 0; c < NC; ++c) {
+            BiState &S = sh[c];
+            int mode = 0;
+            if (first) {
+                const double rr = tot[c];
+                const double n0 = sqrt(rr);
+                S.norm0 = n0;
+                S.final_res = n0;
+                S.thresh = fmax(o.atol, o.rtol * n0);
+                S.iter = 0;
+                S.restarts = 0;
+                S.alpha = S.omega = S.beta = S.rhat2 = 0.0;
+                S.converged = n0 < S.thresh;
+                S.rr = rr;
+                S.rho_old = rr;
+                S.phase = S.converged ? BI_DONE : BI_ACTIVE;
+                mode = S.converged ? 0 : 2;
+            } else if (S.phase == BI_ACTIVE) {  // r was updated by the last U launch
+                const double rr = tot[2 * c], rhr = tot[2 * c + 1];
+                const double nr = sqrt(rr);
+                S.final_res = nr;
+                S.rr = rr;
+                if (nr < S.thresh) {
+                    S.converged = 1;
+                    S.phase = BI_DONE;
+                } else if (fabs(rhr) <= 1e-6 * sqrt(S.rhat2) * nr) {
+                    bi_break(S, o);
+                    if (S.phase == BI_RESTART) {  // restart at once
+                        S.rho_old = rr;
+                        S.phase = BI_ACTIVE;
+                        mode = 2;
+                    }
+                } else {
+                    S.beta = (rhr / S.rho_old) * (S.alpha / S.omega);
+                    S.rho_old = rhr;
+                    if (S.iter >= o.max_iter)
+                        S.phase = BI_DONE;
+                    else
+                        mode = 1;
+                }
+            } else if (S.phase == BI_RESTART) {  // a breakdown in T or U: r is unchanged
+                S.rho_old = S.rr;
+                S.phase = BI_ACTIVE;
+                mode = 2;
+            }
+            mode_s[c] = mode;
+        }
+        if (t == 0) {
+            int any = 0;
+            for (int c = 0; c < NC; ++c) any |= sh[c].phase != BI_DONE;
+            o.active[s] = any;
+        }
+    }
+    __syncthreads();
+    bi_state_store(o, q, s, t, sh);
+    int mode[NC];
+    double be[NC], om[NC];
+    bool work = false;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        mode[c] = mode_s[c];
+        be[c] = sh[c].beta;
+        om[c] = sh[c].omega;
+        work |= mode[c] != 0;
+    }
+    if (!work) return;
+    double v[BQ] = {};
+    for (int i = d.w + t * TS + (int)threadIdx.x; i < r1; i += W * TS) {
+        const int sl = i >> 6, lane = i & 63;
+        const int base = o.sell_ptr[sl];
+        const int Wl = (o.sell_ptr[sl + 1] - base) >> 6;
+        const double idi = o.inv_d[i], dgi = o.dg[i];
+        double acc[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const size_t k = (size_t)c * o.C + i;
+            double pi = 0.0;
+            if (mode[c] == 1) pi = o.r[k] + be[c] * (po[k] - om[c] * vo[k]);
+            else if (mode[c] == 2) pi = o.r[k];
+            if (mode[c]) {
+                pn[k] = pi;
+                o.y[k] = idi * pi;
+            }
+            acc[c] = dgi * (idi * pi);
+        }
+        for (int kk = 0; kk < Wl; ++kk) {
+            const int e = base + kk * 64 + lane;
+            const int col = o.sell_col[e];
+            const double a = o.val[e] * o.inv_d[col];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                const size_t k = (size_t)c * o.C + col;
+                if (mode[c] == 1) acc[c] += a * (o.r[k] + be[c] * (po[k] - om[c] * vo[k]));
+                else if (mode[c] == 2) acc[c] += a * o.r[k];
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            if (!mode[c]) continue;
+            const size_t k = (size_t)c * o.C + i;
+            double h;
+            if (mode[c] == 2) {
+                h = o.r[k];
+                o.rh[k] = h;
+            } else {
+                h = o.rh[k];
+            }
+            vn[k] = acc[c];
+            v[3 * c] += h * acc[c];
+            v[3 * c + 1] += h * h;
+            v[3 * c + 2] += acc[c] * acc[c];
+        }
+    }
+    wg_sum<BQ>(v);
+    bi_partials_out<BQ>(o, q, w, v);
+}
+
+// T (see above); vv: the v written by the last V launch
+__global__ __launch_bounds__(TS) void k_sbi_t(BiOps o, int q, const double *__restrict__ vv) {
+    __shared__ BiState sh[NC];
+    const int w = blockIdx.x;
+    const int4 d = o.wg[w];
+    const int s = d.x, w0 = d.y, t = w - w0, W = d.z, r1 = o.wg_end[w];
+    double tot[BQ];
+    load_partials<BQ>(bi_part_in(o, q), o.G, w0, W, tot);
+    bi_state_load(o, q, s, sh);
+    __syncthreads();
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) any |= sh[c].phase == BI_ACTIVE;
+    if (!any) {
+        bi_state_store(o, q, s, t, sh);
+        return;
+    }
+    wg_sum<BQ>(tot);
+    if (threadIdx.x == 0)
+        for (int c = 0; c < NC; ++c) {
+            BiState &S = sh[c];
+            if (S.phase != BI_ACTIVE) continue;
+            S.iter += 1;
+            S.rhat2 = tot[3 * c + 1];
+            const double den = tot[3 * c];
+            if (fabs(den) <= 1e-6 * sqrt(tot[3 * c + 1]) * sqrt(tot[3 * c + 2]))
+                bi_break(S, o);
+            else
+                S.alpha = S.rho_old / den;
+        }
+    __syncthreads();
+    bi_state_store(o, q, s, t, sh);
+    bool act[NC];
+    double al[NC];
+    bool work = false;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        act[c] = sh[c].phase == BI_ACTIVE;
+        al[c] = sh[c].alpha;
+        work |= act[c];
+    }
+    if (!work) return;
+    double v[BQ] = {};
+    for (int i = d.w + t * TS + (int)threadIdx.x; i < r1; i += W * TS) {
+        const int sl = i >> 6, lane = i & 63;
+        const int base = o.sell_ptr[sl];
+        const int Wl = (o.sell_ptr[sl + 1] - base) >> 6;
+        const double idi = o.inv_d[i], dgi = o.dg[i];
+        double acc[NC], si[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const size_t k = (size_t)c * o.C + i;
+            si[c] = act[c] ? o.r[k] - al[c] * vv[k] : 0.0;
+            acc[c] = dgi * (idi * si[c]);
+        }
+        for (int kk = 0; kk < Wl; ++kk) {
+            const int e = base + kk * 64 + lane;
+            const int col = o.sell_col[e];
+            const double a = o.val[e] * o.inv_d[col];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                if (!act[c]) continue;
+                const size_t k = (size_t)c * o.C + col;
+                acc[c] += a * (o.r[k] - al[c] * vv[k]);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            if (!act[c]) continue;
+            const size_t k = (size_t)c * o.C + i;
+            o.s[k] = si[c];
+            o.z[k] = idi * si[c];
+            o.t[k] = acc[c];
+            v[3 * c] += si[c] * si[c];
+            v[3 * c + 1] += acc[c] * acc[c];
+            v[3 * c + 2] += acc[c] * si[c];
+        }
+    }
+    wg_sum<BQ>(v);
+    bi_partials_out<BQ>(o, q, w, v);
+}
+
+// U (see above)
+__global__ __launch_bounds__(TS) void k_sbi_u(BiOps o, int q) {
+    __shared__ BiState sh[NC];
+    __shared__ int mode_s[NC];  // 0 idle, 1 full update, 2 converged on s (x += alpha y)
+    const int w = blockIdx.x;
+    const int4 d = o.wg[w];
+    const int s = d.x, w0 = d.y, t = w - w0, W = d.z, r1 = o.wg_end[w];
+    double tot[BQ];
+    load_partials<BQ>(bi_part_in(o, q), o.G, w0, W, tot);
+    bi_state_load(o, q, s, sh);
+    __syncthreads();
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) any |= sh[c].phase == BI_ACTIVE;
+    if (!any) {
+        bi_state_store(o, q, s, t, sh);
+        return;
+    }
+    wg_sum<BQ>(tot);
+    if (threadIdx.x == 0)
+        for (int c = 0; c < NC; ++c) {
+            BiState &S = sh[c];
+            int mode = 0;
+            if (S.phase == BI_ACTIVE) {
+                const double ns = sqrt(tot[3 * c]);
+                S.final_res = ns;
+                if (ns < S.thresh) {
+                    S.converged = 1;
+                    S.phase = BI_DONE;
+                    mode = 2;
+                } else {
+                    const double tt = tot[3 * c + 1];
+                    bool brk = fabs(tt) <= 1e-6 * sqrt(tt) * sqrt(tt);
+                    double om = 0.0;
+                    if (!brk) {
+                        om = tot[3 * c + 2] / tt;
+                        brk = fabs(om) <= 1e-6;
+                    }
+                    if (brk) {
+                        bi_break(S, o);
+                    } else {
+                        S.omega = om;
+                        mode = 1;
+                    }
+                }
+            }
+            mode_s[c] = mode;
+        }
+    __syncthreads();
+    bi_state_store(o, q, s, t, sh);
+    int mode[NC];
+    double al[NC], om[NC];
+    bool work = false;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        mode[c] = mode_s[c];
+        al[c] = sh[c].alpha;
+        om[c] = sh[c].omega;
+        work |= mode[c] != 0;
+    }
+    if (!work) return;
+    double v[2 * NC] = {};
+    for (int i = d.w + t * TS + (int)threadIdx.x; i < r1; i += W * TS) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const size_t k = (size_t)c * o.C + i;
+            if (mode[c] == 2) {
+                o.x[k] = o.x[k] + al[c] * o.y[k];
+            } else if (mode[c] == 1) {
+                o.x[k] = o.x[k] + al[c] * o.y[k] + om[c] * o.z[k];
+                const double rr = o.s[k] - om[c] * o.t[k];
+                o.r[k] = rr;
+                v[2 * c] += rr * rr;
+                v[2 * c + 1] += o.rh[k] * rr;
+            }
+        }
+    }
+    wg_sum<2 * NC>(v);
+    bi_partials_out<2 * NC>(o, q, w, v);
+}
+
+// [C][3] <-> [3][C]
+__global__ void k_sbi_transpose(int C, const double *__restrict__ in, double *__restrict__ out, int to_cm) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= C) return;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        if (to_cm)
+            out[(size_t)c * C + i] = in[(size_t)i * NC + c];
+        else
+            out[(size_t)i * NC + c] = in[(size_t)c * C + i];
+    }
+}
+
+__global__ void k_sbi_stats(int n, const BiState *__restrict__ st, int32_t *__restrict__ iters,
+                            double *__restrict__ res, uint8_t *__restrict__ conv) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    iters[k] = st[k].iter;
+    res[2 * k] = st[k].norm0;
+    res[2 * k + 1] = st[k].final_res;
+    conv[k] = (uint8_t)st[k].converged;
+}
+
+struct BiCarve {
+    size_t val, dg, inv_d, b, x, r, rh, y, s, z, t, p0, p1, v0, v1, part, st, act, wg, wg_end, iters, res, conv, total;
+};
+BiCarve bi_carve(int C, int n_seg, int64_t nnz, int G) {
+    BiCarve c{};
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = o;
+        o += al(bytes);
+        return at;
+    };
+    const size_t vec = sizeof(double) * NC * (size_t)C;
+    c.val = take(sizeof(double) * (size_t)nnz);
+    c.dg = take(sizeof(double) * C);
+    c.inv_d = take(sizeof(double) * C);
+    c.b = take(vec);
+    c.x = take(vec);
+    c.r = take(vec);
+    c.rh = take(vec);
+    c.y = take(vec);
+    c.s = take(vec);
+    c.z = take(vec);
+    c.t = take(vec);
+    c.p0 = take(vec);
+    c.p1 = take(vec);
+    c.v0 = take(vec);
+    c.v1 = take(vec);
+    c.part = take(sizeof(double) * 2 * BQ * (size_t)G);
+    c.st = take(sizeof(BiState) * 2 * NC * (size_t)n_seg);
+    c.act = take(sizeof(int32_t) * n_seg);
+    c.wg = take(sizeof(int4) * G);
+    c.wg_end = take(sizeof(int32_t) * G);
+    c.iters = take(sizeof(int32_t) * NC * n_seg);
+    c.res = take(sizeof(double) * 2 * NC * n_seg);
+    c.conv = take(NC * (size_t)n_seg);
+    c.total = o;
+    return c;
+}
+
+}  // namespace
+}  // namespace gf
+
+extern "C" int gf_seg_bicgstab_workspace(int32_t n_cells, int32_t n_seg, int64_t nnz_pad, int64_t *bytes) {
+    GF_REQUIRE(bytes && n_cells >= 0 && n_seg >= 1 && nnz_pad >= 0, "bad arguments");
+    *bytes = (int64_t)bi_carve(n_cells, n_seg, nnz_pad, max_workgroups(n_cells, n_seg)).total;
+    return GF_OK;
+}
+
+extern "C" int gf_seg_bicgstab(const gf_mesh *m, int32_t n_seg, const int32_t *seg_row_host, const double *diag,
+                               const double *upper, const double *lower, const double *b, double *x,
+                               int32_t precond, double atol, double rtol, int32_t max_iter, int32_t max_restart,
+                               int32_t check_every, void *workspace, int64_t workspace_bytes, int32_t *iters_out,
+                               double *res_out, uint8_t *conv_out, int32_t *n_active_host, gf_stream_t stream) {
+    GF_REQUIRE(m && seg_row_host && diag && upper && lower && b && x && workspace && iters_out && res_out &&
+                   conv_out && n_active_host,
+               "null argument");
+    GF_REQUIRE(n_seg >= 1 && max_iter >= 0 && max_restart >= 0 && check_every >= 1, "bad solver parameters");
+    const int C = m->n_cells;
+    GF_REQUIRE(seg_row_host[0] == 0 && seg_row_host[n_seg] == C, "segments must cover the rows");
+    std::vector<int4> wg_tab;
+    std::vector<int32_t> wg_end;
+    for (int s = 0; s < n_seg; ++s) {
+        const int rows = seg_row_host[s + 1] - seg_row_host[s];
+        GF_REQUIRE(rows >= 1, "every segment needs at least one row");
+        const int w0 = (int)wg_tab.size(), W = tiles_of(rows);
+        for (int k = 0; k < W; ++k) {
+            wg_tab.push_back(make_int4(s, w0, W, seg_row_host[s]));
+            wg_end.push_back(seg_row_host[s + 1]);
+        }
+    }
+    const int G = (int)wg_tab.size();
+    GF_REQUIRE(G <= max_workgroups(C, n_seg), "workgroup table overflow");
+    const BiCarve cv = bi_carve(C, n_seg, m->nnz_pad, max_workgroups(C, n_seg));
+    GF_REQUIRE(workspace_bytes >= (int64_t)cv.total, "workspace too small (gf_seg_bicgstab_workspace)");
+    char *w = static_cast<char *>(workspace);
+    hipStream_t st = S(stream);
+    BiOps o{};
+    o.C = C;
+    o.G = G;
+    o.sell_ptr = m->sell_ptr;
+    o.sell_col = m->sell_col;
+    o.val = (double *)(w + cv.val);
+    o.dg = (double *)(w + cv.dg);
+    o.inv_d = (double *)(w + cv.inv_d);
+    o.wg = (const int4 *)(w + cv.wg);
+    o.wg_end = (const int32_t *)(w + cv.wg_end);
+    o.x = (double *)(w + cv.x);
+    o.r = (double *)(w + cv.r);
+    o.rh = (double *)(w + cv.rh);
+    o.y = (double *)(w + cv.y);
+    o.s = (double *)(w + cv.s);
+    o.z = (double *)(w + cv.z);
+    o.t = (double *)(w + cv.t);
+    double *pbuf[2] = {(double *)(w + cv.p0), (double *)(w + cv.p1)};
+    double *vbuf[2] = {(double *)(w + cv.v0), (double *)(w + cv.v1)};
+    o.part = (double *)(w + cv.part);
+    o.st = (BiState *)(w + cv.st);
+    o.active = (int32_t *)(w + cv.act);
+    o.n_seg = n_seg;
+    o.atol = atol;
+    o.rtol = rtol;
+    o.max_iter = max_iter;
+    o.max_restart = max_restart;
+    double *wb = (double *)(w + cv.b);
+    GF_CHECK_HIP(hipMemcpyAsync((void *)o.wg, wg_tab.data(), sizeof(int4) * G, hipMemcpyHostToDevice, st));
+    GF_CHECK_HIP(hipMemcpyAsync((void *)o.wg_end, wg_end.data(), sizeof(int32_t) * G, hipMemcpyHostToDevice, st));
+    if (m->nnz_pad > 0)
+        hipLaunchKernelGGL(k_seg_pack, dim3(grid_for(m->nnz_pad)), dim3(kBlock), 0, st, (int64_t)m->nnz_pad,
+                           m->sell_code, upper, lower, (double *)o.val);
+    hipLaunchKernelGGL(k_seg_diag, dim3(grid_for(C)), dim3(kBlock), 0, st, C, diag, precond, (double *)o.dg,
+                       (double *)o.inv_d);
+    hipLaunchKernelGGL(k_sbi_transpose, dim3(grid_for(C)), dim3(kBlock), 0, st, C, x, o.x, 1);
+    hipLaunchKernelGGL(k_sbi_transpose, dim3(grid_for(C)), dim3(kBlock), 0, st, C, b, wb, 1);
+    int q = 0;
+    hipLaunchKernelGGL(k_sbi_init, dim3(G), dim3(TS), 0, st, o, q++, (const double *)wb);
+    GF_CHECK_LAUNCH();
+    GF_CHECK_HIP(hipStreamSynchronize(st));  // the host tables above must outlive their copies
+    int *host_flag = n_active_host;  // pinned host words: [2][n_seg] active flags
+    for (int k = 0; k < 2 * n_seg; ++k) host_flag[k] = 1;
+    hipEvent_t ev[2];
+    GF_CHECK_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    GF_CHECK_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    int it = 0, pb = 0, chunk = 0, pending = -1, rc = GF_OK;
+    while (it < max_iter) {
+        const int k = std::min(check_every, max_iter - it);
+        for (int j = 0; j < k; ++j) {
+            hipLaunchKernelGGL(k_sbi_v, dim3(G), dim3(TS), 0, st, o, q++, it + j == 0 ? 1 : 0,
+                               (const double *)pbuf[pb], (const double *)vbuf[pb], pbuf[pb ^ 1], vbuf[pb ^ 1]);
+            pb ^= 1;  // the new p, v
+            hipLaunchKernelGGL(k_sbi_t, dim3(G), dim3(TS), 0, st, o, q++, (const double *)vbuf[pb]);
+            hipLaunchKernelGGL(k_sbi_u, dim3(G), dim3(TS), 0, st, o, q++);
+        }
+        it += k;
+        const int slot = chunk++ & 1;
+        if (hipMemcpyAsync(host_flag + slot * n_seg, o.active, sizeof(int32_t) * n_seg, hipMemcpyDeviceToHost, st) !=
+                hipSuccess ||
+            hipEventRecord(ev[slot], st) != hipSuccess) {
+            set_error("gf_seg_bicgstab: could not post the active flags");
+            rc = GF_EINVAL;
+            break;
+        }
+        if (pending >= 0) {
+            if (hipEventSynchronize(ev[pending]) != hipSuccess) {
+                set_error("gf_seg_bicgstab: event wait failed");
+                rc = GF_EINVAL;
+                break;
+            }
+            int any = 0;
+            for (int s = 0; s < n_seg; ++s) any |= host_flag[pending * n_seg + s];
+            if (!any) break;
+        }
+        pending = slot;
+    }
+    if (rc == GF_OK) {
+        // the tests of the last update (every component that is still active stops here)
+        hipLaunchKernelGGL(k_sbi_v, dim3(G), dim3(TS), 0, st, o, q++, max_iter == 0 ? 1 : 0,
+                           (const double *)pbuf[pb], (const double *)vbuf[pb], pbuf[pb ^ 1], vbuf[pb ^ 1]);
+        const BiState *last = o.st + (size_t)((q - 1) & 1) * n_seg * NC;
+        int32_t *d_it = (int32_t *)(w + cv.iters);
+        double *d_res = (double *)(w + cv.res);
+        uint8_t *d_conv = (uint8_t *)(w + cv.conv);
+        hipLaunchKernelGGL(k_sbi_stats, dim3(grid_for(NC * n_seg)), dim3(kBlock), 0, st, NC * n_seg, last, d_it,
+                           d_res, d_conv);
+        hipLaunchKernelGGL(k_sbi_transpose, dim3(grid_for(C)), dim3(kBlock), 0, st, C, (const double *)o.x, x, 0);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(iters_out, d_it, sizeof(int32_t) * NC * n_seg, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipMemcpyAsync(res_out, d_res, sizeof(double) * 2 * NC * n_seg, hipMemcpyDeviceToHost, st) !=
+                hipSuccess ||
+            hipMemcpyAsync(conv_out, d_conv, NC * n_seg, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess) {
+            set_error("gf_seg_bicgstab: could not read the statistics");
+            rc = GF_EINVAL;
+        }
+    }
+    (void)hipStreamSynchronize(st);
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    return rc;
+}

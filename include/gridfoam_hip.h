@@ -93,7 +93,7 @@ int gf_abi_sizes(int64_t out[3]);
 /* ---- segmented PCG for batched serving.  No counterpart in the reference
  * (it has no serving mode): several cases concatenated into one mesh form a
  * block-diagonal operator whose rows of case s are [seg_row[s], seg_row[s+1])
- * (every segment at least 256 rows; seg_row on the device and on the host,
+ * (every segment at least one row; seg_row on the device and on the host,
  * row_seg [C] the segment of each row).  Each segment runs the Jacobi PCG of
  * solvers/cg.py:75-174 with its own scalars and its own threshold
  * max(atol, rtol |r0|) (solvers/base.py:140-162).  x [C] holds the initial
@@ -106,6 +106,20 @@ int gf_seg_pcg(const gf_mesh *m, int32_t n_seg, const int32_t *seg_row, const in
                const double *b, double *x, int32_t precond, double atol, double rtol, int32_t max_iter,
                int32_t check_every, void *workspace, int64_t workspace_bytes, int32_t *iters_out,
                double *res_out, uint8_t *conv_out, int32_t *n_active_host, gf_stream_t stream);
+/* ---- segmented BiCGSTAB for the momentum systems of a batch.  No
+ * counterpart in the reference (serving only): the three components of
+ * every segment run the right-preconditioned BiCGSTAB of
+ * solvers/bicgstab.py:76-272 (restarts on breakdown, 2-norm) with their
+ * own scalars and thresholds.  b, x: [C][3] row-major (x: initial guess in,
+ * solution out); per (segment, component) s*3+c: iterations, (|r0|, final
+ * residual), converged.  workspace from gf_seg_bicgstab_workspace;
+ * n_active_host: 2 n_seg pinned host words. */
+int gf_seg_bicgstab_workspace(int32_t n_cells, int32_t n_seg, int64_t nnz_pad, int64_t *bytes);
+int gf_seg_bicgstab(const gf_mesh *m, int32_t n_seg, const int32_t *seg_row_host, const double *diag,
+                    const double *upper, const double *lower, const double *b, double *x, int32_t precond,
+                    double atol, double rtol, int32_t max_iter, int32_t max_restart, int32_t check_every,
+                    void *workspace, int64_t workspace_bytes, int32_t *iters_out, double *res_out,
+                    uint8_t *conv_out, int32_t *n_active_host, gf_stream_t stream);
 
 /* ---- axis-projected immersed boundary (replaces the AP payload fluxel's
  * builder hands to AxisProjectedGrid, src/gridfoam/core/grid/axis_projected.py:163-211;

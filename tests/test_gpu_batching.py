@@ -109,6 +109,68 @@ def test_segmented_cg_follows_each_case_cg(rtol, max_iter):
         assert all(c == max_iter for c in counts) and not any(st.converged)
 
 
+def _convection(grids, seed, shifts):
+    """An upwind convection-diffusion operator per grid (random face flux and
+    diffusivity, diagonal shifted): the shape of a momentum matrix."""
+    from gridfoam_amd.config import DivScheme, DomainBoundaryPatch
+    from gridfoam_amd.fields import CellField, FaceField, NeumannBC
+    from gridfoam_amd.fv import fvm
+
+    gen = torch.Generator().manual_seed(seed)
+    out = []
+    for k, (g, shift) in enumerate(zip(grids, shifts)):
+        q = CellField(g, f"c{k}", ())
+        q.add_boundary_conditions({p: NeumannBC(0.0) for p in DomainBoundaryPatch})
+        phi = FaceField(g, f"phi_c{k}")
+        phi.single_data = (0.3 * torch.randn(g.num_internal_faces, dtype=torch.float64, generator=gen)).to(g.device)
+        gam = (torch.rand(g.num_cells, dtype=torch.float64, generator=gen) * 0.05 + 0.01).to(g.device)
+        m = fvm.assemble(q, div=1.0, lap=-1.0, phi=phi, gamma=gam, scheme=DivScheme.UPWIND, corrected=False)
+        m.diag = m.diag.clone() * (1.0 + shift) + 1e-3
+        out.append((q, m))
+    return out
+
+
+@pytest.mark.parametrize("rtol,max_iter", [(1e-10, 2000), (1e-10, 4)])
+def test_segmented_bicgstab_follows_each_case_bicgstab(rtol, max_iter):
+    """Three segments of an asymmetric operator with three right-hand sides
+    each: every (segment, component) takes the iterations of the single-case
+    BiCGSTAB on that component (to two iterations) and reaches its solution
+    (1e-8 relative), also when the iteration budget stops it."""
+    from gridfoam_amd.batching import SegmentedBiCGSTAB
+    from gridfoam_amd.config import SolverConfig
+    from gridfoam_amd.fvmatrix import Equation
+    from gridfoam_amd.solvers import create_solver
+
+    _say(f"segmented bicgstab rtol {rtol} max_iter {max_iter}")
+    grids = [_box(n) for n in (10, 15, 12)]
+    mats = _convection(grids, 7, [0.05, 0.01, 0.2])
+    M, rows, _, _ = _combined(grids, mats)
+    gen = torch.Generator().manual_seed(8)
+    B = torch.randn((rows[-1], 3), dtype=torch.float64, generator=gen).to(M.grid.device)
+    X0 = 0.1 * torch.randn((rows[-1], 3), dtype=torch.float64, generator=gen).to(M.grid.device)
+    x, st = SegmentedBiCGSTAB(0.0, rtol, max_iter, 5).solve(M, B, X0, rows)
+    solver = create_solver(SolverConfig(method="bicgstab", preconditioner="jacobi", tolerance=0.0,
+                                        rel_tolerance=rtol, max_iter=max_iter))
+    for k, (q, m) in enumerate(mats):
+        a, b = rows[k], rows[k + 1]
+        for c in range(3):
+            m.source = B[a:b, c].contiguous()
+            q.data = X0[a:b, c].contiguous()
+            r = solver.solve(Equation(q, m))
+            s = r.stats[0]
+            j = 3 * k + c
+            ref = r.solution
+            rel = float(torch.linalg.vector_norm(x[a:b, c] - ref) / torch.linalg.vector_norm(ref))
+            _say(f"segment {k} component {c}: {st.iterations[j]} / {s.iterations} iterations, rel {rel:.2e}")
+            # BiCGSTAB's residual is not monotone: over ~150 iterations the
+            # order of the dot products' sums can move the crossing of the
+            # threshold by an iteration or two
+            assert abs(st.iterations[j] - s.iterations) <= 2
+            assert st.converged[j] == s.converged
+            assert abs(st.initial_residual[j] - s.initial_residual) <= 1e-12 * s.initial_residual
+            assert rel < 1e-8
+
+
 def _requests(n, seed, n_range=(16, 22), residual=1e-6, max_iterations=2000):
     from gridfoam_amd.serving import random_case
 
@@ -120,7 +182,7 @@ def _requests(n, seed, n_range=(16, 22), residual=1e-6, max_iterations=2000):
 def test_batched_simple_is_independent_of_the_batch_and_matches_the_driver():
     """15 batched SIMPLE iterations of three cases with obstacles: each case's
     U and p equal those of the same case batched alone bit for bit, and
-    those of its own SIMPLE driver to 1e-11 (relative)."""
+    those of its own SIMPLE driver to 1e-10 (relative)."""
     from gridfoam_amd.algorithms import create_case
     from gridfoam_amd.batching import Batch, new_case
     from gridfoam_amd.config import load_config
@@ -128,6 +190,9 @@ def test_batched_simple_is_independent_of_the_batch_and_matches_the_driver():
     raws = _requests(3, 3)
     for r in raws:
         r["simulator"]["fvSolution"]["algorithm"]["residualControl"] = {}
+        # momentum solved far below rounding of the fields: the driver's
+        # BiCGSTAB and the segmented one then agree whatever their sums' order
+        r["simulator"]["fvSolution"]["solvers"]["U"].update(tolerance=1e-14, rel_tolerance=1e-13)
     _say("batched simple: mix of 3")
     mix = Batch([new_case(k, load_config(r), r) for k, r in enumerate(raws)])
     assert sum(c.grid.num_immersed_faces for c in mix.cases) > 0
@@ -148,7 +213,7 @@ def test_batched_simple_is_independent_of_the_batch_and_matches_the_driver():
             alg.step()
         for a, b in ((one.cases[0].U, alg.U.data), (one.cases[0].p, alg.p.data)):
             rel = float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
-            assert rel < 1e-11, rel
+            assert rel < 1e-10, rel
 
 
 def test_server_retires_each_case_with_its_alone_result():
