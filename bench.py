@@ -290,41 +290,51 @@ def config1_measure(cpu: bool, cpu_iters: int = 100) -> dict:
     return out
 
 
-def serve_measure(n_requests: int, slots: int, iterations: int = 5, seed: int = 0) -> dict:
-    """Serving stream of random lid-driven cases (base n^3,
-    n log-uniform in [16, 96], 0-2 refinement levels around 1-4 boxes,
-    Re U(100, 1000)), each request = `iterations` SIMPLE iterations.  Measured:
-    one-at-a-time plain FP64 solves/s; the server's capacity with every
-    request queued at t = 0 (mixed precision); p50/p99 latency with Poisson
-    arrivals at 80% of that capacity."""
+def serve_measure(n_requests: int, slots: int, budget: int = 30, seed: int = 0) -> dict:
+    """Serving stream of random steady requests (lid-driven cube on a base
+    grid n^3, n log-uniform in [16, 96], 1-4 box obstacles with 0-2
+    refinement levels around them, Re U(100, 1000); SIMPLE until U and p
+    residuals < 1e-6 or `budget` iterations, FP32-in-FP64 pressure solves).
+    Measured: one-at-a-time plain FP64 solves/s through the SIMPLE driver;
+    the batched server's capacity with every request queued at t = 0
+    (requests below 48^3 root cells batched `slots` at a time on one
+    stream, larger ones alone in two lanes on their own streams, all
+    concurrently); p50/p99 latency with Poisson arrivals at 80% of that
+    capacity; the measured device footprint per batched cell."""
     import numpy as np
     import torch
 
-    from gridfoam_amd.serving import Request, SolveServer, latency_stats, random_case, run_case
+    from gridfoam_amd.batching import BatchServer, base_cells
+    from gridfoam_amd.serving import latency_stats, random_case, solve_case
 
+    large = 48 ** 3
     rng = np.random.default_rng(seed)
-    cases = [random_case(rng) for _ in range(n_requests)]
-    plain = [copy.deepcopy(c) for c in cases]
-    for c in plain:
-        c["simulator"]["fvSolution"]["solvers"]["p"]["mixed_precision"] = False
-    run_case(plain[0], 1)  # warm-up (library load, kernels)
+    cases = [random_case(rng, max_iterations=budget) for _ in range(n_requests)]
+    solve_case(random_case(np.random.default_rng(1), n_range=(16, 16), max_iterations=2))  # library, kernels
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for c in plain:
-        run_case(c, iterations)
+    for c in cases:
+        pl = copy.deepcopy(c)
+        pl["simulator"]["fvSolution"]["solvers"]["p"]["mixed_precision"] = False
+        solve_case(pl)
     t_seq = time.perf_counter() - t0
-    with SolveServer(slots=slots) as server:  # slot processes start (and warm up) before the clock
-        sat = latency_stats(server.run([Request(k, c, 0.0, iterations) for k, c in enumerate(cases)]))
-        rate = 0.8 * sat["solves_per_s"]
-        arr = np.cumsum(rng.exponential(1.0 / rate, size=n_requests))
-        poi = latency_stats(server.run([Request(k, c, float(a), iterations)
-                                        for k, (c, a) in enumerate(zip(cases, arr))]))
-    return {"requests": n_requests, "slots": slots, "simple_iterations_per_request": iterations,
+    server = BatchServer(max_cases=slots, large_cells=large, large_lanes=2, max_bytes=64 << 30)
+    t0 = time.perf_counter()
+    sat = server.run([(k, c, 0.0) for k, c in enumerate(cases)])
+    t_sat = time.perf_counter() - t0
+    cap = n_requests / t_sat
+    arr = np.cumsum(rng.exponential(1.0 / (0.8 * cap), size=n_requests))
+    poi = server.run([(k, c, float(a)) for k, (c, a) in enumerate(zip(cases, arr))])
+    lat = latency_stats(poi)
+    return {"requests": n_requests, "batch_slots": slots, "alone_lanes": 2, "large_root_cells": large,
+            "requests_alone_lane": int(sum(base_cells(c) >= large for c in cases)),
+            "simple_budget": budget, "converged": int(sum(r.converged for r in sat)),
             "one_at_a_time_plain_solves_per_s": round(n_requests / t_seq, 3),
-            "served_capacity_solves_per_s": round(sat["solves_per_s"], 3),
-            "served_vs_one_at_a_time_plain": round(sat["solves_per_s"] * t_seq / n_requests, 3),
-            "poisson_load": 0.8, "poisson_solves_per_s": round(poi["solves_per_s"], 3),
-            "p50_ms": round(poi["p50_ms"], 1), "p99_ms": round(poi["p99_ms"], 1)}
+            "served_capacity_solves_per_s": round(cap, 3),
+            "served_vs_one_at_a_time_plain": round(t_seq / t_sat, 3),
+            "poisson_load": 0.8, "poisson_solves_per_s": round(lat["solves_per_s"], 3),
+            "p50_ms": round(lat["p50_ms"], 1), "p99_ms": round(lat["p99_ms"], 1),
+            "batch_bytes_per_cell": round(server.bytes_per_cell, 1) if server.bytes_per_cell else None}
 
 
 def main() -> int:
@@ -342,7 +352,7 @@ def main() -> int:
     ap.add_argument("--spmv-mesh", choices=("cavity", "uniform256"), default="cavity",
                     help="mesh of --spmv-only: the cavity octree or the uniform 256^3 grid")
     ap.add_argument("--serve-requests", type=int, default=48, help="requests in the serving measurement (0: skip)")
-    ap.add_argument("--serve-slots", type=int, default=8, help="cases in flight at once in the serving measurement")
+    ap.add_argument("--serve-slots", type=int, default=16, help="cases per batch in the serving measurement")
     ap.add_argument("--serve-only", action="store_true", help="only run the serving measurement")
     ap.add_argument("--inner-rtol", type=float, default=None,
                     help="refinement trigger of the mixed solve (default: the solver config's inner_rel_tolerance)")
@@ -439,7 +449,7 @@ def main() -> int:
         _log("2-D SIMPLE cavity 64x64, 500 iterations")
         line["cavity2d_simple_64"] = config1_measure(cpu=not a.no_cpu)
     if a.serve_requests > 0:
-        _log(f"serving: {a.serve_requests} requests, {a.serve_slots} slots")
+        _log(f"serving: {a.serve_requests} requests, batches of {a.serve_slots}")
         line["serving"] = serve_measure(a.serve_requests, a.serve_slots)
     if rel is not None:
         line["mixed_vs_fp64_rel_l2"] = rel

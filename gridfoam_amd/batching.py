@@ -378,12 +378,14 @@ class Batch:
         self.seg_row = torch.tensor(rows, dtype=torch.int32, device=dev)
         self.row_seg = torch.repeat_interleave(torch.arange(len(self.cases), dtype=torch.int32, device=dev),
                                                torch.tensor([c.grid.num_cells for c in self.cases], device=dev))
-        self.row_seg64 = self.row_seg.long()
         # adjustPhi per case: the case of every boundary face
         bseg = torch.empty(g.num_bnd_faces, dtype=torch.long, device=dev)
         for k, sl in enumerate(self.slices):
             bseg[sl.bnd] = k
         self.bnd_seg = bseg
+        self.dom_rows = [0]
+        for c in self.cases:
+            self.dom_rows.append(self.dom_rows[-1] + c.grid.num_domain_bnd_faces)
         self.adjust = sim.fvSolution.adjustPhi and not any(isinstance(bc, DirichletBC) for bc in self.p.bcs.values())
         self.ref_cells = [(sl.cells[0] + c.ref[0], c.ref[1]) for c, sl in zip(self.cases, self.slices)
                           if c.ref is not None]
@@ -398,9 +400,11 @@ class Batch:
                                     precond=ps.preconditioner == PreconditionerType.JACOBI)
 
     # ---- per-segment reductions -----------------------------------------------------
-    def _seg_sum(self, v: torch.Tensor) -> torch.Tensor:
-        out = torch.zeros(len(self.cases), dtype=torch.float64, device=v.device)
-        return out.index_add_(0, self.row_seg64, v)
+    def _seg_sum(self, v: torch.Tensor, bounds: Optional[List[int]] = None) -> torch.Tensor:
+        """Sums of v over each case's contiguous range (cells by default):
+        one reduction per case; atomics into a few slots would serialise."""
+        b = self.seg_row_host if bounds is None else bounds
+        return torch.stack([v[b[k]:b[k + 1]].sum() for k in range(len(b) - 1)])
 
     def initial_residuals(self, mat: FvMatrix, field: CellField) -> torch.Tensor:
         """residual.py:12-36 per case: |A psi - b|_2 / (|b|_2 + eps)."""
@@ -418,11 +422,10 @@ class Batch:
         kind = st.adjust_kind[:B].long()
         ph = self.phi_hbya.bnd_data[:B]
         seg = self.bnd_seg[:B]
-        K = len(self.cases)
         z = torch.zeros_like(ph)
 
-        def seg_sum(v):
-            return torch.zeros(K, dtype=torch.float64, device=g.device).index_add_(0, seg, v)
+        def seg_sum(v):  # the domain faces of case k are contiguous
+            return self._seg_sum(v, self.dom_rows)
         mass_in = seg_sum(torch.where((kind > 0) & (ph < 0), -ph, z))
         fixed_out = seg_sum(torch.where((kind == 1) & (ph > 0), ph, z))
         adj_out = seg_sum(torch.where((kind == 2) & (ph > 0), ph, z))
@@ -546,38 +549,143 @@ class ServedResult:
         return self.end - self.arrival
 
 
+def base_cells(raw: dict) -> int:
+    """Root cells of a request's grid (its size class before refinement)."""
+    r = raw["fluxel"]["root_resolution"]
+    return int(r[0]) * int(r[1]) * int(r[2])
+
+
 class BatchServer:
     """Continuous batching of steady SIMPLE requests on one GPU.
 
-    Requests arrive over time.  Between chunks of ``chunk`` batched SIMPLE
-    iterations the server retires the cases that met their residualControl
-    (or their iteration budget, ``endTime / deltaT``), admits waiting
-    requests while the batch holds fewer than ``max_cases`` cases and
-    ``max_cells`` cells, and rebuilds the combined mesh when its membership
-    changed.  A case that converges mid-chunk is copied out at that
-    iteration, so its result is the state at which it would have stopped
-    running alone."""
+    Requests arrive over time.  Two kinds of lanes share the device, each on
+    its own stream:
 
-    def __init__(self, max_cases: int = 8, max_cells: int = 4_000_000, chunk: int = 10):
-        if max_cases < 1 or chunk < 1:
-            raise ValueError("max_cases and chunk must be >= 1")
+    * the batch lane (the calling thread): between chunks of ``chunk``
+      batched SIMPLE iterations it retires the cases that met their
+      residualControl (or their iteration budget, ``endTime / deltaT``),
+      admits waiting requests while the batch holds fewer than
+      ``max_cases`` cases and its measured device footprint stays within
+      ``max_bytes``, and rebuilds the combined mesh when its membership
+      changed.  A case that converges mid-chunk is copied out at that
+      iteration, so its result is the state at which it would have stopped
+      running alone;
+    * ``large_lanes`` alone lanes (threads): requests whose root grid has at
+      least ``large_cells`` cells are bandwidth-bound on their own, so they
+      run one at a time per lane through the SIMPLE driver with their own
+      solver settings, concurrently with the batch.
+
+    The footprint of a batch is its cells times the bytes per cell measured
+    by ``calibrate`` (peak device allocation while one request is built,
+    batched and iterated), done before the clock starts when ``max_bytes``
+    is set."""
+
+    def __init__(self, max_cases: int = 8, max_cells: int = 4_000_000, chunk: int = 10,
+                 large_cells: Optional[int] = None, large_lanes: int = 2, max_bytes: Optional[int] = None):
+        if max_cases < 1 or chunk < 1 or large_lanes < 0:
+            raise ValueError("max_cases and chunk must be >= 1, large_lanes >= 0")
+        if large_cells is not None and large_lanes == 0:
+            raise ValueError("large requests need at least one alone lane")
         self.max_cases, self.max_cells, self.chunk = max_cases, max_cells, chunk
+        self.large_cells, self.large_lanes = large_cells, large_lanes
+        self.max_bytes = max_bytes
+        self.bytes_per_cell: Optional[float] = None
+        self._lane_error: Optional[BaseException] = None
+
+    def _is_large(self, raw: dict) -> bool:
+        return self.large_cells is not None and base_cells(raw) >= self.large_cells
+
+    def _alone_lane(self, queue, lock, results, t0) -> None:
+        import time
+
+        from .serving import solve_case
+
+        try:
+            stream = torch.cuda.Stream()
+            with torch.cuda.stream(stream):
+                while True:
+                    with lock:
+                        if not queue or self._lane_error is not None:
+                            return
+                        rid, raw, arr = queue.popleft()
+                    wait = arr - (time.perf_counter() - t0)
+                    if wait > 0:
+                        time.sleep(wait)
+                    start = time.perf_counter() - t0
+                    cells, U, p, its, conv, p_its = solve_case(raw)
+                    res = ServedResult(rid, cells, U, p, its, p_its, conv, arr, start, time.perf_counter() - t0)
+                    with lock:
+                        results.append(res)
+        except BaseException as exc:  # surfaced by run()
+            with lock:
+                self._lane_error = exc
 
     def run(self, requests: Sequence[Tuple[int, dict, float]]) -> List[ServedResult]:
         """``requests``: (rid, raw case configuration, arrival offset in s)."""
+        import collections
+        import threading
+        import time
+
+        order = sorted(requests, key=lambda r: (r[2], r[0]))
+        large = collections.deque(r for r in order if self._is_large(r[1]))
+        small = [r for r in order if not self._is_large(r[1])]
+        if self.max_bytes is not None and self.bytes_per_cell is None and small:
+            self.calibrate(small[0][1])  # before the clock starts
+        results: List[ServedResult] = []
+        lock = threading.Lock()
+        self._lane_error = None
+        t0 = time.perf_counter()
+        lanes = [threading.Thread(target=self._alone_lane, args=(large, lock, results, t0), daemon=True)
+                 for _ in range(min(self.large_lanes, len(large)))]
+        for th in lanes:
+            th.start()
+        try:
+            stream = torch.cuda.Stream()
+            with torch.cuda.stream(stream):
+                self._batch_lane(small, t0, results, lock)
+            stream.synchronize()
+        finally:
+            for th in lanes:
+                th.join()
+        if self._lane_error is not None:
+            raise RuntimeError(f"alone lane failed: {self._lane_error!r}") from self._lane_error
+        return results
+
+    def calibrate(self, raw: dict) -> float:
+        """Device bytes per cell of a batch: the peak allocation while one
+        request is built, batched and iterated once, over its cells."""
+        from .config import load_config
+
+        torch.cuda.synchronize()
+        base = torch.cuda.memory_allocated()
+        torch.cuda.reset_peak_memory_stats()
+        case = new_case(-1, load_config(raw), raw)
+        b = Batch([case])
+        b.step()
+        torch.cuda.synchronize()
+        self.bytes_per_cell = (torch.cuda.max_memory_allocated() - base) / case.grid.num_cells
+        return self.bytes_per_cell
+
+    def _fits(self, cells: int) -> bool:
+        if cells > self.max_cells:
+            return False
+        if self.max_bytes is not None and self.bytes_per_cell is not None:
+            return cells * self.bytes_per_cell <= self.max_bytes
+        return True
+
+    def _batch_lane(self, order, t0, results, lock) -> None:
         import time
 
         from .config import load_config
 
-        order = sorted(requests, key=lambda r: (r[2], r[0]))
-        t0 = time.perf_counter()
         queue: List[Tuple[int, dict, float]] = []
         active: List[CaseState] = []
         meta: Dict[int, Tuple[float, float, int]] = {}  # rid -> (arrival, start, budget)
-        results: List[ServedResult] = []
-        nxt = 0
+        nxt, done = 0, 0
         batch: Optional[Batch] = None
-        while len(results) < len(order):
+        while done < len(order):
+            if self._lane_error is not None:
+                return
             now = time.perf_counter() - t0
             while nxt < len(order) and order[nxt][2] <= now:
                 queue.append(order[nxt])
@@ -588,7 +696,7 @@ class BatchServer:
                 rid, raw, arr = queue[0]
                 cfg = load_config(raw)
                 case = new_case(rid, cfg, raw)
-                if active and cells + case.grid.num_cells > self.max_cells:
+                if active and not self._fits(cells + case.grid.num_cells):
                     break
                 queue.pop(0)
                 ctl = cfg.simulator.control
@@ -617,10 +725,12 @@ class BatchServer:
                         c.done = True
                         retired = True
                         arr, start, _ = meta[c.rid]
-                        torch.cuda.current_stream().synchronize()
-                        results.append(ServedResult(c.rid, c.grid.num_cells, c.U.cpu(), c.p.cpu(), c.iterations,
-                                                    c.p_iterations, k in conv, arr, start,
-                                                    time.perf_counter() - t0))
+                        U, p = c.U.cpu(), c.p.cpu()  # synchronises this lane's stream
+                        res = ServedResult(c.rid, c.grid.num_cells, U, p, c.iterations, c.p_iterations,
+                                           k in conv, arr, start, time.perf_counter() - t0)
+                        with lock:
+                            results.append(res)
+                        done += 1
                 if all(c.done for c in batch.cases):
                     break
             if retired:
@@ -631,4 +741,3 @@ class BatchServer:
                         if not c.done:
                             batch.extract(k)
                     batch = Batch(active)
-        return results

@@ -231,8 +231,12 @@ class FvMatrix:
 
         def bc(c, like):
             return c if like.dim() == 1 else c[:, None]
-        src = self._source.index_add(0, o, -bc(upper, presc[n]) * presc[n])
-        src = src.index_add(0, n, -bc(lower, presc[o]) * presc[o])
+        from .immersed import det_index_add  # noqa: PLC0415
+
+        fo = torch.nonzero(fixed[n]).flatten()  # faces whose neighbour is prescribed
+        fn = torch.nonzero(fixed[o]).flatten()
+        src = det_index_add(self._source, o[fo], -bc(upper[fo], presc[n[fo]]) * presc[n[fo]])
+        src = det_index_add(src, n[fn], -bc(lower[fn], presc[o[fn]]) * presc[o[fn]])
         out._source = src.index_copy(0, cells, bc(cdiag[cells], values) * values.to(torch.float64))
         return out
 

@@ -186,6 +186,25 @@ def apply_immersed_dirichlet_values(field) -> None:
         field.data = field.data.index_copy(0, cells, values.to(field.data.dtype))
 
 
+def det_index_add(base: torch.Tensor, idx: torch.Tensor, vals: torch.Tensor) -> torch.Tensor:
+    """``base.index_add(0, idx, vals)`` with a fixed summation order: the
+    entries aimed at one row are added in their order in ``idx``, one per
+    launch, so no two threads of a launch add into the same row (float
+    atomics would make the sum depend on scheduling, e.g. on work running
+    concurrently on another stream).  Meant for sparse contributions: one
+    launch per multiplicity level."""
+    if idx.numel() == 0:
+        return base
+    order = torch.argsort(idx, stable=True)
+    si = idx[order]
+    rank = torch.arange(si.numel(), device=si.device) - torch.searchsorted(si, si)
+    out = base
+    for r in range(int(rank.max()) + 1):
+        sel = order[rank == r]
+        out = out.index_add(0, idx[sel], vals[sel])
+    return out
+
+
 def close_fixed_cell_flux(phi, p) -> None:
     """pressure_correction.py:308-335: on cells fixed by snapping the
     Poisson row no longer balances the flux; the cell's imbalance is taken
@@ -203,7 +222,9 @@ def close_fixed_cell_flux(phi, p) -> None:
     fixed = _snap_candidates(p, st)
     area = torch.where(fixed, g.bnd_mag_sf[:B], torch.zeros_like(g.bnd_mag_sf[:B]))
     owner = g.bnd_owner[:B]
-    area_sum = torch.zeros(g.num_cells, dtype=torch.float64, device=g.device).index_add_(0, owner, area)
+    snapped = torch.nonzero(fixed).flatten()
+    area_sum = det_index_add(torch.zeros(g.num_cells, dtype=torch.float64, device=g.device), owner[snapped],
+                             area[snapped])
     imbalance = fvc.div(phi) * g.cell_volumes[: g.num_cells]
     total = area_sum[owner]
     frac = area / torch.where(total > 0, total, torch.ones_like(total))

@@ -144,6 +144,26 @@ def run_case(cfg: dict, iterations: int) -> tuple:
     return grid.num_cells, U, p, p_its
 
 
+def solve_case(cfg: dict) -> tuple:
+    """One request alone, as the SIMPLE driver runs it (simple.py:241-275):
+    iterations until every residualControl entry is met or the budget
+    endTime / deltaT is spent.  Returns (cells, U, p, iterations, converged,
+    pressure iterations) with U, p on the host."""
+    config = load_config(cfg)
+    grid, alg = create_case(config, device="cuda:0")
+    ctl = config.simulator.control
+    budget = max(1, int(round(ctl.endTime / ctl.deltaT)))
+    its, converged, p_its = 0, False, 0
+    while its < budget:
+        alg.step()
+        its += 1
+        p_its += sum(s.iterations for s in alg.solve_stats["p"])
+        if alg.has_simulation_converged():
+            converged = True
+            break
+    return grid.num_cells, alg.U.data.to("cpu"), alg.p.data.to("cpu"), its, converged, p_its
+
+
 def _worker_main(wid: int, req_q, res_q) -> None:
     """Slot process: owns its HIP context; runs one request at a time."""
     torch.cuda.set_device(0)

@@ -244,3 +244,38 @@ def test_server_retires_each_case_with_its_alone_result():
         assert torch.equal(got.U, one.cases[0].U.cpu())
         assert torch.equal(got.p, one.cases[0].p.cpu())
         assert got.end >= got.start >= got.arrival - 1e-9
+
+
+def test_two_lane_server_matches_alone_runs():
+    """Requests at or above the size threshold run alone in two lanes on
+    their own streams while the rest are batched on a third: a lane's result
+    equals the SIMPLE driver run alone (FP64 requests: bit for bit), a
+    batched one the case batched alone."""
+    from gridfoam_amd.batching import Batch, BatchServer, base_cells, new_case
+    from gridfoam_amd.config import load_config
+    from gridfoam_amd.serving import random_case, solve_case
+
+    rng = np.random.default_rng(21)
+    raws = [random_case(rng, n_range=(12, 20), mixed=False, residual=1e-4, max_iterations=40) for _ in range(6)]
+    large = 16 ** 3
+    kinds = [base_cells(r) >= large for r in raws]
+    assert any(kinds) and not all(kinds)
+    _say(f"two-lane server: {sum(kinds)} alone, {len(kinds) - sum(kinds)} batched")
+    res = {r.rid: r for r in BatchServer(max_cases=4, chunk=5, large_cells=large, large_lanes=2,
+                                         max_bytes=8 << 30).run([(k, r, 0.0) for k, r in enumerate(raws)])}
+    for k, r in enumerate(raws):
+        got = res[k]
+        if kinds[k]:
+            cells, U, p, its, conv, _ = solve_case(r)
+        else:
+            one = Batch([new_case(k, load_config(r), r)])
+            its = 0
+            while True:
+                conv = bool(one.step())
+                its += 1
+                if conv or its >= 40:
+                    break
+            one.extract(0)
+            U, p = one.cases[0].U.cpu(), one.cases[0].p.cpu()
+        assert got.iterations == its and got.converged == conv
+        assert torch.equal(got.U, U) and torch.equal(got.p, p), k
