@@ -137,17 +137,22 @@ def spmv_roofline(grid, pmat, fp32: bool, launches: int = 200) -> dict:
     ref = pmat.multiply(x)
     err = float(torch.linalg.norm(y - ref) / torch.linalg.norm(ref))
     C, F = grid.num_cells, grid.num_internal_faces
-    # Algorithmic bytes of one k_cg_dir launch: the symmetric LDU SpMV
-    # (diagonal s_d C, one off-diagonal value per face s_v F, two int32
-    # indices per face 8F, x read and Ap written 2 s_x C) plus the direction
-    # update it carries (p_old read, p_new written: 2 s_x C).
-    # FP64: 40C + 16F ; mixed operator (FP32 values/vectors, FP64 diagonal): 24C + 12F.
-    sd, sv, sx = (8, 4, 4) if fp32 else (8, 8, 8)
-    B = sd * C + sv * F + 8 * F + 4 * sx * C
+    if fp32:
+        # Algorithmic bytes of one fused mixed-precision launch k_cg_fused: the
+        # symmetric LDU SpMV values (one FP32 value per face, 4F, two int32
+        # indices per face, 8F) plus ten FP32 vectors of C: row sum and M^-1,
+        # r, Ap, x and p_old read; x, r, p and Ap written.  40C + 12F.
+        B = 40 * C + 12 * F
+    else:
+        # Algorithmic bytes of one k_cg_dir launch: the symmetric LDU SpMV
+        # (diagonal 8C, one off-diagonal value per face 8F, two int32 indices
+        # per face 8F, x read and Ap written 16C) plus the direction update it
+        # carries (p_old read, p written: 16C).  40C + 16F.
+        B = 40 * C + 16 * F
     achieved = B / avg_s / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _traffic(fp32, C),
-            "kernel": "k_cg_dir<float, 1, true>" if fp32 else "k_cg_dir<double, 1, false>",
+            "kernel": f"k_cg_fused<1024, {os.environ.get('GRIDFOAM_FUSED_BRICKS', '1')}>" if fp32 else "k_cg_dir<double, 1, false>",
             "cells": C, "bricks": int(grid.mesh.n_bricks), "brick_cells_frac": round(64 * grid.mesh.n_bricks / C, 4),
             "bytes_per_launch": int(B), "avg_us": round(avg_s * 1e6, 2), "rel_err_vs_fp64": err}
 

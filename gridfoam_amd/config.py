@@ -222,7 +222,7 @@ class SolverConfig(BaseModel, frozen=True):  # meta/config.py:269-316
     max_restart: int = 5
     log_interval: int = 25
     mixed_precision: bool = False
-    inner_rel_tolerance: float = 0.7  # mixed: refine once the FP32 residual fell to this fraction of the last true one
+    inner_rel_tolerance: float = 0.5  # mixed: refine once the FP32 residual fell to this fraction of the last true one
 
 
 class RelaxationFactorsConfig(BaseModel, frozen=True):

@@ -501,6 +501,18 @@ typedef const __attribute__((address_space(4))) int32_t ConstI32;
 // bricks, all of whose loads are issued before the solver state is read, so
 // the dependent neighbour-brick gathers of several bricks overlap.
 constexpr int kBricks = 2;
+// Threads per workgroup of the fused mixed-precision launch, and its bricks
+// per unit (GRIDFOAM_FUSED_BRICKS = 1 or 2 overrides the default for tuning).
+constexpr int kFusedTB = 1024;
+constexpr int kFusedBricks = 1;
+int fused_bricks() {
+    static int kb = [] {
+        const char *e = getenv("GRIDFOAM_FUSED_BRICKS");
+        const int v = e ? atoi(e) : kFusedBricks;
+        return v == 2 ? 2 : 1;
+    }();
+    return kb;
+}
 
 // Waves that carry bricks: whole workgroups only (bricks left over go
 // through the sliced ELL with the other rows).
@@ -517,6 +529,30 @@ __host__ __device__ __forceinline__ int brick_workgroups(int nbrick, int tb) {
 int pcg_dir_workgroups(int C, int nbrick) {
     const int nsl = (C + 63) / 64, nbw = brick_waves(nbrick, pcg_tile()), per = pcg_tile() / 64;
     return brick_workgroups(nbrick, pcg_tile()) + round8((nsl - nbw * kBricks + per - 1) / per);
+}
+// Workgroups of the persistent fused launch: one per CU (GRIDFOAM_FUSED_WG
+// overrides it for tuning), a multiple of 8.
+int fused_workgroups() {
+    static int g = [] {
+        const char *e = getenv("GRIDFOAM_FUSED_WG");
+        int v = e ? atoi(e) : 0;
+        if (v <= 0) {
+            int dev = 0, n = 0;
+            if (hipGetDevice(&dev) == hipSuccess &&
+                hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+                v = n;
+            else
+                v = 256;
+        }
+        return round8(v);
+    }();
+    return g;
+}
+// Partials per (rhs, quantity) of a solve: the largest workgroup count of any of its kernels.
+int solve_pstride(int C, int nbrick) {
+    int m = partial_stride(C);
+    for (int v : {pcg_dir_workgroups(C, nbrick), fused_workgroups()}) m = v > m ? v : m;
+    return m;
 }
 
 // Everything a brick row needs that does not depend on the solver state.
@@ -968,6 +1004,422 @@ __global__ __launch_bounds__(RB) void k_cg_upd(Ops o, int s, int pb, const VT *_
     state_out<NR>(o, s, S);
 }
 
+// ---------------------------------------------------------------- mixed PCG: one launch per iteration
+// The FP32 inner solve of the mixed-precision PCG runs one launch per
+// iteration.  Launch k finishes the update of iteration k-1 and forms the
+// next direction in the same pass:
+//   alpha = g / d ; r' = r - alpha Ap ; x' = x + alpha p_old ; z = M^-1 r' ;
+//   beta = (g - 2 alpha m + alpha^2 n) / g ; p = z + beta p_old ; Ap = A p
+// with g = r.z, d = p.Ap, m = Ap.z, n = Ap.M^-1 Ap of the previous launch
+// (its partials), so the new r'.z (the numerator of beta) is expanded from
+// the previous launch's dot products instead of reduced after the update.
+// Every gathered neighbour p_j is rebuilt from r_j, Ap_j, M^-1_j and p_old_j
+// by the same function its owner row uses (bit-identical).  r, Ap and p are
+// double-buffered by launch parity: a launch reads buffer pb and writes
+// pb ^ 1, so no workgroup reads what another one writes in the same launch.
+// The operator is the FP32 off-diagonals and the FP32-rounded FP64 row sum
+// e = diag + sum_j a_ij: (A p)_i = e_i p_i + sum_j a_ij (p_j - p_i), summed
+// in FP64, so the near-null constant mode of a Neumann pressure operator
+// keeps its FP64 row sums.  Launches without an update (fresh != 0): the
+// first direction p = z (fresh 1), and the direction rebuilt after an FP64
+// refinement replaced r, p = z + (r.z / rz_old) p_old (fresh 2).
+enum FQ : int { FQ_NR = 0, FQ_G = 1, FQ_D = 2, FQ_M = 3, FQ_N = 4, kFQ = 5 };
+
+struct FusedT {
+    bool act, upd, start;
+    float al, be;
+};
+
+// State transition of a fused launch, applied by every wave to the same
+// inputs.  A residual (of the previous update) below the threshold or a
+// spent budget hands the solve to the FP64 refinement (which decides on the
+// true residual); nothing is updated then.
+__device__ __forceinline__ void fused_transition(const Ops &o, int pb, const double (&t)[1][kFQ], KState &S,
+                                                 FusedT &T) {
+    T.act = T.upd = T.start = false;
+    T.al = T.be = 0.0f;
+    if (S.phase != PH_ACTIVE) return;
+    if (S.fresh == 0) {
+        const double nr = nrm_fin(t[0][FQ_NR], o.norm_inf);
+        if (nr < S.thresh || S.iter >= o.max_iter) {
+            S.phase = PH_RESTART;
+            return;
+        }
+        const double g = t[0][FQ_G], al = g / t[0][FQ_D];
+        const double rz = g - 2.0 * al * t[0][FQ_M] + al * al * t[0][FQ_N];
+        S.alpha = al;
+        S.beta = rz / g;
+        S.rz_old = g;
+        S.iter += 1;
+        T.upd = true;
+    } else {
+        T.start = S.fresh == 1;
+        S.alpha = 0.0;
+        S.beta = T.start ? 0.0 : t[0][FQ_G] / S.rz_old;
+        // A true residual much larger than the recursive one it replaced
+        // (beta > 4: twice its M^-1 norm) means the FP32 iteration had run
+        // into rounding noise, and so has its direction: restart from p = z.
+        if (S.beta > 4.0) {
+            S.beta = 0.0;
+            T.start = true;
+        }
+    }
+    S.fresh = 0;
+    S.pcur = pb ^ 1;
+    T.act = true;
+    T.al = (float)S.alpha;
+    T.be = (float)S.beta;
+}
+
+// r' , z and p of one row from its previous r, Ap, p and M^-1 (owner rows
+// and gathered neighbours alike).
+__device__ __forceinline__ float fused_dir(const FusedT &T, float r, float ap, float id, float po, float &rn,
+                                           float &z) {
+    rn = T.upd ? __builtin_fmaf(-T.al, ap, r) : r;
+    z = __fmul_rn(id, rn);
+    return T.start ? z : __builtin_fmaf(T.be, po, z);
+}
+
+struct FusedVec {
+    const float *e, *id;
+    float *x, *r0, *r1, *a0, *a1, *p0, *p1;
+};
+
+// Everything a brick row of the fused launch needs that does not depend on
+// the solver state: its own row, its +axis values, the -axis values toward
+// neighbour bricks and the mirror lanes across brick sides (its own row
+// where there is none).  Explicit sides are loaded when they are applied.
+struct FBrick {
+    float e, id, r, ap, x, po;
+    float vp[3], vm[3];
+    float xr[3], xa[3], xi[3], xp[3];
+    int nb[6];
+};
+
+__device__ __forceinline__ void fbrick_load(const Ops &o, int sl, int lane, const FusedVec &V, const float *rin,
+                                            const float *ain, const float *pin, const float *__restrict__ bv,
+                                            const float *__restrict__ bm, const float *__restrict__ ev, FBrick &B) {
+    const int i = sl * 64 + lane;
+    ConstI32 *nbp = (ConstI32 *)(o.bnbr + 6 * sl);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) B.nb[k] = nbp[k];
+    B.e = V.e[i];
+    B.id = V.id[i];
+    B.r = rin[i];
+    B.ap = ain[i];
+    B.x = V.x[i];
+    B.po = pin[i];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        B.vp[a] = bv[((size_t)sl * 3 + a) * 64 + lane];
+        const int c = bcoord(lane, a);
+        const int nbr = c == 3 ? B.nb[2 * a + 1] : B.nb[2 * a];
+        const bool hx = (c == 0 || c == 3) && nbr >= 0;
+        const int e = hx ? nbr * 64 + (lane ^ (3 << (2 * a))) : i;
+        B.xr[a] = rin[e];
+        B.xa[a] = ain[e];
+        B.xi[a] = V.id[e];
+        B.xp[a] = pin[e];
+        const float m = bm[((size_t)sl * 3 + a) * 16 + bface(lane, a)];
+        B.vm[a] = (c == 0 && hx) ? m : 0.0f;
+    }
+}
+
+// (A p)_i of a brick row whose new direction value is p, in the difference
+// form e p + sum_j a_j (p_j - p) summed in FP32 (a constant p gives exactly
+// e p, so the near-null mode keeps its FP64 row sum).  Every lane of the
+// wave takes part in the permutes (a lane masked off would read as zero).
+__device__ __forceinline__ float fbrick_apply(const Ops &o, int lane, const FusedT &T, float p, const FBrick &B,
+                                              const FusedVec &V, const float *rin, const float *ain,
+                                              const float *pin, const float *__restrict__ ev) {
+    float acc = __fmul_rn(B.e, p);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int st = 1 << (2 * a), c = bcoord(lane, a);
+        float rn, z;
+        const float pe = fused_dir(T, B.xr[a], B.xa[a], B.xi[a], B.xp[a], rn, z);
+        const float sp = __shfl(p, lane + st);
+        const float sm = __shfl(p, lane - st);
+        const float sv = __shfl(B.vp[a], lane - st);
+        const float qp = c == 3 ? pe : sp;  // a boundary lane takes its outside neighbour
+        const float qm = c == 0 ? pe : sm;  // (zero value where there is none)
+        const float vmn = c == 0 ? B.vm[a] : sv;
+        acc = __builtin_fmaf(B.vp[a], qp - p, acc);
+        acc = __builtin_fmaf(vmn, qm - p, acc);
+    }
+    // explicit side dd: slot l = 4 q + k holds face k of boundary cell q; the
+    // four slots of a cell are summed across lanes (sum a (p_j - p_q)), then
+    // read by the cell's lane
+#pragma unroll
+    for (int dd = 0; dd < 6; ++dd) {  // explicit sides (2:1 level changes): loaded here
+        if (B.nb[dd] <= -2) {
+            const int a = dd >> 1;
+            const int e = (-2 - B.nb[dd]) * 64 + lane;
+            const int c = o.ecol[e];
+            float rn, z;
+            const float pe = fused_dir(T, rin[c], ain[c], V.id[c], pin[c], rn, z);
+            // the boundary cell of slot l is lane cl: p of that cell from its lane
+            const int cellq = (lane >> 2);  // q of this slot
+            const int cl = a == 0 ? ((dd & 1) ? 3 : 0) + 4 * (cellq & 3) + 16 * (cellq >> 2)
+                         : a == 1 ? (cellq >> 2) + 4 * ((dd & 1) ? 3 : 0) + 16 * (cellq & 3)
+                                  : (cellq & 3) + 4 * (cellq >> 2) + 16 * ((dd & 1) ? 3 : 0);
+            const float pq = __shfl(p, cl);
+            float t = __fmul_rn(ev[e], pe - pq);
+            t += __shfl_xor(t, 1);
+            t += __shfl_xor(t, 2);
+            const float tl = __shfl(t, 4 * bface(lane, a));
+            if (bcoord(lane, a) == ((dd & 1) ? 3 : 0)) acc += tl;
+        }
+    }
+    return acc;
+}
+
+// Own-row outputs of a fused launch and its partials.
+__device__ __forceinline__ void fused_out(const Ops &o, const FusedT &T, int i, float id, float x, float po,
+                                          float rn, float z, float p, float a, const FusedVec &V, float *rout,
+                                          float *aout, float *pout, double (&v)[1][kFQ]) {
+    if (T.upd) V.x[i] = __builtin_fmaf(T.al, po, x);
+    rout[i] = rn;
+    pout[i] = p;
+    aout[i] = a;
+    const double rd = rn, ad = a;
+    v[0][FQ_NR] = o.norm_inf ? fmax(v[0][FQ_NR], fabs(rd)) : v[0][FQ_NR] + rd * rd;
+    v[0][FQ_G] += rd * (double)z;
+    v[0][FQ_D] += (double)p * ad;
+    v[0][FQ_M] += ad * (double)z;
+    v[0][FQ_N] += ad * (double)__fmul_rn(id, a);
+}
+
+// One unit of KB bricks of a fused launch whose loads were issued into Bk.
+template <int KB>
+__device__ __forceinline__ void fused_unit(const Ops &o, int lane, const FusedT &T, int sl, const FBrick (&Bk)[KB],
+                                           const FusedVec &V, const float *rin, const float *ain, const float *pin,
+                                           float *rout, float *aout, float *pout, const float *__restrict__ ev,
+                                           double (&v)[1][kFQ]) {
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+        float rn, z;
+        const float p = fused_dir(T, Bk[k].r, Bk[k].ap, Bk[k].id, Bk[k].po, rn, z);
+        const float acc = fbrick_apply(o, lane, T, p, Bk[k], V, rin, ain, pin, ev);
+        fused_out(o, T, (sl + k) * 64 + lane, Bk[k].id, Bk[k].x, Bk[k].po, rn, z, p, acc, V, rout, aout, pout, v);
+    }
+}
+
+// One slice of 64 rows of the sliced ELL (rows past C take no part).
+__device__ __forceinline__ void fused_slice(const Ops &o, int lane, const FusedT &T, int sl,
+                                            const float *__restrict__ val, const FusedVec &V, const float *rin,
+                                            const float *ain, const float *pin, float *rout, float *aout,
+                                            float *pout, double (&v)[1][kFQ]) {
+    constexpr int kHead = 6;
+    const int i = sl * 64 + lane;
+    const int ic = i < o.C ? i : o.C - 1;
+    const float e = V.e[ic], id = V.id[ic], r = rin[ic], ap = ain[ic], x = V.x[ic], po = pin[ic];
+    const int base = o.sell_ptr[sl];
+    const int W = (o.sell_ptr[sl + 1] - base) >> 6;
+    int col[kHead];
+    float vv[kHead], jr[kHead], ja[kHead], ji[kHead], jp[kHead];
+#pragma unroll
+    for (int u = 0; u < kHead; ++u) {
+        col[u] = ic;
+        vv[u] = 0.0f;
+        if (u < W) {
+            const int k = base + u * 64 + lane;
+            col[u] = __builtin_nontemporal_load(o.sell_col + k);
+            vv[u] = __builtin_nontemporal_load(val + k);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kHead; ++u) {
+        jr[u] = rin[col[u]];
+        ja[u] = ain[col[u]];
+        ji[u] = V.id[col[u]];
+        jp[u] = pin[col[u]];
+    }
+    float rn, z;
+    const float p = fused_dir(T, r, ap, id, po, rn, z);
+    float acc = __fmul_rn(e, p);  // difference form summed in FP32, as for brick rows
+#pragma unroll
+    for (int u = 0; u < kHead; ++u)
+        if (u < W) {
+            float rj, zj;
+            const float q = fused_dir(T, jr[u], ja[u], ji[u], jp[u], rj, zj);
+            acc = __builtin_fmaf(vv[u], q - p, acc);
+        }
+    for (int kb = kHead; kb < W; kb += kBatch) {  // rows wider than the head batch
+        int c2[kBatch];
+        float v2[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u) {
+            c2[u] = ic;
+            v2[u] = 0.0f;
+            if (kb + u < W) {
+                const int k = base + (kb + u) * 64 + lane;
+                c2[u] = __builtin_nontemporal_load(o.sell_col + k);
+                v2[u] = __builtin_nontemporal_load(val + k);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u)
+            if (kb + u < W) {
+                float rj, zj;
+                const float q = fused_dir(T, rin[c2[u]], ain[c2[u]], V.id[c2[u]], pin[c2[u]], rj, zj);
+                acc = __builtin_fmaf(v2[u], q - p, acc);
+            }
+    }
+    if (i < o.C) fused_out(o, T, i, id, x, po, rn, z, p, acc, V, rout, aout, pout, v);
+}
+
+// A persistent launch: workgroup t of G (XCD-tiled, one per CU by default)
+// owns a contiguous range of brick units (KB bricks each; bricks beyond the
+// last whole unit go through the sliced ELL) and a contiguous range of the
+// remaining sliced-ELL slices; wave w of it takes every (TB/64)-th unit,
+// then every (TB/64)-th slice.  The loads of a wave's first unit are issued
+// before the solver state is read and stay in flight during the reduction
+// of the previous launch's G partial sets; partials are written once per
+// workgroup, so the next launch reduces G sets.
+template <int TB, int KB>
+__global__ __launch_bounds__(TB) void k_cg_fused(Ops o, int s, int nb_prev, int pb, const float *__restrict__ val,
+                                                 FusedVec V, const float *__restrict__ bv,
+                                                 const float *__restrict__ bm, const float *__restrict__ ev) {
+    constexpr int NW = TB / 64;
+    Prefetched<1, kFQ, 1, TB> pre;
+    prefetch_prev(o, s, nb_prev, pre);
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int G = gridDim.x, t = xcd_tile(blockIdx.x, G);
+    const int U = o.nbrick / KB, pu = (U + G - 1) / G;
+    const int u0 = min(U, t * pu), u1 = min(U, u0 + pu);
+    const int nsl = (o.C + 63) >> 6, e0 = U * KB, pe = (nsl - e0 + G - 1) / G;
+    const int s0 = min(nsl, e0 + t * pe), s1 = min(nsl, s0 + pe);
+    const float *rin = pb ? V.r1 : V.r0, *ain = pb ? V.a1 : V.a0, *pin = pb ? V.p1 : V.p0;
+    float *rout = pb ? V.r0 : V.r1, *aout = pb ? V.a0 : V.a1, *pout = pb ? V.p0 : V.p1;
+    FBrick Bk[KB];
+    int u = u0 + w;
+    if (u < u1) {
+#pragma unroll
+        for (int k = 0; k < KB; ++k) fbrick_load(o, u * KB + k, lane, V, rin, ain, pin, bv, bm, ev, Bk[k]);
+    }
+    asm volatile("" ::: "memory");  // keep the loads above issued before the state test
+    KState S[1];
+    S[0] = state_in(o, s, 0);
+    if (S[0].phase != PH_ACTIVE) {
+        state_out<1>(o, s, S);
+        return;
+    }
+    const bool mx[kFQ] = {(bool)o.norm_inf, false, false, false, false};
+    double tot[1][kFQ];
+    reduce_prev<1, kFQ, TB, 1>(o, s, nb_prev, mx, tot, &pre);
+    FusedT T;
+    fused_transition(o, pb, tot, S[0], T);
+    double v[1][kFQ] = {};
+    if (T.act) {
+        if (u < u1) {
+            fused_unit<KB>(o, lane, T, u * KB, Bk, V, rin, ain, pin, rout, aout, pout, ev, v);
+            for (u += NW; u < u1; u += NW) {
+#pragma unroll
+                for (int k = 0; k < KB; ++k) fbrick_load(o, u * KB + k, lane, V, rin, ain, pin, bv, bm, ev, Bk[k]);
+                fused_unit<KB>(o, lane, T, u * KB, Bk, V, rin, ain, pin, rout, aout, pout, ev, v);
+            }
+        }
+        for (int sl = s0 + w; sl < s1; sl += NW) fused_slice(o, lane, T, sl, val, V, rin, ain, pin, rout, aout, pout, v);
+    }
+    write_partials<1, kFQ, TB>(o, s, v, mx);
+    state_out<1>(o, s, S);
+}
+
+// Refinement for the fused launches, part 1: the trigger (the last launch's
+// |r| below reliable * the last true residual, or a stop the launches
+// asked for), then x64 += x32 ; x32 = 0 ; the direction of the last update
+// moves to buffer 0 (where the rebuilding launch reads p_old).
+__global__ __launch_bounds__(RB) void k_mxf_accumulate(Ops o, int s, int nb_prev, float *__restrict__ x32,
+                                                      double *__restrict__ x64, float *__restrict__ p0,
+                                                      const float *__restrict__ p1) {
+    KState S[1];
+    S[0] = state_in(o, s, 0);
+    bool due = S[0].phase == PH_RESTART;
+    if (S[0].phase == PH_ACTIVE) {
+        const bool mx[kFQ] = {(bool)o.norm_inf, false, false, false, false};
+        double tot[1][kFQ];
+        reduce_prev<1, kFQ, RB>(o, s, nb_prev, mx, tot);
+        const double nr = nrm_fin(tot[0][FQ_NR], o.norm_inf);
+        if (nr < o.reliable * S[0].scale) {
+            S[0].phase = PH_RESTART;
+            due = true;
+        }
+    }
+    if (due) {
+        const bool mv = S[0].pcur == 0;  // p_old of the last update is in buffer pcur ^ 1
+        int lo, hi;
+        chunk_range(o.C, gridDim.x, xcd_tile(blockIdx.x, gridDim.x), lo, hi);
+        for (int i = lo + threadIdx.x; i < hi; i += RB) {
+            x64[i] += (double)x32[i];
+            x32[i] = 0.0f;
+            if (mv) p0[i] = p1[i];
+        }
+        S[0].pcur = 1;
+    }
+    state_out<1>(o, s, S);
+}
+
+// Refinement, part 3 (one workgroup): decide on the true residual (first
+// solve start or refinement), or pass the last fused launch's partials
+// through.  Either way one reduced set of the kFQ quantities is written for
+// the fused launch that follows.
+__global__ __launch_bounds__(RB) void k_mxf_refine(Ops o, int s, int first, int nb_res, int nb_upd) {
+    __shared__ KState sh[1];
+    __shared__ double red[kFQ];
+    load_state<1>(o, s, sh);
+    const bool rep = first || sh[0].phase == PH_RESTART;
+    const bool act = sh[0].phase == PH_ACTIVE;
+    if (rep || act) {
+        const bool mx[kFQ] = {(bool)o.norm_inf, false, false, false, false};
+        double tot[1][kFQ];
+        if (rep) {
+            const bool mx2[2] = {(bool)o.norm_inf, false};
+            double t2[1][2];
+            reduce_prev<1, 2, RB>(o, s, nb_res, mx2, t2);
+            tot[0][FQ_NR] = t2[0][0];
+            tot[0][FQ_G] = t2[0][1];
+            tot[0][FQ_D] = tot[0][FQ_M] = tot[0][FQ_N] = 0.0;
+        } else {
+            reduce_prev<1, kFQ, RB>(o, s, nb_upd, mx, tot);
+        }
+        if (threadIdx.x == 0) {
+            for (int q = 0; q < kFQ; ++q) red[q] = tot[0][q];
+            if (rep) {
+                KState &S = sh[0];
+                const double nr = nrm_fin(tot[0][FQ_NR], o.norm_inf);
+                if (first) {
+                    S.norm0 = nr;
+                    S.thresh = fmax(o.atol, o.rtol * nr);
+                    S.iter = 0;
+                    S.restarts = 0;
+                    S.converged = 0;
+                    S.rz_old = tot[0][FQ_G];
+                    S.pcur = 0;
+                }
+                S.final_res = nr;
+                if (nr < S.thresh) {
+                    S.converged = 1;
+                    S.phase = PH_DONE;
+                } else if (S.iter >= o.max_iter) {
+                    S.phase = PH_DONE;
+                } else {
+                    S.scale = nr;  // true residual at this refinement
+                    S.fresh = first ? 1 : 2;
+                    S.phase = PH_ACTIVE;
+                }
+            }
+        }
+        lds_sync();
+        if (blockIdx.x == 0 && threadIdx.x < kFQ) {
+            double *P = o.partials + (size_t)(s & 1) * kMaxRhs * kNQ * o.pstride;
+            P[(size_t)threadIdx.x * o.pstride] = red[threadIdx.x];
+        }
+    }
+    store_state<1>(o, s, sh);
+}
+
 // r = b - A x ; z = M^-1 r ; partials |r|, r.z  (cg.py:88-118)
 template <int NR, int TB>
 __global__ __launch_bounds__(TB) void k_cg_init(Ops o, int s, const double *__restrict__ val,
@@ -1027,171 +1479,62 @@ __global__ __launch_bounds__(RB) void k_cg_init_fin(Ops o, int s, int nb_prev) {
 
 // ---------------------------------------------------------------- mixed precision
 // FP32-storage PCG with FP64 residual replacement ("reliable updates").
-// Off-diagonal values and every CG vector are stored in FP32; the diagonal
-// is stored in FP64 and rebuilt so that each row of the FP32 operator has
-// the row sum of the FP64 operator.  FP32 x FP32 products are exact in FP64
-// and are accumulated in FP64, so the near-null (constant) mode of a
-// reference-pinned Neumann pressure matrix does not suffer cancellation.
-// When the recursive residual has dropped by `reliable` since the last
-// refinement (or below the threshold, or the budget ran out) the FP32
-// correction is added to the FP64 solution and the FP64 residual b - A x
-// replaces the recursive one; the direction recurrence continues, and
-// convergence is decided on that true residual only.
+// Off-diagonal values and every CG vector are stored in FP32, the diagonal
+// as the FP32-rounded FP64 row sum (see the fused launch above).  FP32 x
+// FP32 products are exact in FP64 and are accumulated in FP64, so the
+// near-null (constant) mode of a reference-pinned Neumann pressure matrix
+// does not suffer cancellation.  At the end of a chunk, when the recursive
+// residual has dropped by `reliable` since the last refinement (or below
+// the threshold, or the budget ran out), the FP32 correction is added to
+// the FP64 solution, the FP64 residual b - A x replaces the recursive one
+// and the next launch rebuilds the direction from it; convergence is
+// decided on that true residual only.
 
-// val32 = (float) val64 ; dgc = diag + sum_row(val64 - val32)
-__global__ void k_mx_prep(int C, const int32_t *sell_ptr, const double *val64, const double *diag,
-                          const int32_t *perm, float *val32, double *dgc) {
+// val32 = (float) val64 ; e = (float)(diag + sum_row val64): the FP64 row sum
+__global__ void k_mxf_prep(int C, const int32_t *sell_ptr, const double *val64, const double *diag,
+                           const int32_t *perm, float *val32, float *e) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= C) return;
     const int sl = i >> 6, lane = i & 63;
     const int base = sell_ptr[sl];
     const int W = (sell_ptr[sl + 1] - base) >> 6;
-    double corr = 0.0;
+    double sum = diag[perm ? perm[i] : i];
     for (int k = 0; k < W; ++k) {
-        const int e = base + k * 64 + lane;
-        const double v = val64[e];
-        const float v32 = (float)v;
-        val32[e] = v32;
-        corr += v - (double)v32;
+        const int q = base + k * 64 + lane;
+        const double v = val64[q];
+        val32[q] = (float)v;
+        sum += v;
     }
-    dgc[i] = diag[perm ? perm[i] : i] + corr;
+    e[i] = (float)sum;
 }
 
-// Refinement, part 1: x64 += x32 ; x32 = 0 ; the current direction moves
-// to buffer 0 if the chunk stopped after an odd number of iterations (the
-// next chunk's first iteration reads p_old from buffer 0).
-template <int NR>
-__global__ __launch_bounds__(RB) void k_mx_accumulate(Ops o, int s, float *__restrict__ x32,
-                                                      double *__restrict__ x64, float *__restrict__ p0,
-                                                      const float *__restrict__ p1) {
-    __shared__ KState sh[NR];
-    load_state<NR>(o, s, sh);
-    if (any_phase<NR>(sh, PH_RESTART)) {
-        bool rs[NR], mv[NR];
-#pragma unroll
-        for (int r = 0; r < NR; ++r) {
-            rs[r] = sh[r].phase == PH_RESTART;
-            mv[r] = rs[r] && sh[r].pcur != 0;
-        }
-        int lo, hi;
-        chunk_range(o.C, gridDim.x, xcd_tile(blockIdx.x, gridDim.x), lo, hi);
-        for (int i = lo + threadIdx.x; i < hi; i += RB)
-#pragma unroll
-            for (int r = 0; r < NR; ++r) {
-                if (!rs[r]) continue;
-                const size_t k = (size_t)r * o.C + i;
-                x64[k] += (double)x32[k];
-                x32[k] = 0.0f;
-                if (mv[r]) p0[k] = p1[k];
-            }
-        if (threadIdx.x == 0)
-            for (int r = 0; r < NR; ++r)
-                if (mv[r]) sh[r].pcur = 0;
-    }
-    store_state<NR>(o, s, sh);
-}
-
-// Refinement, part 2: r64 = b - A64 x64 ; r32 = r64 ; z32 = M^-1 r32 ;
-// partials |r64| (norm type), r32.z32.  `first` also starts the solve (x32 = 0).
-template <int NR, int TB>
+// Refinement, part 2: r64 = b - A64 x64 ; r32 = r64 ; z = M^-1 r32 ;
+// partials |r64| (norm type), r32.z.  `first` also starts the solve (x32 = 0).
+template <int TB>
 __global__ __launch_bounds__(TB) void k_mx_residual(Ops o, int s, int first, const double *__restrict__ val,
                                                     const double *__restrict__ dg, const float *__restrict__ inv_d,
                                                     const double *__restrict__ b, const double *__restrict__ x64,
-                                                    float *__restrict__ r32, float *__restrict__ z32,
-                                                    float *__restrict__ x32) {
-    __shared__ KState sh[NR];
-    load_state<NR>(o, s, sh);
-    bool act[NR];
-    bool any = false;
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-        act[r] = first || sh[r].phase == PH_RESTART;
-        any |= act[r];
-    }
-    if (any) {
+                                                    float *__restrict__ r32, float *__restrict__ x32) {
+    __shared__ KState sh[1];
+    load_state<1>(o, s, sh);
+    if (first || sh[0].phase == PH_RESTART) {
         const int i = xcd_tile(blockIdx.x, gridDim.x) * TB + threadIdx.x;
-        double v[NR][2] = {};
+        double v[1][2] = {};
         if (i < o.C) {
-            double ax[NR];
-            sell_row<double, NR>(o.sell_ptr, o.sell_col, val, dg[i], i, x64, o.C, ax);
-            const float id = inv_d[i];
-#pragma unroll
-            for (int r = 0; r < NR; ++r) {
-                if (!act[r]) continue;
-                const size_t k = (size_t)r * o.C + i;
-                const double rr = b[k] - ax[r];
-                const float r32v = (float)rr;
-                const float zz = id * r32v;
-                r32[k] = r32v;
-                z32[k] = zz;
-                if (first) x32[k] = 0.0f;
-                v[r][0] = o.norm_inf ? fabs(rr) : rr * rr;
-                v[r][1] = (double)r32v * (double)zz;
-            }
+            double ax[1];
+            sell_row<double, 1>(o.sell_ptr, o.sell_col, val, dg[i], i, x64, o.C, ax);
+            const double rr = b[i] - ax[0];
+            const float r32v = (float)rr;
+            const float zz = __fmul_rn(inv_d[i], r32v);
+            r32[i] = r32v;
+            if (first) x32[i] = 0.0f;
+            v[0][0] = o.norm_inf ? fabs(rr) : rr * rr;
+            v[0][1] = (double)r32v * (double)zz;
         }
         const bool mx[2] = {(bool)o.norm_inf, false};
-        write_partials<NR, 2, TB>(o, s, v, mx);
+        write_partials<1, 2, TB>(o, s, v, mx);
     }
-    store_state<NR>(o, s, sh);
-}
-
-// Refinement, part 3 (one workgroup): decide on the true residual, or pass
-// the last update's |r|, r.z through.  Either way one reduced partial set is
-// written for the k_cg_dir launch that follows.
-template <int NR>
-__global__ __launch_bounds__(RB) void k_mx_refine(Ops o, int s, int first, int nb_res, int nb_upd) {
-    __shared__ KState sh[NR];
-    __shared__ double red[NR][2];
-    load_state<NR>(o, s, sh);
-    bool rep[NR];
-    bool any_rep = false, any_act = false;
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-        rep[r] = first || sh[r].phase == PH_RESTART;
-        any_rep |= rep[r];
-        any_act |= sh[r].phase == PH_ACTIVE;
-    }
-    if (any_rep || any_act) {
-        const bool mx[2] = {(bool)o.norm_inf, false};
-        double tot[NR][2];
-        reduce_prev<NR, 2, RB>(o, s, any_rep ? nb_res : nb_upd, mx, tot);
-        if (threadIdx.x == 0) {
-            for (int r = 0; r < NR; ++r) {
-                red[r][0] = tot[r][0];
-                red[r][1] = tot[r][1];
-                if (!rep[r]) continue;
-                KState &S = sh[r];
-                const double nr = nrm_fin(tot[r][0], o.norm_inf);
-                if (first) {
-                    S.norm0 = nr;
-                    S.thresh = fmax(o.atol, o.rtol * nr);
-                    S.iter = 0;
-                    S.restarts = 0;
-                    S.converged = 0;
-                    S.rz_old = tot[r][1];
-                    S.pcur = 0;
-                }
-                S.final_res = nr;
-                if (nr < S.thresh) {
-                    S.converged = 1;
-                    S.phase = PH_DONE;
-                } else if (S.iter >= o.max_iter) {
-                    S.phase = PH_DONE;
-                } else {
-                    S.scale = nr;  // true residual at this refinement
-                    S.fresh = first ? 1 : 2;
-                    S.phase = PH_ACTIVE;
-                }
-            }
-        }
-        lds_sync();
-        if (blockIdx.x == 0 && threadIdx.x < NR * 2) {
-            double *P = o.partials + (size_t)(s & 1) * kMaxRhs * kNQ * o.pstride;
-            const int r = threadIdx.x / 2, q = threadIdx.x % 2;
-            P[((size_t)r * kNQ + q) * o.pstride] = red[r][q];
-        }
-    }
-    store_state<NR>(o, s, sh);
+    store_state<1>(o, s, sh);
 }
 
 // ---------------------------------------------------------------- BiCGSTAB (FP64)
@@ -1568,10 +1911,10 @@ struct WS {
 
 struct Layout {
     double *val64, *dg64, *inv64, *b, *x, *r, *z, *p, *p1, *ap, *y, *s, *t, *rh, *v;
-    float *val32, *dg32, *inv32, *x32, *r32, *z32, *p32, *p32b, *ap32;
+    float *val32, *dg32, *inv32, *e32, *x32, *r32, *r32b, *p32, *p32b, *ap32, *ap32b;
     double *bv64, *bm64, *ev64;  // brick values (plain PCG), null without bricks
     float *bv32, *bm32, *ev32;   // brick values (mixed PCG)
-    double *partials, *dgc;
+    double *partials;
     KState *st;
     size_t bytes;
 };
@@ -1606,13 +1949,14 @@ Layout layout(char *base, int64_t C, int64_t nnz, int nrhs, const gf_krylov_cfg 
         L.val32 = w.take<float>(nnz);
         L.dg32 = w.take<float>(C);
         L.inv32 = w.take<float>(C);
-        L.dgc = w.take<double>(C);
+        L.e32 = w.take<float>(C);
         L.x32 = w.take<float>(n);
         L.r32 = w.take<float>(n);
-        L.z32 = w.take<float>(n);
+        L.r32b = w.take<float>(n);
         L.p32 = w.take<float>(n);
         L.p32b = w.take<float>(n);
         L.ap32 = w.take<float>(n);
+        L.ap32b = w.take<float>(n);
     } else {
         L.z = w.take<double>(n);
         L.p = w.take<double>(n);
@@ -1638,7 +1982,7 @@ Layout layout(char *base, int64_t C, int64_t nnz, int nrhs, const gf_krylov_cfg 
             L.ev64 = w.take<double>(ne);
         }
     }
-    L.partials = w.take<double>((size_t)2 * kMaxRhs * kNQ * partial_stride((int)C));
+    L.partials = w.take<double>((size_t)2 * kMaxRhs * kNQ * solve_pstride((int)C, bp.on ? bp.nbs : 0));
     L.st = w.take<KState>(2 * kMaxRhs);
     L.bytes = w.off + 256;
     return L;
@@ -1806,26 +2150,32 @@ void enqueue_cg_chunk(const Ops &o, const Layout &L, bool mixed, int every, int 
     const dim3 GS(o.gs), GD(o.gd), BS(TB), GU(o.gu), BU(RB), G1(1);
     Ops ol = o;  // the chunk's last launch posts the poll word
     ol.poll = poll;
-    for (int j = 0; j < every; ++j) {
-        const int nb_prev = j == 0 ? first_prev : o.gu;
-        const int pb = j & 1;  // buffer of p_old; `every` is even
-        if (!mixed) {
+    if (!mixed) {
+        for (int j = 0; j < every; ++j) {
+            const int nb_prev = j == 0 ? first_prev : o.gu;
+            const int pb = j & 1;  // buffer of p_old; `every` is even
             hipLaunchKernelGGL((k_cg_dir<double, NR, false, TB>), GD, BS, 0, st, o, s++, nb_prev, 0, pb, L.val64,
                                L.dg64, L.z, L.p, L.p1, L.ap, L.bv64, L.bm64, L.ev64);
             hipLaunchKernelGGL((k_cg_upd<double, NR>), GU, BU, 0, st, j == every - 1 ? ol : o, s++, pb ^ 1, L.inv64,
                                L.x, L.r, L.z, L.p, L.p1, L.ap);
-        } else {
-            hipLaunchKernelGGL((k_cg_dir<float, NR, true, TB>), GD, BS, 0, st, o, s++, nb_prev, j == every - 1, pb,
-                               L.val32, L.dgc, L.z32, L.p32, L.p32b, L.ap32, L.bv32, L.bm32, L.ev32);
-            hipLaunchKernelGGL((k_cg_upd<float, NR>), GU, BU, 0, st, o, s++, pb ^ 1, L.inv32, L.x32, L.r32, L.z32,
-                               L.p32, L.p32b, L.ap32);
         }
+        return;
     }
-    if (mixed) {  // FP64 refinement (no-ops unless a component asked for it)
-        hipLaunchKernelGGL(k_mx_accumulate<NR>, GU, BU, 0, st, o, s++, L.x32, L.x, L.p32, L.p32b);
-        hipLaunchKernelGGL((k_mx_residual<NR, TB>), GS, BS, 0, st, o, s++, 0, L.val64, L.dg64, L.inv32, L.b, L.x,
-                           L.r32, L.z32, L.x32);
-        hipLaunchKernelGGL(k_mx_refine<NR>, G1, BU, 0, st, ol, s++, 0, o.gs, o.gu);
+    if constexpr (NR == 1) {
+        // one fused launch per iteration (buffers alternate by launch; `every`
+        // is even), then the FP64 refinement (no-ops unless it is due)
+        const FusedVec V{L.e32, L.inv32, L.x32, L.r32, L.r32b, L.ap32, L.ap32b, L.p32, L.p32b};
+        for (int j = 0; j < every; ++j)
+            if (fused_bricks() == 2)
+                hipLaunchKernelGGL((k_cg_fused<kFusedTB, 2>), GD, dim3(kFusedTB), 0, st, o, s++,
+                                   j == 0 ? first_prev : o.gd, j & 1, L.val32, V, L.bv32, L.bm32, L.ev32);
+            else
+                hipLaunchKernelGGL((k_cg_fused<kFusedTB, 1>), GD, dim3(kFusedTB), 0, st, o, s++,
+                                   j == 0 ? first_prev : o.gd, j & 1, L.val32, V, L.bv32, L.bm32, L.ev32);
+        hipLaunchKernelGGL(k_mxf_accumulate, GU, BU, 0, st, o, s++, o.gd, L.x32, L.x, L.p32, L.p32b);
+        hipLaunchKernelGGL((k_mx_residual<TB>), GS, BS, 0, st, o, s++, 0, L.val64, L.dg64, L.inv32, L.b, L.x, L.r32,
+                           L.x32);
+        hipLaunchKernelGGL(k_mxf_refine, G1, BU, 0, st, ol, s++, 0, o.gs, o.gd);
     }
 }
 
@@ -1893,12 +2243,12 @@ int run_cg(const void *ws, Ops o, const Layout &L, const gf_krylov_cfg &cfg, hip
         hipLaunchKernelGGL((k_cg_init<NR, TB>), GS, BS, 0, st, o, s++, L.val64, L.dg64, L.inv64, L.b, L.x, L.r, L.z);
         hipLaunchKernelGGL(k_cg_init_fin<NR>, G1, BU, 0, st, o, s++, o.gs);
     } else {
-        hipLaunchKernelGGL((k_mx_residual<NR, TB>), GS, BS, 0, st, o, s++, 1, L.val64, L.dg64, L.inv32, L.b, L.x, L.r32,
-                           L.z32, L.x32);
-        hipLaunchKernelGGL(k_mx_refine<NR>, G1, BU, 0, st, o, s++, 1, o.gs, o.gu);
+        hipLaunchKernelGGL((k_mx_residual<TB>), GS, BS, 0, st, o, s++, 1, L.val64, L.dg64, L.inv32, L.b, L.x, L.r32,
+                           L.x32);
+        hipLaunchKernelGGL(k_mxf_refine, G1, BU, 0, st, o, s++, 1, o.gs, o.gd);
     }
     GF_CHECK_LAUNCH();
-    const int per_chunk = 2 * every + (cfg.mixed ? 3 : 0);
+    const int per_chunk = cfg.mixed ? every + 3 : 2 * every;
     int k = 1;
     const long cap = (long)cfg.max_iter + 4L * every;
     for (;;) {
@@ -1995,8 +2345,8 @@ extern "C" int gf_krylov_solve(const gf_mesh *m, const gf_krylov_cfg *cfg, int32
     hipLaunchKernelGGL(k_diag_prep<double>, dim3(grid_for(C)), dim3(kBlock), 0, s, C, diag, perm, cfg->precond,
                        L.dg64, L.inv64);
     if (cfg->method == 0 && cfg->mixed) {
-        hipLaunchKernelGGL(k_mx_prep, dim3(grid_for(C)), dim3(kBlock), 0, s, C, sptr, L.val64, diag, perm, L.val32,
-                           L.dgc);
+        hipLaunchKernelGGL(k_mxf_prep, dim3(grid_for(C)), dim3(kBlock), 0, s, C, sptr, L.val64, diag, perm, L.val32,
+                           L.e32);
         hipLaunchKernelGGL(k_diag_prep<float>, dim3(grid_for(C)), dim3(kBlock), 0, s, C, diag, perm, cfg->precond,
                            L.dg32, L.inv32);
     }
@@ -2038,9 +2388,10 @@ extern "C" int gf_krylov_solve(const gf_mesh *m, const gf_krylov_cfg *cfg, int32
     o.nbs = spmv_workgroups(C);
     o.tb = pcg_tile();
     o.gs = pcg_spmv_workgroups(C);
-    o.gd = pcg_dir_workgroups(C, o.nbrick);
+    o.gd = (cfg->method == 0 && cfg->mixed) ? fused_workgroups()
+                                            : pcg_dir_workgroups(C, o.nbrick);
     o.gu = pcg_vector_workgroups(C);
-    o.pstride = partial_stride(C);
+    o.pstride = solve_pstride(C, o.nbrick);
     o.norm_inf = cfg->norm;
     o.max_iter = cfg->max_iter;
     o.max_restart = cfg->max_restart;
@@ -2097,12 +2448,14 @@ extern "C" int gf_krylov_release(const void *workspace) {
     return GF_OK;
 }
 
-// ---- benchmark of the direction kernel on its own -------------------------
-// The PCG direction/SpMV kernel in its steady state with beta = 0, so
-// p = z = x and Ap = A x, on the layout a symmetric one-rhs PCG solve uses
-// (bricks first when the mesh has them).  It also writes p, reads the
-// previous direction (zeros) and reduces the (empty) previous partials, as
-// inside the solver.
+// ---- benchmark of the dominant PCG kernel on its own -----------------------
+// FP64: the direction/SpMV kernel k_cg_dir in its steady state with beta =
+// 0, so p = z = x and Ap = A x.  FP32: the fused mixed-precision launch
+// k_cg_fused in its update state with alpha = 0, beta = 1, M^-1 = 1 and
+// p_old = 0, so r' = r = x, x' = x, p = z = x and Ap = A x, while every
+// operand of a solver launch is read and written.  Both run on the layout a
+// symmetric one-rhs PCG solve uses (bricks first when the mesh has them),
+// read the previous launch's partials and reduce them, as inside the solver.
 namespace gf {
 namespace {
 template <typename VT>
@@ -2121,6 +2474,30 @@ __global__ void k_bench_out(int C, const VT *__restrict__ ap, const int32_t *__r
     if (i >= C) return;
     y[perm ? perm[i] : i] = (double)ap[i];
 }
+__global__ void k_fill_f32(int C, float v, float *__restrict__ a, float *__restrict__ b, float *__restrict__ c) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= C) return;
+    a[i] = v;
+    b[i] = v;
+    c[i] = v;
+}
+// Update state of the fused launch for the benchmark: every launch reads
+// state[0] and one reduced partial set (g = 1, d = 1e300, m = n = 0, |r| = 1).
+__global__ void k_fused_bench_state(KState *st, int n, double *partials, int pstride) {
+    int i = threadIdx.x;
+    if (i < n) {
+        KState S{};
+        S.phase = PH_ACTIVE;
+        S.fresh = 0;
+        S.rz_old = 1.0;
+        S.thresh = -1.0;
+        st[i] = S;
+    }
+    if (i < kFQ) {
+        const double v[kFQ] = {1.0, 1.0, 1e300, 0.0, 0.0};
+        partials[(size_t)i * pstride] = v[i];
+    }
+}
 
 struct BenchLayout {
     Ops o;
@@ -2128,7 +2505,8 @@ struct BenchLayout {
     const int32_t *perm, *scode;
     int64_t nnz;
     void *val, *bv, *bm, *ev, *z, *p0, *p1, *ap;
-    double *dg;
+    double *dg, *val64;
+    float *e32, *inv32, *x32, *r1, *a1;
     size_t bytes;
 };
 BenchLayout bench_layout(const gf_mesh *m, int fp32, char *base) {
@@ -2156,20 +2534,29 @@ BenchLayout bench_layout(const gf_mesh *m, int fp32, char *base) {
     o.nbs = spmv_workgroups(C);
     o.tb = pcg_tile();
     o.gs = pcg_spmv_workgroups(C);
-    o.gd = pcg_dir_workgroups(C, o.nbrick);
+    o.gd = fp32 ? fused_workgroups() : pcg_dir_workgroups(C, o.nbrick);
     o.gu = pcg_vector_workgroups(C);
-    o.pstride = partial_stride(C);
+    o.pstride = solve_pstride(C, o.nbrick);
+    o.max_iter = 1 << 30;
     o.partials = w.take<double>((size_t)2 * kMaxRhs * kNQ * o.pstride);
     o.st = w.take<KState>(2 * kMaxRhs);
     B.val = w.take<char>(vb * B.nnz);
     B.bv = w.take<char>(vb * ((size_t)B.bp.nbs * 192 + 1));
     B.bm = w.take<char>(vb * ((size_t)B.bp.nbs * 48 + 1));
     B.ev = w.take<char>(vb * ((size_t)B.bp.ne * 64 + 1));
-    B.z = w.take<char>(vb * C);
+    B.z = w.take<char>(vb * C);  // FP32: r of buffer 0
     B.p0 = w.take<char>(vb * C);
     B.p1 = w.take<char>(vb * C);
-    B.ap = w.take<char>(vb * C);
+    B.ap = w.take<char>(vb * C);  // FP32: Ap of buffer 0
     B.dg = w.take<double>(C);
+    if (fp32) {
+        B.val64 = w.take<double>(B.nnz);
+        B.e32 = w.take<float>(C);
+        B.inv32 = w.take<float>(C);
+        B.x32 = w.take<float>(C);
+        B.r1 = w.take<float>(C);
+        B.a1 = w.take<float>(C);
+    }
     B.bytes = w.off + 256;
     return B;
 }
@@ -2193,18 +2580,44 @@ void bench_pack(const gf_mesh *m, const BenchLayout &B, const double *diag, cons
                        (double *)nullptr);
     hipLaunchKernelGGL(k_bench_in<VT>, dim3(grid_for(C)), dim3(kBlock), 0, s, C, x, B.perm, (VT *)B.z, (VT *)B.p0,
                        (VT *)B.p1);
-    hipLaunchKernelGGL(k_spmv_bench_state, dim3(1), dim3(64), 0, s, B.o.st, 2 * kMaxRhs);
+    if constexpr (sizeof(VT) == 4) {
+        hipLaunchKernelGGL(k_sell_pack<double>, dim3(grid_for(B.nnz)), dim3(kBlock), 0, s, B.nnz, B.scode, upper,
+                           lower, 0, B.val64);
+        hipLaunchKernelGGL(k_mxf_prep, dim3(grid_for(C)), dim3(kBlock), 0, s, C, B.o.sell_ptr, B.val64, diag, B.perm,
+                           (float *)B.val, B.e32);
+        hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(C)), dim3(kBlock), 0, s, C, 1.0f, B.inv32, B.inv32, B.inv32);
+        hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(C)), dim3(kBlock), 0, s, C, 0.0f, B.x32, (float *)B.ap, B.a1);
+        hipLaunchKernelGGL(k_fused_bench_state, dim3(1), dim3(64), 0, s, B.o.st, 2 * kMaxRhs, B.o.partials,
+                           B.o.pstride);
+    } else {
+        hipLaunchKernelGGL(k_spmv_bench_state, dim3(1), dim3(64), 0, s, B.o.st, 2 * kMaxRhs);
+    }
 }
-template <typename VT, bool MX>
+template <typename VT>
 void bench_run(const BenchLayout &B, int reps, hipStream_t s) {
     auto launch = [&](auto tbc) {
         constexpr int TB = decltype(tbc)::value;
         // every launch uses sequence number 1: it reads state[0] (never
         // written here) and writes state[1] and partials[1]
-        for (int k = 0; k < reps; ++k)
-            hipLaunchKernelGGL((k_cg_dir<VT, 1, MX, TB>), dim3(B.o.gd), dim3(TB), 0, s, B.o, 1, 0, 0, 0,
-                               (const VT *)B.val, (const double *)B.dg, (const VT *)B.z, (VT *)B.p0, (VT *)B.p1,
-                               (VT *)B.ap, (const VT *)B.bv, (const VT *)B.bm, (const VT *)B.ev);
+        for (int k = 0; k < reps; ++k) {
+            if constexpr (sizeof(VT) == 4) {
+                const FusedVec V{B.e32, B.inv32, B.x32, (float *)B.z, B.r1, (float *)B.ap, B.a1, (float *)B.p0,
+                                 (float *)B.p1};
+                if (fused_bricks() == 2)
+                    hipLaunchKernelGGL((k_cg_fused<kFusedTB, 2>), dim3(B.o.gd), dim3(kFusedTB), 0, s, B.o, 1, 1, 0,
+                                       (const float *)B.val, V, (const float *)B.bv, (const float *)B.bm,
+                                       (const float *)B.ev);
+                else
+                    hipLaunchKernelGGL((k_cg_fused<kFusedTB, 1>), dim3(B.o.gd), dim3(kFusedTB), 0, s, B.o, 1, 1, 0,
+                                       (const float *)B.val, V, (const float *)B.bv, (const float *)B.bm,
+                                       (const float *)B.ev);
+            } else {
+                hipLaunchKernelGGL((k_cg_dir<double, 1, false, TB>), dim3(B.o.gd), dim3(TB), 0, s, B.o, 1, 0, 0, 0,
+                                   (const double *)B.val, (const double *)B.dg, (const double *)B.z,
+                                   (double *)B.p0, (double *)B.p1, (double *)B.ap, (const double *)B.bv,
+                                   (const double *)B.bm, (const double *)B.ev);
+            }
+        }
     };
     if (B.o.tb == 1024)
         launch(std::integral_constant<int, 1024>{});
@@ -2237,15 +2650,15 @@ extern "C" int gf_pcg_dir_bench(const gf_mesh *m, int32_t phase, int32_t fp32, i
             bench_pack<double>(m, B, diag, upper, lower, x, s);
     } else if (phase == 1) {
         GF_REQUIRE(reps >= 1, "reps must be >= 1");
-        if (fp32)  // mixed-precision operator: FP32 values and vectors, FP64 diagonal and accumulation
-            bench_run<float, true>(B, reps, s);
+        if (fp32)  // the fused mixed-precision launch: FP32 values and vectors, FP64 accumulation
+            bench_run<float>(B, reps, s);
         else
-            bench_run<double, false>(B, reps, s);
+            bench_run<double>(B, reps, s);
     } else {
         GF_REQUIRE(y, "null argument");
-        if (fp32)
+        if (fp32)  // Ap of the launch: buffer 1 (launches read buffer 0)
             hipLaunchKernelGGL(k_bench_out<float>, dim3(grid_for(m->n_cells)), dim3(kBlock), 0, s, m->n_cells,
-                               (const float *)B.ap, B.perm, y);
+                               (const float *)B.a1, B.perm, y);
         else
             hipLaunchKernelGGL(k_bench_out<double>, dim3(grid_for(m->n_cells)), dim3(kBlock), 0, s, m->n_cells,
                                (const double *)B.ap, B.perm, y);
