@@ -295,7 +295,7 @@ def config1_measure(cpu: bool, cpu_iters: int = 100) -> dict:
     return out
 
 
-def serve_measure(n_requests: int, slots: int, budget: int = 30, seed: int = 0) -> dict:
+def serve_measure(n_requests: int, slots: int, budget: int = 30, seed: int = 0, lanes: int = 2) -> dict:
     """Serving stream of random steady requests (lid-driven cube on a base
     grid n^3, n log-uniform in [16, 96], 1-4 box obstacles with 0-2
     refinement levels around them, Re U(100, 1000); SIMPLE until U and p
@@ -323,7 +323,7 @@ def serve_measure(n_requests: int, slots: int, budget: int = 30, seed: int = 0) 
         pl["simulator"]["fvSolution"]["solvers"]["p"]["mixed_precision"] = False
         solve_case(pl)
     t_seq = time.perf_counter() - t0
-    server = BatchServer(max_cases=slots, large_cells=large, large_lanes=2, max_bytes=64 << 30)
+    server = BatchServer(max_cases=slots, large_cells=large, large_lanes=lanes, max_bytes=64 << 30)
     t0 = time.perf_counter()
     sat = server.run([(k, c, 0.0) for k, c in enumerate(cases)])
     t_sat = time.perf_counter() - t0
@@ -331,7 +331,7 @@ def serve_measure(n_requests: int, slots: int, budget: int = 30, seed: int = 0) 
     arr = np.cumsum(rng.exponential(1.0 / (0.8 * cap), size=n_requests))
     poi = server.run([(k, c, float(a)) for k, (c, a) in enumerate(zip(cases, arr))])
     lat = latency_stats(poi)
-    return {"requests": n_requests, "batch_slots": slots, "alone_lanes": 2, "large_root_cells": large,
+    return {"requests": n_requests, "batch_slots": slots, "alone_lanes": lanes, "large_root_cells": large,
             "requests_alone_lane": int(sum(base_cells(c) >= large for c in cases)),
             "simple_budget": budget, "converged": int(sum(r.converged for r in sat)),
             "one_at_a_time_plain_solves_per_s": round(n_requests / t_seq, 3),
@@ -354,11 +354,12 @@ def main() -> int:
     ap.add_argument("--no-config1", action="store_true", help="skip the 64x64 SIMPLE cavity measurement")
     ap.add_argument("--spmv-only", choices=("fp32", "fp64"), default=None,
                     help="only launch the SpMV kernel (for counter runs) and exit")
-    ap.add_argument("--spmv-mesh", choices=("cavity", "uniform256"), default="cavity",
+    ap.add_argument("--spmv-mesh", default="cavity",
                     help="mesh of --spmv-only: the cavity octree or the uniform 256^3 grid")
     ap.add_argument("--serve-requests", type=int, default=48, help="requests in the serving measurement (0: skip)")
     ap.add_argument("--serve-slots", type=int, default=16, help="cases per batch in the serving measurement")
     ap.add_argument("--serve-only", action="store_true", help="only run the serving measurement")
+    ap.add_argument("--serve-lanes", type=int, default=2, help="alone lanes for requests of >= 48^3 root cells")
     ap.add_argument("--inner-rtol", type=float, default=None,
                     help="refinement trigger of the mixed solve (default: the solver config's inner_rel_tolerance)")
     a = ap.parse_args()
@@ -372,14 +373,14 @@ def main() -> int:
 
     torch.cuda.set_device(0)
     if a.serve_only:
-        print(json.dumps({"serving": serve_measure(a.serve_requests, a.serve_slots)}), flush=True)
+        print(json.dumps({"serving": serve_measure(a.serve_requests, a.serve_slots, lanes=a.serve_lanes)}), flush=True)
         return 0
     if a.spmv_only:
         from gridfoam_amd.algorithms import create_case
         from gridfoam_amd.config import load_config
 
-        if a.spmv_mesh == "uniform256":
-            grid = uniform_grid(256)
+        if a.spmv_mesh.startswith("uniform"):
+            grid = uniform_grid(int(a.spmv_mesh[len("uniform"):]))
         else:
             grid, _ = create_case(load_config(_cfg(False)), device="cuda:0")
         r = spmv_roofline(grid, laplacian_operator(grid), fp32=a.spmv_only == "fp32", launches=20)
@@ -455,7 +456,7 @@ def main() -> int:
         line["cavity2d_simple_64"] = config1_measure(cpu=not a.no_cpu)
     if a.serve_requests > 0:
         _log(f"serving: {a.serve_requests} requests, batches of {a.serve_slots}")
-        line["serving"] = serve_measure(a.serve_requests, a.serve_slots)
+        line["serving"] = serve_measure(a.serve_requests, a.serve_slots, lanes=a.serve_lanes)
     if rel is not None:
         line["mixed_vs_fp64_rel_l2"] = rel
         line["speedup_mixed_vs_fp64"] = round(out["plain"]["dt"] / out["mixed"]["dt"], 3)

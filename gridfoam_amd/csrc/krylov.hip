@@ -505,6 +505,8 @@ constexpr int kBricks = 2;
 // per unit (GRIDFOAM_FUSED_BRICKS = 1 or 2 overrides the default for tuning).
 constexpr int kFusedTB = 1024;
 constexpr int kFusedBricks = 1;
+// Bricks (or 64-row slices) whose operands fit in one XCD's 4 MB L2.
+constexpr int kL2Bricks = 2048;
 int fused_bricks() {
     static int kb = [] {
         const char *e = getenv("GRIDFOAM_FUSED_BRICKS");
@@ -1269,11 +1271,10 @@ __device__ __forceinline__ void fused_slice(const Ops &o, int lane, const FusedT
     if (i < o.C) fused_out(o, T, i, id, x, po, rn, z, p, acc, V, rout, aout, pout, v);
 }
 
-// A persistent launch: workgroup t of G (XCD-tiled, one per CU by default)
-// owns a contiguous range of brick units (KB bricks each; bricks beyond the
-// last whole unit go through the sliced ELL) and a contiguous range of the
-// remaining sliced-ELL slices; wave w of it takes every (TB/64)-th unit,
-// then every (TB/64)-th slice.  The loads of a wave's first unit are issued
+// A persistent launch of G workgroups (one per CU by default, a multiple of
+// 8): the waves on one XCD share a contiguous eighth of the brick units (KB
+// bricks each; bricks beyond the last whole unit go through the sliced ELL)
+// and of the remaining sliced-ELL slices, interleaved wave by wave.  The loads of a wave's first unit are issued
 // before the solver state is read and stay in flight during the reduction
 // of the previous launch's G partial sets; partials are written once per
 // workgroup, so the next launch reduces G sets.
@@ -1286,15 +1287,32 @@ __global__ __launch_bounds__(TB) void k_cg_fused(Ops o, int s, int nb_prev, int 
     prefetch_prev(o, s, nb_prev, pre);
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int G = gridDim.x, t = xcd_tile(blockIdx.x, G);
-    const int U = o.nbrick / KB, pu = (U + G - 1) / G;
-    const int u0 = min(U, t * pu), u1 = min(U, u0 + pu);
-    const int nsl = (o.C + 63) >> 6, e0 = U * KB, pe = (nsl - e0 + G - 1) / G;
-    const int s0 = min(nsl, e0 + t * pe), s1 = min(nsl, s0 + pe);
+    // XCD x = blockIdx.x % 8 (the hardware's round-robin) owns a contiguous
+    // eighth of the units and of the slices.  The units go in chunks of NW
+    // consecutive ones to a workgroup, one per wave, so the waves of a CU
+    // gather from each other's bricks through its L1.  When the eighth's
+    // bricks fit in the XCD's 4 MB L2 (about 1.5 KB of operands per brick),
+    // each workgroup takes a contiguous run of chunks; otherwise chunks go
+    // round the XCD's workgroups, so all its waves sweep the range together
+    // and the neighbour bricks they gather were loaded into the L2 moments
+    // before.  Slices go round the XCD's workgroups first.
+    const int G = gridDim.x, xcd = blockIdx.x & 7, PX = G >> 3, m = blockIdx.x >> 3;
+    const int U = o.nbrick / KB, pu = (U + 7) >> 3;
+    const int u0 = min(U, xcd * pu), u1 = min(U, u0 + pu);
+    const int nch = (pu + NW - 1) / NW, cpw = (nch + PX - 1) / PX;
+    const bool fits = pu * KB <= kL2Bricks;
+    const int nsl = (o.C + 63) >> 6, e0 = U * KB, pe = (nsl - e0 + 7) >> 3;
+    const int s0 = min(nsl, e0 + xcd * pe), s1 = min(nsl, s0 + pe);
+    // j-th unit of this wave (u1 or beyond: none)
+    auto unit_of = [&](int j) {
+        if (fits && j >= cpw) return u1;
+        const int c = fits ? m * cpw + j : m + PX * j;
+        return u0 + c * NW + w;
+    };
     const float *rin = pb ? V.r1 : V.r0, *ain = pb ? V.a1 : V.a0, *pin = pb ? V.p1 : V.p0;
     float *rout = pb ? V.r0 : V.r1, *aout = pb ? V.a0 : V.a1, *pout = pb ? V.p0 : V.p1;
     FBrick Bk[KB];
-    int u = u0 + w;
+    int ju = 0, u = unit_of(0);
     if (u < u1) {
 #pragma unroll
         for (int k = 0; k < KB; ++k) fbrick_load(o, u * KB + k, lane, V, rin, ain, pin, bv, bm, ev, Bk[k]);
@@ -1315,13 +1333,16 @@ __global__ __launch_bounds__(TB) void k_cg_fused(Ops o, int s, int nb_prev, int 
     if (T.act) {
         if (u < u1) {
             fused_unit<KB>(o, lane, T, u * KB, Bk, V, rin, ain, pin, rout, aout, pout, ev, v);
-            for (u += NW; u < u1; u += NW) {
+            for (u = unit_of(++ju); u < u1; u = unit_of(++ju)) {
 #pragma unroll
                 for (int k = 0; k < KB; ++k) fbrick_load(o, u * KB + k, lane, V, rin, ain, pin, bv, bm, ev, Bk[k]);
                 fused_unit<KB>(o, lane, T, u * KB, Bk, V, rin, ain, pin, rout, aout, pout, ev, v);
             }
         }
-        for (int sl = s0 + w; sl < s1; sl += NW) fused_slice(o, lane, T, sl, val, V, rin, ain, pin, rout, aout, pout, v);
+        // slices are heavier than brick units: spread over the XCD's
+        // workgroups first (slice j goes to workgroup j % (G / 8))
+        for (int sl = s0 + m + PX * w; sl < s1; sl += PX * NW)
+            fused_slice(o, lane, T, sl, val, V, rin, ain, pin, rout, aout, pout, v);
     }
     write_partials<1, kFQ, TB>(o, s, v, mx);
     state_out<1>(o, s, S);
