@@ -532,9 +532,11 @@ int pcg_dir_workgroups(int C, int nbrick) {
     const int nsl = (C + 63) / 64, nbw = brick_waves(nbrick, pcg_tile()), per = pcg_tile() / 64;
     return brick_workgroups(nbrick, pcg_tile()) + round8((nsl - nbw * kBricks + per - 1) / per);
 }
-// Workgroups of the persistent fused launch: one per CU (GRIDFOAM_FUSED_WG
-// overrides it for tuning), a multiple of 8.
-int fused_workgroups() {
+// Workgroups of the persistent fused launch, a multiple of 8: at most one per
+// CU (GRIDFOAM_FUSED_WG overrides the cap for tuning), and about two chunks
+// of 16 units each (a 64-row slice counted as two units) per workgroup, so a
+// small grid leaves CUs to solves that run concurrently on other streams.
+int fused_workgroup_cap() {
     static int g = [] {
         const char *e = getenv("GRIDFOAM_FUSED_WG");
         int v = e ? atoi(e) : 0;
@@ -550,10 +552,17 @@ int fused_workgroups() {
     }();
     return g;
 }
+int fused_workgroups(int C, int nbrick) {
+    const int nsl = (C + 63) / 64;
+    const long work = (long)nbrick + 2L * (nsl - nbrick);
+    const long want = (work + 2L * (kFusedTB / 64) - 1) / (2L * (kFusedTB / 64));
+    const int cap = fused_workgroup_cap();
+    return want >= cap ? cap : round8((int)(want > 8 ? want : 8));
+}
 // Partials per (rhs, quantity) of a solve: the largest workgroup count of any of its kernels.
 int solve_pstride(int C, int nbrick) {
     int m = partial_stride(C);
-    for (int v : {pcg_dir_workgroups(C, nbrick), fused_workgroups()}) m = v > m ? v : m;
+    for (int v : {pcg_dir_workgroups(C, nbrick), fused_workgroups(C, nbrick)}) m = v > m ? v : m;
     return m;
 }
 
@@ -1192,6 +1201,90 @@ __device__ __forceinline__ void fused_out(const Ops &o, const FusedT &T, int i, 
     v[0][FQ_N] += ad * (double)__fmul_rn(id, a);
 }
 
+// ---- fast fixed-order reductions of the fused launch -----------------------
+// DPP moves a value across lanes of a row of 16 without the LDS crossbar;
+// 64-bit values move as two 32-bit halves.
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane_d(double v, int l) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                            __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+__device__ __forceinline__ double red_op(double a, double b, bool mx) { return mx ? fmax(a, b) : a + b; }
+// Sum (or max) over the 64 lanes of a wave: pairs, quads, halves and rows by
+// symmetric permutes (every lane of a row then holds the same row total),
+// then the four rows in a fixed order.  Every lane gets the same result.
+__device__ __forceinline__ double wave_total(double v, bool mx) {
+    v = red_op(v, dpp_d<0xB1>(v), mx);   // quad_perm [1,0,3,2]
+    v = red_op(v, dpp_d<0x4E>(v), mx);   // quad_perm [2,3,0,1]
+    v = red_op(v, dpp_d<0x141>(v), mx);  // row_half_mirror
+    v = red_op(v, dpp_d<0x140>(v), mx);  // row_mirror
+    return red_op(red_op(lane_d(v, 0), lane_d(v, 16), mx), red_op(lane_d(v, 32), lane_d(v, 48), mx), mx);
+}
+
+// The previous launch's kFQ partial sets reduced in every thread: the first
+// share was prefetched (one set per thread), later shares are added in
+// thread order, then wave totals, then the waves' totals in wave order.
+template <int TB>
+__device__ __forceinline__ void fused_reduce_prev(const Ops &o, int s, int nb_prev,
+                                                  const Prefetched<1, kFQ, 1, TB> &pf, double (&tot)[1][kFQ]) {
+    constexpr int NW = TB / 64;
+    __shared__ double lds[kFQ][NW];
+    const double *P = o.partials + (size_t)((s - 1) & 1) * kMaxRhs * kNQ * o.pstride;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    double a[kFQ];
+#pragma unroll
+    for (int q = 0; q < kFQ; ++q) a[q] = pf.x[0][0][q];
+    for (int j = threadIdx.x + TB; j < nb_prev; j += TB)
+#pragma unroll
+        for (int q = 0; q < kFQ; ++q) a[q] = red_op(a[q], P[(size_t)q * o.pstride + j], q == FQ_NR && o.norm_inf);
+#pragma unroll
+    for (int q = 0; q < kFQ; ++q) {
+        const double t = wave_total(a[q], q == FQ_NR && o.norm_inf);
+        if (lane == 0) lds[q][wid] = t;
+    }
+    lds_sync();
+#pragma unroll
+    for (int q = 0; q < kFQ; ++q) {
+        double w[NW];
+#pragma unroll
+        for (int k = 0; k < NW; ++k) w[k] = lds[q][k];
+        double t = w[0];
+#pragma unroll
+        for (int k = 1; k < NW; ++k) t = red_op(t, w[k], q == FQ_NR && o.norm_inf);
+        tot[0][q] = t;
+    }
+}
+
+// This workgroup's kFQ partials: wave totals, then the waves' totals in wave order.
+template <int TB>
+__device__ __forceinline__ void fused_write_partials(const Ops &o, int s, const double (&v)[1][kFQ]) {
+    constexpr int NW = TB / 64;
+    __shared__ double lds[kFQ][NW];
+    double *P = o.partials + (size_t)(s & 1) * kMaxRhs * kNQ * o.pstride;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < kFQ; ++q) {
+        const double t = wave_total(v[0][q], q == FQ_NR && o.norm_inf);
+        if (lane == 0) lds[q][wid] = t;
+    }
+    lds_sync();
+    if (threadIdx.x < kFQ) {
+        const int q = threadIdx.x;
+        double w[NW];
+#pragma unroll
+        for (int k = 0; k < NW; ++k) w[k] = lds[q][k];
+        double t = w[0];
+#pragma unroll
+        for (int k = 1; k < NW; ++k) t = red_op(t, w[k], q == FQ_NR && o.norm_inf);
+        P[(size_t)q * o.pstride + blockIdx.x] = t;
+    }
+}
+
 // One unit of KB bricks of a fused launch whose loads were issued into Bk.
 template <int KB>
 __device__ __forceinline__ void fused_unit(const Ops &o, int lane, const FusedT &T, int sl, const FBrick (&Bk)[KB],
@@ -1324,9 +1417,8 @@ __global__ __launch_bounds__(TB) void k_cg_fused(Ops o, int s, int nb_prev, int 
         state_out<1>(o, s, S);
         return;
     }
-    const bool mx[kFQ] = {(bool)o.norm_inf, false, false, false, false};
     double tot[1][kFQ];
-    reduce_prev<1, kFQ, TB, 1>(o, s, nb_prev, mx, tot, &pre);
+    fused_reduce_prev<TB>(o, s, nb_prev, pre, tot);
     FusedT T;
     fused_transition(o, pb, tot, S[0], T);
     double v[1][kFQ] = {};
@@ -1344,7 +1436,7 @@ __global__ __launch_bounds__(TB) void k_cg_fused(Ops o, int s, int nb_prev, int 
         for (int sl = s0 + m + PX * w; sl < s1; sl += PX * NW)
             fused_slice(o, lane, T, sl, val, V, rin, ain, pin, rout, aout, pout, v);
     }
-    write_partials<1, kFQ, TB>(o, s, v, mx);
+    fused_write_partials<TB>(o, s, v);
     state_out<1>(o, s, S);
 }
 
@@ -2409,7 +2501,7 @@ extern "C" int gf_krylov_solve(const gf_mesh *m, const gf_krylov_cfg *cfg, int32
     o.nbs = spmv_workgroups(C);
     o.tb = pcg_tile();
     o.gs = pcg_spmv_workgroups(C);
-    o.gd = (cfg->method == 0 && cfg->mixed) ? fused_workgroups()
+    o.gd = (cfg->method == 0 && cfg->mixed) ? fused_workgroups(C, o.nbrick)
                                             : pcg_dir_workgroups(C, o.nbrick);
     o.gu = pcg_vector_workgroups(C);
     o.pstride = solve_pstride(C, o.nbrick);
@@ -2555,7 +2647,7 @@ BenchLayout bench_layout(const gf_mesh *m, int fp32, char *base) {
     o.nbs = spmv_workgroups(C);
     o.tb = pcg_tile();
     o.gs = pcg_spmv_workgroups(C);
-    o.gd = fp32 ? fused_workgroups() : pcg_dir_workgroups(C, o.nbrick);
+    o.gd = fp32 ? fused_workgroups(C, o.nbrick) : pcg_dir_workgroups(C, o.nbrick);
     o.gu = pcg_vector_workgroups(C);
     o.pstride = solve_pstride(C, o.nbrick);
     o.max_iter = 1 << 30;
