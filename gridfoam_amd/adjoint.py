@@ -139,9 +139,14 @@ class TransientStep:
     recomputed step sees what the original saw.  Those conditions' own
     values (inletOutlet's switch, fixedFluxPressure's gradient) are
     re-evaluated without a graph, so gradients do not run through them.
-    PIMPLE runs every outer corrector (no residual-control exit).  Not
-    covered: SIMPLEC (``consistent``), backward (BDF2) time derivatives, TVD
-    convection schemes and snapped immersed cells; those raise."""
+    PIMPLE runs every outer corrector (no residual-control exit).  Velocity
+    cells fixed by immersed snapping (Gibou constraints, fv/boundary_ops.py:
+    440-535) are eliminated from the momentum solve by the differentiable
+    ``with_fixed_values`` and get their prescribed values back after the
+    velocity correction (pressure_correction.py:338-363).  Not covered:
+    SIMPLEC (``consistent``), backward (BDF2) time derivatives, TVD
+    convection schemes and snapped pressure cells (a pressure condition that
+    prescribes values on an immersed body); those raise."""
 
     def __init__(self, alg, *, nu: Union[float, torch.Tensor, None] = None):
         from .algorithms import PIMPLE, PISO
@@ -169,7 +174,19 @@ class TransientStep:
             from .immersed import immersed_dirichlet_constraints
 
             if immersed_dirichlet_constraints(field)[0].numel():
-                raise NotImplementedError("snapped immersed cells are not differentiable here")
+                raise NotImplementedError("snapped immersed pressure cells are not differentiable here")
+
+    def _snapped(self, Udata: torch.Tensor) -> torch.Tensor:
+        """The prescribed values of snapped velocity cells written back
+        (boundary_ops.py:537-541) with a graph-carrying index_copy."""
+        if self.grid.num_immersed_faces == 0:
+            return Udata
+        from .immersed import immersed_dirichlet_constraints
+
+        cells, vals = immersed_dirichlet_constraints(self.alg.U)
+        if cells.numel() == 0:
+            return Udata
+        return Udata.index_copy(0, cells, vals.to(Udata.dtype))
 
     def _bnd_values(self, field) -> torch.Tensor:
         from .fv import boundary_values
@@ -291,10 +308,10 @@ class TransientStep:
         phi_old = phi_s
         for outer in range(self.n_outer):
             _set_face(alg.phi, phi_s, phi_b)
-            self._check_unconstrained(U)
             UEqn = fvm.assemble(U, ddt=1.0, div=1.0, lap=-1.0, phi=alg.phi, gamma=self.nu, abc=(1.0, 1.0, 0.0))
             pred = UEqn.with_source(UEqn.source - V[:, None] * fvc.grad(p))
             U.data = alg.solvers["U"].solve(Equation(U, pred)).solution
+            UEqn = Equation(U, UEqn).fv_matrix  # piso.py:257 / pimple.py:346: snapped cells eliminated
             for i in range(self.n_corr):
                 final = outer == self.n_outer - 1 and i == self.n_corr - 1
                 rAU = 1.0 / UEqn.A()
@@ -303,7 +320,7 @@ class TransientStep:
                 pmat = self._pressure(rAU, ph_s, ph_b, final)
                 phi_s, phi_b = self._correct_phi(ph_s, ph_b, pmat, rAU)
                 _set_face(alg.phi, phi_s, phi_b)
-                U.data = HbyA - rAU[:, None] * fvc.grad(p)
+                U.data = self._snapped(HbyA - rAU[:, None] * fvc.grad(p))
         return U.data, phi_s, phi_b, p.data, ph_s, ph_b, rAU
 
 

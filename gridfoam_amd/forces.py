@@ -114,9 +114,11 @@ class SideForces:
 def _boundary_values(field, B0: int, sel: torch.Tensor) -> torch.Tensor:
     """sampling.py:28-65: f ref_v + (1 - f)(psi_P + |d| ref_g) on the
     selected boundary faces, the cell value where no condition owns them
-    (the boundary state holds exactly that)."""
-    st = field.boundary_state()
-    return st.psi_b[B0 + sel].reshape((sel.numel(), *field.component_shape))
+    (the boundary state holds exactly that); autograd-connected to the
+    field and its condition values when they require grad."""
+    from .fv import boundary_values
+
+    return boundary_values(field)[B0 + sel].reshape((sel.numel(), *field.component_shape))
 
 
 def integrate_patch(grid, *, p, U, nu_eff, patch: str, rho: float, center_of_rotation: torch.Tensor,
@@ -138,7 +140,10 @@ def integrate_patch(grid, *, p, U, nu_eff, patch: str, rho: float, center_of_rot
     nbr_i = grid.bnd_owner[Bd + Fi:Bd + 2 * Fi]
     Sf_i = grid.bnd_Sf[Bd:Bd + Fi]
     assert int(imm.sum()) == Fi and F_all >= Fi
-    nu = nu_eff if torch.is_tensor(nu_eff) else torch.full((grid.num_cells,), float(nu_eff), **f64)
+    if torch.is_tensor(nu_eff):
+        nu = nu_eff.expand(grid.num_cells) if nu_eff.dim() == 0 else nu_eff
+    else:
+        nu = torch.full((grid.num_cells,), float(nu_eff), **f64)
     sides = ((True, up_mask, own_i, Sf_i, grid.ap_dist_owner_to_bnd, grid.ap_owner_normal, Bd),
              (False, lo_mask, nbr_i, -Sf_i, grid.ap_dist_neighbour_to_bnd, grid.ap_neighbour_normal, Bd + Fi))
     for upper, mask, cells_all, Sf_all, dist_all, nout_all, B0 in sides:

@@ -1,0 +1,114 @@
+"""GPU: the adjoint of the 2-D channel around the Fourier body at full size
+(205,600 cells, 2,144 immersed faces, PIMPLE with 2 outer and 2 inner
+correctors, FP64).
+
+``TransientStep`` must reproduce the PIMPLE driver on this grid.  The drag
+coefficient of the body (post/forces/evaluator.py:22-180) after 20
+steps, run by ``checkpointed_rollout`` with a checkpoint every 10 steps, has
+gradients with respect to the viscosity and the inlet speed that must match
+central differences to 1e-5 (relative)."""
+
+from __future__ import annotations
+
+import copy
+import sys
+
+import pytest
+import torch
+
+from tests.cases import case_dict
+
+pytestmark = pytest.mark.gpu
+
+FORCE = {"patches": ["body"], "rho": 1.0, "magU_ref": 1.0, "A_ref": 1.0, "L_ref": 1.0,
+         "local_coord": {"drag_dir": [1.0, 0.0, 0.0], "lift_dir": [0.0, 1.0, 0.0],
+                         "center_of_rotation": [5.0, 4.0, 0.25]}}
+STEPS, EVERY, WARM = 20, 10, 5
+
+
+def _say(msg: str) -> None:
+    print(f"[gpu-case] {msg}", file=sys.stderr, flush=True)
+
+
+def _config(tight: bool) -> dict:
+    cfg = copy.deepcopy(case_dict("channel2d_fourier_pimple.yaml"))
+    if tight:  # central differences need solves far below the step h = 1e-6
+        sol = cfg["simulator"]["fvSolution"]["solvers"]
+        sol["U"].update({"tolerance": 0.0, "rel_tolerance": 1e-13, "max_iter": 2000})
+        sol["p"].update({"tolerance": 0.0, "rel_tolerance": 1e-13, "max_iter": 8000})
+    return cfg
+
+
+def _case(tight: bool, warm: int = 0):
+    from gridfoam_amd.algorithms import create_case
+    from gridfoam_amd.config import load_config
+
+    grid, alg = create_case(load_config(_config(tight)))
+    for _ in range(warm):
+        alg.step()
+    return grid, alg
+
+
+def test_transient_step_reproduces_the_driver_around_the_fourier_body():
+    from gridfoam_amd.adjoint import TransientStep, transient_state
+
+    _say("Fourier channel: TransientStep vs the PIMPLE driver, 2 steps after 3")
+    grid, alg = _case(False, warm=3)
+    assert grid.num_cells == 205600 and grid.num_immersed_faces == 2144
+    state = transient_state(alg)
+    step = TransientStep(alg)
+    with torch.no_grad():
+        s = state
+        for _ in range(2):
+            s = step(s)
+    grid2, alg2 = _case(False, warm=5)
+    ref = (alg2.U.data, alg2.phi.single_data, alg2.phi.bnd_data, alg2.p.data)
+    for a, b in zip(s, ref):
+        den = max(float(torch.linalg.vector_norm(b)), 1e-300)
+        assert float(torch.linalg.vector_norm(a - b)) / den < 1e-10
+
+
+def test_drag_gradient_through_checkpointed_pimple_steps_matches_central_differences():
+    from gridfoam_amd.adjoint import TransientStep, checkpointed_rollout, transient_state
+    from gridfoam_amd.config import ForceCoeffConfig
+    from gridfoam_amd.forces import ForceEvaluator
+
+    _say(f"Fourier channel adjoint: drag after {STEPS} PIMPLE steps, checkpoints every {EVERY}")
+    grid, alg = _case(True, warm=WARM)
+    state0 = transient_state(alg)
+    ev = ForceEvaluator(grid, config=ForceCoeffConfig.model_validate(copy.deepcopy(FORCE)))
+    bc = alg.U.bcs["x_minus"]  # the inlet
+    v_base = bc.value.detach().clone()
+    nu_base = float(alg.nu)
+
+    def run(nu, speed, grad):
+        bc.value = v_base * speed
+        alg.U._state = None
+        step = TransientStep(alg, nu=nu)
+        s = tuple(t.clone() for t in state0)
+        with torch.set_grad_enabled(grad):
+            out = checkpointed_rollout(step, s, STEPS, every=EVERY)
+            alg.U.data, alg.p.data = out[0], out[3]
+            alg.U._state = None
+            alg.p._state = None
+            return ev.evaluate(turbulence=nu).Cd
+
+    nu = torch.tensor(nu_base, dtype=torch.float64, device=grid.device, requires_grad=True)
+    speed = torch.tensor(1.0, dtype=torch.float64, requires_grad=True)  # scales the condition's (host) value
+    cd = run(nu, speed, True)
+    g_nu, g_speed = torch.autograd.grad(cd, [nu, speed])
+    _say(f"Cd = {float(cd):.10f}; adjoint dCd/dnu {float(g_nu):.8e}, dCd/dU_in {float(g_speed):.8e}")
+
+    def fd(f, h):
+        return (float(f(h)) - float(f(-h))) / (2 * h)
+
+    with torch.no_grad():
+        f64 = dict(dtype=torch.float64, device=grid.device)
+        one = lambda h: torch.tensor(1.0 + h, dtype=torch.float64)  # noqa: E731
+        fd_nu = fd(lambda h: run(torch.tensor(nu_base + h, **f64), one(0.0), False), 1e-6 * nu_base)
+        fd_speed = fd(lambda h: run(torch.tensor(nu_base, **f64), one(h), False), 1e-6)
+    _say(f"central differences dCd/dnu {fd_nu:.8e}, dCd/dU_in {fd_speed:.8e}")
+    bc.value = v_base
+    alg.U._state = None
+    for an, ref in ((float(g_nu), fd_nu), (float(g_speed), fd_speed)):
+        assert abs(an - ref) <= 1e-5 * abs(ref), (an, ref)

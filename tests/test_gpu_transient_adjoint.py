@@ -6,7 +6,8 @@ boundary / divergence VJP kernels, tensor arithmetic for the rest).  Its
 forward must reproduce the driver's own step, and the gradient of an
 objective of the state after several checkpointed steps must match central
 finite differences to 1e-5 (relative), for the viscosity, a Dirichlet
-value (the lid speed) and the initial velocity field."""
+value (the lid speed) and the initial velocity field.  Two cases carry
+immersed bodies whose Gibou snapping fixes velocity cells."""
 
 from __future__ import annotations
 
@@ -17,7 +18,7 @@ import numpy as np
 import pytest
 import torch
 
-from tests.cases import channel_flow, small_cavity2d, small_cavity3d
+from tests.cases import box_cavity, channel_flow, fourier_channel, small_cavity2d, small_cavity3d
 
 pytestmark = pytest.mark.gpu
 
@@ -50,7 +51,14 @@ CASES = {
     "cavity2d_pimple_upwind": lambda: _transient(small_cavity2d(8), "pimple", "upwind"),
     "octree_piso_linear": lambda: _transient(small_cavity3d(root=4, steps=3), "piso", "linear"),
     "channel_piso_upwind": lambda: _transient(channel_flow("piso", p_outlet="dirichlet"), "piso", "upwind"),
+    # immersed bodies whose Gibou snapping fixes velocity cells (8 and 2 of them)
+    "box2d_piso_snapped": lambda: _transient(box_cavity(16, algorithm="piso"), "piso", "upwind"),
+    "fourier_pimple_snapped": lambda: _transient(fourier_channel(root=(40, 16, 1), level=1), "pimple", "upwind"),
 }
+# The coarse Fourier channel from a random velocity field is not a stable
+# flow (its objective grows by 1e8 over four steps), so it is checked for
+# the forward step only.
+GRADIENT_CASES = [k for k in CASES if k != "fourier_pimple_snapped"]
 
 
 def _setup(name, warm: int = 0):
@@ -96,11 +104,34 @@ def test_transient_step_reproduces_the_driver(name):
         assert float(torch.linalg.vector_norm(a - b)) / den < 1e-10
 
 
+def _fluid(grid) -> torch.Tensor:
+    """Cells connected to cell 0 (a corner of the fluid) through coupled
+    internal faces.  Immersed faces couple nothing, so the cells inside a
+    body, or a cell whose every face is cut, form regions of their own with
+    only zero-gradient pressure conditions: their pressure is fixed up to a
+    constant by nothing but the solver's iteration, so an objective must not
+    depend on it."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+
+    o = grid.owner.detach().cpu().numpy()
+    n = grid.neighbour.detach().cpu().numpy()
+    C = grid.num_cells
+    adj = coo_matrix((np.ones(o.size), (o, n)), shape=(C, C))
+    _, label = connected_components(adj, directed=False)
+    return torch.as_tensor(label == label[0], device=grid.device)
+
+
 def _objective(grid, seed=9):
     gen = torch.Generator().manual_seed(seed)
     wU = torch.randn((grid.num_cells, 3), dtype=torch.float64, generator=gen).to(grid.device)
     wp = torch.randn(grid.num_cells, dtype=torch.float64, generator=gen).to(grid.device)
     wf = torch.randn(grid.num_internal_faces, dtype=torch.float64, generator=gen).to(grid.device)
+    if grid.num_immersed_faces:  # fluid cells and the faces between two of them only
+        fl = _fluid(grid)
+        wU = wU * fl[:, None]
+        wp = wp * fl
+        wf = wf * (fl[grid.owner] & fl[grid.neighbour])
 
     def J(state):
         U, phi_s, _, p = state[:4]
@@ -108,7 +139,7 @@ def _objective(grid, seed=9):
     return J
 
 
-@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("name", GRADIENT_CASES)
 def test_gradients_through_checkpointed_steps_match_central_differences(name):
     """dJ/d(nu), dJ/d(lid or inlet value) and dJ/dU0 . v through 4 steps,
     checkpointed every 2, against central differences (1e-5 relative)."""
