@@ -53,8 +53,6 @@ def manual_step(config_path: Union[str, Path, dict], device: str = "cuda") -> It
     the caller calls ``algorithm.step()``.  Diagnostics record each step."""
     grid = manual_run(config_path, device)
     sim = grid.sim_config
-    if sim.post_processing is not None and sim.post_processing.forceCoeff is not None:
-        raise ValueError("force coefficients need immersed bodies, which this build does not construct")
     algorithm = create_algorithm(grid)
     output_dir, _ = _output(sim.control)
     output_dir.mkdir(parents=True, exist_ok=True)
@@ -67,6 +65,15 @@ def manual_step(config_path: Union[str, Path, dict], device: str = "cuda") -> It
     try:
         for step in range(1, n_steps + 1):
             yield StepData(step=step, control=sim.control, algorithm=algorithm, ugrid=ugrid)
+        # runner.py:132-143: force coefficients of the body patches at the end
+        # of a run that went through every step (coefficients.csv; the
+        # surface-mesh VTU of the sampled forces is not written)
+        if sim.post_processing is not None and sim.post_processing.forceCoeff is not None:
+            from .forces import ForceEvaluator  # noqa: PLC0415
+
+            ev = ForceEvaluator(grid)
+            ev.evaluate(grid, time=float(sim.control.endTime))
+            ev.write_csv(output_dir / "coefficients.csv")
     finally:
         if diagnostics is not None:
             diagnostics.close()

@@ -157,3 +157,34 @@ def test_cube_low_re_drag_with_simple(tmp_path):
     ev.write_csv(out)
     rows = out.read_text().splitlines()
     assert rows[0].split(",") == CSV_HEADER and len(rows) == 2
+
+
+def test_runner_writes_force_coefficients_at_the_end_of_the_run(tmp_path):
+    """runner.py:132-143: with post_processing.forceCoeff the runner evaluates
+    the body's coefficients after the last step into coefficients.csv; they
+    equal the evaluator's on the same run driven step by step."""
+    from gridfoam_amd.algorithms import create_case
+    from gridfoam_amd.config import load_config
+    from gridfoam_amd.forces import CSV_HEADER, ForceEvaluator
+    from gridfoam_amd.runner import all_run
+
+    cfg = fourier_channel(root=(40, 16, 1), level=1)
+    sim = cfg["simulator"]
+    sim["control"]["endTime"] = 3 * sim["control"]["deltaT"]
+    sim["control"]["writeInterval"] = 1000
+    sim["control"]["output"] = {"output_dir": str(tmp_path), "base_name": "channel"}
+    force = copy.deepcopy(FORCE)
+    force["local_coord"]["center_of_rotation"] = [5.0, 4.0, 0.25]
+    sim["post_processing"] = {"forceCoeff": force}
+    _say("runner: Fourier channel 3 PIMPLE steps with forceCoeff")
+    assert all_run(copy.deepcopy(cfg)) == 3
+    rows = (tmp_path / "coefficients.csv").read_text().splitlines()
+    assert rows[0].split(",") == CSV_HEADER and len(rows) == 2
+    grid, alg = create_case(load_config(cfg))
+    for _ in range(3):
+        alg.step()
+    co = ForceEvaluator(grid).evaluate(grid, time=float(sim["control"]["endTime"]))
+    vals = [float(v) for v in rows[1].split(",")]
+    assert vals[0] == pytest.approx(float(sim["control"]["endTime"]))
+    assert vals[1] == pytest.approx(float(co.Cd), rel=1e-12, abs=1e-14)
+    assert vals[4] == pytest.approx(float(co.Cl), rel=1e-12, abs=1e-14)
