@@ -152,7 +152,7 @@ def spmv_roofline(grid, pmat, fp32: bool, launches: int = 200) -> dict:
     achieved = B / avg_s / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": _traffic(fp32, C),
-            "kernel": f"k_cg_fused<1024, {os.environ.get('GRIDFOAM_FUSED_BRICKS', '1')}>" if fp32 else "k_cg_dir<double, 1, false>",
+            "kernel": f"k_cg_fused<{os.environ.get('GRIDFOAM_FUSED_TB', '1024')}, 1>" if fp32 else "k_cg_dir<double, 1, false>",
             "cells": C, "bricks": int(grid.mesh.n_bricks), "brick_cells_frac": round(64 * grid.mesh.n_bricks / C, 4),
             "bytes_per_launch": int(B), "avg_us": round(avg_s * 1e6, 2), "rel_err_vs_fp64": err}
 

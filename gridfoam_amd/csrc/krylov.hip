@@ -501,19 +501,19 @@ typedef const __attribute__((address_space(4))) int32_t ConstI32;
 // bricks, all of whose loads are issued before the solver state is read, so
 // the dependent neighbour-brick gathers of several bricks overlap.
 constexpr int kBricks = 2;
-// Threads per workgroup of the fused mixed-precision launch, and its bricks
-// per unit (GRIDFOAM_FUSED_BRICKS = 1 or 2 overrides the default for tuning).
-constexpr int kFusedTB = 1024;
+// Threads per workgroup of the fused mixed-precision launch: 1024 (one
+// workgroup per CU), or 512 / 256 with as many more workgroups
+// (GRIDFOAM_FUSED_TB overrides the default for tuning).  One brick per unit.
 constexpr int kFusedBricks = 1;
 // Bricks (or 64-row slices) whose operands fit in one XCD's 4 MB L2.
 constexpr int kL2Bricks = 2048;
-int fused_bricks() {
-    static int kb = [] {
-        const char *e = getenv("GRIDFOAM_FUSED_BRICKS");
-        const int v = e ? atoi(e) : kFusedBricks;
-        return v == 2 ? 2 : 1;
+int fused_tb() {
+    static int tb = [] {
+        const char *e = getenv("GRIDFOAM_FUSED_TB");
+        const int v = e ? atoi(e) : 1024;
+        return (v == 256 || v == 512) ? v : 1024;
     }();
-    return kb;
+    return tb;
 }
 
 // Waves that carry bricks: whole workgroups only (bricks left over go
@@ -547,15 +547,16 @@ int fused_workgroup_cap() {
                 v = n;
             else
                 v = 256;
+            v *= 1024 / fused_tb();
         }
         return round8(v);
     }();
     return g;
 }
 int fused_workgroups(int C, int nbrick) {
-    const int nsl = (C + 63) / 64;
+    const int nsl = (C + 63) / 64, nw = fused_tb() / 64;
     const long work = (long)nbrick + 2L * (nsl - nbrick);
-    const long want = (work + 2L * (kFusedTB / 64) - 1) / (2L * (kFusedTB / 64));
+    const long want = (work + 2L * nw - 1) / (2L * nw);
     const int cap = fused_workgroup_cap();
     return want >= cap ? cap : round8((int)(want > 8 ? want : 8));
 }
@@ -1440,6 +1441,21 @@ __global__ __launch_bounds__(TB) void k_cg_fused(Ops o, int s, int nb_prev, int 
     state_out<1>(o, s, S);
 }
 
+// The fused launch with the configured workgroup size.
+void launch_fused(dim3 grid, hipStream_t st, const Ops &o, int s, int nb_prev, int pb, const float *val,
+                  const FusedVec &V, const float *bv, const float *bm, const float *ev) {
+    const int tb = fused_tb();
+    if (tb == 256)
+        hipLaunchKernelGGL((k_cg_fused<256, kFusedBricks>), grid, dim3(256), 0, st, o, s, nb_prev, pb, val, V, bv,
+                           bm, ev);
+    else if (tb == 512)
+        hipLaunchKernelGGL((k_cg_fused<512, kFusedBricks>), grid, dim3(512), 0, st, o, s, nb_prev, pb, val, V, bv,
+                           bm, ev);
+    else
+        hipLaunchKernelGGL((k_cg_fused<1024, kFusedBricks>), grid, dim3(1024), 0, st, o, s, nb_prev, pb, val, V,
+                           bv, bm, ev);
+}
+
 // Refinement for the fused launches, part 1: the trigger (the last launch's
 // |r| below reliable * the last true residual, or a stop the launches
 // asked for), then x64 += x32 ; x32 = 0 ; the direction of the last update
@@ -2279,12 +2295,7 @@ void enqueue_cg_chunk(const Ops &o, const Layout &L, bool mixed, int every, int 
         // is even), then the FP64 refinement (no-ops unless it is due)
         const FusedVec V{L.e32, L.inv32, L.x32, L.r32, L.r32b, L.ap32, L.ap32b, L.p32, L.p32b};
         for (int j = 0; j < every; ++j)
-            if (fused_bricks() == 2)
-                hipLaunchKernelGGL((k_cg_fused<kFusedTB, 2>), GD, dim3(kFusedTB), 0, st, o, s++,
-                                   j == 0 ? first_prev : o.gd, j & 1, L.val32, V, L.bv32, L.bm32, L.ev32);
-            else
-                hipLaunchKernelGGL((k_cg_fused<kFusedTB, 1>), GD, dim3(kFusedTB), 0, st, o, s++,
-                                   j == 0 ? first_prev : o.gd, j & 1, L.val32, V, L.bv32, L.bm32, L.ev32);
+            launch_fused(GD, st, o, s++, j == 0 ? first_prev : o.gd, j & 1, L.val32, V, L.bv32, L.bm32, L.ev32);
         hipLaunchKernelGGL(k_mxf_accumulate, GU, BU, 0, st, o, s++, o.gd, L.x32, L.x, L.p32, L.p32b);
         hipLaunchKernelGGL((k_mx_residual<TB>), GS, BS, 0, st, o, s++, 0, L.val64, L.dg64, L.inv32, L.b, L.x, L.r32,
                            L.x32);
@@ -2716,14 +2727,8 @@ void bench_run(const BenchLayout &B, int reps, hipStream_t s) {
             if constexpr (sizeof(VT) == 4) {
                 const FusedVec V{B.e32, B.inv32, B.x32, (float *)B.z, B.r1, (float *)B.ap, B.a1, (float *)B.p0,
                                  (float *)B.p1};
-                if (fused_bricks() == 2)
-                    hipLaunchKernelGGL((k_cg_fused<kFusedTB, 2>), dim3(B.o.gd), dim3(kFusedTB), 0, s, B.o, 1, 1, 0,
-                                       (const float *)B.val, V, (const float *)B.bv, (const float *)B.bm,
-                                       (const float *)B.ev);
-                else
-                    hipLaunchKernelGGL((k_cg_fused<kFusedTB, 1>), dim3(B.o.gd), dim3(kFusedTB), 0, s, B.o, 1, 1, 0,
-                                       (const float *)B.val, V, (const float *)B.bv, (const float *)B.bm,
-                                       (const float *)B.ev);
+                launch_fused(dim3(B.o.gd), s, B.o, 1, 1, 0, (const float *)B.val, V, (const float *)B.bv,
+                             (const float *)B.bm, (const float *)B.ev);
             } else {
                 hipLaunchKernelGGL((k_cg_dir<double, 1, false, TB>), dim3(B.o.gd), dim3(TB), 0, s, B.o, 1, 0, 0, 0,
                                    (const double *)B.val, (const double *)B.dg, (const double *)B.z,
