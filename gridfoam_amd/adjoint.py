@@ -285,7 +285,8 @@ class TransientStep:
         rv = (st.ref_v_ad if st.ref_v_ad is not None else st.ref_v.clone())[:B, 0]
         rg = (st.ref_g_ad if st.ref_g_ad is not None else st.ref_g.clone())[:B, 0]
         active = (f != 0) & ~(g.bnd_snap[:B].bool() & (f == 1))
-        coef = torch.where(active, f / torch.where(active, g.bnd_mag_d[:B], torch.ones_like(f)), torch.zeros_like(f))
+        dist = torch.where(active, g.bnd_distances()[:B], torch.ones_like(f))
+        coef = torch.where(active, f / dist, torch.zeros_like(f))
         cells = g.bnd_owner[:B]
         sng = coef * (rv - p[cells]) + (1.0 - f) * rg
         cov = st.covered[:B].bool()

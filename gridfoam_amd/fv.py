@@ -316,9 +316,13 @@ class fvc:
                      nat.ptr(fi.bnd_buffer), nat.ptr(out), nat.stream())
             return out
         if track:  # a new tensor carrying the graph (``out`` is not written)
-            return _LsqGradFn.apply(field, st, hang_only, field.data, rv, rg)
-        nat.call("gf_lsq_grad", g.mesh_ref, ctypes.c_int32(field.nc), nat.ptr(field.data), nat.ptr(st.psi_b),
-                 nat.ptr(st.covered), ctypes.c_int32(int(hang_only)), nat.ptr(out), nat.stream())
+            out = _LsqGradFn.apply(field, st, hang_only, field.data, rv, rg)
+        else:
+            nat.call("gf_lsq_grad", g.mesh_ref, ctypes.c_int32(field.nc), nat.ptr(field.data), nat.ptr(st.psi_b),
+                     nat.ptr(st.covered), ctypes.c_int32(int(hang_only)), nat.ptr(out), nat.stream())
+        if g.shape is not None and not hang_only and torch.is_grad_enabled():
+            # |d| of the body's sides depends on its outline (a new tensor; ``out`` keeps the values)
+            out = g.shape.add_lsq_terms(field, st, out)
         return out
 
     @staticmethod
@@ -393,7 +397,8 @@ class fvc:
         st = field.boundary_state()
         f = st.frac[:B]
         active = (f != 0) & ~(g.bnd_snap[:B].bool() & (f == 1))
-        coef = torch.where(active, f / torch.where(active, g.bnd_mag_d[:B], torch.ones_like(f)), torch.zeros_like(f))
+        dist = torch.where(active, g.bnd_distances()[:B], torch.ones_like(f))
+        coef = torch.where(active, f / dist, torch.zeros_like(f))
         psi = field.data.reshape(g.num_cells, nc)[g.bnd_owner[:B]]
         out = coef[:, None] * (st.ref_v[:B] - psi) + (1.0 - f)[:, None] * st.ref_g[:B]
         out = torch.where(st.covered[:B].bool()[:, None], out, torch.zeros_like(out))
@@ -615,6 +620,8 @@ class fvm:
         mat._diag, mat._upper, mat._lower, mat._source = diag, upper, lower, source
         if spec.has_ffc:
             mat.face_flux_correction = ffc[: g.num_internal_faces]
+        if lap and g.shape is not None and track:  # |d| of the body's sides depends on its outline
+            g.shape.add_laplacian_terms(mat, field, st, rv, rg, lap, gamma)
         return mat
 
     @staticmethod

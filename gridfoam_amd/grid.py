@@ -53,6 +53,9 @@ def _scan(cnt: torch.Tensor, n: int) -> torch.Tensor:
 class OctreeGrid:
     """Device-resident octree grid (no immersed boundary)."""
 
+    # the active FourierShape (gridfoam_amd.shape) while shape derivatives are taken
+    shape = None
+
     def __init__(self, fluxel: FluxelConfig, *, dt: float = 1.0, device: str | torch.device = "cuda",
                  topology: dict | None = None):
         """Build the octree on the GPU from the fluxel block, or adopt a
@@ -196,6 +199,7 @@ class OctreeGrid:
             self.ap_owner_bnd_patch_id = self.ap_neighbour_bnd_patch_id = torch.zeros(0, dtype=torch.long,
                                                                                     device=dev)
             self.ap_owner_normal = self.ap_neighbour_normal = torch.zeros((0, 3), dtype=torch.float64, device=dev)
+            self.ap_face_centers = torch.zeros((0, 3), dtype=torch.float64, device=dev)
             return
         imm = self.ap_is_immersed_faces
         single = ~imm
@@ -205,6 +209,7 @@ class OctreeGrid:
         self.ap_owner_normal, self.ap_neighbour_normal = cut.owner_normal, cut.neighbour_normal
         o_all, n_all, a_all = (t[:F_all].long() for t in (self.owner_i32, self.neighbour_i32, self.axis_i32))
         io, inb, ia = o_all[imm], n_all[imm], a_all[imm]
+        self.ap_face_centers = self.face_centers[:F_all][imm].contiguous()  # the axis rays' tangential position
         Sf_i = self.Sf[:F_all][imm]
         mag_sf_i = torch.linalg.vector_norm(Sf_i, dim=1)
         e = torch.zeros((F_i, 3), dtype=torch.float64, device=dev)
@@ -396,6 +401,12 @@ class OctreeGrid:
         if key in {p.value for p in DomainBoundaryPatch}:
             return DomainBoundaryPatch(key).to_direction()
         return self.patch_name_to_id.get(key, -1)
+
+    def bnd_distances(self) -> torch.Tensor:
+        """|d| of every boundary face; while a FourierShape is active, the
+        same values carrying their derivative with respect to its
+        coefficients."""
+        return self.bnd_mag_d if self.shape is None else self.shape.bnd_mag_d
 
     def bnd_patch_mask(self, patch) -> torch.Tensor:
         """Mask over all boundary faces (domain and immersed sides) of a patch."""
