@@ -72,6 +72,57 @@ def run_gpu(mixed: bool, steps: int, warmup: int):
     return grid, alg, dt, p_iters
 
 
+def bluff_measure(steps: int, warmup: int = 1, check_every: int = 10) -> dict:
+    """The 3-D bluff body in a tunnel (21.0M cells, gridfoam_amd/cases/bluff3d_piso.yaml):
+    ``steps`` timed PISO steps with the FP32-in-FP64 pressure solve and the
+    FP64 continuity check of the corrected flux every ``check_every`` steps,
+    inside the timed region; the grid build and ``warmup`` steps before it."""
+    import torch
+    import yaml
+
+    from gridfoam_amd.algorithms import compute_continuity_error, create_case
+    from gridfoam_amd.config import CASES, load_config
+    from gridfoam_amd.forces import ForceEvaluator
+
+    cfg = yaml.safe_load((CASES / "bluff3d_piso.yaml").read_text())
+    torch.cuda.reset_peak_memory_stats()
+    t0 = time.perf_counter()
+    grid, alg = create_case(load_config(cfg), device="cuda:0")
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t0
+    _log(f"bluff body: {grid.num_cells} cells built in {t_build:.1f} s; {warmup} warmup + {steps} timed steps")
+    for _ in range(warmup):
+        alg.step()
+    torch.cuda.synchronize()
+    p_iters, checks = 0, []
+    t0 = time.perf_counter()
+    for k in range(steps):
+        alg.step()
+        it = sum(s.iterations for s in alg.p_solve_history[-alg.n_correctors:])
+        p_iters += it
+        msg = f"bluff body: step {k + 1}, {it} p-CG iterations"
+        if (k + 1) % check_every == 0:
+            checks.append(compute_continuity_error(alg.phi, grid.dt)[0])
+            msg += f", continuity error {checks[-1]:.3e}"
+        _log(msg)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    cd = float(ForceEvaluator(grid).evaluate(turbulence=float(alg.nu)).Cd)
+    C = grid.num_cells
+    _log("bluff body: SpMV roofline on its mesh")
+    pm = laplacian_operator(grid)
+    roof = {"fp32": spmv_roofline(grid, pm, fp32=True, launches=50),
+            "fp64": spmv_roofline(grid, pm, fp32=False, launches=50)}
+    return {"workload": "3-D bluff body in a tunnel: 20x4x3 root, surface level 4, wake box level 3, "
+                        "3 leaf refinements (7 levels), PISO 2 correctors, Re=1e4, FP32-in-FP64 pressure",
+            "cells": C, "immersed_faces": grid.num_immersed_faces, "steps": steps, "warmup": warmup,
+            "value": round(C * steps / dt / 1e6, 3), "unit": "Mcell-steps/s",
+            "ms_per_step": round(dt / steps * 1e3, 1), "p_cg_iters_per_step": round(p_iters / steps, 1),
+            "continuity_error_local_fp64_every_10": checks, "Cd_after": round(cd, 6),
+            "grid_build_s": round(t_build, 2),
+            "peak_allocated_GiB": round(torch.cuda.max_memory_allocated() / 2 ** 30, 2), "roofline": roof}
+
+
 def laplacian_operator(grid, seed: int = 0):
     """Pressure-type operator for the SpMV measurement: -laplacian(gamma, p)
     with all-Neumann walls, gamma ~ U(0.5, 1.5) (torch generator seed 0),
@@ -355,11 +406,14 @@ def main() -> int:
     ap.add_argument("--spmv-only", choices=("fp32", "fp64"), default=None,
                     help="only launch the SpMV kernel (for counter runs) and exit")
     ap.add_argument("--spmv-mesh", default="cavity",
-                    help="mesh of --spmv-only: the cavity octree or the uniform 256^3 grid")
+                    help="mesh of --spmv-only: cavity (the octree cavity), uniformN (N^3) or bluff (21.0M cells)")
     ap.add_argument("--serve-requests", type=int, default=48, help="requests in the serving measurement (0: skip)")
     ap.add_argument("--serve-slots", type=int, default=16, help="cases per batch in the serving measurement")
     ap.add_argument("--serve-only", action="store_true", help="only run the serving measurement")
     ap.add_argument("--serve-lanes", type=int, default=2, help="alone lanes for requests of >= 48^3 root cells")
+    ap.add_argument("--bluff-steps", type=int, default=0,
+                    help="timed PISO steps of the 21.0M-cell bluff-body case (0: skip)")
+    ap.add_argument("--bluff-only", action="store_true", help="only run the bluff-body measurement")
     ap.add_argument("--inner-rtol", type=float, default=None,
                     help="refinement trigger of the mixed solve (default: the solver config's inner_rel_tolerance)")
     a = ap.parse_args()
@@ -372,6 +426,9 @@ def main() -> int:
     import torch
 
     torch.cuda.set_device(0)
+    if a.bluff_only:
+        print(json.dumps({"bluff_body": bluff_measure(a.bluff_steps or 10)}), flush=True)
+        return 0
     if a.serve_only:
         print(json.dumps({"serving": serve_measure(a.serve_requests, a.serve_slots, lanes=a.serve_lanes)}), flush=True)
         return 0
@@ -381,6 +438,14 @@ def main() -> int:
 
         if a.spmv_mesh.startswith("uniform"):
             grid = uniform_grid(int(a.spmv_mesh[len("uniform"):]))
+        elif a.spmv_mesh == "bluff":
+            import yaml
+
+            from gridfoam_amd.config import CASES
+            from gridfoam_amd.grid import create_grid
+
+            grid = create_grid(load_config(yaml.safe_load((CASES / "bluff3d_piso.yaml").read_text())),
+                               device="cuda:0")
         else:
             grid, _ = create_case(load_config(_cfg(False)), device="cuda:0")
         r = spmv_roofline(grid, laplacian_operator(grid), fp32=a.spmv_only == "fp32", launches=20)
@@ -457,6 +522,8 @@ def main() -> int:
     if a.serve_requests > 0:
         _log(f"serving: {a.serve_requests} requests, batches of {a.serve_slots}")
         line["serving"] = serve_measure(a.serve_requests, a.serve_slots, lanes=a.serve_lanes)
+    if a.bluff_steps > 0:
+        line["bluff_body"] = bluff_measure(a.bluff_steps)
     if rel is not None:
         line["mixed_vs_fp64_rel_l2"] = rel
         line["speedup_mixed_vs_fp64"] = round(out["plain"]["dt"] / out["mixed"]["dt"], 3)

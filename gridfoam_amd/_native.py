@@ -69,6 +69,8 @@ _SIGS: dict[str, tuple[Any, list[Any]]] = {
     "gf_device_count": (_i32, [_vp]),
     "gf_abi_sizes": (_i32, [_vp]),
     "gf_octree_build": (_i32, [_vp, _vp, _vp, _i32, _vp, _vp, _vp, _i32, _vp, _vp, _vp]),
+    "gf_octree_build_surface": (_i32, [_vp, _vp, _vp, _i32, _vp, _vp, _vp, _i32, _P(GfShape), _i32, _i32, _vp,
+                                       _vp, _vp]),
     "gf_octree_fetch": (_i32, [_vp] * 8 + [_vp]),
     "gf_octree_free": (None, [_vp]),
     "gf_grid_geometry": (_i32, [_i32, _i32, _i32] + [_vp] * 17 + [_vp]),

@@ -151,7 +151,11 @@ class ImmersedBodyConfig(BaseModel, frozen=True):
     kind "plane": normal to `axis` at `offset` (an open surface);
     "box": axis-aligned, `min` - `max`;
     "fourier": prism along z over r(theta) = radius (1 + sum_k a_k cos k theta
-    + b_k sin k theta), k = 1..len(a), around `center` (x, y)."""
+    + b_k sin k theta), k = 1..len(a), around `center` (x, y);
+    "bluff": the box `min` - `max` with its nose (low-x) edges rounded by
+    `radius`, the top rear edge cut by a plane at `slant_angle` degrees over
+    the last `slant_length`, standing on up to four vertical `struts` (x, y)
+    of radius `strut_radius` that reach `strut_height` below it."""
 
     name: str
     kind: str
@@ -163,11 +167,28 @@ class ImmersedBodyConfig(BaseModel, frozen=True):
     radius: float = 0.0
     a: List[float] = Field(default_factory=list)
     b: List[float] = Field(default_factory=list)
+    slant_angle: float = 0.0
+    slant_length: float = 0.0
+    struts: List[List[float]] = Field(default_factory=list)
+    strut_radius: float = 0.0
+    strut_height: float = 0.0
 
     @model_validator(mode="after")
     def _check(self) -> "ImmersedBodyConfig":
-        if self.kind not in ("plane", "box", "fourier"):
+        if self.kind not in ("plane", "box", "fourier", "bluff"):
             raise ValueError(f"unknown immersed body kind {self.kind!r}")
+        if self.kind == "bluff":
+            if self.min is None or self.max is None or len(self.min) != 3 or len(self.max) != 3:
+                raise ValueError("a bluff body needs min and max (3 values each)")
+            ext = [hi - lo for lo, hi in zip(self.min, self.max)]
+            if min(ext) <= 0.0 or not 0.0 <= self.radius <= 0.5 * min(ext[1], ext[2]) or self.radius > ext[0]:
+                raise ValueError("a bluff body needs min < max and 0 <= radius <= half its width and height")
+            if not 0.0 <= self.slant_angle < 90.0 or not 0.0 <= self.slant_length <= ext[0]:
+                raise ValueError("slant_angle must be in [0, 90) degrees and slant_length within the length")
+            if len(self.struts) > 4 or any(len(st) != 2 for st in self.struts):
+                raise ValueError("at most 4 struts, each (x, y)")
+            if self.struts and (self.strut_radius <= 0.0 or self.strut_height <= 0.0):
+                raise ValueError("struts need strut_radius > 0 and strut_height > 0")
         if self.name in {p.value for p in DomainBoundaryPatch}:
             raise ValueError(f"immersed body name {self.name!r} is a domain patch name")
         if self.kind == "box" and (self.min is None or self.max is None or len(self.min) != 3 or len(self.max) != 3):

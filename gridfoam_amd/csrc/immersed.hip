@@ -17,13 +17,15 @@
 //
 // Bodies: a plane normal to one axis (open surface, both sides fluid); an
 // axis-aligned box; a prism along z over a 2-D Fourier outline
-// r(theta) = r0 (1 + sum_k a_k cos k theta + b_k sin k theta) around (cx, cy).
-// One thread per face; the Fourier outline is intersected by bisection on
-// the sign of |P - c| - r(theta), which changes once along a segment shorter
-// than the outline's features.
+// r(theta) = r0 (1 + sum_k a_k cos k theta + b_k sin k theta) around (cx, cy);
+// a bluff body (a box with rounded nose edges, a rear slant and struts,
+// gf_shapes.h).  One thread per face; the Fourier outline and the bluff body
+// are intersected by bisection on the sign of their implicit function, which
+// changes once along a segment shorter than the body's features.
 #include <cmath>
 
 #include "gf_common.h"
+#include "gf_shapes.h"
 
 namespace gf {
 namespace {
@@ -42,6 +44,11 @@ __device__ double fourier_g(const gf_shape &b, double x, double y) {
     double r = 1.0;
     for (int k = 1; k <= 8; ++k) r += b.p[2 + k] * cos(k * th) + b.p[10 + k] * sin(k * th);
     return rho - b.p[2] * r;
+}
+
+// Implicit function of a Fourier (kind 2) or bluff (kind 3) body at q.
+__device__ double implicit_g(const gf_shape &b, const double q[3]) {
+    return b.kind == 2 ? fourier_g(b, q[0], q[1]) : bluff_g(b, q[0], q[1], q[2]);
 }
 
 // Hit of the segment s in [s0, s1] (axis a, other coordinates those of P) with
@@ -68,13 +75,13 @@ __device__ bool body_hit(const gf_shape &b, int a, const double P[3], double s0,
         }
         return true;
     }
-    if (b.kind == 2) {  // Fourier prism along z
-        if (a == 2) return false;
+    if (b.kind == 2 || b.kind == 3) {  // Fourier prism along z, bluff body
+        if (b.kind == 2 && a == 2) return false;
         double q0[3] = {P[0], P[1], P[2]}, q1[3] = {P[0], P[1], P[2]};
         q0[a] = s0;
         q1[a] = s1;
-        double g0 = fourier_g(b, q0[0], q0[1]);
-        const double g1 = fourier_g(b, q1[0], q1[1]);
+        double g0 = implicit_g(b, q0);
+        const double g1 = implicit_g(b, q1);
         if (g0 * g1 > 0.0) return false;
         if (g0 == 0.0) {
             *hit = s0;
@@ -90,7 +97,7 @@ __device__ bool body_hit(const gf_shape &b, int a, const double P[3], double s0,
             if (mid <= lo || mid >= hi) break;
             double q[3] = {P[0], P[1], P[2]};
             q[a] = mid;
-            const double gm = fourier_g(b, q[0], q[1]);
+            const double gm = implicit_g(b, q);
             if ((gm <= 0.0) == (g0 <= 0.0)) {
                 lo = mid;
                 g0 = gm;
@@ -112,6 +119,21 @@ __device__ void body_normal(const gf_shape &b, int a, const double P[3], double 
         n[b.axis] = 1.0;
     } else if (b.kind == 1) {
         n[a] = at_lo ? -1.0 : 1.0;
+    } else if (b.kind == 3) {
+#pragma clang fp contract(off)
+        // central differences of the implicit function (step 1e-7)
+        const double e = 1e-7;
+        double q[3] = {P[0], P[1], P[2]};
+        q[a] = s;
+        double gr[3];
+        for (int j = 0; j < 3; ++j) {
+            double qp[3] = {q[0], q[1], q[2]}, qm[3] = {q[0], q[1], q[2]};
+            qp[j] = q[j] + e;
+            qm[j] = q[j] - e;
+            gr[j] = (bluff_g(b, qp[0], qp[1], qp[2]) - bluff_g(b, qm[0], qm[1], qm[2])) / (2.0 * e);
+        }
+        const double gn = sqrt(gr[0] * gr[0] + gr[1] * gr[1] + gr[2] * gr[2]);
+        for (int j = 0; j < 3; ++j) n[j] = gr[j] / gn;
     } else {
         // grad(|P - c| - R r(theta)) in the plane of the outline
         double q[3] = {P[0], P[1], P[2]};
@@ -198,7 +220,8 @@ extern "C" int gf_ap_cut(int32_t n_faces, const int32_t *owner, const int32_t *n
     Shapes sh{};
     sh.n = n_shapes;
     for (int k = 0; k < n_shapes; ++k) {
-        GF_REQUIRE(shapes[k].kind >= 0 && shapes[k].kind <= 2, "unknown body kind");
+        GF_REQUIRE(shapes[k].kind >= 0 && shapes[k].kind <= 3, "unknown body kind");
+        GF_REQUIRE(shapes[k].kind != 3 || (shapes[k].axis >= 0 && shapes[k].axis <= 4), "at most 4 struts");
         GF_REQUIRE(shapes[k].patch >= 0, "body patch id must be >= 0");
         GF_REQUIRE(shapes[k].kind != 0 || (shapes[k].axis >= 0 && shapes[k].axis <= 2), "plane axis must be 0..2");
         sh.s[k] = shapes[k];

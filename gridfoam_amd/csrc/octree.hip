@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "gf_common.h"
+#include "gf_shapes.h"
 
 namespace gf {
 
@@ -97,6 +98,8 @@ struct Params {
     int32_t n_regions;
     const double *rmin, *rmax;
     const int32_t *rlevel;
+    int32_t n_shapes, target_level;  // surface refinement of analytic bodies
+    const gf_shape *shapes;
 };
 
 __device__ __forceinline__ uint64_t leaf_key(int32_t i, int32_t j, int32_t k, int level, int lmax) {
@@ -133,12 +136,38 @@ __device__ __forceinline__ bool intersects(const Params &p, const Leaf &l, int r
     return true;
 }
 
+// A body's surface may pass through the leaf: |G(centre)| <= half the
+// leaf's diagonal (G bounds the distance to the surface from below).  Plane
+// (kind 0), box (1) and bluff bodies (3).
+__device__ bool near_surface(const Params &p, const Leaf &l) {
+#pragma clang fp contract(off)
+    const int32_t idx[3] = {l.i, l.j, l.k};
+    double c[3], h[3];
+    for (int a = 0; a < 3; ++a) {
+        h[a] = (p.upper[a] - p.lower[a]) / ldexp((double)p.root[a], l.level);
+        c[a] = fma((double)idx[a] + 0.5, h[a], p.lower[a]);
+    }
+    const double hd = 0.5 * sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+    for (int k = 0; k < p.n_shapes; ++k) {
+        const gf_shape &b = p.shapes[k];
+        double g;
+        if (b.kind == 0)
+            g = c[b.axis] - b.p[0];
+        else if (b.kind == 1)
+            g = box_g(b, c[0], c[1], c[2]);
+        else
+            g = bluff_g(b, c[0], c[1], c[2]);
+        if (fabs(g) <= hd) return true;
+    }
+    return false;
+}
+
 __global__ void k_region_flags(int32_t n, Params p, const Leaf *cells, int32_t *flag, int32_t *count) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
     const Leaf l = cells[c];
-    int f = 0;
-    for (int r = 0; r < p.n_regions; ++r)
+    int f = l.level < p.target_level && near_surface(p, l);
+    for (int r = 0; r < p.n_regions && !f; ++r)
         if (l.level < p.rlevel[r] && intersects(p, l, r)) {
             f = 1;
             break;
@@ -375,12 +404,29 @@ extern "C" int gf_octree_build(const double lower[3], const double upper[3], con
                                int32_t n_regions, const double *rmin, const double *rmax,
                                const int32_t *rlevel, int32_t n_leaf, gf_octree **out, int64_t counts_out[4],
                                gf_stream_t stream) {
+    return gf_octree_build_surface(lower, upper, root, n_regions, rmin, rmax, rlevel, 0, nullptr, 0, n_leaf, out,
+                                   counts_out, stream);
+}
+
+extern "C" int gf_octree_build_surface(const double lower[3], const double upper[3], const int32_t root[3],
+                                       int32_t n_regions, const double *rmin, const double *rmax,
+                                       const int32_t *rlevel, int32_t n_shapes, const gf_shape *shapes,
+                                       int32_t target_level, int32_t n_leaf, gf_octree **out,
+                                       int64_t counts_out[4], gf_stream_t stream) {
     GF_REQUIRE(lower && upper && root && out && counts_out, "null argument");
     GF_REQUIRE(n_regions >= 0 && n_leaf >= 0, "negative region/leaf count");
+    GF_REQUIRE(n_shapes >= 0 && n_shapes <= 16 && target_level >= 0 && (shapes || n_shapes == 0),
+               "bad surface refinement arguments (at most 16 bodies)");
+    for (int k = 0; k < n_shapes; ++k) {
+        GF_REQUIRE(shapes[k].kind == 0 || shapes[k].kind == 1 || shapes[k].kind == 3,
+                   "surface refinement is defined for plane, box and bluff bodies");
+        GF_REQUIRE(shapes[k].kind != 0 || (shapes[k].axis >= 0 && shapes[k].axis <= 2), "plane axis must be 0..2");
+        GF_REQUIRE(shapes[k].kind != 3 || (shapes[k].axis >= 0 && shapes[k].axis <= 4), "at most 4 struts");
+    }
     for (int a = 0; a < 3; ++a) GF_REQUIRE(root[a] >= 1 && upper[a] > lower[a], "bad domain/root");
     hipStream_t s = S(stream);
     Params p{};
-    int32_t lmax = 0;
+    int32_t lmax = n_shapes > 0 ? target_level : 0;
     for (int r = 0; r < n_regions; ++r) lmax = rlevel[r] > lmax ? rlevel[r] : lmax;
     lmax += n_leaf;
     int maxroot = root[0] > root[1] ? root[0] : root[1];
@@ -407,6 +453,13 @@ extern "C" int gf_octree_build(const double lower[3], const double upper[3], con
     p.rmin = d_rmin.as<double>();
     p.rmax = d_rmax.as<double>();
     p.rlevel = d_rlev.as<int32_t>();
+    DevBuf d_shapes;
+    GF_CHECK_HIP(d_shapes.alloc(sizeof(gf_shape) * (n_shapes + 1)));
+    if (n_shapes > 0)
+        GF_CHECK_HIP(hipMemcpyAsync(d_shapes.p, shapes, sizeof(gf_shape) * n_shapes, hipMemcpyHostToDevice, s));
+    p.n_shapes = n_shapes;
+    p.target_level = n_shapes > 0 ? target_level : 0;
+    p.shapes = d_shapes.as<gf_shape>();
 
     // roots in Morton order
     const int64_t ncode = (int64_t)P * P * P;
@@ -428,7 +481,8 @@ extern "C" int gf_octree_build(const double lower[3], const double upper[3], con
         if (cells) (void)hipFree(cells);
         return rc;
     };
-    // region refinement until no leaf below a box level intersects its box
+    // surface and region refinement until no leaf below the target level
+    // meets a body's surface and none below a box level intersects its box
     // (each pass raises the finest level by one, so lmax + 1 passes suffice)
     for (int pass = 0;; ++pass) {
         if (pass > lmax + 1) {

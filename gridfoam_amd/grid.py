@@ -5,7 +5,7 @@
 owner/neighbour/axis, domain faces, cell centres/sizes/volumes, face
 centres, Sf -- built entirely by the HIP library:
 
-1. ``gf_octree_build`` (Morton-ordered leaves, region refinement, 2:1 face
+1. ``gf_octree_build`` (Morton-ordered leaves, surface and region refinement, 2:1 face
    balance, face enumeration) replaces fluxel-cfd (core/grid/factory.py:32-65);
 2. ``gf_grid_geometry`` restates axis_projected.py:130-161, :715-811;
 3. ``gf_face_geometry`` / ``gf_lsq_ata_inv`` / ``gf_hang_geometry`` restate
@@ -94,8 +94,16 @@ class OctreeGrid:
         rlev = (ctypes.c_int32 * (nr + 1))(*[r.level for r in regs])
         handle = ctypes.c_void_p()
         counts = (ctypes.c_int64 * 4)()
-        nat.call("gf_octree_build", lo, hi, root, ctypes.c_int32(nr), rmin, rmax, rlev,
-                 ctypes.c_int32(fluxel.n_leaf_refinement), ctypes.byref(handle), counts, nat.stream())
+        bodies = list(fluxel.immersed_bodies)
+        if fluxel.target_level > 0 and bodies:  # refine to target_level around the bodies' surfaces
+            from .immersed import shapes_from_config
+
+            nat.call("gf_octree_build_surface", lo, hi, root, ctypes.c_int32(nr), rmin, rmax, rlev,
+                     ctypes.c_int32(len(bodies)), shapes_from_config(bodies), ctypes.c_int32(fluxel.target_level),
+                     ctypes.c_int32(fluxel.n_leaf_refinement), ctypes.byref(handle), counts, nat.stream())
+        else:
+            nat.call("gf_octree_build", lo, hi, root, ctypes.c_int32(nr), rmin, rmax, rlev,
+                     ctypes.c_int32(fluxel.n_leaf_refinement), ctypes.byref(handle), counts, nat.stream())
         C, F, B, lmax = (int(v) for v in counts)
         self.num_cells, self.num_internal_faces, self.num_domain_bnd_faces, self.lmax = C, F, B, lmax
         self.level = _i32(C, dev)

@@ -21,6 +21,7 @@ handled by ``immersed_dirichlet_constraints`` below and the elimination of
 from __future__ import annotations
 
 import ctypes
+import math
 from dataclasses import dataclass
 from typing import List, Sequence
 
@@ -45,6 +46,15 @@ def shapes_from_config(bodies: Sequence[ImmersedBodyConfig]):
             s.kind = 1
             for j in range(3):
                 s.p[j], s.p[3 + j] = float(b.min[j]), float(b.max[j])
+        elif b.kind == "bluff":
+            s.kind, s.axis = 3, len(b.struts)
+            for j in range(3):
+                s.p[j], s.p[3 + j] = float(b.min[j]), float(b.max[j])
+            s.p[6] = float(b.radius)
+            s.p[7], s.p[8] = math.sin(math.radians(b.slant_angle)), math.cos(math.radians(b.slant_angle))
+            s.p[9], s.p[10], s.p[11] = float(b.slant_length), float(b.strut_radius), float(b.strut_height)
+            for k1, (x, y) in enumerate(b.struts):
+                s.p[12 + 2 * k1], s.p[13 + 2 * k1] = float(x), float(y)
         else:
             s.kind = 2
             s.p[0], s.p[1], s.p[2] = float(b.center[0]), float(b.center[1]), float(b.radius)

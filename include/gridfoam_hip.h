@@ -127,7 +127,12 @@ int gf_seg_bicgstab(const gf_mesh *m, int32_t n_seg, const int32_t *seg_row_host
  * for analytic bodies.  kind 0: plane normal to `axis` at p[0] (open surface);
  * 1: box p[0..2] - p[3..5]; 2: prism along z over the outline
  * r(theta) = p[2] (1 + sum_k p[2+k] cos k theta + p[10+k] sin k theta), k = 1..8,
- * centred at (p[0], p[1]).  Both sides of a cut face carry the body's patch. */
+ * centred at (p[0], p[1]); 3: bluff body, the box p[0..2] - p[3..5] (nose at
+ * x = p[0]) with nose edges rounded by p[6], the top rear edge cut by a plane
+ * of outward normal (p[7], 0, p[8]) through the top p[9] ahead of the rear,
+ * and `axis` (<= 4) vertical struts of radius p[10] at (p[12+2k], p[13+2k])
+ * reaching p[11] below the body.  Both sides of a cut face carry the body's
+ * patch. */
 typedef struct gf_shape {
     int32_t kind, axis, patch, pad;
     double p[20];
@@ -153,6 +158,18 @@ int gf_octree_build(const double lower[3], const double upper[3], const int32_t 
                     int32_t n_regions, const double *rmin, const double *rmax,
                     const int32_t *rlevel, int32_t n_leaf, gf_octree **out,
                     int64_t counts_out[4], gf_stream_t stream);
+/* The same build with fluxel's surface refinement (meta/config.py:100-106,
+ * `target_level`: the finest level around the input surface; passed to the
+ * mesher at core/grid/factory.py:52-64) for analytic bodies: before the
+ * region boxes, every leaf below target_level whose centre lies within half
+ * its diagonal of a body's surface (|G| of gf_shapes' implicit functions) is
+ * split, to a fixed point.  Bodies of kind 0 (plane), 1 (box) and 3 (bluff
+ * body); at most 16. */
+int gf_octree_build_surface(const double lower[3], const double upper[3], const int32_t root[3],
+                            int32_t n_regions, const double *rmin, const double *rmax,
+                            const int32_t *rlevel, int32_t n_shapes, const gf_shape *shapes,
+                            int32_t target_level, int32_t n_leaf, gf_octree **out,
+                            int64_t counts_out[4], gf_stream_t stream);
 /* device-to-device copy into caller buffers */
 int gf_octree_fetch(const gf_octree *t, int32_t *level, int32_t *ijk, int32_t *f_owner,
                     int32_t *f_nbr, int32_t *f_axis, int32_t *b_owner, int32_t *b_dir,

@@ -38,6 +38,7 @@ def _lib() -> ctypes.CDLL:
             raise RuntimeError(f"oracle library missing: {path} (run make -C oracle)")
         lib = ctypes.CDLL(str(path))
         lib.oct_oracle_build.restype = ctypes.c_void_p
+        lib.oct_oracle_build_surface.restype = ctypes.c_void_p
         lib.oct_oracle_fetch.restype = None
         lib.oct_oracle_free.restype = None
         lib.oct_oracle_free.argtypes = [ctypes.c_void_p]
@@ -61,8 +62,34 @@ class OctreeTopology:
     bnd_dir: np.ndarray  # [F_b] int32
 
 
-def build_octree(lower, upper, root, regions=(), n_leaf: int = 0) -> OctreeTopology:
-    """Octree leaves and face lists (contract: core/grid/factory.py:32-65)."""
+def _shape_params(body: dict):
+    """(kind code, axis, p[20]) of a body for the C build: plane 0 (axis,
+    offset), box 1 (min, max), bluff body 3 (min, max, nose radius, slant
+    sin / cos, slant length, strut radius and height, strut count in axis,
+    strut centres)."""
+    p = np.zeros(20)
+    kind = body["kind"]
+    if kind == "plane":
+        p[0] = float(body.get("offset", 0.0))
+        return 0, int(body.get("axis", 0)), p
+    if kind in ("box", "bluff"):
+        p[0:3], p[3:6] = body["min"], body["max"]
+        if kind == "box":
+            return 1, 0, p
+        ang = math.radians(float(body.get("slant_angle", 0.0)))
+        p[6], p[7], p[8] = float(body.get("radius", 0.0)), math.sin(ang), math.cos(ang)
+        p[9], p[10], p[11] = (float(body.get(k, 0.0)) for k in ("slant_length", "strut_radius", "strut_height"))
+        struts = body.get("struts", []) or []
+        for k, (x, y) in enumerate(struts):
+            p[12 + 2 * k], p[13 + 2 * k] = x, y
+        return 3, len(struts), p
+    raise ValueError(f"surface refinement is not defined for {kind!r} bodies")
+
+
+def build_octree(lower, upper, root, regions=(), n_leaf: int = 0, bodies=(), target_level: int = 0) -> OctreeTopology:
+    """Octree leaves and face lists (contract: core/grid/factory.py:32-65);
+    with bodies and target_level > 0, leaves are refined to target_level
+    where a body's surface may pass (meta/config.py:100-106)."""
     lib = _lib()
     lo = np.ascontiguousarray(lower, dtype=np.float64)
     hi = np.ascontiguousarray(upper, dtype=np.float64)
@@ -72,10 +99,21 @@ def build_octree(lower, upper, root, regions=(), n_leaf: int = 0) -> OctreeTopol
     rmax = np.ascontiguousarray([r[1] for r in regions] or [[0, 0, 0]], dtype=np.float64)
     rlev = np.ascontiguousarray([r[2] for r in regions] or [0], dtype=np.int32)
     counts = np.zeros(4, dtype=np.int64)
-    h = lib.oct_oracle_build(
-        _ptr(lo), _ptr(hi), _ptr(rt), ctypes.c_int32(nr), _ptr(rmin), _ptr(rmax),
-        _ptr(rlev), ctypes.c_int32(n_leaf), _ptr(counts),
-    )
+    if target_level > 0 and len(bodies):
+        packed = [_shape_params(b) for b in bodies]
+        kinds = np.ascontiguousarray([k for k, _, _ in packed], dtype=np.int32)
+        axes = np.ascontiguousarray([a for _, a, _ in packed], dtype=np.int32)
+        params = np.ascontiguousarray(np.stack([q for _, _, q in packed]), dtype=np.float64)
+        h = lib.oct_oracle_build_surface(
+            _ptr(lo), _ptr(hi), _ptr(rt), ctypes.c_int32(nr), _ptr(rmin), _ptr(rmax), _ptr(rlev),
+            ctypes.c_int32(len(bodies)), _ptr(kinds), _ptr(axes), _ptr(params), ctypes.c_int32(target_level),
+            ctypes.c_int32(n_leaf), _ptr(counts),
+        )
+    else:
+        h = lib.oct_oracle_build(
+            _ptr(lo), _ptr(hi), _ptr(rt), ctypes.c_int32(nr), _ptr(rmin), _ptr(rmax),
+            _ptr(rlev), ctypes.c_int32(n_leaf), _ptr(counts),
+        )
     try:
         C, F, B, lmax = (int(v) for v in counts)
         level = np.zeros(C, np.int32)
@@ -126,6 +164,32 @@ def _fourier_outline(body: dict, x: np.ndarray, y: np.ndarray) -> np.ndarray:
     return rho - body["radius"] * r
 
 
+def _bluff_outline(body: dict, x: np.ndarray, y: np.ndarray, z: np.ndarray) -> np.ndarray:
+    """Implicit function of a bluff body (negative inside): the box min - max
+    with its nose (low-x) edges rounded by `radius` (for x < x0 + r:
+    max(|(a, b_y)|, |(a, b_z)|) - r, a = x0 + r - x, b_y and b_z the
+    distances past the rounding's inner lines), the top rear edge cut by the
+    plane of outward normal (sin, 0, cos) of `slant_angle` through the top
+    `slant_length` ahead of the rear, and vertical struts of `strut_radius`
+    from `strut_height` below the body into it.  Every piece is 1-Lipschitz,
+    so |G| bounds the distance to the surface from below."""
+    _, n_st, p = _shape_params(body)
+    x0, y0, z0, x1, y1, z1, r = p[0], p[1], p[2], p[3], p[4], p[5], p[6]
+    g = np.maximum(np.maximum(np.maximum(x0 - x, x - x1), np.maximum(y0 - y, y - y1)),
+                   np.maximum(z0 - z, z - z1))
+    a = (x0 + r) - x
+    by = np.maximum(np.maximum((y0 + r) - y, y - (y1 - r)), 0.0)
+    bz = np.maximum(np.maximum((z0 + r) - z, z - (z1 - r)), 0.0)
+    nose = np.maximum(np.sqrt(a * a + by * by) - r, np.sqrt(a * a + bz * bz) - r)
+    g = np.where(x < x0 + r, np.maximum(g, nose), g)
+    g = np.maximum(g, p[7] * (x - (x1 - p[9])) + p[8] * (z - z1))
+    for k in range(n_st):
+        dx, dy = x - p[12 + 2 * k], y - p[13 + 2 * k]
+        gs = np.maximum(np.sqrt(dx * dx + dy * dy) - p[10], np.maximum((z0 - p[11]) - z, z - (z0 + p[10])))
+        g = np.minimum(g, gs)
+    return g
+
+
 def _body_hits(body: dict, ax: np.ndarray, P: np.ndarray, s0: np.ndarray, s1: np.ndarray):
     """Hit coordinate along each face's axis ray [s0, s1] (through the face
     centre P, endpoints included) seen from the s0 end and from the s1 end;
@@ -148,11 +212,14 @@ def _body_hits(body: dict, ax: np.ndarray, P: np.ndarray, s0: np.ndarray, s1: np
         from_lo = np.where(in_lo, lo, np.where(in_hi, hi, nan))
         from_hi = np.where(in_hi, hi, np.where(in_lo, lo, nan))
         return from_lo, from_hi
-    if kind == "fourier":
+    if kind in ("fourier", "bluff"):
         def outline(s):
-            return _fourier_outline(body, np.where(ax == 0, s, P[:, 0]), np.where(ax == 1, s, P[:, 1]))
+            x, y = np.where(ax == 0, s, P[:, 0]), np.where(ax == 1, s, P[:, 1])
+            if kind == "bluff":
+                return _bluff_outline(body, x, y, np.where(ax == 2, s, P[:, 2]))
+            return _fourier_outline(body, x, y)
         g0, g1 = outline(s0), outline(s1)
-        ok = (ax != 2) & ~(g0 * g1 > 0.0)
+        ok = ((ax != 2) | (kind == "bluff")) & ~(g0 * g1 > 0.0)
         h = np.where(ok & (g0 == 0.0), s0, np.where(ok & (g1 == 0.0), s1, nan))
         need = ok & (g0 != 0.0) & (g1 != 0.0)
         lo, hi, glo = s0.copy(), s1.copy(), g0.copy()
@@ -194,6 +261,18 @@ def _body_normal(body: dict, ax: np.ndarray, P: np.ndarray, s: np.ndarray) -> np
     elif kind == "box":
         lo = np.asarray(body["min"], np.float64)[ax]
         n[np.arange(len(ax)), ax] = np.where(s == lo, -1.0, 1.0)
+    elif kind == "bluff":  # central differences of the implicit function, step 1e-7
+        q = P.copy()
+        q[np.arange(len(ax)), ax] = s
+        e = 1e-7
+        gr = np.zeros((len(ax), 3))
+        for j in range(3):
+            qp, qm = q.copy(), q.copy()
+            qp[:, j] = q[:, j] + e
+            qm[:, j] = q[:, j] - e
+            gr[:, j] = (_bluff_outline(body, *qp.T) - _bluff_outline(body, *qm.T)) / (2.0 * e)
+        gn = np.sqrt(gr[:, 0] * gr[:, 0] + gr[:, 1] * gr[:, 1] + gr[:, 2] * gr[:, 2])
+        n = gr / gn[:, None]
     else:
         x = np.where(ax == 0, s, P[:, 0])
         y = np.where(ax == 1, s, P[:, 1])
@@ -257,8 +336,8 @@ class Grid:
     [domain faces | upper sides | lower sides]; the internal arrays the
     single-sided faces in ascending order (face_geometry.py:128-131)."""
 
-    def __init__(self, lower, upper, root, regions=(), n_leaf=0, dt=1.0, bodies=()):
-        topo = build_octree(lower, upper, root, regions, n_leaf)
+    def __init__(self, lower, upper, root, regions=(), n_leaf=0, dt=1.0, bodies=(), target_level=0):
+        topo = build_octree(lower, upper, root, regions, n_leaf, bodies, target_level)
         self.topo = topo
         self.lower = np.asarray(lower, np.float64)
         self.upper = np.asarray(upper, np.float64)
@@ -1417,7 +1496,7 @@ def grid_from_cfg(cfg: dict) -> Grid:
     regions = [(r["min"], r["max"], r["level"]) for r in fx.get("refinement_regions", []) or []]
     return Grid(fx["domain"]["lower"], fx["domain"]["upper"], fx["root_resolution"], regions,
                 fx.get("n_leaf_refinement", 0), cfg["simulator"]["control"]["deltaT"],
-                fx.get("immersed_bodies", []) or [])
+                fx.get("immersed_bodies", []) or [], int(fx.get("target_level", 0) or 0))
 
 
 def _solver_cfg(d: dict) -> SolverCfg:

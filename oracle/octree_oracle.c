@@ -6,8 +6,17 @@
  * is consumed by src/gridfoam/core/grid/axis_projected.py:97-161 and
  * configured by src/gridfoam/meta/config.py:90-145:
  *
- *   root block grid  -> refine every leaf that intersects a refinement box
- *   below the box level -> 2:1 face balance -> n_leaf uniform splits.
+ *   root block grid  -> refine every leaf that meets an analytic body's
+ *   surface below target_level (meta/config.py:100-106) or intersects a
+ *   refinement box below the box level -> 2:1 face balance -> n_leaf
+ *   uniform splits.
+ *
+ * Surface test: |G(leaf centre)| <= half the leaf's diagonal, with G the
+ * body's implicit function (plane: signed offset along its axis; box:
+ * max(lo - x, x - hi) over the axes; bluff body: that box with its nose
+ * edges rounded, a rear slant plane and vertical struts).  These are
+ * restated from their definition, operation for operation, so that without
+ * contraction both builds round alike.
  *
  * Output contract (axis_projected.py:97-124, :763-767, :781-782):
  *   internal faces (owner, neighbour, axis) with the neighbour on the +axis
@@ -40,12 +49,19 @@ typedef struct {
 } leaf_t;
 
 typedef struct {
+    int32_t kind, axis; /* kind 0 plane, 1 box, 3 bluff body; axis: plane axis / strut count */
+    double p[20];
+} shape_t;
+
+typedef struct {
     double lower[3], upper[3];
     int32_t root[3];
     int32_t lmax;
     int32_t n_regions;
     double *rmin, *rmax;
     int32_t *rlevel;
+    int32_t n_shapes, target_level;
+    shape_t *shapes;
     int32_t n_leaf;
     /* results */
     leaf_t *cells;
@@ -91,6 +107,57 @@ static int intersects_region(const oct_t *o, const leaf_t *c, int r) {
         if (!(lo < o->rmax[3 * r + a] && hi > o->rmin[3 * r + a])) return 0;
     }
     return 1;
+}
+
+static double box_g(const shape_t *b, double x, double y, double z) {
+    double gx = fmax(b->p[0] - x, x - b->p[3]);
+    double gy = fmax(b->p[1] - y, y - b->p[4]);
+    double gz = fmax(b->p[2] - z, z - b->p[5]);
+    return fmax(fmax(gx, gy), gz);
+}
+
+/* Bluff body: box p[0..2] - p[3..5]; nose edges rounded by p[6]; slant
+ * plane with normal (p[7], 0, p[8]) through the top p[9] ahead of the rear;
+ * `axis` struts of radius p[10] at (p[12+2k], p[13+2k]) reaching p[11] below. */
+static double bluff_g(const shape_t *b, double x, double y, double z) {
+    double x0 = b->p[0], y0 = b->p[1], z0 = b->p[2], x1 = b->p[3], y1 = b->p[4], z1 = b->p[5];
+    double r = b->p[6];
+    double g = box_g(b, x, y, z);
+    if (x < x0 + r) {
+        double a = (x0 + r) - x;
+        double by = fmax(fmax((y0 + r) - y, y - (y1 - r)), 0.0);
+        double bz = fmax(fmax((z0 + r) - z, z - (z1 - r)), 0.0);
+        g = fmax(g, fmax(sqrt(a * a + by * by) - r, sqrt(a * a + bz * bz) - r));
+    }
+    g = fmax(g, b->p[7] * (x - (x1 - b->p[9])) + b->p[8] * (z - z1));
+    for (int k = 0; k < b->axis && k < 4; ++k) {
+        double dx = x - b->p[12 + 2 * k], dy = y - b->p[13 + 2 * k];
+        double gs = fmax(sqrt(dx * dx + dy * dy) - b->p[10], fmax((z0 - b->p[11]) - z, z - (z0 + b->p[10])));
+        g = fmin(g, gs);
+    }
+    return g;
+}
+
+static int near_surface(const oct_t *o, const leaf_t *c) {
+    const int32_t idx[3] = {c->i, c->j, c->k};
+    double ctr[3], h[3];
+    for (int a = 0; a < 3; ++a) {
+        h[a] = cell_h(o, a, c->level);
+        ctr[a] = fma((double)idx[a] + 0.5, h[a], o->lower[a]);
+    }
+    double hd = 0.5 * sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+    for (int k = 0; k < o->n_shapes; ++k) {
+        const shape_t *b = &o->shapes[k];
+        double g;
+        if (b->kind == 0)
+            g = ctr[b->axis] - b->p[0];
+        else if (b->kind == 1)
+            g = box_g(b, ctr[0], ctr[1], ctr[2]);
+        else
+            g = bluff_g(b, ctr[0], ctr[1], ctr[2]);
+        if (fabs(g) <= hd) return 1;
+    }
+    return 0;
 }
 
 /* Replace every flagged leaf by its 8 children. */
@@ -156,6 +223,11 @@ static int64_t refine_regions(oct_t *o) {
         uint8_t *flag = (uint8_t *)calloc((size_t)o->n_cells, 1);
         int64_t n_flag = 0;
         for (int64_t c = 0; c < o->n_cells; ++c) {
+            if (o->cells[c].level < o->target_level && near_surface(o, &o->cells[c])) {
+                flag[c] = 1;
+                ++n_flag;
+                continue;
+            }
             for (int r = 0; r < o->n_regions; ++r) {
                 if (o->cells[c].level < o->rlevel[r] && intersects_region(o, &o->cells[c], r)) {
                     flag[c] = 1;
@@ -268,10 +340,34 @@ static void enumerate_faces(oct_t *o) {
 
 /* ---- public C API used by the tests through ctypes ---------------------- */
 
+void *oct_oracle_build_surface(const double *lower, const double *upper, const int32_t *root,
+                               int32_t n_regions, const double *rmin, const double *rmax,
+                               const int32_t *rlevel, int32_t n_shapes, const int32_t *kind,
+                               const int32_t *axis, const double *params, int32_t target_level,
+                               int32_t n_leaf, int64_t *counts_out);
+
 void *oct_oracle_build(const double *lower, const double *upper, const int32_t *root,
                        int32_t n_regions, const double *rmin, const double *rmax,
                        const int32_t *rlevel, int32_t n_leaf, int64_t *counts_out) {
+    return oct_oracle_build_surface(lower, upper, root, n_regions, rmin, rmax, rlevel, 0, NULL, NULL, NULL, 0,
+                                    n_leaf, counts_out);
+}
+
+/* kind/axis [n_shapes], params [n_shapes][20] */
+void *oct_oracle_build_surface(const double *lower, const double *upper, const int32_t *root,
+                               int32_t n_regions, const double *rmin, const double *rmax,
+                               const int32_t *rlevel, int32_t n_shapes, const int32_t *kind,
+                               const int32_t *axis, const double *params, int32_t target_level,
+                               int32_t n_leaf, int64_t *counts_out) {
     oct_t *o = (oct_t *)calloc(1, sizeof(oct_t));
+    o->n_shapes = n_shapes;
+    o->target_level = n_shapes > 0 ? target_level : 0;
+    o->shapes = (shape_t *)calloc((size_t)(n_shapes + 1), sizeof(shape_t));
+    for (int k = 0; k < n_shapes; ++k) {
+        o->shapes[k].kind = kind[k];
+        o->shapes[k].axis = axis[k];
+        memcpy(o->shapes[k].p, params + 20 * k, sizeof(double) * 20);
+    }
     memcpy(o->lower, lower, sizeof(double) * 3);
     memcpy(o->upper, upper, sizeof(double) * 3);
     memcpy(o->root, root, sizeof(int32_t) * 3);
@@ -279,7 +375,7 @@ void *oct_oracle_build(const double *lower, const double *upper, const int32_t *
     o->rmin = (double *)malloc(sizeof(double) * 3 * (size_t)(n_regions + 1));
     o->rmax = (double *)malloc(sizeof(double) * 3 * (size_t)(n_regions + 1));
     o->rlevel = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n_regions + 1));
-    int32_t lmax = 0;
+    int32_t lmax = o->target_level;
     for (int r = 0; r < n_regions; ++r) {
         for (int a = 0; a < 3; ++a) {
             o->rmin[3 * r + a] = rmin[3 * r + a];
@@ -347,5 +443,6 @@ void oct_oracle_free(void *h) {
     free(o->rmin);
     free(o->rmax);
     free(o->rlevel);
+    free(o->shapes);
     free(o);
 }

@@ -190,3 +190,26 @@ def box_cavity(n: int = 16, boxes=None, algorithm: str = "simple", depth: int = 
         s["tolerance"], s["rel_tolerance"] = 1e-11, 0.0
         s["max_iter"] = 4000
     return cfg
+
+
+def bluff_body(root=(20, 4, 3), target_level: int = 4, n_leaf: int = 0, wake_level=None, dt=None,
+               mixed: bool = True) -> dict:
+    """The 3-D bluff-body tunnel case with a given root grid, surface level,
+    leaf refinement and wake-box level (None: no wake box); dt defaults to
+    CFL 0.5 on the finest cells."""
+    cfg = copy.deepcopy(case_dict("bluff3d_piso.yaml"))
+    fx = cfg["fluxel"]
+    fx["root_resolution"] = list(root)
+    fx["target_level"] = target_level
+    fx["n_leaf_refinement"] = n_leaf
+    if wake_level is None:
+        fx["refinement_regions"] = []
+    else:
+        fx["refinement_regions"][0]["level"] = wake_level
+    h0 = 10.0 / root[0]
+    levels = max(target_level, wake_level or 0) + n_leaf
+    sim = cfg["simulator"]
+    sim["control"]["deltaT"] = dt if dt is not None else 0.5 * h0 / 2 ** levels
+    sim["control"]["endTime"] = 100 * sim["control"]["deltaT"]
+    sim["fvSolution"]["solvers"]["p"]["mixed_precision"] = mixed
+    return cfg
