@@ -72,11 +72,14 @@ def run_gpu(mixed: bool, steps: int, warmup: int):
     return grid, alg, dt, p_iters
 
 
-def bluff_measure(steps: int, warmup: int = 1, check_every: int = 10) -> dict:
+def bluff_measure(steps: int, warmup: int = 1, check_every: int = 10, mixed: bool = True,
+                  keep: bool = False) -> dict:
     """The 3-D bluff body in a tunnel (21.0M cells, gridfoam_amd/cases/bluff3d_piso.yaml):
-    ``steps`` timed PISO steps with the FP32-in-FP64 pressure solve and the
-    FP64 continuity check of the corrected flux every ``check_every`` steps,
-    inside the timed region; the grid build and ``warmup`` steps before it."""
+    ``steps`` timed PISO steps with the FP32-in-FP64 pressure solve (plain
+    FP64 with ``mixed`` False) and the FP64 continuity check of the
+    corrected flux every ``check_every`` steps, inside the timed region; the
+    grid build and ``warmup`` steps before it.  ``keep``: the final U and p
+    (host arrays) under "_U" / "_p"."""
     import torch
     import yaml
 
@@ -85,6 +88,7 @@ def bluff_measure(steps: int, warmup: int = 1, check_every: int = 10) -> dict:
     from gridfoam_amd.forces import ForceEvaluator
 
     cfg = yaml.safe_load((CASES / "bluff3d_piso.yaml").read_text())
+    cfg["simulator"]["fvSolution"]["solvers"]["p"]["mixed_precision"] = bool(mixed)
     torch.cuda.reset_peak_memory_stats()
     t0 = time.perf_counter()
     grid, alg = create_case(load_config(cfg), device="cuda:0")
@@ -113,14 +117,127 @@ def bluff_measure(steps: int, warmup: int = 1, check_every: int = 10) -> dict:
     pm = laplacian_operator(grid)
     roof = {"fp32": spmv_roofline(grid, pm, fp32=True, launches=50),
             "fp64": spmv_roofline(grid, pm, fp32=False, launches=50)}
-    return {"workload": "3-D bluff body in a tunnel: 20x4x3 root, surface level 4, wake box level 3, "
-                        "3 leaf refinements (7 levels), PISO 2 correctors, Re=1e4, FP32-in-FP64 pressure",
+    kept = {"_U": alg.U.data.detach().cpu().numpy(), "_p": alg.p.data.detach().cpu().numpy(),
+            "_fluid": fluid_cells(grid)} if keep else {}
+    return {**kept, "workload": "3-D bluff body in a tunnel: 20x4x3 root, surface level 4, wake box level 3, "
+                                "3 leaf refinements (7 levels), PISO 2 correctors, Re=1e4",
+            "pressure_solve": "f32-in-f64" if mixed else "f64",
             "cells": C, "immersed_faces": grid.num_immersed_faces, "steps": steps, "warmup": warmup,
             "value": round(C * steps / dt / 1e6, 3), "unit": "Mcell-steps/s",
             "ms_per_step": round(dt / steps * 1e3, 1), "p_cg_iters_per_step": round(p_iters / steps, 1),
             "continuity_error_local_fp64_every_10": checks, "Cd_after": round(cd, 6),
             "grid_build_s": round(t_build, 2),
             "peak_allocated_GiB": round(torch.cuda.max_memory_allocated() / 2 ** 30, 2), "roofline": roof}
+
+
+def adjoint_measure(steps: int, every: int = 10, fd_coefs=(1,), h: float = 1e-6) -> dict:
+    """The Fourier-body channel (205,600 cells, PIMPLE 2x2, FP64,
+    gridfoam_amd/cases/channel2d_fourier_pimple.yaml) from its initial
+    state: the drag coefficient after ``steps`` steps and its gradient with
+    respect to the body's 16 outline coefficients, through checkpointed
+    steps (a checkpoint every ``every`` steps); the forward without a graph
+    for comparison; central differences (step ``h``) of the coefficients in
+    ``fd_coefs`` on grids cut with perturbed outlines.  Solver tolerances
+    are tightened to 1e-13 so that the differences resolve the gradient."""
+    import copy
+
+    import torch
+    import yaml
+
+    from gridfoam_amd.adjoint import TransientStep, checkpointed_rollout, transient_state
+    from gridfoam_amd.algorithms import create_case
+    from gridfoam_amd.config import CASES, ForceCoeffConfig, load_config
+    from gridfoam_amd.forces import ForceEvaluator
+    from gridfoam_amd.shape import FourierShape
+
+    base = yaml.safe_load((CASES / "channel2d_fourier_pimple.yaml").read_text())
+    sol = base["simulator"]["fvSolution"]["solvers"]
+    sol["U"].update({"tolerance": 0.0, "rel_tolerance": 1e-13, "max_iter": 2000})
+    sol["p"].update({"tolerance": 0.0, "rel_tolerance": 1e-13, "max_iter": 8000})
+    force = ForceCoeffConfig.model_validate({
+        "patches": ["body"], "rho": 1.0, "magU_ref": 1.0, "A_ref": 1.0, "L_ref": 1.0,
+        "local_coord": {"drag_dir": [1.0, 0.0, 0.0], "lift_dir": [0.0, 1.0, 0.0],
+                        "center_of_rotation": [5.0, 4.0, 0.25]}})
+
+    def case(coef=None):
+        cfg = copy.deepcopy(base)
+        if coef is not None:
+            body = cfg["fluxel"]["immersed_bodies"][0]
+            body["a"], body["b"] = [float(v) for v in coef[:8]], [float(v) for v in coef[8:]]
+        return create_case(load_config(cfg), device="cuda:0")
+
+    def drag(alg, grid, state0, grad):
+        s = tuple(t.clone() for t in state0)
+        with torch.set_grad_enabled(grad):
+            out = checkpointed_rollout(TransientStep(alg), s, steps, every=every if grad else None)
+            alg.U.data, alg.p.data = out[0], out[3]
+            alg.U._state = alg.p._state = None
+            return ForceEvaluator(grid, config=force).evaluate(turbulence=float(alg.nu)).Cd
+
+    grid, alg = case()
+    state0 = transient_state(alg)
+    shape = FourierShape(grid, "body")
+    c0 = shape.coefficients.detach().cpu().tolist()
+    _log(f"adjoint: forward without a graph, {steps} PIMPLE steps")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    cd_plain = float(drag(alg, grid, state0, False))
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    _log(f"adjoint: {t_plain:.1f} s; checkpointed forward and backward")
+    torch.cuda.reset_peak_memory_stats()
+    mem0 = torch.cuda.memory_allocated()
+    with shape:
+        t0 = time.perf_counter()
+        cd = drag(alg, grid, state0, True)
+        torch.cuda.synchronize()
+        t_fwd = time.perf_counter() - t0
+        (g,) = torch.autograd.grad(cd, shape.coefficients)
+        torch.cuda.synchronize()
+        t_bwd = time.perf_counter() - t0 - t_fwd
+    peak = (torch.cuda.max_memory_allocated() - mem0) / 2 ** 30
+    _log(f"adjoint: forward {t_fwd:.1f} s, backward {t_bwd:.1f} s, Cd {float(cd):.10f}")
+    fd = {}
+    for j in fd_coefs:
+        vals = []
+        for sgn in (1.0, -1.0):
+            c = list(c0)
+            c[j] += sgn * h
+            gj, aj = case(c)
+            same = all(torch.equal(getattr(gj, k), getattr(grid, k)) for k in (
+                "ap_is_immersed_faces", "ap_owner_near_boundary", "ap_neighbour_near_boundary"))
+            if not same:
+                raise RuntimeError("perturbed outline changed the cut topology")
+            vals.append(float(drag(aj, gj, state0, False)))
+            del gj, aj
+        fd[j] = (vals[0] - vals[1]) / (2 * h)
+        _log(f"adjoint: coefficient {j}: adjoint {float(g[j]):.8e}, central difference {fd[j]:.8e}")
+    return {"workload": "2-D channel around the Fourier body (205,600 cells), PIMPLE 2x2 from the initial state, "
+                        "drag coefficient gradient w.r.t. the 16 outline coefficients",
+            "steps": steps, "checkpoint_every": every, "Cd": round(float(cd), 10), "Cd_no_graph": round(cd_plain, 10),
+            "forward_no_graph_s": round(t_plain, 2), "forward_checkpointed_s": round(t_fwd, 2),
+            "backward_s": round(t_bwd, 2), "gradient_vs_forward": round((t_fwd + t_bwd) / t_plain, 2),
+            "peak_extra_GiB": round(peak, 3), "dCd_dcoef": [float(v) for v in g.cpu()],
+            "central_differences": {str(j): v for j, v in fd.items()},
+            "rel_err_vs_central_differences": {str(j): abs(float(g[j]) - v) / abs(v) for j, v in fd.items()}}
+
+
+def fluid_cells(grid):
+    """Mask of the cells connected to cell 0 through the internal faces (host).
+    Immersed faces couple nothing, so the cells inside a body form regions
+    whose pressure has only zero-gradient conditions: it is fixed up to a
+    constant by each solver's iteration alone and is left out of the
+    comparisons."""
+    import numpy as np
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import connected_components
+
+    C, F = grid.num_cells, grid.num_internal_faces
+    o = grid.owner_i32[:F].cpu().numpy()
+    n = grid.neighbour_i32[:F].cpu().numpy()
+    adj = sp.coo_matrix((np.ones(F, np.int8), (o, n)), shape=(C, C)).tocsr()
+    _, lab = connected_components(adj, directed=False)
+    return lab == lab[0]
 
 
 def laplacian_operator(grid, seed: int = 0):
@@ -414,6 +531,8 @@ def main() -> int:
     ap.add_argument("--bluff-steps", type=int, default=0,
                     help="timed PISO steps of the 21.0M-cell bluff-body case (0: skip)")
     ap.add_argument("--bluff-only", action="store_true", help="only run the bluff-body measurement")
+    ap.add_argument("--adjoint-steps", type=int, default=0,
+                    help="only run the channel shape-gradient measurement over this many PIMPLE steps")
     ap.add_argument("--inner-rtol", type=float, default=None,
                     help="refinement trigger of the mixed solve (default: the solver config's inner_rel_tolerance)")
     a = ap.parse_args()
@@ -426,8 +545,31 @@ def main() -> int:
     import torch
 
     torch.cuda.set_device(0)
+    if a.adjoint_steps > 0:
+        print(json.dumps({"shape_adjoint": adjoint_measure(a.adjoint_steps)}), flush=True)
+        return 0
     if a.bluff_only:
-        print(json.dumps({"bluff_body": bluff_measure(a.bluff_steps or 10)}), flush=True)
+        n = a.bluff_steps or 10
+        res = {}
+        for mode in (("plain", "mixed") if a.mode == "both" else (a.mode,)):
+            res[mode] = bluff_measure(n, mixed=mode == "mixed", keep=a.mode == "both")
+            _log(f"bluff body {mode}: {res[mode]['ms_per_step']} ms/step")
+        if a.mode == "both":
+            P, M = res["plain"], res["mixed"]
+            res["speedup_mixed_vs_fp64"] = round(P["ms_per_step"] / M["ms_per_step"], 3)
+            fl = P["_fluid"]
+
+            def rel(k, mask):
+                a, b = M["_" + k][mask], P["_" + k][mask]
+                return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+            res["fluid_cells"] = int(fl.sum())
+            res["mixed_vs_fp64_rel_l2"] = {k: rel(k, fl) for k in ("U", "p")}
+            # all cells: the pressure inside the body is fixed by each solver's iteration alone
+            res["mixed_vs_fp64_rel_l2_all_cells"] = {k: rel(k, slice(None)) for k in ("U", "p")}
+            for r in (P, M):
+                del r["_U"], r["_p"], r["_fluid"]
+        print(json.dumps({"bluff_body": res}), flush=True)
         return 0
     if a.serve_only:
         print(json.dumps({"serving": serve_measure(a.serve_requests, a.serve_slots, lanes=a.serve_lanes)}), flush=True)
