@@ -130,15 +130,16 @@ def bluff_measure(steps: int, warmup: int = 1, check_every: int = 10, mixed: boo
             "peak_allocated_GiB": round(torch.cuda.max_memory_allocated() / 2 ** 30, 2), "roofline": roof}
 
 
-def adjoint_measure(steps: int, every: int = 10, fd_coefs=(1,), h: float = 1e-6) -> dict:
+def adjoint_measure(steps: int, every: int = 10, fd_coefs=(1,), h: float = 1e-4, tol=None) -> dict:
     """The Fourier-body channel (205,600 cells, PIMPLE 2x2, FP64,
-    gridfoam_amd/cases/channel2d_fourier_pimple.yaml) from its initial
-    state: the drag coefficient after ``steps`` steps and its gradient with
-    respect to the body's 16 outline coefficients, through checkpointed
-    steps (a checkpoint every ``every`` steps); the forward without a graph
-    for comparison; central differences (step ``h``) of the coefficients in
-    ``fd_coefs`` on grids cut with perturbed outlines.  Solver tolerances
-    are tightened to 1e-13 so that the differences resolve the gradient."""
+    gridfoam_amd/cases/channel2d_fourier_pimple.yaml, its own solver
+    tolerances) from its initial state: the drag coefficient after ``steps``
+    steps and its gradient with respect to the body's 16 outline
+    coefficients, through checkpointed steps (a checkpoint every ``every``
+    steps); the forward without a graph for comparison; central differences
+    (step ``h``) of the coefficients in ``fd_coefs`` on grids cut with
+    perturbed outlines, a check at the resolution the solves allow.  ``tol``:
+    relative tolerance of both solvers instead of the case's."""
     import copy
 
     import torch
@@ -151,9 +152,10 @@ def adjoint_measure(steps: int, every: int = 10, fd_coefs=(1,), h: float = 1e-6)
     from gridfoam_amd.shape import FourierShape
 
     base = yaml.safe_load((CASES / "channel2d_fourier_pimple.yaml").read_text())
-    sol = base["simulator"]["fvSolution"]["solvers"]
-    sol["U"].update({"tolerance": 0.0, "rel_tolerance": 1e-13, "max_iter": 2000})
-    sol["p"].update({"tolerance": 0.0, "rel_tolerance": 1e-13, "max_iter": 8000})
+    if tol is not None:
+        for k, it in (("U", 2000), ("p", 8000)):
+            base["simulator"]["fvSolution"]["solvers"][k].update({"tolerance": 0.0, "rel_tolerance": tol,
+                                                                   "max_iter": it})
     force = ForceCoeffConfig.model_validate({
         "patches": ["body"], "rho": 1.0, "magU_ref": 1.0, "A_ref": 1.0, "L_ref": 1.0,
         "local_coord": {"drag_dir": [1.0, 0.0, 0.0], "lift_dir": [0.0, 1.0, 0.0],
@@ -168,8 +170,18 @@ def adjoint_measure(steps: int, every: int = 10, fd_coefs=(1,), h: float = 1e-6)
 
     def drag(alg, grid, state0, grad):
         s = tuple(t.clone() for t in state0)
+        step = TransientStep(alg)
+        calls = [0]
+
+        def logged(st):
+            calls[0] += 1
+            if calls[0] % 50 == 0:
+                _log(f"adjoint: step call {calls[0]}")
+            return step(st)
+
         with torch.set_grad_enabled(grad):
-            out = checkpointed_rollout(TransientStep(alg), s, steps, every=every if grad else None)
+            params = grid.shape.tensors() if grad and grid.shape is not None else None
+            out = checkpointed_rollout(logged, s, steps, every=every if grad else None, params=params)
             alg.U.data, alg.p.data = out[0], out[3]
             alg.U._state = alg.p._state = None
             return ForceEvaluator(grid, config=force).evaluate(turbulence=float(alg.nu)).Cd
@@ -214,6 +226,8 @@ def adjoint_measure(steps: int, every: int = 10, fd_coefs=(1,), h: float = 1e-6)
         _log(f"adjoint: coefficient {j}: adjoint {float(g[j]):.8e}, central difference {fd[j]:.8e}")
     return {"workload": "2-D channel around the Fourier body (205,600 cells), PIMPLE 2x2 from the initial state, "
                         "drag coefficient gradient w.r.t. the 16 outline coefficients",
+            "solver_tolerances": ("the case's: U 1e-8 absolute, p 1e-8 relative" if tol is None else
+                                  f"U and p {tol:g} relative"), "fd_step": h,
             "steps": steps, "checkpoint_every": every, "Cd": round(float(cd), 10), "Cd_no_graph": round(cd_plain, 10),
             "forward_no_graph_s": round(t_plain, 2), "forward_checkpointed_s": round(t_fwd, 2),
             "backward_s": round(t_bwd, 2), "gradient_vs_forward": round((t_fwd + t_bwd) / t_plain, 2),
@@ -533,6 +547,9 @@ def main() -> int:
     ap.add_argument("--bluff-only", action="store_true", help="only run the bluff-body measurement")
     ap.add_argument("--adjoint-steps", type=int, default=0,
                     help="only run the channel shape-gradient measurement over this many PIMPLE steps")
+    ap.add_argument("--adjoint-tol", type=float, default=None,
+                    help="relative solver tolerance of the shape-gradient measurement (default: the case's)")
+    ap.add_argument("--adjoint-h", type=float, default=1e-4, help="central-difference step of that measurement")
     ap.add_argument("--inner-rtol", type=float, default=None,
                     help="refinement trigger of the mixed solve (default: the solver config's inner_rel_tolerance)")
     a = ap.parse_args()
@@ -546,7 +563,8 @@ def main() -> int:
 
     torch.cuda.set_device(0)
     if a.adjoint_steps > 0:
-        print(json.dumps({"shape_adjoint": adjoint_measure(a.adjoint_steps)}), flush=True)
+        print(json.dumps({"shape_adjoint": adjoint_measure(a.adjoint_steps, h=a.adjoint_h, tol=a.adjoint_tol)}),
+              flush=True)
         return 0
     if a.bluff_only:
         n = a.bluff_steps or 10

@@ -31,11 +31,60 @@ from .solvers import LinearSolver, create_solver
 State = Tuple[torch.Tensor, ...]
 
 
+class _Segment(torch.autograd.Function):
+    """``k`` calls of ``step`` run without a graph; the backward recomputes
+    them with one from the saved input state and differentiates the
+    recomputation with respect to that state and to ``params`` (tensors the
+    step reads from its closure).  Only the segment's input state is held
+    between the passes, whatever the step's own nodes keep."""
+
+    @staticmethod
+    def forward(ctx, step, k, n_state, *tensors):  # noqa: D401
+        state, params = tensors[:n_state], tensors[n_state:]
+        ctx.step, ctx.k, ctx.n_state, ctx.params = step, k, n_state, params
+        ctx.save_for_backward(*state)
+        out = tuple(state)
+        for _ in range(k):
+            out = step(out)
+        return tuple(o.clone() if any(o is t for t in state) else o for o in out)
+
+    @staticmethod
+    def backward(ctx, *g_out):
+        n = ctx.n_state
+        need = ctx.needs_input_grad[3:]
+        with torch.enable_grad():
+            s = tuple(t.detach().requires_grad_(bool(need[i])) for i, t in enumerate(ctx.saved_tensors))
+            out = s
+            for _ in range(ctx.k):
+                out = ctx.step(out)
+        pairs = [(o, g) for o, g in zip(out, g_out) if g is not None and o.requires_grad]
+        wanted = [i for i in range(len(need)) if need[i]]
+        ins = [s[i] if i < n else ctx.params[i - n] for i in wanted]
+        grads = [None] * len(need)
+        if pairs and ins:
+            got = torch.autograd.grad([o for o, _ in pairs], ins, [g for _, g in pairs], allow_unused=True)
+            for i, g in zip(wanted, got):
+                grads[i] = g
+        return (None, None, None, *grads)
+
+
 def checkpointed_rollout(step: Callable[[State], State], state: State, n_steps: int,
-                         every: Optional[int] = None) -> State:
+                         every: Optional[int] = None, params: Optional[Sequence[torch.Tensor]] = None) -> State:
     """Advance ``state`` by ``n_steps`` calls of ``step``.  With ``every``,
-    the steps run in segments of that many under activation checkpointing:
-    the backward pass recomputes a segment from its saved input state."""
+    the steps run in segments of that many and the backward pass recomputes
+    a segment from its saved input state.
+
+    With ``params`` (every tensor that the step reads from outside ``state``
+    and whose gradient is wanted: a viscosity, boundary values, the
+    immersed geometry of a ``FourierShape``), a segment's forward keeps no
+    graph at all, so the memory held is the segment boundary states plus,
+    in the backward pass, one segment's graph; gradients reach exactly the
+    state and ``params``, which must not be computed from one another (a
+    gradient reaching both a tensor and its ancestor would count twice).
+    Without ``params``, torch's activation
+    checkpointing runs the segments: gradients reach every tensor the step
+    reads, but the nodes of the native operations keep their launch
+    descriptions (and the tensors those hold) for every step."""
     if n_steps < 0:
         raise ValueError("n_steps must be >= 0")
     if every is not None and every < 1:
@@ -54,6 +103,9 @@ def checkpointed_rollout(step: Callable[[State], State], state: State, n_steps: 
         k = n_steps - done if every is None else min(every, n_steps - done)
         if every is None or not torch.is_grad_enabled():
             state = segment(k)(*state)
+        elif params is not None:
+            ps = tuple(t for t in params if torch.is_tensor(t))  # numbers carry no gradient
+            state = _Segment.apply(step, k, len(state), *state, *ps)
         else:
             state = checkpoint(segment(k), *state, use_reentrant=False)
         done += k
@@ -213,8 +265,15 @@ class TransientStep:
         w = g.w[:F]
         U0f = w[:, None] * U_old[o] + (1.0 - w)[:, None] * U_old[n]
         corr = phi_old - (U0f * Sf).sum(dim=1)
+        # 1 - min(|corr| / (|phi_old| + tiny), 1), differentiated only where the
+        # ratio is below 1 and phi_old is not zero (elsewhere its derivative is
+        # 0 and the quotient rule would give 0 * inf)
         tiny = torch.finfo(torch.float64).tiny
-        coupling = 1.0 - torch.clamp(corr.abs() / (phi_old.abs() + tiny), max=1.0)
+        a, den = corr.abs(), phi_old.abs() + tiny
+        smooth = (a < den) & (phi_old != 0)
+        ratio = torch.where(smooth, a / torch.where(smooth, den, torch.ones_like(den)),
+                            torch.where(a < den, a.detach() / den.detach(), torch.ones_like(den)))
+        coupling = 1.0 - ratio
         rAUf = w * rAU[o] + (1.0 - w) * rAU[n]
         phi_s = phi_s + rAUf * (coupling * corr / g.dt)
         # constrainHbyA: the velocity condition's flux on non-assignable faces

@@ -177,6 +177,12 @@ class FourierShape:
         g.shape = self
         return self
 
+    def tensors(self) -> Tuple[torch.Tensor, ...]:
+        """The geometry tensor that time steps read (the boundary distances;
+        the distance and normal arrays feed only the force sampling): the
+        ``params`` of a graph-free checkpointed rollout."""
+        return (self.bnd_mag_d,)
+
     def __exit__(self, *exc) -> None:
         g = self.grid
         for k, v in self._base.items():

@@ -87,7 +87,7 @@ def test_drag_gradient_through_checkpointed_pimple_steps_matches_central_differe
         step = TransientStep(alg, nu=nu)
         s = tuple(t.clone() for t in state0)
         with torch.set_grad_enabled(grad):
-            out = checkpointed_rollout(step, s, STEPS, every=EVERY)
+            out = checkpointed_rollout(step, s, STEPS, every=EVERY, params=(nu, bc.value))
             alg.U.data, alg.p.data = out[0], out[3]
             alg.U._state = None
             alg.p._state = None

@@ -61,7 +61,8 @@ def _drag(alg, ev, state0, grad: bool):
     step = TransientStep(alg)
     s = tuple(t.clone() for t in state0)
     with torch.set_grad_enabled(grad):
-        out = checkpointed_rollout(step, s, STEPS, every=EVERY)
+        params = alg.grid.shape.tensors() if grad and alg.grid.shape is not None else None
+        out = checkpointed_rollout(step, s, STEPS, every=EVERY, params=params)
         alg.U.data, alg.p.data = out[0], out[3]
         alg.U._state = None
         alg.p._state = None

@@ -166,7 +166,7 @@ def test_gradients_through_checkpointed_steps_match_central_differences(name):
         step = TransientStep(alg, nu=nu)
         s = (U0,) + tuple(t.clone() for t in state0[1:])
         with torch.set_grad_enabled(grad):
-            return J(checkpointed_rollout(step, s, 4, every=2))
+            return J(checkpointed_rollout(step, s, 4, every=2, params=(nu, value)))
 
     nu = torch.tensor(nu_base, dtype=torch.float64, device=grid.device, requires_grad=True)
     value = v_base.clone().requires_grad_(True)
