@@ -97,7 +97,7 @@ def test_drag_gradient_through_checkpointed_pimple_steps_matches_central_differe
     speed = torch.tensor(1.0, dtype=torch.float64, requires_grad=True)  # scales the condition's (host) value
     cd = run(nu, speed, True)
     g_nu, g_speed = torch.autograd.grad(cd, [nu, speed])
-    _say(f"Cd = {float(cd):.10f}; adjoint dCd/dnu {float(g_nu):.8e}, dCd/dU_in {float(g_speed):.8e}")
+    _say(f"Cd = {float(cd.detach()):.10f}; adjoint dCd/dnu {float(g_nu):.8e}, dCd/dU_in {float(g_speed):.8e}")
 
     def fd(f, h):
         return (float(f(h)) - float(f(-h))) / (2 * h)
