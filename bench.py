@@ -89,6 +89,8 @@ def bluff_measure(steps: int, warmup: int = 1, check_every: int = 10, mixed: boo
 
     cfg = yaml.safe_load((CASES / "bluff3d_piso.yaml").read_text())
     cfg["simulator"]["fvSolution"]["solvers"]["p"]["mixed_precision"] = bool(mixed)
+    if mixed and INNER_RTOL is not None:
+        cfg["simulator"]["fvSolution"]["solvers"]["p"]["inner_rel_tolerance"] = INNER_RTOL
     torch.cuda.reset_peak_memory_stats()
     t0 = time.perf_counter()
     grid, alg = create_case(load_config(cfg), device="cuda:0")

@@ -243,7 +243,9 @@ class SolverConfig(BaseModel, frozen=True):  # meta/config.py:269-316
     max_restart: int = 5
     log_interval: int = 25
     mixed_precision: bool = False
-    inner_rel_tolerance: float = 0.5  # mixed: refine once the FP32 residual fell to this fraction of the last true one
+    # mixed: refine once the FP32 residual fell to this fraction of the last
+    # true one (or after 128 FP32 iterations, whichever comes first)
+    inner_rel_tolerance: float = 0.01
 
 
 class RelaxationFactorsConfig(BaseModel, frozen=True):

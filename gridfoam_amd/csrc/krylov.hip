@@ -41,9 +41,12 @@ struct KState {
     double rz_old, alpha, beta;
     double rho_old, omega, rhat2;
     double scale;  // mixed precision: true residual at the last refinement
+    double rel;    // mixed precision: this solve's refinement trigger (see k_mxf_refine)
+    double rec;    // mixed precision: the FP32 recursive residual that triggered the refinement
     int iter, phase, converged, restarts;
     int fresh;  // PCG: 1 = first direction (p = z), 2 = residual just replaced, 0 = steady
     int pcur;   // PCG: which of the two p buffers holds the current direction
+    int iter_ref;  // mixed precision: the iteration count at the last refinement
 };
 
 constexpr int kMaxRhs = 3;
@@ -81,6 +84,7 @@ struct Ops {
     int norm_inf, max_iter, max_restart;
     double atol, rtol;
     double reliable;   // mixed PCG: refine once |r| < reliable * (last true residual)
+    int reliable_run;  // mixed PCG: ... or once this many FP32 iterations ran since the last refinement
     KState *st;        // [2][kMaxRhs]
     double *partials;  // [2][kMaxRhs][kNQ][pstride]
     int *poll;         // last launch of a chunk: pinned host word <- (s << 1) | all components done
@@ -234,12 +238,15 @@ __device__ __forceinline__ KState state_in(const Ops &o, int s, int r) {
     v.omega = c->omega;
     v.rhat2 = c->rhat2;
     v.scale = c->scale;
+    v.rel = c->rel;
+    v.rec = c->rec;
     v.iter = c->iter;
     v.phase = c->phase;
     v.converged = c->converged;
     v.restarts = c->restarts;
     v.fresh = c->fresh;
     v.pcur = c->pcur;
+    v.iter_ref = c->iter_ref;
     return v;
 }
 // Workgroup 0 writes this launch's state (every wave holds the same copy).
@@ -1055,6 +1062,7 @@ __device__ __forceinline__ void fused_transition(const Ops &o, int pb, const dou
         const double nr = nrm_fin(t[0][FQ_NR], o.norm_inf);
         if (nr < S.thresh || S.iter >= o.max_iter) {
             S.phase = PH_RESTART;
+            S.rec = nr;
             return;
         }
         const double g = t[0][FQ_G], al = g / t[0][FQ_D];
@@ -1471,8 +1479,9 @@ __global__ __launch_bounds__(RB) void k_mxf_accumulate(Ops o, int s, int nb_prev
         double tot[1][kFQ];
         reduce_prev<1, kFQ, RB>(o, s, nb_prev, mx, tot);
         const double nr = nrm_fin(tot[0][FQ_NR], o.norm_inf);
-        if (nr < o.reliable * S[0].scale) {
+        if (nr < S[0].rel * S[0].scale || S[0].iter - S[0].iter_ref >= o.reliable_run) {
             S[0].phase = PH_RESTART;
+            S[0].rec = nr;
             due = true;
         }
     }
@@ -1489,6 +1498,25 @@ __global__ __launch_bounds__(RB) void k_mxf_accumulate(Ops o, int s, int nb_prev
     }
     state_out<1>(o, s, S);
 }
+
+// FP32 iterations after which the mixed PCG refines whatever its residual
+// drop (GRIDFOAM_MIXED_RUN overrides the default for tuning).
+int mixed_run() {
+    static int v = [] {
+        const char *e = getenv("GRIDFOAM_MIXED_RUN");
+        const int r = e ? atoi(e) : 128;
+        return r < 1 ? 1 : r;
+    }();
+    return v;
+}
+
+// Adaptive refinement trigger of the fused mixed-precision PCG: a solve
+// starts refining once its FP32 recursive residual has dropped by
+// `reliable` since the last true residual; a refinement whose true residual
+// exceeds kDrift times the recursive one that triggered it multiplies that
+// factor by kRelGrow, up to kRelMax (ill-conditioned systems, whose FP32
+// recursion drifts early, end up refining after smaller drops).
+constexpr double kDrift = 2.0, kRelGrow = 10.0, kRelMax = 0.5;
 
 // Refinement, part 3 (one workgroup): decide on the true residual (first
 // solve start or refinement), or pass the last fused launch's partials
@@ -1526,6 +1554,12 @@ __global__ __launch_bounds__(RB) void k_mxf_refine(Ops o, int s, int first, int 
                     S.converged = 0;
                     S.rz_old = tot[0][FQ_G];
                     S.pcur = 0;
+                    S.rel = o.reliable;
+                    S.rec = 0.0;
+                } else if (S.rec > 0.0 && nr > kDrift * S.rec) {
+                    // The FP32 recursion had drifted from the true residual:
+                    // refine after smaller drops for the rest of this solve.
+                    S.rel = fmin(S.rel * kRelGrow, kRelMax);
                 }
                 S.final_res = nr;
                 if (nr < S.thresh) {
@@ -1535,6 +1569,7 @@ __global__ __launch_bounds__(RB) void k_mxf_refine(Ops o, int s, int first, int 
                     S.phase = PH_DONE;
                 } else {
                     S.scale = nr;  // true residual at this refinement
+                    S.iter_ref = S.iter;
                     S.fresh = first ? 1 : 2;
                     S.phase = PH_ACTIVE;
                 }
@@ -2522,6 +2557,7 @@ extern "C" int gf_krylov_solve(const gf_mesh *m, const gf_krylov_cfg *cfg, int32
     o.atol = cfg->atol;
     o.rtol = cfg->rtol;
     o.reliable = cfg->inner_rtol;
+    o.reliable_run = mixed_run();
     o.st = L.st;
     o.partials = L.partials;
     int rc;
