@@ -687,13 +687,13 @@ __device__ __forceinline__ double brick_apply(const Ops &o, int lane, bool start
 
 // The state transition of the direction kernel (cg.py:150-170), applied by
 // every wave to the same inputs: convergence test on the previous update's
-// residual, beta, the direction buffer flip.  Mixed precision (MX): the test
-// only decides whether the FP64 refinement is due; the reliable-update
-// trigger is only taken where the host queued a refinement right after this
-// iteration (may_refine), so no launches idle.
+// residual, beta, the direction buffer flip.  The FP64 PCG only: the mixed
+// precision PCG runs in k_cg_fused (MX stays false; may_refine is unused).
 template <int NR, bool MX>
 __device__ __forceinline__ void dir_transition(const Ops &o, int may_refine, const double (&tot)[NR][2],
                                                KState (&S)[NR], bool (&act)[NR], bool (&start)[NR]) {
+    static_assert(!MX, "the mixed-precision PCG runs in k_cg_fused");
+    (void)may_refine;
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
         KState &St = S[r];
@@ -703,18 +703,13 @@ __device__ __forceinline__ void dir_transition(const Ops &o, int may_refine, con
             if (St.fresh == 0) {
                 const double nr = nrm_fin(tot[r][0], o.norm_inf);
                 St.iter += 1;
-                if (!MX) {
-                    St.final_res = nr;
-                    if (nr < St.thresh) {
-                        St.converged = 1;
-                        St.phase = PH_DONE;
-                        go = false;
-                    } else if (St.iter >= o.max_iter) {
-                        St.phase = PH_DONE;
-                        go = false;
-                    }
-                } else if (nr < St.thresh || St.iter >= o.max_iter || (may_refine && nr < o.reliable * St.scale)) {
-                    St.phase = PH_RESTART;  // FP64 refinement due
+                St.final_res = nr;
+                if (nr < St.thresh) {
+                    St.converged = 1;
+                    St.phase = PH_DONE;
+                    go = false;
+                } else if (St.iter >= o.max_iter) {
+                    St.phase = PH_DONE;
                     go = false;
                 }
             }
