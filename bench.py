@@ -502,15 +502,20 @@ def serve_measure(n_requests: int, slots: int, budget: int = 30, seed: int = 0, 
     solve_case(random_case(np.random.default_rng(1), n_range=(16, 16), max_iterations=2))  # library, kernels
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for c in cases:
+    for k, c in enumerate(cases):
         pl = copy.deepcopy(c)
         pl["simulator"]["fvSolution"]["solvers"]["p"]["mixed_precision"] = False
         solve_case(pl)
+        _log(f"serving: one at a time {k + 1}/{n_requests} ({base_cells(c)} root cells), "
+             f"{time.perf_counter() - t0:.1f} s")
     t_seq = time.perf_counter() - t0
     server = BatchServer(max_cases=slots, large_cells=large, large_lanes=lanes, max_bytes=64 << 30)
+    server.on_result = lambda r: _log(f"serving: request {r.rid} done at {r.end:.1f} s "
+                                      f"({r.iterations} SIMPLE iterations, converged {r.converged})")
     t0 = time.perf_counter()
     sat = server.run([(k, c, 0.0) for k, c in enumerate(cases)])
     t_sat = time.perf_counter() - t0
+    _log(f"serving: all {n_requests} queued at once served in {t_sat:.1f} s; Poisson arrivals next")
     cap = n_requests / t_sat
     arr = np.cumsum(rng.exponential(1.0 / (0.8 * cap), size=n_requests))
     poi = server.run([(k, c, float(a)) for k, (c, a) in enumerate(zip(cases, arr))])
@@ -544,6 +549,7 @@ def main() -> int:
     ap.add_argument("--serve-slots", type=int, default=16, help="cases per batch in the serving measurement")
     ap.add_argument("--serve-only", action="store_true", help="only run the serving measurement")
     ap.add_argument("--serve-lanes", type=int, default=2, help="alone lanes for requests of >= 48^3 root cells")
+    ap.add_argument("--serve-budget", type=int, default=30, help="SIMPLE iteration budget per served request")
     ap.add_argument("--bluff-steps", type=int, default=0,
                     help="timed PISO steps of the 21.0M-cell bluff-body case (0: skip)")
     ap.add_argument("--bluff-only", action="store_true", help="only run the bluff-body measurement")
@@ -592,7 +598,8 @@ def main() -> int:
         print(json.dumps({"bluff_body": res}), flush=True)
         return 0
     if a.serve_only:
-        print(json.dumps({"serving": serve_measure(a.serve_requests, a.serve_slots, lanes=a.serve_lanes)}), flush=True)
+        print(json.dumps({"serving": serve_measure(a.serve_requests, a.serve_slots, budget=a.serve_budget,
+                                                    lanes=a.serve_lanes)}), flush=True)
         return 0
     if a.spmv_only:
         from gridfoam_amd.algorithms import create_case
@@ -683,7 +690,7 @@ def main() -> int:
         line["cavity2d_simple_64"] = config1_measure(cpu=not a.no_cpu)
     if a.serve_requests > 0:
         _log(f"serving: {a.serve_requests} requests, batches of {a.serve_slots}")
-        line["serving"] = serve_measure(a.serve_requests, a.serve_slots, lanes=a.serve_lanes)
+        line["serving"] = serve_measure(a.serve_requests, a.serve_slots, budget=a.serve_budget, lanes=a.serve_lanes)
     if a.bluff_steps > 0:
         line["bluff_body"] = bluff_measure(a.bluff_steps)
     if rel is not None:

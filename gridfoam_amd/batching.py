@@ -23,7 +23,7 @@ import copy
 import ctypes
 import weakref
 from dataclasses import dataclass, field
-from typing import Dict, List, Optional, Sequence, Tuple
+from typing import Callable, Dict, List, Optional, Sequence, Tuple
 
 import torch
 
@@ -591,6 +591,8 @@ class BatchServer:
         self.max_bytes = max_bytes
         self.bytes_per_cell: Optional[float] = None
         self._lane_error: Optional[BaseException] = None
+        # called with each ServedResult as it completes (under the results lock)
+        self.on_result: Optional[Callable[["ServedResult"], None]] = None
 
     def _is_large(self, raw: dict) -> bool:
         return self.large_cells is not None and base_cells(raw) >= self.large_cells
@@ -616,6 +618,8 @@ class BatchServer:
                     res = ServedResult(rid, cells, U, p, its, p_its, conv, arr, start, time.perf_counter() - t0)
                     with lock:
                         results.append(res)
+                        if self.on_result is not None:
+                            self.on_result(res)
         except BaseException as exc:  # surfaced by run()
             with lock:
                 self._lane_error = exc
@@ -730,6 +734,8 @@ class BatchServer:
                                            k in conv, arr, start, time.perf_counter() - t0)
                         with lock:
                             results.append(res)
+                            if self.on_result is not None:
+                                self.on_result(res)
                         done += 1
                 if all(c.done for c in batch.cases):
                     break
