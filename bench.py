@@ -12,7 +12,9 @@ within 1e-6 relative L2 of the plain FP64 run.
 Also measured: the dominant kernel (the PCG SpMV) timed with HIP events on
 the solver's stream, against the HBM roofline, and the FP64 CPU oracle on
 the host cores as a bounded sample of the same step; the 2-D SIMPLE cavity
-(64x64, 500 iterations) on the GPU beside the oracle; the serving stream.
+(64x64, 500 iterations) on the GPU beside the oracle; the serving stream;
+plain FP64 against mixed on the 21.0M-cell bluff body in a tunnel (a few
+steps each, ``--bluff-steps``).
 """
 
 from __future__ import annotations
@@ -236,6 +238,35 @@ def adjoint_measure(steps: int, every: int = 10, fd_coefs=(1,), h: float = 1e-4,
             "peak_extra_GiB": round(peak, 3), "dCd_dcoef": [float(v) for v in g.cpu()],
             "central_differences": {str(j): v for j, v in fd.items()},
             "rel_err_vs_central_differences": {str(j): abs(float(g[j]) - v) / abs(v) for j, v in fd.items()}}
+
+
+def bluff_compare(steps: int, mode: str = "both") -> dict:
+    """bluff_measure in plain FP64 and mixed precision (``mode`` "both") from
+    the same initial state: rates, their ratio and the relative L2 of the
+    mixed U and p against the plain ones over the cells connected to the
+    fluid (and over all cells)."""
+    import numpy as np
+
+    res = {}
+    for m in (("plain", "mixed") if mode == "both" else (mode,)):
+        res[m] = bluff_measure(steps, mixed=m == "mixed", keep=mode == "both")
+        _log(f"bluff body {m}: {res[m]['ms_per_step']} ms/step")
+    if mode == "both":
+        P, M = res["plain"], res["mixed"]
+        res["speedup_mixed_vs_fp64"] = round(P["ms_per_step"] / M["ms_per_step"], 3)
+        fl = P["_fluid"]
+
+        def rel(k, mask):
+            x, y = M["_" + k][mask], P["_" + k][mask]
+            return float(np.linalg.norm(x - y) / np.linalg.norm(y))
+
+        res["fluid_cells"] = int(fl.sum())
+        res["mixed_vs_fp64_rel_l2"] = {k: rel(k, fl) for k in ("U", "p")}
+        # all cells: the pressure inside the body is fixed by each solver's iteration alone
+        res["mixed_vs_fp64_rel_l2_all_cells"] = {k: rel(k, slice(None)) for k in ("U", "p")}
+        for r in (P, M):
+            del r["_U"], r["_p"], r["_fluid"]
+    return res
 
 
 def fluid_cells(grid):
@@ -550,8 +581,8 @@ def main() -> int:
     ap.add_argument("--serve-only", action="store_true", help="only run the serving measurement")
     ap.add_argument("--serve-lanes", type=int, default=2, help="alone lanes for requests of >= 48^3 root cells")
     ap.add_argument("--serve-budget", type=int, default=30, help="SIMPLE iteration budget per served request")
-    ap.add_argument("--bluff-steps", type=int, default=0,
-                    help="timed PISO steps of the 21.0M-cell bluff-body case (0: skip)")
+    ap.add_argument("--bluff-steps", type=int, default=5,
+                    help="timed PISO steps per mode of the 21.0M-cell bluff-body comparison (0: skip)")
     ap.add_argument("--bluff-only", action="store_true", help="only run the bluff-body measurement")
     ap.add_argument("--adjoint-steps", type=int, default=0,
                     help="only run the channel shape-gradient measurement over this many PIMPLE steps")
@@ -575,27 +606,7 @@ def main() -> int:
               flush=True)
         return 0
     if a.bluff_only:
-        n = a.bluff_steps or 10
-        res = {}
-        for mode in (("plain", "mixed") if a.mode == "both" else (a.mode,)):
-            res[mode] = bluff_measure(n, mixed=mode == "mixed", keep=a.mode == "both")
-            _log(f"bluff body {mode}: {res[mode]['ms_per_step']} ms/step")
-        if a.mode == "both":
-            P, M = res["plain"], res["mixed"]
-            res["speedup_mixed_vs_fp64"] = round(P["ms_per_step"] / M["ms_per_step"], 3)
-            fl = P["_fluid"]
-
-            def rel(k, mask):
-                a, b = M["_" + k][mask], P["_" + k][mask]
-                return float(np.linalg.norm(a - b) / np.linalg.norm(b))
-
-            res["fluid_cells"] = int(fl.sum())
-            res["mixed_vs_fp64_rel_l2"] = {k: rel(k, fl) for k in ("U", "p")}
-            # all cells: the pressure inside the body is fixed by each solver's iteration alone
-            res["mixed_vs_fp64_rel_l2_all_cells"] = {k: rel(k, slice(None)) for k in ("U", "p")}
-            for r in (P, M):
-                del r["_U"], r["_p"], r["_fluid"]
-        print(json.dumps({"bluff_body": res}), flush=True)
+        print(json.dumps({"bluff_body": bluff_compare(a.bluff_steps or 10, a.mode)}), flush=True)
         return 0
     if a.serve_only:
         print(json.dumps({"serving": serve_measure(a.serve_requests, a.serve_slots, budget=a.serve_budget,
@@ -692,7 +703,8 @@ def main() -> int:
         _log(f"serving: {a.serve_requests} requests, batches of {a.serve_slots}")
         line["serving"] = serve_measure(a.serve_requests, a.serve_slots, budget=a.serve_budget, lanes=a.serve_lanes)
     if a.bluff_steps > 0:
-        line["bluff_body"] = bluff_measure(a.bluff_steps)
+        _log(f"bluff body: plain FP64 and mixed, {a.bluff_steps} steps each")
+        line["bluff_body"] = bluff_compare(a.bluff_steps)
     if rel is not None:
         line["mixed_vs_fp64_rel_l2"] = rel
         line["speedup_mixed_vs_fp64"] = round(out["plain"]["dt"] / out["mixed"]["dt"], 3)
